@@ -1,0 +1,177 @@
+/*
+ * pimd_ttm.h - C ABI of the MI355X (gfx950) TTM2.1-F / TTM3-F force path and
+ * ring-polymer propagation.
+ *
+ * All P beads of one system are evaluated together on one GPU.  Arrays keep
+ * the reference's Fortran column-major layouts read as C row-major:
+ *   RRt(3,Natoms,Nbeads)   -> double [P][Natoms][3]   (atoms in O,H,H order)
+ *   virt(3,3)              -> double [col][row]
+ *   dip_momIt(3,Nwaters,P) -> double [P][Nwaters][3]
+ * Units are the reference's: Angstrom, kcal/mol, amu*Angstrom/ps, ps.
+ * dRRt is the gradient dU/dr (not the force), as in the reference.
+ *
+ * Every entry point returns PIMD_OK (0) or an error code; the message of the
+ * last failure on the calling thread is available from pimd_last_error().
+ * A context is not thread-safe: calls on one context must be serialised.
+ */
+#ifndef PIMD_TTM_H
+#define PIMD_TTM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIMD_OK 0
+#define PIMD_EINVAL 1   /* invalid argument or configuration */
+#define PIMD_EDEVICE 2  /* HIP runtime / device failure */
+#define PIMD_ENOMEM 3   /* device allocation failed */
+#define PIMD_ESTATE 4   /* call out of order (e.g. nm_evolve before nm_init) */
+
+#define PIMD_TTM21F 2
+#define PIMD_TTM3F 3
+
+#define PIMD_FP64 0     /* plain double precision */
+#define PIMD_MIXED 1    /* FP32 pair/matvec terms, FP64 refinement (reserved) */
+
+typedef struct pimd_ctx pimd_ctx;
+
+/* Run parameters.  Sentinels follow the reference's derived defaults
+ * (InputOutput.f90:118-125): rc <= 0 -> min(box)/2; rc1 < 0 -> 0.8*rc with
+ * 0.8 a single-precision literal. */
+typedef struct {
+  int pot_model;      /* PIMD_TTM21F or PIMD_TTM3F (potential.f90:84-113)    */
+  int nwaters;        /* Natoms = 3*nwaters                                  */
+  int nbeads;         /* P                                                   */
+  int polar_maxiter;  /* input_file_reader.f90:79, default 15                */
+  int guess_initdip;  /* 4-point dipole predictor (pot_ttm.f90:281-297); the
+                         history is kept per bead, as in a run with one MPI
+                         rank per bead                                        */
+  int precision;      /* PIMD_FP64                                           */
+  int device;         /* HIP device ordinal                                  */
+  int reserved0;
+  double box[3];      /* Angstrom                                            */
+  double rc;          /* real-space cut-off (Angstrom)                       */
+  double rc1;         /* vdW switch start (TTM3-F shift, potential.f90:107)  */
+  double eps_ewald;   /* Ewald accuracy, sets alpha and |k| cut (ewald.f90:312) */
+  double polar_sor;   /* SOR mixing (default real(4) 0.7)                    */
+  double polar_eps;   /* dipole convergence, Debye RMS (pot_ttm.f90:306,381) */
+  double mass_o;      /* amu, used by the Langevin thermostat                */
+  double mass_h;
+} pimd_params;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Replaces init_pot + ewald_set(.true.) (potential.f90:43-199,
+ * ewald.f90:293-378) and the per-rank allocations. */
+int pimd_ctx_create(const pimd_params *params, pimd_ctx **out);
+int pimd_ctx_destroy(pimd_ctx *ctx);
+/* Copies the message of the last failed call on this thread (NUL-terminated). */
+int pimd_last_error(char *buf, size_t len);
+
+/* Derived set-up values: out[0]=aewald, out[1]=Uvdw_lrc, out[2]=shiftA,
+ * out[3]=shiftB, out[4..7]=polfac(1:4), out[8]=nkvec, out[9]=hmax,
+ * out[10]=kmax, out[11]=lmax (12 doubles). */
+int pimd_ctx_info(const pimd_ctx *ctx, double *out);
+
+/* ---- forces: force_calc.f90:26-106 full_bead_forces ------------------- */
+
+/* One pot_ttm evaluation (pot_ttm.f90:3-630) for each of the P beads, all in
+ * one set of launches.  Host arrays:
+ *   RRt[P][Natoms][3], RRc[Natoms][3]           in
+ *   Upott[P], dRRt[P][Natoms][3], virt[P][3][3] (column-major per bead),
+ *   virialct[P], dip_momIt[P][N][3], dip_momEt[P][N][3], chg[P][Natoms],
+ *   iters[P] (SOR iterations run; negative when polar_maxiter was reached
+ *   without convergence)                         out, any may be NULL
+ * t is the MD step index passed to pot_ttm (drives the dipole predictor). */
+int pimd_forces_all_beads(pimd_ctx *ctx, const double *RRt, const double *RRc, int t,
+                          double *Upott, double *dRRt, double *virt, double *virialct,
+                          double *dip_momIt, double *dip_momEt, double *chg, int *iters);
+
+/* Same computation on device-resident arrays (same shapes), enqueued on the
+ * context's stream.  Outputs may be NULL.  Does not synchronise. */
+int pimd_forces_all_beads_dev(pimd_ctx *ctx, const double *d_RRt, const double *d_RRc, int t,
+                              double *d_Upott, double *d_dRRt, double *d_virt,
+                              double *d_virialct, double *d_dip_momIt, double *d_dip_momEt,
+                              double *d_chg, int *d_iters);
+
+/* Energy terms of the last force call, per bead: [P][5] =
+ * Umon, Uvdw, Uelec, Uind, Uvdw_lrc (system_mod.f90:30 module variables). */
+int pimd_energy_terms(pimd_ctx *ctx, double *terms);
+
+/* ---- ring polymer: NormalModes.f90 / Langevin.f90 / Nose_Hoover.f90 --- */
+
+/* InitNormalModes(Nbeads, omegan, delta, setNMfreq) (NormalModes.f90:68-218);
+ * delta in ps. */
+int pimd_nm_init(pimd_ctx *ctx, double omegan, double delta, double setNMfreq);
+/* Module data C[mode][bead], A11..A22, omegalist, MassScaleFactor (any NULL). */
+int pimd_nm_get(pimd_ctx *ctx, double *C, double *A11, double *A12, double *A21, double *A22,
+                double *omegalist, double *mass_scale);
+/* EvolveRing for every atom at once (replaces the 3N calls of MD.f90:52-58).
+ * RRt, PPt [P][Natoms][3] updated in place; masses[Natoms] in amu. */
+int pimd_nm_evolve(pimd_ctx *ctx, double *RRt, double *PPt, const double *masses);
+
+/* Init_Langevin_NM(delt2, CENTROIDTHERMOSTAT, tau_bead, Nbeads, temp)
+ * (Langevin.f90:68-98); temp is the bead temperature P*T as the reference
+ * passes it (InputOutput.f90:405). */
+int pimd_langevin_init(pimd_ctx *ctx, double delt2, int centroid_thermostat, double tau_bead,
+                       double temp);
+/* Langevin_NM(PPt) (Langevin.f90:30-64), PPt [P][Natoms][3] in place.
+ * noise: [Natoms][P][3] standard normals in the reference's consumption
+ * order, or NULL to draw them on the device (counter-based, seed/counter). */
+int pimd_langevin(pimd_ctx *ctx, double *PPt, const double *noise, uint64_t seed,
+                  uint64_t counter);
+
+/* Nose_Hoover(s, uk, 2, vxi, tau, delt2, L, temp) (Nose_Hoover.f90:33-112),
+ * chain length 2, on the host (scalar). */
+int pimd_nose_hoover(double *s, double *uk, double *vxi, double tau, double delt2, int L,
+                     double temp);
+
+/* ---- device-resident PIMD velocity-Verlet: MD.f90:35-110 ------------- */
+
+typedef struct {
+  double delt_fs;        /* time step (fs)                                  */
+  double temp;           /* K                                                */
+  int bead_thermostat;   /* 0 none, 1 Langevin (NM, PILE)                    */
+  int centroid_thermostat;
+  double tau_centroid;   /* ps                                               */
+  int global_nh;         /* global Nose-Hoover chain (MD.f90:43-46)          */
+  int reserved0;
+  double tau;            /* ps, global thermostat                            */
+  double setNMfreq;      /* cm^-1, 0 = RPMD                                  */
+} pimd_md_params;
+
+/* Configures the integrator: runs pimd_nm_init/pimd_langevin_init with the
+ * reference's derived values (omegan = kB*T*P/hbar, delt2 = delt/2). */
+int pimd_md_init(pimd_ctx *ctx, const pimd_md_params *md);
+/* Loads RRt, PPt, dRRt [P][Natoms][3] into device state (dRRt may be NULL ->
+ * zero, as the reference starts). */
+int pimd_state_set(pimd_ctx *ctx, const double *RRt, const double *PPt, const double *dRRt);
+/* Copies the device state out (any may be NULL); RRc [Natoms][3]. */
+int pimd_state_get(pimd_ctx *ctx, double *RRt, double *PPt, double *dRRt, double *RRc);
+/* Runs nsteps PIMD_VV steps on the device starting at MD step t0.
+ * noise: NULL (device RNG keyed by seed) or [nsteps][2][Natoms][P][3]
+ * standard normals for the two Langevin calls of each step.
+ * Per step outputs (any may be NULL): Upot[nsteps] = sum over beads,
+ * virial[nsteps], virialc[nsteps] (force_calc.f90:94-97), uk[nsteps]. */
+int pimd_md_run(pimd_ctx *ctx, int t0, int nsteps, const double *noise, uint64_t seed,
+                double *Upot, double *virial, double *virialc, double *uk);
+
+/* ---- instrumentation --------------------------------------------------- */
+
+/* Enables per-kernel HIP-event timing on the context stream. */
+int pimd_profile_enable(pimd_ctx *ctx, int on);
+/* Number of timed kernels, then per kernel: name, launch count, total ms. */
+int pimd_profile_count(pimd_ctx *ctx);
+int pimd_profile_get(pimd_ctx *ctx, int i, char *name, size_t len, long *launches,
+                     double *total_ms);
+int pimd_profile_reset(pimd_ctx *ctx);
+/* Blocks until all work queued on the context stream is done. */
+int pimd_sync(pimd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIMD_TTM_H */

@@ -1,0 +1,849 @@
+// C ABI (include/pimd_ttm.h): context set-up, the all-beads force pipeline,
+// ring-polymer entry points, the device-resident PIMD_VV loop and timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pimd_internal.h"
+
+using namespace pimd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(PIMD_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),      \
+                  __FILE__, __LINE__);                                               \
+  } while (0)
+
+struct Pending {
+  int id;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct pimd_ctx {
+  pimd_params p{};
+  SysConst c{};
+  DevBufs d{};
+  hipStream_t stream = nullptr;
+  int group = 64;
+  int natoms = 0;
+  std::vector<double> kvec, Ak, p4vAk;
+  double polfac[4] = {0, 0, 0, 0};
+  std::vector<int> hkl;
+  // ring polymer
+  bool nm_ready = false, lang_ready = false, md_ready = false, state_ready = false;
+  std::vector<double> C, A11, A12, A21, A22, om, msf, c1, c2;
+  pimd_md_params md{};
+  double *d_vxi = nullptr, *d_s = nullptr, *d_step = nullptr;
+  int *h_nactive = nullptr;
+  // profiling
+  bool prof = false;
+  std::vector<std::string> names;
+  std::map<std::string, int> name_id;
+  std::vector<long> launches;
+  std::vector<double> total_ms;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  int kid(const char *n) {
+    auto it = name_id.find(n);
+    if (it != name_id.end()) return it->second;
+    int id = (int)names.size();
+    names.push_back(n);
+    name_id[n] = id;
+    launches.push_back(0);
+    total_ms.push_back(0.0);
+    return id;
+  }
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void resolve() {
+    if (pending.empty()) return;
+    (void)hipStreamSynchronize(stream);
+    for (auto &q : pending) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, q.a, q.b);
+      launches[q.id] += 1;
+      total_ms[q.id] += ms;
+      pool.push_back(q.a);
+      pool.push_back(q.b);
+    }
+    pending.clear();
+  }
+  template <class F>
+  void timed(const char *n, F &&f) {
+    if (!prof) {
+      f();
+      return;
+    }
+    if (pending.size() > 4096) resolve();
+    Pending q{kid(n), ev(), ev()};
+    (void)hipEventRecord(q.a, stream);
+    f();
+    (void)hipEventRecord(q.b, stream);
+    pending.push_back(q);
+  }
+};
+
+namespace {
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(PIMD_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T),
+                                   hipGetErrorString(e));
+  return PIMD_OK;
+}
+
+void free_all(pimd_ctx *x) {
+  DevBufs &d = x->d;
+  void *ptrs[] = {d.RR,   d.RRc,   d.site,   d.Rcs,    d.grdq,   d.phi,   d.Efq,   d.dR,
+                  d.dip,  d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.tab,
+                  d.Sq,   d.Sm,    d.Sv,     d.virk,   d.active, d.iters, d.nactive, d.kv,
+                  d.khkl, d.cols,  d.colk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
+                  d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,     d.red,   x->d_vxi, x->d_s,
+                  x->d_step};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (x->h_nactive) (void)hipHostFree(x->h_nactive);
+  for (auto &q : x->pending) {
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  for (auto e : x->pool) (void)hipEventDestroy(e);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+}
+
+float f32(double v) { return (float)v; }
+
+// init_pot + ewald_set (potential.f90:43-199, ewald.f90:293-378) on the host.
+int setup_system(pimd_ctx *x) {
+  const pimd_params &p = x->p;
+  SysConst &c = x->c;
+  c.N = p.nwaters;
+  c.P = p.nbeads;
+  c.model = p.pot_model;
+  for (int k = 0; k < 3; ++k) {
+    c.box[k] = p.box[k];
+    c.boxi[k] = 1.0 / p.box[k];
+  }
+  c.vol = p.box[0] * p.box[1] * p.box[2];
+  const double Rc = p.rc, rc1 = p.rc1;
+  c.rc2 = Rc * Rc;
+  c.rc1 = rc1;
+  double pfO, pfH, pfM, aDD, aCC;
+  if (p.pot_model == PIMD_TTM21F) {  // pot_mod.f90:7-14
+    c.vdwA = -1.329565985e+6; c.vdwB = 3.632560798e+5; c.vdwC = -2.147141323e+3;
+    c.vdwD = 1e13; c.vdwE = 13.2;
+    pfO = 0.837; pfH = 0.496; pfM = 0.837;
+    c.polar[0] = 0.837; c.polar[1] = 0.496; c.polar[2] = 0.496; c.polar[3] = 0.837;
+    c.gammaM = 0.426706882; aDD = 0.3; aCC = 0.2;
+    c.dms1 = c.dms2 = c.dms3 = 0.0;
+    c.nd = 3; c.tdip = 0;
+    c.shiftA = c.shiftB = 0.0;
+  } else {  // pot_mod.f90:19-26
+    c.vdwA = 0.0; c.vdwB = 0.0; c.vdwC = -0.72298855e+03; c.vdwD = 0.10211829e+06;
+    c.vdwE = 0.37170376e+01;
+    pfO = 0.837; pfH = 0.496; pfM = 0.837;
+    c.polar[0] = 0.0; c.polar[1] = 0.0; c.polar[2] = 0.0; c.polar[3] = 1.444;
+    c.gammaM = 0.46; aDD = 0.175; aCC = 0.175;
+    c.dms1 = 0.5; c.dms2 = 0.9578; c.dms3 = 0.012;
+    c.nd = 1; c.tdip = 3;
+    if (Rc == rc1) {
+      c.shiftA = c.shiftB = 0.0;
+    } else {  // potential.f90:107-113
+      const double Rc8 = std::pow(Rc, 8);
+      c.shiftA = -(10 * Rc - 7 * rc1) / (Rc8 * ((Rc - rc1) * (Rc - rc1)));
+      c.shiftB = (9 * Rc - 7 * rc1) / (Rc8 * ((Rc - rc1) * (Rc - rc1) * (Rc - rc1)));
+    }
+  }
+  c.Uvdw_lrc = TWOPI * (double)(c.N * c.N) / c.vol *
+               (c.vdwA / (9.0 * std::pow(Rc, 9)) + c.vdwB / (7.0 * std::pow(Rc, 7)) +
+                c.vdwC / (3.0 * (Rc * Rc * Rc)));
+  double pf6[4] = {std::pow(pfO, 1.0 / 6.0), std::pow(pfH, 1.0 / 6.0), std::pow(pfH, 1.0 / 6.0),
+                   std::pow(pfM, 1.0 / 6.0)};
+  double pf[4];
+  for (int i = 0; i < 4; ++i) pf[i] = 1.0 / (pf6[i] * pf6[i] * pf6[i]);
+  for (int i = 0; i < 4; ++i) x->polfac[i] = pf[i];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      c.apolCC[i][j] = aCC * pf[i] * pf[j];
+      c.raCC[i][j] = std::pow(c.apolCC[i][j], 1.0 / 3.0);
+      c.apolDD[i][j] = aDD * pf[i] * pf[j];
+      c.apol3DD[i][j] = c.apolDD[i][j];
+    }
+  c.same_a = std::fabs(aCC - aDD) < 1e-12;
+  // ewald_set(.true.)
+  const double PP = -std::log(p.eps_ewald);
+  c.aewald = std::sqrt(PP) / Rc;
+  const double kcut = 2.0 * c.aewald * std::sqrt(PP), kcut2 = kcut * kcut;
+  c.hmax = (int)std::floor(kcut / TWOPI * p.box[0]);
+  c.kmax = (int)std::floor(kcut / TWOPI * p.box[1]);
+  c.lmax = (int)std::floor(kcut / TWOPI * p.box[2]);
+  const double fac = -0.25 / (c.aewald * c.aewald);
+  x->kvec.clear();
+  x->Ak.clear();
+  x->hkl.clear();
+  for (int kx = 0; kx <= c.hmax; ++kx) {
+    const double kv0 = TWOPI * c.boxi[0] * (double)kx;
+    const int ky1 = kx == 0 ? 0 : -c.kmax;
+    for (int ky = ky1; ky <= c.kmax; ++ky) {
+      const double kv1 = TWOPI * c.boxi[1] * (double)ky;
+      const int kz1 = (kx == 0 && ky == 0) ? 1 : -c.lmax;
+      const double kvxy = kv0 * kv0 + kv1 * kv1;
+      for (int kz = kz1; kz <= c.lmax; ++kz) {
+        const double kv2 = TWOPI * c.boxi[2] * (double)kz;
+        const double kvsq = kvxy + kv2 * kv2;
+        if (kvsq < kcut2 && kvsq > 1e-16) {
+          x->hkl.insert(x->hkl.end(), {kx, ky, kz});
+          x->kvec.insert(x->kvec.end(), {kv0, kv1, kv2});
+          x->Ak.push_back(2.0 * std::exp(fac * kvsq) / kvsq);
+        }
+      }
+    }
+  }
+  c.nk = (int)x->Ak.size();
+  if (c.nk == 0) return fail(PIMD_EINVAL, "no k-vectors inside the Ewald cut-off");
+  c.T = (c.hmax + 1) + (c.kmax + 1) + (c.lmax + 1);
+  c.a2 = c.aewald * c.aewald;
+  c.cts1 = 2.0 * c.aewald / SQPI;
+  c.cts2 = 4.0 * c.aewald * c.aewald * c.aewald / (3.0 * SQPI);
+  c.cts3 = 8.0 * std::pow(c.aewald, 5) / (15.0 * SQPI);
+  c.self_fac = c.aewald * TSP;
+  c.fac_dd = 4.0 * (c.aewald * c.aewald * c.aewald) / (3.0 * std::sqrt(PI));
+  c.maxiter = p.polar_maxiter;
+  c.guess = p.guess_initdip;
+  c.sor = p.polar_sor;
+  c.peps = p.polar_eps;
+  c.stath = DEBYE / CHARGECON / std::sqrt((double)(3 * c.N));
+  c.p4v = FOURPI / c.vol;
+  c.nbr_cap = std::max(c.N - 1, 1);
+  x->p4vAk.resize(c.nk);
+  for (int k = 0; k < c.nk; ++k) x->p4vAk[k] = FOURPI * x->Ak[k] / c.vol;
+  return PIMD_OK;
+}
+
+int alloc_system(pimd_ctx *x) {
+  SysConst &c = x->c;
+  DevBufs &d = x->d;
+  const size_t P = c.P, N = c.N, S = 4 * N, na = 3 * N, nk = c.nk, nd = c.nd;
+  int rc;
+#define ALLOC(ptr, n) \
+  if ((rc = dalloc(&(ptr), (n))) != PIMD_OK) return rc
+  ALLOC(d.RR, P * na * 3);
+  ALLOC(d.RRc, na * 3);
+  ALLOC(d.site, P * S);
+  ALLOC(d.Rcs, S * 3);
+  ALLOC(d.grdq, P * N * 27);
+  ALLOC(d.phi, P * S);
+  ALLOC(d.Efq, P * S * 3);
+  ALLOC(d.dR, P * S * 3);
+  ALLOC(d.dip, P * N * nd * 3);
+  ALLOC(d.Efd, P * N * nd * 3);
+  ALLOC(d.diff2, P * N * nd);
+  ALLOC(d.tally, P * N * T_COUNT);
+  ALLOC(d.hist, P * 4 * N * nd * 3);
+  ALLOC(d.nbr, P * N * (size_t)c.nbr_cap);
+  ALLOC(d.nbrc, P * N);
+  ALLOC(d.tab, P * (size_t)c.T * S);
+  ALLOC(d.Sq, P * nk * 2);
+  ALLOC(d.Sm, P * nk * 2);
+  ALLOC(d.Sv, P * nk * 6);
+  ALLOC(d.virk, P * ((nk + 255) / 256) * 18);
+  ALLOC(d.active, P);
+  ALLOC(d.iters, P);
+  ALLOC(d.nactive, (size_t)c.maxiter + 1);
+  ALLOC(d.kv, nk);
+  ALLOC(d.khkl, nk);
+  ALLOC(d.red, nk);  // A_k
+  ALLOC(d.Upot, P);
+  ALLOC(d.virt, P * 9);
+  ALLOC(d.virialc, P);
+  ALLOC(d.terms, P * 5);
+  ALLOC(d.dRR, P * na * 3);
+  ALLOC(d.dipI, P * N * 3);
+  ALLOC(d.dipE, P * N * 3);
+  ALLOC(d.chg, P * na);
+  ALLOC(d.iters_out, P);
+  ALLOC(d.C, P * P);
+  ALLOC(d.A, 4 * P);
+  ALLOC(d.msf, P);
+  ALLOC(d.c1, P);
+  ALLOC(d.c2, P);
+  ALLOC(d.masses, na);
+  ALLOC(d.PP, P * na * 3);
+  ALLOC(d.noise, 2 * na * P * 3);
+  ALLOC(d.uk, 1);
+  ALLOC(x->d_vxi, 2);
+  ALLOC(x->d_s, 1);
+  ALLOC(x->d_step, 4);
+#undef ALLOC
+  HIPCHK(hipMemset(d.hist, 0, sizeof(double) * P * 4 * N * nd * 3));
+  // k-vectors and kz-columns
+  std::vector<double4> kv(nk);
+  std::vector<int4> kh(nk);
+  std::vector<int4> cols;
+  std::vector<int> colk;
+  for (size_t k = 0; k < nk; ++k) {
+    kv[k] = make_double4(x->kvec[3 * k], x->kvec[3 * k + 1], x->kvec[3 * k + 2], x->p4vAk[k]);
+    kh[k] = make_int4(x->hkl[3 * k], x->hkl[3 * k + 1], x->hkl[3 * k + 2], 0);
+    const bool cont = k > 0 && x->hkl[3 * k] == x->hkl[3 * k - 3] &&
+                      x->hkl[3 * k + 1] == x->hkl[3 * k - 2] &&
+                      x->hkl[3 * k + 2] == x->hkl[3 * k - 1] + 1;
+    if (cont) {
+      cols.back().w += 1;
+    } else {
+      cols.push_back(make_int4(x->hkl[3 * k], x->hkl[3 * k + 1], x->hkl[3 * k + 2], 1));
+      colk.push_back((int)k);
+    }
+  }
+  c.ncol = (int)cols.size();
+  int rc2;
+  if ((rc2 = dalloc(&d.cols, cols.size())) != PIMD_OK) return rc2;
+  if ((rc2 = dalloc(&d.colk, colk.size())) != PIMD_OK) return rc2;
+  HIPCHK(hipMemcpy(d.kv, kv.data(), sizeof(double4) * nk, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.khkl, kh.data(), sizeof(int4) * nk, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.red, x->Ak.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.cols, cols.data(), sizeof(int4) * cols.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.colk, colk.data(), sizeof(int) * colk.size(), hipMemcpyHostToDevice));
+  std::vector<double> masses(na);
+  for (size_t a = 0; a < na; ++a) masses[a] = (a % 3 == 0) ? x->p.mass_o : x->p.mass_h;
+  HIPCHK(hipMemcpy(d.masses, masses.data(), sizeof(double) * na, hipMemcpyHostToDevice));
+  HIPCHK(hipHostMalloc((void **)&x->h_nactive, sizeof(int) * (c.maxiter + 1), hipHostMallocDefault));
+  return PIMD_OK;
+}
+
+// The pair kernels give each molecule G lanes; fewer lanes for short lists.
+int pick_group(const SysConst &c) {
+  const double frac = (4.0 / 3.0) * PI * std::pow(std::sqrt(c.rc2), 3) / c.vol;
+  const double nnb = std::min((double)(c.N - 1), frac * c.N);
+  if (nnb >= 96) return 64;
+  if (nnb >= 40) return 32;
+  return 16;
+}
+
+// One pot_ttm evaluation per bead on device arrays (pot_ttm.f90:3-630).
+int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *Upot,
+               double *dRR, double *virt, double *virialc, double *dipI, double *dipE,
+               double *chg, int *iters) {
+  const SysConst &c = x->c;
+  DevBufs &d = x->d;
+  hipStream_t s = x->stream;
+  x->timed("monomer", [&] { launch_monomer(c, d, RR, RRc, chg, s); });
+  x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
+  x->timed("pair_qq", [&] { launch_pair_qq(c, d, x->group, s); });
+  x->timed("ewald_tables", [&] { launch_tables(c, d, s); });
+  x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, s); });
+  x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
+  x->timed("post_qq", [&] { launch_post_qq(c, d, t, s); });
+  for (int it = 1; it <= c.maxiter; ++it) {
+    x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, s); });
+    x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
+    x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
+    x->timed("converge", [&] { launch_converge(c, d, it, s); });
+    if (it < c.maxiter) {
+      HIPCHK(hipMemcpyAsync(x->h_nactive + it, d.nactive + it, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (x->h_nactive[it] == 0) break;
+    }
+  }
+  x->timed("pair_final", [&] { launch_pair_final(c, d, x->group, s); });
+  x->timed("sfac_v", [&] { launch_sfac(c, d, SF_V, s); });
+  x->timed("field_qd", [&] { launch_field(c, d, FD_QD, 0, s); });
+  x->timed("virial_k", [&] { launch_virk(c, d, s); });
+  x->timed("finalize", [&] { launch_finalize(c, d, t, dRR, dipI, dipE, s); });
+  x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, iters, s); });
+  HIPCHK(hipGetLastError());
+  return PIMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pimd_last_error(char *buf, size_t len) {
+  if (!buf || len == 0) return PIMD_EINVAL;
+  snprintf(buf, len, "%s", g_err.c_str());
+  return PIMD_OK;
+}
+
+int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
+  if (!params || !out) return fail(PIMD_EINVAL, "null argument");
+  *out = nullptr;
+  pimd_params p = *params;
+  if (p.pot_model != PIMD_TTM21F && p.pot_model != PIMD_TTM3F)
+    return fail(PIMD_EINVAL, "pot_model must be 2 (TTM2.1-F) or 3 (TTM3-F), got %d", p.pot_model);
+  if (p.nwaters < 2) return fail(PIMD_EINVAL, "need at least 2 waters, got %d", p.nwaters);
+  if (p.nbeads < 1 || p.nbeads > 64) return fail(PIMD_EINVAL, "nbeads must be 1..64, got %d", p.nbeads);
+  for (int k = 0; k < 3; ++k)
+    if (!(p.box[k] > 0.0)) return fail(PIMD_EINVAL, "box must be positive");
+  if (p.rc <= 0.0) p.rc = std::min(p.box[0], std::min(p.box[1], p.box[2])) / 2.0;
+  if (p.rc1 < 0.0) p.rc1 = (double)f32(0.8) * p.rc;
+  const double lmin = std::min(p.box[0], std::min(p.box[1], p.box[2]));
+  if (p.rc > lmin / 2.0 * (1.0 + 1e-12))  // InputOutput.f90:277-281
+    return fail(PIMD_EINVAL, "cut-off %.6f exceeds half the box (%.6f)", p.rc, lmin / 2.0);
+  if (!(p.eps_ewald > 0.0 && p.eps_ewald < 1.0)) return fail(PIMD_EINVAL, "eps_ewald must be in (0,1)");
+  if (p.polar_maxiter < 1) return fail(PIMD_EINVAL, "polar_maxiter must be >= 1");
+  if (p.precision != PIMD_FP64) return fail(PIMD_EINVAL, "only PIMD_FP64 is available");
+  if (p.mass_o <= 0.0) p.mass_o = (double)f32(15.994);
+  if (p.mass_h <= 0.0) p.mass_h = (double)f32(1.008);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(PIMD_EDEVICE, "no HIP device available");
+  if (p.device < 0 || p.device >= ndev) return fail(PIMD_EINVAL, "device %d out of range", p.device);
+  if (hipSetDevice(p.device) != hipSuccess) return fail(PIMD_EDEVICE, "hipSetDevice failed");
+  pimd_ctx *x = new pimd_ctx();
+  x->p = p;
+  x->natoms = 3 * p.nwaters;
+  int rc = setup_system(x);
+  if (rc == PIMD_OK && hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(PIMD_EDEVICE, "hipStreamCreate failed");
+  if (rc == PIMD_OK) rc = alloc_system(x);
+  if (rc == PIMD_OK) {
+    set_constant_tables();
+    x->group = pick_group(x->c);
+  }
+  if (rc != PIMD_OK) {
+    free_all(x);
+    delete x;
+    return rc;
+  }
+  *out = x;
+  return PIMD_OK;
+}
+
+int pimd_ctx_destroy(pimd_ctx *x) {
+  if (!x) return PIMD_OK;
+  (void)hipSetDevice(x->p.device);
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  free_all(x);
+  delete x;
+  return PIMD_OK;
+}
+
+int pimd_ctx_info(const pimd_ctx *x, double *out) {
+  if (!x || !out) return fail(PIMD_EINVAL, "null argument");
+  const SysConst &c = x->c;
+  out[0] = c.aewald;
+  out[1] = c.Uvdw_lrc;
+  out[2] = c.shiftA;
+  out[3] = c.shiftB;
+  for (int i = 0; i < 4; ++i) out[4 + i] = x->polfac[i];
+  out[8] = c.nk;
+  out[9] = c.hmax;
+  out[10] = c.kmax;
+  out[11] = c.lmax;
+  return PIMD_OK;
+}
+
+int pimd_forces_all_beads_dev(pimd_ctx *x, const double *RR, const double *RRc, int t,
+                              double *Upot, double *dRR, double *virt, double *virialc,
+                              double *dipI, double *dipE, double *chg, int *iters) {
+  if (!x || !RR || !RRc) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  DevBufs &d = x->d;
+  return run_forces(x, RR, RRc, t, Upot ? Upot : d.Upot, dRR ? dRR : d.dRR,
+                    virt ? virt : d.virt, virialc ? virialc : d.virialc, dipI ? dipI : d.dipI,
+                    dipE ? dipE : d.dipE, chg ? chg : d.chg, iters ? iters : d.iters_out);
+}
+
+int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int t,
+                          double *Upott, double *dRRt, double *virt, double *virialct,
+                          double *dipI, double *dipE, double *chg, int *iters) {
+  if (!x || !RRt || !RRc) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  const size_t P = x->c.P, na = x->natoms, N = x->c.N;
+  DevBufs &d = x->d;
+  hipStream_t s = x->stream;
+  HIPCHK(hipMemcpyAsync(d.RR, RRt, sizeof(double) * P * na * 3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.RRc, RRc, sizeof(double) * na * 3, hipMemcpyHostToDevice, s));
+  int rc = run_forces(x, d.RR, d.RRc, t, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
+                      d.iters_out);
+  if (rc != PIMD_OK) return rc;
+  if (Upott) HIPCHK(hipMemcpyAsync(Upott, d.Upot, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  if (dRRt) HIPCHK(hipMemcpyAsync(dRRt, d.dRR, sizeof(double) * P * na * 3, hipMemcpyDeviceToHost, s));
+  if (virt) HIPCHK(hipMemcpyAsync(virt, d.virt, sizeof(double) * P * 9, hipMemcpyDeviceToHost, s));
+  if (virialct) HIPCHK(hipMemcpyAsync(virialct, d.virialc, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  if (dipI) HIPCHK(hipMemcpyAsync(dipI, d.dipI, sizeof(double) * P * N * 3, hipMemcpyDeviceToHost, s));
+  if (dipE) HIPCHK(hipMemcpyAsync(dipE, d.dipE, sizeof(double) * P * N * 3, hipMemcpyDeviceToHost, s));
+  if (chg) HIPCHK(hipMemcpyAsync(chg, d.chg, sizeof(double) * P * na, hipMemcpyDeviceToHost, s));
+  if (iters) HIPCHK(hipMemcpyAsync(iters, d.iters_out, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return PIMD_OK;
+}
+
+int pimd_energy_terms(pimd_ctx *x, double *terms) {
+  if (!x || !terms) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipMemcpyAsync(terms, x->d.terms, sizeof(double) * x->c.P * 5, hipMemcpyDeviceToHost,
+                        x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return PIMD_OK;
+}
+
+// InitNormalModes (NormalModes.f90:68-218), on the host; uploads C and A.
+int pimd_nm_init(pimd_ctx *x, double omegan, double delta, double setnm) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  const int P = x->c.P;
+  const int l = (P % 2 == 0) ? (P - 2) / 2 : (P - 1) / 2;
+  std::vector<double> C(P * P), A11(P), A12(P), A21(P), A22(P), om(P), msf(P, 1.0);
+  for (int i = 1; i <= P; ++i) {
+    C[0 * P + (i - 1)] = std::sqrt(1.0 / P);
+    for (int k = 1; k <= l; ++k) {
+      C[k * P + (i - 1)] = std::sqrt(2.0 / P) * std::cos((double)(k * i) * 2.0 * PI / P);
+      C[(k + l) * P + (i - 1)] = std::sqrt(2.0 / P) * std::sin((double)(k * i) * 2.0 * PI / P);
+    }
+    if (P % 2 == 0) C[(2 * l + 1) * P + (i - 1)] = std::sqrt(1.0 / P) * ((i % 2) ? -1.0 : 1.0);
+  }
+  auto setmode = [&](int j, double w) {
+    om[j] = w;
+    A11[j] = std::cos(w * delta);
+    A12[j] = (1 / w) * std::sin(w * delta);
+    A21[j] = -w * std::sin(w * delta);
+    A22[j] = std::cos(w * delta);
+  };
+  om[0] = 0.0; A11[0] = 1.0; A12[0] = delta; A21[0] = 0.0; A22[0] = 1.0;
+  for (int k = 1; k <= l; ++k) {
+    const double w = 2 * omegan * std::sin(PI * (double)(f32(k) / f32(P)));  // REAL(4) ratio
+    setmode(k, w);
+    setmode(k + l, w);
+  }
+  if (P % 2 == 0) setmode(2 * l + 1, 2 * omegan * std::sin(PI * (double)(f32(l + 1) / f32(P))));
+  if (setnm != 0.0) {  // NormalModes.f90:168-206
+    const double w = (2 * PI) * setnm / 33.33333;
+    for (int i = 0; i < P; ++i) msf[i] = om[i] == 0 ? 1.0 : (om[i] / w) * (om[i] / w);
+    const int last = (P % 2 == 0) ? 2 * l + 1 : 2 * l;
+    for (int j = 1; j <= last; ++j) setmode(j, w);
+  }
+  x->C = C; x->A11 = A11; x->A12 = A12; x->A21 = A21; x->A22 = A22; x->om = om; x->msf = msf;
+  std::vector<double> A(4 * P);
+  for (int j = 0; j < P; ++j) {
+    A[j] = A11[j];
+    A[P + j] = A12[j];
+    A[2 * P + j] = A21[j];
+    A[3 * P + j] = A22[j];
+  }
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipMemcpy(x->d.C, C.data(), sizeof(double) * P * P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->d.A, A.data(), sizeof(double) * 4 * P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->d.msf, msf.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+  x->nm_ready = true;
+  return PIMD_OK;
+}
+
+int pimd_nm_get(pimd_ctx *x, double *C, double *A11, double *A12, double *A21, double *A22,
+                double *om, double *msf) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  if (!x->nm_ready) return fail(PIMD_ESTATE, "pimd_nm_init has not been called");
+  const size_t P = x->c.P;
+  if (C) std::memcpy(C, x->C.data(), sizeof(double) * P * P);
+  if (A11) std::memcpy(A11, x->A11.data(), sizeof(double) * P);
+  if (A12) std::memcpy(A12, x->A12.data(), sizeof(double) * P);
+  if (A21) std::memcpy(A21, x->A21.data(), sizeof(double) * P);
+  if (A22) std::memcpy(A22, x->A22.data(), sizeof(double) * P);
+  if (om) std::memcpy(om, x->om.data(), sizeof(double) * P);
+  if (msf) std::memcpy(msf, x->msf.data(), sizeof(double) * P);
+  return PIMD_OK;
+}
+
+int pimd_nm_evolve(pimd_ctx *x, double *RRt, double *PPt, const double *masses) {
+  if (!x || !RRt || !PPt) return fail(PIMD_EINVAL, "null argument");
+  if (!x->nm_ready) return fail(PIMD_ESTATE, "pimd_nm_init has not been called");
+  HIPCHK(hipSetDevice(x->p.device));
+  const size_t P = x->c.P, na = x->natoms, n = P * na * 3;
+  hipStream_t s = x->stream;
+  DevBufs &d = x->d;
+  if (masses) HIPCHK(hipMemcpyAsync(d.masses, masses, sizeof(double) * na, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.RR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.PP, PPt, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  x->timed("nm_evolve", [&] { launch_nm_evolve((int)P, (int)na, d, d.RR, d.PP, s); });
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(RRt, d.RR, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(PPt, d.PP, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return PIMD_OK;
+}
+
+// Init_Langevin_NM (Langevin.f90:68-98).
+int pimd_langevin_init(pimd_ctx *x, double delt2, int centroid, double tau_bead, double temp) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  if (!x->nm_ready) return fail(PIMD_ESTATE, "pimd_nm_init has not been called");
+  const int P = x->c.P;
+  x->c1.assign(P, 0.0);
+  x->c2.assign(P, 0.0);
+  for (int i = 0; i < P; ++i) {
+    const double g = i == 0 ? (centroid ? 1 / tau_bead : 0.0) : 2 * x->om[i];
+    x->c1[i] = std::exp(-delt2 * g);
+    x->c2[i] = std::sqrt(1 - x->c1[i] * x->c1[i]);
+    x->c2[i] = std::sqrt(KB_AMU * temp) * x->c2[i];
+  }
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipMemcpy(x->d.c1, x->c1.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->d.c2, x->c2.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+  x->lang_ready = true;
+  return PIMD_OK;
+}
+
+int pimd_langevin(pimd_ctx *x, double *PPt, const double *noise, uint64_t seed, uint64_t counter) {
+  if (!x || !PPt) return fail(PIMD_EINVAL, "null argument");
+  if (!x->lang_ready) return fail(PIMD_ESTATE, "pimd_langevin_init has not been called");
+  HIPCHK(hipSetDevice(x->p.device));
+  const size_t P = x->c.P, na = x->natoms, n = P * na * 3;
+  hipStream_t s = x->stream;
+  DevBufs &d = x->d;
+  HIPCHK(hipMemcpyAsync(d.PP, PPt, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  if (noise) HIPCHK(hipMemcpyAsync(d.noise, noise, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  x->timed("langevin", [&] {
+    launch_langevin((int)P, (int)na, d, d.PP, noise ? d.noise : nullptr, seed, counter,
+                    x->p.mass_o, x->p.mass_h, s);
+  });
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(PPt, d.PP, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return PIMD_OK;
+}
+
+int pimd_nose_hoover(double *s, double *uk, double *vxi, double tau, double delt2, int L,
+                     double temp) {
+  if (!s || !uk || !vxi) return fail(PIMD_EINVAL, "null argument");
+  const double delt4 = delt2 / 2.0, delt8 = delt2 / 4.0;
+  const double kT = KB_AMU * temp;
+  const double M1 = L * kT * (tau * tau), M2 = M1;
+  vxi[1] = vxi[1] + (M1 * (vxi[0] * vxi[0]) - kT) * delt4 / M2;
+  vxi[0] = vxi[0] * std::exp(-vxi[1] * delt8);
+  vxi[0] = vxi[0] + (2 * *uk - L * kT) * delt4 / M1;
+  vxi[0] = vxi[0] * std::exp(-vxi[1] * delt8);
+  *s = std::exp(-vxi[0] * delt2);
+  *uk = *uk * *s * *s;
+  vxi[0] = vxi[0] * std::exp(-vxi[1] * delt8);
+  vxi[0] = vxi[0] + (2 * *uk - L * kT) * delt4 / M1;
+  vxi[0] = vxi[0] * std::exp(-vxi[1] * delt8);
+  vxi[1] = vxi[1] + (M1 * (vxi[0] * vxi[0]) - kT) * delt4 / M2;
+  return PIMD_OK;
+}
+
+int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
+  if (!x || !md) return fail(PIMD_EINVAL, "null argument");
+  if (!(md->delt_fs > 0.0) || !(md->temp > 0.0)) return fail(PIMD_EINVAL, "delt_fs and temp must be > 0");
+  if (md->bead_thermostat != 0 && md->bead_thermostat != 1)
+    return fail(PIMD_EINVAL, "bead_thermostat must be 0 (none) or 1 (Langevin)");
+  x->md = *md;
+  const int P = x->c.P;
+  const double delt = md->delt_fs / 1000.0, delt2 = delt / 2.0;
+  const double omegan = KB_AMU * md->temp * P / HBAR;  // InputOutput.f90:140
+  int rc = pimd_nm_init(x, omegan, delt, md->setNMfreq);
+  if (rc != PIMD_OK) return rc;
+  if (md->bead_thermostat == 1) {
+    rc = pimd_langevin_init(x, delt2, md->centroid_thermostat, md->tau_centroid, P * md->temp);
+    if (rc != PIMD_OK) return rc;
+  }
+  const double vxi0[2] = {1.0, 1.0}, one = 1.0;  // vxi_global = 1 (InputOutput.f90:399)
+  HIPCHK(hipMemcpy(x->d_vxi, vxi0, sizeof vxi0, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->d_s, &one, sizeof one, hipMemcpyHostToDevice));
+  x->md_ready = true;
+  return PIMD_OK;
+}
+
+int pimd_state_set(pimd_ctx *x, const double *RRt, const double *PPt, const double *dRRt) {
+  if (!x || !RRt || !PPt) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  const size_t n = (size_t)x->c.P * x->natoms * 3;
+  DevBufs &d = x->d;
+  HIPCHK(hipMemcpy(d.RR, RRt, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.PP, PPt, sizeof(double) * n, hipMemcpyHostToDevice));
+  if (dRRt)
+    HIPCHK(hipMemcpy(d.dRR, dRRt, sizeof(double) * n, hipMemcpyHostToDevice));
+  else
+    HIPCHK(hipMemset(d.dRR, 0, sizeof(double) * n));
+  // centroid for the first step's bookkeeping
+  std::vector<double> rc(x->natoms * 3, 0.0);
+  for (int k = 0; k < x->c.P; ++k)
+    for (size_t i = 0; i < rc.size(); ++i) rc[i] += RRt[k * rc.size() + i];
+  for (auto &v : rc) v /= x->c.P;
+  HIPCHK(hipMemcpy(d.RRc, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice));
+  x->state_ready = true;
+  return PIMD_OK;
+}
+
+int pimd_state_get(pimd_ctx *x, double *RRt, double *PPt, double *dRRt, double *RRc) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  const size_t n = (size_t)x->c.P * x->natoms * 3;
+  DevBufs &d = x->d;
+  if (RRt) HIPCHK(hipMemcpy(RRt, d.RR, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (PPt) HIPCHK(hipMemcpy(PPt, d.PP, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (dRRt) HIPCHK(hipMemcpy(dRRt, d.dRR, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (RRc) HIPCHK(hipMemcpy(RRc, d.RRc, sizeof(double) * x->natoms * 3, hipMemcpyDeviceToHost));
+  return PIMD_OK;
+}
+
+}  // extern "C"
+
+namespace pimd {
+__global__ void k_step_record(int P, const double *Upot, const double *virt, const double *virialc,
+                              const double *uk, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double u = 0.0, v = 0.0, vc = 0.0;
+  for (int b = 0; b < P; ++b) {
+    u += Upot[b];
+    v += virt[b * 9 + 0] + virt[b * 9 + 4] + virt[b * 9 + 8];
+    vc += virialc[b];
+  }
+  out[0] = u;
+  out[1] = v;
+  out[2] = vc / P;
+  out[3] = uk[0];
+}
+}  // namespace pimd
+
+extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise, uint64_t seed,
+                           double *Upot, double *virial, double *virialc, double *uk) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  if (!x->md_ready) return fail(PIMD_ESTATE, "pimd_md_init has not been called");
+  if (!x->state_ready) return fail(PIMD_ESTATE, "pimd_state_set has not been called");
+  if (nsteps < 0) return fail(PIMD_EINVAL, "nsteps must be >= 0");
+  HIPCHK(hipSetDevice(x->p.device));
+  const int P = x->c.P, na = x->natoms;
+  const size_t n = (size_t)P * na * 3;
+  const pimd_md_params &md = x->md;
+  const double delt = md.delt_fs / 1000.0, delt2 = delt / 2.0;
+  const int L = 3 * na * P;
+  DevBufs &d = x->d;
+  hipStream_t s = x->stream;
+  std::vector<double> rec((size_t)nsteps * 4);
+  double *d_rec = nullptr;
+  if (nsteps > 0) {
+    int rc = dalloc(&d_rec, (size_t)nsteps * 4);
+    if (rc != PIMD_OK) return rc;
+  }
+  auto thermostat = [&](int step, int half) {
+    if (md.bead_thermostat != 1) return;
+    const double *nz = nullptr;
+    if (noise) {
+      (void)hipMemcpyAsync(d.noise, noise + ((size_t)step * 2 + half) * n, sizeof(double) * n,
+                           hipMemcpyHostToDevice, s);
+      nz = d.noise;
+    }
+    x->timed("langevin", [&] {
+      launch_langevin(P, na, d, d.PP, nz, seed, (uint64_t)(t0 + step) * 2 + half, x->p.mass_o,
+                      x->p.mass_h, s);
+    });
+  };
+  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, s); });
+  for (int st = 0; st < nsteps; ++st) {
+    thermostat(st, 0);
+    if (md.global_nh) {
+      x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
+      x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
+    }
+    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+    if (P > 1)
+      x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s); });
+    else
+      x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, delt, s); });
+    x->timed("centroid_pbc", [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
+    int rc = run_forces(x, d.RR, d.RRc, t0 + st, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE,
+                        d.chg, d.iters_out);
+    if (rc != PIMD_OK) {
+      (void)hipFree(d_rec);
+      return rc;
+    }
+    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, s); });
+    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+    thermostat(st, 1);
+    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, s); });
+    if (md.global_nh) {
+      x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
+      x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
+    }
+    hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, s, P, d.Upot, d.virt, d.virialc, d.uk,
+                       d_rec + (size_t)st * 4);
+  }
+  HIPCHK(hipGetLastError());
+  if (nsteps > 0)
+    HIPCHK(hipMemcpyAsync(rec.data(), d_rec, sizeof(double) * nsteps * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (d_rec) (void)hipFree(d_rec);
+  for (int st = 0; st < nsteps; ++st) {
+    if (Upot) Upot[st] = rec[st * 4];
+    if (virial) virial[st] = rec[st * 4 + 1];
+    if (virialc) virialc[st] = rec[st * 4 + 2];
+    if (uk) uk[st] = rec[st * 4 + 3];
+  }
+  return PIMD_OK;
+}
+
+extern "C" {
+
+int pimd_profile_enable(pimd_ctx *x, int on) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  x->prof = on != 0;
+  return PIMD_OK;
+}
+
+int pimd_profile_count(pimd_ctx *x) {
+  if (!x) return 0;
+  x->resolve();
+  return (int)x->names.size();
+}
+
+int pimd_profile_get(pimd_ctx *x, int i, char *name, size_t len, long *launches, double *total_ms) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  x->resolve();
+  if (i < 0 || i >= (int)x->names.size()) return fail(PIMD_EINVAL, "profile index out of range");
+  if (name && len) snprintf(name, len, "%s", x->names[i].c_str());
+  if (launches) *launches = x->launches[i];
+  if (total_ms) *total_ms = x->total_ms[i];
+  return PIMD_OK;
+}
+
+int pimd_profile_reset(pimd_ctx *x) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  x->resolve();
+  for (auto &v : x->launches) v = 0;
+  for (auto &v : x->total_ms) v = 0.0;
+  return PIMD_OK;
+}
+
+int pimd_sync(pimd_ctx *x) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return PIMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// Internal declarations shared by the HIP kernels and the host-side C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pimd_ttm.h"
+
+namespace pimd {
+
+// consts.f90:22-42
+constexpr double CHARGECON = 18.22234397655801030455;
+constexpr double DEBYE = 4.8033324;
+constexpr double MASSCON = 418.4;
+constexpr double KB_AMU = 0.831447148;   // KB_amuA2ps2perK
+constexpr double HBAR = 6.35077993041;   // amu*A^2/ps
+constexpr double PI = 3.14159265358979323844;
+constexpr double TWOPI = 6.28318530717958647688;
+constexpr double FOURPI = 12.56637061435917295376;
+constexpr double TSP = 1.12837916709551257390;   // 2/sqrt(pi)
+constexpr double SQPI = 1.77245385090551602729;
+
+// One site of a molecule record: position and charge (charge in
+// CHARGECON-scaled units, as the reference's charge array).
+struct __align__(32) Site {
+  double x, y, z, q;
+};
+
+// Everything the kernels need about the model and the box, passed by value.
+struct SysConst {
+  int N, P, model;        // waters, beads, 2 or 3
+  int nd, tdip;           // dipole sites per molecule and the first dipole type
+  int nk, ncol, T;        // k-vectors, kz-columns, phase-table rows per site
+  int hmax, kmax, lmax;
+  int maxiter, guess;
+  int nbr_cap;
+  double box[3], boxi[3], vol;
+  double rc2, rc1;
+  double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
+  double aewald, a2, cts1, cts2, cts3, self_fac, fac_dd;
+  double gammaM;
+  double polar[4];
+  double apolCC[4][4], raCC[4][4], apolDD[4][4], apol3DD[4][4];
+  int same_a;             // |aCCaCD - aDD| < 1e-12 (pot_ttm.f90:443)
+  double dms1, dms2, dms3;
+  double sor, stath, peps;
+  double Uvdw_lrc;
+  double p4v;             // FOURPI / volume
+};
+
+// Tally slots per (bead, molecule): scalar partial sums reduced per bead.
+enum Tally { T_UMON = 0, T_UVDW, T_UELEC, T_UIND, T_VIRC, T_VIRT /* 9 slots, [col][row] */,
+             T_COUNT = T_VIRT + 9 };
+
+// Device buffers of one context.
+struct DevBufs {
+  double *RR = nullptr, *RRc = nullptr;         // staging of host inputs
+  Site *site = nullptr;                         // [P][N][4]
+  double *Rcs = nullptr;                        // [N][4][3] centroid sites
+  double *grdq = nullptr;                       // [P][N][27]
+  double *phi = nullptr;                        // [P][N][4]
+  double *Efq = nullptr;                        // [P][N][4][3]
+  double *dR = nullptr;                         // [P][N][4][3]
+  double *dip = nullptr;                        // [P][N][nd][3]
+  double *Efd = nullptr;                        // [P][N][nd][3]
+  double *diff2 = nullptr;                      // [P][N*nd]
+  double *tally = nullptr;                      // [P][N][T_COUNT]
+  double *hist = nullptr;                       // [P][4][N*nd][3]
+  int *nbr = nullptr, *nbrc = nullptr;          // [P][N][cap], [P][N]
+  double2 *tab = nullptr;                       // [P][T][4N]
+  double *Sq = nullptr, *Sm = nullptr;          // [P][nk][2]
+  double *Sv = nullptr;                         // [P][nk][6]
+  double *virk = nullptr;                       // [P][nkb][9]
+  int *active = nullptr, *iters = nullptr;      // [P]
+  int *nactive = nullptr;                       // [1]
+  double4 *kv = nullptr;                        // [nk] kx, ky, kz, Ak
+  int4 *khkl = nullptr;                         // [nk] h, k, l, 0
+  int4 *cols = nullptr;                         // [ncol] h, k, l_first, count
+  int *colk = nullptr;                          // [ncol] first k index
+  // outputs of the last force call
+  double *Upot = nullptr, *virt = nullptr, *virialc = nullptr, *terms = nullptr;  // [P],[P][9],[P],[P][5]
+  double *dRR = nullptr, *dipI = nullptr, *dipE = nullptr, *chg = nullptr;
+  int *iters_out = nullptr;
+  // ring polymer
+  double *C = nullptr, *A = nullptr;            // [P][P], [4][P] (A11,A12,A21,A22)
+  double *msf = nullptr, *c1 = nullptr, *c2 = nullptr;
+  double *masses = nullptr;                     // [Natoms]
+  double *PP = nullptr, *noise = nullptr;       // [P][Natoms][3], [2][Natoms][P][3]
+  double *uk = nullptr;                         // [1] kinetic energy
+  double *red = nullptr;                        // scratch for reductions
+};
+
+// Host-side launchers (ttm_kernels.hip / rp_kernels.hip).
+void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
+                    double *chg_out, hipStream_t s);
+void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s);
+void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s);
+void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s);
+void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s);
+void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s);
+void launch_post_qq(const SysConst &c, DevBufs &b, int t, hipStream_t s);
+void launch_pair_dd(const SysConst &c, DevBufs &b, int group, hipStream_t s);
+void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s);
+void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s);
+void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s);
+void launch_finalize(const SysConst &c, DevBufs &b, int t, double *dRR, double *dipI,
+                     double *dipE, hipStream_t s);
+void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
+                        double *virialc, int *iters, hipStream_t s);
+void set_constant_tables();
+
+enum SfacMode { SF_Q = 0, SF_D = 1, SF_V = 2 };
+enum FieldMode { FD_Q = 0, FD_D = 1, FD_QD = 2 };
+
+// Ring polymer launchers (rp_kernels.hip).
+void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s);
+void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
+                     uint64_t seed, uint64_t counter, double massO, double massH,
+                     hipStream_t s);
+void launch_kick(int n, double *PP, const double *dRR, double f, hipStream_t s);
+void launch_scale(int n, double *PP, const double *sc, hipStream_t s);
+void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,
+                         double *RRc, hipStream_t s);
+void launch_kinetic(int P, int natoms, const double *masses, const double *PP, double *uk,
+                    hipStream_t s);
+void launch_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
+                        double temp, hipStream_t s);
+void launch_free_flight(int n, double *RR, const double *PP, const double *masses,
+                        double delt, hipStream_t s);
+
+}  // namespace pimd

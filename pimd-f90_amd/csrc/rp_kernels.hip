@@ -1,0 +1,269 @@
+// Ring-polymer propagation kernels: normal-mode free-ring step
+// (NormalModes.f90:34-63), PILE Langevin thermostat in normal-mode space
+// (Langevin.f90:30-64), velocity-Verlet kicks, centroid periodic wrap
+// (main_stuff.f90:107-125), kinetic energy (main_stuff.f90:349-388) and the
+// global Nose-Hoover chain (Nose_Hoover.f90:33-112).
+//
+// Layout: RRt/PPt [P][Natoms][3]; C [mode][bead]; A [4][P] = A11, A12, A21, A22.
+// One thread per (atom, component) holds the atom's P beads; C lives in LDS.
+#include "pimd_internal.h"
+
+namespace pimd {
+
+// real2NM / NM2real (NormalModes.f90:308-341) around the exact harmonic
+// propagator of each free-ring mode.
+__global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const double *__restrict__ Cg,
+                                                  const double *__restrict__ A,
+                                                  const double *__restrict__ msf,
+                                                  const double *__restrict__ masses, double *RR,
+                                                  double *PP) {
+  extern __shared__ double Cs[];  // [P][P]
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
+  __syncthreads();
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= natoms * 3) return;
+  const int a = id / 3;
+  const size_t stride = (size_t)natoms * 3;
+  const double m = masses[a];
+  double r[64], p[64];  // bead coordinates, then mode coordinates (P <= 64)
+  for (int k = 0; k < P; ++k) {
+    r[k] = RR[k * stride + id];
+    p[k] = PP[k * stride + id];
+  }
+  double rt[64], pt[64];
+  for (int j = 0; j < P; ++j) {
+    double vr = 0.0, vp = 0.0;
+    for (int k = 0; k < P; ++k) {
+      vr = vr + Cs[j * P + k] * r[k];
+      vp = vp + Cs[j * P + k] * p[k];
+    }
+    rt[j] = vr;
+    pt[j] = vp;
+  }
+  for (int j = 0; j < P; ++j) {
+    const double t2 = A[j] * rt[j] + A[P + j] * pt[j] / (m * msf[j]);
+    pt[j] = A[2 * P + j] * m * msf[j] * rt[j] + A[3 * P + j] * pt[j];
+    rt[j] = t2;
+  }
+  for (int j = 0; j < P; ++j) {
+    double vr = 0.0, vp = 0.0;
+    for (int k = 0; k < P; ++k) {
+      vr = vr + Cs[k * P + j] * rt[k];
+      vp = vp + Cs[k * P + j] * pt[k];
+    }
+    RR[j * stride + id] = vr;
+    PP[j * stride + id] = vp;
+  }
+}
+
+void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s) {
+  const int n = natoms * 3;
+  hipLaunchKernelGGL(k_nm_evolve, dim3((n + 255) / 256), dim3(256), sizeof(double) * P * P, s, P,
+                     natoms, b.C, b.A, b.msf, b.masses, RR, PP);
+}
+
+// Philox4x32-10 counter-based generator (J. K. Salmon et al., "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11), written from the published round
+// function, and a Box-Muller transform to standard normals.
+__device__ __forceinline__ void philox_round(uint32_t ctr[4], const uint32_t key[2]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint64_t p0 = (uint64_t)M0 * ctr[0], p1 = (uint64_t)M1 * ctr[2];
+  const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  const uint32_t n0 = hi1 ^ ctr[1] ^ key[0], n2 = hi0 ^ ctr[3] ^ key[1];
+  ctr[0] = n0;
+  ctr[1] = lo1;
+  ctr[2] = n2;
+  ctr[3] = lo0;
+}
+
+__device__ __forceinline__ void philox4x32(uint32_t ctr[4], uint32_t key[2]) {
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    philox_round(ctr, key);
+    key[0] += W0;
+    key[1] += W1;
+  }
+}
+
+// Two standard normals for stream index idx under (seed, counter).
+__device__ __forceinline__ void normal2(uint64_t seed, uint64_t counter, uint64_t idx, double &z0,
+                                        double &z1) {
+  uint32_t ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)counter,
+                     (uint32_t)(counter >> 32)};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  philox4x32(ctr, key);
+  const double two53 = 1.0 / 9007199254740992.0;
+  const uint64_t a = ((uint64_t)ctr[0] << 21) ^ (ctr[1] >> 11);
+  const uint64_t bb = ((uint64_t)ctr[2] << 21) ^ (ctr[3] >> 11);
+  const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) * two53;  // (0, 1]
+  const double u2 = (double)(bb & ((1ull << 53) - 1)) * two53;          // [0, 1)
+  const double rr = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincos(TWOPI * u2, &sn, &cs);
+  z0 = rr * cs;
+  z1 = rr * sn;
+}
+
+// Langevin_NM: p~_j <- c1_j p~_j + c2_j sqrt(m s_j) xi (Langevin.f90:41-60).
+// noise [Natoms][P][3] (the reference's consumption order) or generated.
+__global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const double *__restrict__ Cg,
+                                                 const double *__restrict__ msf,
+                                                 const double *__restrict__ c1,
+                                                 const double *__restrict__ c2, double massO,
+                                                 double massH, double *PP,
+                                                 const double *__restrict__ noise, uint64_t seed,
+                                                 uint64_t counter) {
+  extern __shared__ double Cs[];
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
+  __syncthreads();
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= natoms * 3) return;
+  const int a = id / 3, comp = id % 3;
+  const size_t stride = (size_t)natoms * 3;
+  const double m = (a % 3 == 0) ? massO : massH;
+  double p[64], pt[64];
+  for (int k = 0; k < P; ++k) p[k] = PP[k * stride + id];
+  for (int j = 0; j < P; ++j) {
+    double v = 0.0;
+    for (int k = 0; k < P; ++k) v = v + Cs[j * P + k] * p[k];
+    pt[j] = v;
+  }
+  for (int j = 0; j < P; ++j) {
+    double xi;
+    const uint64_t n = ((uint64_t)a * P + j) * 3 + comp;
+    if (noise) {
+      xi = noise[n];
+    } else {
+      double z0, z1;
+      normal2(seed, counter, n >> 1, z0, z1);
+      xi = (n & 1) ? z1 : z0;
+    }
+    const double sm = sqrt(m * msf[j]);
+    pt[j] = c1[j] * pt[j] + c2[j] * (sm * xi);
+  }
+  for (int j = 0; j < P; ++j) {
+    double v = 0.0;
+    for (int k = 0; k < P; ++k) v = v + Cs[k * P + j] * pt[k];
+    PP[j * stride + id] = v;
+  }
+}
+
+void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
+                     uint64_t seed, uint64_t counter, double massO, double massH,
+                     hipStream_t s) {
+  const int n = natoms * 3;
+  hipLaunchKernelGGL(k_langevin, dim3((n + 255) / 256), dim3(256), sizeof(double) * P * P, s, P,
+                     natoms, b.C, b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter);
+}
+
+// PPt = PPt - MASSCON*dRRt*delt2 (MD.f90:49, 94).
+__global__ void k_kick(int n, double *PP, const double *__restrict__ dRR, double delt2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) PP[i] = PP[i] - MASSCON * dRR[i] * delt2;
+}
+
+void launch_kick(int n, double *PP, const double *dRR, double delt2, hipStream_t s) {
+  hipLaunchKernelGGL(k_kick, dim3((n + 255) / 256), dim3(256), 0, s, n, PP, dRR, delt2);
+}
+
+// PPt = PPt*s with s on the device (MD.f90:45, 105).
+__global__ void k_scale(int n, double *PP, const double *__restrict__ sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) PP[i] = PP[i] * sc[0];
+}
+
+void launch_scale(int n, double *PP, const double *sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, PP, sc);
+}
+
+// Free flight for P = 1 (MD.f90:60-68).
+__global__ void k_free_flight(int n, double *RR, const double *__restrict__ PP,
+                              const double *__restrict__ masses, double delt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) RR[i] = RR[i] + (1.0 / masses[i / 3]) * PP[i] * delt;
+}
+
+void launch_free_flight(int n, double *RR, const double *PP, const double *masses, double delt,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_free_flight, dim3((n + 255) / 256), dim3(256), 0, s, n, RR, PP, masses,
+                     delt);
+}
+
+// RRc = sum(RRt,3)/Nbeads, then the centroid wrap of PBCs (MD.f90:70;
+// main_stuff.f90:107-125): shifts use box(1) on every axis.
+__global__ void k_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,
+                               double *RRc) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= natoms * 3) return;
+  const size_t stride = (size_t)natoms * 3;
+  double s = 0.0;
+  for (int k = 0; k < P; ++k) s = s + RR[k * stride + id];
+  const double c = s / P;
+  const double sh = box0 * round(c * boxi0);
+  RRc[id] = c - sh;
+  for (int k = 0; k < P; ++k) RR[k * stride + id] = RR[k * stride + id] - sh;
+}
+
+void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR, double *RRc,
+                         hipStream_t s) {
+  const int n = natoms * 3;
+  hipLaunchKernelGGL(k_centroid_pbc, dim3((n + 255) / 256), dim3(256), 0, s, P, natoms, box0,
+                     boxi0, RR, RRc);
+}
+
+// uk = 1/2 sum_beads sum_atoms |p|^2 / m (calc_uk, RPMD branch), one block.
+__global__ __launch_bounds__(256) void k_kinetic(int P, int natoms, const double *__restrict__ masses,
+                                                const double *__restrict__ PP, double *uk) {
+  __shared__ double red[256];
+  const size_t n = (size_t)P * natoms;
+  double s = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) {
+    const int a = (int)(i % natoms);
+    const double *p = PP + i * 3;
+    s += (1.0 / masses[a]) * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) uk[0] = 0.5 * red[0];
+}
+
+void launch_kinetic(int P, int natoms, const double *masses, const double *PP, double *uk,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_kinetic, dim3(1), dim3(256), 0, s, P, natoms, masses, PP, uk);
+}
+
+// Nose_Hoover half step, chain length 2 (Nose_Hoover.f90:158-198), scalars on device.
+__global__ void k_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
+                              double temp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double delt4 = delt2 / 2.0, delt8 = delt2 / 4.0;
+  const double kT = KB_AMU * temp;
+  const double M1 = L * kT * (tau * tau), M2 = M1;
+  double v0 = vxi[0], v1 = vxi[1], u = uk[0];
+  v1 = v1 + (M1 * (v0 * v0) - kT) * delt4 / M2;
+  v0 = v0 * exp(-v1 * delt8);
+  v0 = v0 + (2 * u - L * kT) * delt4 / M1;
+  v0 = v0 * exp(-v1 * delt8);
+  const double s = exp(-v0 * delt2);
+  u = u * s * s;
+  v0 = v0 * exp(-v1 * delt8);
+  v0 = v0 + (2 * u - L * kT) * delt4 / M1;
+  v0 = v0 * exp(-v1 * delt8);
+  v1 = v1 + (M1 * (v0 * v0) - kT) * delt4 / M2;
+  vxi[0] = v0;
+  vxi[1] = v1;
+  uk[0] = u;
+  sc[0] = s;
+}
+
+void launch_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
+                        double temp, hipStream_t s) {
+  hipLaunchKernelGGL(k_nose_hoover, dim3(1), dim3(64), 0, s, uk, vxi, sc, tau, delt2, L, temp);
+}
+
+}  // namespace pimd

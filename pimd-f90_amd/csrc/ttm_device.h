@@ -1,0 +1,334 @@
+// Device-side scalar kernels of the TTM model: the A&S erfc, the truncated
+// incomplete gamma function, the Ewald-screened Thole-smeared interaction
+// tensors and the Partridge-Schwenke monomer PES / dipole surface.
+#pragma once
+#include "pimd_internal.h"
+#include "ps_tables.h"
+
+namespace pimd {
+
+// Combined PES coefficients (c5z of nasa.f90:45) and exponent tables,
+// uploaded once to constant memory.
+extern __constant__ double c_c5z[PS_NTERM];
+extern __constant__ int c_idx[PS_NTERM][3];
+extern __constant__ double c_coefd[PS_NDMS];
+extern __constant__ int c_idxd[PS_NDMS][3];
+
+__device__ __forceinline__ double anint(double x) { return round(x); }
+
+// Minimum image r - box*anint(r/box) (pot_ttm.f90:64 and throughout).  box*n
+// is exact for the integer n in {-1, 0, 1}, so contraction cannot change it.
+__device__ __forceinline__ void minimg(const SysConst &c, double &x, double &y, double &z) {
+  x = x - c.box[0] * anint(x * c.boxi[0]);
+  y = y - c.box[1] * anint(y * c.boxi[1]);
+  z = z - c.box[2] * anint(z * c.boxi[2]);
+}
+
+// x*x + y*y + z*z rounded exactly as the reference (no fused multiply-add), so
+// cut-off tests (r^2 < Rc^2, r^2 < 25) decide identically.
+__device__ __forceinline__ double norm2_exact(double x, double y, double z) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+// Abramowitz-Stegun 7.1.26 complementary error function (math.f90:287-314)
+// with the exp(-x^2) factor supplied by the caller.
+__device__ __forceinline__ double erfc_as(double x, double expmx2) {
+  const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
+               A5 = 1.061405429, P = 0.3275911;
+  double T = 1.0 / (1.0 + P * x);
+  return T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5)))) * expmx2;
+}
+
+// Regularised upper incomplete gamma Q(2/3, x) with the reference's series /
+// continued-fraction split, tolerance and iteration caps (math.f90:188-285).
+__device__ inline double gammq23(double x) {
+  const double a = 2.0 / 3.0, gln = 0.303150275147547, EPS = 3e-7, FPMIN = 1e-30;
+  if (x < a + 1.0) {
+    if (x <= 0.0) return 1.0;
+    double ap = a, sum = 1.0 / a, del = sum;
+    for (int n = 1; n <= 800; ++n) {
+      ap = ap + 1.0;
+      del = del * x / ap;
+      sum = sum + del;
+      if (fabs(del) < fabs(sum) * EPS) break;
+    }
+    return 1.0 - sum * exp(-x + a * log(x) - gln);
+  }
+  double b = x + 1.0 - a, cc = 1.0 / FPMIN, d = 1.0 / b, h = d;
+  for (int i = 1; i <= 100; ++i) {
+    double an = -((double)i * ((double)i - a));
+    b = b + 2.0;
+    d = an * d + b;
+    if (fabs(d) < FPMIN) d = FPMIN;
+    cc = b + an / cc;
+    if (fabs(cc) < FPMIN) cc = FPMIN;
+    d = 1.0 / d;
+    double del = d * cc;
+    h = h * del;
+    if (fabs(del - 1.0) < EPS) break;
+  }
+  return exp(-x + a * log(x) - gln) * h;
+}
+
+// Screened Coulomb derivatives shared by PBCsmear01/2/3 (smear.f90:2-107):
+// ts_n = erfc-screened 1/r^(2n+1) terms plus the Thole correction for r < 5.
+// One exp(-alpha^2 r^2) serves both the explicit Gaussian and the erfc factor.
+struct Smear {
+  double dd, dri, drsqi, expon, er;
+};
+
+__device__ __forceinline__ Smear smear_base(const SysConst &c, double drsq) {
+  Smear s;
+  s.dd = sqrt(drsq);
+  s.dri = 1.0 / s.dd;
+  s.drsqi = s.dri * s.dri;
+  s.expon = exp(-(c.a2 * drsq));
+  s.er = erfc_as(c.aewald * s.dd, s.expon);
+  return s;
+}
+
+// PBCsmear01 (smear.f90:2-34): ts0, ts1 for the charge-charge terms.
+__device__ __forceinline__ void smear01(const SysConst &c, double drsq, double apol, double ra,
+                                        double &ts0, double &ts1) {
+  Smear s = smear_base(c, drsq);
+  double t0 = s.er * s.dri;
+  double t1 = (t0 + c.cts1 * s.expon) * s.drsqi;
+  if (drsq < 25.0) {
+    double rA3 = s.dd * drsq * apol;
+    double exp1 = exp(-rA3);
+    double ch1 = -1.0 * s.dri * s.drsqi;
+    double sr1 = (1.0 - exp1) * s.dri * s.drsqi;
+    t1 = t1 + ch1 + sr1;
+    t0 = t0 - exp1 * s.dri + ra * 1.3541179394264 * gammq23(rA3);
+  }
+  ts0 = t0;
+  ts1 = t1;
+}
+
+// PBCsmear2 (smear.f90:36-69): ts1, ts2 for the charge-dipole and
+// dipole-dipole field tensors.
+__device__ __forceinline__ void smear2(const SysConst &c, double drsq, double apol, double &ts1,
+                                       double &ts2) {
+  Smear s = smear_base(c, drsq);
+  double t1 = (s.er * s.dri + c.cts1 * s.expon) * s.drsqi;
+  double t2 = (t1 + c.cts2 * s.expon) * s.drsqi;
+  if (drsq < 25.0) {
+    double rA3 = (s.dd * drsq) * apol;
+    double exp1 = exp(-rA3);
+    double ch1 = -1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi;
+    double sr1 = (1.0 - exp1) * s.dri * s.drsqi;
+    double sr2 = (sr1 - exp1 * apol) * s.drsqi;
+    t1 = t1 + ch1 + sr1;
+    t2 = t2 + ch2 + sr2;
+  }
+  ts1 = t1;
+  ts2 = t2;
+}
+
+// PBCsmear3 (smear.f90:71-107): ts1, ts2, ts3 for the dipole-dipole forces.
+__device__ __forceinline__ void smear3(const SysConst &c, double drsq, double apol, double &ts1,
+                                       double &ts2, double &ts3) {
+  Smear s = smear_base(c, drsq);
+  double t1 = (s.er * s.dri + c.cts1 * s.expon) * s.drsqi;
+  double t2 = (t1 + c.cts2 * s.expon) * s.drsqi;
+  double t3 = (t2 + c.cts3 * s.expon) * s.drsqi;
+  if (drsq < 25.0) {
+    double rA3 = (s.dd * drsq) * apol;
+    double exp1 = exp(-rA3);
+    double ch1 = -1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi, ch3 = ch2 * s.drsqi;
+    double sr1 = (1.0 - exp1) * s.dri * s.drsqi;
+    double sr2 = (sr1 - exp1 * apol) * s.drsqi;
+    double sr3 = (sr2 - 0.6 * exp1 * s.dd * apol * apol) * s.drsqi;
+    t1 = t1 + ch1 + sr1;
+    t2 = t2 + ch2 + sr2;
+    t3 = t3 + ch3 + sr3;
+  }
+  ts1 = t1;
+  ts2 = t2;
+  ts3 = t3;
+}
+
+// self1 / self2 (smear.f90:110-156): screened minus bare Coulomb for the
+// intramolecular (excluded) pairs.
+__device__ __forceinline__ void self1(const SysConst &c, double drsq, double &ts0, double &ts1) {
+  double dd = sqrt(drsq), drsqi = 1.0 / drsq, dri = sqrt(drsqi);
+  double expon = exp(-(c.a2 * drsq));
+  double kr0 = erfc_as(c.aewald * dd, expon) * dri;
+  double kr1 = (kr0 + c.cts1 * expon) * drsqi;
+  double ch0 = -dri, ch1 = ch0 * drsqi;
+  ts0 = ch0 + kr0;
+  ts1 = kr1 + ch1;
+}
+
+__device__ __forceinline__ void self2(const SysConst &c, double drsq, double &ts1, double &ts2) {
+  double dd = sqrt(drsq), drsqi = 1.0 / drsq, dri = sqrt(drsqi);
+  double expon = exp(-(c.a2 * drsq));
+  double kr0 = erfc_as(c.aewald * dd, expon) * dri;
+  double kr1 = (kr0 + c.cts1 * expon) * drsqi;
+  double kr2 = (kr1 + c.cts2 * expon) * drsqi;
+  double ch0 = -dri, ch1 = ch0 * drsqi, ch2 = ch1 * drsqi;
+  ts1 = kr1 + ch1;
+  ts2 = kr2 + ch2;
+}
+
+// get_dd3 (pot_ttm.f90:633-647) applied to a vector: out = (3 ts2 r r^T - ts1) v.
+__device__ __forceinline__ void dd3_apply(const double r[3], double ts1, double ts2,
+                                          const double v[3], double out[3]) {
+  double t = 3.0 * ts2;
+  double rv = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+  out[0] = t * r[0] * rv - ts1 * v[0];
+  out[1] = t * r[1] * rv - ts1 * v[1];
+  out[2] = t * r[2] * rv - ts1 * v[2];
+}
+
+// Partridge-Schwenke monomer: energy and gradient (nasa.f90:2-103) and the
+// dipole-moment-surface charges with their gradients (nasa.f90:105-293).
+// r[atom][xyz] for O, H1, H2.  gq[disp][charge][xyz] with disp and charge in
+// the order (H1, H2, O), as the reference's gradq.
+__device__ inline void ps_monomer(const SysConst &c, const double r[3][3], double &e1,
+                                  double dr1[3][3], double q3[3], double gq[3][3][3]) {
+  double R1[3], R2[3], RH[3];
+  for (int k = 0; k < 3; ++k) {
+    R1[k] = r[1][k] - r[0][k];
+    R2[k] = r[2][k] - r[0][k];
+    RH[k] = r[1][k] - r[2][k];
+  }
+  minimg(c, R1[0], R1[1], R1[2]);
+  minimg(c, R2[0], R2[1], R2[2]);
+  minimg(c, RH[0], RH[1], RH[2]);
+  double d1 = sqrt(R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2]);
+  double d2 = sqrt(R2[0] * R2[0] + R2[1] * R2[1] + R2[2] * R2[2]);
+  double dh = sqrt(RH[0] * RH[0] + RH[1] * RH[1] + RH[2] * RH[2]);
+  double costh = (R1[0] * R2[0] + R1[1] * R2[1] + R1[2] * R2[2]) / (d1 * d2);
+  const double costhe = -.24780227221366464506;
+  double x1 = (d1 - PS_REOH) / PS_REOH, x2 = (d2 - PS_REOH) / PS_REOH, x3 = costh - costhe;
+  // powers x^(n-1) for n = 0..15 with x^(-1) := 0 (fmat of nasa.f90:61-67)
+  double f1[16], f2[16], f3[16];
+  f1[0] = f2[0] = f3[0] = 0.0;
+  f1[1] = f2[1] = f3[1] = 1.0;
+#pragma unroll
+  for (int j = 2; j < 16; ++j) {
+    f1[j] = f1[j - 1] * x1;
+    f2[j] = f2[j - 1] * x2;
+    f3[j] = f3[j - 1] * x3;
+  }
+  // ---- potential energy surface
+  double deoh = PS_F5Z * PS_DEOHA;
+  double phh1 = PS_F5Z * PS_PHH1A;
+  phh1 = phh1 * exp(PS_PHH2);
+  double ex1 = exp(-PS_ALPHAOH * (d1 - PS_ROH));
+  double ex2 = exp(-PS_ALPHAOH * (d2 - PS_ROH));
+  double Va = deoh * (ex1 * (ex1 - 2.0) + ex2 * (ex2 - 2.0));
+  double Vb = phh1 * exp(-PS_PHH2 * dh);
+  double dVa1 = 2.0 * PS_ALPHAOH * deoh * ex1 * (1.0 - ex1) / d1;
+  double dVa2 = 2.0 * PS_ALPHAOH * deoh * ex2 * (1.0 - ex2) / d2;
+  double dVb = -PS_PHH2 * Vb / dh;
+  double efac = exp(-PS_B1 * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int j = 1; j < PS_NTERM; ++j) {
+    int I = c_idx[j][0], J = c_idx[j][1], K = c_idx[j][2];
+    double cz = c_c5z[j];
+    double fI1 = f1[I], fJ2 = f2[J], fJ1 = f1[J], fI2 = f2[I], fK3 = f3[K];
+    s0 = s0 + cz * (fI1 * fJ2 + fJ1 * fI2) * fK3;
+    s1 = s1 + cz * ((double)(I - 1) * f1[I - 1] * fJ2 + (double)(J - 1) * f1[J - 1] * fI2) * fK3;
+    s2 = s2 + cz * ((double)(J - 1) * fI1 * f2[J - 1] + (double)(I - 1) * fJ1 * f2[I - 1]) * fK3;
+    s3 = s3 + cz * (fI1 * fJ2 + fJ1 * fI2) * (double)(K - 1) * f3[K - 1];
+  }
+  double Vc = 2.0 * c_c5z[0] + efac * s0;
+  double e = Va + Vb + Vc;
+  e = e + 0.44739574026257;
+  e1 = e * 0.00285914375100642899;
+  double dVcdr1 = (-2.0 * PS_B1 * efac * (d1 - PS_REOH) * s0 + efac * s1 / PS_REOH) / d1;
+  double dVcdr2 = (-2.0 * PS_B1 * efac * (d2 - PS_REOH) * s0 + efac * s2 / PS_REOH) / d2;
+  double dVcdcth = efac * s3;
+  for (int k = 0; k < 3; ++k) {
+    double g1 = dVa1 * R1[k] + dVb * RH[k] + dVcdr1 * R1[k] +
+                dVcdcth * (R2[k] / (d1 * d2) - costh * R1[k] / (d1 * d1));
+    double g2 = dVa2 * R2[k] - dVb * RH[k] + dVcdr2 * R2[k] +
+                dVcdcth * (R1[k] / (d1 * d2) - costh * R2[k] / (d2 * d2));
+    dr1[1][k] = g1 * .00285914375100642899;
+    dr1[2][k] = g2 * .00285914375100642899;
+    dr1[0][k] = -(g1 + g2) * .00285914375100642899;
+  }
+  // ---- dipole moment surface
+  const double ath0 = 1.82400520401572996557;
+  const double B2A = 0.529177249;
+  const double bfac = 1.0 / (double)0.529177249f;  // real(4) literal, nasa.f90:218
+  double efd = exp(-DMS_B1D * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
+  double P1 = 0, P2 = 0, PL1 = costh, PL2 = 0.5 * (3.0 * PL1 * PL1 - 1.0);
+  double p1r1 = 0, p1r2 = 0, p1c = 0, p2r1 = 0, p2r2 = 0, p2c = 0;
+  for (int j = 1; j < PS_NDMS; ++j) {
+    int I = c_idxd[j][0], J = c_idxd[j][1], K = c_idxd[j][2];
+    double cj = c_coefd[j];
+    P1 = P1 + cj * f1[I] * f2[J] * f3[K];
+    P2 = P2 + cj * f1[J] * f2[I] * f3[K];
+    p1r1 = p1r1 + cj * ((double)(I - 1) * f1[I - 1] * f2[J] * f3[K]);
+    p1r2 = p1r2 + cj * ((double)(J - 1) * f1[I] * f2[J - 1] * f3[K]);
+    p1c = p1c + cj * ((double)(K - 1) * f1[I] * f2[J] * f3[K - 1]);
+    p2r1 = p2r1 + cj * ((double)(J - 1) * f1[J - 1] * f2[I] * f3[K]);
+    p2r2 = p2r2 + cj * ((double)(I - 1) * f1[J] * f2[I - 1] * f3[K]);
+    p2c = p2c + cj * ((double)(K - 1) * f1[J] * f2[I] * f3[K - 1]);
+  }
+  p1r1 = p1r1 / PS_REOH * B2A;
+  p1r2 = p1r2 / PS_REOH * B2A;
+  p2r1 = p2r1 / PS_REOH * B2A;
+  p2r2 = p2r2 / PS_REOH * B2A;
+  double pw1 = pow(d1, DMS_B), pw2 = pow(d2, DMS_B);
+  double poly = DMS_C0 + PL1 * DMS_C1 + PL2 * DMS_C2;
+  double PC0 = DMS_A * (pw1 + pw2) * poly;
+  double dpc0dr1 = DMS_A * (DMS_B * pow(d1, DMS_B - 1)) * poly * B2A * B2A;
+  double dpc0dr2 = DMS_A * (DMS_B * pow(d2, DMS_B - 1)) * poly * B2A * B2A;
+  double dpc0dc = DMS_A * (pw1 + pw2) * (DMS_C1 + 0.5 * (6.0 * PL1) * DMS_C2) * B2A;
+  double defdr1 = -2.0 * DMS_B1D * (d1 - PS_REOH) * efd * B2A;
+  double defdr2 = -2.0 * DMS_B1D * (d2 - PS_REOH) * efd * B2A;
+  p1r1 = p1r1 * efd + P1 * defdr1 + dpc0dr1;
+  p1r2 = p1r2 * efd + P1 * defdr2 + dpc0dr2;
+  p1c = p1c * efd + dpc0dc;
+  p2r1 = p2r1 * efd + P2 * defdr1 + dpc0dr1;
+  p2r2 = p2r2 * efd + P2 * defdr2 + dpc0dr2;
+  p2c = p2c * efd + dpc0dc;
+  P1 = c_coefd[0] + P1 * efd + PC0 * B2A;
+  P2 = c_coefd[0] + P2 * efd + PC0 * B2A;
+  q3[0] = -(P1 + P2);
+  q3[1] = P1;
+  q3[2] = P2;
+  p1r1 = p1r1 * bfac;
+  p1r2 = p1r2 * bfac;
+  p2r1 = p2r1 * bfac;
+  p2r2 = p2r2 * bfac;
+  double ax = R1[1] * R2[2] - R1[2] * R2[1];
+  double ay = -R1[0] * R2[2] + R1[2] * R2[0];
+  double az = R1[0] * R2[1] - R1[1] * R2[0];
+  double sinth = sqrt(ax * ax + ay * ay + az * az) / (d1 * d2);
+  double ang = atan2(sinth, costh);
+  P1 = c.dms1 * (d1 - c.dms2) + c.dms3 * (ang - ath0);
+  P2 = c.dms1 * (d2 - c.dms2) + c.dms3 * (ang - ath0);
+  q3[0] = q3[0] - (P1 + P2);
+  q3[1] = q3[1] + P1;
+  q3[2] = q3[2] + P2;
+  p1r1 = p1r1 + c.dms1;
+  p2r2 = p2r2 + c.dms1;
+  p1c = p1c - c.dms3 / sinth;
+  p2c = p2c - c.dms3 / sinth;
+  double f1q1r13 = (p1r1 - (p1c * costh / d1)) / d1;
+  double f1q1r23 = p1c / (d1 * d2);
+  double f2q1r23 = (p1r2 - (p1c * costh / d2)) / d2;
+  double f2q1r13 = p1c / (d2 * d1);
+  double f1q2r13 = (p2r1 - (p2c * costh / d1)) / d1;
+  double f1q2r23 = p2c / (d1 * d2);
+  double f2q2r23 = (p2r2 - (p2c * costh / d2)) / d2;
+  double f2q2r13 = p2c / (d2 * d1);
+  for (int k = 0; k < 3; ++k) {
+    gq[0][0][k] = f1q1r13 * R1[k] + f1q1r23 * R2[k];
+    gq[1][0][k] = f2q1r13 * R1[k] + f2q1r23 * R2[k];
+    gq[2][0][k] = -(gq[0][0][k] + gq[1][0][k]);
+    gq[0][1][k] = f1q2r13 * R1[k] + f1q2r23 * R2[k];
+    gq[1][1][k] = f2q2r13 * R1[k] + f2q2r23 * R2[k];
+    gq[2][1][k] = -(gq[0][1][k] + gq[1][1][k]);
+    gq[0][2][k] = -(gq[0][0][k] + gq[0][1][k]);
+    gq[1][2][k] = -(gq[1][0][k] + gq[1][1][k]);
+    gq[2][2][k] = -(gq[2][0][k] + gq[2][1][k]);
+  }
+}
+
+}  // namespace pimd

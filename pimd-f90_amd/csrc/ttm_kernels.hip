@@ -1,0 +1,1304 @@
+// HIP kernels of the TTM2.1-F / TTM3-F force path for all P beads of a system.
+//
+// Layout in HBM (per context):
+//   site  [P][N][4]       Site records (x, y, z, q) of O, H1, H2, M
+//   phi, Efq, dR          per-site potential, charge field, gradient
+//   dip, Efd [P][N][nd][3] induced dipoles and their field (nd = 1 for TTM3-F
+//                           (M site), 3 for TTM2.1-F (O, H1, H2))
+//   tab   [P][T][4N]      e^{i 2 pi h x/L} per site for h = 0..hmax (x),
+//                         0..kmax (y), 0..lmax (z)
+//   Sq, Sm, Sv [P][nk]    structure factors of charges, dipoles, dipole vectors
+//   tally [P][N][14]      per-molecule partial energies / virials
+//
+// Real-space kernels give each molecule a group of G lanes that walks its full
+// neighbour list and accumulates only that molecule's sites, so every result is
+// written by exactly one owner and sums are deterministic without atomics.
+// Pair scalars (energies, virials) are counted once, from the lower-index
+// molecule, in the reference's orientation.
+#include "ttm_device.h"
+
+namespace pimd {
+
+__constant__ double c_c5z[PS_NTERM];
+__constant__ int c_idx[PS_NTERM][3];
+__constant__ double c_coefd[PS_NDMS];
+__constant__ int c_idxd[PS_NDMS][3];
+
+// c5z = f5z*c5zA + fbasis*cbasis + fcore*ccore + frest*crest (nasa.f90:45).
+void set_constant_tables() {
+  static bool done = false;
+  if (done) return;
+  double c5z[PS_NTERM];
+  int idx[PS_NTERM][3], idxd[PS_NDMS][3];
+  for (int j = 0; j < PS_NTERM; ++j) {
+    c5z[j] = PS_F5Z * PS_C5ZA[j] + PS_FBASIS * PS_CBASIS[j] + PS_FCORE * PS_CCORE[j] +
+             PS_FREST * PS_CREST[j];
+    idx[j][0] = PS_IDX1[j];
+    idx[j][1] = PS_IDX2[j];
+    idx[j][2] = PS_IDX3[j];
+  }
+  for (int j = 0; j < PS_NDMS; ++j) {
+    idxd[j][0] = PS_IDXD1[j];
+    idxd[j][1] = PS_IDXD2[j];
+    idxd[j][2] = PS_IDXD3[j];
+  }
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_c5z), c5z, sizeof c5z);
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_idx), idx, sizeof idx);
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_coefd), PS_COEFD, sizeof(double) * PS_NDMS);
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_idxd), idxd, sizeof idxd);
+  done = true;
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+__device__ __forceinline__ void vadd9(double *t, const double *v) {
+  for (int i = 0; i < 9; ++i) t[i] += v[i];
+}
+
+// ---------------------------------------------------------------------------
+// Monomer stage (pot_ttm.f90:56-151 and 232-261): M sites, PS energy and
+// gradient, DMS charges and their gradients, intramolecular virial, Ewald
+// self energy and the screened intramolecular charge pairs.  One thread per
+// (bead, molecule).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__restrict__ RR,
+                                               const double *__restrict__ RRc, Site *site,
+                                               double *Rcs, double *grdq, double *phi,
+                                               double *Efq, double *dR, double *tally,
+                                               double *chg_out) {
+  int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= c.P * c.N) return;
+  const int b = gid / c.N, w = gid % c.N, N = c.N;
+  const double *rr = RR + ((size_t)b * 3 * N + 3 * w) * 3;
+  double r[3][3], rc[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) {
+      r[a][k] = rr[3 * a + k];
+      rc[a][k] = RRc[(3 * w + a) * 3 + k];
+    }
+  const double gM = c.gammaM;
+  double pos[4][3], cen[4][3];
+  {
+    double o1[3], o2[3], c1[3], c2[3];
+    for (int k = 0; k < 3; ++k) {
+      o1[k] = r[1][k] - r[0][k];
+      o2[k] = r[2][k] - r[0][k];
+      c1[k] = rc[1][k] - rc[0][k];
+      c2[k] = rc[2][k] - rc[0][k];
+    }
+    minimg(c, o1[0], o1[1], o1[2]);
+    minimg(c, o2[0], o2[1], o2[2]);
+    minimg(c, c1[0], c1[1], c1[2]);
+    minimg(c, c2[0], c2[1], c2[2]);
+    for (int k = 0; k < 3; ++k) {
+      for (int a = 0; a < 3; ++a) {
+        pos[a][k] = r[a][k];
+        cen[a][k] = rc[a][k];
+      }
+      pos[3][k] = 0.5 * gM * (o1[k] + o2[k]) + r[0][k];
+      cen[3][k] = 0.5 * gM * (c1[k] + c2[k]) + rc[0][k];
+    }
+  }
+  if (b == 0)
+    for (int t = 0; t < 4; ++t)
+      for (int k = 0; k < 3; ++k) Rcs[(w * 4 + t) * 3 + k] = cen[t][k];
+
+  double e1, dr1[3][3], q3[3], gq[3][3][3];
+  ps_monomer(c, r, e1, dr1, q3, gq);
+  for (int a = 0; a < 3; ++a) q3[a] = q3[a] * CHARGECON;
+  if (chg_out)
+    for (int a = 0; a < 3; ++a) chg_out[(size_t)b * 3 * N + 3 * w + a] = q3[a];
+  const double tq = 0.5 * gM / (1.0 - gM);
+  double q[4];
+  q[0] = 0.0;
+  q[1] = q3[1] + tq * (q3[1] + q3[2]);
+  q[2] = q3[2] + tq * (q3[1] + q3[2]);
+  q[3] = q3[0] / (1.0 - gM);
+  Site *sm = site + ((size_t)b * N + w) * 4;
+  for (int t = 0; t < 4; ++t) sm[t] = Site{pos[t][0], pos[t][1], pos[t][2], q[t]};
+  double *g = grdq + ((size_t)b * N + w) * 27;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) {
+      double d0 = gq[a][0][k] * CHARGECON, d1 = gq[a][1][k] * CHARGECON,
+             d2 = gq[a][2][k] * CHARGECON;
+      g[a * 9 + 0 * 3 + k] = d0 + tq * (d0 + d1);
+      g[a * 9 + 1 * 3 + k] = d1 + tq * (d0 + d1);
+      g[a * 9 + 2 * 3 + k] = d2 - 2.0 * tq * (d0 + d1);
+    }
+  // intramolecular virial of the PS gradient (pot_ttm.f90:109-131)
+  double vt[9] = {0}, vc = 0.0;
+  {
+    double o1[3], o2[3], c1[3], c2[3];
+    for (int k = 0; k < 3; ++k) {
+      o1[k] = r[1][k] - r[0][k];
+      o2[k] = r[2][k] - r[0][k];
+      c1[k] = rc[1][k] - rc[0][k];
+      c2[k] = rc[2][k] - rc[0][k];
+    }
+    minimg(c, o1[0], o1[1], o1[2]);
+    minimg(c, o2[0], o2[1], o2[2]);
+    minimg(c, c1[0], c1[1], c1[2]);
+    minimg(c, c2[0], c2[1], c2[2]);
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row)
+        vt[col * 3 + row] += o1[row] * dr1[1][col] + o2[row] * dr1[2][col];
+    vc = vc + (c1[0] * dr1[1][0] + c1[1] * dr1[1][1] + c1[2] * dr1[1][2]);
+    vc = vc + (c2[0] * dr1[2][0] + c2[1] * dr1[2][1] + c2[2] * dr1[2][2]);
+  }
+  // Ewald self energy and screened intramolecular charge pairs (pot_ttm.f90:232-261)
+  double ph[4], ef[4][3];
+  for (int t = 0; t < 4; ++t) {
+    ph[t] = t == 0 ? 0.0 : -c.self_fac * q[t];
+    ef[t][0] = ef[t][1] = ef[t][2] = 0.0;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 4; ++j) {
+      double x = pos[i][0] - pos[j][0], y = pos[i][1] - pos[j][1], z = pos[i][2] - pos[j][2];
+      minimg(c, x, y, z);
+      double d2 = norm2_exact(x, y, z), ts0, ts1;
+      self1(c, d2, ts0, ts1);
+      ph[i] = ph[i] + ts0 * q[j];
+      ph[j] = ph[j] + ts0 * q[i];
+      double rv[3] = {x, y, z}, d[3];
+      for (int k = 0; k < 3; ++k) {
+        ef[i][k] = ef[i][k] + ts1 * q[j] * rv[k];
+        ef[j][k] = ef[j][k] - ts1 * q[i] * rv[k];
+        d[k] = -(ts1 * q[i] * q[j] * rv[k]);
+      }
+      double cx = cen[i][0] - cen[j][0], cy = cen[i][1] - cen[j][1], cz = cen[i][2] - cen[j][2];
+      minimg(c, cx, cy, cz);
+      vc = vc + (cx * d[0] + cy * d[1] + cz * d[2]);
+      for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * rv[row];
+    }
+  const size_t base = ((size_t)b * N + w) * 4;
+  for (int t = 0; t < 4; ++t) {
+    phi[base + t] = ph[t];
+    for (int k = 0; k < 3; ++k) {
+      Efq[(base + t) * 3 + k] = ef[t][k];
+      dR[(base + t) * 3 + k] = t < 3 ? dr1[t][k] : 0.0;
+    }
+  }
+  double *tl = tally + ((size_t)b * N + w) * T_COUNT;
+  tl[T_UMON] = e1;
+  tl[T_UVDW] = 0.0;
+  tl[T_UELEC] = 0.0;
+  tl[T_UIND] = 0.0;
+  tl[T_VIRC] = vc;
+  for (int i = 0; i < 9; ++i) tl[T_VIRT + i] = vt[i];
+}
+
+void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
+                    double *chg_out, hipStream_t s) {
+  int n = c.P * c.N;
+  hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.site, b.Rcs,
+                     b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out);
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists (find_neigh.f90:14-120): for every bead and molecule i, all
+// molecules j != i whose minimum-image O-O distance satisfies r^2 < Rc^2, in
+// ascending j.  One wavefront per (bead, i); ballot + mbcnt compaction.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__restrict__ site,
+                                                  int *nbr, int *nbrc) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= c.P * c.N) return;
+  const int b = wave / c.N, i = wave % c.N, N = c.N;
+  const Site Oi = site[((size_t)b * N + i) * 4];
+  int count = 0;
+  int *out = nbr + (size_t)wave * c.nbr_cap;
+  for (int j0 = 0; j0 < N; j0 += 64) {
+    const int j = j0 + lane;
+    bool pred = false;
+    if (j < N && j != i) {
+      const Site Oj = site[((size_t)b * N + j) * 4];
+      double x = Oi.x - Oj.x, y = Oi.y - Oj.y, z = Oi.z - Oj.z;
+      minimg(c, x, y, z);
+      pred = norm2_exact(x, y, z) < c.rc2;
+    }
+    const unsigned long long mask = __ballot(pred);
+    if (pred) {
+      const int pos = count + __popcll(mask & ((1ull << lane) - 1ull));
+      if (pos < c.nbr_cap) out[pos] = j;
+    }
+    count += __popcll(mask);
+  }
+  if (lane == 0) nbrc[wave] = count < c.nbr_cap ? count : c.nbr_cap;
+}
+
+void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
+  long threads = (long)c.P * c.N * 64;
+  hipLaunchKernelGGL(k_neighbors, dim3((threads + 255) / 256), dim3(256), 0, s, c, b.site,
+                     b.nbr, b.nbrc);
+}
+
+// Shift vector Rij0 of the reference's pair loops for the ordered pair
+// (lo, hi): Rij - R(iO) + R(jO) with Rij the O-O minimum image
+// (pot_ttm.f90:193, 318-319, 406-407); wrapped when asked.
+__device__ __forceinline__ void shift0(const SysConst &c, const Site &Olo, const Site &Ohi,
+                                       bool wrap, double s0[3]) {
+  double x = Olo.x - Ohi.x, y = Olo.y - Ohi.y, z = Olo.z - Ohi.z;
+  minimg(c, x, y, z);
+  s0[0] = x - Olo.x + Ohi.x;
+  s0[1] = y - Olo.y + Ohi.y;
+  s0[2] = z - Olo.z + Ohi.z;
+  if (wrap) minimg(c, s0[0], s0[1], s0[2]);
+}
+
+// Site-site vector of the reference for (lo site, hi site): min image of
+// (R_lo + Rij0) - R_hi.
+__device__ __forceinline__ void site_vec(const SysConst &c, const Site &Slo, const Site &Shi,
+                                         const double s0[3], double v[3]) {
+  v[0] = (Slo.x + s0[0]) - Shi.x;
+  v[1] = (Slo.y + s0[1]) - Shi.y;
+  v[2] = (Slo.z + s0[2]) - Shi.z;
+  minimg(c, v[0], v[1], v[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Real-space vdW + charge-charge pass (pot_ttm.f90:159-228).
+// ---------------------------------------------------------------------------
+template <int MODEL, int G>
+__global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restrict__ site,
+                                                const double *__restrict__ Rcs,
+                                                const int *__restrict__ nbr,
+                                                const int *__restrict__ nbrc, double *phi,
+                                                double *Efq, double *dR, double *tally) {
+  constexpr int SA0 = MODEL == 2 ? 0 : 1;  // TTM3-F: O carries neither charge nor dipole
+  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long grp = gtid / G;
+  const int lane = (int)(gtid % G);
+  const int N = c.N;
+  const bool valid = grp < (long)c.P * N;
+  const int b = valid ? (int)(grp / N) : 0, a = valid ? (int)(grp % N) : 0;
+  Site A[4];
+  double RcA[4][3];
+  for (int t = 0; t < 4; ++t) {
+    A[t] = site[((size_t)b * N + a) * 4 + t];
+    for (int k = 0; k < 3; ++k) RcA[t][k] = Rcs[(a * 4 + t) * 3 + k];
+  }
+  double ph[4] = {0, 0, 0, 0}, ef[4][3] = {{0}}, dro[3] = {0, 0, 0};
+  double uvdw = 0.0, vc = 0.0, vt[9] = {0};
+  const int cnt = valid ? nbrc[grp] : 0;
+  const int *list = nbr + (size_t)grp * c.nbr_cap;
+  for (int n = lane; n < cnt; n += G) {
+    const int j = list[n];
+    const bool lo = a < j;
+    Site B[4];
+    for (int t = 0; t < 4; ++t) B[t] = site[((size_t)b * N + j) * 4 + t];
+    // vdW between oxygens
+    double r[3] = {A[0].x - B[0].x, A[0].y - B[0].y, A[0].z - B[0].z};
+    minimg(c, r[0], r[1], r[2]);
+    const double r2 = norm2_exact(r[0], r[1], r[2]);
+    const double dRij = sqrt(r2);
+    const double R2i = 1.0 / r2, R4i = R2i * R2i, R6i = R2i * R2i * R2i, R8i = R6i * R2i;
+    const double ex = c.vdwD * exp(-c.vdwE * dRij);
+    const double uij = R6i * ((c.vdwA * R6i + c.vdwB * R4i) + c.vdwC) + ex;
+    const double xs = dRij - c.rc1;
+    const double fsc = -(R8i * 6.0 * c.vdwC + ex * c.vdwE / dRij + c.shiftA * (xs * xs) +
+                         c.shiftB * (xs * xs * xs));
+    const double d[3] = {fsc * r[0], fsc * r[1], fsc * r[2]};
+    for (int k = 0; k < 3; ++k) dro[k] += d[k];
+    if (lo) {
+      uvdw += uij;
+      double cx = RcA[0][0] - Rcs[(j * 4) * 3 + 0], cy = RcA[0][1] - Rcs[(j * 4) * 3 + 1],
+             cz = RcA[0][2] - Rcs[(j * 4) * 3 + 2];
+      minimg(c, cx, cy, cz);
+      vc += cx * d[0] + cy * d[1] + cz * d[2];
+      for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
+    }
+    double s0[3];
+    if (lo)
+      shift0(c, A[0], B[0], false, s0);
+    else
+      shift0(c, B[0], A[0], false, s0);
+#pragma unroll
+    for (int sa = SA0; sa < 4; ++sa) {
+#pragma unroll
+      for (int sb = 1; sb < 4; ++sb) {
+        double v[3], rv[3];
+        if (lo) {
+          site_vec(c, A[sa], B[sb], s0, v);
+          rv[0] = v[0]; rv[1] = v[1]; rv[2] = v[2];
+        } else {
+          site_vec(c, B[sb], A[sa], s0, v);
+          rv[0] = -v[0]; rv[1] = -v[1]; rv[2] = -v[2];
+        }
+        const double d2 = norm2_exact(v[0], v[1], v[2]);
+        double ts0, ts1;
+        smear01(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa],
+                lo ? c.raCC[sa][sb] : c.raCC[sb][sa], ts0, ts1);
+        const double qb = B[sb].q;
+        if (sa > 0) ph[sa] += ts0 * qb;
+        for (int k = 0; k < 3; ++k) ef[sa][k] += ts1 * qb * rv[k];
+        if (lo && sa > 0) {
+          const double qa = A[sa].q;
+          double dd[3];
+          for (int k = 0; k < 3; ++k) dd[k] = -(ts1 * qa * qb * v[k]);
+          double rcv[3];
+          for (int k = 0; k < 3; ++k) rcv[k] = RcA[sa][k] - Rcs[(j * 4 + sb) * 3 + k] + s0[k];
+          vc += rcv[0] * dd[0] + rcv[1] * dd[1] + rcv[2] * dd[2];
+          for (int col = 0; col < 3; ++col)
+            for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dd[col] * v[row];
+        }
+      }
+    }
+  }
+  for (int t = 1; t < 4; ++t) ph[t] = group_sum<G>(ph[t]);
+  for (int t = SA0; t < 4; ++t)
+    for (int k = 0; k < 3; ++k) ef[t][k] = group_sum<G>(ef[t][k]);
+  for (int k = 0; k < 3; ++k) dro[k] = group_sum<G>(dro[k]);
+  uvdw = group_sum<G>(uvdw);
+  vc = group_sum<G>(vc);
+  for (int i = 0; i < 9; ++i) vt[i] = group_sum<G>(vt[i]);
+  if (valid && lane == 0) {
+    const size_t base = ((size_t)b * N + a) * 4;
+    for (int t = 1; t < 4; ++t) phi[base + t] += ph[t];
+    for (int t = SA0; t < 4; ++t)
+      for (int k = 0; k < 3; ++k) Efq[(base + t) * 3 + k] += ef[t][k];
+    for (int k = 0; k < 3; ++k) dR[base * 3 + k] += dro[k];
+    double *tl = tally + ((size_t)b * N + a) * T_COUNT;
+    tl[T_UVDW] += uvdw;
+    tl[T_VIRC] += vc;
+    vadd9(tl + T_VIRT, vt);
+  }
+}
+
+template <int G>
+static void launch_qq_g(const SysConst &c, DevBufs &b, hipStream_t s) {
+  long threads = (long)c.P * c.N * G;
+  dim3 grid((threads + 255) / 256);
+  if (c.model == 2)
+    hipLaunchKernelGGL((k_pair_qq<2, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr, b.nbrc,
+                       b.phi, b.Efq, b.dR, b.tally);
+  else
+    hipLaunchKernelGGL((k_pair_qq<3, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr, b.nbrc,
+                       b.phi, b.Efq, b.dR, b.tally);
+}
+
+void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  if (group == 64)
+    launch_qq_g<64>(c, b, s);
+  else if (group == 32)
+    launch_qq_g<32>(c, b, s);
+  else
+    launch_qq_g<16>(c, b, s);
+}
+
+// ---------------------------------------------------------------------------
+// Ewald reciprocal space (ewald.f90:1-226, 403-437).
+// ---------------------------------------------------------------------------
+
+// Per-site phase tables, the reference's ssincos recurrence (ewald.f90:403-437).
+__global__ __launch_bounds__(256) void k_tables(SysConst c, const Site *__restrict__ site,
+                                               double2 *tab) {
+  const int S = 4 * c.N;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const Site p = site[(size_t)b * S + s];
+  const double xyz[3] = {p.x, p.y, p.z};
+  const int hm[3] = {c.hmax, c.kmax, c.lmax};
+  int row = 0;
+  double2 *tb = tab + (size_t)b * c.T * S + s;
+  for (int ax = 0; ax < 3; ++ax) {
+    const double arg = TWOPI * c.boxi[ax] * xyz[ax];
+    const double c1 = cos(arg), s1 = sin(arg);
+    double cc = 1.0, ss = 0.0;
+    tb[(size_t)row++ * S] = make_double2(1.0, 0.0);
+    for (int h = 1; h <= hm[ax]; ++h) {
+      double nc, ns;
+      if (h == 1) {
+        nc = c1;
+        ns = s1;
+      } else {
+        nc = cc * c1 - ss * s1;
+        ns = cc * s1 + ss * c1;
+      }
+      cc = nc;
+      ss = ns;
+      tb[(size_t)row++ * S] = make_double2(cc, ss);
+    }
+  }
+}
+
+void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s) {
+  const int S = 4 * c.N;
+  hipLaunchKernelGGL(k_tables, dim3((S + 255) / 256, c.P), dim3(256), 0, s, c, b.site, b.tab);
+}
+
+// Site index of the m-th member of a site set.  SET_CHG: H1, H2, M of every
+// molecule; SET_DIP: the dipole sites; SET_ALL: all four.
+enum SiteSet { SET_CHG = 0, SET_DIP = 1, SET_ALL = 2 };
+
+__device__ __forceinline__ int set_size(const SysConst &c, int set) {
+  return set == SET_CHG ? 3 * c.N : set == SET_DIP ? c.nd * c.N : 4 * c.N;
+}
+__device__ __forceinline__ int set_site(const SysConst &c, int set, int m) {
+  if (set == SET_CHG) return (m / 3) * 4 + 1 + m % 3;
+  if (set == SET_DIP) return (m / c.nd) * 4 + c.tdip + m % c.nd;
+  return m;
+}
+
+// Phase e^{i k.r} of one site from its table rows, in the reference's
+// product order (ewald.f90:30-39).
+__device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, double &ck,
+                                           double &sk) {
+  const double cxy = x.x * y.x - x.y * y.y;
+  const double sxy = x.y * y.x + x.x * y.y;
+  ck = cxy * z.x - sxy * z.y;
+  sk = sxy * z.x + cxy * z.y;
+}
+
+// Structure factors over a site set, one k per lane; the block's four waves
+// split the sites and combine through LDS.
+//   SF_Q: (sum q cos, sum q sin)                      -> Sq  (ewald.f90:41-44)
+//   SF_D: (-sum (k.mu) sin, sum (k.mu) cos)           -> Sm  (ewald.f90:99-110)
+//   SF_V: (sum mu cos, sum mu sin) per component      -> Sv  (ewald.f90:208-213)
+constexpr int SF_TILE = 32;
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sfac(SysConst c, const Site *__restrict__ site,
+                                             const double2 *__restrict__ tab,
+                                             const double *__restrict__ dip,
+                                             const double4 *__restrict__ kv,
+                                             const int4 *__restrict__ khkl,
+                                             const int *__restrict__ active, double *out) {
+  constexpr int NV = MODE == SF_V ? 6 : 2;
+  constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
+  extern __shared__ double2 lds_tab[];  // [T][SF_TILE]
+  __shared__ double lds_w[SF_TILE][3];
+  __shared__ double lds_red[3][64][NV];
+  const int b = blockIdx.y;
+  if (MODE == SF_D && !active[b]) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + lane;
+  const int kk = k < c.nk ? k : c.nk - 1;
+  const int4 h = khkl[kk];
+  const double4 kvec = kv[kk];
+  const int S = 4 * c.N;
+  const int rowy = c.hmax + 1 + abs(h.y), rowz = c.hmax + c.kmax + 2 + abs(h.z);
+  const double sy = h.y < 0 ? -1.0 : 1.0, sz = h.z < 0 ? -1.0 : 1.0;
+  double acc[NV];
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const int M = set_size(c, SET);
+  const double2 *tb = tab + (size_t)b * c.T * S;
+  for (int m0 = 0; m0 < M; m0 += SF_TILE) {
+    const int nt = min(SF_TILE, M - m0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < c.T * SF_TILE; e += blockDim.x) {
+      const int row = e / SF_TILE, col = e % SF_TILE;
+      if (col < nt) lds_tab[e] = tb[(size_t)row * S + set_site(c, SET, m0 + col)];
+    }
+    for (int e = threadIdx.x; e < SF_TILE; e += blockDim.x) {
+      if (e < nt) {
+        const int m = m0 + e;
+        if (MODE == SF_Q) {
+          lds_w[e][0] = site[(size_t)b * S + set_site(c, SET, m)].q;
+        } else {
+          const double *mu = dip + ((size_t)b * c.N * c.nd + m) * 3;
+          lds_w[e][0] = mu[0];
+          lds_w[e][1] = mu[1];
+          lds_w[e][2] = mu[2];
+        }
+      }
+    }
+    __syncthreads();
+    for (int e = wv; e < nt; e += 4) {
+      const double2 x = lds_tab[h.x * SF_TILE + e];
+      double2 y = lds_tab[rowy * SF_TILE + e];
+      double2 z = lds_tab[rowz * SF_TILE + e];
+      y.y *= sy;
+      z.y *= sz;
+      double ck, sk;
+      phase_from(x, y, z, ck, sk);
+      if (MODE == SF_Q) {
+        const double q = lds_w[e][0];
+        acc[0] += q * ck;
+        acc[1] += q * sk;
+      } else if (MODE == SF_D) {
+        const double dk = (kvec.x * lds_w[e][0] + kvec.y * lds_w[e][1]) + kvec.z * lds_w[e][2];
+        acc[0] -= dk * sk;
+        acc[1] += dk * ck;
+      } else {
+        for (int cc = 0; cc < 3; ++cc) {
+          acc[cc] += lds_w[e][cc] * ck;
+          acc[3 + cc] += lds_w[e][cc] * sk;
+        }
+      }
+    }
+  }
+  if (wv > 0)
+    for (int v = 0; v < NV; ++v) lds_red[wv - 1][lane][v] = acc[v];
+  __syncthreads();
+  if (wv == 0 && k < c.nk) {
+    for (int v = 0; v < NV; ++v) {
+      double t = ((acc[v] + lds_red[0][lane][v]) + lds_red[1][lane][v]) + lds_red[2][lane][v];
+      out[((size_t)b * c.nk + k) * NV + v] = t;
+    }
+  }
+}
+
+void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
+  dim3 grid((c.nk + 63) / 64, c.P);
+  size_t sh = sizeof(double2) * c.T * SF_TILE;
+  if (mode == SF_Q)
+    hipLaunchKernelGGL(k_sfac<SF_Q>, grid, dim3(256), sh, s, c, b.site, b.tab, b.dip, b.kv,
+                       b.khkl, b.active, b.Sq);
+  else if (mode == SF_D)
+    hipLaunchKernelGGL(k_sfac<SF_D>, grid, dim3(256), sh, s, c, b.site, b.tab, b.dip, b.kv,
+                       b.khkl, b.active, b.Sm);
+  else
+    hipLaunchKernelGGL(k_sfac<SF_V>, grid, dim3(256), sh, s, c, b.site, b.tab, b.dip, b.kv,
+                       b.khkl, b.active, b.Sv);
+}
+
+// Reciprocal-space fields at each site, walking the k list column by column
+// (fixed kx, ky; consecutive kz) with the phase advanced by e^{i 2 pi z/L}.
+//   FD_Q : Efq and phi from the charge structure factor (ewald.f90:45-49)
+//   FD_D : Efd from the dipole structure factor (ewald.f90:114-117), then the
+//          SOR update of the induced dipole (pot_ttm.f90:371-385)
+//   FD_QD: phi and dR from the stale dipole and the total structure factors
+//          (ewald.f90:184-193)
+template <int MODE, int MODEL>
+__global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restrict__ site,
+                                              const double2 *__restrict__ tab,
+                                              const double4 *__restrict__ kv,
+                                              const int4 *__restrict__ cols,
+                                              const int *__restrict__ colk,
+                                              const double *__restrict__ Sq,
+                                              const double *__restrict__ Sm, double *phi,
+                                              double *Efq, double *dR, double *dip,
+                                              const double *__restrict__ Efd, double *diff2,
+                                              const int *__restrict__ active, int iter) {
+  constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
+  const int b = blockIdx.y;
+  if (MODE == FD_D && !active[b]) return;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = set_size(c, SET);
+  if (m >= M) return;
+  const int S = 4 * c.N;
+  const int s = set_site(c, SET, m);
+  const int type = s & 3;
+  const Site p = site[(size_t)b * S + s];
+  const double2 *tb = tab + (size_t)b * c.T * S + s;
+  // e^{i 2 pi z/L}; with lmax = 0 every column holds one k and no step is taken
+  const double2 zstep = c.lmax >= 1 ? tb[(size_t)(c.hmax + c.kmax + 3) * S] : make_double2(1.0, 0.0);
+  const bool has_q = type != 0;
+  const bool has_d = (type >= c.tdip && type < c.tdip + c.nd);
+  double mu[3] = {0, 0, 0};
+  const size_t dslot = ((size_t)b * c.N + (s >> 2)) * c.nd + (type - c.tdip);
+  if ((MODE == FD_D || MODE == FD_QD) && has_d)
+    for (int k = 0; k < 3; ++k) mu[k] = dip[dslot * 3 + k];
+  double e0 = 0, e1 = 0, e2 = 0, ph = 0;
+  for (int col = 0; col < c.ncol; ++col) {
+    const int4 cl = cols[col];
+    const int k0 = colk[col];
+    double2 x = tb[(size_t)cl.x * S];
+    double2 y = tb[(size_t)(c.hmax + 1 + abs(cl.y)) * S];
+    double2 z = tb[(size_t)(c.hmax + c.kmax + 2 + abs(cl.z)) * S];
+    if (cl.y < 0) y.y = -y.y;
+    if (cl.z < 0) z.y = -z.y;
+    double ck, sk;
+    phase_from(x, y, z, ck, sk);
+    for (int i = 0; i < cl.w; ++i) {
+      if (i > 0) {
+        const double nc = ck * zstep.x - sk * zstep.y;
+        sk = sk * zstep.x + ck * zstep.y;
+        ck = nc;
+      }
+      const int k = k0 + i;
+      const double4 kk = kv[k];  // kx, ky, kz, 4 pi A_k / V
+      if (MODE == FD_Q) {
+        const double re = Sq[((size_t)b * c.nk + k) * 2], im = Sq[((size_t)b * c.nk + k) * 2 + 1];
+        const double t = kk.w * (ck * im - sk * re);
+        e0 -= t * kk.x;
+        e1 -= t * kk.y;
+        e2 -= t * kk.z;
+        ph += kk.w * (ck * re + sk * im);
+      } else if (MODE == FD_D) {
+        const double re = Sm[((size_t)b * c.nk + k) * 2], im = Sm[((size_t)b * c.nk + k) * 2 + 1];
+        const double t = kk.w * (ck * im - sk * re);
+        e0 -= kk.x * t;
+        e1 -= kk.y * t;
+        e2 -= kk.z * t;
+      } else {
+        const double rqm = Sm[((size_t)b * c.nk + k) * 2], iqm = Sm[((size_t)b * c.nk + k) * 2 + 1];
+        if (has_q) {
+          const double t = p.q * kk.w * (ck * iqm - sk * rqm);
+          ph += kk.w * (ck * rqm + sk * iqm);
+          e0 += t * kk.x;
+          e1 += t * kk.y;
+          e2 += t * kk.z;
+        }
+        if (has_d) {
+          const double rqqm = Sq[((size_t)b * c.nk + k) * 2] + rqm;
+          const double iqqm = Sq[((size_t)b * c.nk + k) * 2 + 1] + iqm;
+          const double dk = (kk.x * mu[0] + kk.y * mu[1]) + kk.z * mu[2];
+          const double t = dk * kk.w * (ck * rqqm + sk * iqqm);
+          e0 -= t * kk.x;
+          e1 -= t * kk.y;
+          e2 -= t * kk.z;
+        }
+      }
+    }
+  }
+  const size_t ss = (size_t)b * S + s;
+  if (MODE == FD_Q) {
+    Efq[ss * 3 + 0] += e0;
+    Efq[ss * 3 + 1] += e1;
+    Efq[ss * 3 + 2] += e2;
+    if (has_q) phi[ss] += ph;
+  } else if (MODE == FD_QD) {
+    dR[ss * 3 + 0] += e0;
+    dR[ss * 3 + 1] += e1;
+    dR[ss * 3 + 2] += e2;
+    if (has_q) phi[ss] += ph;
+  } else {
+    const double e[3] = {e0, e1, e2};
+    const double om = c.sor, pol = c.polar[type];
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double etot = (Efd[dslot * 3 + k] + e[k]) + Efq[ss * 3 + k] + c.fac_dd * mu[k];
+      const double nm = (1.0 - om) * mu[k] + om * pol * etot;
+      const double df = nm - (iter == 1 ? 0.0 : mu[k]);
+      d2 += df * df;
+      dip[dslot * 3 + k] = nm;
+    }
+    diff2[(size_t)b * c.N * c.nd + m] = d2;
+  }
+}
+
+void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s) {
+  const int M = mode == FD_D ? c.nd * c.N : (c.model == 2 ? 4 * c.N : 3 * c.N);
+  dim3 grid((M + 255) / 256, c.P), blk(256);
+#define PIMD_FIELD(MO, ML)                                                                       \
+  hipLaunchKernelGGL((k_field<MO, ML>), grid, blk, 0, s, c, b.site, b.tab, b.kv, b.cols, b.colk, \
+                     b.Sq, b.Sm, b.phi, b.Efq, b.dR, b.dip, b.Efd, b.diff2, b.active, iter)
+  if (c.model == 2) {
+    if (mode == FD_Q) PIMD_FIELD(FD_Q, 2);
+    else if (mode == FD_D) PIMD_FIELD(FD_D, 2);
+    else PIMD_FIELD(FD_QD, 2);
+  } else {
+    if (mode == FD_Q) PIMD_FIELD(FD_Q, 3);
+    else if (mode == FD_D) PIMD_FIELD(FD_D, 3);
+    else PIMD_FIELD(FD_QD, 3);
+  }
+#undef PIMD_FIELD
+}
+
+// ---------------------------------------------------------------------------
+// After the charge pass: gradient of the charge energy, Uelec partials and the
+// initial induced dipoles (pot_ttm.f90:273-297).  One thread per (bead, molecule).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restrict__ site,
+                                               const double *__restrict__ phi,
+                                               const double *__restrict__ Efq, double *dR,
+                                               double *dip, const double *__restrict__ hist,
+                                               double *tally, int *active, int *iters,
+                                               int *nactive, int t) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= c.P * c.N) return;
+  const int b = gid / c.N, w = gid % c.N, N = c.N;
+  const size_t base = ((size_t)b * N + w) * 4;
+  double ue = 0.0;
+  for (int ty = 1; ty < 4; ++ty) {
+    const double q = site[base + ty].q;
+    ue += q * phi[base + ty];
+    for (int k = 0; k < 3; ++k) dR[(base + ty) * 3 + k] -= q * Efq[(base + ty) * 3 + k];
+  }
+  tally[((size_t)b * N + w) * T_COUNT + T_UELEC] = ue;
+  const int ndip = N * c.nd;
+  double dp[4];
+  const bool pred = t > 4 && c.guess;
+  if (pred) {  // calc_dpcf (pot_ttm.f90:649-675) for md_step > 4
+    const double tab4[4][4] = {{-1, 4, -6, 4}, {4, -1, 4, -6}, {-6, 4, -1, 4}, {4, -6, 4, -1}};
+    for (int i = 0; i < 4; ++i) dp[i] = tab4[t % 4][i];
+  }
+  for (int u = 0; u < c.nd; ++u) {
+    const int ty = c.tdip + u;
+    const size_t dslot = ((size_t)b * N + w) * c.nd + u;
+    for (int k = 0; k < 3; ++k) {
+      double v;
+      if (pred) {
+        const size_t m = (size_t)w * c.nd + u;
+        v = dp[0] * hist[(((size_t)b * 4 + 0) * ndip + m) * 3 + k];
+        for (int i = 1; i < 4; ++i) v = v + dp[i] * hist[(((size_t)b * 4 + i) * ndip + m) * 3 + k];
+      } else {
+        v = c.polar[ty] * Efq[(base + ty) * 3 + k];
+      }
+      dip[dslot * 3 + k] = v;
+    }
+  }
+  if (w == 0) {
+    active[b] = 1;
+    iters[b] = 0;
+  }
+  if (gid == 0)
+    for (int i = 0; i <= c.maxiter; ++i) nactive[i] = 0;
+}
+
+void launch_post_qq(const SysConst &c, DevBufs &b, int t, hipStream_t s) {
+  int n = c.P * c.N;
+  hipLaunchKernelGGL(k_post_qq, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.phi, b.Efq,
+                     b.dR, b.dip, b.hist, b.tally, b.active, b.iters, b.nactive, t);
+}
+
+// ---------------------------------------------------------------------------
+// Real-space dipole field T.mu of one SOR iteration (pot_ttm.f90:311-365).
+// ---------------------------------------------------------------------------
+template <int MODEL, int G>
+__global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restrict__ site,
+                                                const double *__restrict__ dip,
+                                                const int *__restrict__ nbr,
+                                                const int *__restrict__ nbrc,
+                                                const int *__restrict__ active, double *Efd) {
+  constexpr int ND = MODEL == 2 ? 3 : 1;
+  constexpr int T0 = MODEL == 2 ? 0 : 3;
+  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long grp = gtid / G;
+  const int lane = (int)(gtid % G);
+  const int N = c.N;
+  if (grp >= (long)c.P * N) return;
+  const int b = (int)(grp / N), a = (int)(grp % N);
+  if (!active[b]) return;
+  Site A[4];
+  for (int t = 0; t < 4; ++t) A[t] = site[((size_t)b * N + a) * 4 + t];
+  double ef[ND][3];
+  for (int u = 0; u < ND; ++u) ef[u][0] = ef[u][1] = ef[u][2] = 0.0;
+  const int cnt = nbrc[grp];
+  const int *list = nbr + (size_t)grp * c.nbr_cap;
+  for (int n = lane; n < cnt; n += G) {
+    const int j = list[n];
+    const bool lo = a < j;
+    const Site *Bs = site + ((size_t)b * N + j) * 4;
+    const Site BO = Bs[0];
+    double s0[3];
+    if (lo)
+      shift0(c, A[0], BO, true, s0);
+    else
+      shift0(c, BO, A[0], true, s0);
+    double mub[ND][3];
+    for (int u = 0; u < ND; ++u)
+      for (int k = 0; k < 3; ++k) mub[u][k] = dip[(((size_t)b * N + j) * ND + u) * 3 + k];
+#pragma unroll
+    for (int ua = 0; ua < ND; ++ua) {
+#pragma unroll
+      for (int ub = 0; ub < ND; ++ub) {
+        const int sa = T0 + ua, sb = T0 + ub;
+        const Site B = Bs[sb];
+        double v[3];
+        if (lo)
+          site_vec(c, A[sa], B, s0, v);
+        else
+          site_vec(c, B, A[sa], s0, v);
+        const double d2 = norm2_exact(v[0], v[1], v[2]);
+        double ts1, ts2;
+        smear2(c, d2, lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa], ts1, ts2);
+        double o[3];
+        dd3_apply(v, ts1, ts2, mub[ub], o);
+        for (int k = 0; k < 3; ++k) ef[ua][k] += o[k];
+      }
+    }
+  }
+  for (int u = 0; u < ND; ++u)
+    for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
+  if (lane == 0) {
+    if (MODEL == 2) {  // intramolecular dipole pairs of TTM2.1-F (pot_ttm.f90:344-365)
+      double mua[ND][3];
+      for (int u = 0; u < ND; ++u)
+        for (int k = 0; k < 3; ++k) mua[u][k] = dip[(((size_t)b * N + a) * ND + u) * 3 + k];
+      for (int u = 0; u < ND - 1; ++u)
+        for (int v2 = u + 1; v2 < ND; ++v2) {
+          const int sa = T0 + u, sb = T0 + v2;
+          double r[3] = {A[sa].x - A[sb].x, A[sa].y - A[sb].y, A[sa].z - A[sb].z};
+          minimg(c, r[0], r[1], r[2]);
+          const double d2 = norm2_exact(r[0], r[1], r[2]);
+          double ts1, ts2, o[3];
+          smear2(c, d2, c.apolDD[sa][sb], ts1, ts2);
+          dd3_apply(r, ts1, ts2, mua[v2], o);
+          for (int k = 0; k < 3; ++k) ef[u][k] += o[k];
+          dd3_apply(r, ts1, ts2, mua[u], o);
+          for (int k = 0; k < 3; ++k) ef[v2][k] += o[k];
+        }
+    }
+    for (int u = 0; u < ND; ++u)
+      for (int k = 0; k < 3; ++k) Efd[(((size_t)b * N + a) * ND + u) * 3 + k] = ef[u][k];
+  }
+}
+
+template <int G>
+static void launch_dd_g(const SysConst &c, DevBufs &b, hipStream_t s) {
+  long threads = (long)c.P * c.N * G;
+  dim3 grid((threads + 255) / 256);
+  if (c.model == 2)
+    hipLaunchKernelGGL((k_pair_dd<2, G>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr, b.nbrc,
+                       b.active, b.Efd);
+  else
+    hipLaunchKernelGGL((k_pair_dd<3, G>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr, b.nbrc,
+                       b.active, b.Efd);
+}
+
+void launch_pair_dd(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  if (group == 64)
+    launch_dd_g<64>(c, b, s);
+  else if (group == 32)
+    launch_dd_g<32>(c, b, s);
+  else
+    launch_dd_g<16>(c, b, s);
+}
+
+// Per-bead convergence test (pot_ttm.f90:380-385): deltadip =
+// DEBYE/CHARGECON/sqrt(Natoms) * ||mu_new - mu_old||, summed in a fixed order.
+__global__ __launch_bounds__(256) void k_converge(SysConst c, const double *__restrict__ diff2,
+                                                 int *active, int *iters, int *nactive,
+                                                 int iter) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  if (!active[b]) return;
+  const int M = c.N * c.nd;
+  double s = 0.0;
+  for (int m = threadIdx.x; m < M; m += 256) s += diff2[(size_t)b * M + m];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double delta = c.stath * sqrt(red[0]);
+    if (delta < c.peps) {
+      active[b] = 0;
+      iters[b] = iter;
+    } else if (iter >= c.maxiter) {
+      active[b] = 0;
+      iters[b] = -iter;
+    } else {
+      iters[b] = iter;
+      atomicAdd(&nactive[iter], 1);
+    }
+  }
+}
+
+void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
+  hipLaunchKernelGGL(k_converge, dim3(c.P), dim3(256), 0, s, c, b.diff2, b.active, b.iters,
+                     b.nactive, iter);
+}
+
+// ---------------------------------------------------------------------------
+// Real-space forces of the induced dipoles (pot_ttm.f90:399-478): dipole-dipole
+// (third-rank tensor) and charge-dipole terms, the dipole potential phi.
+// ---------------------------------------------------------------------------
+template <int MODEL>
+__device__ __forceinline__ bool is_dip(int t) {
+  return MODEL == 2 ? t < 3 : t == 3;
+}
+
+template <int MODEL, int G>
+__global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__restrict__ site,
+                                                   const double *__restrict__ Rcs,
+                                                   const double *__restrict__ dip,
+                                                   const int *__restrict__ nbr,
+                                                   const int *__restrict__ nbrc, double *phi,
+                                                   double *dR, double *tally) {
+  constexpr int ND = MODEL == 2 ? 3 : 1;
+  constexpr int T0 = MODEL == 2 ? 0 : 3;
+  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long grp = gtid / G;
+  const int lane = (int)(gtid % G);
+  const int N = c.N;
+  const bool valid = grp < (long)c.P * N;
+  const int b = valid ? (int)(grp / N) : 0, a = valid ? (int)(grp % N) : 0;
+  Site A[4];
+  double mua[4][3];
+  for (int t = 0; t < 4; ++t) {
+    A[t] = site[((size_t)b * N + a) * 4 + t];
+    for (int k = 0; k < 3; ++k)
+      mua[t][k] = is_dip<MODEL>(t) ? dip[(((size_t)b * N + a) * ND + (t - T0)) * 3 + k] : 0.0;
+  }
+  double ph[4] = {0, 0, 0, 0}, fr[4][3] = {{0}}, vc = 0.0, vt[9] = {0};
+  const int cnt = valid ? nbrc[grp] : 0;
+  const int *list = nbr + (size_t)grp * c.nbr_cap;
+  for (int n = lane; n < cnt; n += G) {
+    const int j = list[n];
+    const bool lo = a < j;
+    Site B[4];
+    double mub[4][3];
+    for (int t = 0; t < 4; ++t) {
+      B[t] = site[((size_t)b * N + j) * 4 + t];
+      for (int k = 0; k < 3; ++k)
+        mub[t][k] = is_dip<MODEL>(t) ? dip[(((size_t)b * N + j) * ND + (t - T0)) * 3 + k] : 0.0;
+    }
+    double s0[3];
+    if (lo)
+      shift0(c, A[0], B[0], true, s0);
+    else
+      shift0(c, B[0], A[0], true, s0);
+#pragma unroll
+    for (int sa = 0; sa < 4; ++sa) {
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb) {
+        const bool da = is_dip<MODEL>(sa), db = is_dip<MODEL>(sb);
+        // Pairs whose force and live potentials vanish identically: neither
+        // site carries a dipole, or the only coupling is a zero charge.
+        if (!da && !db) continue;
+        if (MODEL == 3 && (sa == 0 || sb == 0)) continue;
+        if (MODEL == 2 && !(da && db) && ((da && sb == 0) || (db && sa == 0))) continue;
+        double v[3], r[3];
+        if (lo) {
+          site_vec(c, A[sa], B[sb], s0, v);
+          r[0] = v[0]; r[1] = v[1]; r[2] = v[2];
+        } else {
+          site_vec(c, B[sb], A[sa], s0, v);
+          r[0] = -v[0]; r[1] = -v[1]; r[2] = -v[2];
+        }
+        const double d2 = norm2_exact(v[0], v[1], v[2]);
+        const double qa = A[sa].q, qb = B[sb].q;
+        double qd[3];
+        for (int k = 0; k < 3; ++k) qd[k] = qb * mua[sa][k] - qa * mub[sb][k];
+        const double qsq = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
+        double dij[3] = {0, 0, 0}, ts1 = 0.0, ts2 = 0.0;
+        bool have_ts1 = false;
+        if (da && db) {
+          double t1d, t2d, t3d;
+          smear3(c, d2, lo ? c.apol3DD[sa][sb] : c.apol3DD[sb][sa], t1d, t2d, t3d);
+          // di . T3(r) . dj with T3_abc = 15 r_a r_b r_c ts3 - 3 ts2 (r_a d_bc + r_b d_ac + r_c d_ab)
+          const double *di = mua[sa], *dj = mub[sb];
+          const double rdi = r[0] * di[0] + r[1] * di[1] + r[2] * di[2];
+          const double rdj = r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2];
+          const double didj = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+          for (int k = 0; k < 3; ++k)
+            dij[k] = 15.0 * t3d * rdi * rdj * r[k] - 3.0 * t2d * (rdi * dj[k] + rdj * di[k] + didj * r[k]);
+          if (qsq > 1e-9) {
+            if (c.same_a) {
+              ts1 = t1d;
+              ts2 = t2d;
+            } else {
+              smear2(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
+            }
+            have_ts1 = true;
+            double o[3];
+            dd3_apply(r, ts1, ts2, qd, o);
+            for (int k = 0; k < 3; ++k) dij[k] += o[k];
+          }
+        } else if (qsq > 1e-9) {
+          smear2(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
+          have_ts1 = true;
+          double o[3];
+          dd3_apply(r, ts1, ts2, qd, o);
+          for (int k = 0; k < 3; ++k) dij[k] += o[k];
+        }
+        if (have_ts1 && db) ph[sa] += ts1 * (r[0] * mub[sb][0] + r[1] * mub[sb][1] + r[2] * mub[sb][2]);
+        for (int k = 0; k < 3; ++k) fr[sa][k] += dij[k];
+        if (lo) {
+          double rcv[3];
+          for (int k = 0; k < 3; ++k) rcv[k] = Rcs[(a * 4 + sa) * 3 + k] - Rcs[(j * 4 + sb) * 3 + k] + s0[k];
+          vc += rcv[0] * dij[0] + rcv[1] * dij[1] + rcv[2] * dij[2];
+          for (int col = 0; col < 3; ++col)
+            for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dij[row] * v[col];
+        }
+      }
+    }
+  }
+  for (int t = 0; t < 4; ++t) {
+    ph[t] = group_sum<G>(ph[t]);
+    for (int k = 0; k < 3; ++k) fr[t][k] = group_sum<G>(fr[t][k]);
+  }
+  vc = group_sum<G>(vc);
+  for (int i = 0; i < 9; ++i) vt[i] = group_sum<G>(vt[i]);
+  if (valid && lane == 0) {
+    const size_t base = ((size_t)b * N + a) * 4;
+    for (int t = 0; t < 4; ++t) {
+      phi[base + t] += ph[t];
+      for (int k = 0; k < 3; ++k) dR[(base + t) * 3 + k] += fr[t][k];
+    }
+    double *tl = tally + ((size_t)b * N + a) * T_COUNT;
+    tl[T_VIRC] += vc;
+    vadd9(tl + T_VIRT, vt);
+  }
+}
+
+template <int G>
+static void launch_final_g(const SysConst &c, DevBufs &b, hipStream_t s) {
+  long threads = (long)c.P * c.N * G;
+  dim3 grid((threads + 255) / 256);
+  if (c.model == 2)
+    hipLaunchKernelGGL((k_pair_final<2, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
+                       b.nbr, b.nbrc, b.phi, b.dR, b.tally);
+  else
+    hipLaunchKernelGGL((k_pair_final<3, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
+                       b.nbr, b.nbrc, b.phi, b.dR, b.tally);
+}
+
+void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  if (group == 64)
+    launch_final_g<64>(c, b, s);
+  else if (group == 32)
+    launch_final_g<32>(c, b, s);
+  else
+    launch_final_g<16>(c, b, s);
+}
+
+// Reciprocal-space virial (ewald.f90:194-224), partial sums per block of k:
+// virk[b][blk][0..8] = sum Ureck (delta - coeff2 k k), [9..17] = dipole part.
+__global__ __launch_bounds__(256) void k_virk(SysConst c, const double4 *__restrict__ kv,
+                                             const double *__restrict__ Ak,
+                                             const double *__restrict__ Sq,
+                                             const double *__restrict__ Sm,
+                                             const double *__restrict__ Sv, double *virk) {
+  __shared__ double red[18][256];
+  const int b = blockIdx.y;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  double v[18];
+  for (int i = 0; i < 18; ++i) v[i] = 0.0;
+  if (k < c.nk) {
+    const double4 kk = kv[k];
+    const double kx[3] = {kk.x, kk.y, kk.z};
+    const double A = Ak[k];
+    const size_t o = (size_t)b * c.nk + k;
+    const double rqqm = Sq[o * 2] + Sm[o * 2], iqqm = Sq[o * 2 + 1] + Sm[o * 2 + 1];
+    const double ksq = kx[0] * kx[0] + kx[1] * kx[1] + kx[2] * kx[2];
+    const double coeff2 = 2.0 / ksq + 0.5 / (c.aewald * c.aewald);
+    const double U = A * (rqqm * rqqm + iqqm * iqqm);
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row)
+        v[col * 3 + row] = (row == col ? U : 0.0) - U * coeff2 * kx[row] * kx[col];
+    const double *sv = Sv + o * 6;  // sum mu cos (3), sum mu sin (3)
+    for (int ix = 0; ix < 3; ++ix) {
+      const double t = 2.0 * A * kx[ix];
+      for (int iy = 0; iy < 3; ++iy)
+        v[9 + iy * 3 + ix] = t * (rqqm * (-sv[3 + iy]) + iqqm * sv[iy]);
+    }
+  }
+  for (int i = 0; i < 18; ++i) red[i][threadIdx.x] = v[i];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int i = 0; i < 18; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 18) virk[((size_t)b * gridDim.x + blockIdx.x) * 18 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// Per-molecule finish (pot_ttm.f90:380-392, 482-542, 556-627): intramolecular
+// dipole terms, Uind, predictor history, charge-derivative chain rule, M-site
+// redistribution and molecular dipoles.  One thread per (bead, molecule).
+// ---------------------------------------------------------------------------
+template <int MODEL>
+__global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restrict__ site,
+                                                const double *__restrict__ Rcs,
+                                                const double *__restrict__ grdq,
+                                                const double *__restrict__ dip,
+                                                const double *__restrict__ Efq, double *phi,
+                                                double *dR, double *hist, double *tally,
+                                                double *dRR, double *dipI, double *dipE,
+                                                int t) {
+  constexpr int ND = MODEL == 2 ? 3 : 1;
+  constexpr int T0 = MODEL == 2 ? 0 : 3;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= c.P * c.N) return;
+  const int b = gid / c.N, w = gid % c.N, N = c.N;
+  const size_t base = ((size_t)b * N + w) * 4;
+  Site A[4];
+  double mu[4][3], cen[4][3], ph[4], fr[4][3];
+  for (int ty = 0; ty < 4; ++ty) {
+    A[ty] = site[base + ty];
+    ph[ty] = phi[base + ty];
+    for (int k = 0; k < 3; ++k) {
+      mu[ty][k] = is_dip<MODEL>(ty) ? dip[(((size_t)b * N + w) * ND + (ty - T0)) * 3 + k] : 0.0;
+      cen[ty][k] = Rcs[(w * 4 + ty) * 3 + k];
+      fr[ty][k] = dR[(base + ty) * 3 + k];
+    }
+  }
+  double vc = 0.0, vt[9] = {0};
+  // intramolecular dipole terms (pot_ttm.f90:482-542)
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 4; ++j) {
+      double r[3] = {A[i].x - A[j].x, A[i].y - A[j].y, A[i].z - A[j].z};
+      minimg(c, r[0], r[1], r[2]);
+      double rc[3] = {cen[i][0] - cen[j][0], cen[i][1] - cen[j][1], cen[i][2] - cen[j][2]};
+      minimg(c, rc[0], rc[1], rc[2]);
+      const double d2 = norm2_exact(r[0], r[1], r[2]);
+      const double *di = mu[i], *dj = mu[j];
+      if (is_dip<MODEL>(i) && is_dip<MODEL>(j)) {
+        double t1, t2, t3;
+        smear3(c, d2, c.apol3DD[i][j], t1, t2, t3);
+        const double rdi = r[0] * di[0] + r[1] * di[1] + r[2] * di[2];
+        const double rdj = r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2];
+        const double didj = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+        double dij[3];
+        for (int k = 0; k < 3; ++k)
+          dij[k] = 15.0 * t3 * rdi * rdj * r[k] - 3.0 * t2 * (rdi * dj[k] + rdj * di[k] + didj * r[k]);
+        for (int k = 0; k < 3; ++k) {
+          fr[i][k] += dij[k];
+          fr[j][k] -= dij[k];
+        }
+        vc += rc[0] * dij[0] + rc[1] * dij[1] + rc[2] * dij[2];
+        for (int col = 0; col < 3; ++col)
+          for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dij[row] * r[col];
+      }
+      double qd[3];
+      for (int k = 0; k < 3; ++k) qd[k] = A[j].q * di[k] - A[i].q * dj[k];
+      double ts1, ts2, dij[3];
+      self2(c, d2, ts1, ts2);
+      dd3_apply(r, ts1, ts2, qd, dij);
+      ph[i] += ts1 * (r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2]);
+      ph[j] -= ts1 * (r[0] * di[0] + r[1] * di[1] + r[2] * di[2]);
+      for (int k = 0; k < 3; ++k) {
+        fr[i][k] += dij[k];
+        fr[j][k] -= dij[k];
+      }
+      vc += rc[0] * dij[0] + rc[1] * dij[1] + rc[2] * dij[2];
+      for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dij[row] * r[col];
+    }
+  // Uind and predictor history (pot_ttm.f90:390-392)
+  double ui = 0.0;
+  for (int u = 0; u < ND; ++u)
+    for (int k = 0; k < 3; ++k) ui += mu[T0 + u][k] * Efq[(base + T0 + u) * 3 + k];
+  if (c.guess) {
+    const int slot = t % 4, ndip = N * ND;
+    for (int u = 0; u < ND; ++u)
+      for (int k = 0; k < 3; ++k)
+        hist[(((size_t)b * 4 + slot) * ndip + (size_t)w * ND + u) * 3 + k] = mu[T0 + u][k];
+  }
+  // chain rule through the DMS charges (pot_ttm.f90:556-584)
+  const double *g = grdq + ((size_t)b * N + w) * 27;
+  double dg[3][3];  // [O, H1, H2][xyz]
+  for (int k = 0; k < 3; ++k) {
+    dg[1][k] = g[0 * 9 + 0 * 3 + k] * ph[1] + g[0 * 9 + 1 * 3 + k] * ph[2] + g[0 * 9 + 2 * 3 + k] * ph[3];
+    dg[2][k] = g[1 * 9 + 0 * 3 + k] * ph[1] + g[1 * 9 + 1 * 3 + k] * ph[2] + g[1 * 9 + 2 * 3 + k] * ph[3];
+    dg[0][k] = g[2 * 9 + 0 * 3 + k] * ph[1] + g[2 * 9 + 1 * 3 + k] * ph[2] + g[2 * 9 + 2 * 3 + k] * ph[3];
+  }
+  for (int k = 0; k < 3; ++k) {
+    fr[1][k] += dg[1][k];
+    fr[2][k] += dg[2][k];
+    fr[0][k] += dg[0][k];
+  }
+  {
+    double o1[3] = {A[1].x - A[0].x, A[1].y - A[0].y, A[1].z - A[0].z};
+    double o2[3] = {A[2].x - A[0].x, A[2].y - A[0].y, A[2].z - A[0].z};
+    minimg(c, o1[0], o1[1], o1[2]);
+    minimg(c, o2[0], o2[1], o2[2]);
+    for (int ix = 0; ix < 3; ++ix)
+      for (int iy = 0; iy < 3; ++iy) vt[iy * 3 + ix] += o1[ix] * dg[1][iy] + o2[ix] * dg[2][iy];
+    double c1[3] = {cen[1][0] - cen[0][0], cen[1][1] - cen[0][1], cen[1][2] - cen[0][2]};
+    double c2[3] = {cen[2][0] - cen[0][0], cen[2][1] - cen[0][1], cen[2][2] - cen[0][2]};
+    minimg(c, c1[0], c1[1], c1[2]);
+    minimg(c, c2[0], c2[1], c2[2]);
+    for (int k = 0; k < 3; ++k) vc += c1[k] * dg[1][k] + c2[k] * dg[2][k];
+  }
+  // M-site redistribution (pot_ttm.f90:597-602)
+  const double gM = c.gammaM;
+  double *out = dRR + ((size_t)b * 3 * N + 3 * w) * 3;
+  for (int k = 0; k < 3; ++k) {
+    out[3 + k] = fr[1][k] + 0.5 * gM * fr[3][k];
+    out[6 + k] = fr[2][k] + 0.5 * gM * fr[3][k];
+    out[k] = fr[0][k] + (1.0 - gM) * fr[3][k];
+  }
+  // molecular dipoles (pot_ttm.f90:605-627)
+  {
+    double h1[3] = {A[1].x - A[3].x, A[1].y - A[3].y, A[1].z - A[3].z};
+    double h2[3] = {A[2].x - A[3].x, A[2].y - A[3].y, A[2].z - A[3].z};
+    minimg(c, h1[0], h1[1], h1[2]);
+    minimg(c, h2[0], h2[1], h2[2]);
+    double dm[3], de[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) dm[k] = A[1].q * h1[k] + A[2].q * h2[k];
+    for (int u = 0; u < ND; ++u)
+      for (int k = 0; k < 3; ++k) {
+        dm[k] = mu[T0 + u][k] + dm[k];
+        de[k] = mu[T0 + u][k];
+      }
+    if (dipI)
+      for (int k = 0; k < 3; ++k) dipI[((size_t)b * N + w) * 3 + k] = dm[k];
+    if (dipE)
+      for (int k = 0; k < 3; ++k) dipE[((size_t)b * N + w) * 3 + k] = de[k];
+  }
+  double *tl = tally + ((size_t)b * N + w) * T_COUNT;
+  tl[T_UIND] = ui;
+  tl[T_VIRC] += vc;
+  vadd9(tl + T_VIRT, vt);
+}
+
+void launch_finalize(const SysConst &c, DevBufs &b, int t, double *dRR, double *dipI,
+                     double *dipE, hipStream_t s) {
+  int n = c.P * c.N;
+  if (c.model == 2)
+    hipLaunchKernelGGL(k_finalize<2>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
+                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, t);
+  else
+    hipLaunchKernelGGL(k_finalize<3>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
+                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, t);
+}
+
+void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s) {
+  dim3 grid((c.nk + 255) / 256, c.P);
+  hipLaunchKernelGGL(k_virk, grid, dim3(256), 0, s, c, b.kv, b.red, b.Sq, b.Sm, b.Sv, b.virk);
+}
+
+// Per-bead totals (pot_ttm.f90:552-553, 588-591, 628; force_calc.f90:48).
+__global__ __launch_bounds__(256) void k_reduce_bead(SysConst c, const double *__restrict__ tally,
+                                                    const double *__restrict__ virk, int nkb,
+                                                    const int *__restrict__ iters_in,
+                                                    double *Upot, double *virt, double *virialc,
+                                                    double *terms, int *iters) {
+  __shared__ double red[T_COUNT][256];
+  const int b = blockIdx.x;
+  double v[T_COUNT];
+  for (int i = 0; i < T_COUNT; ++i) v[i] = 0.0;
+  for (int w = threadIdx.x; w < c.N; w += 256)
+    for (int i = 0; i < T_COUNT; ++i) v[i] += tally[((size_t)b * c.N + w) * T_COUNT + i];
+  for (int i = 0; i < T_COUNT; ++i) red[i][threadIdx.x] = v[i];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int i = 0; i < T_COUNT; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double vr[18];
+    for (int i = 0; i < 18; ++i) vr[i] = 0.0;
+    for (int blk = 0; blk < nkb; ++blk)
+      for (int i = 0; i < 18; ++i) vr[i] += virk[((size_t)b * nkb + blk) * 18 + i];
+    double vrec[9];
+    for (int i = 0; i < 9; ++i)
+      vrec[i] = -0.5 * vr[i] * FOURPI / c.vol + -0.5 * vr[9 + i] * FOURPI / c.vol;
+    const double umon = red[T_UMON][0], uvdw = red[T_UVDW][0];
+    const double uelec = 0.5 * red[T_UELEC][0], uind = -0.5 * red[T_UIND][0];
+    double vt[9];
+    for (int i = 0; i < 9; ++i) vt[i] = red[T_VIRT + i][0] + vrec[i];
+    vt[0] -= c.Uvdw_lrc;
+    vt[4] -= c.Uvdw_lrc;
+    vt[8] -= c.Uvdw_lrc;
+    const double vcc = red[T_VIRC][0] + vrec[0] + vrec[4] + vrec[8] - 3 * c.Uvdw_lrc;
+    if (Upot) Upot[b] = umon + uvdw + uelec + uind + c.Uvdw_lrc;
+    if (virt)
+      for (int i = 0; i < 9; ++i) virt[b * 9 + i] = vt[i];
+    if (virialc) virialc[b] = vcc;
+    if (terms) {
+      terms[b * 5 + 0] = umon;
+      terms[b * 5 + 1] = uvdw;
+      terms[b * 5 + 2] = uelec;
+      terms[b * 5 + 3] = uind;
+      terms[b * 5 + 4] = c.Uvdw_lrc;
+    }
+    if (iters) iters[b] = iters_in[b];
+  }
+}
+
+void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
+                        double *virialc, int *iters, hipStream_t s) {
+  const int nkb = (c.nk + 255) / 256;
+  hipLaunchKernelGGL(k_reduce_bead, dim3(c.P), dim3(256), 0, s, c, b.tally, b.virk, nkb,
+                     b.iters, Upot, virt, virialc, b.terms, iters);
+}
+
+}  // namespace pimd

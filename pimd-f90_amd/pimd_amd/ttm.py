@@ -1,0 +1,295 @@
+"""Python host mirror of the reference's force and ring-polymer interface.
+
+Thin ctypes layer over the C ABI in include/pimd_ttm.h (libpimd_ttm.so, built
+in-tree by `make -C pimd-f90_amd`).  Names follow the reference:
+
+  TTMForceField.full_bead_forces   force_calc.f90:26-106 (all P beads at once)
+  TTMForceField.potential          force_calc.f90:112-170 / pot_ttm.f90:3 (one bead)
+  RingPolymer.init_normal_modes    NormalModes.f90:68-218
+  RingPolymer.evolve_ring          NormalModes.f90:34-63, every atom in one call
+  RingPolymer.init_langevin_nm     Langevin.f90:68-98
+  RingPolymer.langevin_nm          Langevin.f90:30-64
+  nose_hoover                      Nose_Hoover.f90:33-112
+  PIMDIntegrator                   MD.f90:35-110 (PIMD_VV), device resident
+
+There is no CPU fallback: a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import TTMParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpimd_ttm.so")
+
+# Every symbol include/pimd_ttm.h declares.
+EXPORTS = (
+    "pimd_ctx_create", "pimd_ctx_destroy", "pimd_last_error", "pimd_ctx_info",
+    "pimd_forces_all_beads", "pimd_forces_all_beads_dev", "pimd_energy_terms",
+    "pimd_nm_init", "pimd_nm_get", "pimd_nm_evolve", "pimd_langevin_init", "pimd_langevin",
+    "pimd_nose_hoover", "pimd_md_init", "pimd_state_set", "pimd_state_get", "pimd_md_run",
+    "pimd_profile_enable", "pimd_profile_count", "pimd_profile_get", "pimd_profile_reset",
+    "pimd_sync",
+)
+
+
+class PimdParams(C.Structure):
+    _fields_ = [("pot_model", C.c_int), ("nwaters", C.c_int), ("nbeads", C.c_int),
+                ("polar_maxiter", C.c_int), ("guess_initdip", C.c_int), ("precision", C.c_int),
+                ("device", C.c_int), ("reserved0", C.c_int), ("box", C.c_double * 3),
+                ("rc", C.c_double), ("rc1", C.c_double), ("eps_ewald", C.c_double),
+                ("polar_sor", C.c_double), ("polar_eps", C.c_double), ("mass_o", C.c_double),
+                ("mass_h", C.c_double)]
+
+
+class MDParams(C.Structure):
+    _fields_ = [("delt_fs", C.c_double), ("temp", C.c_double), ("bead_thermostat", C.c_int),
+                ("centroid_thermostat", C.c_int), ("tau_centroid", C.c_double),
+                ("global_nh", C.c_int), ("reserved0", C.c_int), ("tau", C.c_double),
+                ("setNMfreq", C.c_double)]
+
+
+class PimdError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libpimd_ttm.so and declares the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PimdError(f"{path} is missing: build it with `make -C pimd-f90_amd`")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    sig = {
+        "pimd_ctx_create": [C.POINTER(PimdParams), C.POINTER(C.c_void_p)],
+        "pimd_ctx_destroy": [vp],
+        "pimd_last_error": [C.c_char_p, C.c_size_t],
+        "pimd_ctx_info": [vp, dp],
+        "pimd_forces_all_beads": [vp, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, dp, ip],
+        "pimd_forces_all_beads_dev": [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp],
+        "pimd_energy_terms": [vp, dp],
+        "pimd_nm_init": [vp, C.c_double, C.c_double, C.c_double],
+        "pimd_nm_get": [vp, dp, dp, dp, dp, dp, dp, dp],
+        "pimd_nm_evolve": [vp, dp, dp, dp],
+        "pimd_langevin_init": [vp, C.c_double, C.c_int, C.c_double, C.c_double],
+        "pimd_langevin": [vp, dp, dp, C.c_uint64, C.c_uint64],
+        "pimd_nose_hoover": [dp, dp, dp, C.c_double, C.c_double, C.c_int, C.c_double],
+        "pimd_md_init": [vp, C.POINTER(MDParams)],
+        "pimd_state_set": [vp, dp, dp, dp],
+        "pimd_state_get": [vp, dp, dp, dp, dp],
+        "pimd_md_run": [vp, C.c_int, C.c_int, dp, C.c_uint64, dp, dp, dp, dp],
+        "pimd_profile_enable": [vp, C.c_int],
+        "pimd_profile_count": [vp],
+        "pimd_profile_get": [vp, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_long), dp],
+        "pimd_profile_reset": [vp],
+        "pimd_sync": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        buf = C.create_string_buffer(512)
+        load_library().pimd_last_error(buf, 512)
+        raise PimdError(f"pimd error {rc}: {buf.value.decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _arr(x, shape):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if a.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
+    return a
+
+
+class TTMForceField:
+    """All-beads TTM2.1-F / TTM3-F force evaluation on one GPU."""
+
+    def __init__(self, params: TTMParams, device: int = 0):
+        p = params.resolved()
+        self.params = p
+        L = load_library()
+        cp = PimdParams(p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip),
+                        0, device, 0, (C.c_double * 3)(*p.box), p.rc, p.rc1, p.eps_ewald,
+                        p.polar_sor, p.polar_eps, p.mass_o, p.mass_h)
+        h = C.c_void_p()
+        _check(L.pimd_ctx_create(C.byref(cp), C.byref(h)))
+        self._h = h
+        self._single = None
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().pimd_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+        if self._single is not None:
+            self._single.close()
+            self._single = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        o = np.zeros(12)
+        _check(load_library().pimd_ctx_info(self._h, _ptr(o)))
+        return {"aewald": o[0], "Uvdw_lrc": o[1], "shiftA": o[2], "shiftB": o[3],
+                "polfac": o[4:8].copy(), "nkvec": int(o[8]), "hmax": int(o[9]),
+                "kmax": int(o[10]), "lmax": int(o[11])}
+
+    def full_bead_forces(self, RRt, RRc, t: int = 1) -> dict:
+        """pot_ttm for every bead; returns Upott, dRRt, virt, virialct, dipoles, chg, iters.
+
+        virt is returned as [P][row][col]; the C ABI's column-major layout is transposed
+        here for convenience.  iters < 0 marks beads that reached polar_maxiter.
+        """
+        p = self.params
+        P, na, N = p.nbeads, p.natoms, p.nwaters
+        RRt = _arr(RRt, (P, na, 3))
+        RRc = _arr(RRc, (na, 3))
+        out = {"Upott": np.zeros(P), "dRRt": np.zeros((P, na, 3)), "virt_cm": np.zeros((P, 3, 3)),
+               "virialct": np.zeros(P), "dip_momIt": np.zeros((P, N, 3)),
+               "dip_momEt": np.zeros((P, N, 3)), "chg": np.zeros((P, na)),
+               "iters": np.zeros(P, dtype=np.int32)}
+        _check(load_library().pimd_forces_all_beads(
+            self._h, _ptr(RRt), _ptr(RRc), int(t), _ptr(out["Upott"]), _ptr(out["dRRt"]),
+            _ptr(out["virt_cm"]), _ptr(out["virialct"]), _ptr(out["dip_momIt"]),
+            _ptr(out["dip_momEt"]), _ptr(out["chg"]),
+            out["iters"].ctypes.data_as(C.POINTER(C.c_int))))
+        out["virt"] = np.transpose(out["virt_cm"], (0, 2, 1)).copy()
+        out["virialt"] = np.trace(out["virt"], axis1=1, axis2=2)
+        return out
+
+    def energy_terms(self) -> np.ndarray:
+        """[P][5]: Umon, Uvdw, Uelec, Uind, Uvdw_lrc of the last call."""
+        t = np.zeros((self.params.nbeads, 5))
+        _check(load_library().pimd_energy_terms(self._h, _ptr(t)))
+        return t
+
+    def potential(self, RR, RRc, t: int = 1) -> dict:
+        """One bead (force_calc.f90:112-170): the reference's per-rank call."""
+        if self._single is None:
+            q = TTMParams(**{**self.params.__dict__, "nbeads": 1})
+            self._single = TTMForceField(q)
+        r = self._single.full_bead_forces(np.asarray(RR)[None], RRc, t)
+        return {k: (v[0] if isinstance(v, np.ndarray) and v.ndim > 0 else v) for k, v in r.items()}
+
+    def profile(self, on: bool = True):
+        _check(load_library().pimd_profile_enable(self._h, int(on)))
+
+    def profile_reset(self):
+        _check(load_library().pimd_profile_reset(self._h))
+
+    def profile_report(self) -> dict:
+        L = load_library()
+        n = L.pimd_profile_count(self._h)
+        res = {}
+        for i in range(n):
+            name = C.create_string_buffer(64)
+            cnt, ms = C.c_long(), C.c_double()
+            _check(L.pimd_profile_get(self._h, i, name, 64, C.byref(cnt), C.byref(ms)))
+            res[name.value.decode()] = (cnt.value, ms.value)
+        return res
+
+
+class RingPolymer:
+    """NormalModes / Langevin module equivalents bound to a force-field context."""
+
+    def __init__(self, ff: TTMForceField):
+        self.ff = ff
+        self.P = ff.params.nbeads
+        self.natoms = ff.params.natoms
+
+    def init_normal_modes(self, omegan: float, delta: float, setNMfreq: float = 0.0):
+        _check(load_library().pimd_nm_init(self.ff.handle, omegan, delta, setNMfreq))
+
+    def module_data(self) -> dict:
+        P = self.P
+        C_, a = np.zeros((P, P)), [np.zeros(P) for _ in range(6)]
+        _check(load_library().pimd_nm_get(self.ff.handle, _ptr(C_), *[_ptr(x) for x in a]))
+        return {"C": C_, "A11": a[0], "A12": a[1], "A21": a[2], "A22": a[3], "omegalist": a[4],
+                "MassScaleFactor": a[5]}
+
+    def evolve_ring(self, RRt, PPt, masses=None):
+        RRt = _arr(RRt, (self.P, self.natoms, 3)).copy()
+        PPt = _arr(PPt, (self.P, self.natoms, 3)).copy()
+        m = None if masses is None else _arr(masses, (self.natoms,))
+        _check(load_library().pimd_nm_evolve(self.ff.handle, _ptr(RRt), _ptr(PPt), _ptr(m)))
+        return RRt, PPt
+
+    def init_langevin_nm(self, delt2: float, centroid_thermostat: bool, tau_bead: float,
+                         temp_beads: float):
+        _check(load_library().pimd_langevin_init(self.ff.handle, delt2, int(centroid_thermostat),
+                                                 tau_bead, temp_beads))
+
+    def langevin_nm(self, PPt, noise=None, seed: int = 0, counter: int = 0):
+        PPt = _arr(PPt, (self.P, self.natoms, 3)).copy()
+        nz = None if noise is None else _arr(noise, (self.natoms, self.P, 3))
+        _check(load_library().pimd_langevin(self.ff.handle, _ptr(PPt), _ptr(nz), seed, counter))
+        return PPt
+
+
+def nose_hoover(uk: float, vxi, tau: float, delt2: float, L: int, temp: float) -> dict:
+    s, u = C.c_double(0.0), C.c_double(uk)
+    v = np.array(vxi, dtype=np.float64)
+    _check(load_library().pimd_nose_hoover(C.byref(s), C.byref(u), _ptr(v), tau, delt2, L, temp))
+    return {"s": s.value, "uk": u.value, "vxi": v}
+
+
+class PIMDIntegrator:
+    """Device-resident PIMD velocity Verlet (MD.f90:35-110) over a force field."""
+
+    def __init__(self, ff: TTMForceField, delt_fs=0.25, temp=300.0, bead_thermostat=1,
+                 centroid_thermostat=False, tau_centroid=0.1, global_nh=False, tau=0.1,
+                 setNMfreq=0.0):
+        self.ff = ff
+        md = MDParams(delt_fs, temp, bead_thermostat, int(centroid_thermostat), tau_centroid,
+                      int(global_nh), 0, tau, setNMfreq)
+        _check(load_library().pimd_md_init(ff.handle, C.byref(md)))
+        self.P, self.natoms = ff.params.nbeads, ff.params.natoms
+
+    def set_state(self, RRt, PPt, dRRt=None):
+        shp = (self.P, self.natoms, 3)
+        a, b = _arr(RRt, shp), _arr(PPt, shp)
+        c = None if dRRt is None else _arr(dRRt, shp)
+        _check(load_library().pimd_state_set(self.ff.handle, _ptr(a), _ptr(b), _ptr(c)))
+
+    def get_state(self) -> dict:
+        shp = (self.P, self.natoms, 3)
+        o = {"RRt": np.zeros(shp), "PPt": np.zeros(shp), "dRRt": np.zeros(shp),
+             "RRc": np.zeros((self.natoms, 3))}
+        _check(load_library().pimd_state_get(self.ff.handle, _ptr(o["RRt"]), _ptr(o["PPt"]),
+                                             _ptr(o["dRRt"]), _ptr(o["RRc"])))
+        return o
+
+    def run(self, nsteps: int, t0: int = 1, noise=None, seed: int = 0) -> dict:
+        nz = None
+        if noise is not None:
+            nz = _arr(noise, (nsteps, 2, self.natoms, self.P, 3))
+        o = {k: np.zeros(nsteps) for k in ("Upot", "virial", "virialc", "uk")}
+        _check(load_library().pimd_md_run(self.ff.handle, int(t0), int(nsteps), _ptr(nz), seed,
+                                          _ptr(o["Upot"]), _ptr(o["virial"]), _ptr(o["virialc"]),
+                                          _ptr(o["uk"])))
+        return o

@@ -1,0 +1,175 @@
+"""Generates the golden vectors in tests/golden/*.npz from the reference.
+
+Each fixture holds seeded synthetic inputs and the outputs of the reference's
+own Fortran routines, built from /root/reference by oracle/Makefile into
+oracle/_ref/ref_driver (see oracle/ref_driver.f90).  Run from the repo root:
+
+    make -C oracle ref && python tests/golden/gen_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "pimd-f90_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import refrun  # noqa: E402
+from pimd_amd.params import TTMParams  # noqa: E402
+from pimd_amd.systems import make_system, KB_AMUA2PS2PERK, HBAR  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def params_dict(p):
+    return {f"p_{k}": np.asarray(v) for k, v in p.__dict__.items()}
+
+
+def save(name, **arrs):
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+    print("wrote", name)
+
+
+# Force cases: (name, model, N, P, lattice, box, rc, eps, seed, extra)
+FORCE_CASES = [
+    # 32 H2O, 0.997 g/cm^3, Rc 4.9, eps 1e-8 (the CPU-runnable benchmark system)
+    ("forces_ttm3f_n32_p4_rc49", 3, 32, 4, None, None, 4.9, 1e-8, 11, {}),
+    ("forces_ttm21f_n32_p3_rc49", 2, 32, 3, None, None, 4.9, 1e-8, 12, {}),
+    # default cut-off L/2 and eps 1e-6
+    ("forces_ttm3f_n32_p2_rchalf", 3, 32, 2, None, None, -1.0, 1e-6, 13, {}),
+    ("forces_ttm21f_n32_p2_rchalf", 2, 32, 2, None, None, -1.0, 1e-6, 14, {}),
+    # 64 H2O, single bead, larger box
+    ("forces_ttm3f_n64_p1_rchalf", 3, 64, 1, None, None, -1.0, 1e-6, 15, {}),
+    # non-cubic box (lattice 4x2x2 in a 12x8x8 box): per-axis k ranges and images
+    ("forces_ttm3f_n16_p2_ortho", 3, 16, 2, (4, 2, 2), (12.0, 8.0, 8.0), 3.9, 1e-7, 16, {}),
+    # SOR cap reached: tight tolerance with 4 iterations
+    ("forces_ttm3f_n32_p2_maxiter4", 3, 32, 2, None, None, 4.9, 1e-8, 17,
+     {"polar_maxiter": 4, "polar_eps": 1e-12}),
+    ("forces_ttm21f_n16_p2_eps1e5", 2, 16, 2, (4, 2, 2), None, -1.0, 1e-5, 18,
+     {"polar_maxiter": 40, "polar_eps": 1e-9}),
+]
+
+
+def force_fixtures():
+    for name, model, N, P, lat, box, rc, eps, seed, extra in FORCE_CASES:
+        if box is None:
+            RRt, PPt, RRc, L = make_system(N, P, seed=seed, lattice=lat)
+            bx = (L, L, L)
+        else:
+            RRt, PPt, RRc, _ = make_system(N, P, seed=seed, lattice=lat, box=box[0])
+            # rescale y, z of a cubic construction into the orthorhombic box
+            sc = np.array(box) / box[0]
+            cen = RRc.reshape(N, 3, 3)
+            oc = cen[:, :1, :]
+            RRt = (RRt.reshape(P, N, 3, 3) - oc[None]) + oc[None] * sc
+            RRt = RRt.reshape(P, 3 * N, 3)
+            RRc = RRt.mean(axis=0)
+            bx = box
+        p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=bx, rc=rc, eps_ewald=eps,
+                      **extra).resolved()
+        # put one bead partly outside the box: the centroid wrap leaves beads there
+        RRt = RRt.copy()
+        RRt[0, 0:3] += np.array([bx[0] * 0.5, 0.0, 0.0])
+        RRc = RRt.mean(axis=0)
+        f = refrun.forces(p, RRt, RRc, t=1)
+        save(name, RRt=RRt, RRc=RRc, t=np.asarray(1), En=f["En"], Umon=f["Umon"], Uvdw=f["Uvdw"],
+             Uelec=f["Uelec"], Uind=f["Uind"], dRR=f["dRR"], virt_cm=f["virt"],
+             virialc=f["virialc"], dipI=f["dipI"], dipE=f["dipE"], chg=f["chg"],
+             iters=f["iters"], **params_dict(p))
+
+
+def predictor_fixture():
+    """guess_initdip: per-bead predictor history over 7 MD steps (t > 4 uses it)."""
+    N, P = 16, 2
+    RRt, PPt, RRc, L = make_system(N, P, seed=21, lattice=(4, 2, 2))
+    p = TTMParams(pot_model=3, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0, eps_ewald=1e-6,
+                  guess_initdip=True).resolved()
+    dR0 = np.zeros_like(RRt)
+    r = refrun.trajectory(p, RRt, PPt, dR0, nsteps=7, temp=300.0, delt_fs=0.25,
+                          bead_thermostat=0, per_bead_hist=True)
+    save("traj_ttm3f_predictor_n16_p2", RRt=RRt, PPt=PPt,
+         Upot=np.array([s["Upot"] for s in r["steps"]]), uk=np.array([s["uk"] for s in r["steps"]]),
+         RRt_end=r["RRt"], PPt_end=r["PPt"], dRRt_end=r["dRRt"], iters=r["iters"],
+         **params_dict(p))
+
+
+def traj_fixture():
+    """PIMD_VV with the NM Langevin thermostat; the normals it consumed are recorded."""
+    for name, model, N, P, lat, glob in [("traj_ttm3f_langevin_n8_p4", 3, 8, 4, (2, 2, 2), False),
+                                         ("traj_ttm21f_langevin_nh_n8_p2", 2, 8, 2, (2, 2, 2), True)]:
+        RRt, PPt, RRc, L = make_system(N, P, seed=31 + N + P, lattice=lat)
+        p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
+                      eps_ewald=1e-6, mass_o=15.995, mass_h=1.008).resolved()
+        dR0 = np.zeros_like(RRt)
+        r = refrun.trajectory(p, RRt, PPt, dR0, nsteps=6, temp=300.0, delt_fs=0.25,
+                              bead_thermostat=1, centroid=True, tau_centroid=0.05,
+                              global_nh=glob, tau=0.1, per_bead_hist=True)
+        noise = np.array([[s["noise1"], s["noise2"]] for s in r["steps"]])
+        save(name, RRt=RRt, PPt=PPt, noise=noise,
+             Upot=np.array([s["Upot"] for s in r["steps"]]),
+             uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
+             dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], global_nh=np.asarray(glob),
+             **params_dict(p))
+
+
+def ring_fixtures():
+    for P, setnm in [(8, 0.0), (7, 0.0), (6, 1500.0)]:
+        N = 4
+        RRt, PPt, RRc, L = make_system(N, P, seed=40 + P, lattice=(2, 2, 1), box=7.0)
+        p = TTMParams(pot_model=3, nwaters=N, nbeads=P, box=(L, L, L), mass_o=15.995,
+                      mass_h=1.008).resolved()
+        r = refrun.nm(p, RRt, PPt, 300.0, 0.25, setnm)
+        r["RRt_out"], r["PPt_out"] = r.pop("RRt"), r.pop("PPt")
+        save(f"nm_p{P}_setnm{int(setnm)}", RRt=RRt, PPt=PPt, temp=np.asarray(300.0),
+             delt_fs=np.asarray(0.25), setnm=np.asarray(setnm), **r, **params_dict(p))
+    for P, cen in [(8, True), (5, False)]:
+        N = 4
+        RRt, PPt, RRc, L = make_system(N, P, seed=50 + P, lattice=(2, 2, 1), box=7.0)
+        p = TTMParams(pot_model=3, nwaters=N, nbeads=P, box=(L, L, L), mass_o=15.995,
+                      mass_h=1.008).resolved()
+        r = refrun.langevin(p, PPt, 300.0, 0.25, tau_centroid=0.05, centroid=cen, seed=777 + P)
+        save(f"langevin_p{P}_centroid{int(cen)}", PPt_in=PPt, temp=np.asarray(300.0),
+             delt_fs=np.asarray(0.25), tau_centroid=np.asarray(0.05), centroid=np.asarray(cen),
+             **r, **params_dict(p))
+    nh = []
+    for uk, v, tau, L, T in [(12.5, (0.3, -0.2), 0.1, 300, 3000.0), (800.0, (1.0, 1.0), 0.05, 2880, 9600.0)]:
+        r = refrun.nose_hoover(uk, v, tau, 0.25, L, T)
+        nh.append([uk, v[0], v[1], tau, L, T, r["s"], r["uk"], r["vxi"][0], r["vxi"][1]])
+    save("nose_hoover", table=np.array(nh))
+
+
+def scalar_fixtures():
+    RRt, PPt, RRc, L = make_system(32, 1, seed=60)
+    p = TTMParams(pot_model=3, nwaters=32, nbeads=1, box=(L, L, L), rc=4.9, eps_ewald=1e-8).resolved()
+    rng = np.random.default_rng(61)
+    xs = np.concatenate([np.linspace(0.0, 6.0, 61), rng.uniform(0, 40, 40)])
+    d2 = np.concatenate([rng.uniform(0.6, 24.99, 70), rng.uniform(25.0, 40.0, 31)])
+    ap = rng.uniform(0.15, 0.45, len(xs))
+    f = refrun.funcs(p, xs, d2, ap)
+    save("scalar_funcs", xs=xs, drsq=d2, apol=ap, **f, **params_dict(p))
+    # monomers: bead geometries of a quantum ring polymer (stretched/bent)
+    RRt, PPt, RRc, L = make_system(8, 8, seed=62, lattice=(2, 2, 2), box=6.3)
+    r1s = RRt.reshape(-1, 3, 3)
+    for model in (3, 2):
+        q = TTMParams(pot_model=model, nwaters=8, nbeads=8, box=(L, L, L)).resolved()
+        m = refrun.monomer(q, r1s)
+        save(f"monomer_model{model}", r1s=r1s, e1=m["e1"], dr1=m["dr1"], q3=m["q3"],
+             dq3_f=m["dq3"], **params_dict(q))
+    for name, (N, box, rc, eps) in {"setup_n32_rc49": (32, None, 4.9, 1e-8),
+                                    "setup_n128_rchalf": (128, None, -1.0, 1e-6)}.items():
+        Lb = make_system(N, 1, seed=1)[3] if box is None else box
+        q = TTMParams(pot_model=3, nwaters=N, nbeads=1, box=(Lb, Lb, Lb), rc=rc,
+                      eps_ewald=eps).resolved()
+        s = refrun.setup(q)
+        save(name, **{k: np.asarray(v) for k, v in s.items()}, **params_dict(q))
+
+
+if __name__ == "__main__":
+    if not refrun.available():
+        sys.exit("build the reference first: make -C oracle ref")
+    force_fixtures()
+    predictor_fixture()
+    traj_fixture()
+    ring_fixtures()
+    scalar_fixtures()

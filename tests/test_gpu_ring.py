@@ -1,0 +1,106 @@
+"""GPU parity of the ring-polymer propagation and of whole PIMD_VV steps.
+
+Normal modes, the Langevin thermostat (with the normals the reference
+consumed, recorded by its driver) and short trajectories are compared with the
+reference's outputs held in tests/golden.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_names, load_golden, params_from
+from helpers import max_abs, rel_err
+from pimd_amd.systems import HBAR, KB_AMUA2PS2PERK, atom_masses
+
+pytestmark = pytest.mark.gpu
+
+
+def _ff(p):
+    from pimd_amd.ttm import TTMForceField
+    return TTMForceField(p)
+
+
+@pytest.mark.parametrize("name", golden_names("nm_"))
+def test_normal_modes_vs_reference(name):
+    from pimd_amd.ttm import RingPolymer
+    g = load_golden(name)
+    p = params_from(g)
+    P = p.nbeads
+    print("case", name, flush=True)
+    rp = RingPolymer(_ff(p))
+    rp.init_normal_modes(KB_AMUA2PS2PERK * float(g["temp"]) * P / HBAR,
+                         float(g["delt_fs"]) / 1000.0, float(g["setnm"]))
+    md = rp.module_data()
+    for k in ("C", "A11", "A12", "A21", "A22", "omegalist", "MassScaleFactor"):
+        assert rel_err(md[k], g[k]) < 1e-14, k
+    R, Pm = rp.evolve_ring(g["RRt"], g["PPt"], atom_masses(p.nwaters, p.mass_o, p.mass_h))
+    assert rel_err(R, g["RRt_out"]) < 1e-12 and rel_err(Pm, g["PPt_out"]) < 1e-12
+
+
+@pytest.mark.parametrize("name", golden_names("langevin_"))
+def test_langevin_vs_reference(name):
+    from pimd_amd.ttm import RingPolymer
+    g = load_golden(name)
+    p = params_from(g)
+    P = p.nbeads
+    T, dt = float(g["temp"]), float(g["delt_fs"]) / 1000.0
+    print("case", name, flush=True)
+    rp = RingPolymer(_ff(p))
+    rp.init_normal_modes(KB_AMUA2PS2PERK * T * P / HBAR, dt)
+    rp.init_langevin_nm(dt / 2, bool(g["centroid"]), float(g["tau_centroid"]), P * T)
+    out = rp.langevin_nm(g["PPt_in"], noise=g["noise"])
+    assert rel_err(out, g["PPt"]) < 1e-12
+
+
+def test_langevin_device_noise_statistics():
+    """Device normals (Philox4x32-10 + Box-Muller): from p = 0 a thermostat call
+    leaves p~_j = c2_j sqrt(m) xi in mode space, so the mode variances must be
+    c2_j^2 m and the samples uncorrelated with zero mean."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import RingPolymer
+    N, P, T = 64, 32, 300.0
+    _, _, _, L = make_system(N, 1, seed=1)
+    p = TTMParams(pot_model=3, nwaters=N, nbeads=P, box=(L, L, L)).resolved()
+    rp = RingPolymer(_ff(p))
+    rp.init_normal_modes(KB_AMUA2PS2PERK * T * P / HBAR, 0.25e-3)
+    rp.init_langevin_nm(0.125e-3, True, 0.1, P * T)
+    md = rp.module_data()
+    out = rp.langevin_nm(np.zeros((P, 3 * N, 3)), seed=1234, counter=7)
+    m = atom_masses(N, p.mass_o, p.mass_h)
+    modes = np.einsum("jk,kan->jan", md["C"], out) / np.sqrt(m)[None, :, None]
+    c1 = np.exp(-0.125e-3 * np.where(np.arange(P) == 0, 10.0, 2 * md["omegalist"]))
+    c2 = np.sqrt(KB_AMUA2PS2PERK * P * T * (1 - c1 ** 2))
+    z = modes / c2[:, None, None]
+    n = z.size
+    assert abs(z.mean()) < 5 / np.sqrt(n)
+    assert abs(z.var() - 1.0) < 0.05
+    again = rp.langevin_nm(np.zeros((P, 3 * N, 3)), seed=1234, counter=7)
+    assert np.array_equal(again, out)
+    other = rp.langevin_nm(np.zeros((P, 3 * N, 3)), seed=1234, counter=8)
+    assert not np.allclose(other, out)
+
+
+@pytest.mark.parametrize("name", golden_names("traj_"))
+def test_trajectory_vs_reference(name):
+    from pimd_amd.ttm import PIMDIntegrator
+    g = load_golden(name)
+    p = params_from(g)
+    noise = g.get("noise")
+    glob = bool(g.get("global_nh", False))
+    nsteps = g["Upot"].shape[0]
+    print("case", name, flush=True)
+    ff = _ff(p)
+    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0,
+                           bead_thermostat=1 if noise is not None else 0,
+                           centroid_thermostat=True, tau_centroid=0.05, global_nh=glob, tau=0.1)
+    integ.set_state(g["RRt"], g["PPt"], None)
+    rec = integ.run(nsteps, t0=1, noise=noise)
+    st = integ.get_state()
+    errs = {"Upot": rel_err(rec["Upot"], g["Upot"][:, :, 0].sum(axis=1)),
+            "uk": rel_err(rec["uk"], g["uk"]), "R": rel_err(st["RRt"], g["RRt_end"]),
+            "P": rel_err(st["PPt"], g["PPt_end"]), "dR": max_abs(st["dRRt"], g["dRRt_end"])}
+    print(name, errs, flush=True)
+    assert errs["Upot"] < 1e-10 and errs["uk"] < 1e-10
+    assert errs["R"] < 1e-12 and errs["P"] < 1e-10 and errs["dR"] < 1e-8
+    if "RRc_end" in g:
+        assert max_abs(st["RRc"], g["RRc_end"]) < 1e-12

@@ -1,0 +1,169 @@
+"""The C restatement (oracle) against the reference's own outputs (golden vectors).
+
+The fixtures were produced by the reference Fortran routines built from source
+(tests/golden/gen_golden.py).  The restatement follows the reference's
+statement order, so agreement is expected to rounding (<= 1e-13 relative).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_names, load_golden, params_from
+from helpers import max_abs, ref_iters, rel_err
+from pimd_amd.systems import HBAR, KB_AMUA2PS2PERK, MASSCON, atom_masses
+
+TIGHT = 1e-13
+
+
+@pytest.mark.parametrize("name", golden_names("forces_"))
+def test_forces_vs_reference(oracle_lib, name):
+    g = load_golden(name)
+    p = params_from(g)
+    o = oracle_lib.System(p).forces(g["RRt"], g["RRc"], t=int(g["t"]))
+    assert rel_err(o["En"], g["En"]) < TIGHT
+    for i, k in enumerate(("Umon", "Uvdw", "Uelec", "Uind")):
+        assert rel_err(o["terms"][:, i], g[k]) < TIGHT, k
+    assert max_abs(o["dRR"], g["dRR"]) < 1e-11
+    assert rel_err(o["virt"], g["virt_cm"]) < TIGHT
+    assert rel_err(o["virialc"], g["virialc"]) < TIGHT
+    assert max_abs(o["dipI"], g["dipI"]) < 1e-11
+    assert max_abs(o["dipE"], g["dipE"]) < 1e-11
+    assert max_abs(o["chg"], g["chg"]) < 1e-12
+    assert np.array_equal(ref_iters(o["iters"], p.polar_maxiter), ref_iters(g["iters"], p.polar_maxiter))
+
+
+def test_scalar_funcs_vs_reference(oracle_lib):
+    g = load_golden("scalar_funcs")
+    s = oracle_lib.System(params_from(g))
+    erfc = np.array([oracle_lib.erfc(x) for x in g["xs"]])
+    assert rel_err(erfc, g["erfc"]) < TIGHT
+    gq = np.array([oracle_lib.gammq23(x) for x in g["xs"]])
+    assert rel_err(gq, g["gammq"]) < TIGHT
+    sm = np.array([s.smear(a, b) for a, b in zip(g["drsq"], g["apol"])])
+    names = ["s01_ts0", "s01_ts1", "s2_ts1", "s2_ts2", "s3_ts1", "s3_ts2", "s3_ts3",
+             "self1_ts0", "self1_ts1", "self2_ts1", "self2_ts2"]
+    for i, n in enumerate(names):
+        assert rel_err(sm[:, i], g[n]) < TIGHT, n
+
+
+@pytest.mark.parametrize("model", [3, 2])
+def test_monomer_vs_reference(oracle_lib, model):
+    g = load_golden(f"monomer_model{model}")
+    p = params_from(g)
+    dms = (0.5, 0.9578, 0.012) if model == 3 else (0.0, 0.0, 0.0)
+    res = [oracle_lib.monomer(r, p.box, dms) for r in g["r1s"]]
+    assert rel_err([r[0] for r in res], g["e1"]) < TIGHT
+    assert rel_err([r[1] for r in res], g["dr1"]) < TIGHT
+    assert rel_err([r[2] for r in res], g["q3"]) < TIGHT
+    # reference gradq(i,j,k) was written column-major: fixture [k][j][i]
+    dq = np.transpose(g["dq3_f"], (0, 3, 2, 1))
+    assert rel_err([r[3] for r in res], dq) < TIGHT
+
+
+@pytest.mark.parametrize("name", golden_names("setup_"))
+def test_setup_vs_reference(oracle_lib, name):
+    g = load_golden(name)
+    s = oracle_lib.System(params_from(g))
+    c = s.consts()
+    kv, Ak, hkl = s.kvecs()
+    assert s.nk == int(g["nkvec"])
+    assert np.array_equal(hkl, g["hkl"])
+    assert rel_err(kv, g["kvec"]) < TIGHT and rel_err(Ak, g["Ak"]) < TIGHT
+    for k in ("aewald", "Uvdw_lrc", "shiftA", "shiftB", "const_ts1", "const_ts2", "const_ts3"):
+        assert abs(c[k] - float(g[k])) <= 1e-14 * max(1.0, abs(float(g[k]))), k
+    assert rel_err(c["polfac"], g["polfac"]) < TIGHT
+
+
+@pytest.mark.parametrize("name", golden_names("nm_"))
+def test_normal_modes_vs_reference(oracle_lib, name):
+    g = load_golden(name)
+    p = params_from(g)
+    P = p.nbeads
+    omegan = KB_AMUA2PS2PERK * float(g["temp"]) * P / HBAR
+    nm = oracle_lib.nm_init(P, omegan, float(g["delt_fs"]) / 1000.0, float(g["setnm"]))
+    for k in ("C", "A11", "A12", "A21", "A22", "omegalist", "MassScaleFactor"):
+        assert rel_err(nm[k], g[k]) < TIGHT, k
+    R, Pm = oracle_lib.nm_evolve(nm, atom_masses(p.nwaters, p.mass_o, p.mass_h), g["RRt"], g["PPt"])
+    assert rel_err(R, g["RRt_out"]) < TIGHT and rel_err(Pm, g["PPt_out"]) < TIGHT
+
+
+@pytest.mark.parametrize("name", golden_names("langevin_"))
+def test_langevin_vs_reference(oracle_lib, name):
+    g = load_golden(name)
+    p = params_from(g)
+    P = p.nbeads
+    T, dt = float(g["temp"]), float(g["delt_fs"]) / 1000.0
+    nm = oracle_lib.nm_init(P, KB_AMUA2PS2PERK * T * P / HBAR, dt)
+    c1, c2 = oracle_lib.langevin_init(P, dt / 2, bool(g["centroid"]), float(g["tau_centroid"]),
+                                      P * T, nm["omegalist"])
+    assert rel_err(c1, g["c1"]) < TIGHT and rel_err(c2, g["c2"]) < TIGHT
+    out = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, g["noise"], g["PPt_in"])
+    assert rel_err(out, g["PPt"]) < 1e-12
+
+
+def test_nose_hoover_vs_reference(oracle_lib):
+    for row in load_golden("nose_hoover")["table"]:
+        uk, v0, v1, tau, L, T, s, uk1, w0, w1 = row
+        r = oracle_lib.nose_hoover(uk, (v0, v1), tau, 0.25 / 2000, int(L), T)
+        assert abs(r["s"] - s) < 1e-15 and abs(r["uk"] - uk1) < 1e-12 * abs(uk1)
+        assert max_abs(r["vxi"], [w0, w1]) < 1e-14
+
+
+def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=None,
+                   centroid=True, tau_centroid=0.05, global_nh=False, tau=0.1):
+    """PIMD_VV (MD.f90:35-110) composed from the oracle's pieces."""
+    P, na = p.nbeads, p.natoms
+    dt = delt_fs / 1000.0
+    dt2 = dt / 2
+    nm = oracle_lib.nm_init(P, KB_AMUA2PS2PERK * temp * P / HBAR, dt)
+    c1, c2 = oracle_lib.langevin_init(P, dt2, centroid, tau_centroid, P * temp, nm["omegalist"])
+    masses = atom_masses(p.nwaters, p.mass_o, p.mass_h)
+    sys_ = oracle_lib.System(p)
+    R, Pm, dR = RRt.copy(), PPt.copy(), dRRt.copy()
+    vxi = np.array([1.0, 1.0])
+    L = 3 * na * P
+
+    def uk_of(Pm):
+        return 0.5 * np.sum((1.0 / masses)[None, :, None] * Pm ** 2)
+
+    uk = uk_of(Pm)
+    out = []
+    for st in range(nsteps):
+        if noise is not None:
+            Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 0], Pm)
+        if global_nh:
+            r = oracle_lib.nose_hoover(uk, vxi, tau, dt2, L, temp * P)
+            vxi, uk = r["vxi"], r["uk"]
+            Pm = Pm * r["s"]
+        Pm = Pm - MASSCON * dR * dt2
+        R, Pm = oracle_lib.nm_evolve(nm, masses, R, Pm)
+        Rc = R.sum(axis=0) / P
+        sh = p.box[0] * np.round(Rc / p.box[0])
+        Rc = Rc - sh
+        R = R - sh[None]
+        f = sys_.forces(R, Rc, t=st + 1)
+        dR = f["dRR"]
+        Pm = Pm - MASSCON * dR * dt2
+        if noise is not None:
+            Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 1], Pm)
+        uk = uk_of(Pm)
+        if global_nh:
+            r = oracle_lib.nose_hoover(uk, vxi, tau, dt2, L, temp * P)
+            vxi, uk = r["vxi"], r["uk"]
+            Pm = Pm * r["s"]
+        out.append((f["En"].sum(), uk))
+    return R, Pm, dR, np.array(out)
+
+
+@pytest.mark.parametrize("name", golden_names("traj_ttm") )
+def test_trajectory_vs_reference(oracle_lib, name):
+    g = load_golden(name)
+    p = params_from(g)
+    noise = g.get("noise")
+    glob = bool(g.get("global_nh", False))
+    nsteps = g["Upot"].shape[0]
+    R, Pm, dR, rec = oracle_pimd_vv(oracle_lib, p, g["RRt"], g["PPt"], np.zeros_like(g["RRt"]),
+                                    nsteps, 300.0, 0.25, noise=noise, global_nh=glob)
+    assert rel_err(rec[:, 0], g["Upot"][:, :, 0].sum(axis=1)) < 1e-12
+    assert rel_err(R, g["RRt_end"]) < 1e-12
+    assert rel_err(Pm, g["PPt_end"]) < 1e-11
+    assert max_abs(dR, g["dRRt_end"]) < 1e-10

@@ -1,0 +1,207 @@
+"""Pins of the oracle that do not depend on the reference: textbook special
+functions, finite differences, invariances and closed forms.  Each is chosen
+so that a dropped term, a wrong sign or a transposed index fails it."""
+import numpy as np
+import pytest
+from scipy import special
+
+from conftest import load_golden, params_from
+from helpers import rel_err
+from pimd_amd.params import TTMParams
+from pimd_amd.systems import HBAR, KB_AMUA2PS2PERK, atom_masses, make_system, nm_matrix
+
+
+def test_erfc_is_abramowitz_stegun(oracle_lib):
+    x = np.linspace(0.0, 6.0, 301)
+    ours = np.array([oracle_lib.erfc(v) for v in x])
+    # A&S 7.1.26 has |error| <= 1.5e-7; it is not the exact function
+    assert np.max(np.abs(ours - special.erfc(x))) < 1.5e-7
+    assert np.max(np.abs(ours - special.erfc(x))) > 1e-9
+
+
+def test_gammq_two_thirds(oracle_lib):
+    x = np.concatenate([np.linspace(1e-3, 1.6, 50), np.linspace(1.7, 60.0, 80)])
+    ours = np.array([oracle_lib.gammq23(v) for v in x])
+    exact = special.gammaincc(2.0 / 3.0, x)
+    # truncation at EPS = 3e-7 of the series / continued fraction
+    assert np.max(np.abs(ours - exact) / np.maximum(exact, 1e-300)) < 2e-6
+
+
+def smear_sys(oracle_lib, eps=1e-8):
+    p = TTMParams(pot_model=3, nwaters=32, nbeads=1, box=(12.0, 12.0, 12.0), rc=6.0,
+                  eps_ewald=eps).resolved()
+    s = oracle_lib.System(p)
+    return s, s.consts()["aewald"]
+
+
+def test_smear_screened_coulomb(oracle_lib):
+    """Beyond the Thole radius ts0 = erfc(ar)/r, ts1 = -(1/r) d ts0/dr,
+    3 ts2 = -(1/r) d ts1/dr and 5 ts3 = -(1/r) d ts2/dr (the dipole tensors are
+    3 ts2 r r - ts1 and 15 ts3 r r r - 3 ts2 (r d + ...)).  A soft splitting
+    (alpha = 0.14/A) keeps erfc large against the A&S error (<= 1.5e-7)."""
+    s, a = smear_sys(oracle_lib, eps=0.5)
+    h = 1e-4
+    for r in (5.2, 6.0, 7.5):
+        o = s.smear(r * r, 0.3)
+        assert abs(o[0] - oracle_lib.erfc(a * r) / r) < 1e-15
+        exact = lambda rr: special.erfc(a * rr) / rr  # noqa: E731
+        d1 = -(exact(r + h) - exact(r - h)) / (2 * h) / r
+        assert abs(o[1] - d1) < 1e-5 * abs(d1)
+        t1 = lambda rr: s.smear(rr * rr, 0.3)[2]  # noqa: E731
+        d2 = -(t1(r + h) - t1(r - h)) / (2 * h) / r
+        assert abs(3 * o[3] - d2) < 1e-5 * abs(d2)
+        t2 = lambda rr: s.smear(rr * rr, 0.3)[5]  # noqa: E731
+        d3 = -(t2(r + h) - t2(r - h)) / (2 * h) / r
+        assert abs(5 * o[6] - d3) < 1e-5 * abs(d3)
+        # PBCsmear2 and PBCsmear3 agree on ts1, ts2; self1/self2 subtract 1/r^n
+        assert o[2] == o[4] and o[3] == o[5]
+        assert abs(o[7] - (o[0] - 1 / r)) < 1e-15 and abs(o[8] - (o[1] - 1 / r ** 3)) < 1e-15
+
+
+def test_smear_thole_closed_form(oracle_lib):
+    """Inside 5 A the Thole correction to the potential is
+    -exp(-a r^3)/r + a^(1/3) Gamma(2/3) Q(2/3, a r^3), whose -(1/r) d/dr is
+    -exp(-a r^3)/r^3 (added to ts1)."""
+    s, a = smear_sys(oracle_lib)
+    for r, ap in ((1.2, 0.21), (2.5, 0.3), (4.0, 0.18)):
+        o = s.smear(r * r, ap)
+        screened0 = oracle_lib.erfc(a * r) / r
+        ex = -np.exp(-ap * r ** 3) / r
+        gm = ap ** (1 / 3) * special.gamma(2 / 3) * special.gammaincc(2 / 3, ap * r ** 3)
+        # the incomplete gamma is truncated at EPS = 3e-7 relative
+        assert abs(o[0] - screened0 - (ex + gm)) < 2e-6 * (abs(ex) + abs(gm))
+        screened1 = (screened0 + 2 * a / np.sqrt(np.pi) * np.exp(-(a * r) ** 2)) / r ** 2
+        assert abs(o[1] - screened1 + np.exp(-ap * r ** 3) / r ** 3 - 0) < 1e-12 * abs(o[1]) + 1e-14
+
+
+def test_monomer_gradient_and_charges(oracle_lib):
+    g = load_golden("monomer_model3")
+    p = params_from(g)
+    rng = np.random.default_rng(3)
+    for r1 in g["r1s"][:6]:
+        e, dr, q, dq = oracle_lib.monomer(r1, p.box)
+        assert abs(q.sum()) < 1e-13  # neutral monomer
+        h = 1e-6
+        for a in range(3):
+            for c in range(3):
+                rp, rm = r1.copy(), r1.copy()
+                rp[a, c] += h
+                rm[a, c] -= h
+                ep, _, qp, _ = oracle_lib.monomer(rp, p.box)
+                em, _, qm, _ = oracle_lib.monomer(rm, p.box)
+                assert abs((ep - em) / (2 * h) - dr[a, c]) < 1e-6 * max(1.0, np.abs(dr).max())
+                # dq[disp][charge] with disp, charge ordered (H1, H2, O)
+                disp = {0: 2, 1: 0, 2: 1}[a]
+                fd = (qp - qm) / (2 * h)  # charges ordered (O, H1, H2)
+                assert abs(fd[1] - dq[disp, 0, c]) < 1e-6
+                assert abs(fd[2] - dq[disp, 1, c]) < 1e-6
+                assert abs(fd[0] - dq[disp, 2, c]) < 1e-6
+        # rigid translation and H1<->H2 exchange
+        shift = rng.normal(size=3)
+        e2, _, q2, _ = oracle_lib.monomer(r1 + shift, p.box)
+        assert abs(e2 - e) < 1e-11 and np.max(np.abs(q2 - q)) < 1e-12
+        e3, _, q3, _ = oracle_lib.monomer(r1[[0, 2, 1]], p.box)
+        assert abs(e3 - e) < 1e-11 and abs(q3[1] - q[2]) < 1e-12 and abs(q3[2] - q[1]) < 1e-12
+
+
+def _vdw_ab_energy(R, L, rc):
+    """sum over O-O pairs inside rc of A/R^12 + B/R^10 with the TTM2.1-F A, B."""
+    A, B = -1.329565985e+6, 3.632560798e+5
+    O = R[0::3]
+    e = 0.0
+    for i in range(len(O)):
+        d = O[i] - O[i + 1:]
+        d -= L * np.round(d / L)
+        r2 = np.sum(d * d, axis=1)
+        r2 = r2[r2 < rc * rc]
+        e += np.sum(A / r2 ** 6 + B / r2 ** 5)
+    return e
+
+
+@pytest.mark.parametrize("model", [3, 2])
+def test_pot_ttm_gradient_is_minus_force(oracle_lib, model):
+    """With rc1 = Rc (no vdW shift) and a converged dipole solve, dRR = dU/dr up
+    to the A&S erfc mismatch (~1e-7 relative).  TTM2.1-F's vdW gradient omits
+    the A/R^12 + B/R^10 terms of its energy (pot_ttm.f90:171-176), so for it the
+    finite difference is taken of En minus that part."""
+    N, P = 8, 1
+    RRt, _, RRc, L = make_system(N, P, seed=5, lattice=(2, 2, 2), box=6.4)
+    p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=3.2, rc1=3.2,
+                  eps_ewald=1e-8, polar_maxiter=200, polar_eps=1e-13).resolved()
+    s = oracle_lib.System(p)
+    f = s.forces(RRt, RRc)
+
+    def energy(R):
+        e = s.forces(R, RRc)["En"][0]
+        return e - _vdw_ab_energy(R[0], L, 3.2) if model == 2 else e
+    rng = np.random.default_rng(9)
+    for _ in range(4):
+        d = rng.normal(size=RRt.shape)
+        d /= np.linalg.norm(d)
+        h = 1e-5
+        fd = (energy(RRt + h * d) - energy(RRt - h * d)) / (2 * h)
+        an = float(np.sum(f["dRR"] * d))
+        assert abs(fd - an) < 2e-5 * np.abs(f["dRR"]).max(), (fd, an)
+    if model == 2:  # and without the correction the two disagree
+        e_full = lambda R: s.forces(R, RRc)["En"][0]  # noqa: E731
+        fd = (e_full(RRt + h * d) - e_full(RRt - h * d)) / (2 * h)
+        assert abs(fd - an) > 1e-3 * np.abs(f["dRR"]).max()
+
+
+def test_ewald_split_independence(oracle_lib):
+    """Changing the Ewald splitting (alpha via eps) moves the electrostatic
+    energy only by the truncation error of both sums."""
+    N, P = 32, 1
+    RRt, _, RRc, L = make_system(N, P, seed=7)
+    E = []
+    for eps in (1e-8, 1e-10):
+        p = TTMParams(pot_model=3, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0, eps_ewald=eps,
+                      polar_maxiter=200, polar_eps=1e-12).resolved()
+        f = oracle_lib.System(p).forces(RRt, RRc)
+        E.append(f["terms"][0, 2] + f["terms"][0, 3])
+    assert abs(E[0] - E[1]) < 2e-4 * abs(E[1])
+
+
+def test_normal_mode_matrix_orthogonal_and_free_ring_conserved(oracle_lib):
+    for P in (2, 5, 8, 32):
+        C = nm_matrix(P)
+        assert np.max(np.abs(C @ C.T - np.eye(P))) < 1e-13
+        nm = oracle_lib.nm_init(P, KB_AMUA2PS2PERK * 300 * P / HBAR, 0.25e-3)
+        assert np.max(np.abs(nm["C"] - C)) < 1e-15
+        # frequencies 2 w_n sin(k pi / P) (unordered)
+        wn = KB_AMUA2PS2PERK * 300 * P / HBAR
+        want = np.sort(2 * wn * np.sin(np.pi * np.arange(P) / P))
+        err = np.max(np.abs(np.sort(nm["omegalist"]) - want))
+        if P & (P - 1) == 0:
+            assert err < 1e-12 * wn
+        else:  # k/P is formed in single precision (NormalModes.f90:126)
+            assert 1e-12 * wn < err < 1e-7 * wn
+    # free ring polymer energy is conserved exactly by the propagator
+    P, N = 8, 2
+    RRt, PPt, RRc, L = make_system(N, P, seed=2, lattice=(2, 1, 1), box=6.0)
+    m = atom_masses(N)
+    wn = KB_AMUA2PS2PERK * 300 * P / HBAR
+    nm = oracle_lib.nm_init(P, wn, 1e-3)
+
+    def ring_energy(R, Pm):
+        kin = 0.5 * np.sum(Pm ** 2 / m[None, :, None])
+        spring = 0.5 * np.sum(m[None, :, None] * wn ** 2 * (R - np.roll(R, -1, axis=0)) ** 2)
+        return kin + spring
+    e0 = ring_energy(RRt, PPt)
+    R, Pm = RRt, PPt
+    for _ in range(50):
+        R, Pm = oracle_lib.nm_evolve(nm, m, R, Pm)
+    assert abs(ring_energy(R, Pm) - e0) < 1e-10 * e0
+    # the centroid moves freely
+    cen0 = RRt.mean(axis=0) + 50e-3 * PPt.sum(axis=0) / P / m[:, None] * 1.0
+    assert np.max(np.abs(R.mean(axis=0) - cen0)) < 1e-10 * np.abs(cen0).max()
+
+
+def test_langevin_coefficients_closed_form(oracle_lib):
+    P, T, dt = 8, 300.0, 0.25e-3
+    wn = KB_AMUA2PS2PERK * T * P / HBAR
+    nm = oracle_lib.nm_init(P, wn, dt)
+    c1, c2 = oracle_lib.langevin_init(P, dt / 2, True, 0.1, P * T, nm["omegalist"])
+    gam = np.where(np.arange(P) == 0, 1 / 0.1, 2 * nm["omegalist"])
+    assert np.max(np.abs(c1 - np.exp(-gam * dt / 2))) < 1e-15
+    assert np.max(np.abs(c2 ** 2 + c1 ** 2 * KB_AMUA2PS2PERK * P * T - KB_AMUA2PS2PERK * P * T)) < 1e-9
