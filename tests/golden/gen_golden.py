@@ -46,7 +46,7 @@ FORCE_CASES = [
     # SOR cap reached: tight tolerance with 4 iterations
     ("forces_ttm3f_n32_p2_maxiter4", 3, 32, 2, None, None, 4.9, 1e-8, 17,
      {"polar_maxiter": 4, "polar_eps": 1e-12}),
-    ("forces_ttm21f_n16_p2_eps1e5", 2, 16, 2, (4, 2, 2), None, -1.0, 1e-5, 18,
+    ("forces_ttm21f_n27_p2_eps1e5", 2, 27, 2, (3, 3, 3), None, -1.0, 1e-5, 18,
      {"polar_maxiter": 40, "polar_eps": 1e-9}),
 ]
 
@@ -68,9 +68,10 @@ def force_fixtures():
             bx = box
         p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=bx, rc=rc, eps_ewald=eps,
                       **extra).resolved()
-        # put one bead partly outside the box: the centroid wrap leaves beads there
+        # move one molecule of bead 0 to a periodic image outside the box: the
+        # centroid wrap leaves beads there (main_stuff.f90:100-106)
         RRt = RRt.copy()
-        RRt[0, 0:3] += np.array([bx[0] * 0.5, 0.0, 0.0])
+        RRt[0, 0:3] += np.array([bx[0], 0.0, -bx[2]])
         RRc = RRt.mean(axis=0)
         f = refrun.forces(p, RRt, RRc, t=1)
         save(name, RRt=RRt, RRc=RRc, t=np.asarray(1), En=f["En"], Umon=f["Umon"], Uvdw=f["Uvdw"],
