@@ -131,11 +131,11 @@ int dalloc(T **p, size_t n) {
 void free_all(pimd_ctx *x) {
   DevBufs &d = x->d;
   void *ptrs[] = {d.RR,   d.RRc,   d.site,   d.Rcs,    d.grdq,   d.phi,   d.Efq,   d.dR,
-                  d.dip,  d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.tab,
-                  d.Sq,   d.Sm,    d.Sv,     d.virk,   d.active, d.iters, d.nactive, d.kv,
+                  d.dip,  d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.nbrn,  d.tab,
+                  d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,     d.red,   x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red,   x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -279,6 +279,7 @@ int alloc_system(pimd_ctx *x) {
   ALLOC(d.hist, P * 4 * N * nd * 3);
   ALLOC(d.nbr, P * N * (size_t)c.nbr_cap);
   ALLOC(d.nbrc, P * N);
+  ALLOC(d.nbrn, P * N);
   ALLOC(d.tab, P * (size_t)c.T * S);
   ALLOC(d.Sq, P * nk * 2);
   ALLOC(d.Sm, P * nk * 2);
@@ -308,6 +309,8 @@ int alloc_system(pimd_ctx *x) {
   ALLOC(d.PP, P * na * 3);
   ALLOC(d.noise, 2 * na * P * 3);
   ALLOC(d.uk, 1);
+  ALLOC(d.ukpart, 256);
+  ALLOC(d.radius, P * N);
   ALLOC(x->d_vxi, 2);
   ALLOC(x->d_s, 1);
   ALLOC(x->d_step, 4);
@@ -331,8 +334,28 @@ int alloc_system(pimd_ctx *x) {
       colk.push_back((int)k);
     }
   }
+  // longest columns first: lanes and column groups that run together get
+  // similar trip counts
+  {
+    std::vector<int> order(cols.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b2) { return cols[a].w > cols[b2].w; });
+    std::vector<int4> cs(cols.size());
+    std::vector<int> ck(colk.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+      cs[i] = cols[order[i]];
+      ck[i] = colk[order[i]];
+    }
+    cols.swap(cs);
+    colk.swap(ck);
+  }
   c.ncol = (int)cols.size();
+  c.kzmax = 0;
+  for (auto &cl : cols) c.kzmax = std::max(c.kzmax, cl.w);
+  c.sf_chunks_max = 32;
   int rc2;
+  if ((rc2 = dalloc(&d.part, P * (size_t)c.sf_chunks_max * nk * 2)) != PIMD_OK) return rc2;
   if ((rc2 = dalloc(&d.cols, cols.size())) != PIMD_OK) return rc2;
   if ((rc2 = dalloc(&d.colk, colk.size())) != PIMD_OK) return rc2;
   HIPCHK(hipMemcpy(d.kv, kv.data(), sizeof(double4) * nk, hipMemcpyHostToDevice));
@@ -764,7 +787,7 @@ extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise,
                       x->p.mass_h, s);
     });
   };
-  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, s); });
+  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
   for (int st = 0; st < nsteps; ++st) {
     thermostat(st, 0);
     if (md.global_nh) {
@@ -783,10 +806,10 @@ extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise,
       (void)hipFree(d_rec);
       return rc;
     }
-    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, s); });
+    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
     x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
     thermostat(st, 1);
-    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, s); });
+    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
     if (md.global_nh) {
       x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
       x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });

@@ -36,6 +36,8 @@ struct SysConst {
   int hmax, kmax, lmax;
   int maxiter, guess;
   int nbr_cap;
+  int kzmax;               // longest kz-column
+  int sf_chunks_max;       // site chunks of the column structure-factor kernel
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
@@ -69,10 +71,12 @@ struct DevBufs {
   double *tally = nullptr;                      // [P][N][T_COUNT]
   double *hist = nullptr;                       // [P][4][N*nd][3]
   int *nbr = nullptr, *nbrc = nullptr;          // [P][N][cap], [P][N]
+  int *nbrn = nullptr;                          // [P][N] near partners (listed first)
   double2 *tab = nullptr;                       // [P][T][4N]
   double *Sq = nullptr, *Sm = nullptr;          // [P][nk][2]
   double *Sv = nullptr;                         // [P][nk][6]
   double *virk = nullptr;                       // [P][nkb][9]
+  double *part = nullptr;                       // [P][sf_chunks_max][nk][2]
   int *active = nullptr, *iters = nullptr;      // [P]
   int *nactive = nullptr;                       // [1]
   double4 *kv = nullptr;                        // [nk] kx, ky, kz, Ak
@@ -89,6 +93,8 @@ struct DevBufs {
   double *masses = nullptr;                     // [Natoms]
   double *PP = nullptr, *noise = nullptr;       // [P][Natoms][3], [2][Natoms][P][3]
   double *uk = nullptr;                         // [1] kinetic energy
+  double *ukpart = nullptr;                     // [256] kinetic partial sums
+  double *radius = nullptr;                     // [P][N] site radius about O
   double *red = nullptr;                        // scratch for reductions
 };
 
@@ -124,7 +130,7 @@ void launch_scale(int n, double *PP, const double *sc, hipStream_t s);
 void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,
                          double *RRc, hipStream_t s);
 void launch_kinetic(int P, int natoms, const double *masses, const double *PP, double *uk,
-                    hipStream_t s);
+                    double *part, hipStream_t s);
 void launch_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
                         double temp, hipStream_t s);
 void launch_free_flight(int n, double *RR, const double *PP, const double *masses,

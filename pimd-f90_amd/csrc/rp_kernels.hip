@@ -5,61 +5,71 @@
 // global Nose-Hoover chain (Nose_Hoover.f90:33-112).
 //
 // Layout: RRt/PPt [P][Natoms][3]; C [mode][bead]; A [4][P] = A11, A12, A21, A22.
-// One thread per (atom, component) holds the atom's P beads; C lives in LDS.
+// The mode transforms work on LDS slabs of P beads x CB (atom, component)
+// columns with C resident in LDS.
+#include <algorithm>
+
 #include "pimd_internal.h"
 
 namespace pimd {
 
+// Columns (atom, component) per block so that one P x CB slab is 8 KB (the
+// four slabs and C of P = 64 then fill 64 KB of LDS).
+__host__ __device__ constexpr int nm_cols(int P) { return 1024 / P; }
+
 // real2NM / NM2real (NormalModes.f90:308-341) around the exact harmonic
-// propagator of each free-ring mode.
+// propagator of each free-ring mode, for a slab of CB (atom, component)
+// columns: X (P x CB) is staged in LDS, Y = C X, the 2x2 map is applied per
+// mode, and X = C^T Y is written back.  Sums run over beads in the
+// reference's order.
 __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const double *__restrict__ Cg,
                                                   const double *__restrict__ A,
                                                   const double *__restrict__ msf,
                                                   const double *__restrict__ masses, double *RR,
                                                   double *PP) {
-  extern __shared__ double Cs[];  // [P][P]
-  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
-  __syncthreads();
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= natoms * 3) return;
-  const int a = id / 3;
+  extern __shared__ double lds[];
+  const int CB = nm_cols(P);
+  double *Cs = lds, *xr = Cs + P * P, *xp = xr + P * CB, *yr = xp + P * CB, *yp = yr + P * CB;
+  const int ncols = natoms * 3, col0 = blockIdx.x * CB;
   const size_t stride = (size_t)natoms * 3;
-  const double m = masses[a];
-  double r[64], p[64];  // bead coordinates, then mode coordinates (P <= 64)
-  for (int k = 0; k < P; ++k) {
-    r[k] = RR[k * stride + id];
-    p[k] = PP[k * stride + id];
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int k = e / CB, cc = e % CB, col = col0 + cc;
+    xr[e] = col < ncols ? RR[k * stride + col] : 0.0;
+    xp[e] = col < ncols ? PP[k * stride + col] : 0.0;
   }
-  double rt[64], pt[64];
-  for (int j = 0; j < P; ++j) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int j = e / CB, cc = e % CB, col = col0 + cc;
     double vr = 0.0, vp = 0.0;
     for (int k = 0; k < P; ++k) {
-      vr = vr + Cs[j * P + k] * r[k];
-      vp = vp + Cs[j * P + k] * p[k];
+      vr = vr + Cs[j * P + k] * xr[k * CB + cc];
+      vp = vp + Cs[j * P + k] * xp[k * CB + cc];
     }
-    rt[j] = vr;
-    pt[j] = vp;
+    const double m = col < ncols ? masses[col / 3] : 1.0;
+    const double t2 = A[j] * vr + A[P + j] * vp / (m * msf[j]);
+    yp[e] = A[2 * P + j] * m * msf[j] * vr + A[3 * P + j] * vp;
+    yr[e] = t2;
   }
-  for (int j = 0; j < P; ++j) {
-    const double t2 = A[j] * rt[j] + A[P + j] * pt[j] / (m * msf[j]);
-    pt[j] = A[2 * P + j] * m * msf[j] * rt[j] + A[3 * P + j] * pt[j];
-    rt[j] = t2;
-  }
-  for (int j = 0; j < P; ++j) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int j = e / CB, cc = e % CB, col = col0 + cc;
+    if (col >= ncols) continue;
     double vr = 0.0, vp = 0.0;
     for (int k = 0; k < P; ++k) {
-      vr = vr + Cs[k * P + j] * rt[k];
-      vp = vp + Cs[k * P + j] * pt[k];
+      vr = vr + Cs[k * P + j] * yr[k * CB + cc];
+      vp = vp + Cs[k * P + j] * yp[k * CB + cc];
     }
-    RR[j * stride + id] = vr;
-    PP[j * stride + id] = vp;
+    RR[j * stride + col] = vr;
+    PP[j * stride + col] = vp;
   }
 }
 
 void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s) {
-  const int n = natoms * 3;
-  hipLaunchKernelGGL(k_nm_evolve, dim3((n + 255) / 256), dim3(256), sizeof(double) * P * P, s, P,
-                     natoms, b.C, b.A, b.msf, b.masses, RR, PP);
+  const int CB = nm_cols(P), n = natoms * 3;
+  const size_t lds = sizeof(double) * ((size_t)P * P + 4 * (size_t)P * CB);
+  hipLaunchKernelGGL(k_nm_evolve, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
+                     b.A, b.msf, b.masses, RR, PP);
 }
 
 // Philox4x32-10 counter-based generator (J. K. Salmon et al., "Parallel random
@@ -105,8 +115,9 @@ __device__ __forceinline__ void normal2(uint64_t seed, uint64_t counter, uint64_
   z1 = rr * sn;
 }
 
-// Langevin_NM: p~_j <- c1_j p~_j + c2_j sqrt(m s_j) xi (Langevin.f90:41-60).
-// noise [Natoms][P][3] (the reference's consumption order) or generated.
+// Langevin_NM: p~_j <- c1_j p~_j + c2_j sqrt(m s_j) xi (Langevin.f90:41-60),
+// on the same LDS slabs as k_nm_evolve.  noise [Natoms][P][3] (the
+// reference's consumption order) or drawn from (seed, counter).
 __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const double *__restrict__ Cg,
                                                  const double *__restrict__ msf,
                                                  const double *__restrict__ c1,
@@ -114,47 +125,55 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
                                                  double massH, double *PP,
                                                  const double *__restrict__ noise, uint64_t seed,
                                                  uint64_t counter) {
-  extern __shared__ double Cs[];
-  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
-  __syncthreads();
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= natoms * 3) return;
-  const int a = id / 3, comp = id % 3;
+  extern __shared__ double lds[];
+  const int CB = nm_cols(P);
+  double *Cs = lds, *xp = Cs + P * P, *yp = xp + P * CB;
+  const int ncols = natoms * 3, col0 = blockIdx.x * CB;
   const size_t stride = (size_t)natoms * 3;
-  const double m = (a % 3 == 0) ? massO : massH;
-  double p[64], pt[64];
-  for (int k = 0; k < P; ++k) p[k] = PP[k * stride + id];
-  for (int j = 0; j < P; ++j) {
-    double v = 0.0;
-    for (int k = 0; k < P; ++k) v = v + Cs[j * P + k] * p[k];
-    pt[j] = v;
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int k = e / CB, cc = e % CB, col = col0 + cc;
+    xp[e] = col < ncols ? PP[k * stride + col] : 0.0;
   }
-  for (int j = 0; j < P; ++j) {
-    double xi;
-    const uint64_t n = ((uint64_t)a * P + j) * 3 + comp;
-    if (noise) {
-      xi = noise[n];
-    } else {
-      double z0, z1;
-      normal2(seed, counter, n >> 1, z0, z1);
-      xi = (n & 1) ? z1 : z0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int j = e / CB, cc = e % CB, col = col0 + cc;
+    double v = 0.0;
+    for (int k = 0; k < P; ++k) v = v + Cs[j * P + k] * xp[k * CB + cc];
+    if (col < ncols) {
+      const int a = col / 3, comp = col % 3;
+      const double m = (a % 3 == 0) ? massO : massH;
+      const uint64_t n = ((uint64_t)a * P + j) * 3 + comp;
+      double xi;
+      if (noise) {
+        xi = noise[n];
+      } else {
+        double z0, z1;
+        normal2(seed, counter, n >> 1, z0, z1);
+        xi = (n & 1) ? z1 : z0;
+      }
+      const double sm = sqrt(m * msf[j]);
+      v = c1[j] * v + c2[j] * (sm * xi);
     }
-    const double sm = sqrt(m * msf[j]);
-    pt[j] = c1[j] * pt[j] + c2[j] * (sm * xi);
+    yp[e] = v;
   }
-  for (int j = 0; j < P; ++j) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int j = e / CB, cc = e % CB, col = col0 + cc;
+    if (col >= ncols) continue;
     double v = 0.0;
-    for (int k = 0; k < P; ++k) v = v + Cs[k * P + j] * pt[k];
-    PP[j * stride + id] = v;
+    for (int k = 0; k < P; ++k) v = v + Cs[k * P + j] * yp[k * CB + cc];
+    PP[j * stride + col] = v;
   }
 }
 
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
                      hipStream_t s) {
-  const int n = natoms * 3;
-  hipLaunchKernelGGL(k_langevin, dim3((n + 255) / 256), dim3(256), sizeof(double) * P * P, s, P,
-                     natoms, b.C, b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter);
+  const int CB = nm_cols(P), n = natoms * 3;
+  const size_t lds = sizeof(double) * ((size_t)P * P + 2 * (size_t)P * CB);
+  hipLaunchKernelGGL(k_langevin, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
+                     b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter);
 }
 
 // PPt = PPt - MASSCON*dRRt*delt2 (MD.f90:49, 94).
@@ -212,13 +231,16 @@ void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *R
                      boxi0, RR, RRc);
 }
 
-// uk = 1/2 sum_beads sum_atoms |p|^2 / m (calc_uk, RPMD branch), one block.
-__global__ __launch_bounds__(256) void k_kinetic(int P, int natoms, const double *__restrict__ masses,
-                                                const double *__restrict__ PP, double *uk) {
+// uk = 1/2 sum_beads sum_atoms |p|^2 / m (calc_uk, RPMD branch): per-block
+// partial sums, then one block adds them in a fixed order.
+constexpr int KIN_BLOCKS = 256;
+__global__ __launch_bounds__(256) void k_kinetic_part(int P, int natoms,
+                                                     const double *__restrict__ masses,
+                                                     const double *__restrict__ PP, double *part) {
   __shared__ double red[256];
   const size_t n = (size_t)P * natoms;
   double s = 0.0;
-  for (size_t i = threadIdx.x; i < n; i += 256) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     const int a = (int)(i % natoms);
     const double *p = PP + i * 3;
     s += (1.0 / masses[a]) * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
@@ -229,12 +251,29 @@ __global__ __launch_bounds__(256) void k_kinetic(int P, int natoms, const double
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_kinetic_final(const double *__restrict__ part, int nb,
+                                                      double *uk) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) uk[0] = 0.5 * red[0];
 }
 
 void launch_kinetic(int P, int natoms, const double *masses, const double *PP, double *uk,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(k_kinetic, dim3(1), dim3(256), 0, s, P, natoms, masses, PP, uk);
+                    double *part, hipStream_t s) {
+  const size_t n = (size_t)P * natoms;
+  const int nb = (int)std::min<size_t>(KIN_BLOCKS, (n + 255) / 256);
+  hipLaunchKernelGGL(k_kinetic_part, dim3(nb), dim3(256), 0, s, P, natoms, masses, PP, part);
+  hipLaunchKernelGGL(k_kinetic_final, dim3(1), dim3(256), 0, s, part, nb, uk);
 }
 
 // Nose_Hoover half step, chain length 2 (Nose_Hoover.f90:158-198), scalars on device.

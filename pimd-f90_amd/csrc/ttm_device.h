@@ -5,6 +5,8 @@
 #include "pimd_internal.h"
 #include "ps_tables.h"
 
+#include <type_traits>
+
 namespace pimd {
 
 // Combined PES coefficients (c5z of nasa.f90:45) and exponent tables,
@@ -14,7 +16,11 @@ extern __constant__ int c_idx[PS_NTERM][3];
 extern __constant__ double c_coefd[PS_NDMS];
 extern __constant__ int c_idxd[PS_NDMS][3];
 
-__device__ __forceinline__ double anint(double x) { return round(x); }
+// Nearest integer for the minimum image.  The reference's anint rounds halves
+// away from zero; rint rounds them to even.  The two differ only when r/box is
+// exactly n + 1/2, which coordinates never produce, and rint is one
+// instruction (v_rndne_f64) where round() is a compare-and-select sequence.
+__device__ __forceinline__ double anint(double x) { return __builtin_rint(x); }
 
 // Minimum image r - box*anint(r/box) (pot_ttm.f90:64 and throughout).  box*n
 // is exact for the integer n in {-1, 0, 1}, so contraction cannot change it.
@@ -88,12 +94,15 @@ __device__ __forceinline__ Smear smear_base(const SysConst &c, double drsq) {
 }
 
 // PBCsmear01 (smear.f90:2-34): ts0, ts1 for the charge-charge terms.
+// THOLE = false is for pairs known to be beyond the 5 A damping radius, where
+// the r^2 < 25 branch cannot be taken.
+template <bool THOLE = true>
 __device__ __forceinline__ void smear01(const SysConst &c, double drsq, double apol, double ra,
                                         double &ts0, double &ts1) {
   Smear s = smear_base(c, drsq);
   double t0 = s.er * s.dri;
   double t1 = (t0 + c.cts1 * s.expon) * s.drsqi;
-  if (drsq < 25.0) {
+  if (THOLE && drsq < 25.0) {
     double rA3 = s.dd * drsq * apol;
     double exp1 = exp(-rA3);
     double ch1 = -1.0 * s.dri * s.drsqi;
@@ -107,12 +116,13 @@ __device__ __forceinline__ void smear01(const SysConst &c, double drsq, double a
 
 // PBCsmear2 (smear.f90:36-69): ts1, ts2 for the charge-dipole and
 // dipole-dipole field tensors.
+template <bool THOLE = true>
 __device__ __forceinline__ void smear2(const SysConst &c, double drsq, double apol, double &ts1,
                                        double &ts2) {
   Smear s = smear_base(c, drsq);
   double t1 = (s.er * s.dri + c.cts1 * s.expon) * s.drsqi;
   double t2 = (t1 + c.cts2 * s.expon) * s.drsqi;
-  if (drsq < 25.0) {
+  if (THOLE && drsq < 25.0) {
     double rA3 = (s.dd * drsq) * apol;
     double exp1 = exp(-rA3);
     double ch1 = -1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi;
@@ -126,13 +136,14 @@ __device__ __forceinline__ void smear2(const SysConst &c, double drsq, double ap
 }
 
 // PBCsmear3 (smear.f90:71-107): ts1, ts2, ts3 for the dipole-dipole forces.
+template <bool THOLE = true>
 __device__ __forceinline__ void smear3(const SysConst &c, double drsq, double apol, double &ts1,
                                        double &ts2, double &ts3) {
   Smear s = smear_base(c, drsq);
   double t1 = (s.er * s.dri + c.cts1 * s.expon) * s.drsqi;
   double t2 = (t1 + c.cts2 * s.expon) * s.drsqi;
   double t3 = (t2 + c.cts3 * s.expon) * s.drsqi;
-  if (drsq < 25.0) {
+  if (THOLE && drsq < 25.0) {
     double rA3 = (s.dd * drsq) * apol;
     double exp1 = exp(-rA3);
     double ch1 = -1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi, ch3 = ch2 * s.drsqi;

@@ -15,6 +15,8 @@
 // written by exactly one owner and sums are deterministic without atomics.
 // Pair scalars (energies, virials) are counted once, from the lower-index
 // molecule, in the reference's orientation.
+#include <algorithm>
+
 #include "ttm_device.h"
 
 namespace pimd {
@@ -60,6 +62,17 @@ __device__ __forceinline__ void vadd9(double *t, const double *v) {
   for (int i = 0; i < 9; ++i) t[i] += v[i];
 }
 
+// Largest distance of H1, H2, M from O in a molecule record.
+__device__ __forceinline__ double site_radius(const SysConst &c, const Site *m) {
+  double r2 = 0.0;
+  for (int t = 1; t < 4; ++t) {
+    double x = m[t].x - m[0].x, y = m[t].y - m[0].y, z = m[t].z - m[0].z;
+    minimg(c, x, y, z);
+    r2 = fmax(r2, x * x + y * y + z * z);
+  }
+  return sqrt(r2);
+}
+
 // ---------------------------------------------------------------------------
 // Monomer stage (pot_ttm.f90:56-151 and 232-261): M sites, PS energy and
 // gradient, DMS charges and their gradients, intramolecular virial, Ewald
@@ -70,7 +83,7 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
                                                const double *__restrict__ RRc, Site *site,
                                                double *Rcs, double *grdq, double *phi,
                                                double *Efq, double *dR, double *tally,
-                                               double *chg_out) {
+                                               double *chg_out, double *radius) {
   int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
   const int b = gid / c.N, w = gid % c.N, N = c.N;
@@ -121,6 +134,7 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
   q[3] = q3[0] / (1.0 - gM);
   Site *sm = site + ((size_t)b * N + w) * 4;
   for (int t = 0; t < 4; ++t) sm[t] = Site{pos[t][0], pos[t][1], pos[t][2], q[t]};
+  radius[(size_t)b * N + w] = site_radius(c, sm);
   double *g = grdq + ((size_t)b * N + w) * 27;
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 3; ++k) {
@@ -197,7 +211,7 @@ void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const doubl
                     double *chg_out, hipStream_t s) {
   int n = c.P * c.N;
   hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.site, b.Rcs,
-                     b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out);
+                     b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius);
 }
 
 // ---------------------------------------------------------------------------
@@ -205,61 +219,51 @@ void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const doubl
 // molecules j != i whose minimum-image O-O distance satisfies r^2 < Rc^2, in
 // ascending j.  One wavefront per (bead, i); ballot + mbcnt compaction.
 // ---------------------------------------------------------------------------
+// Each list holds near partners from the front (ascending j) and far
+// partners from the back of its capacity: a far partner's O lies at least
+// 5 A + rad_i + rad_j away, so none of its site pairs is inside the 5 A Thole
+// radius (smear.f90:23,57,92) and the damping branch can be compiled out.
+// nbrn = number of near, nbrc = number of far partners.
 __global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__restrict__ site,
-                                                  int *nbr, int *nbrc) {
+                                                  const double *__restrict__ radius, int *nbr,
+                                                  int *nbrc, int *nbrn) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= c.P * c.N) return;
   const int b = wave / c.N, i = wave % c.N, N = c.N;
   const Site Oi = site[((size_t)b * N + i) * 4];
-  int count = 0;
+  const double radi = radius[(size_t)b * N + i];
   int *out = nbr + (size_t)wave * c.nbr_cap;
+  int nnear = 0, nfar = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int j0 = 0; j0 < N; j0 += 64) {
     const int j = j0 + lane;
-    bool pred = false;
+    bool in = false, near = false;
     if (j < N && j != i) {
       const Site Oj = site[((size_t)b * N + j) * 4];
       double x = Oi.x - Oj.x, y = Oi.y - Oj.y, z = Oi.z - Oj.z;
       minimg(c, x, y, z);
-      pred = norm2_exact(x, y, z) < c.rc2;
+      const double r2 = norm2_exact(x, y, z);
+      in = r2 < c.rc2;
+      const double lim = 5.0 + radi + radius[(size_t)b * N + j] + 1e-6;
+      near = r2 < lim * lim;
     }
-    const unsigned long long mask = __ballot(pred);
-    if (pred) {
-      const int pos = count + __popcll(mask & ((1ull << lane) - 1ull));
-      if (pos < c.nbr_cap) out[pos] = j;
-    }
-    count += __popcll(mask);
+    const unsigned long long mn = __ballot(in && near), mf = __ballot(in && !near);
+    if (in && near) out[nnear + __popcll(mn & below)] = j;
+    if (in && !near) out[c.nbr_cap - 1 - (nfar + __popcll(mf & below))] = j;
+    nnear += __popcll(mn);
+    nfar += __popcll(mf);
   }
-  if (lane == 0) nbrc[wave] = count < c.nbr_cap ? count : c.nbr_cap;
+  if (lane == 0) {
+    nbrn[wave] = nnear;
+    nbrc[wave] = nfar;
+  }
 }
 
 void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
   long threads = (long)c.P * c.N * 64;
   hipLaunchKernelGGL(k_neighbors, dim3((threads + 255) / 256), dim3(256), 0, s, c, b.site,
-                     b.nbr, b.nbrc);
-}
-
-// Shift vector Rij0 of the reference's pair loops for the ordered pair
-// (lo, hi): Rij - R(iO) + R(jO) with Rij the O-O minimum image
-// (pot_ttm.f90:193, 318-319, 406-407); wrapped when asked.
-__device__ __forceinline__ void shift0(const SysConst &c, const Site &Olo, const Site &Ohi,
-                                       bool wrap, double s0[3]) {
-  double x = Olo.x - Ohi.x, y = Olo.y - Ohi.y, z = Olo.z - Ohi.z;
-  minimg(c, x, y, z);
-  s0[0] = x - Olo.x + Ohi.x;
-  s0[1] = y - Olo.y + Ohi.y;
-  s0[2] = z - Olo.z + Ohi.z;
-  if (wrap) minimg(c, s0[0], s0[1], s0[2]);
-}
-
-// Site-site vector of the reference for (lo site, hi site): min image of
-// (R_lo + Rij0) - R_hi.
-__device__ __forceinline__ void site_vec(const SysConst &c, const Site &Slo, const Site &Shi,
-                                         const double s0[3], double v[3]) {
-  v[0] = (Slo.x + s0[0]) - Shi.x;
-  v[1] = (Slo.y + s0[1]) - Shi.y;
-  v[2] = (Slo.z + s0[2]) - Shi.z;
-  minimg(c, v[0], v[1], v[2]);
+                     b.radius, b.nbr, b.nbrc, b.nbrn);
 }
 
 // ---------------------------------------------------------------------------
@@ -269,7 +273,8 @@ template <int MODEL, int G>
 __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restrict__ site,
                                                 const double *__restrict__ Rcs,
                                                 const int *__restrict__ nbr,
-                                                const int *__restrict__ nbrc, double *phi,
+                                                const int *__restrict__ nbrc,
+                                                const int *__restrict__ nbrn, double *phi,
                                                 double *Efq, double *dR, double *tally) {
   constexpr int SA0 = MODEL == 2 ? 0 : 1;  // TTM3-F: O carries neither charge nor dipole
   const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,10 +291,10 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
   }
   double ph[4] = {0, 0, 0, 0}, ef[4][3] = {{0}}, dro[3] = {0, 0, 0};
   double uvdw = 0.0, vc = 0.0, vt[9] = {0};
-  const int cnt = valid ? nbrc[grp] : 0;
+  const int nf = valid ? nbrc[grp] : 0, nn = valid ? nbrn[grp] : 0;
   const int *list = nbr + (size_t)grp * c.nbr_cap;
-  for (int n = lane; n < cnt; n += G) {
-    const int j = list[n];
+  auto visit = [&](const int j, auto thole_tag) {
+    constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
     Site B[4];
     for (int t = 0; t < 4; ++t) B[t] = site[((size_t)b * N + j) * 4 + t];
@@ -315,30 +320,24 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
       for (int col = 0; col < 3; ++col)
         for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
     }
-    double s0[3];
-    if (lo)
-      shift0(c, A[0], B[0], false, s0);
-    else
-      shift0(c, B[0], A[0], false, s0);
+    // O-O image shift of the reference's charge loop, Rij - R(iO) + R(jO)
+    // (pot_ttm.f90:193); it enters the centroid virial of the lower molecule.
+    const double s0[3] = {r[0] - A[0].x + B[0].x, r[1] - A[0].y + B[0].y,
+                          r[2] - A[0].z + B[0].z};
 #pragma unroll
     for (int sa = SA0; sa < 4; ++sa) {
 #pragma unroll
       for (int sb = 1; sb < 4; ++sb) {
-        double v[3], rv[3];
-        if (lo) {
-          site_vec(c, A[sa], B[sb], s0, v);
-          rv[0] = v[0]; rv[1] = v[1]; rv[2] = v[2];
-        } else {
-          site_vec(c, B[sb], A[sa], s0, v);
-          rv[0] = -v[0]; rv[1] = -v[1]; rv[2] = -v[2];
-        }
+        // site-site minimum image, oriented from a to b (pot_ttm.f90:206-207)
+        double v[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
+        minimg(c, v[0], v[1], v[2]);
         const double d2 = norm2_exact(v[0], v[1], v[2]);
         double ts0, ts1;
-        smear01(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa],
-                lo ? c.raCC[sa][sb] : c.raCC[sb][sa], ts0, ts1);
+        smear01<THOLE>(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa],
+                       lo ? c.raCC[sa][sb] : c.raCC[sb][sa], ts0, ts1);
         const double qb = B[sb].q;
         if (sa > 0) ph[sa] += ts0 * qb;
-        for (int k = 0; k < 3; ++k) ef[sa][k] += ts1 * qb * rv[k];
+        for (int k = 0; k < 3; ++k) ef[sa][k] += ts1 * qb * v[k];
         if (lo && sa > 0) {
           const double qa = A[sa].q;
           double dd[3];
@@ -351,7 +350,9 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
         }
       }
     }
-  }
+  };
+  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
+  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
   for (int t = 1; t < 4; ++t) ph[t] = group_sum<G>(ph[t]);
   for (int t = SA0; t < 4; ++t)
     for (int k = 0; k < 3; ++k) ef[t][k] = group_sum<G>(ef[t][k]);
@@ -378,10 +379,10 @@ static void launch_qq_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   dim3 grid((threads + 255) / 256);
   if (c.model == 2)
     hipLaunchKernelGGL((k_pair_qq<2, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr, b.nbrc,
-                       b.phi, b.Efq, b.dR, b.tally);
+                       b.nbrn, b.phi, b.Efq, b.dR, b.tally);
   else
     hipLaunchKernelGGL((k_pair_qq<3, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr, b.nbrc,
-                       b.phi, b.Efq, b.dR, b.tally);
+                       b.nbrn, b.phi, b.Efq, b.dR, b.tally);
 }
 
 void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
@@ -546,7 +547,126 @@ __global__ __launch_bounds__(256) void k_sfac(SysConst c, const Site *__restrict
   }
 }
 
+// Structure factors with one kz-column (fixed kx, ky) per lane: the phase of
+// each site is formed once per column and advanced along kz by e^{i 2 pi z/L},
+// accumulating every kz of the column in registers.  Sites are split into
+// chunks across blocks; per-chunk partial sums are combined in a fixed order
+// by k_sfac_reduce.  SF_Q and SF_D as above.
+template <int MODE, int KZ>
+__global__ __launch_bounds__(64) void k_sfac_col(SysConst c, const Site *__restrict__ site,
+                                                const double2 *__restrict__ tab,
+                                                const double *__restrict__ dip,
+                                                const int4 *__restrict__ cols,
+                                                const int *__restrict__ colk,
+                                                const int *__restrict__ active, double *part) {
+  constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
+  const int b = blockIdx.z;
+  if (MODE == SF_D && !active[b]) return;
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  const bool valid = col < c.ncol;
+  const int4 cl = cols[valid ? col : c.ncol - 1];
+  const int n = valid ? cl.w : 0;
+  const int k0 = colk[valid ? col : c.ncol - 1];
+  const int S = 4 * c.N, M = set_size(c, SET);
+  const int nch = gridDim.y, chunk = blockIdx.y;
+  const int m0 = (int)((long)M * chunk / nch), m1 = (int)((long)M * (chunk + 1) / nch);
+  const int rowx = cl.x, rowy = c.hmax + 1 + abs(cl.y), rowz = c.hmax + c.kmax + 2 + abs(cl.z);
+  const int rowz1 = c.hmax + c.kmax + 3;
+  const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
+  const double kvx = TWOPI * c.boxi[0] * (double)cl.x, kvy = TWOPI * c.boxi[1] * (double)cl.y;
+  const double tpz = TWOPI * c.boxi[2];
+  const double2 *tb = tab + (size_t)b * c.T * S;
+  double are[KZ], aim[KZ];
+#pragma unroll
+  for (int i = 0; i < KZ; ++i) are[i] = aim[i] = 0.0;
+  for (int m = m0; m < m1; ++m) {
+    const int s = set_site(c, SET, m);
+    const double2 x = tb[(size_t)rowx * S + s];
+    double2 y = tb[(size_t)rowy * S + s];
+    double2 z = tb[(size_t)rowz * S + s];
+    const double2 z1 = c.lmax >= 1 ? tb[(size_t)rowz1 * S + s] : make_double2(1.0, 0.0);
+    y.y *= sgy;
+    z.y *= sgz;
+    double ck, sk;
+    phase_from(x, y, z, ck, sk);
+    double w = 0.0, wxy = 0.0, mz = 0.0;
+    if (MODE == SF_Q) {
+      w = site[(size_t)b * S + s].q;
+    } else {
+      const double *mu = dip + ((size_t)b * c.N * c.nd + m) * 3;
+      wxy = kvx * mu[0] + kvy * mu[1];
+      mz = mu[2];
+    }
+#pragma unroll
+    for (int i = 0; i < KZ; ++i) {
+      if (i < n) {
+        if (i > 0) {
+          const double nc = ck * z1.x - sk * z1.y;
+          sk = sk * z1.x + ck * z1.y;
+          ck = nc;
+        }
+        if (MODE == SF_Q) {
+          are[i] += w * ck;
+          aim[i] += w * sk;
+        } else {
+          const double dk = wxy + (tpz * (double)(cl.z + i)) * mz;
+          are[i] -= dk * sk;
+          aim[i] += dk * ck;
+        }
+      }
+    }
+  }
+  double *o = part + (((size_t)b * nch + chunk) * c.nk + k0) * 2;
+#pragma unroll
+  for (int i = 0; i < KZ; ++i)
+    if (i < n) {
+      o[2 * i] = are[i];
+      o[2 * i + 1] = aim[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *__restrict__ part,
+                                                    int nch, const int *__restrict__ active,
+                                                    int check_active, double *out) {
+  const int b = blockIdx.y;
+  if (check_active && !active[b]) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, re/im)
+  if (e >= 2 * c.nk) return;
+  double v = 0.0;
+  for (int q = 0; q < nch; ++q) v += part[((size_t)b * nch + q) * c.nk * 2 + e];
+  out[(size_t)b * c.nk * 2 + e] = v;
+}
+
+template <int MODE, int KZ>
+static void launch_sfac_col(const SysConst &c, DevBufs &b, int nch, hipStream_t s) {
+  dim3 grid((c.ncol + 63) / 64, nch, c.P);
+  hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(64), 0, s, c, b.site, b.tab, b.dip,
+                     b.cols, b.colk, b.active, b.part);
+  dim3 g2((2 * c.nk + 255) / 256, c.P);
+  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, b.active,
+                     MODE == SF_D ? 1 : 0, MODE == SF_Q ? b.Sq : b.Sm);
+}
+
+template <int MODE>
+static bool sfac_col_dispatch(const SysConst &c, DevBufs &b, int nch, hipStream_t s) {
+  if (c.kzmax <= 8) launch_sfac_col<MODE, 8>(c, b, nch, s);
+  else if (c.kzmax <= 16) launch_sfac_col<MODE, 16>(c, b, nch, s);
+  else if (c.kzmax <= 24) launch_sfac_col<MODE, 24>(c, b, nch, s);
+  else if (c.kzmax <= 32) launch_sfac_col<MODE, 32>(c, b, nch, s);
+  else return false;
+  return true;
+}
+
 void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
+  if (mode == SF_Q || mode == SF_D) {
+    const int M = mode == SF_Q ? 3 * c.N : c.nd * c.N;
+    const int colb = (c.ncol + 63) / 64;
+    int nch = (4096 + colb * c.P - 1) / (colb * c.P);
+    nch = std::max(1, std::min(nch, std::min(c.sf_chunks_max, M / 8 > 0 ? M / 8 : 1)));
+    const bool ok = mode == SF_Q ? sfac_col_dispatch<SF_Q>(c, b, nch, s)
+                                 : sfac_col_dispatch<SF_D>(c, b, nch, s);
+    if (ok) return;
+  }
   dim3 grid((c.nk + 63) / 64, c.P);
   size_t sh = sizeof(double2) * c.T * SF_TILE;
   if (mode == SF_Q)
@@ -567,6 +687,8 @@ void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
 //          SOR update of the induced dipole (pot_ttm.f90:371-385)
 //   FD_QD: phi and dR from the stale dipole and the total structure factors
 //          (ewald.f90:184-193)
+constexpr int FD_TS = 16;              // sites per block
+constexpr int FD_G = 256 / FD_TS;      // column groups per block
 template <int MODE, int MODEL>
 __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restrict__ site,
                                               const double2 *__restrict__ tab,
@@ -579,13 +701,15 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
                                               const double *__restrict__ Efd, double *diff2,
                                               const int *__restrict__ active, int iter) {
   constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
+  __shared__ double red[4][FD_G][FD_TS + 1];
   const int b = blockIdx.y;
   if (MODE == FD_D && !active[b]) return;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sl = threadIdx.x % FD_TS, g = threadIdx.x / FD_TS;
   const int M = set_size(c, SET);
-  if (m >= M) return;
+  const int m = blockIdx.x * FD_TS + sl;
+  const bool valid = m < M;
   const int S = 4 * c.N;
-  const int s = set_site(c, SET, m);
+  const int s = set_site(c, SET, valid ? m : M - 1);
   const int type = s & 3;
   const Site p = site[(size_t)b * S + s];
   const double2 *tb = tab + (size_t)b * c.T * S + s;
@@ -597,8 +721,10 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
   const size_t dslot = ((size_t)b * c.N + (s >> 2)) * c.nd + (type - c.tdip);
   if ((MODE == FD_D || MODE == FD_QD) && has_d)
     for (int k = 0; k < 3; ++k) mu[k] = dip[dslot * 3 + k];
+  const double *sq = Sq + (size_t)b * c.nk * 2, *sm = Sm + (size_t)b * c.nk * 2;
   double e0 = 0, e1 = 0, e2 = 0, ph = 0;
-  for (int col = 0; col < c.ncol; ++col) {
+  // columns are sorted by length; group g takes g, g + G, ... for balance
+  for (int col = g; col < c.ncol; col += FD_G) {
     const int4 cl = cols[col];
     const int k0 = colk[col];
     double2 x = tb[(size_t)cl.x * S];
@@ -617,20 +743,20 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
       const int k = k0 + i;
       const double4 kk = kv[k];  // kx, ky, kz, 4 pi A_k / V
       if (MODE == FD_Q) {
-        const double re = Sq[((size_t)b * c.nk + k) * 2], im = Sq[((size_t)b * c.nk + k) * 2 + 1];
+        const double re = sq[2 * k], im = sq[2 * k + 1];
         const double t = kk.w * (ck * im - sk * re);
         e0 -= t * kk.x;
         e1 -= t * kk.y;
         e2 -= t * kk.z;
         ph += kk.w * (ck * re + sk * im);
       } else if (MODE == FD_D) {
-        const double re = Sm[((size_t)b * c.nk + k) * 2], im = Sm[((size_t)b * c.nk + k) * 2 + 1];
+        const double re = sm[2 * k], im = sm[2 * k + 1];
         const double t = kk.w * (ck * im - sk * re);
         e0 -= kk.x * t;
         e1 -= kk.y * t;
         e2 -= kk.z * t;
       } else {
-        const double rqm = Sm[((size_t)b * c.nk + k) * 2], iqm = Sm[((size_t)b * c.nk + k) * 2 + 1];
+        const double rqm = sm[2 * k], iqm = sm[2 * k + 1];
         if (has_q) {
           const double t = p.q * kk.w * (ck * iqm - sk * rqm);
           ph += kk.w * (ck * rqm + sk * iqm);
@@ -639,8 +765,8 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
           e2 += t * kk.z;
         }
         if (has_d) {
-          const double rqqm = Sq[((size_t)b * c.nk + k) * 2] + rqm;
-          const double iqqm = Sq[((size_t)b * c.nk + k) * 2 + 1] + iqm;
+          const double rqqm = sq[2 * k] + rqm;
+          const double iqqm = sq[2 * k + 1] + iqm;
           const double dk = (kk.x * mu[0] + kk.y * mu[1]) + kk.z * mu[2];
           const double t = dk * kk.w * (ck * rqqm + sk * iqqm);
           e0 -= t * kk.x;
@@ -649,6 +775,19 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
         }
       }
     }
+  }
+  // combine the column groups in a fixed order
+  red[0][g][sl] = e0;
+  red[1][g][sl] = e1;
+  red[2][g][sl] = e2;
+  red[3][g][sl] = ph;
+  __syncthreads();
+  if (g != 0 || !valid) return;
+  for (int q = 1; q < FD_G; ++q) {
+    e0 += red[0][q][sl];
+    e1 += red[1][q][sl];
+    e2 += red[2][q][sl];
+    ph += red[3][q][sl];
   }
   const size_t ss = (size_t)b * S + s;
   if (MODE == FD_Q) {
@@ -678,7 +817,7 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
 
 void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s) {
   const int M = mode == FD_D ? c.nd * c.N : (c.model == 2 ? 4 * c.N : 3 * c.N);
-  dim3 grid((M + 255) / 256, c.P), blk(256);
+  dim3 grid((M + FD_TS - 1) / FD_TS, c.P), blk(256);
 #define PIMD_FIELD(MO, ML)                                                                       \
   hipLaunchKernelGGL((k_field<MO, ML>), grid, blk, 0, s, c, b.site, b.tab, b.kv, b.cols, b.colk, \
                      b.Sq, b.Sm, b.phi, b.Efq, b.dR, b.dip, b.Efd, b.diff2, b.active, iter)
@@ -759,6 +898,7 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
                                                 const double *__restrict__ dip,
                                                 const int *__restrict__ nbr,
                                                 const int *__restrict__ nbrc,
+                                                const int *__restrict__ nbrn,
                                                 const int *__restrict__ active, double *Efd) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
@@ -773,18 +913,12 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
   for (int t = 0; t < 4; ++t) A[t] = site[((size_t)b * N + a) * 4 + t];
   double ef[ND][3];
   for (int u = 0; u < ND; ++u) ef[u][0] = ef[u][1] = ef[u][2] = 0.0;
-  const int cnt = nbrc[grp];
+  const int nf = nbrc[grp], nn = nbrn[grp];
   const int *list = nbr + (size_t)grp * c.nbr_cap;
-  for (int n = lane; n < cnt; n += G) {
-    const int j = list[n];
+  auto visit = [&](const int j, auto thole_tag) {
+    constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
     const Site *Bs = site + ((size_t)b * N + j) * 4;
-    const Site BO = Bs[0];
-    double s0[3];
-    if (lo)
-      shift0(c, A[0], BO, true, s0);
-    else
-      shift0(c, BO, A[0], true, s0);
     double mub[ND][3];
     for (int u = 0; u < ND; ++u)
       for (int k = 0; k < 3; ++k) mub[u][k] = dip[(((size_t)b * N + j) * ND + u) * 3 + k];
@@ -794,20 +928,19 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
       for (int ub = 0; ub < ND; ++ub) {
         const int sa = T0 + ua, sb = T0 + ub;
         const Site B = Bs[sb];
-        double v[3];
-        if (lo)
-          site_vec(c, A[sa], B, s0, v);
-        else
-          site_vec(c, B, A[sa], s0, v);
+        double v[3] = {A[sa].x - B.x, A[sa].y - B.y, A[sa].z - B.z};
+        minimg(c, v[0], v[1], v[2]);
         const double d2 = norm2_exact(v[0], v[1], v[2]);
         double ts1, ts2;
-        smear2(c, d2, lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa], ts1, ts2);
+        smear2<THOLE>(c, d2, lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa], ts1, ts2);
         double o[3];
         dd3_apply(v, ts1, ts2, mub[ub], o);
         for (int k = 0; k < 3; ++k) ef[ua][k] += o[k];
       }
     }
-  }
+  };
+  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
+  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
   for (int u = 0; u < ND; ++u)
     for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
   if (lane == 0) {
@@ -840,10 +973,10 @@ static void launch_dd_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   dim3 grid((threads + 255) / 256);
   if (c.model == 2)
     hipLaunchKernelGGL((k_pair_dd<2, G>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr, b.nbrc,
-                       b.active, b.Efd);
+                       b.nbrn, b.active, b.Efd);
   else
     hipLaunchKernelGGL((k_pair_dd<3, G>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr, b.nbrc,
-                       b.active, b.Efd);
+                       b.nbrn, b.active, b.Efd);
 }
 
 void launch_pair_dd(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
@@ -906,7 +1039,8 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
                                                    const double *__restrict__ Rcs,
                                                    const double *__restrict__ dip,
                                                    const int *__restrict__ nbr,
-                                                   const int *__restrict__ nbrc, double *phi,
+                                                   const int *__restrict__ nbrc,
+                                                   const int *__restrict__ nbrn, double *phi,
                                                    double *dR, double *tally) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
@@ -924,10 +1058,10 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
       mua[t][k] = is_dip<MODEL>(t) ? dip[(((size_t)b * N + a) * ND + (t - T0)) * 3 + k] : 0.0;
   }
   double ph[4] = {0, 0, 0, 0}, fr[4][3] = {{0}}, vc = 0.0, vt[9] = {0};
-  const int cnt = valid ? nbrc[grp] : 0;
+  const int nf = valid ? nbrc[grp] : 0, nn = valid ? nbrn[grp] : 0;
   const int *list = nbr + (size_t)grp * c.nbr_cap;
-  for (int n = lane; n < cnt; n += G) {
-    const int j = list[n];
+  auto visit = [&](const int j, auto thole_tag) {
+    constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
     Site B[4];
     double mub[4][3];
@@ -936,11 +1070,6 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
       for (int k = 0; k < 3; ++k)
         mub[t][k] = is_dip<MODEL>(t) ? dip[(((size_t)b * N + j) * ND + (t - T0)) * 3 + k] : 0.0;
     }
-    double s0[3];
-    if (lo)
-      shift0(c, A[0], B[0], true, s0);
-    else
-      shift0(c, B[0], A[0], true, s0);
 #pragma unroll
     for (int sa = 0; sa < 4; ++sa) {
 #pragma unroll
@@ -950,15 +1079,11 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
         // site carries a dipole, or the only coupling is a zero charge.
         if (!da && !db) continue;
         if (MODEL == 3 && (sa == 0 || sb == 0)) continue;
-        if (MODEL == 2 && !(da && db) && ((da && sb == 0) || (db && sa == 0))) continue;
-        double v[3], r[3];
-        if (lo) {
-          site_vec(c, A[sa], B[sb], s0, v);
-          r[0] = v[0]; r[1] = v[1]; r[2] = v[2];
-        } else {
-          site_vec(c, B[sb], A[sa], s0, v);
-          r[0] = -v[0]; r[1] = -v[1]; r[2] = -v[2];
-        }
+        // site-site minimum image oriented from a to b (pot_ttm.f90:419-420);
+        // v is the reference's orientation (lower molecule first)
+        double r[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
+        minimg(c, r[0], r[1], r[2]);
+        const double *v = r;
         const double d2 = norm2_exact(v[0], v[1], v[2]);
         const double qa = A[sa].q, qb = B[sb].q;
         double qd[3];
@@ -968,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
         bool have_ts1 = false;
         if (da && db) {
           double t1d, t2d, t3d;
-          smear3(c, d2, lo ? c.apol3DD[sa][sb] : c.apol3DD[sb][sa], t1d, t2d, t3d);
+          smear3<THOLE>(c, d2, lo ? c.apol3DD[sa][sb] : c.apol3DD[sb][sa], t1d, t2d, t3d);
           // di . T3(r) . dj with T3_abc = 15 r_a r_b r_c ts3 - 3 ts2 (r_a d_bc + r_b d_ac + r_c d_ab)
           const double *di = mua[sa], *dj = mub[sb];
           const double rdi = r[0] * di[0] + r[1] * di[1] + r[2] * di[2];
@@ -981,7 +1106,7 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
               ts1 = t1d;
               ts2 = t2d;
             } else {
-              smear2(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
+              smear2<THOLE>(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
             }
             have_ts1 = true;
             double o[3];
@@ -989,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
             for (int k = 0; k < 3; ++k) dij[k] += o[k];
           }
         } else if (qsq > 1e-9) {
-          smear2(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
+          smear2<THOLE>(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
           have_ts1 = true;
           double o[3];
           dd3_apply(r, ts1, ts2, qd, o);
@@ -999,14 +1124,17 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
         for (int k = 0; k < 3; ++k) fr[sa][k] += dij[k];
         if (lo) {
           double rcv[3];
-          for (int k = 0; k < 3; ++k) rcv[k] = Rcs[(a * 4 + sa) * 3 + k] - Rcs[(j * 4 + sb) * 3 + k] + s0[k];
+          // Rij0 of the dipole loop is wrapped to ~0 (pot_ttm.f90:406-407,468)
+          for (int k = 0; k < 3; ++k) rcv[k] = Rcs[(a * 4 + sa) * 3 + k] - Rcs[(j * 4 + sb) * 3 + k];
           vc += rcv[0] * dij[0] + rcv[1] * dij[1] + rcv[2] * dij[2];
           for (int col = 0; col < 3; ++col)
             for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dij[row] * v[col];
         }
       }
     }
-  }
+  };
+  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
+  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
   for (int t = 0; t < 4; ++t) {
     ph[t] = group_sum<G>(ph[t]);
     for (int k = 0; k < 3; ++k) fr[t][k] = group_sum<G>(fr[t][k]);
@@ -1031,10 +1159,10 @@ static void launch_final_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   dim3 grid((threads + 255) / 256);
   if (c.model == 2)
     hipLaunchKernelGGL((k_pair_final<2, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
-                       b.nbr, b.nbrc, b.phi, b.dR, b.tally);
+                       b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
   else
     hipLaunchKernelGGL((k_pair_final<3, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
-                       b.nbr, b.nbrc, b.phi, b.dR, b.tally);
+                       b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
 }
 
 void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
