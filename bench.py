@@ -67,8 +67,10 @@ def build_system(workload, seed=2024):
     from pimd_amd.systems import make_system
     model, N, P, rc, eps = WORKLOADS[workload]
     RRt, PPt, RRc, L = make_system(N, P, seed=seed)
+    # guess_initdip: the reference's default (input_file_reader.f90:83), the
+    # 4-point dipole predictor from the previous steps
     p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=rc, eps_ewald=eps,
-                  mass_o=15.995, mass_h=1.008).resolved()
+                  mass_o=15.995, mass_h=1.008, guess_initdip=True).resolved()
     return p, RRt, PPt, RRc
 
 
@@ -180,11 +182,14 @@ def main():
                    "model": "TTM3-F" if p.pot_model == 3 else "TTM2.1-F", "precision": "fp64",
                    "rc": round(p.rc, 4), "eps_ewald": p.eps_ewald, "nkvec": info["nkvec"],
                    "dt_fs": 0.25, "thermostat": "NM Langevin (PILE), 300 K",
-                   "polar_maxiter": p.polar_maxiter, "polar_eps": p.polar_eps},
+                   "polar_maxiter": p.polar_maxiter, "polar_eps": p.polar_eps,
+                   "guess_initdip": p.guess_initdip},
         "roofline": roof,
         "cpu_baseline": cpu,
         "energy_per_step_last": float(rec["Upot"][-1]) if args.steps else None,
         "kernels_ms_per_step": {k: round(v[1] / max(args.steps, 1), 4) for k, v in kernels.items()},
+        "sor_iterations_per_step": (round(kernels["field_d_sor"][0] / max(args.steps, 1), 2)
+                                    if "field_d_sor" in kernels else None),
     }
     print(json.dumps(out), flush=True)
 
