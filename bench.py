@@ -7,8 +7,12 @@ real-space Ewald with Thole damping, induced-dipole SOR solve, reciprocal
 space), all device resident.  Inputs are synthetic (seeded water box at
 0.997 g/cm^3 with free ring-polymer bead offsets at 300 K).
 
-Prints one JSON line.  value = molecule-bead force evaluations per second
-(whole job); ns_per_day is reported beside it.
+Both precision forms are timed: FP64 (the reference's arithmetic) and mixed
+(FP32 real-space pair terms with FP64 accumulation, FP64 refinement of the
+dipoles, FP64 k-space and ring-polymer steps).  Prints one JSON line.
+value = molecule-bead force evaluations per second of the mixed form (whole
+job); the FP64 form, the ratio and the mixed form's error against FP64 are
+reported beside it, with ns_per_day.
 """
 import argparse
 import json
@@ -109,33 +113,22 @@ def cpu_baseline(p, RRt, RRc, budget_s=15.0):
                       f"serial, {dt:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="h2o512_p32_ttm3f", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-profile", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the force path "
-              "shards by beads or replicas, which this build does not launch.", file=sys.stderr)
-        sys.exit(2)
+PEAK_FP32_TFLOPS = 157.3
+FP32_KERNELS = ("pair_qq", "pair_dd", "pair_final")
 
-    from pimd_amd.ttm import PIMDIntegrator, TTMForceField
-    p, RRt, PPt, RRc = build_system(args.workload)
+
+def run_form(p, RRt, PPt, RRc, args):
+    """Times args.steps PIMD steps of one precision form; returns its figures."""
+    from pimd_amd.ttm import PIMDIntegrator, TTMForceField, load_library
     ff = TTMForceField(p)
     info = ff.info()
     integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=1,
                            centroid_thermostat=True, tau_centroid=0.1)
-    integ.set_state(RRt, PPt, None)
     # start from consistent forces (the reference's first step kicks with zero)
     f0 = ff.full_bead_forces(RRt, RRc)
     integ.set_state(RRt, PPt, f0["dRRt"])
     if args.warmup > 0:
         integ.run(args.warmup, t0=1, seed=99)
-    from pimd_amd.ttm import load_library
     L = load_library()
     L.pimd_sync(ff.handle)
     t0 = time.perf_counter()
@@ -143,54 +136,115 @@ def main():
     L.pimd_sync(ff.handle)
     elapsed = time.perf_counter() - t0
     ms = 1000.0 * elapsed / args.steps
-    evals = p.nwaters * p.nbeads / (ms / 1000.0)
-    ns_day = (1000.0 / ms) * 0.25e-6 * 86400.0
+    out = {"ms_per_step": ms, "value": p.nwaters * p.nbeads / (ms / 1000.0),
+           "ns_per_day": (1000.0 / ms) * 0.25e-6 * 86400.0, "f0": f0, "rec": rec,
+           "info": info, "kernels": {}, "roofline": None}
+    if args.no_profile:
+        return out
+    ff.profile(True)
+    ff.profile_reset()
+    integ.run(args.steps, t0=1 + args.warmup + args.steps, seed=99)
+    kernels = ff.profile_report()
+    ff.profile(False)
+    out["kernels"] = kernels
+    nnb = (4.0 / 3.0) * np.pi * p.rc ** 3 / (p.box[0] * p.box[1] * p.box[2]) * p.nwaters
+    c = {"N": p.nwaters, "P": p.nbeads, "nk": info["nkvec"], "nnb": min(nnb, p.nwaters - 1),
+         "nd": 1 if p.pot_model == 3 else 3}
+    tot = {k: v[1] for k, v in kernels.items()}
+    dom = max(tot, key=tot.get)
+    cnt, tms = kernels[dom]
+    fl = flop_model(dom, c)
+    if fl is not None and cnt > 0:
+        fp32 = p.precision == 1 and dom in FP32_KERNELS
+        peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
+        avg_s = tms / cnt / 1000.0
+        ach = fl / avg_s / 1e12
+        out["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak,
+                           "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
+                           "kernel": dom, "arith": "fp32" if fp32 else "fp64",
+                           "avg_launch_ms": round(tms / cnt, 4), "flops_per_launch": fl}
+    ff.close()
+    return out
 
-    roof = None
-    kernels = {}
-    if not args.no_profile:
-        ff.profile(True)
-        ff.profile_reset()
-        integ.run(args.steps, t0=1 + args.warmup + args.steps, seed=99)
-        kernels = ff.profile_report()
-        ff.profile(False)
-        nnb = (4.0 / 3.0) * np.pi * p.rc ** 3 / (p.box[0] * p.box[1] * p.box[2]) * p.nwaters
-        c = {"N": p.nwaters, "P": p.nbeads, "nk": info["nkvec"], "nnb": min(nnb, p.nwaters - 1),
-             "nd": 1 if p.pot_model == 3 else 3}
-        tot = {k: v[1] for k, v in kernels.items()}
-        dom = max(tot, key=tot.get)
-        cnt, tms = kernels[dom]
-        fl = flop_model(dom, c)
-        if fl is not None and cnt > 0:
-            avg_s = tms / cnt / 1000.0
-            ach = fl / avg_s / 1e12
-            roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                    "kernel": dom, "avg_launch_ms": round(tms / cnt, 4),
-                    "flops_per_launch": fl}
+
+def _kern(res, steps):
+    return {k: round(v[1] / max(steps, 1), 4) for k, v in res["kernels"].items()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="h2o512_p32_ttm3f", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=("fp64", "mixed", "both"),
+                    help="form(s) to time; with both, value is the mixed form")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of "
+                         "known HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the force path "
+              "shards by beads or replicas, which this build does not launch.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_mib > 0:
+        import torch
+        n = args.copy_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    from pimd_amd.params import TTMParams
+    p, RRt, PPt, RRc = build_system(args.workload)
+    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {}
+    for f in forms:
+        q = TTMParams(**{**p.__dict__, "precision": 1 if f == "mixed" else 0})
+        res[f] = run_form(q, RRt, PPt, RRc, args)
+    main_form = forms[-1]
+    r = res[main_form]
 
     cpu = None if args.no_cpu else cpu_baseline(p, RRt, RRc)
     out = {
         "metric": "PIMD ns/day + molecule-bead force evals/s, 512 H2O P=32 TTM3-F, mixed vs FP64",
-        "value": round(evals, 1),
+        "value": round(r["value"], 1),
         "unit": "molecule-bead force evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms, 4), "ns_per_day": round(ns_day, 3),
+        "ms_per_step": round(r["ms_per_step"], 4), "ns_per_day": round(r["ns_per_day"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "fp32+fp64" if main_form == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": args.workload, "nwaters": p.nwaters, "nbeads": p.nbeads,
-                   "model": "TTM3-F" if p.pot_model == 3 else "TTM2.1-F", "precision": "fp64",
-                   "rc": round(p.rc, 4), "eps_ewald": p.eps_ewald, "nkvec": info["nkvec"],
+                   "model": "TTM3-F" if p.pot_model == 3 else "TTM2.1-F", "precision": main_form,
+                   "rc": round(p.rc, 4), "eps_ewald": p.eps_ewald, "nkvec": r["info"]["nkvec"],
                    "dt_fs": 0.25, "thermostat": "NM Langevin (PILE), 300 K",
                    "polar_maxiter": p.polar_maxiter, "polar_eps": p.polar_eps,
                    "guess_initdip": p.guess_initdip},
-        "roofline": roof,
+        "roofline": r["roofline"],
         "cpu_baseline": cpu,
-        "energy_per_step_last": float(rec["Upot"][-1]) if args.steps else None,
-        "kernels_ms_per_step": {k: round(v[1] / max(args.steps, 1), 4) for k, v in kernels.items()},
-        "sor_iterations_per_step": (round(kernels["field_d_sor"][0] / max(args.steps, 1), 2)
-                                    if "field_d_sor" in kernels else None),
+        "kernels_ms_per_step": _kern(r, args.steps),
+        "sor_iterations_per_step": (round(r["kernels"]["field_d_sor"][0] / max(args.steps, 1), 2)
+                                    if "field_d_sor" in r["kernels"] else None),
     }
+    if len(forms) == 2:
+        a, m = res["fp64"], res["mixed"]
+        df = m["f0"]["dRRt"] - a["f0"]["dRRt"]
+        out["fp64"] = {"value": round(a["value"], 1), "ms_per_step": round(a["ms_per_step"], 4),
+                       "ns_per_day": round(a["ns_per_day"], 3), "roofline": a["roofline"],
+                       "kernels_ms_per_step": _kern(a, args.steps)}
+        out["mixed_over_fp64"] = round(m["value"] / a["value"], 3)
+        out["mixed_error_vs_fp64"] = {
+            "force_rms_rel": float(np.sqrt(np.mean(df ** 2)) / np.sqrt(np.mean(a["f0"]["dRRt"] ** 2))),
+            "force_max_rel": float(np.max(np.abs(df)) / np.max(np.abs(a["f0"]["dRRt"]))),
+            "energy_rel_max": float(np.max(np.abs(m["f0"]["Upott"] - a["f0"]["Upott"])
+                                           / np.abs(a["f0"]["Upott"]))),
+            "trajectory_upot_rel_max": float(np.max(np.abs(m["rec"]["Upot"] - a["rec"]["Upot"])
+                                                    / np.abs(a["rec"]["Upot"]))),
+            "note": "first evaluation on the start configuration, and the summed potential "
+                    "over the timed steps of both forms run from the same state and noise"}
     print(json.dumps(out), flush=True)
 
 

@@ -33,8 +33,12 @@ extern "C" {
 #define PIMD_TTM21F 2
 #define PIMD_TTM3F 3
 
-#define PIMD_FP64 0     /* plain double precision */
-#define PIMD_MIXED 1    /* FP32 pair/matvec terms, FP64 refinement (reserved) */
+#define PIMD_FP64 0     /* plain double precision, the reference's arithmetic */
+#define PIMD_MIXED 1    /* real-space pair terms and the real-space dipole field
+                           of the SOR solve in FP32 (accumulated in FP64); the
+                           last SOR step of every bead redone in FP64; Ewald
+                           k-space, energies, virials and ring-polymer steps
+                           in FP64.  No counterpart in the reference.         */
 
 typedef struct pimd_ctx pimd_ctx;
 
@@ -49,7 +53,7 @@ typedef struct {
   int guess_initdip;  /* 4-point dipole predictor (pot_ttm.f90:281-297); the
                          history is kept per bead, as in a run with one MPI
                          rank per bead                                        */
-  int precision;      /* PIMD_FP64                                           */
+  int precision;      /* PIMD_FP64 or PIMD_MIXED                             */
   int device;         /* HIP device ordinal                                  */
   int reserved0;
   double box[3];      /* Angstrom                                            */

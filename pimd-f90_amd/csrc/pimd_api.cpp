@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -131,9 +132,9 @@ int dalloc(T **p, size_t n) {
 void free_all(pimd_ctx *x) {
   DevBufs &d = x->d;
   void *ptrs[] = {d.RR,   d.RRc,   d.site,   d.Rcs,    d.grdq,   d.phi,   d.Efq,   d.dR,
-                  d.dip,  d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.nbrn,  d.tab,
+                  d.dip,  d.dip_prev, d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.nbrn,  d.tab,
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
-                  d.khkl, d.cols,  d.colk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
+                  d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
                   d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red,   x->d_vxi, x->d_s,
                   x->d_step};
@@ -239,6 +240,10 @@ int setup_system(pimd_ctx *x) {
   c.nk = (int)x->Ak.size();
   if (c.nk == 0) return fail(PIMD_EINVAL, "no k-vectors inside the Ewald cut-off");
   c.T = (c.hmax + 1) + (c.kmax + 1) + (c.lmax + 1);
+  // the reciprocal-space kernels stage T table rows of up to 64 sites in LDS
+  if (c.T > 48)
+    return fail(PIMD_EINVAL, "%d Ewald table rows (hmax+kmax+lmax+3) exceed the limit of 48; "
+                "raise eps_ewald or the cut-off", c.T);
   c.a2 = c.aewald * c.aewald;
   c.cts1 = 2.0 * c.aewald / SQPI;
   c.cts2 = 4.0 * c.aewald * c.aewald * c.aewald / (3.0 * SQPI);
@@ -247,6 +252,7 @@ int setup_system(pimd_ctx *x) {
   c.fac_dd = 4.0 * (c.aewald * c.aewald * c.aewald) / (3.0 * std::sqrt(PI));
   c.maxiter = p.polar_maxiter;
   c.guess = p.guess_initdip;
+  c.mixed = p.precision == PIMD_MIXED ? 1 : 0;
   c.sor = p.polar_sor;
   c.peps = p.polar_eps;
   c.stath = DEBYE / CHARGECON / std::sqrt((double)(3 * c.N));
@@ -255,6 +261,13 @@ int setup_system(pimd_ctx *x) {
   x->p4vAk.resize(c.nk);
   for (int k = 0; k < c.nk; ++k) x->p4vAk[k] = FOURPI * x->Ak[k] / c.vol;
   return PIMD_OK;
+}
+
+int env_int(const char *name, int dflt, int lo, int hi) {
+  const char *v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  const int n = std::atoi(v);
+  return n < lo ? lo : n > hi ? hi : n;
 }
 
 int alloc_system(pimd_ctx *x) {
@@ -273,6 +286,7 @@ int alloc_system(pimd_ctx *x) {
   ALLOC(d.Efq, P * S * 3);
   ALLOC(d.dR, P * S * 3);
   ALLOC(d.dip, P * N * nd * 3);
+  ALLOC(d.dip_prev, P * N * nd * 3);
   ALLOC(d.Efd, P * N * nd * 3);
   ALLOC(d.diff2, P * N * nd);
   ALLOC(d.tally, P * N * T_COUNT);
@@ -353,9 +367,44 @@ int alloc_system(pimd_ctx *x) {
   c.ncol = (int)cols.size();
   c.kzmax = 0;
   for (auto &cl : cols) c.kzmax = std::max(c.kzmax, cl.w);
-  c.sf_chunks_max = 32;
+  // segments for the structure-factor kernel: the columns cut into pieces of
+  // at most segmax consecutive kz, longest first
+  std::vector<int4> segs;
+  std::vector<int> segk;
+  {
+    const int seg = env_int("PIMD_SFAC_SEG", 8, 1, 32);
+    for (size_t i = 0; i < cols.size(); ++i)
+      for (int o = 0; o < cols[i].w; o += seg) {
+        segs.push_back(make_int4(cols[i].x, cols[i].y, cols[i].z + o, std::min(seg, cols[i].w - o)));
+        segk.push_back(colk[i] + o);
+      }
+    std::vector<int> order(segs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b2) { return segs[a].w > segs[b2].w; });
+    std::vector<int4> ss(segs.size());
+    std::vector<int> sk(segk.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+      ss[i] = segs[order[i]];
+      sk[i] = segk[order[i]];
+    }
+    segs.swap(ss);
+    segk.swap(sk);
+  }
+  c.nseg = (int)segs.size();
+  c.segmax = 0;
+  for (auto &sg : segs) c.segmax = std::max(c.segmax, sg.w);
+  // site chunks of at most 64 sites keep the staged tables within LDS
+  c.sf_chunks_max = std::max(32, (3 * c.N + 63) / 64);
+  // launch-shape tuning (results do not depend on them)
+  c.sf_sites = env_int("PIMD_SFAC_SITES", 32, 1, 64);
+  c.fd_cg = env_int("PIMD_FIELD_CG", 0, 0, 16);
+  c.sf_kind = env_int("PIMD_SFAC_KIND", 1, 0, 1);
+  c.g_dd = env_int("PIMD_G_DD", 16, 0, 64);
+  c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
+  if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
   int rc2;
-  if ((rc2 = dalloc(&d.part, P * (size_t)c.sf_chunks_max * nk * 2)) != PIMD_OK) return rc2;
+  if ((rc2 = dalloc(&d.part, P * (size_t)c.sf_chunks_max * nk * 6)) != PIMD_OK) return rc2;
   if ((rc2 = dalloc(&d.cols, cols.size())) != PIMD_OK) return rc2;
   if ((rc2 = dalloc(&d.colk, colk.size())) != PIMD_OK) return rc2;
   HIPCHK(hipMemcpy(d.kv, kv.data(), sizeof(double4) * nk, hipMemcpyHostToDevice));
@@ -363,6 +412,10 @@ int alloc_system(pimd_ctx *x) {
   HIPCHK(hipMemcpy(d.red, x->Ak.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.cols, cols.data(), sizeof(int4) * cols.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.colk, colk.data(), sizeof(int) * colk.size(), hipMemcpyHostToDevice));
+  if ((rc2 = dalloc(&d.segs, segs.size())) != PIMD_OK) return rc2;
+  if ((rc2 = dalloc(&d.segk, segk.size())) != PIMD_OK) return rc2;
+  HIPCHK(hipMemcpy(d.segs, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.segk, segk.data(), sizeof(int) * segk.size(), hipMemcpyHostToDevice));
   std::vector<double> masses(na);
   for (size_t a = 0; a < na; ++a) masses[a] = (a % 3 == 0) ? x->p.mass_o : x->p.mass_h;
   HIPCHK(hipMemcpy(d.masses, masses.data(), sizeof(double) * na, hipMemcpyHostToDevice));
@@ -394,15 +447,28 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *
   x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
   x->timed("post_qq", [&] { launch_post_qq(c, d, t, s); });
   for (int it = 1; it <= c.maxiter; ++it) {
-    x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, s); });
+    x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s); });
     x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
     x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
     x->timed("converge", [&] { launch_converge(c, d, it, s); });
-    if (it < c.maxiter) {
+    // No host round trip: converged beads are switched off on the device
+    // and every kernel of the loop skips them, so the remaining iterations
+    // of a converged system are near-empty launches.
+    if (c.poll && it < c.maxiter) {
       HIPCHK(hipMemcpyAsync(x->h_nactive + it, d.nactive + it, sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (x->h_nactive[it] == 0) break;
     }
+  }
+  if (c.mixed) {
+    // Iterative refinement: redo each bead's last SOR step in FP64 from the
+    // dipoles it started from.  The reciprocal-space dipole structure factor
+    // of that step is still in place (beads stop updating it once converged).
+    x->timed("refine_dd", [&] {
+      launch_refine_prep(c, d, s);
+      launch_pair_dd(c, d, x->group, true, s);
+      launch_field(c, d, FD_D, c.maxiter + 1, s);
+    });
   }
   x->timed("pair_final", [&] { launch_pair_final(c, d, x->group, s); });
   x->timed("sfac_v", [&] { launch_sfac(c, d, SF_V, s); });
@@ -441,7 +507,8 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
     return fail(PIMD_EINVAL, "cut-off %.6f exceeds half the box (%.6f)", p.rc, lmin / 2.0);
   if (!(p.eps_ewald > 0.0 && p.eps_ewald < 1.0)) return fail(PIMD_EINVAL, "eps_ewald must be in (0,1)");
   if (p.polar_maxiter < 1) return fail(PIMD_EINVAL, "polar_maxiter must be >= 1");
-  if (p.precision != PIMD_FP64) return fail(PIMD_EINVAL, "only PIMD_FP64 is available");
+  if (p.precision != PIMD_FP64 && p.precision != PIMD_MIXED)
+    return fail(PIMD_EINVAL, "precision must be PIMD_FP64 (0) or PIMD_MIXED (1), got %d", p.precision);
   if (p.mass_o <= 0.0) p.mass_o = (double)f32(15.994);
   if (p.mass_h <= 0.0) p.mass_h = (double)f32(1.008);
   int ndev = 0;

@@ -35,9 +35,16 @@ struct SysConst {
   int nk, ncol, T;        // k-vectors, kz-columns, phase-table rows per site
   int hmax, kmax, lmax;
   int maxiter, guess;
+  int mixed;               // PIMD_MIXED: FP32 real-space pair arithmetic
   int nbr_cap;
   int kzmax;               // longest kz-column
+  int nseg, segmax;        // kz-columns cut into segments of at most segmax k
   int sf_chunks_max;       // site chunks of the column structure-factor kernel
+  int sf_sites;            // fewest sites per structure-factor chunk
+  int fd_cg;               // column groups per field block (0: by occupancy)
+  int sf_kind;             // structure-factor kernel: 0 one k per lane, 1 kz-columns
+  int g_dd;                // lanes per molecule of the LDS-staged dipole kernel (0: gather kernel)
+  int poll;                // 1: read the active-bead count back after every SOR iteration
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
@@ -66,6 +73,7 @@ struct DevBufs {
   double *Efq = nullptr;                        // [P][N][4][3]
   double *dR = nullptr;                         // [P][N][4][3]
   double *dip = nullptr;                        // [P][N][nd][3]
+  double *dip_prev = nullptr;                   // [P][N][nd][3] input of the last SOR step
   double *Efd = nullptr;                        // [P][N][nd][3]
   double *diff2 = nullptr;                      // [P][N*nd]
   double *tally = nullptr;                      // [P][N][T_COUNT]
@@ -83,6 +91,8 @@ struct DevBufs {
   int4 *khkl = nullptr;                         // [nk] h, k, l, 0
   int4 *cols = nullptr;                         // [ncol] h, k, l_first, count
   int *colk = nullptr;                          // [ncol] first k index
+  int4 *segs = nullptr;                         // [nseg] h, k, l_first, count
+  int *segk = nullptr;                          // [nseg] first k index
   // outputs of the last force call
   double *Upot = nullptr, *virt = nullptr, *virialc = nullptr, *terms = nullptr;  // [P],[P][9],[P],[P][5]
   double *dRR = nullptr, *dipI = nullptr, *dipE = nullptr, *chg = nullptr;
@@ -107,7 +117,8 @@ void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s);
 void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s);
 void launch_post_qq(const SysConst &c, DevBufs &b, int t, hipStream_t s);
-void launch_pair_dd(const SysConst &c, DevBufs &b, int group, hipStream_t s);
+void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s);
+void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s);
 void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s);
 void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s);

@@ -36,79 +36,104 @@ __device__ __forceinline__ double norm2_exact(double x, double y, double z) {
   return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
 }
 
+// Elementary functions of the compute type R.  FP64 is the reference's
+// arithmetic; FP32 (the mixed-precision pair terms) uses the hardware
+// square root, reciprocal and the ocml exp / log.
+template <typename R>
+struct RM;
+template <>
+struct RM<double> {
+  static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double exp_(double x) { return exp(x); }
+  static __device__ __forceinline__ double log_(double x) { return log(x); }
+};
+template <>
+struct RM<float> {
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float exp_(float x) { return expf(x); }
+  static __device__ __forceinline__ float log_(float x) { return logf(x); }
+};
+
 // Abramowitz-Stegun 7.1.26 complementary error function (math.f90:287-314)
 // with the exp(-x^2) factor supplied by the caller.
-__device__ __forceinline__ double erfc_as(double x, double expmx2) {
-  const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
-               A5 = 1.061405429, P = 0.3275911;
-  double T = 1.0 / (1.0 + P * x);
+template <typename R>
+__device__ __forceinline__ R erfc_as(R x, R expmx2) {
+  const R A1 = (R)0.254829592, A2 = (R)-0.284496736, A3 = (R)1.421413741, A4 = (R)-1.453152027,
+          A5 = (R)1.061405429, P = (R)0.3275911;
+  R T = RM<R>::rcp((R)1.0 + P * x);
   return T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5)))) * expmx2;
 }
 
 // Regularised upper incomplete gamma Q(2/3, x) with the reference's series /
 // continued-fraction split, tolerance and iteration caps (math.f90:188-285).
-__device__ inline double gammq23(double x) {
-  const double a = 2.0 / 3.0, gln = 0.303150275147547, EPS = 3e-7, FPMIN = 1e-30;
-  if (x < a + 1.0) {
-    if (x <= 0.0) return 1.0;
-    double ap = a, sum = 1.0 / a, del = sum;
+template <typename R>
+__device__ inline R gammq23(R x) {
+  const R a = (R)(2.0 / 3.0), gln = (R)0.303150275147547, EPS = (R)3e-7, FPMIN = (R)1e-30;
+  if (x < a + (R)1.0) {
+    if (x <= (R)0.0) return (R)1.0;
+    R ap = a, sum = RM<R>::rcp(a), del = sum;
     for (int n = 1; n <= 800; ++n) {
-      ap = ap + 1.0;
-      del = del * x / ap;
+      ap = ap + (R)1.0;
+      del = RM<R>::div(del * x, ap);
       sum = sum + del;
       if (fabs(del) < fabs(sum) * EPS) break;
     }
-    return 1.0 - sum * exp(-x + a * log(x) - gln);
+    return (R)1.0 - sum * RM<R>::exp_(-x + a * RM<R>::log_(x) - gln);
   }
-  double b = x + 1.0 - a, cc = 1.0 / FPMIN, d = 1.0 / b, h = d;
+  R b = x + (R)1.0 - a, cc = RM<R>::rcp(FPMIN), d = RM<R>::rcp(b), h = d;
   for (int i = 1; i <= 100; ++i) {
-    double an = -((double)i * ((double)i - a));
-    b = b + 2.0;
+    R an = -((R)i * ((R)i - a));
+    b = b + (R)2.0;
     d = an * d + b;
     if (fabs(d) < FPMIN) d = FPMIN;
-    cc = b + an / cc;
+    cc = b + RM<R>::div(an, cc);
     if (fabs(cc) < FPMIN) cc = FPMIN;
-    d = 1.0 / d;
-    double del = d * cc;
+    d = RM<R>::rcp(d);
+    R del = d * cc;
     h = h * del;
-    if (fabs(del - 1.0) < EPS) break;
+    if (fabs(del - (R)1.0) < EPS) break;
   }
-  return exp(-x + a * log(x) - gln) * h;
+  return RM<R>::exp_(-x + a * RM<R>::log_(x) - gln) * h;
 }
 
 // Screened Coulomb derivatives shared by PBCsmear01/2/3 (smear.f90:2-107):
 // ts_n = erfc-screened 1/r^(2n+1) terms plus the Thole correction for r < 5.
 // One exp(-alpha^2 r^2) serves both the explicit Gaussian and the erfc factor.
+template <typename R>
 struct Smear {
-  double dd, dri, drsqi, expon, er;
+  R dd, dri, drsqi, expon, er;
 };
 
-__device__ __forceinline__ Smear smear_base(const SysConst &c, double drsq) {
-  Smear s;
-  s.dd = sqrt(drsq);
-  s.dri = 1.0 / s.dd;
+template <typename R>
+__device__ __forceinline__ Smear<R> smear_base(const SysConst &c, R drsq) {
+  Smear<R> s;
+  s.dd = RM<R>::sqrt_(drsq);
+  s.dri = RM<R>::rcp(s.dd);
   s.drsqi = s.dri * s.dri;
-  s.expon = exp(-(c.a2 * drsq));
-  s.er = erfc_as(c.aewald * s.dd, s.expon);
+  s.expon = RM<R>::exp_(-((R)c.a2 * drsq));
+  s.er = erfc_as<R>((R)c.aewald * s.dd, s.expon);
   return s;
 }
 
 // PBCsmear01 (smear.f90:2-34): ts0, ts1 for the charge-charge terms.
 // THOLE = false is for pairs known to be beyond the 5 A damping radius, where
 // the r^2 < 25 branch cannot be taken.
-template <bool THOLE = true>
-__device__ __forceinline__ void smear01(const SysConst &c, double drsq, double apol, double ra,
-                                        double &ts0, double &ts1) {
-  Smear s = smear_base(c, drsq);
-  double t0 = s.er * s.dri;
-  double t1 = (t0 + c.cts1 * s.expon) * s.drsqi;
-  if (THOLE && drsq < 25.0) {
-    double rA3 = s.dd * drsq * apol;
-    double exp1 = exp(-rA3);
-    double ch1 = -1.0 * s.dri * s.drsqi;
-    double sr1 = (1.0 - exp1) * s.dri * s.drsqi;
+template <bool THOLE = true, typename R>
+__device__ __forceinline__ void smear01(const SysConst &c, R drsq, R apol, R ra, R &ts0, R &ts1) {
+  Smear<R> s = smear_base<R>(c, drsq);
+  R t0 = s.er * s.dri;
+  R t1 = (t0 + (R)c.cts1 * s.expon) * s.drsqi;
+  if (THOLE && drsq < (R)25.0) {
+    R rA3 = s.dd * drsq * apol;
+    R exp1 = RM<R>::exp_(-rA3);
+    R ch1 = (R)-1.0 * s.dri * s.drsqi;
+    R sr1 = ((R)1.0 - exp1) * s.dri * s.drsqi;
     t1 = t1 + ch1 + sr1;
-    t0 = t0 - exp1 * s.dri + ra * 1.3541179394264 * gammq23(rA3);
+    t0 = t0 - exp1 * s.dri + ra * (R)1.3541179394264 * gammq23<R>(rA3);
   }
   ts0 = t0;
   ts1 = t1;
@@ -116,18 +141,17 @@ __device__ __forceinline__ void smear01(const SysConst &c, double drsq, double a
 
 // PBCsmear2 (smear.f90:36-69): ts1, ts2 for the charge-dipole and
 // dipole-dipole field tensors.
-template <bool THOLE = true>
-__device__ __forceinline__ void smear2(const SysConst &c, double drsq, double apol, double &ts1,
-                                       double &ts2) {
-  Smear s = smear_base(c, drsq);
-  double t1 = (s.er * s.dri + c.cts1 * s.expon) * s.drsqi;
-  double t2 = (t1 + c.cts2 * s.expon) * s.drsqi;
-  if (THOLE && drsq < 25.0) {
-    double rA3 = (s.dd * drsq) * apol;
-    double exp1 = exp(-rA3);
-    double ch1 = -1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi;
-    double sr1 = (1.0 - exp1) * s.dri * s.drsqi;
-    double sr2 = (sr1 - exp1 * apol) * s.drsqi;
+template <bool THOLE = true, typename R>
+__device__ __forceinline__ void smear2(const SysConst &c, R drsq, R apol, R &ts1, R &ts2) {
+  Smear<R> s = smear_base<R>(c, drsq);
+  R t1 = (s.er * s.dri + (R)c.cts1 * s.expon) * s.drsqi;
+  R t2 = (t1 + (R)c.cts2 * s.expon) * s.drsqi;
+  if (THOLE && drsq < (R)25.0) {
+    R rA3 = (s.dd * drsq) * apol;
+    R exp1 = RM<R>::exp_(-rA3);
+    R ch1 = (R)-1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi;
+    R sr1 = ((R)1.0 - exp1) * s.dri * s.drsqi;
+    R sr2 = (sr1 - exp1 * apol) * s.drsqi;
     t1 = t1 + ch1 + sr1;
     t2 = t2 + ch2 + sr2;
   }
@@ -136,20 +160,19 @@ __device__ __forceinline__ void smear2(const SysConst &c, double drsq, double ap
 }
 
 // PBCsmear3 (smear.f90:71-107): ts1, ts2, ts3 for the dipole-dipole forces.
-template <bool THOLE = true>
-__device__ __forceinline__ void smear3(const SysConst &c, double drsq, double apol, double &ts1,
-                                       double &ts2, double &ts3) {
-  Smear s = smear_base(c, drsq);
-  double t1 = (s.er * s.dri + c.cts1 * s.expon) * s.drsqi;
-  double t2 = (t1 + c.cts2 * s.expon) * s.drsqi;
-  double t3 = (t2 + c.cts3 * s.expon) * s.drsqi;
-  if (THOLE && drsq < 25.0) {
-    double rA3 = (s.dd * drsq) * apol;
-    double exp1 = exp(-rA3);
-    double ch1 = -1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi, ch3 = ch2 * s.drsqi;
-    double sr1 = (1.0 - exp1) * s.dri * s.drsqi;
-    double sr2 = (sr1 - exp1 * apol) * s.drsqi;
-    double sr3 = (sr2 - 0.6 * exp1 * s.dd * apol * apol) * s.drsqi;
+template <bool THOLE = true, typename R>
+__device__ __forceinline__ void smear3(const SysConst &c, R drsq, R apol, R &ts1, R &ts2, R &ts3) {
+  Smear<R> s = smear_base<R>(c, drsq);
+  R t1 = (s.er * s.dri + (R)c.cts1 * s.expon) * s.drsqi;
+  R t2 = (t1 + (R)c.cts2 * s.expon) * s.drsqi;
+  R t3 = (t2 + (R)c.cts3 * s.expon) * s.drsqi;
+  if (THOLE && drsq < (R)25.0) {
+    R rA3 = (s.dd * drsq) * apol;
+    R exp1 = RM<R>::exp_(-rA3);
+    R ch1 = (R)-1.0 * s.dri * s.drsqi, ch2 = ch1 * s.drsqi, ch3 = ch2 * s.drsqi;
+    R sr1 = ((R)1.0 - exp1) * s.dri * s.drsqi;
+    R sr2 = (sr1 - exp1 * apol) * s.drsqi;
+    R sr3 = (sr2 - (R)0.6 * exp1 * s.dd * apol * apol) * s.drsqi;
     t1 = t1 + ch1 + sr1;
     t2 = t2 + ch2 + sr2;
     t3 = t3 + ch3 + sr3;
@@ -183,10 +206,10 @@ __device__ __forceinline__ void self2(const SysConst &c, double drsq, double &ts
 }
 
 // get_dd3 (pot_ttm.f90:633-647) applied to a vector: out = (3 ts2 r r^T - ts1) v.
-__device__ __forceinline__ void dd3_apply(const double r[3], double ts1, double ts2,
-                                          const double v[3], double out[3]) {
-  double t = 3.0 * ts2;
-  double rv = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+template <typename R>
+__device__ __forceinline__ void dd3_apply(const R r[3], R ts1, R ts2, const R v[3], R out[3]) {
+  R t = (R)3.0 * ts2;
+  R rv = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
   out[0] = t * r[0] * rv - ts1 * v[0];
   out[1] = t * r[1] * rv - ts1 * v[1];
   out[2] = t * r[2] * rv - ts1 * v[2];

@@ -269,7 +269,10 @@ void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Real-space vdW + charge-charge pass (pot_ttm.f90:159-228).
 // ---------------------------------------------------------------------------
-template <int MODEL, int G>
+// R is the type of the pair arithmetic: double for the FP64 form; float for
+// the mixed form, where site-site vectors and cut-off tests stay FP64 and each
+// pair's contributions are accumulated in FP64.
+template <int MODEL, int G, typename R>
 __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restrict__ site,
                                                 const double *__restrict__ Rcs,
                                                 const int *__restrict__ nbr,
@@ -302,17 +305,19 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
     double r[3] = {A[0].x - B[0].x, A[0].y - B[0].y, A[0].z - B[0].z};
     minimg(c, r[0], r[1], r[2]);
     const double r2 = norm2_exact(r[0], r[1], r[2]);
-    const double dRij = sqrt(r2);
-    const double R2i = 1.0 / r2, R4i = R2i * R2i, R6i = R2i * R2i * R2i, R8i = R6i * R2i;
-    const double ex = c.vdwD * exp(-c.vdwE * dRij);
-    const double uij = R6i * ((c.vdwA * R6i + c.vdwB * R4i) + c.vdwC) + ex;
-    const double xs = dRij - c.rc1;
-    const double fsc = -(R8i * 6.0 * c.vdwC + ex * c.vdwE / dRij + c.shiftA * (xs * xs) +
-                         c.shiftB * (xs * xs * xs));
-    const double d[3] = {fsc * r[0], fsc * r[1], fsc * r[2]};
+    const R r2r = (R)r2;
+    const R dRij = RM<R>::sqrt_(r2r);
+    const R R2i = RM<R>::rcp(r2r), R4i = R2i * R2i, R6i = R2i * R2i * R2i, R8i = R6i * R2i;
+    const R ex = (R)c.vdwD * RM<R>::exp_(-(R)c.vdwE * dRij);
+    const R uij = R6i * (((R)c.vdwA * R6i + (R)c.vdwB * R4i) + (R)c.vdwC) + ex;
+    const R xs = dRij - (R)c.rc1;
+    const R fsc = -(R8i * (R)6.0 * (R)c.vdwC + RM<R>::div(ex * (R)c.vdwE, dRij) +
+                    (R)c.shiftA * (xs * xs) + (R)c.shiftB * (xs * xs * xs));
+    const double d[3] = {(double)(fsc * (R)r[0]), (double)(fsc * (R)r[1]),
+                         (double)(fsc * (R)r[2])};
     for (int k = 0; k < 3; ++k) dro[k] += d[k];
     if (lo) {
-      uvdw += uij;
+      uvdw += (double)uij;
       double cx = RcA[0][0] - Rcs[(j * 4) * 3 + 0], cy = RcA[0][1] - Rcs[(j * 4) * 3 + 1],
              cz = RcA[0][2] - Rcs[(j * 4) * 3 + 2];
       minimg(c, cx, cy, cz);
@@ -332,16 +337,17 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
         double v[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
         minimg(c, v[0], v[1], v[2]);
         const double d2 = norm2_exact(v[0], v[1], v[2]);
-        double ts0, ts1;
-        smear01<THOLE>(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa],
-                       lo ? c.raCC[sa][sb] : c.raCC[sb][sa], ts0, ts1);
-        const double qb = B[sb].q;
-        if (sa > 0) ph[sa] += ts0 * qb;
-        for (int k = 0; k < 3; ++k) ef[sa][k] += ts1 * qb * v[k];
+        const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
+        R ts0, ts1;
+        smear01<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]),
+                       (R)(lo ? c.raCC[sa][sb] : c.raCC[sb][sa]), ts0, ts1);
+        const R qb = (R)B[sb].q;
+        if (sa > 0) ph[sa] += (double)(ts0 * qb);
+        for (int k = 0; k < 3; ++k) ef[sa][k] += (double)(ts1 * qb * vr[k]);
         if (lo && sa > 0) {
-          const double qa = A[sa].q;
+          const R qa = (R)A[sa].q;
           double dd[3];
-          for (int k = 0; k < 3; ++k) dd[k] = -(ts1 * qa * qb * v[k]);
+          for (int k = 0; k < 3; ++k) dd[k] = (double)(-(ts1 * qa * qb * vr[k]));
           double rcv[3];
           for (int k = 0; k < 3; ++k) rcv[k] = RcA[sa][k] - Rcs[(j * 4 + sb) * 3 + k] + s0[k];
           vc += rcv[0] * dd[0] + rcv[1] * dd[1] + rcv[2] * dd[2];
@@ -373,25 +379,31 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
   }
 }
 
-template <int G>
+template <int G, typename R>
 static void launch_qq_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   long threads = (long)c.P * c.N * G;
   dim3 grid((threads + 255) / 256);
   if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_qq<2, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr, b.nbrc,
-                       b.nbrn, b.phi, b.Efq, b.dR, b.tally);
+    hipLaunchKernelGGL((k_pair_qq<2, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr,
+                       b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
   else
-    hipLaunchKernelGGL((k_pair_qq<3, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr, b.nbrc,
-                       b.nbrn, b.phi, b.Efq, b.dR, b.tally);
+    hipLaunchKernelGGL((k_pair_qq<3, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr,
+                       b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
+}
+
+template <typename R>
+static void launch_qq_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  if (group == 64)
+    launch_qq_g<64, R>(c, b, s);
+  else if (group == 32)
+    launch_qq_g<32, R>(c, b, s);
+  else
+    launch_qq_g<16, R>(c, b, s);
 }
 
 void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
-  if (group == 64)
-    launch_qq_g<64>(c, b, s);
-  else if (group == 32)
-    launch_qq_g<32>(c, b, s);
-  else
-    launch_qq_g<16>(c, b, s);
+  if (c.mixed) launch_qq_r<float>(c, b, group, s);
+  else launch_qq_r<double>(c, b, group, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,143 +471,76 @@ __device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, doub
   sk = sxy * z.x + cxy * z.y;
 }
 
-// Structure factors over a site set, one k per lane; the block's four waves
-// split the sites and combine through LDS.
-//   SF_Q: (sum q cos, sum q sin)                      -> Sq  (ewald.f90:41-44)
-//   SF_D: (-sum (k.mu) sin, sum (k.mu) cos)           -> Sm  (ewald.f90:99-110)
-//   SF_V: (sum mu cos, sum mu sin) per component      -> Sv  (ewald.f90:208-213)
-constexpr int SF_TILE = 32;
-template <int MODE>
-__global__ __launch_bounds__(256) void k_sfac(SysConst c, const Site *__restrict__ site,
-                                             const double2 *__restrict__ tab,
-                                             const double *__restrict__ dip,
-                                             const double4 *__restrict__ kv,
-                                             const int4 *__restrict__ khkl,
-                                             const int *__restrict__ active, double *out) {
-  constexpr int NV = MODE == SF_V ? 6 : 2;
-  constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
-  extern __shared__ double2 lds_tab[];  // [T][SF_TILE]
-  __shared__ double lds_w[SF_TILE][3];
-  __shared__ double lds_red[3][64][NV];
-  const int b = blockIdx.y;
-  if (MODE == SF_D && !active[b]) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int k = blockIdx.x * 64 + lane;
-  const int kk = k < c.nk ? k : c.nk - 1;
-  const int4 h = khkl[kk];
-  const double4 kvec = kv[kk];
-  const int S = 4 * c.N;
-  const int rowy = c.hmax + 1 + abs(h.y), rowz = c.hmax + c.kmax + 2 + abs(h.z);
-  const double sy = h.y < 0 ? -1.0 : 1.0, sz = h.z < 0 ? -1.0 : 1.0;
-  double acc[NV];
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  const int M = set_size(c, SET);
-  const double2 *tb = tab + (size_t)b * c.T * S;
-  for (int m0 = 0; m0 < M; m0 += SF_TILE) {
-    const int nt = min(SF_TILE, M - m0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < c.T * SF_TILE; e += blockDim.x) {
-      const int row = e / SF_TILE, col = e % SF_TILE;
-      if (col < nt) lds_tab[e] = tb[(size_t)row * S + set_site(c, SET, m0 + col)];
-    }
-    for (int e = threadIdx.x; e < SF_TILE; e += blockDim.x) {
-      if (e < nt) {
-        const int m = m0 + e;
-        if (MODE == SF_Q) {
-          lds_w[e][0] = site[(size_t)b * S + set_site(c, SET, m)].q;
-        } else {
-          const double *mu = dip + ((size_t)b * c.N * c.nd + m) * 3;
-          lds_w[e][0] = mu[0];
-          lds_w[e][1] = mu[1];
-          lds_w[e][2] = mu[2];
-        }
-      }
-    }
-    __syncthreads();
-    for (int e = wv; e < nt; e += 4) {
-      const double2 x = lds_tab[h.x * SF_TILE + e];
-      double2 y = lds_tab[rowy * SF_TILE + e];
-      double2 z = lds_tab[rowz * SF_TILE + e];
-      y.y *= sy;
-      z.y *= sz;
-      double ck, sk;
-      phase_from(x, y, z, ck, sk);
-      if (MODE == SF_Q) {
-        const double q = lds_w[e][0];
-        acc[0] += q * ck;
-        acc[1] += q * sk;
-      } else if (MODE == SF_D) {
-        const double dk = (kvec.x * lds_w[e][0] + kvec.y * lds_w[e][1]) + kvec.z * lds_w[e][2];
-        acc[0] -= dk * sk;
-        acc[1] += dk * ck;
-      } else {
-        for (int cc = 0; cc < 3; ++cc) {
-          acc[cc] += lds_w[e][cc] * ck;
-          acc[3 + cc] += lds_w[e][cc] * sk;
-        }
-      }
-    }
-  }
-  if (wv > 0)
-    for (int v = 0; v < NV; ++v) lds_red[wv - 1][lane][v] = acc[v];
-  __syncthreads();
-  if (wv == 0 && k < c.nk) {
-    for (int v = 0; v < NV; ++v) {
-      double t = ((acc[v] + lds_red[0][lane][v]) + lds_red[1][lane][v]) + lds_red[2][lane][v];
-      out[((size_t)b * c.nk + k) * NV + v] = t;
-    }
-  }
-}
-
-// Structure factors with one kz-column (fixed kx, ky) per lane: the phase of
-// each site is formed once per column and advanced along kz by e^{i 2 pi z/L},
-// accumulating every kz of the column in registers.  Sites are split into
-// chunks across blocks; per-chunk partial sums are combined in a fixed order
-// by k_sfac_reduce.  SF_Q and SF_D as above.
+// Structure factors with one kz-segment (fixed kx, ky, up to KZ consecutive kz)
+// per lane: the phase of each site is formed once per segment and advanced
+// along kz by e^{i 2 pi z/L}, accumulating every kz of the segment in
+// registers.  A block takes a chunk of
+// sites of one bead; their phase tables and weights are staged in LDS (rows
+// padded so that lanes reading different rows of one site hit different
+// banks).  Per-chunk partial sums are combined in a fixed order by
+// k_sfac_reduce.  SF_Q and SF_D as above.
 template <int MODE, int KZ>
-__global__ __launch_bounds__(64) void k_sfac_col(SysConst c, const Site *__restrict__ site,
-                                                const double2 *__restrict__ tab,
-                                                const double *__restrict__ dip,
-                                                const int4 *__restrict__ cols,
-                                                const int *__restrict__ colk,
-                                                const int *__restrict__ active, double *part) {
+__global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__restrict__ site,
+                                                 const double2 *__restrict__ tab,
+                                                 const double *__restrict__ dip,
+                                                 const int4 *__restrict__ cols,
+                                                 const int *__restrict__ colk,
+                                                 const int *__restrict__ active,
+                                                 double *__restrict__ part, int sc, int scp) {
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
+  constexpr int NW = MODE == SF_Q ? 1 : 3;
+  extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
+  double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
   const int b = blockIdx.z;
   if (MODE == SF_D && !active[b]) return;
-  const int col = blockIdx.x * 64 + threadIdx.x;
-  const bool valid = col < c.ncol;
-  const int4 cl = cols[valid ? col : c.ncol - 1];
-  const int n = valid ? cl.w : 0;
-  const int k0 = colk[valid ? col : c.ncol - 1];
   const int S = 4 * c.N, M = set_size(c, SET);
   const int nch = gridDim.y, chunk = blockIdx.y;
-  const int m0 = (int)((long)M * chunk / nch), m1 = (int)((long)M * (chunk + 1) / nch);
+  const int m0 = chunk * sc, nt = min(sc, M - m0);
+  const double2 *tb = tab + (size_t)b * c.T * S;
+  for (int e = threadIdx.x; e < c.T * sc; e += blockDim.x) {
+    const int row = e / sc, l = e % sc;
+    if (l < nt) lds_sf[row * scp + l] = tb[(size_t)row * S + set_site(c, SET, m0 + l)];
+  }
+  for (int l = threadIdx.x; l < nt; l += blockDim.x) {
+    const int sidx = set_site(c, SET, m0 + l);
+    if (MODE == SF_Q) {
+      lw[l] = site[(size_t)b * S + sidx].q;
+    } else {
+      const double *mu = dip + ((size_t)b * c.N * c.nd + m0 + l) * 3;
+      lw[l * 3 + 0] = mu[0];
+      lw[l * 3 + 1] = mu[1];
+      lw[l * 3 + 2] = mu[2];
+    }
+  }
+  __syncthreads();
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= c.nseg) return;
+  const int4 cl = cols[col];
+  const int n = cl.w;
+  const int k0 = colk[col];
   const int rowx = cl.x, rowy = c.hmax + 1 + abs(cl.y), rowz = c.hmax + c.kmax + 2 + abs(cl.z);
   const int rowz1 = c.hmax + c.kmax + 3;
   const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
   const double kvx = TWOPI * c.boxi[0] * (double)cl.x, kvy = TWOPI * c.boxi[1] * (double)cl.y;
   const double tpz = TWOPI * c.boxi[2];
-  const double2 *tb = tab + (size_t)b * c.T * S;
   double are[KZ], aim[KZ];
 #pragma unroll
   for (int i = 0; i < KZ; ++i) are[i] = aim[i] = 0.0;
-  for (int m = m0; m < m1; ++m) {
-    const int s = set_site(c, SET, m);
-    const double2 x = tb[(size_t)rowx * S + s];
-    double2 y = tb[(size_t)rowy * S + s];
-    double2 z = tb[(size_t)rowz * S + s];
-    const double2 z1 = c.lmax >= 1 ? tb[(size_t)rowz1 * S + s] : make_double2(1.0, 0.0);
+  for (int e = 0; e < nt; ++e) {
+    const double2 x = lds_sf[rowx * scp + e];
+    double2 y = lds_sf[rowy * scp + e];
+    double2 z = lds_sf[rowz * scp + e];
+    const double2 z1 = c.lmax >= 1 ? lds_sf[rowz1 * scp + e] : make_double2(1.0, 0.0);
     y.y *= sgy;
     z.y *= sgz;
     double ck, sk;
     phase_from(x, y, z, ck, sk);
     double w = 0.0, wxy = 0.0, mz = 0.0;
     if (MODE == SF_Q) {
-      w = site[(size_t)b * S + s].q;
+      w = lw[e];
     } else {
-      const double *mu = dip + ((size_t)b * c.N * c.nd + m) * 3;
-      wxy = kvx * mu[0] + kvy * mu[1];
-      mz = mu[2];
+      wxy = kvx * lw[e * 3 + 0] + kvy * lw[e * 3 + 1];
+      mz = lw[e * 3 + 2];
     }
 #pragma unroll
     for (int i = 0; i < KZ; ++i) {
@@ -625,169 +570,303 @@ __global__ __launch_bounds__(64) void k_sfac_col(SysConst c, const Site *__restr
     }
 }
 
-__global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *__restrict__ part,
-                                                    int nch, const int *__restrict__ active,
-                                                    int check_active, double *out) {
-  const int b = blockIdx.y;
-  if (check_active && !active[b]) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, re/im)
-  if (e >= 2 * c.nk) return;
-  double v = 0.0;
-  for (int q = 0; q < nch; ++q) v += part[((size_t)b * nch + q) * c.nk * 2 + e];
-  out[(size_t)b * c.nk * 2 + e] = v;
+// Structure factors with one k-vector per lane.  A block holds 256 k of one
+// bead and a chunk of its sites; the sites' phase tables and weights are
+// staged in LDS (rows padded so that lanes reading different rows of one site
+// hit different banks; lanes on the same row read the same address).  Each
+// lane forms e^{i k.r} = X(kx) Y(ky) Z(kz) of every site from the tables and
+// accumulates its weighted sum; per-chunk partial sums are combined in a fixed
+// order by k_sfac_reduce.
+//   SF_Q: (sum q cos, sum q sin)                      (ewald.f90:41-44)
+//   SF_D: (-sum (k.mu) sin, sum (k.mu) cos)           (ewald.f90:99-110)
+//   SF_V: (sum mu cos, sum mu sin) per component      (ewald.f90:208-213)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sfac_k(SysConst c, const Site *__restrict__ site,
+                                               const double2 *__restrict__ tab,
+                                               const double *__restrict__ dip,
+                                               const double4 *__restrict__ kv,
+                                               const int4 *__restrict__ khkl,
+                                               const int *__restrict__ active,
+                                               double *__restrict__ part, int sc, int scp) {
+  constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
+  constexpr int NW = MODE == SF_Q ? 1 : 3;
+  constexpr int NV = MODE == SF_V ? 6 : 2;
+  extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
+  double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
+  const int b = blockIdx.z;
+  if (MODE == SF_D && !active[b]) return;
+  const int S = 4 * c.N, M = set_size(c, SET);
+  const int nch = gridDim.y, chunk = blockIdx.y;
+  const int m0 = chunk * sc, nt = min(sc, M - m0);
+  const double2 *tb = tab + (size_t)b * c.T * S;
+  for (int e = threadIdx.x; e < c.T * sc; e += blockDim.x) {
+    const int row = e / sc, l = e % sc;
+    if (l < nt) lds_sf[row * scp + l] = tb[(size_t)row * S + set_site(c, SET, m0 + l)];
+  }
+  for (int l = threadIdx.x; l < nt; l += blockDim.x) {
+    if (MODE == SF_Q) {
+      lw[l] = site[(size_t)b * S + set_site(c, SET, m0 + l)].q;
+    } else {
+      const double *mu = dip + ((size_t)b * c.N * c.nd + m0 + l) * 3;
+      lw[l * 3 + 0] = mu[0];
+      lw[l * 3 + 1] = mu[1];
+      lw[l * 3 + 2] = mu[2];
+    }
+  }
+  __syncthreads();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= c.nk) return;
+  const int4 h = khkl[k];
+  const double4 kvec = kv[k];
+  const int rowx = h.x * scp, rowy = (c.hmax + 1 + abs(h.y)) * scp;
+  const int rowz = (c.hmax + c.kmax + 2 + abs(h.z)) * scp;
+  const double sy = h.y < 0 ? -1.0 : 1.0, sz = h.z < 0 ? -1.0 : 1.0;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll 2
+  for (int e = 0; e < nt; ++e) {
+    const double2 x = lds_sf[rowx + e];
+    double2 y = lds_sf[rowy + e];
+    double2 z = lds_sf[rowz + e];
+    y.y *= sy;
+    z.y *= sz;
+    double ck, sk;
+    phase_from(x, y, z, ck, sk);
+    if (MODE == SF_Q) {
+      const double q = lw[e];
+      acc[0] += q * ck;
+      acc[1] += q * sk;
+    } else if (MODE == SF_D) {
+      const double dk = (kvec.x * lw[e * 3] + kvec.y * lw[e * 3 + 1]) + kvec.z * lw[e * 3 + 2];
+      acc[0] -= dk * sk;
+      acc[1] += dk * ck;
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        acc[cc] += lw[e * 3 + cc] * ck;
+        acc[3 + cc] += lw[e * 3 + cc] * sk;
+      }
+    }
+  }
+  double *o = part + (((size_t)b * nch + chunk) * c.nk + k) * NV;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) o[v] = acc[v];
 }
 
+__global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *__restrict__ part,
+                                                    int nch, int nv,
+                                                    const int *__restrict__ active,
+                                                    int check_active, double *__restrict__ out) {
+  const int b = blockIdx.y;
+  if (check_active && !active[b]) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, component)
+  if (e >= nv * c.nk) return;
+  double v = 0.0;
+  for (int q = 0; q < nch; ++q) v += part[((size_t)b * nch + q) * c.nk * nv + e];
+  out[(size_t)b * c.nk * nv + e] = v;
+}
+
+// Site chunk of the column structure-factor kernel and its padded LDS row
+// (a multiple of 16 plus one double2, so rows start 4 banks apart).
+static void sfac_chunking(const SysConst &c, int M, int &nch, int &sc, int &scp) {
+  nch = std::max(1, std::min(c.sf_chunks_max, (M + c.sf_sites - 1) / c.sf_sites));
+  sc = (M + nch - 1) / nch;
+  nch = (M + sc - 1) / sc;
+  scp = ((sc + 15) / 16) * 16 + 1;
+}
+
+static size_t sfac_lds(const SysConst &c, int mode, int sc, int scp) {
+  return sizeof(double2) * c.T * scp + sizeof(double) * sc * (mode == SF_Q ? 1 : 3);
+}
+
+
 template <int MODE, int KZ>
-static void launch_sfac_col(const SysConst &c, DevBufs &b, int nch, hipStream_t s) {
-  dim3 grid((c.ncol + 63) / 64, nch, c.P);
-  hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(64), 0, s, c, b.site, b.tab, b.dip,
-                     b.cols, b.colk, b.active, b.part);
+static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
+  int nch, sc, scp;
+  sfac_chunking(c, M, nch, sc, scp);
+  const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
+  dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
+  hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), sfac_lds(c, MODE, sc, scp), s, c,
+                     b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
   dim3 g2((2 * c.nk + 255) / 256, c.P);
-  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, b.active,
+  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, 2, b.active,
                      MODE == SF_D ? 1 : 0, MODE == SF_Q ? b.Sq : b.Sm);
 }
 
 template <int MODE>
-static bool sfac_col_dispatch(const SysConst &c, DevBufs &b, int nch, hipStream_t s) {
-  if (c.kzmax <= 8) launch_sfac_col<MODE, 8>(c, b, nch, s);
-  else if (c.kzmax <= 16) launch_sfac_col<MODE, 16>(c, b, nch, s);
-  else if (c.kzmax <= 24) launch_sfac_col<MODE, 24>(c, b, nch, s);
-  else if (c.kzmax <= 32) launch_sfac_col<MODE, 32>(c, b, nch, s);
+static bool sfac_col_dispatch(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
+  int nch, sc, scp;
+  sfac_chunking(c, M, nch, sc, scp);
+  if (sfac_lds(c, MODE, sc, scp) > 64 * 1024) return false;
+  if (c.segmax <= 4) launch_sfac_col<MODE, 4>(c, b, M, s);
+  else if (c.segmax <= 8) launch_sfac_col<MODE, 8>(c, b, M, s);
+  else if (c.segmax <= 12) launch_sfac_col<MODE, 12>(c, b, M, s);
+  else if (c.segmax <= 16) launch_sfac_col<MODE, 16>(c, b, M, s);
+  else if (c.segmax <= 24) launch_sfac_col<MODE, 24>(c, b, M, s);
+  else if (c.segmax <= 32) launch_sfac_col<MODE, 32>(c, b, M, s);
   else return false;
   return true;
 }
 
-void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
-  if (mode == SF_Q || mode == SF_D) {
-    const int M = mode == SF_Q ? 3 * c.N : c.nd * c.N;
-    const int colb = (c.ncol + 63) / 64;
-    int nch = (4096 + colb * c.P - 1) / (colb * c.P);
-    nch = std::max(1, std::min(nch, std::min(c.sf_chunks_max, M / 8 > 0 ? M / 8 : 1)));
-    const bool ok = mode == SF_Q ? sfac_col_dispatch<SF_Q>(c, b, nch, s)
-                                 : sfac_col_dispatch<SF_D>(c, b, nch, s);
-    if (ok) return;
-  }
-  dim3 grid((c.nk + 63) / 64, c.P);
-  size_t sh = sizeof(double2) * c.T * SF_TILE;
-  if (mode == SF_Q)
-    hipLaunchKernelGGL(k_sfac<SF_Q>, grid, dim3(256), sh, s, c, b.site, b.tab, b.dip, b.kv,
-                       b.khkl, b.active, b.Sq);
-  else if (mode == SF_D)
-    hipLaunchKernelGGL(k_sfac<SF_D>, grid, dim3(256), sh, s, c, b.site, b.tab, b.dip, b.kv,
-                       b.khkl, b.active, b.Sm);
-  else
-    hipLaunchKernelGGL(k_sfac<SF_V>, grid, dim3(256), sh, s, c, b.site, b.tab, b.dip, b.kv,
-                       b.khkl, b.active, b.Sv);
+template <int MODE>
+static void launch_sfac_k(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
+  constexpr int NV = MODE == SF_V ? 6 : 2;
+  int nch, sc, scp;
+  sfac_chunking(c, M, nch, sc, scp);
+  dim3 grid((c.nk + 255) / 256, nch, c.P);
+  hipLaunchKernelGGL((k_sfac_k<MODE>), grid, dim3(256), sfac_lds(c, MODE, sc, scp), s, c, b.site,
+                     b.tab, b.dip, b.kv, b.khkl, b.active, b.part, sc, scp);
+  dim3 g2((NV * c.nk + 255) / 256, c.P);
+  double *out = MODE == SF_Q ? b.Sq : MODE == SF_D ? b.Sm : b.Sv;
+  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, NV, b.active,
+                     MODE == SF_D ? 1 : 0, out);
 }
 
-// Reciprocal-space fields at each site, walking the k list column by column
-// (fixed kx, ky; consecutive kz) with the phase advanced by e^{i 2 pi z/L}.
+void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
+  const int M = mode == SF_Q ? 3 * c.N : c.nd * c.N;
+  if (c.sf_kind == 1 && mode != SF_V) {
+    const bool ok = mode == SF_Q ? sfac_col_dispatch<SF_Q>(c, b, M, s)
+                                 : sfac_col_dispatch<SF_D>(c, b, M, s);
+    if (ok) return;
+  }
+  if (mode == SF_Q) launch_sfac_k<SF_Q>(c, b, M, s);
+  else if (mode == SF_D) launch_sfac_k<SF_D>(c, b, M, s);
+  else launch_sfac_k<SF_V>(c, b, M, s);
+}
+
+// Reciprocal-space fields at each site.  A block holds 64 sites of one bead,
+// one per lane, and CG waves; wave g walks kz-columns g, g + CG, ... (columns
+// are sorted by length, so the waves get similar trip counts).  All lanes of a
+// wave are on the same k at the same time, so the k data (k-vector, 4 pi A_k/V,
+// structure factor) is wave-uniform and read with scalar loads; each lane's
+// phase e^{i k.r} comes from its site's tables, staged in LDS, and is advanced
+// along the column by e^{i 2 pi z/L}.  Along a column kx and ky are fixed, so
+// the x and y field components are kx and ky times one column sum.  The CG
+// partial fields of a site are combined in a fixed order through LDS.
 //   FD_Q : Efq and phi from the charge structure factor (ewald.f90:45-49)
 //   FD_D : Efd from the dipole structure factor (ewald.f90:114-117), then the
 //          SOR update of the induced dipole (pot_ttm.f90:371-385)
 //   FD_QD: phi and dR from the stale dipole and the total structure factors
 //          (ewald.f90:184-193)
-constexpr int FD_TS = 16;              // sites per block
-constexpr int FD_G = 256 / FD_TS;      // column groups per block
-template <int MODE, int MODEL>
-__global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restrict__ site,
-                                              const double2 *__restrict__ tab,
-                                              const double4 *__restrict__ kv,
-                                              const int4 *__restrict__ cols,
-                                              const int *__restrict__ colk,
-                                              const double *__restrict__ Sq,
-                                              const double *__restrict__ Sm, double *phi,
-                                              double *Efq, double *dR, double *dip,
-                                              const double *__restrict__ Efd, double *diff2,
-                                              const int *__restrict__ active, int iter) {
+constexpr int FD_SB = 64;  // sites per block, one per lane
+template <int MODE, int MODEL, int CG>
+__global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__restrict__ site,
+                                                   const double2 *__restrict__ tab,
+                                                   const double4 *__restrict__ kv,
+                                                   const int4 *__restrict__ cols,
+                                                   const int *__restrict__ colk,
+                                                   const double *__restrict__ Sq,
+                                                   const double *__restrict__ Sm,
+                                                   double *__restrict__ phi,
+                                                   double *__restrict__ Efq,
+                                                   double *__restrict__ dR,
+                                                   double *__restrict__ dip,
+                                                   double *__restrict__ dip_prev,
+                                                   const double *__restrict__ Efd,
+                                                   double *__restrict__ diff2,
+                                                   const int *__restrict__ active, int iter) {
   constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
-  __shared__ double red[4][FD_G][FD_TS + 1];
+  // [T][64] phase tables of the block's sites, later reused for the reduction
+  extern __shared__ double2 lds_tab[];
   const int b = blockIdx.y;
   if (MODE == FD_D && !active[b]) return;
-  const int sl = threadIdx.x % FD_TS, g = threadIdx.x / FD_TS;
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int M = set_size(c, SET);
-  const int m = blockIdx.x * FD_TS + sl;
-  const bool valid = m < M;
   const int S = 4 * c.N;
+  const int mbase = blockIdx.x * FD_SB;
+  const double2 *tb = tab + (size_t)b * c.T * S;
+  for (int e = threadIdx.x; e < c.T * FD_SB; e += 64 * CG) {
+    const int row = e >> 6, l = e & 63;
+    lds_tab[e] = tb[(size_t)row * S + set_site(c, SET, min(mbase + l, M - 1))];
+  }
+  __syncthreads();
+  const int m = mbase + lane;
+  const bool valid = m < M;
   const int s = set_site(c, SET, valid ? m : M - 1);
   const int type = s & 3;
-  const Site p = site[(size_t)b * S + s];
-  const double2 *tb = tab + (size_t)b * c.T * S + s;
-  // e^{i 2 pi z/L}; with lmax = 0 every column holds one k and no step is taken
-  const double2 zstep = c.lmax >= 1 ? tb[(size_t)(c.hmax + c.kmax + 3) * S] : make_double2(1.0, 0.0);
   const bool has_q = type != 0;
   const bool has_d = (type >= c.tdip && type < c.tdip + c.nd);
-  double mu[3] = {0, 0, 0};
   const size_t dslot = ((size_t)b * c.N + (s >> 2)) * c.nd + (type - c.tdip);
+  double mu[3] = {0, 0, 0};
   if ((MODE == FD_D || MODE == FD_QD) && has_d)
     for (int k = 0; k < 3; ++k) mu[k] = dip[dslot * 3 + k];
+  const double q = (MODE == FD_QD) ? site[(size_t)b * S + s].q : 0.0;
+  // e^{i 2 pi z/L}; with lmax = 0 every column holds one k and no step is taken
+  const double2 zstep =
+      c.lmax >= 1 ? lds_tab[(c.hmax + c.kmax + 3) * FD_SB + lane] : make_double2(1.0, 0.0);
   const double *sq = Sq + (size_t)b * c.nk * 2, *sm = Sm + (size_t)b * c.nk * 2;
   double e0 = 0, e1 = 0, e2 = 0, ph = 0;
-  // columns are sorted by length; group g takes g, g + G, ... for balance
-  for (int col = g; col < c.ncol; col += FD_G) {
+  for (int col = g; col < c.ncol; col += CG) {
     const int4 cl = cols[col];
     const int k0 = colk[col];
-    double2 x = tb[(size_t)cl.x * S];
-    double2 y = tb[(size_t)(c.hmax + 1 + abs(cl.y)) * S];
-    double2 z = tb[(size_t)(c.hmax + c.kmax + 2 + abs(cl.z)) * S];
+    double2 x = lds_tab[cl.x * FD_SB + lane];
+    double2 y = lds_tab[(c.hmax + 1 + abs(cl.y)) * FD_SB + lane];
+    double2 z = lds_tab[(c.hmax + c.kmax + 2 + abs(cl.z)) * FD_SB + lane];
     if (cl.y < 0) y.y = -y.y;
     if (cl.z < 0) z.y = -z.y;
     double ck, sk;
     phase_from(x, y, z, ck, sk);
+    const double kx = kv[k0].x, ky = kv[k0].y;
+    // column sums of t and kz t (t carries the sign of each mode's field)
+    double at = 0.0, az = 0.0;
+    // FD_QD dipole sites: k.mu = (kx mux + ky muy) + kz muz
+    const double dxy = (MODE == FD_QD) ? kx * mu[0] + ky * mu[1] : 0.0;
+#pragma unroll 4
     for (int i = 0; i < cl.w; ++i) {
-      if (i > 0) {
-        const double nc = ck * zstep.x - sk * zstep.y;
-        sk = sk * zstep.x + ck * zstep.y;
-        ck = nc;
-      }
       const int k = k0 + i;
       const double4 kk = kv[k];  // kx, ky, kz, 4 pi A_k / V
       if (MODE == FD_Q) {
         const double re = sq[2 * k], im = sq[2 * k + 1];
         const double t = kk.w * (ck * im - sk * re);
-        e0 -= t * kk.x;
-        e1 -= t * kk.y;
-        e2 -= t * kk.z;
+        at += t;
+        az += kk.z * t;
         ph += kk.w * (ck * re + sk * im);
       } else if (MODE == FD_D) {
         const double re = sm[2 * k], im = sm[2 * k + 1];
         const double t = kk.w * (ck * im - sk * re);
-        e0 -= kk.x * t;
-        e1 -= kk.y * t;
-        e2 -= kk.z * t;
+        at += t;
+        az += kk.z * t;
       } else {
         const double rqm = sm[2 * k], iqm = sm[2 * k + 1];
+        double t = 0.0;
         if (has_q) {
-          const double t = p.q * kk.w * (ck * iqm - sk * rqm);
+          t = -(q * kk.w * (ck * iqm - sk * rqm));
           ph += kk.w * (ck * rqm + sk * iqm);
-          e0 += t * kk.x;
-          e1 += t * kk.y;
-          e2 += t * kk.z;
         }
         if (has_d) {
           const double rqqm = sq[2 * k] + rqm;
           const double iqqm = sq[2 * k + 1] + iqm;
-          const double dk = (kk.x * mu[0] + kk.y * mu[1]) + kk.z * mu[2];
-          const double t = dk * kk.w * (ck * rqqm + sk * iqqm);
-          e0 -= t * kk.x;
-          e1 -= t * kk.y;
-          e2 -= t * kk.z;
+          const double dk = dxy + kk.z * mu[2];
+          t += dk * kk.w * (ck * rqqm + sk * iqqm);
         }
+        at += t;
+        az += kk.z * t;
       }
+      // advance the phase to the next kz of the column
+      const double nc = ck * zstep.x - sk * zstep.y;
+      sk = sk * zstep.x + ck * zstep.y;
+      ck = nc;
     }
+    e0 -= kx * at;
+    e1 -= ky * at;
+    e2 -= az;
   }
-  // combine the column groups in a fixed order
-  red[0][g][sl] = e0;
-  red[1][g][sl] = e1;
-  red[2][g][sl] = e2;
-  red[3][g][sl] = ph;
+  // combine the CG column groups of each site in a fixed order
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(lds_tab);  // [CG][4][64]
+  red[(g * 4 + 0) * FD_SB + lane] = e0;
+  red[(g * 4 + 1) * FD_SB + lane] = e1;
+  red[(g * 4 + 2) * FD_SB + lane] = e2;
+  red[(g * 4 + 3) * FD_SB + lane] = ph;
   __syncthreads();
   if (g != 0 || !valid) return;
-  for (int q = 1; q < FD_G; ++q) {
-    e0 += red[0][q][sl];
-    e1 += red[1][q][sl];
-    e2 += red[2][q][sl];
-    ph += red[3][q][sl];
+  for (int w = 1; w < CG; ++w) {
+    e0 += red[(w * 4 + 0) * FD_SB + lane];
+    e1 += red[(w * 4 + 1) * FD_SB + lane];
+    e2 += red[(w * 4 + 2) * FD_SB + lane];
+    ph += red[(w * 4 + 3) * FD_SB + lane];
   }
   const size_t ss = (size_t)b * S + s;
   if (MODE == FD_Q) {
@@ -810,27 +889,47 @@ __global__ __launch_bounds__(256) void k_field(SysConst c, const Site *__restric
       const double df = nm - (iter == 1 ? 0.0 : mu[k]);
       d2 += df * df;
       dip[dslot * 3 + k] = nm;
+      if (c.mixed && iter <= c.maxiter) dip_prev[dslot * 3 + k] = mu[k];
     }
     diff2[(size_t)b * c.N * c.nd + m] = d2;
   }
 }
 
+// Shared memory of k_field: the phase tables, reused for the CG-way reduction.
+static size_t field_lds(const SysConst &c, int cg) {
+  return std::max(sizeof(double2) * c.T * FD_SB, sizeof(double) * cg * 4 * FD_SB);
+}
+
+template <int MO, int ML, int CG>
+static void launch_field_cg(const SysConst &c, DevBufs &b, int iter, dim3 grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_field<MO, ML, CG>), grid, dim3(64 * CG), field_lds(c, CG), s, c, b.site,
+                     b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR, b.dip,
+                     b.dip_prev, b.Efd, b.diff2, b.active, iter);
+}
+
+template <int MO, int ML>
+static void launch_field_m(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
+  const int M = MO == FD_D ? c.nd * c.N : (ML == 2 ? 4 * c.N : 3 * c.N);
+  dim3 grid((M + FD_SB - 1) / FD_SB, c.P);
+  // enough waves to fill the device: about 16 per CU
+  const long blocks = (long)grid.x * grid.y;
+  int cg = c.fd_cg;
+  if (cg == 0) cg = (blocks * 4 >= 4096 || c.ncol < 8) ? 4 : (blocks * 8 >= 4096 || c.ncol < 16) ? 8 : 16;
+  if (cg == 4) launch_field_cg<MO, ML, 4>(c, b, iter, grid, s);
+  else if (cg == 8) launch_field_cg<MO, ML, 8>(c, b, iter, grid, s);
+  else launch_field_cg<MO, ML, 16>(c, b, iter, grid, s);
+}
+
 void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s) {
-  const int M = mode == FD_D ? c.nd * c.N : (c.model == 2 ? 4 * c.N : 3 * c.N);
-  dim3 grid((M + FD_TS - 1) / FD_TS, c.P), blk(256);
-#define PIMD_FIELD(MO, ML)                                                                       \
-  hipLaunchKernelGGL((k_field<MO, ML>), grid, blk, 0, s, c, b.site, b.tab, b.kv, b.cols, b.colk, \
-                     b.Sq, b.Sm, b.phi, b.Efq, b.dR, b.dip, b.Efd, b.diff2, b.active, iter)
   if (c.model == 2) {
-    if (mode == FD_Q) PIMD_FIELD(FD_Q, 2);
-    else if (mode == FD_D) PIMD_FIELD(FD_D, 2);
-    else PIMD_FIELD(FD_QD, 2);
+    if (mode == FD_Q) launch_field_m<FD_Q, 2>(c, b, iter, s);
+    else if (mode == FD_D) launch_field_m<FD_D, 2>(c, b, iter, s);
+    else launch_field_m<FD_QD, 2>(c, b, iter, s);
   } else {
-    if (mode == FD_Q) PIMD_FIELD(FD_Q, 3);
-    else if (mode == FD_D) PIMD_FIELD(FD_D, 3);
-    else PIMD_FIELD(FD_QD, 3);
+    if (mode == FD_Q) launch_field_m<FD_Q, 3>(c, b, iter, s);
+    else if (mode == FD_D) launch_field_m<FD_D, 3>(c, b, iter, s);
+    else launch_field_m<FD_QD, 3>(c, b, iter, s);
   }
-#undef PIMD_FIELD
 }
 
 // ---------------------------------------------------------------------------
@@ -893,7 +992,7 @@ void launch_post_qq(const SysConst &c, DevBufs &b, int t, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Real-space dipole field T.mu of one SOR iteration (pot_ttm.f90:311-365).
 // ---------------------------------------------------------------------------
-template <int MODEL, int G>
+template <int MODEL, int G, typename R>
 __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restrict__ site,
                                                 const double *__restrict__ dip,
                                                 const int *__restrict__ nbr,
@@ -931,11 +1030,13 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
         double v[3] = {A[sa].x - B.x, A[sa].y - B.y, A[sa].z - B.z};
         minimg(c, v[0], v[1], v[2]);
         const double d2 = norm2_exact(v[0], v[1], v[2]);
-        double ts1, ts2;
-        smear2<THOLE>(c, d2, lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa], ts1, ts2);
-        double o[3];
-        dd3_apply(v, ts1, ts2, mub[ub], o);
-        for (int k = 0; k < 3; ++k) ef[ua][k] += o[k];
+        R ts1, ts2;
+        smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa]), ts1, ts2);
+        const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
+        const R mr[3] = {(R)mub[ub][0], (R)mub[ub][1], (R)mub[ub][2]};
+        R o[3];
+        dd3_apply(vr, ts1, ts2, mr, o);
+        for (int k = 0; k < 3; ++k) ef[ua][k] += (double)o[k];
       }
     }
   };
@@ -967,25 +1068,168 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
   }
 }
 
-template <int G>
+// Real-space dipole field with the bead's dipole sites staged in LDS: a block
+// of up to 1024 threads takes 1024/G molecules of one bead and first copies
+// every dipole site of that bead (position and dipole, 48 bytes) into LDS, so
+// the neighbour loop reads LDS instead of gathering scattered records from
+// L2.  Arithmetic and summation order are those of k_pair_dd.
+constexpr int DD_BLOCK = 1024;
+template <int MODEL, int G, typename R>
+__global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site *__restrict__ site,
+                                                         const double *__restrict__ dip,
+                                                         const int *__restrict__ nbr,
+                                                         const int *__restrict__ nbrc,
+                                                         const int *__restrict__ nbrn,
+                                                         const int *__restrict__ active,
+                                                         double *__restrict__ Efd) {
+  constexpr int ND = MODEL == 2 ? 3 : 1;
+  constexpr int T0 = MODEL == 2 ? 0 : 3;
+  extern __shared__ double lds_dd[];  // [N][ND][6]: x, y, z, mu_x, mu_y, mu_z
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const int N = c.N;
+  for (int e = threadIdx.x; e < N * ND; e += blockDim.x) {
+    const int j = e / ND, u = e % ND;
+    const Site sj = site[((size_t)b * N + j) * 4 + T0 + u];
+    const double *mu = dip + (((size_t)b * N + j) * ND + u) * 3;
+    double *o = lds_dd + (size_t)e * 6;
+    o[0] = sj.x;
+    o[1] = sj.y;
+    o[2] = sj.z;
+    o[3] = mu[0];
+    o[4] = mu[1];
+    o[5] = mu[2];
+  }
+  __syncthreads();
+  const int a = blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (a >= N) return;
+  const long grp = (long)b * N + a;
+  double A[ND][3];
+  for (int u = 0; u < ND; ++u)
+    for (int k = 0; k < 3; ++k) A[u][k] = lds_dd[((size_t)a * ND + u) * 6 + k];
+  double ef[ND][3];
+  for (int u = 0; u < ND; ++u) ef[u][0] = ef[u][1] = ef[u][2] = 0.0;
+  const int nf = nbrc[grp], nn = nbrn[grp];
+  const int *list = nbr + (size_t)grp * c.nbr_cap;
+  auto visit = [&](const int j, auto thole_tag) {
+    constexpr bool THOLE = decltype(thole_tag)::value;
+    const bool lo = a < j;
+#pragma unroll
+    for (int ua = 0; ua < ND; ++ua) {
+#pragma unroll
+      for (int ub = 0; ub < ND; ++ub) {
+        const int sa = T0 + ua, sb = T0 + ub;
+        const double *B = lds_dd + ((size_t)j * ND + ub) * 6;
+        double v[3] = {A[ua][0] - B[0], A[ua][1] - B[1], A[ua][2] - B[2]};
+        minimg(c, v[0], v[1], v[2]);
+        const double d2 = norm2_exact(v[0], v[1], v[2]);
+        R ts1, ts2;
+        smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa]), ts1, ts2);
+        const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
+        const R mr[3] = {(R)B[3], (R)B[4], (R)B[5]};
+        R o[3];
+        dd3_apply(vr, ts1, ts2, mr, o);
+        for (int k = 0; k < 3; ++k) ef[ua][k] += (double)o[k];
+      }
+    }
+  };
+  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
+  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
+  for (int u = 0; u < ND; ++u)
+    for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
+  if (lane == 0) {
+    if (MODEL == 2) {  // intramolecular dipole pairs of TTM2.1-F (pot_ttm.f90:344-365)
+      for (int u = 0; u < ND - 1; ++u)
+        for (int v2 = u + 1; v2 < ND; ++v2) {
+          const int sa = T0 + u, sb = T0 + v2;
+          double r[3] = {A[u][0] - A[v2][0], A[u][1] - A[v2][1], A[u][2] - A[v2][2]};
+          minimg(c, r[0], r[1], r[2]);
+          const double d2 = norm2_exact(r[0], r[1], r[2]);
+          double ts1, ts2, o[3];
+          smear2(c, d2, c.apolDD[sa][sb], ts1, ts2);
+          const double *mu_v = lds_dd + ((size_t)a * ND + v2) * 6 + 3;
+          const double *mu_u = lds_dd + ((size_t)a * ND + u) * 6 + 3;
+          dd3_apply(r, ts1, ts2, mu_v, o);
+          for (int k = 0; k < 3; ++k) ef[u][k] += o[k];
+          dd3_apply(r, ts1, ts2, mu_u, o);
+          for (int k = 0; k < 3; ++k) ef[v2][k] += o[k];
+        }
+    }
+    for (int u = 0; u < ND; ++u)
+      for (int k = 0; k < 3; ++k) Efd[(grp * ND + u) * 3 + k] = ef[u][k];
+  }
+}
+
+static size_t dd_lds(const SysConst &c) { return sizeof(double) * 6 * c.N * c.nd; }
+
+template <int G, typename R>
+static bool launch_dd_lds_g(const SysConst &c, DevBufs &b, hipStream_t s) {
+  if (dd_lds(c) > 64 * 1024) return false;
+  dim3 grid((c.N + DD_BLOCK / G - 1) / (DD_BLOCK / G), c.P);
+  if (c.model == 2)
+    hipLaunchKernelGGL((k_pair_dd_lds<2, G, R>), grid, dim3(DD_BLOCK), dd_lds(c), s, c, b.site,
+                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd);
+  else
+    hipLaunchKernelGGL((k_pair_dd_lds<3, G, R>), grid, dim3(DD_BLOCK), dd_lds(c), s, c, b.site,
+                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd);
+  return true;
+}
+
+template <typename R>
+static bool launch_dd_lds_r(const SysConst &c, DevBufs &b, hipStream_t s) {
+  const int g = c.g_dd;
+  if (g == 64) return launch_dd_lds_g<64, R>(c, b, s);
+  if (g == 32) return launch_dd_lds_g<32, R>(c, b, s);
+  if (g == 8) return launch_dd_lds_g<8, R>(c, b, s);
+  return launch_dd_lds_g<16, R>(c, b, s);
+}
+
+template <int G, typename R>
 static void launch_dd_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   long threads = (long)c.P * c.N * G;
   dim3 grid((threads + 255) / 256);
   if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_dd<2, G>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr, b.nbrc,
-                       b.nbrn, b.active, b.Efd);
+    hipLaunchKernelGGL((k_pair_dd<2, G, R>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr,
+                       b.nbrc, b.nbrn, b.active, b.Efd);
   else
-    hipLaunchKernelGGL((k_pair_dd<3, G>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr, b.nbrc,
-                       b.nbrn, b.active, b.Efd);
+    hipLaunchKernelGGL((k_pair_dd<3, G, R>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr,
+                       b.nbrc, b.nbrn, b.active, b.Efd);
 }
 
-void launch_pair_dd(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+template <typename R>
+static void launch_dd_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
   if (group == 64)
-    launch_dd_g<64>(c, b, s);
+    launch_dd_g<64, R>(c, b, s);
   else if (group == 32)
-    launch_dd_g<32>(c, b, s);
+    launch_dd_g<32, R>(c, b, s);
   else
-    launch_dd_g<16>(c, b, s);
+    launch_dd_g<16, R>(c, b, s);
+}
+
+// fp64 forces the FP64 form (the refinement pass of the mixed solve).
+void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s) {
+  const bool f32 = c.mixed && !fp64;
+  if (c.g_dd > 0 && (f32 ? launch_dd_lds_r<float>(c, b, s) : launch_dd_lds_r<double>(c, b, s)))
+    return;
+  if (f32) launch_dd_r<float>(c, b, group, s);
+  else launch_dd_r<double>(c, b, group, s);
+}
+
+// Start of the FP64 refinement of the mixed solve: every bead's dipoles go
+// back to the input of its last SOR step and every bead is active again.
+__global__ __launch_bounds__(256) void k_refine_prep(SysConst c, const double *__restrict__ dip_prev,
+                                                    double *__restrict__ dip, int *active) {
+  const size_t n = (size_t)c.P * c.N * c.nd * 3;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dip[i] = dip_prev[i];
+  if (i < (size_t)c.P) active[i] = 1;
+}
+
+void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s) {
+  const size_t n = std::max((size_t)c.P * c.N * c.nd * 3, (size_t)c.P);
+  hipLaunchKernelGGL(k_refine_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c,
+                     b.dip_prev, b.dip, b.active);
 }
 
 // Per-bead convergence test (pot_ttm.f90:380-385): deltadip =
@@ -1034,7 +1278,7 @@ __device__ __forceinline__ bool is_dip(int t) {
   return MODEL == 2 ? t < 3 : t == 3;
 }
 
-template <int MODEL, int G>
+template <int MODEL, int G, typename R>
 __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__restrict__ site,
                                                    const double *__restrict__ Rcs,
                                                    const double *__restrict__ dip,
@@ -1081,46 +1325,53 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
         if (MODEL == 3 && (sa == 0 || sb == 0)) continue;
         // site-site minimum image oriented from a to b (pot_ttm.f90:419-420);
         // v is the reference's orientation (lower molecule first)
-        double r[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
-        minimg(c, r[0], r[1], r[2]);
-        const double *v = r;
-        const double d2 = norm2_exact(v[0], v[1], v[2]);
-        const double qa = A[sa].q, qb = B[sb].q;
-        double qd[3];
-        for (int k = 0; k < 3; ++k) qd[k] = qb * mua[sa][k] - qa * mub[sb][k];
-        const double qsq = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
-        double dij[3] = {0, 0, 0}, ts1 = 0.0, ts2 = 0.0;
+        double rd[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
+        minimg(c, rd[0], rd[1], rd[2]);
+        const double *v = rd;
+        const double d2 = norm2_exact(rd[0], rd[1], rd[2]);
+        const R r[3] = {(R)rd[0], (R)rd[1], (R)rd[2]};
+        const R qa = (R)A[sa].q, qb = (R)B[sb].q;
+        R di[3], dj[3];
+        for (int k = 0; k < 3; ++k) {
+          di[k] = (R)mua[sa][k];
+          dj[k] = (R)mub[sb][k];
+        }
+        R qd[3];
+        for (int k = 0; k < 3; ++k) qd[k] = qb * di[k] - qa * dj[k];
+        const R qsq = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
+        R dijr[3] = {0, 0, 0}, ts1 = 0, ts2 = 0;
         bool have_ts1 = false;
         if (da && db) {
-          double t1d, t2d, t3d;
-          smear3<THOLE>(c, d2, lo ? c.apol3DD[sa][sb] : c.apol3DD[sb][sa], t1d, t2d, t3d);
+          R t1d, t2d, t3d;
+          smear3<THOLE>(c, (R)d2, (R)(lo ? c.apol3DD[sa][sb] : c.apol3DD[sb][sa]), t1d, t2d, t3d);
           // di . T3(r) . dj with T3_abc = 15 r_a r_b r_c ts3 - 3 ts2 (r_a d_bc + r_b d_ac + r_c d_ab)
-          const double *di = mua[sa], *dj = mub[sb];
-          const double rdi = r[0] * di[0] + r[1] * di[1] + r[2] * di[2];
-          const double rdj = r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2];
-          const double didj = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+          const R rdi = r[0] * di[0] + r[1] * di[1] + r[2] * di[2];
+          const R rdj = r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2];
+          const R didj = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
           for (int k = 0; k < 3; ++k)
-            dij[k] = 15.0 * t3d * rdi * rdj * r[k] - 3.0 * t2d * (rdi * dj[k] + rdj * di[k] + didj * r[k]);
-          if (qsq > 1e-9) {
+            dijr[k] = (R)15.0 * t3d * rdi * rdj * r[k] -
+                      (R)3.0 * t2d * (rdi * dj[k] + rdj * di[k] + didj * r[k]);
+          if (qsq > (R)1e-9) {
             if (c.same_a) {
               ts1 = t1d;
               ts2 = t2d;
             } else {
-              smear2<THOLE>(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
+              smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]), ts1, ts2);
             }
             have_ts1 = true;
-            double o[3];
+            R o[3];
             dd3_apply(r, ts1, ts2, qd, o);
-            for (int k = 0; k < 3; ++k) dij[k] += o[k];
+            for (int k = 0; k < 3; ++k) dijr[k] += o[k];
           }
-        } else if (qsq > 1e-9) {
-          smear2<THOLE>(c, d2, lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa], ts1, ts2);
+        } else if (qsq > (R)1e-9) {
+          smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]), ts1, ts2);
           have_ts1 = true;
-          double o[3];
+          R o[3];
           dd3_apply(r, ts1, ts2, qd, o);
-          for (int k = 0; k < 3; ++k) dij[k] += o[k];
+          for (int k = 0; k < 3; ++k) dijr[k] += o[k];
         }
-        if (have_ts1 && db) ph[sa] += ts1 * (r[0] * mub[sb][0] + r[1] * mub[sb][1] + r[2] * mub[sb][2]);
+        if (have_ts1 && db) ph[sa] += (double)(ts1 * (r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2]));
+        const double dij[3] = {(double)dijr[0], (double)dijr[1], (double)dijr[2]};
         for (int k = 0; k < 3; ++k) fr[sa][k] += dij[k];
         if (lo) {
           double rcv[3];
@@ -1153,25 +1404,31 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
   }
 }
 
-template <int G>
+template <int G, typename R>
 static void launch_final_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   long threads = (long)c.P * c.N * G;
   dim3 grid((threads + 255) / 256);
   if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_final<2, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
+    hipLaunchKernelGGL((k_pair_final<2, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
                        b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
   else
-    hipLaunchKernelGGL((k_pair_final<3, G>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
+    hipLaunchKernelGGL((k_pair_final<3, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
                        b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
 }
 
-void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+template <typename R>
+static void launch_final_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
   if (group == 64)
-    launch_final_g<64>(c, b, s);
+    launch_final_g<64, R>(c, b, s);
   else if (group == 32)
-    launch_final_g<32>(c, b, s);
+    launch_final_g<32, R>(c, b, s);
   else
-    launch_final_g<16>(c, b, s);
+    launch_final_g<16, R>(c, b, s);
+}
+
+void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  if (c.mixed) launch_final_r<float>(c, b, group, s);
+  else launch_final_r<double>(c, b, group, s);
 }
 
 // Reciprocal-space virial (ewald.f90:194-224), partial sums per block of k:

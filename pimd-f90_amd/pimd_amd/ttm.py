@@ -127,7 +127,7 @@ class TTMForceField:
         self.params = p
         L = load_library()
         cp = PimdParams(p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip),
-                        0, device, 0, (C.c_double * 3)(*p.box), p.rc, p.rc1, p.eps_ewald,
+                        int(p.precision), device, 0, (C.c_double * 3)(*p.box), p.rc, p.rc1, p.eps_ewald,
                         p.polar_sor, p.polar_eps, p.mass_o, p.mass_h)
         h = C.c_void_p()
         _check(L.pimd_ctx_create(C.byref(cp), C.byref(h)))
