@@ -31,6 +31,10 @@ import numpy as np  # noqa: E402
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
+# 64 independent replicas: waters from {32..512}, beads from {8..64}, T in
+# [250, 350] K, models alternating TTM2.1-F / TTM3-F, Rc = L/2, eps 1e-8
+REPLICAS64 = "replicas64_mixed_sizes"
+
 WORKLOADS = {
     # name: (pot_model, nwaters, nbeads, rc, eps_ewald)
     "h2o512_p32_ttm3f": (3, 512, 32, -1.0, 1e-6),
@@ -51,6 +55,8 @@ def flop_model(kernel, c):
         return P * N * nd * nk * 21
     if kernel == "pair_dd":       # per ordered pair and dipole-site pair: vectors 21, smear2 ~45, T.mu 18
         return pairs * nd * nd * 84
+    if kernel == "dd_sfac_d":     # pair_dd and sfac_d run side by side on two streams
+        return pairs * nd * nd * 84 + P * N * nd * nk * 21
     if kernel == "pair_qq":       # vdW ~30 + per site pair: vectors 21, smear01 ~40 (+Thole), field 10
         return pairs * (30 + 9 * 71)
     if kernel == "pair_final":    # 5 site pairs (TTM3-F): smear2/3 ~50, tensors ~40, vectors 21
@@ -114,7 +120,7 @@ def cpu_baseline(p, RRt, RRc, budget_s=15.0):
 
 
 PEAK_FP32_TFLOPS = 157.3
-FP32_KERNELS = ("pair_qq", "pair_dd", "pair_final")
+FP32_KERNELS = ("pair_qq", "pair_dd", "pair_final")  # FP32 arithmetic in the mixed form
 
 
 def run_form(p, RRt, PPt, RRc, args):
@@ -171,12 +177,97 @@ def _kern(res, steps):
     return {k: round(v[1] / max(steps, 1), 4) for k, v in res["kernels"].items()}
 
 
+def replica_specs(n=64, seed=3000):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for r in range(n):
+        specs.append({"model": 2 if r % 2 == 0 else 3,
+                      "nwaters": int(rng.choice([32, 64, 128, 256, 512])),
+                      "nbeads": int(rng.choice([8, 16, 32, 64])),
+                      "temp": float(rng.uniform(250.0, 350.0)), "seed": seed + r})
+    return specs
+
+
+def run_replicas(args):
+    """Aggregate throughput of 64 uneven replicas: run one after another with
+    the default launch shapes, then side by side as one batch, untuned and
+    autotuned.  Every run starts each replica from the same state."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PIMDIntegrator, ReplicaBatch, TTMForceField, load_library
+    specs = replica_specs()
+    reps = []
+    for sp in specs:
+        RRt, PPt, RRc, L = make_system(sp["nwaters"], sp["nbeads"], temp=sp["temp"], seed=sp["seed"])
+        p = TTMParams(pot_model=sp["model"], nwaters=sp["nwaters"], nbeads=sp["nbeads"],
+                      box=(L, L, L), rc=-1.0, eps_ewald=1e-8, mass_o=15.995, mass_h=1.008,
+                      guess_initdip=True, precision=1 if args.precision == "mixed" else 0)
+        ff = TTMForceField(p)
+        integ = PIMDIntegrator(ff, delt_fs=0.25, temp=sp["temp"], bead_thermostat=1,
+                               centroid_thermostat=True, tau_centroid=0.1)
+        f0 = ff.full_bead_forces(RRt, RRc)
+        reps.append((integ, ff, RRt, PPt, f0["dRRt"]))
+    work = sum(sp["nwaters"] * sp["nbeads"] for sp in specs)
+    seeds = [sp["seed"] for sp in specs]
+    L = load_library()
+
+    def reset():
+        for integ, ff, RRt, PPt, g in reps:
+            integ.set_state(RRt, PPt, g)
+
+    def sequential():
+        reset()
+        for (integ, *_), sd in zip(reps, seeds):
+            integ.run(args.warmup, t0=1, seed=sd)
+        t0 = time.perf_counter()
+        for (integ, *_), sd in zip(reps, seeds):
+            integ.run(args.steps, t0=1 + args.warmup, seed=sd)
+        return (time.perf_counter() - t0) / args.steps
+
+    def batched(batch):
+        reset()
+        batch.run(args.warmup, seeds, t0=1)
+        for integ, ff, *_ in reps:
+            L.pimd_sync(ff.handle)
+        t0 = time.perf_counter()
+        batch.run(args.steps, seeds, t0=1 + args.warmup)
+        return (time.perf_counter() - t0) / args.steps
+
+    s_seq = sequential()
+    batch = ReplicaBatch([r[0] for r in reps])
+    s_bat = batched(batch)
+    reset()
+    tune_ms = batch.autotune()
+    shapes = batch.shapes()
+    s_tuned = batched(batch)
+    return {
+        "metric": "PIMD ns/day + molecule-bead force evals/s, 512 H2O P=32 TTM3-F, mixed vs FP64",
+        "value": round(work / s_tuned, 1), "unit": "molecule-bead force evals/s (sum over replicas)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000 * s_tuned, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": REPLICAS64, "replicas": len(specs),
+                   "molecule_beads_per_step": work, "eps_ewald": 1e-8, "rc": "L/2",
+                   "precision": args.precision if args.precision != "both" else "fp64"},
+        "sequential": {"value": round(work / s_seq, 1), "ms_per_step": round(1000 * s_seq, 3)},
+        "batched_untuned": {"value": round(work / s_bat, 1), "ms_per_step": round(1000 * s_bat, 3)},
+        "batched_autotuned": {"value": round(work / s_tuned, 1),
+                              "ms_per_step": round(1000 * s_tuned, 3)},
+        "batch_over_sequential": round(s_seq / s_tuned, 3),
+        "autotuned_shapes": {"fields": ["pair_group", "dd_group", "field_cg", "sfac_sites"],
+                             "per_replica": shapes[:8], "force_ms": [round(x, 4) for x in tune_ms[:8]]},
+        "replicas": specs[:8],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="h2o512_p32_ttm3f", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="h2o512_p32_ttm3f",
+                    choices=sorted(WORKLOADS) + [REPLICAS64])
     ap.add_argument("--precision", default="both", choices=("fp64", "mixed", "both"),
                     help="form(s) to time; with both, value is the mixed form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -197,6 +288,10 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+
+    if args.workload == REPLICAS64:
+        print(json.dumps(run_replicas(args)), flush=True)
+        return
 
     from pimd_amd.params import TTMParams
     p, RRt, PPt, RRc = build_system(args.workload)

@@ -163,6 +163,27 @@ int pimd_state_get(pimd_ctx *ctx, double *RRt, double *PPt, double *dRRt, double
 int pimd_md_run(pimd_ctx *ctx, int t0, int nsteps, const double *noise, uint64_t seed,
                 double *Upot, double *virial, double *virialc, double *uk);
 
+/* ---- batches of independent systems ------------------------------------
+ * No counterpart in the reference, which runs one system per job.  A batch
+ * runs several contexts (replicas of any size, model and bead count, each
+ * already set up with pimd_md_init and pimd_state_set) side by side on one
+ * device, each on its own stream. */
+typedef struct pimd_batch pimd_batch;
+int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out);
+int pimd_batch_destroy(pimd_batch *batch);
+/* Picks each replica's launch shapes (lanes per molecule of the pair kernels,
+ * of the dipole kernel, column groups of the field kernels, sites per
+ * structure-factor chunk) by timing force evaluations of its loaded
+ * coordinates; ms[n] (may be NULL) receives the best time per replica.  The
+ * MD state, including the dipole predictor history, is left as it was. */
+int pimd_batch_autotune(pimd_batch *batch, double *ms);
+/* The four launch-shape values of replica i, in the order above. */
+int pimd_batch_shape(pimd_batch *batch, int i, int *shape);
+/* nsteps PIMD_VV steps of every replica from MD step t0 (device normals,
+ * seeds[n]); Upot[n][nsteps] and uk[n][nsteps] (may be NULL). */
+int pimd_batch_md_run(pimd_batch *batch, int t0, int nsteps, const uint64_t *seeds, double *Upot,
+                      double *uk);
+
 /* ---- instrumentation --------------------------------------------------- */
 
 /* Enables per-kernel HIP-event timing on the context stream. */

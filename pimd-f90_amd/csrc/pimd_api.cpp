@@ -50,6 +50,11 @@ struct pimd_ctx {
   SysConst c{};
   DevBufs d{};
   hipStream_t stream = nullptr;
+  // second stream: the real-space dipole field of each SOR iteration runs on
+  // it beside the reciprocal-space structure factor (both read the same
+  // dipoles); fork/join through the two events
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int group = 64;
   int natoms = 0;
   std::vector<double> kvec, Ak, p4vAk;
@@ -60,6 +65,8 @@ struct pimd_ctx {
   std::vector<double> C, A11, A12, A21, A22, om, msf, c1, c2;
   pimd_md_params md{};
   double *d_vxi = nullptr, *d_s = nullptr, *d_step = nullptr;
+  double *d_rec = nullptr;  // per-step records of md runs
+  int rec_cap = 0;
   int *h_nactive = nullptr;
   // profiling
   bool prof = false;
@@ -147,6 +154,10 @@ void free_all(pimd_ctx *x) {
   }
   for (auto e : x->pool) (void)hipEventDestroy(e);
   if (x->stream) (void)hipStreamDestroy(x->stream);
+  if (x->stream2) (void)hipStreamDestroy(x->stream2);
+  if (x->d_rec) (void)hipFree(x->d_rec);
+  if (x->ev_fork) (void)hipEventDestroy(x->ev_fork);
+  if (x->ev_join) (void)hipEventDestroy(x->ev_join);
 }
 
 float f32(double v) { return (float)v; }
@@ -400,8 +411,12 @@ int alloc_system(pimd_ctx *x) {
   c.sf_sites = env_int("PIMD_SFAC_SITES", 32, 1, 64);
   c.fd_cg = env_int("PIMD_FIELD_CG", 0, 0, 16);
   c.sf_kind = env_int("PIMD_SFAC_KIND", 1, 0, 1);
+  c.fd_klds = env_int("PIMD_FIELD_KLDS", 1, 0, 1);
+  // phase tables of 64 sites, or the 16-group reduction if larger (in double2)
+  c.fd_tab_d2 = std::max(c.T * 64, 16 * 4 * 64 / 2);
   c.g_dd = env_int("PIMD_G_DD", 16, 0, 64);
   c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
+  c.overlap = env_int("PIMD_SOR_OVERLAP", 0, 0, 1);
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
   int rc2;
   if ((rc2 = dalloc(&d.part, P * (size_t)c.sf_chunks_max * nk * 6)) != PIMD_OK) return rc2;
@@ -432,6 +447,18 @@ int pick_group(const SysConst &c) {
   return 16;
 }
 
+// The real-space dipole field on the second stream beside the dipole
+// structure factor on the first; the first stream waits for both.
+int dd_beside_sfac(pimd_ctx *x, hipStream_t s) {
+  HIPCHK(hipEventRecord(x->ev_fork, s));
+  HIPCHK(hipStreamWaitEvent(x->stream2, x->ev_fork, 0));
+  launch_pair_dd(x->c, x->d, x->group, false, x->stream2);
+  HIPCHK(hipEventRecord(x->ev_join, x->stream2));
+  launch_sfac(x->c, x->d, SF_D, s);
+  HIPCHK(hipStreamWaitEvent(s, x->ev_join, 0));
+  return PIMD_OK;
+}
+
 // One pot_ttm evaluation per bead on device arrays (pot_ttm.f90:3-630).
 int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *Upot,
                double *dRR, double *virt, double *virialc, double *dipI, double *dipE,
@@ -447,8 +474,15 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *
   x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
   x->timed("post_qq", [&] { launch_post_qq(c, d, t, s); });
   for (int it = 1; it <= c.maxiter; ++it) {
-    x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s); });
-    x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
+    if (c.overlap) {
+      // pair_dd on the second stream, concurrent with the structure factor
+      int err = PIMD_OK;
+      x->timed("dd_sfac_d", [&] { err = dd_beside_sfac(x, s); });
+      if (err != PIMD_OK) return err;
+    } else {
+      x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s); });
+      x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
+    }
     x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
     x->timed("converge", [&] { launch_converge(c, d, it, s); });
     // No host round trip: converged beads are switched off on the device
@@ -520,8 +554,12 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
   x->p = p;
   x->natoms = 3 * p.nwaters;
   int rc = setup_system(x);
-  if (rc == PIMD_OK && hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
+  if (rc == PIMD_OK && (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess ||
+                        hipStreamCreateWithFlags(&x->stream2, hipStreamNonBlocking) != hipSuccess))
     rc = fail(PIMD_EDEVICE, "hipStreamCreate failed");
+  if (rc == PIMD_OK && (hipEventCreateWithFlags(&x->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&x->ev_join, hipEventDisableTiming) != hipSuccess))
+    rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
   if (rc == PIMD_OK) rc = alloc_system(x);
   if (rc == PIMD_OK) {
     set_constant_tables();
@@ -821,13 +859,35 @@ __global__ void k_step_record(int P, const double *Upot, const double *virt, con
 }
 }  // namespace pimd
 
-extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise, uint64_t seed,
-                           double *Upot, double *virial, double *virialc, double *uk) {
+// Checks that a context is ready for device-resident MD steps.
+static int md_check(pimd_ctx *x) {
   if (!x) return fail(PIMD_EINVAL, "null argument");
   if (!x->md_ready) return fail(PIMD_ESTATE, "pimd_md_init has not been called");
   if (!x->state_ready) return fail(PIMD_ESTATE, "pimd_state_set has not been called");
-  if (nsteps < 0) return fail(PIMD_EINVAL, "nsteps must be >= 0");
-  HIPCHK(hipSetDevice(x->p.device));
+  return PIMD_OK;
+}
+
+// Device buffer for the per-step records of nsteps steps (grown, never shrunk).
+static int md_records(pimd_ctx *x, int nsteps) {
+  if (nsteps <= x->rec_cap) return PIMD_OK;
+  if (x->d_rec) (void)hipFree(x->d_rec);
+  x->d_rec = nullptr;
+  x->rec_cap = 0;
+  int rc = dalloc(&x->d_rec, (size_t)nsteps * 4);
+  if (rc == PIMD_OK) x->rec_cap = nsteps;
+  return rc;
+}
+
+// Kinetic energy of the loaded state (needed by the first thermostat call).
+static void md_begin(pimd_ctx *x) {
+  DevBufs &d = x->d;
+  x->timed("kinetic", [&] { launch_kinetic(x->c.P, x->natoms, d.masses, d.PP, d.uk, d.ukpart, x->stream); });
+}
+
+// Enqueues PIMD_VV step st (MD step t0 + st, MD.f90:35-110) on the context's
+// stream and records Upot, virial, virialc, uk into record slot st.  Does not
+// synchronise.  noise: NULL or the normals of this step, [2][Natoms][P][3].
+static int md_step(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
   const int P = x->c.P, na = x->natoms;
   const size_t n = (size_t)P * na * 3;
   const pimd_md_params &md = x->md;
@@ -835,66 +895,244 @@ extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise,
   const int L = 3 * na * P;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
-  std::vector<double> rec((size_t)nsteps * 4);
-  double *d_rec = nullptr;
-  if (nsteps > 0) {
-    int rc = dalloc(&d_rec, (size_t)nsteps * 4);
-    if (rc != PIMD_OK) return rc;
-  }
-  auto thermostat = [&](int step, int half) {
+  auto thermostat = [&](int half) {
     if (md.bead_thermostat != 1) return;
     const double *nz = nullptr;
     if (noise) {
-      (void)hipMemcpyAsync(d.noise, noise + ((size_t)step * 2 + half) * n, sizeof(double) * n,
+      (void)hipMemcpyAsync(d.noise, noise + (size_t)half * n, sizeof(double) * n,
                            hipMemcpyHostToDevice, s);
       nz = d.noise;
     }
     x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, (uint64_t)(t0 + step) * 2 + half, x->p.mass_o,
+      launch_langevin(P, na, d, d.PP, nz, seed, (uint64_t)(t0 + st) * 2 + half, x->p.mass_o,
                       x->p.mass_h, s);
     });
   };
-  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
-  for (int st = 0; st < nsteps; ++st) {
-    thermostat(st, 0);
-    if (md.global_nh) {
-      x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
-      x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
-    }
-    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-    if (P > 1)
-      x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s); });
-    else
-      x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, delt, s); });
-    x->timed("centroid_pbc", [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
-    int rc = run_forces(x, d.RR, d.RRc, t0 + st, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE,
-                        d.chg, d.iters_out);
-    if (rc != PIMD_OK) {
-      (void)hipFree(d_rec);
-      return rc;
-    }
-    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
-    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-    thermostat(st, 1);
-    x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
-    if (md.global_nh) {
-      x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
-      x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
-    }
-    hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, s, P, d.Upot, d.virt, d.virialc, d.uk,
-                       d_rec + (size_t)st * 4);
+  thermostat(0);
+  if (md.global_nh) {
+    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
+    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
+  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+  if (P > 1)
+    x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s); });
+  else
+    x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, delt, s); });
+  x->timed("centroid_pbc", [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
+  int rc = run_forces(x, d.RR, d.RRc, t0 + st, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE,
+                      d.chg, d.iters_out);
+  if (rc != PIMD_OK) return rc;
+  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+  thermostat(1);
+  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  if (md.global_nh) {
+    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
+    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
+  }
+  hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, s, P, d.Upot, d.virt, d.virialc, d.uk,
+                     x->d_rec + (size_t)st * 4);
   HIPCHK(hipGetLastError());
-  if (nsteps > 0)
-    HIPCHK(hipMemcpyAsync(rec.data(), d_rec, sizeof(double) * nsteps * 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (d_rec) (void)hipFree(d_rec);
+  return PIMD_OK;
+}
+
+// Copies nsteps records to the host arrays (any may be NULL); the stream must
+// be synchronised.
+static int md_copy_records(pimd_ctx *x, int nsteps, double *Upot, double *virial, double *virialc,
+                           double *uk) {
+  if (nsteps <= 0) return PIMD_OK;
+  std::vector<double> rec((size_t)nsteps * 4);
+  HIPCHK(hipMemcpy(rec.data(), x->d_rec, sizeof(double) * nsteps * 4, hipMemcpyDeviceToHost));
   for (int st = 0; st < nsteps; ++st) {
     if (Upot) Upot[st] = rec[st * 4];
     if (virial) virial[st] = rec[st * 4 + 1];
     if (virialc) virialc[st] = rec[st * 4 + 2];
     if (uk) uk[st] = rec[st * 4 + 3];
   }
+  return PIMD_OK;
+}
+
+extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise, uint64_t seed,
+                           double *Upot, double *virial, double *virialc, double *uk) {
+  int rc = md_check(x);
+  if (rc != PIMD_OK) return rc;
+  if (nsteps < 0) return fail(PIMD_EINVAL, "nsteps must be >= 0");
+  HIPCHK(hipSetDevice(x->p.device));
+  if ((rc = md_records(x, nsteps)) != PIMD_OK) return rc;
+  const size_t n = (size_t)x->c.P * x->natoms * 3;
+  md_begin(x);
+  for (int st = 0; st < nsteps; ++st)
+    if ((rc = md_step(x, t0, st, noise ? noise + (size_t)st * 2 * n : nullptr, seed)) != PIMD_OK)
+      return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return md_copy_records(x, nsteps, Upot, virial, virialc, uk);
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent systems (replicas) run side by side: each context
+// keeps its own stream, the host enqueues step st of every replica before
+// step st + 1 of any, and small replicas fill the device together.
+// ---------------------------------------------------------------------------
+struct pimd_batch {
+  std::vector<pimd_ctx *> ctx;
+};
+
+// Launch shapes of one context: lanes per molecule of the pair kernels and of
+// the LDS-staged dipole kernel, column groups of the field kernels and sites
+// per structure-factor chunk.
+struct Shape {
+  int group, g_dd, fd_cg, sf_sites;
+};
+
+static Shape get_shape(const pimd_ctx *x) {
+  return Shape{x->group, x->c.g_dd, x->c.fd_cg, x->c.sf_sites};
+}
+static void set_shape(pimd_ctx *x, const Shape &sh) {
+  x->group = sh.group;
+  x->c.g_dd = sh.g_dd;
+  x->c.fd_cg = sh.fd_cg;
+  x->c.sf_sites = sh.sf_sites;
+}
+
+// Mean time of one all-beads force evaluation on the loaded coordinates (ms),
+// writing the gradient to scratch so the MD state is untouched.
+static int time_forces(pimd_ctx *x, double *scratch, hipEvent_t e0, hipEvent_t e1, int reps,
+                       double &ms) {
+  DevBufs &d = x->d;
+  hipStream_t s = x->stream;
+  int rc = run_forces(x, d.RR, d.RRc, 1, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
+                      d.chg, d.iters_out);
+  if (rc != PIMD_OK) return rc;
+  HIPCHK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = run_forces(x, d.RR, d.RRc, 1, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
+                         d.chg, d.iters_out)) != PIMD_OK)
+      return rc;
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, e0, e1));
+  ms = (double)f / reps;
+  return PIMD_OK;
+}
+
+// Coordinate search over the launch shapes of one context, timing force
+// evaluations of its current coordinates.  Results do not depend on the
+// shapes beyond summation order.  The dipole predictor history is saved and
+// restored around the trials.
+static int autotune_ctx(pimd_ctx *x, double *best_ms) {
+  HIPCHK(hipSetDevice(x->p.device));
+  DevBufs &d = x->d;
+  const SysConst &c = x->c;
+  const size_t nh = (size_t)c.P * 4 * c.N * c.nd * 3, ng = (size_t)c.P * x->natoms * 3;
+  double *hist_save = nullptr, *scratch = nullptr;
+  int rc = dalloc(&hist_save, nh);
+  if (rc == PIMD_OK) rc = dalloc(&scratch, ng);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == PIMD_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
+  const bool prof = x->prof;
+  x->prof = false;
+  if (rc == PIMD_OK &&
+      hipMemcpyAsync(hist_save, d.hist, sizeof(double) * nh, hipMemcpyDeviceToDevice, x->stream) !=
+          hipSuccess)
+    rc = fail(PIMD_EDEVICE, "hipMemcpyAsync failed");
+  Shape best = get_shape(x);
+  double tbest = 0.0;
+  if (rc == PIMD_OK) rc = time_forces(x, scratch, e0, e1, 2, tbest);
+  const int groups[] = {16, 32, 64}, gdd[] = {16, 32, 64}, cgs[] = {4, 8, 16}, sites[] = {16, 32, 64};
+  for (int knob = 0; knob < 4 && rc == PIMD_OK; ++knob) {
+    const int *vals = knob == 0 ? groups : knob == 1 ? gdd : knob == 2 ? cgs : sites;
+    for (int v = 0; v < 3 && rc == PIMD_OK; ++v) {
+      Shape sh = best;
+      int &field = knob == 0 ? sh.group : knob == 1 ? sh.g_dd : knob == 2 ? sh.fd_cg : sh.sf_sites;
+      if (field == vals[v]) continue;
+      field = vals[v];
+      set_shape(x, sh);
+      double t = 0.0;
+      rc = time_forces(x, scratch, e0, e1, 2, t);
+      if (rc == PIMD_OK && t < tbest) {
+        tbest = t;
+        best = sh;
+      }
+    }
+  }
+  set_shape(x, best);
+  x->prof = prof;
+  if (hist_save) {
+    (void)hipMemcpyAsync(d.hist, hist_save, sizeof(double) * nh, hipMemcpyDeviceToDevice, x->stream);
+    (void)hipStreamSynchronize(x->stream);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (hist_save) (void)hipFree(hist_save);
+  if (scratch) (void)hipFree(scratch);
+  if (best_ms) *best_ms = tbest;
+  return rc;
+}
+
+extern "C" int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out) {
+  if (!ctxs || !out || n < 1) return fail(PIMD_EINVAL, "need at least one context");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(PIMD_EINVAL, "context %d is null", i);
+    if (ctxs[i]->p.device != ctxs[0]->p.device)
+      return fail(PIMD_EINVAL, "all contexts of a batch must be on one device");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(PIMD_EINVAL, "context %d appears twice", i);
+  }
+  pimd_batch *b = new pimd_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  *out = b;
+  return PIMD_OK;
+}
+
+extern "C" int pimd_batch_destroy(pimd_batch *b) {
+  delete b;
+  return PIMD_OK;
+}
+
+extern "C" int pimd_batch_autotune(pimd_batch *b, double *ms) {
+  if (!b) return fail(PIMD_EINVAL, "null argument");
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    pimd_ctx *x = b->ctx[i];
+    if (!x->state_ready) return fail(PIMD_ESTATE, "replica %zu: pimd_state_set has not been called", i);
+    int rc = autotune_ctx(x, ms ? ms + i : nullptr);
+    if (rc != PIMD_OK) return rc;
+  }
+  return PIMD_OK;
+}
+
+extern "C" int pimd_batch_shape(pimd_batch *b, int i, int *shape) {
+  if (!b || !shape || i < 0 || i >= (int)b->ctx.size()) return fail(PIMD_EINVAL, "bad argument");
+  const Shape sh = get_shape(b->ctx[i]);
+  shape[0] = sh.group;
+  shape[1] = sh.g_dd;
+  shape[2] = sh.fd_cg;
+  shape[3] = sh.sf_sites;
+  return PIMD_OK;
+}
+
+extern "C" int pimd_batch_md_run(pimd_batch *b, int t0, int nsteps, const uint64_t *seeds,
+                                 double *Upot, double *uk) {
+  if (!b || !seeds) return fail(PIMD_EINVAL, "null argument");
+  if (nsteps < 0) return fail(PIMD_EINVAL, "nsteps must be >= 0");
+  const int n = (int)b->ctx.size();
+  int rc;
+  HIPCHK(hipSetDevice(b->ctx[0]->p.device));
+  for (int r = 0; r < n; ++r) {
+    if ((rc = md_check(b->ctx[r])) != PIMD_OK) return rc;
+    if ((rc = md_records(b->ctx[r], nsteps)) != PIMD_OK) return rc;
+  }
+  for (int r = 0; r < n; ++r) md_begin(b->ctx[r]);
+  for (int st = 0; st < nsteps; ++st)
+    for (int r = 0; r < n; ++r)
+      if ((rc = md_step(b->ctx[r], t0, st, nullptr, seeds[r])) != PIMD_OK) return rc;
+  for (int r = 0; r < n; ++r) HIPCHK(hipStreamSynchronize(b->ctx[r]->stream));
+  for (int r = 0; r < n; ++r)
+    if ((rc = md_copy_records(b->ctx[r], nsteps, Upot ? Upot + (size_t)r * nsteps : nullptr,
+                              nullptr, nullptr, uk ? uk + (size_t)r * nsteps : nullptr)) != PIMD_OK)
+      return rc;
   return PIMD_OK;
 }
 

@@ -42,9 +42,12 @@ struct SysConst {
   int sf_chunks_max;       // site chunks of the column structure-factor kernel
   int sf_sites;            // fewest sites per structure-factor chunk
   int fd_cg;               // column groups per field block (0: by occupancy)
+  int fd_klds;             // 1: field kernels stage the bead's k data in LDS
+  int fd_tab_d2;           // double2 slots of the field kernel's table / reduction area
   int sf_kind;             // structure-factor kernel: 0 one k per lane, 1 kz-columns
   int g_dd;                // lanes per molecule of the LDS-staged dipole kernel (0: gather kernel)
   int poll;                // 1: read the active-bead count back after every SOR iteration
+  int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;

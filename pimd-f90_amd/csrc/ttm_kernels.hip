@@ -663,7 +663,9 @@ __global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *_
   const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, component)
   if (e >= nv * c.nk) return;
   double v = 0.0;
-  for (int q = 0; q < nch; ++q) v += part[((size_t)b * nch + q) * c.nk * nv + e];
+  const double *pp = part + (size_t)b * nch * c.nk * nv + e;
+#pragma unroll 8
+  for (int q = 0; q < nch; ++q) v += pp[(size_t)q * c.nk * nv];
   out[(size_t)b * c.nk * nv + e] = v;
 }
 
@@ -750,7 +752,7 @@ void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
 //   FD_QD: phi and dR from the stale dipole and the total structure factors
 //          (ewald.f90:184-193)
 constexpr int FD_SB = 64;  // sites per block, one per lane
-template <int MODE, int MODEL, int CG>
+template <int MODE, int MODEL, int CG, bool KL>
 __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__restrict__ site,
                                                    const double2 *__restrict__ tab,
                                                    const double4 *__restrict__ kv,
@@ -767,7 +769,9 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
                                                    double *__restrict__ diff2,
                                                    const int *__restrict__ active, int iter) {
   constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
-  // [T][64] phase tables of the block's sites, later reused for the reduction
+  // [T][64] phase tables of the block's sites (later reused for the
+  // reduction), then with KL the bead's k data: [nk] (w Re S, w Im S, kz, w)
+  // with w = 4 pi A_k / V, and for FD_QD [nk] (w Re(Sq+Sm), w Im(Sq+Sm))
   extern __shared__ double2 lds_tab[];
   const int b = blockIdx.y;
   if (MODE == FD_D && !active[b]) return;
@@ -780,6 +784,18 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
   for (int e = threadIdx.x; e < c.T * FD_SB; e += 64 * CG) {
     const int row = e >> 6, l = e & 63;
     lds_tab[e] = tb[(size_t)row * S + set_site(c, SET, min(mbase + l, M - 1))];
+  }
+  double4 *kd = reinterpret_cast<double4 *>(lds_tab + c.fd_tab_d2);
+  double2 *kq = reinterpret_cast<double2 *>(kd + c.nk);
+  if (KL) {
+    const double *sx = (MODE == FD_Q ? Sq : Sm) + (size_t)b * c.nk * 2;
+    const double *sq2 = Sq + (size_t)b * c.nk * 2;
+    for (int k = threadIdx.x; k < c.nk; k += 64 * CG) {
+      const double4 kk = kv[k];
+      kd[k] = make_double4(kk.w * sx[2 * k], kk.w * sx[2 * k + 1], kk.z, kk.w);
+      if (MODE == FD_QD) kq[k] = make_double2(kk.w * (sq2[2 * k] + sx[2 * k]),
+                                              kk.w * (sq2[2 * k + 1] + sx[2 * k + 1]));
+    }
   }
   __syncthreads();
   const int m = mbase + lane;
@@ -816,6 +832,36 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
 #pragma unroll 4
     for (int i = 0; i < cl.w; ++i) {
       const int k = k0 + i;
+      if (KL) {
+        const double4 q4 = kd[k];  // w Re S, w Im S, kz, w
+        if (MODE == FD_Q) {
+          const double t = ck * q4.y - sk * q4.x;
+          at += t;
+          az += q4.z * t;
+          ph += ck * q4.x + sk * q4.y;
+        } else if (MODE == FD_D) {
+          const double t = ck * q4.y - sk * q4.x;
+          at += t;
+          az += q4.z * t;
+        } else {
+          double t = 0.0;
+          if (has_q) {
+            t = -(q * (ck * q4.y - sk * q4.x));
+            ph += ck * q4.x + sk * q4.y;
+          }
+          if (has_d) {
+            const double2 qq = kq[k];
+            const double dk = dxy + q4.z * mu[2];
+            t += dk * (ck * qq.x + sk * qq.y);
+          }
+          at += t;
+          az += q4.z * t;
+        }
+        const double nc = ck * zstep.x - sk * zstep.y;
+        sk = sk * zstep.x + ck * zstep.y;
+        ck = nc;
+        continue;
+      }
       const double4 kk = kv[k];  // kx, ky, kz, 4 pi A_k / V
       if (MODE == FD_Q) {
         const double re = sq[2 * k], im = sq[2 * k + 1];
@@ -895,16 +941,24 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
   }
 }
 
-// Shared memory of k_field: the phase tables, reused for the CG-way reduction.
-static size_t field_lds(const SysConst &c, int cg) {
-  return std::max(sizeof(double2) * c.T * FD_SB, sizeof(double) * cg * 4 * FD_SB);
+// Shared memory of k_field: the phase tables (reused for the CG-way
+// reduction; fd_tab_d2 double2 slots cover both), then the staged k data.
+static size_t field_lds(const SysConst &c, int mode, bool kl) {
+  return sizeof(double2) * c.fd_tab_d2 +
+         (kl ? c.nk * (sizeof(double4) + (mode == FD_QD ? sizeof(double2) : 0)) : 0);
 }
 
 template <int MO, int ML, int CG>
 static void launch_field_cg(const SysConst &c, DevBufs &b, int iter, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_field<MO, ML, CG>), grid, dim3(64 * CG), field_lds(c, CG), s, c, b.site,
-                     b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR, b.dip,
-                     b.dip_prev, b.Efd, b.diff2, b.active, iter);
+  const bool kl = c.fd_klds && field_lds(c, MO, true) <= 160 * 1024;
+  if (kl)
+    hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), field_lds(c, MO, true), s,
+                       c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
+                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
+  else
+    hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), field_lds(c, MO, false),
+                       s, c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
+                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
 }
 
 template <int MO, int ML>

@@ -33,7 +33,8 @@ EXPORTS = (
     "pimd_nm_init", "pimd_nm_get", "pimd_nm_evolve", "pimd_langevin_init", "pimd_langevin",
     "pimd_nose_hoover", "pimd_md_init", "pimd_state_set", "pimd_state_get", "pimd_md_run",
     "pimd_profile_enable", "pimd_profile_count", "pimd_profile_get", "pimd_profile_reset",
-    "pimd_sync",
+    "pimd_sync", "pimd_batch_create", "pimd_batch_destroy", "pimd_batch_autotune",
+    "pimd_batch_shape", "pimd_batch_md_run",
 )
 
 
@@ -92,6 +93,11 @@ def load_library(path: str = LIB_PATH):
         "pimd_profile_get": [vp, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_long), dp],
         "pimd_profile_reset": [vp],
         "pimd_sync": [vp],
+        "pimd_batch_create": [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)],
+        "pimd_batch_destroy": [vp],
+        "pimd_batch_autotune": [vp, dp],
+        "pimd_batch_shape": [vp, C.c_int, ip],
+        "pimd_batch_md_run": [vp, C.c_int, C.c_int, C.POINTER(C.c_uint64), dp, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -292,4 +298,50 @@ class PIMDIntegrator:
         _check(load_library().pimd_md_run(self.ff.handle, int(t0), int(nsteps), _ptr(nz), seed,
                                           _ptr(o["Upot"]), _ptr(o["virial"]), _ptr(o["virialc"]),
                                           _ptr(o["uk"])))
+        return o
+
+
+class ReplicaBatch:
+    """Independent PIMD systems (replicas of any size, model and bead count) run
+    side by side on one GPU, each integrator already loaded with its state.
+    No counterpart in the reference, which runs one system per job."""
+
+    def __init__(self, integrators):
+        self.integrators = list(integrators)
+        handles = (C.c_void_p * len(self.integrators))(*[i.ff.handle.value for i in self.integrators])
+        h = C.c_void_p()
+        _check(load_library().pimd_batch_create(handles, len(self.integrators), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().pimd_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self) -> np.ndarray:
+        """Picks launch shapes per replica; returns the best force time (ms) of each."""
+        ms = np.zeros(len(self.integrators))
+        _check(load_library().pimd_batch_autotune(self._h, _ptr(ms)))
+        return ms
+
+    def shapes(self) -> list:
+        out = []
+        for i in range(len(self.integrators)):
+            sh = (C.c_int * 4)()
+            _check(load_library().pimd_batch_shape(self._h, i, sh))
+            out.append(tuple(sh))
+        return out
+
+    def run(self, nsteps: int, seeds, t0: int = 1) -> dict:
+        n = len(self.integrators)
+        sd = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        o = {"Upot": np.zeros((n, nsteps)), "uk": np.zeros((n, nsteps))}
+        _check(load_library().pimd_batch_md_run(self._h, int(t0), int(nsteps), sd,
+                                                _ptr(o["Upot"]), _ptr(o["uk"])))
         return o
