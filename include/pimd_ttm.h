@@ -184,6 +184,23 @@ int pimd_batch_shape(pimd_batch *batch, int i, int *shape);
 int pimd_batch_md_run(pimd_batch *batch, int t0, int nsteps, const uint64_t *seeds, double *Upot,
                       double *uk);
 
+/* ---- PES parameters and their adjoint ------------------------------------
+ * No counterpart in the reference, which fits the monomer surface offline
+ * (PES_fitter/fit_energy.m).  theta[PIMD_PES_NPARAM]: the 245 combined
+ * Partridge-Schwenke coefficients c5z (nasa.f90:45), then deohA, alphaoh,
+ * roh, phh1A, phh2 (nasa.f90:46-57).  A context starts with the reference's
+ * values; later force calls use the values set. */
+#define PIMD_PES_NPARAM 250
+int pimd_pes_params_get(pimd_ctx *ctx, double *theta);
+int pimd_pes_params_set(pimd_ctx *ctx, const double *theta);
+/* Gradient of a loss over the P configurations held as beads,
+ *   L = sum_b (E_b - E*_b)^2 + sum_{b,atoms} |dE_b/dr - (dE_b/dr)*|^2,
+ * with respect to theta, given the configurations RRt[P][Natoms][3], the
+ * weights ew[P] = 2 (E_b - E*_b) and gw[P][Natoms][3] = 2 (dE_b/dr - target):
+ *   grad[j] = sum_b ew_b dE_b/dtheta_j + sum gw . d(dE_b/dr)/dtheta_j. */
+int pimd_pes_param_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
+                        double *grad);
+
 /* ---- instrumentation --------------------------------------------------- */
 
 /* Enables per-kernel HIP-event timing on the context stream. */

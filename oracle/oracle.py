@@ -47,6 +47,8 @@ def lib():
         L.orc_gammq23.argtypes = [C.c_double]
         L.orc_smear.argtypes = [C.c_void_p, C.c_double, C.c_double, dp]
         L.orc_monomer.argtypes = [dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double]
+        L.orc_pes_default.argtypes = [dp]
+        L.orc_pes_monomer.argtypes = [dp, dp, dp, dp, dp]
         L.orc_nm_init.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double] + [dp] * 7
         L.orc_nm_evolve.argtypes = [C.c_int, C.c_int] + [dp] * 9
         L.orc_langevin_init.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -157,3 +159,20 @@ def nose_hoover(uk, vxi, tau, delt2, L, temp):
     s, u, v = C.c_double(0.0), C.c_double(uk), np.array(vxi, dtype=np.float64)
     lib().orc_nose_hoover(C.byref(s), C.byref(u), v, tau, delt2, L, temp)
     return {"s": s.value, "uk": u.value, "vxi": v}
+
+
+def pes_default():
+    """The reference PES parameters (combined c5z, deohA, alphaoh, roh, phh1A, phh2)."""
+    th = np.zeros(250)
+    lib().orc_pes_default(th)
+    return th
+
+
+def pes_monomer(r1, box, theta):
+    """pot_nasa of one monomer r1[atom][xyz] with parameters theta: (e1, dr1[3][3])."""
+    e1 = np.zeros(1)
+    dr1 = np.zeros((3, 3))
+    lib().orc_pes_monomer(np.ascontiguousarray(r1, dtype=np.float64),
+                          np.ascontiguousarray(box, dtype=np.float64),
+                          np.ascontiguousarray(theta, dtype=np.float64), e1, dr1)
+    return float(e1[0]), dr1

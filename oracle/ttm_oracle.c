@@ -223,9 +223,31 @@ static void fmat_fill(double f[16][3], double x1, double x2, double x3) {
 
 static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-/* pot_nasa, nasa.f90:2-103.  r1[atom][xyz] (O,H1,H2) -> e1, dr1[atom][xyz] */
-static void pot_nasa(const double r1[3][3], double dr1[3][3], double *e1, const double *box,
-                     const double *boxi) {
+/* The reference PES parameters: the combined c5z (nasa.f90:45), then deohA,
+ * alphaoh, roh, phh1A, phh2. */
+void orc_pes_default(double *theta) {
+  for (int j = 0; j < PS_NTERM; ++j)
+    theta[j] = PS_F5Z * PS_C5ZA[j] + PS_FBASIS * PS_CBASIS[j] + PS_FCORE * PS_CCORE[j] +
+               PS_FREST * PS_CREST[j];
+  theta[PS_NTERM + 0] = PS_DEOHA;
+  theta[PS_NTERM + 1] = PS_ALPHAOH;
+  theta[PS_NTERM + 2] = PS_ROH;
+  theta[PS_NTERM + 3] = PS_PHH1A;
+  theta[PS_NTERM + 4] = PS_PHH2;
+}
+
+/* pot_nasa, nasa.f90:2-103.  r1[atom][xyz] (O,H1,H2) -> e1, dr1[atom][xyz];
+ * theta (NULL: the reference's values) as orc_pes_default. */
+static void pot_nasa_theta(const double r1[3][3], double dr1[3][3], double *e1, const double *box,
+                           const double *boxi, const double *theta) {
+  double dflt[PS_NTERM + 5];
+  if (!theta) {
+    orc_pes_default(dflt);
+    theta = dflt;
+  }
+  const double *c5z = theta;
+  const double deohA = theta[PS_NTERM], alphaoh = theta[PS_NTERM + 1], roh = theta[PS_NTERM + 2],
+               phh1A = theta[PS_NTERM + 3], phh2 = theta[PS_NTERM + 4];
   double ROH1[3], ROH2[3], RHH[3];
   for (int c = 0; c < 3; ++c) {
     ROH1[c] = r1[1][c] - r1[0][c];
@@ -238,21 +260,17 @@ static void pot_nasa(const double r1[3][3], double dr1[3][3], double *e1, const 
   double dROH1 = sqrt(dot3(ROH1, ROH1)), dROH2 = sqrt(dot3(ROH2, ROH2));
   double dRHH = sqrt(dot3(RHH, RHH));
   double costh = (ROH1[0] * ROH2[0] + ROH1[1] * ROH2[1] + ROH1[2] * ROH2[2]) / (dROH1 * dROH2);
-  double c5z[PS_NTERM];
-  for (int j = 0; j < PS_NTERM; ++j)
-    c5z[j] = PS_F5Z * PS_C5ZA[j] + PS_FBASIS * PS_CBASIS[j] + PS_FCORE * PS_CCORE[j] +
-             PS_FREST * PS_CREST[j];
-  double deoh = PS_F5Z * PS_DEOHA;
-  double phh1 = PS_F5Z * PS_PHH1A;
-  phh1 = phh1 * exp(PS_PHH2);
+  double deoh = PS_F5Z * deohA;
+  double phh1 = PS_F5Z * phh1A;
+  phh1 = phh1 * exp(phh2);
   const double costhe = -.24780227221366464506;
-  double exp1 = exp(-PS_ALPHAOH * (dROH1 - PS_ROH));
-  double exp2 = exp(-PS_ALPHAOH * (dROH2 - PS_ROH));
+  double exp1 = exp(-alphaoh * (dROH1 - roh));
+  double exp2 = exp(-alphaoh * (dROH2 - roh));
   double Va = deoh * (exp1 * (exp1 - 2.0) + exp2 * (exp2 - 2.0));
-  double Vb = phh1 * exp(-PS_PHH2 * dRHH);
-  double dVa1 = 2.0 * PS_ALPHAOH * deoh * exp1 * (1.0 - exp1) / dROH1;
-  double dVa2 = 2.0 * PS_ALPHAOH * deoh * exp2 * (1.0 - exp2) / dROH2;
-  double dVb = -PS_PHH2 * Vb / dRHH;
+  double Vb = phh1 * exp(-phh2 * dRHH);
+  double dVa1 = 2.0 * alphaoh * deoh * exp1 * (1.0 - exp1) / dROH1;
+  double dVa2 = 2.0 * alphaoh * deoh * exp2 * (1.0 - exp2) / dROH2;
+  double dVb = -phh2 * Vb / dRHH;
   double x1 = (dROH1 - PS_REOH) / PS_REOH, x2 = (dROH2 - PS_REOH) / PS_REOH, x3 = costh - costhe;
   double f[16][3];
   fmat_fill(f, x1, x2, x3);
@@ -284,6 +302,21 @@ static void pot_nasa(const double r1[3][3], double dr1[3][3], double *e1, const 
   }
   for (int a = 0; a < 3; ++a)
     for (int c = 0; c < 3; ++c) dr1[a][c] = dr1[a][c] * .00285914375100642899;
+}
+
+static void pot_nasa(const double r1[3][3], double dr1[3][3], double *e1, const double *box,
+                     const double *boxi) {
+  pot_nasa_theta(r1, dr1, e1, box, boxi, NULL);
+}
+
+/* pot_nasa with the PES parameters theta (orc_pes_default order). */
+void orc_pes_monomer(const double *r1, const double *box, const double *theta, double *e1,
+                     double *dr1) {
+  double boxi[3] = {1.0 / box[0], 1.0 / box[1], 1.0 / box[2]};
+  double r[3][3], d[3][3];
+  memcpy(r, r1, sizeof r);
+  pot_nasa_theta((const double(*)[3])r, d, e1, box, boxi, theta);
+  memcpy(dr1, d, sizeof d);
 }
 
 /* dms_nasa, nasa.f90:105-293.  q3 = (qO, qH1, qH2); gq[disp][charge][xyz] with

@@ -42,6 +42,10 @@ void orc_smear(const orc_sys *s, double drsq, double apol, double *out13);
 void orc_monomer(const double *r1, double *e1, double *dr1, double *q3,
                  double *dq3, const double *box, double dms1, double dms2,
                  double dms3);
+/* PES with its parameters as inputs (for finite differences of a fit). */
+void orc_pes_default(double *theta);
+void orc_pes_monomer(const double *r1, const double *box, const double *theta,
+                     double *e1, double *dr1);
 
 /* Ring polymer (NormalModes.f90, Langevin.f90, Nose_Hoover.f90). */
 void orc_nm_init(int P, double omegan, double delta, double setnm, double *C,

@@ -143,7 +143,7 @@ void free_all(pimd_ctx *x) {
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red,   x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj,   x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -336,11 +336,18 @@ int alloc_system(pimd_ctx *x) {
   ALLOC(d.uk, 1);
   ALLOC(d.ukpart, 256);
   ALLOC(d.radius, P * N);
+  ALLOC(d.pes, PES_NPARAM);
+  ALLOC(d.adj, 2 * P * na * 3 + P + (size_t)pes_adjoint_blocks(c) * PES_NPARAM + PES_NPARAM);
   ALLOC(x->d_vxi, 2);
   ALLOC(x->d_s, 1);
   ALLOC(x->d_step, 4);
 #undef ALLOC
   HIPCHK(hipMemset(d.hist, 0, sizeof(double) * P * 4 * N * nd * 3));
+  {
+    double theta[PES_NPARAM];
+    pes_default_params(theta);
+    HIPCHK(hipMemcpy(d.pes, theta, sizeof theta, hipMemcpyHostToDevice));
+  }
   // k-vectors and kz-columns
   std::vector<double4> kv(nk);
   std::vector<int4> kh(nk);
@@ -1133,6 +1140,47 @@ extern "C" int pimd_batch_md_run(pimd_batch *b, int t0, int nsteps, const uint64
     if ((rc = md_copy_records(b->ctx[r], nsteps, Upot ? Upot + (size_t)r * nsteps : nullptr,
                               nullptr, nullptr, uk ? uk + (size_t)r * nsteps : nullptr)) != PIMD_OK)
       return rc;
+  return PIMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PES parameters and their adjoint (for fitting the monomer surface).
+// ---------------------------------------------------------------------------
+extern "C" int pimd_pes_params_get(pimd_ctx *x, double *theta) {
+  if (!x || !theta) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipMemcpy(theta, x->d.pes, sizeof(double) * PES_NPARAM, hipMemcpyDeviceToHost));
+  return PIMD_OK;
+}
+
+extern "C" int pimd_pes_params_set(pimd_ctx *x, const double *theta) {
+  if (!x || !theta) return fail(PIMD_EINVAL, "null argument");
+  for (int j = 0; j < PES_NPARAM; ++j)
+    if (!std::isfinite(theta[j])) return fail(PIMD_EINVAL, "PES parameter %d is not finite", j);
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(x->d.pes, theta, sizeof(double) * PES_NPARAM, hipMemcpyHostToDevice));
+  return PIMD_OK;
+}
+
+extern "C" int pimd_pes_param_grad(pimd_ctx *x, const double *RRt, const double *ew,
+                                   const double *gw, double *grad) {
+  if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  const SysConst &c = x->c;
+  DevBufs &d = x->d;
+  hipStream_t s = x->stream;
+  const size_t n = (size_t)c.P * x->natoms * 3;
+  const int nb = pes_adjoint_blocks(c);
+  double *dR = d.adj, *dG = d.adj + n, *dE = d.adj + 2 * n, *part = dE + c.P;
+  double *dgrad = part + (size_t)nb * PES_NPARAM;
+  HIPCHK(hipMemcpyAsync(dR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dG, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dE, ew, sizeof(double) * c.P, hipMemcpyHostToDevice, s));
+  x->timed("pes_adjoint", [&] { launch_pes_adjoint(c, d, dR, dE, dG, part, nb, dgrad, s); });
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(grad, dgrad, sizeof(double) * PES_NPARAM, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
   return PIMD_OK;
 }
 

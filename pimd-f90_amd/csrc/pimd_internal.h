@@ -22,6 +22,12 @@ constexpr double FOURPI = 12.56637061435917295376;
 constexpr double TSP = 1.12837916709551257390;   // 2/sqrt(pi)
 constexpr double SQPI = 1.77245385090551602729;
 
+// Fitted Partridge-Schwenke PES parameters of a context: the 245 combined
+// coefficients c5z (nasa.f90:45), then these five (nasa_mod.f90 values).
+constexpr int PES_NC = 245;
+constexpr int PES_DEOHA = 245, PES_ALPHAOH = 246, PES_ROH = 247, PES_PHH1A = 248, PES_PHH2 = 249;
+constexpr int PES_NPARAM = 250;
+
 // One site of a molecule record: position and charge (charge in
 // CHARGECON-scaled units, as the reference's charge array).
 struct __align__(32) Site {
@@ -108,6 +114,8 @@ struct DevBufs {
   double *uk = nullptr;                         // [1] kinetic energy
   double *ukpart = nullptr;                     // [256] kinetic partial sums
   double *radius = nullptr;                     // [P][N] site radius about O
+  double *pes = nullptr;                        // [PES_NPARAM] PES parameters
+  double *adj = nullptr;                        // adjoint inputs and partial sums
   double *red = nullptr;                        // scratch for reductions
 };
 
@@ -130,6 +138,11 @@ void launch_finalize(const SysConst &c, DevBufs &b, int t, double *dRR, double *
 void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
                         double *virialc, int *iters, hipStream_t s);
 void set_constant_tables();
+void pes_default_params(double *theta);
+void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
+                        const double *gw, double *partial, int nblocks, double *grad,
+                        hipStream_t s);
+int pes_adjoint_blocks(const SysConst &c);
 
 enum SfacMode { SF_Q = 0, SF_D = 1, SF_V = 2 };
 enum FieldMode { FD_Q = 0, FD_D = 1, FD_QD = 2 };

@@ -9,9 +9,8 @@
 
 namespace pimd {
 
-// Combined PES coefficients (c5z of nasa.f90:45) and exponent tables,
-// uploaded once to constant memory.
-extern __constant__ double c_c5z[PS_NTERM];
+// Exponent tables of the PES and DMS bases and the DMS coefficients, uploaded
+// once to constant memory (the PES coefficients are per context).
 extern __constant__ int c_idx[PS_NTERM][3];
 extern __constant__ double c_coefd[PS_NDMS];
 extern __constant__ int c_idxd[PS_NDMS][3];
@@ -219,8 +218,13 @@ __device__ __forceinline__ void dd3_apply(const R r[3], R ts1, R ts2, const R v[
 // dipole-moment-surface charges with their gradients (nasa.f90:105-293).
 // r[atom][xyz] for O, H1, H2.  gq[disp][charge][xyz] with disp and charge in
 // the order (H1, H2, O), as the reference's gradq.
+// pes: the fitted PES parameters, [0, 245) the combined coefficients c5z of
+// nasa.f90:45, then deohA, alphaoh, roh, phh1A, phh2 (PES_* indices).
 __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], double &e1,
-                                  double dr1[3][3], double q3[3], double gq[3][3][3]) {
+                                  double dr1[3][3], double q3[3], double gq[3][3][3],
+                                  const double *__restrict__ pes) {
+  const double deohA = pes[PES_DEOHA], alphaoh = pes[PES_ALPHAOH], roh = pes[PES_ROH],
+               phh1A = pes[PES_PHH1A], phh2 = pes[PES_PHH2];
   double R1[3], R2[3], RH[3];
   for (int k = 0; k < 3; ++k) {
     R1[k] = r[1][k] - r[0][k];
@@ -247,28 +251,28 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
     f3[j] = f3[j - 1] * x3;
   }
   // ---- potential energy surface
-  double deoh = PS_F5Z * PS_DEOHA;
-  double phh1 = PS_F5Z * PS_PHH1A;
-  phh1 = phh1 * exp(PS_PHH2);
-  double ex1 = exp(-PS_ALPHAOH * (d1 - PS_ROH));
-  double ex2 = exp(-PS_ALPHAOH * (d2 - PS_ROH));
+  double deoh = PS_F5Z * deohA;
+  double phh1 = PS_F5Z * phh1A;
+  phh1 = phh1 * exp(phh2);
+  double ex1 = exp(-alphaoh * (d1 - roh));
+  double ex2 = exp(-alphaoh * (d2 - roh));
   double Va = deoh * (ex1 * (ex1 - 2.0) + ex2 * (ex2 - 2.0));
-  double Vb = phh1 * exp(-PS_PHH2 * dh);
-  double dVa1 = 2.0 * PS_ALPHAOH * deoh * ex1 * (1.0 - ex1) / d1;
-  double dVa2 = 2.0 * PS_ALPHAOH * deoh * ex2 * (1.0 - ex2) / d2;
-  double dVb = -PS_PHH2 * Vb / dh;
+  double Vb = phh1 * exp(-phh2 * dh);
+  double dVa1 = 2.0 * alphaoh * deoh * ex1 * (1.0 - ex1) / d1;
+  double dVa2 = 2.0 * alphaoh * deoh * ex2 * (1.0 - ex2) / d2;
+  double dVb = -phh2 * Vb / dh;
   double efac = exp(-PS_B1 * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (int j = 1; j < PS_NTERM; ++j) {
     int I = c_idx[j][0], J = c_idx[j][1], K = c_idx[j][2];
-    double cz = c_c5z[j];
+    double cz = pes[j];
     double fI1 = f1[I], fJ2 = f2[J], fJ1 = f1[J], fI2 = f2[I], fK3 = f3[K];
     s0 = s0 + cz * (fI1 * fJ2 + fJ1 * fI2) * fK3;
     s1 = s1 + cz * ((double)(I - 1) * f1[I - 1] * fJ2 + (double)(J - 1) * f1[J - 1] * fI2) * fK3;
     s2 = s2 + cz * ((double)(J - 1) * fI1 * f2[J - 1] + (double)(I - 1) * fJ1 * f2[I - 1]) * fK3;
     s3 = s3 + cz * (fI1 * fJ2 + fJ1 * fI2) * (double)(K - 1) * f3[K - 1];
   }
-  double Vc = 2.0 * c_c5z[0] + efac * s0;
+  double Vc = 2.0 * pes[0] + efac * s0;
   double e = Va + Vb + Vc;
   e = e + 0.44739574026257;
   e1 = e * 0.00285914375100642899;

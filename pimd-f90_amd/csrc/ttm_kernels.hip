@@ -21,20 +21,28 @@
 
 namespace pimd {
 
-__constant__ double c_c5z[PS_NTERM];
 __constant__ int c_idx[PS_NTERM][3];
 __constant__ double c_coefd[PS_NDMS];
 __constant__ int c_idxd[PS_NDMS][3];
 
 // c5z = f5z*c5zA + fbasis*cbasis + fcore*ccore + frest*crest (nasa.f90:45).
+// The reference's PES parameters in the order of the pes arrays.
+void pes_default_params(double *theta) {
+  for (int j = 0; j < PS_NTERM; ++j)
+    theta[j] = PS_F5Z * PS_C5ZA[j] + PS_FBASIS * PS_CBASIS[j] + PS_FCORE * PS_CCORE[j] +
+               PS_FREST * PS_CREST[j];
+  theta[PES_DEOHA] = PS_DEOHA;
+  theta[PES_ALPHAOH] = PS_ALPHAOH;
+  theta[PES_ROH] = PS_ROH;
+  theta[PES_PHH1A] = PS_PHH1A;
+  theta[PES_PHH2] = PS_PHH2;
+}
+
 void set_constant_tables() {
   static bool done = false;
   if (done) return;
-  double c5z[PS_NTERM];
   int idx[PS_NTERM][3], idxd[PS_NDMS][3];
   for (int j = 0; j < PS_NTERM; ++j) {
-    c5z[j] = PS_F5Z * PS_C5ZA[j] + PS_FBASIS * PS_CBASIS[j] + PS_FCORE * PS_CCORE[j] +
-             PS_FREST * PS_CREST[j];
     idx[j][0] = PS_IDX1[j];
     idx[j][1] = PS_IDX2[j];
     idx[j][2] = PS_IDX3[j];
@@ -44,7 +52,6 @@ void set_constant_tables() {
     idxd[j][1] = PS_IDXD2[j];
     idxd[j][2] = PS_IDXD3[j];
   }
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_c5z), c5z, sizeof c5z);
   (void)hipMemcpyToSymbol(HIP_SYMBOL(c_idx), idx, sizeof idx);
   (void)hipMemcpyToSymbol(HIP_SYMBOL(c_coefd), PS_COEFD, sizeof(double) * PS_NDMS);
   (void)hipMemcpyToSymbol(HIP_SYMBOL(c_idxd), idxd, sizeof idxd);
@@ -83,7 +90,8 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
                                                const double *__restrict__ RRc, Site *site,
                                                double *Rcs, double *grdq, double *phi,
                                                double *Efq, double *dR, double *tally,
-                                               double *chg_out, double *radius) {
+                                               double *chg_out, double *radius,
+                                               const double *__restrict__ pes) {
   int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
   const int b = gid / c.N, w = gid % c.N, N = c.N;
@@ -122,7 +130,7 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
       for (int k = 0; k < 3; ++k) Rcs[(w * 4 + t) * 3 + k] = cen[t][k];
 
   double e1, dr1[3][3], q3[3], gq[3][3][3];
-  ps_monomer(c, r, e1, dr1, q3, gq);
+  ps_monomer(c, r, e1, dr1, q3, gq, pes);
   for (int a = 0; a < 3; ++a) q3[a] = q3[a] * CHARGECON;
   if (chg_out)
     for (int a = 0; a < 3; ++a) chg_out[(size_t)b * 3 * N + 3 * w + a] = q3[a];
@@ -211,7 +219,7 @@ void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const doubl
                     double *chg_out, hipStream_t s) {
   int n = c.P * c.N;
   hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.site, b.Rcs,
-                     b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius);
+                     b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius, b.pes);
 }
 
 // ---------------------------------------------------------------------------

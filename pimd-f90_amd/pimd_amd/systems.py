@@ -170,3 +170,46 @@ def make_system(nwaters: int, nbeads: int, temp: float = 300.0, seed: int = 0, b
     RRt, PPt = ring_polymer(cen, nbeads, temp=temp, seed=seed + 7919)
     RRc = RRt.mean(axis=0)
     return RRt, PPt, RRc, L
+
+
+def _monomer(rng: np.random.Generator, roh_sd: float, ang_sd_deg: float) -> np.ndarray:
+    """A monomer about its oxygen with Gaussian-distorted bonds and angle,
+    randomly oriented: rows O, H1, H2."""
+    r1, r2 = OH_BOND + roh_sd * rng.standard_normal(2)
+    th = np.deg2rad(HOH_ANGLE_DEG + ang_sd_deg * rng.standard_normal())
+    m = np.array([[0.0, 0.0, 0.0], [r1, 0.0, 0.0], [r2 * np.cos(th), r2 * np.sin(th), 0.0]])
+    return m @ _random_rotation(rng).T
+
+
+def water_cluster(nwaters: int = 5, seed: int = 0, oo_min: float = 2.6, oo_max: float = 3.2,
+                  roh_sd: float = 0.02, ang_sd_deg: float = 3.0) -> np.ndarray:
+    """A gas-phase cluster: each new oxygen sits oo_min..oo_max from a random
+    earlier one and no closer than oo_min to any; monomers randomly oriented
+    with distorted geometry.  Returns (3*nwaters, 3) coordinates, O H H per
+    molecule, centred on the origin (for a large periodic box)."""
+    rng = np.random.default_rng(seed)
+    oxy = [np.zeros(3)]
+    while len(oxy) < nwaters:
+        base = oxy[rng.integers(len(oxy))]
+        d = rng.standard_normal(3)
+        cand = base + rng.uniform(oo_min, oo_max) * d / np.linalg.norm(d)
+        if all(np.linalg.norm(cand - o) >= oo_min for o in oxy):
+            oxy.append(cand)
+    oxy = np.array(oxy) - np.mean(oxy, axis=0)
+    out = np.zeros((3 * nwaters, 3))
+    for i, o in enumerate(oxy):
+        out[3 * i:3 * i + 3] = o + _monomer(rng, roh_sd, ang_sd_deg)
+    return out
+
+
+def distorted_box(nwaters: int, seed: int = 0, roh_sd: float = 0.02,
+                  ang_sd_deg: float = 3.0) -> tuple[np.ndarray, float]:
+    """The periodic water box of the benchmark construction with each monomer's
+    bonds and angle distorted as in water_cluster: (coordinates, box edge)."""
+    rng = np.random.default_rng(seed + 7919)
+    cen, L = water_box(nwaters, seed=seed)
+    out = cen.copy()
+    for i in range(nwaters):
+        o = cen[3 * i]
+        out[3 * i:3 * i + 3] = o + _monomer(rng, roh_sd, ang_sd_deg)
+    return out, L

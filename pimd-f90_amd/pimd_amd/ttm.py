@@ -34,8 +34,11 @@ EXPORTS = (
     "pimd_nose_hoover", "pimd_md_init", "pimd_state_set", "pimd_state_get", "pimd_md_run",
     "pimd_profile_enable", "pimd_profile_count", "pimd_profile_get", "pimd_profile_reset",
     "pimd_sync", "pimd_batch_create", "pimd_batch_destroy", "pimd_batch_autotune",
-    "pimd_batch_shape", "pimd_batch_md_run",
+    "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
+    "pimd_pes_param_grad",
 )
+
+PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
 
 
 class PimdParams(C.Structure):
@@ -98,6 +101,9 @@ def load_library(path: str = LIB_PATH):
         "pimd_batch_autotune": [vp, dp],
         "pimd_batch_shape": [vp, C.c_int, ip],
         "pimd_batch_md_run": [vp, C.c_int, C.c_int, C.POINTER(C.c_uint64), dp, dp],
+        "pimd_pes_params_get": [vp, dp],
+        "pimd_pes_params_set": [vp, dp],
+        "pimd_pes_param_grad": [vp, dp, dp, dp, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -193,6 +199,27 @@ class TTMForceField:
         t = np.zeros((self.params.nbeads, 5))
         _check(load_library().pimd_energy_terms(self._h, _ptr(t)))
         return t
+
+    def pes_params(self) -> np.ndarray:
+        """The monomer PES parameters in use (combined c5z, deohA, alphaoh, roh, phh1A, phh2)."""
+        th = np.zeros(PES_NPARAM)
+        _check(load_library().pimd_pes_params_get(self._h, _ptr(th)))
+        return th
+
+    def set_pes_params(self, theta):
+        th = _arr(theta, (PES_NPARAM,))
+        _check(load_library().pimd_pes_params_set(self._h, _ptr(th)))
+        if self._single is not None:
+            self._single.set_pes_params(th)
+
+    def pes_param_grad(self, RRt, ew, gw) -> np.ndarray:
+        """Adjoint of the PES parameters: sum_b ew_b dE_b/dtheta + sum gw . d(dE/dr)/dtheta,
+        for configurations RRt[P][Natoms][3] (see pimd_pes_param_grad)."""
+        P, na = self.params.nbeads, self.params.natoms
+        r, e, g = _arr(RRt, (P, na, 3)), _arr(ew, (P,)), _arr(gw, (P, na, 3))
+        out = np.zeros(PES_NPARAM)
+        _check(load_library().pimd_pes_param_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(out)))
+        return out
 
     def potential(self, RR, RRc, t: int = 1) -> dict:
         """One bead (force_calc.f90:112-170): the reference's per-rank call."""

@@ -205,3 +205,22 @@ def test_langevin_coefficients_closed_form(oracle_lib):
     gam = np.where(np.arange(P) == 0, 1 / 0.1, 2 * nm["omegalist"])
     assert np.max(np.abs(c1 - np.exp(-gam * dt / 2))) < 1e-15
     assert np.max(np.abs(c2 ** 2 + c1 ** 2 * KB_AMUA2PS2PERK * P * T - KB_AMUA2PS2PERK * P * T)) < 1e-9
+
+
+def test_pes_parameter_form(oracle_lib):
+    """The PES restatement with explicit parameters reproduces the reference
+    PES at the reference values and is affine in the 245 coefficients."""
+    rng = np.random.default_rng(5)
+    box = np.array([30.0, 30.0, 30.0])
+    th0 = oracle_lib.pes_default()
+    for _ in range(5):
+        r1 = np.array([[0, 0, 0], [0.96, 0.02, 0], [-0.25, 0.93, 0.03]]) + 0.03 * rng.standard_normal((3, 3))
+        e_ref, d_ref, _, _ = oracle_lib.monomer(r1, box)
+        e, d = oracle_lib.pes_monomer(r1, box, th0)
+        assert e == e_ref and np.array_equal(d, d_ref)
+        v = np.zeros(250)
+        v[:245] = 1e-3 * rng.standard_normal(245) * np.abs(th0[:245])
+        e1, d1 = oracle_lib.pes_monomer(r1, box, th0 + v)
+        e2, d2 = oracle_lib.pes_monomer(r1, box, th0 + 2 * v)
+        assert abs((e2 - e1) - (e1 - e)) < 1e-9 * max(1.0, abs(e2 - e))
+        assert np.max(np.abs((d2 - d1) - (d1 - d))) < 1e-9 * max(1.0, np.max(np.abs(d2 - d)))
