@@ -255,7 +255,8 @@ def run_replicas(args):
         "batched_autotuned": {"value": round(work / s_tuned, 1),
                               "ms_per_step": round(1000 * s_tuned, 3)},
         "batch_over_sequential": round(s_seq / s_tuned, 3),
-        "autotuned_shapes": {"fields": ["pair_group", "dd_group", "field_cg", "sfac_sites"],
+        "autotuned_shapes": {"fields": ["qq_group", "dd_group", "final_group", "field_cg",
+                                        "sfac_sites"],
                              "per_replica": shapes[:8], "force_ms": [round(x, 4) for x in tune_ms[:8]]},
         "replicas": specs[:8],
     }

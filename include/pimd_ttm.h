@@ -171,13 +171,14 @@ int pimd_md_run(pimd_ctx *ctx, int t0, int nsteps, const double *noise, uint64_t
 typedef struct pimd_batch pimd_batch;
 int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out);
 int pimd_batch_destroy(pimd_batch *batch);
-/* Picks each replica's launch shapes (lanes per molecule of the pair kernels,
- * of the dipole kernel, column groups of the field kernels, sites per
- * structure-factor chunk) by timing force evaluations of its loaded
- * coordinates; ms[n] (may be NULL) receives the best time per replica.  The
- * MD state, including the dipole predictor history, is left as it was. */
+/* Picks each replica's launch shapes (lanes per molecule of the charge pair
+ * kernel, of the dipole kernel and of the final pair kernel, column groups
+ * of the field kernels, sites per structure-factor chunk) by timing force
+ * evaluations of its loaded coordinates; ms[n] (may be NULL) receives the
+ * best time per replica.  The MD state, including the dipole predictor
+ * history, is left as it was. */
 int pimd_batch_autotune(pimd_batch *batch, double *ms);
-/* The four launch-shape values of replica i, in the order above. */
+/* The five launch-shape values of replica i (int[5]), in the order above. */
 int pimd_batch_shape(pimd_batch *batch, int i, int *shape);
 /* nsteps PIMD_VV steps of every replica from MD step t0 (device normals,
  * seeds[n]); Upot[n][nsteps] and uk[n][nsteps] (may be NULL). */

@@ -422,6 +422,8 @@ int alloc_system(pimd_ctx *x) {
   // phase tables of 64 sites, or the 16-group reduction if larger (in double2)
   c.fd_tab_d2 = std::max(c.T * 64, 16 * 4 * 64 / 2);
   c.g_dd = env_int("PIMD_G_DD", 16, 0, 64);
+  c.g_qq = env_int("PIMD_G_QQ", 8, 0, 32);
+  c.g_fin = env_int("PIMD_G_FIN", 8, 0, 32);
   c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
   c.overlap = env_int("PIMD_SOR_OVERLAP", 0, 0, 1);
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
@@ -988,15 +990,17 @@ struct pimd_batch {
 // the LDS-staged dipole kernel, column groups of the field kernels and sites
 // per structure-factor chunk.
 struct Shape {
-  int group, g_dd, fd_cg, sf_sites;
+  int g_qq, g_dd, g_fin, fd_cg, sf_sites;
 };
+constexpr int NSHAPE = 5;
 
 static Shape get_shape(const pimd_ctx *x) {
-  return Shape{x->group, x->c.g_dd, x->c.fd_cg, x->c.sf_sites};
+  return Shape{x->c.g_qq, x->c.g_dd, x->c.g_fin, x->c.fd_cg, x->c.sf_sites};
 }
 static void set_shape(pimd_ctx *x, const Shape &sh) {
-  x->group = sh.group;
+  x->c.g_qq = sh.g_qq;
   x->c.g_dd = sh.g_dd;
+  x->c.g_fin = sh.g_fin;
   x->c.fd_cg = sh.fd_cg;
   x->c.sf_sites = sh.sf_sites;
 }
@@ -1047,12 +1051,14 @@ static int autotune_ctx(pimd_ctx *x, double *best_ms) {
   Shape best = get_shape(x);
   double tbest = 0.0;
   if (rc == PIMD_OK) rc = time_forces(x, scratch, e0, e1, 2, tbest);
-  const int groups[] = {16, 32, 64}, gdd[] = {16, 32, 64}, cgs[] = {4, 8, 16}, sites[] = {16, 32, 64};
-  for (int knob = 0; knob < 4 && rc == PIMD_OK; ++knob) {
-    const int *vals = knob == 0 ? groups : knob == 1 ? gdd : knob == 2 ? cgs : sites;
+  // candidate values per knob, in the order of Shape
+  const int cand[NSHAPE][3] = {{4, 8, 16}, {16, 32, 64}, {4, 8, 16}, {4, 8, 16}, {16, 32, 64}};
+  for (int knob = 0; knob < NSHAPE && rc == PIMD_OK; ++knob) {
+    const int *vals = cand[knob];
     for (int v = 0; v < 3 && rc == PIMD_OK; ++v) {
       Shape sh = best;
-      int &field = knob == 0 ? sh.group : knob == 1 ? sh.g_dd : knob == 2 ? sh.fd_cg : sh.sf_sites;
+      int *fields[NSHAPE] = {&sh.g_qq, &sh.g_dd, &sh.g_fin, &sh.fd_cg, &sh.sf_sites};
+      int &field = *fields[knob];
       if (field == vals[v]) continue;
       field = vals[v];
       set_shape(x, sh);
@@ -1113,10 +1119,11 @@ extern "C" int pimd_batch_autotune(pimd_batch *b, double *ms) {
 extern "C" int pimd_batch_shape(pimd_batch *b, int i, int *shape) {
   if (!b || !shape || i < 0 || i >= (int)b->ctx.size()) return fail(PIMD_EINVAL, "bad argument");
   const Shape sh = get_shape(b->ctx[i]);
-  shape[0] = sh.group;
+  shape[0] = sh.g_qq;
   shape[1] = sh.g_dd;
-  shape[2] = sh.fd_cg;
-  shape[3] = sh.sf_sites;
+  shape[2] = sh.g_fin;
+  shape[3] = sh.fd_cg;
+  shape[4] = sh.sf_sites;
   return PIMD_OK;
 }
 

@@ -52,6 +52,7 @@ struct SysConst {
   int fd_tab_d2;           // double2 slots of the field kernel's table / reduction area
   int sf_kind;             // structure-factor kernel: 0 one k per lane, 1 kz-columns
   int g_dd;                // lanes per molecule of the LDS-staged dipole kernel (0: gather kernel)
+  int g_qq, g_fin;         // the same for the charge and final pair kernels
   int poll;                // 1: read the active-bead count back after every SOR iteration
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   double box[3], boxi[3], vol;

@@ -280,7 +280,10 @@ void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
 // R is the type of the pair arithmetic: double for the FP64 form; float for
 // the mixed form, where site-site vectors and cut-off tests stay FP64 and each
 // pair's contributions are accumulated in FP64.
-template <int MODEL, int G, typename R>
+// LDS = true: a block holds 256/G molecules of one bead (grid (molecule
+// blocks, beads)) and first copies the bead's molecule records into LDS, so
+// neighbour sites are read from LDS instead of gathered from L2.
+template <int MODEL, int G, typename R, bool LDS = false>
 __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restrict__ site,
                                                 const double *__restrict__ Rcs,
                                                 const int *__restrict__ nbr,
@@ -288,16 +291,32 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
                                                 const int *__restrict__ nbrn, double *phi,
                                                 double *Efq, double *dR, double *tally) {
   constexpr int SA0 = MODEL == 2 ? 0 : 1;  // TTM3-F: O carries neither charge nor dipole
-  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long grp = gtid / G;
-  const int lane = (int)(gtid % G);
+  extern __shared__ Site lds_site[];
   const int N = c.N;
-  const bool valid = grp < (long)c.P * N;
+  long grp;
+  int lane;
+  bool valid;
+  if (LDS) {
+    const int bb = blockIdx.y;
+    for (int e = threadIdx.x; e < 4 * N; e += blockDim.x) lds_site[e] = site[(size_t)bb * 4 * N + e];
+    __syncthreads();
+    const int aa = blockIdx.x * (256 / G) + threadIdx.x / G;
+    lane = threadIdx.x % G;
+    valid = aa < N;
+    grp = (long)bb * N + (valid ? aa : 0);
+  } else {
+    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    grp = gtid / G;
+    lane = (int)(gtid % G);
+    valid = grp < (long)c.P * N;
+  }
   const int b = valid ? (int)(grp / N) : 0, a = valid ? (int)(grp % N) : 0;
+  // molecule records of this bead: LDS or global
+  const Site *bead_sites = LDS ? lds_site : site + (size_t)b * N * 4;
   Site A[4];
   double RcA[4][3];
   for (int t = 0; t < 4; ++t) {
-    A[t] = site[((size_t)b * N + a) * 4 + t];
+    A[t] = bead_sites[a * 4 + t];
     for (int k = 0; k < 3; ++k) RcA[t][k] = Rcs[(a * 4 + t) * 3 + k];
   }
   double ph[4] = {0, 0, 0, 0}, ef[4][3] = {{0}}, dro[3] = {0, 0, 0};
@@ -308,7 +327,7 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
     constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
     Site B[4];
-    for (int t = 0; t < 4; ++t) B[t] = site[((size_t)b * N + j) * 4 + t];
+    for (int t = 0; t < 4; ++t) B[t] = bead_sites[j * 4 + t];
     // vdW between oxygens
     double r[3] = {A[0].x - B[0].x, A[0].y - B[0].y, A[0].z - B[0].z};
     minimg(c, r[0], r[1], r[2]);
@@ -387,6 +406,10 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
   }
 }
 
+// Bytes of a bead's molecule records staged by the LDS pair kernels.
+static size_t bead_sites_lds(const SysConst &c) { return sizeof(Site) * 4 * c.N; }
+constexpr size_t PAIR_LDS_MAX = 150 * 1024;
+
 template <int G, typename R>
 static void launch_qq_g(const SysConst &c, DevBufs &b, hipStream_t s) {
   long threads = (long)c.P * c.N * G;
@@ -399,8 +422,27 @@ static void launch_qq_g(const SysConst &c, DevBufs &b, hipStream_t s) {
                        b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
 }
 
+template <int G, typename R>
+static void launch_qq_lds_g(const SysConst &c, DevBufs &b, hipStream_t s) {
+  dim3 grid((c.N + 256 / G - 1) / (256 / G), c.P);
+  const size_t lds = bead_sites_lds(c);
+  if (c.model == 2)
+    hipLaunchKernelGGL((k_pair_qq<2, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
+                       b.nbr, b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
+  else
+    hipLaunchKernelGGL((k_pair_qq<3, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
+                       b.nbr, b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
+}
+
 template <typename R>
 static void launch_qq_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  if (c.g_qq > 0 && bead_sites_lds(c) <= PAIR_LDS_MAX) {
+    if (c.g_qq == 4) launch_qq_lds_g<4, R>(c, b, s);
+    else if (c.g_qq == 8) launch_qq_lds_g<8, R>(c, b, s);
+    else if (c.g_qq == 32) launch_qq_lds_g<32, R>(c, b, s);
+    else launch_qq_lds_g<16, R>(c, b, s);
+    return;
+  }
   if (group == 64)
     launch_qq_g<64, R>(c, b, s);
   else if (group == 32)
@@ -1340,7 +1382,8 @@ __device__ __forceinline__ bool is_dip(int t) {
   return MODEL == 2 ? t < 3 : t == 3;
 }
 
-template <int MODEL, int G, typename R>
+// LDS = true: as k_pair_qq, with the bead's dipoles staged beside the sites.
+template <int MODEL, int G, typename R, bool LDS = false>
 __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__restrict__ site,
                                                    const double *__restrict__ Rcs,
                                                    const double *__restrict__ dip,
@@ -1350,18 +1393,38 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
                                                    double *dR, double *tally) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
-  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long grp = gtid / G;
-  const int lane = (int)(gtid % G);
+  extern __shared__ Site lds_site[];  // [4N] sites, then [N][ND][3] dipoles
+  double *lds_dip = reinterpret_cast<double *>(lds_site + 4 * c.N);
   const int N = c.N;
-  const bool valid = grp < (long)c.P * N;
+  long grp;
+  int lane;
+  bool valid;
+  if (LDS) {
+    const int bb = blockIdx.y;
+    for (int e = threadIdx.x; e < 4 * N; e += blockDim.x) lds_site[e] = site[(size_t)bb * 4 * N + e];
+    for (int e = threadIdx.x; e < N * ND * 3; e += blockDim.x)
+      lds_dip[e] = dip[(size_t)bb * N * ND * 3 + e];
+    __syncthreads();
+    const int aa = blockIdx.x * (256 / G) + threadIdx.x / G;
+    lane = threadIdx.x % G;
+    valid = aa < N;
+    grp = (long)bb * N + (valid ? aa : 0);
+  } else {
+    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    grp = gtid / G;
+    lane = (int)(gtid % G);
+    valid = grp < (long)c.P * N;
+  }
   const int b = valid ? (int)(grp / N) : 0, a = valid ? (int)(grp % N) : 0;
+  // molecule records and dipoles of this bead: LDS or global
+  const Site *bead_sites = LDS ? lds_site : site + (size_t)b * N * 4;
+  const double *bead_dip = LDS ? lds_dip : dip + (size_t)b * N * ND * 3;
   Site A[4];
   double mua[4][3];
   for (int t = 0; t < 4; ++t) {
-    A[t] = site[((size_t)b * N + a) * 4 + t];
+    A[t] = bead_sites[a * 4 + t];
     for (int k = 0; k < 3; ++k)
-      mua[t][k] = is_dip<MODEL>(t) ? dip[(((size_t)b * N + a) * ND + (t - T0)) * 3 + k] : 0.0;
+      mua[t][k] = is_dip<MODEL>(t) ? bead_dip[((size_t)a * ND + (t - T0)) * 3 + k] : 0.0;
   }
   double ph[4] = {0, 0, 0, 0}, fr[4][3] = {{0}}, vc = 0.0, vt[9] = {0};
   const int nf = valid ? nbrc[grp] : 0, nn = valid ? nbrn[grp] : 0;
@@ -1372,9 +1435,9 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
     Site B[4];
     double mub[4][3];
     for (int t = 0; t < 4; ++t) {
-      B[t] = site[((size_t)b * N + j) * 4 + t];
+      B[t] = bead_sites[j * 4 + t];
       for (int k = 0; k < 3; ++k)
-        mub[t][k] = is_dip<MODEL>(t) ? dip[(((size_t)b * N + j) * ND + (t - T0)) * 3 + k] : 0.0;
+        mub[t][k] = is_dip<MODEL>(t) ? bead_dip[((size_t)j * ND + (t - T0)) * 3 + k] : 0.0;
     }
 #pragma unroll
     for (int sa = 0; sa < 4; ++sa) {
@@ -1478,8 +1541,27 @@ static void launch_final_g(const SysConst &c, DevBufs &b, hipStream_t s) {
                        b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
 }
 
+template <int G, typename R>
+static void launch_final_lds_g(const SysConst &c, DevBufs &b, size_t lds, hipStream_t s) {
+  dim3 grid((c.N + 256 / G - 1) / (256 / G), c.P);
+  if (c.model == 2)
+    hipLaunchKernelGGL((k_pair_final<2, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
+                       b.dip, b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
+  else
+    hipLaunchKernelGGL((k_pair_final<3, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
+                       b.dip, b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
+}
+
 template <typename R>
 static void launch_final_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
+  const size_t lds = bead_sites_lds(c) + sizeof(double) * c.N * c.nd * 3;
+  if (c.g_fin > 0 && lds <= PAIR_LDS_MAX) {
+    if (c.g_fin == 4) launch_final_lds_g<4, R>(c, b, lds, s);
+    else if (c.g_fin == 8) launch_final_lds_g<8, R>(c, b, lds, s);
+    else if (c.g_fin == 32) launch_final_lds_g<32, R>(c, b, lds, s);
+    else launch_final_lds_g<16, R>(c, b, lds, s);
+    return;
+  }
   if (group == 64)
     launch_final_g<64, R>(c, b, s);
   else if (group == 32)

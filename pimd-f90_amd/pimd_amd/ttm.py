@@ -360,7 +360,7 @@ class ReplicaBatch:
     def shapes(self) -> list:
         out = []
         for i in range(len(self.integrators)):
-            sh = (C.c_int * 4)()
+            sh = (C.c_int * 5)()
             _check(load_library().pimd_batch_shape(self._h, i, sh))
             out.append(tuple(sh))
         return out
