@@ -128,6 +128,8 @@ def run_form(p, RRt, PPt, RRc, args):
     from pimd_amd.ttm import PIMDIntegrator, TTMForceField, load_library
     ff = TTMForceField(p)
     info = ff.info()
+    if args.copy_mib > 0:
+        ff.profile_copy(args.copy_mib << 20)
     integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=1,
                            centroid_thermostat=True, tau_centroid=0.1)
     # start from consistent forces (the reference's first step kicks with zero)
@@ -145,6 +147,7 @@ def run_form(p, RRt, PPt, RRc, args):
     out = {"ms_per_step": ms, "value": p.nwaters * p.nbeads / (ms / 1000.0),
            "ns_per_day": (1000.0 / ms) * 0.25e-6 * 86400.0, "f0": f0, "rec": rec,
            "info": info, "kernels": {}, "roofline": None}
+    out["force_only"] = force_only(ff, integ, p, args.steps)
     if args.no_profile:
         return out
     ff.profile(True)
@@ -165,12 +168,76 @@ def run_form(p, RRt, PPt, RRc, args):
         peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
         avg_s = tms / cnt / 1000.0
         ach = fl / avg_s / 1e12
+        traffic = measured_traffic(args.workload, "mixed" if p.precision == 1 else "fp64", dom)
         out["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak,
-                           "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
+                           "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic,
                            "kernel": dom, "arith": "fp32" if fp32 else "fp64",
-                           "avg_launch_ms": round(tms / cnt, 4), "flops_per_launch": fl}
+                           "avg_launch_ms": round(tms / cnt, 4), "flops_per_launch": fl,
+                           "traffic_source": "profiles/hbm_traffic_h2o512_p32_ttm3f.md"}
     ff.close()
     return out
+
+
+def force_only(ff, integ, p, steps):
+    """The all-beads force call alone (full_bead_forces equivalent) on the
+    device-resident coordinates of the last MD step, dipole predictor off
+    (every call starts the SOR solve from alpha E_q)."""
+    from pimd_amd.ttm import load_library
+    L = load_library()
+    integ.state_forces(1)
+    L.pimd_sync(ff.handle)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        integ.state_forces(1)
+    L.pimd_sync(ff.handle)
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    return {"ms_per_call": round(ms, 4), "value": round(p.nwaters * p.nbeads / (ms / 1000.0), 1),
+            "unit": "molecule-bead force evals/s", "predictor": "off"}
+
+
+def cpu_parallel_baseline(p, RRt, RRc, threads=16, budget_s=10.0):
+    """The C restatement over beads in parallel threads (the reference's
+    MPI-over-beads decomposition on the host cores), on as many beads of the
+    same system as fit the budget."""
+    import concurrent.futures
+    import oracle
+    from pimd_amd.params import TTMParams
+    threads = max(1, min(threads, os.cpu_count() or 1, p.nbeads))
+    one = TTMParams(**{**p.__dict__, "nbeads": 1})
+    systems = [oracle.System(one) for _ in range(threads)]
+    t0 = time.perf_counter()
+    systems[0].forces(RRt[:1], RRc)
+    per_bead = time.perf_counter() - t0
+    nb = int(min(p.nbeads, max(threads, threads * budget_s // max(per_bead, 1e-3))))
+
+    def work(i):
+        sysm = systems[i % threads]
+        for b in range(i, nb, threads):
+            sysm.forces(RRt[b:b + 1], RRc)
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(threads)))
+    dt = time.perf_counter() - t0
+    return {"value": p.nwaters * nb / dt, "unit": "molecule-bead force evals/s", "cores": threads,
+            "kind": "port",
+            "sample": f"C restatement (gcc -O2) on {nb} of the {p.nbeads} beads, one bead per "
+                      f"thread at a time, {threads} threads, {dt:.1f} s"}
+
+
+def measured_traffic(workload, form, key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_h2o512_p32_ttm3f.md")
+    if workload != "h2o512_p32_ttm3f" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == form and cells[2] == key:
+            try:
+                return float(cells[5])
+            except ValueError:
+                return None
+    return None
 
 
 def _kern(res, steps):
@@ -274,22 +341,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="first run one device-to-device copy of this many MiB (a kernel of "
-                         "known HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+                    help="run one device-to-device copy of this many MiB (k_copy_probe, a kernel "
+                         "of known HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the force path "
               "shards by beads or replicas, which this build does not launch.", file=sys.stderr)
         sys.exit(2)
-    if args.copy_mib > 0:
-        import torch
-        n = args.copy_mib * (1 << 20) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
-
     if args.workload == REPLICAS64:
         print(json.dumps(run_replicas(args)), flush=True)
         return
@@ -305,6 +363,7 @@ def main():
     r = res[main_form]
 
     cpu = None if args.no_cpu else cpu_baseline(p, RRt, RRc)
+    cpu_par = None if args.no_cpu else cpu_parallel_baseline(p, RRt, RRc)
     out = {
         "metric": "PIMD ns/day + molecule-bead force evals/s, 512 H2O P=32 TTM3-F, mixed vs FP64",
         "value": round(r["value"], 1),
@@ -321,6 +380,8 @@ def main():
                    "guess_initdip": p.guess_initdip},
         "roofline": r["roofline"],
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_par,
+        "force_only": r["force_only"],
         "kernels_ms_per_step": _kern(r, args.steps),
         "sor_iterations_per_step": (round(r["kernels"]["field_d_sor"][0] / max(args.steps, 1), 2)
                                     if "field_d_sor" in r["kernels"] else None),
@@ -330,6 +391,7 @@ def main():
         df = m["f0"]["dRRt"] - a["f0"]["dRRt"]
         out["fp64"] = {"value": round(a["value"], 1), "ms_per_step": round(a["ms_per_step"], 4),
                        "ns_per_day": round(a["ns_per_day"], 3), "roofline": a["roofline"],
+                       "force_only": a["force_only"],
                        "kernels_ms_per_step": _kern(a, args.steps)}
         out["mixed_over_fp64"] = round(m["value"] / a["value"], 3)
         out["mixed_error_vs_fp64"] = {

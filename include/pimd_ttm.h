@@ -155,6 +155,9 @@ int pimd_md_init(pimd_ctx *ctx, const pimd_md_params *md);
 int pimd_state_set(pimd_ctx *ctx, const double *RRt, const double *PPt, const double *dRRt);
 /* Copies the device state out (any may be NULL); RRc [Natoms][3]. */
 int pimd_state_get(pimd_ctx *ctx, double *RRt, double *PPt, double *dRRt, double *RRc);
+/* One force evaluation (full_bead_forces, force_calc.f90:26-106) on the
+ * device-resident state coordinates; the gradient replaces the state's. */
+int pimd_state_forces(pimd_ctx *ctx, int t);
 /* Runs nsteps PIMD_VV steps on the device starting at MD step t0.
  * noise: NULL (device RNG keyed by seed) or [nsteps][2][Natoms][P][3]
  * standard normals for the two Langevin calls of each step.
@@ -211,6 +214,10 @@ int pimd_profile_count(pimd_ctx *ctx);
 int pimd_profile_get(pimd_ctx *ctx, int i, char *name, size_t len, long *launches,
                      double *total_ms);
 int pimd_profile_reset(pimd_ctx *ctx);
+/* A plain device-to-device copy of `bytes` on the context stream, a kernel
+ * of known HBM traffic (bytes read, bytes written) for calibrating
+ * counter-based byte counts; ms receives its duration. */
+int pimd_profile_copy(pimd_ctx *ctx, size_t bytes, double *ms);
 /* Blocks until all work queued on the context stream is done. */
 int pimd_sync(pimd_ctx *ctx);
 

@@ -497,7 +497,9 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *
     // No host round trip: converged beads are switched off on the device
     // and every kernel of the loop skips them, so the remaining iterations
     // of a converged system are near-empty launches.
-    if (c.poll && it < c.maxiter) {
+    // (with per-kernel timing on, poll so only live iterations are launched
+    // and timed)
+    if ((c.poll || x->prof) && it < c.maxiter) {
       HIPCHK(hipMemcpyAsync(x->h_nactive + it, d.nactive + it, sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (x->h_nactive[it] == 0) break;
@@ -1189,6 +1191,49 @@ extern "C" int pimd_pes_param_grad(pimd_ctx *x, const double *RRt, const double 
   HIPCHK(hipMemcpyAsync(grad, dgrad, sizeof(double) * PES_NPARAM, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return PIMD_OK;
+}
+
+extern "C" int pimd_state_forces(pimd_ctx *x, int t) {
+  int rc = md_check(x);
+  if (rc != PIMD_OK) return rc;
+  HIPCHK(hipSetDevice(x->p.device));
+  DevBufs &d = x->d;
+  return run_forces(x, d.RR, d.RRc, t, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
+                    d.iters_out);
+}
+
+__global__ void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+extern "C" int pimd_profile_copy(pimd_ctx *x, size_t bytes, double *ms) {
+  if (!x || bytes < 32) return fail(PIMD_EINVAL, "need a context and at least 32 bytes");
+  HIPCHK(hipSetDevice(x->p.device));
+  const size_t n = bytes / 32;
+  double4 *a = nullptr, *b = nullptr;
+  int rc = dalloc(&a, n);
+  if (rc == PIMD_OK) rc = dalloc(&b, n);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == PIMD_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
+  if (rc == PIMD_OK) {
+    hipStream_t s = x->stream;
+    (void)hipMemsetAsync(a, 0, n * 32, s);
+    (void)hipEventRecord(e0, s);
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, a, b, n);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(PIMD_EDEVICE, "copy probe failed");
+    float f = 0.f;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    if (ms) *ms = f;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 extern "C" {

@@ -576,6 +576,7 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
   double are[KZ], aim[KZ];
 #pragma unroll
   for (int i = 0; i < KZ; ++i) are[i] = aim[i] = 0.0;
+  #pragma unroll 2
   for (int e = 0; e < nt; ++e) {
     const double2 x = lds_sf[rowx * scp + e];
     double2 y = lds_sf[rowy * scp + e];

@@ -35,7 +35,7 @@ EXPORTS = (
     "pimd_profile_enable", "pimd_profile_count", "pimd_profile_get", "pimd_profile_reset",
     "pimd_sync", "pimd_batch_create", "pimd_batch_destroy", "pimd_batch_autotune",
     "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
-    "pimd_pes_param_grad",
+    "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -104,6 +104,8 @@ def load_library(path: str = LIB_PATH):
         "pimd_pes_params_get": [vp, dp],
         "pimd_pes_params_set": [vp, dp],
         "pimd_pes_param_grad": [vp, dp, dp, dp, dp],
+        "pimd_state_forces": [vp, C.c_int],
+        "pimd_profile_copy": [vp, C.c_size_t, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -193,6 +195,12 @@ class TTMForceField:
         out["virt"] = np.transpose(out["virt_cm"], (0, 2, 1)).copy()
         out["virialt"] = np.trace(out["virt"], axis1=1, axis2=2)
         return out
+
+    def profile_copy(self, nbytes: int) -> float:
+        """A device copy of nbytes (known HBM traffic, for counter calibration); ms."""
+        ms = C.c_double(0.0)
+        _check(load_library().pimd_profile_copy(self._h, int(nbytes), C.byref(ms)))
+        return ms.value
 
     def energy_terms(self) -> np.ndarray:
         """[P][5]: Umon, Uvdw, Uelec, Uind, Uvdw_lrc of the last call."""
@@ -316,6 +324,10 @@ class PIMDIntegrator:
         _check(load_library().pimd_state_get(self.ff.handle, _ptr(o["RRt"]), _ptr(o["PPt"]),
                                              _ptr(o["dRRt"]), _ptr(o["RRc"])))
         return o
+
+    def state_forces(self, t: int = 1):
+        """Forces of the device-resident state coordinates (not synchronised)."""
+        _check(load_library().pimd_state_forces(self.ff.handle, int(t)))
 
     def run(self, nsteps: int, t0: int = 1, noise=None, seed: int = 0) -> dict:
         nz = None
