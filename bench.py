@@ -240,6 +240,76 @@ def measured_traffic(workload, form, key):
     return None
 
 
+def spring_energy(RRt, masses, P, temp):
+    """Ring-polymer spring energy sum_b sum_i m_i omega_n^2 |r_b - r_(b+1)|^2 / 2 in
+    kcal/mol, omega_n = kB T P / hbar (the frequency the normal-mode propagator
+    uses, NormalModes.f90:125-153)."""
+    from pimd_amd.systems import HBAR, KB_AMUA2PS2PERK, MASSCON
+    wn = KB_AMUA2PS2PERK * temp * P / HBAR
+    d = RRt - np.roll(RRt, -1, axis=0)
+    return float(0.5 * wn * wn * np.sum(masses[None, :, None] * d * d) / MASSCON)
+
+
+def accuracy_report(p, RRt, PPt, RRc, nsteps=2000, every=100):
+    """The mixed form against FP64 over a trajectory: (1) an FP64 Langevin
+    trajectory of nsteps steps whose configurations are replayed every `every`
+    steps through both forms (identical coordinates, predictor off), and (2)
+    the drift of the ring-polymer Hamiltonian in NVE runs of both forms from
+    the same state."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import MASSCON, atom_masses
+    from pimd_amd.ttm import PIMDIntegrator, TTMForceField
+    q64 = TTMParams(**{**p.__dict__, "precision": 0})
+    qmx = TTMParams(**{**p.__dict__, "precision": 1})
+    ff = TTMForceField(q64)
+    ev64, evmx = TTMForceField(q64), TTMForceField(qmx)
+    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=1,
+                           centroid_thermostat=True, tau_centroid=0.1)
+    f0 = ff.full_bead_forces(RRt, RRc)
+    integ.set_state(RRt, PPt, f0["dRRt"])
+    frms, fmax, erel = [], [], []
+    t = 1
+    for _ in range(nsteps // every):
+        integ.run(every, t0=t, seed=7)
+        t += every
+        st = integ.get_state()
+        a = ev64.full_bead_forces(st["RRt"], st["RRc"])
+        m = evmx.full_bead_forces(st["RRt"], st["RRc"])
+        d = m["dRRt"] - a["dRRt"]
+        frms.append(float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(a["dRRt"] ** 2))))
+        fmax.append(float(np.max(np.abs(d)) / np.max(np.abs(a["dRRt"]))))
+        # per molecule: a bead's total energy (monomer strain minus cohesion) can pass near zero
+        erel.append(float(np.max(np.abs(m["Upott"] - a["Upott"])) / p.nwaters))
+    masses = atom_masses(p.nwaters, p.mass_o, p.mass_h)
+    drift = {}
+    for name, q in (("fp64", q64), ("mixed", qmx)):
+        f = TTMForceField(q)
+        it = PIMDIntegrator(f, delt_fs=0.25, temp=300.0, bead_thermostat=0)
+        g = f.full_bead_forces(RRt, RRc)
+        it.set_state(RRt, PPt, g["dRRt"])
+        H, times = [], []
+        chunk = 50
+        for c in range(nsteps // chunk):
+            rec = it.run(chunk, t0=1 + c * chunk, seed=0)
+            st = it.get_state()
+            H.append(rec["uk"][-1] / MASSCON + spring_energy(st["RRt"], masses, p.nbeads, 300.0)
+                     + rec["Upot"][-1])
+            times.append((c + 1) * chunk * 0.25e-3)
+        H = np.array(H)
+        slope = float(np.polyfit(np.array(times), H, 1)[0])
+        drift[name] = {"kcal_mol_per_ps": round(slope, 5),
+                       "per_molecule_bead_kcal_mol_per_ps": round(slope / (p.nwaters * p.nbeads), 9),
+                       "H_first": round(float(H[0]), 4), "H_last": round(float(H[-1]), 4),
+                       "H_rms_fluct": round(float(np.std(H)), 5)}
+    return {"replayed_configurations": len(frms), "trajectory_steps": nsteps,
+            "force_rms_rel_max": max(frms), "force_max_rel_max": max(fmax),
+            "energy_abs_max_kcal_mol_per_molecule": max(erel),
+            "nve_drift": drift,
+            "note": "FP64 Langevin trajectory replayed through both forms on identical "
+                    "coordinates; NVE runs from the same state, H = kinetic + springs + "
+                    "sum of bead potentials"}
+
+
 def _kern(res, steps):
     return {k: round(v[1] / max(steps, 1), 4) for k, v in res["kernels"].items()}
 
@@ -340,6 +410,8 @@ def main():
                     help="form(s) to time; with both, value is the mixed form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=2000,
+                    help="length of the mixed-vs-FP64 trajectory report (0 skips it)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (k_copy_probe, a kernel "
                          "of known HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -386,6 +458,8 @@ def main():
         "sor_iterations_per_step": (round(r["kernels"]["field_d_sor"][0] / max(args.steps, 1), 2)
                                     if "field_d_sor" in r["kernels"] else None),
     }
+    if len(forms) == 2 and args.accuracy_steps > 0:
+        out["mixed_accuracy"] = accuracy_report(p, RRt, PPt, RRc, nsteps=args.accuracy_steps)
     if len(forms) == 2:
         a, m = res["fp64"], res["mixed"]
         df = m["f0"]["dRRt"] - a["f0"]["dRRt"]
