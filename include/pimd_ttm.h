@@ -55,7 +55,8 @@ typedef struct {
                          rank per bead                                        */
   int precision;      /* PIMD_FP64 or PIMD_MIXED                             */
   int device;         /* HIP device ordinal                                  */
-  int reserved0;
+  int contraction;    /* 1: pot_ttm's CONTRACTION branch, no monomer PES energy
+                         or gradient (pot_ttm.f90:92-132); 0 otherwise       */
   double box[3];      /* Angstrom                                            */
   double rc;          /* real-space cut-off (Angstrom)                       */
   double rc1;         /* vdW switch start (TTM3-F shift, potential.f90:107)  */
@@ -142,7 +143,10 @@ typedef struct {
   int centroid_thermostat;
   double tau_centroid;   /* ps                                               */
   int global_nh;         /* global Nose-Hoover chain (MD.f90:43-46)          */
-  int reserved0;
+  int intra_timesteps;   /* 0: PIMD_VV (MD.f90:35-110); n >= 1: contracted_MD
+                            (MD.f90:276-449), monomer forces on the beads in n
+                            inner steps of delt/n, TTM forces of the centroid
+                            once per step                                     */
   double tau;            /* ps, global thermostat                            */
   double setNMfreq;      /* cm^-1, 0 = RPMD                                  */
 } pimd_md_params;
@@ -158,11 +162,17 @@ int pimd_state_get(pimd_ctx *ctx, double *RRt, double *PPt, double *dRRt, double
 /* One force evaluation (full_bead_forces, force_calc.f90:26-106) on the
  * device-resident state coordinates; the gradient replaces the state's. */
 int pimd_state_forces(pimd_ctx *ctx, int t);
-/* Runs nsteps PIMD_VV steps on the device starting at MD step t0.
+/* Runs nsteps PIMD_VV steps (contracted_MD steps when intra_timesteps >= 1)
+ * on the device starting at MD step t0.
  * noise: NULL (device RNG keyed by seed) or [nsteps][2][Natoms][P][3]
  * standard normals for the two Langevin calls of each step.
  * Per step outputs (any may be NULL): Upot[nsteps] = sum over beads,
- * virial[nsteps], virialc[nsteps] (force_calc.f90:94-97), uk[nsteps]. */
+ * virial[nsteps], virialc[nsteps] (force_calc.f90:94-97), uk[nsteps].
+ * Contracted: Upot sums the monomer energies of every bead and P times the
+ * centroid's energy, virial = monomer virial + trace of the centroid's,
+ * virialc = monomer centroid virial / P + the centroid's (MD.f90:391-418).
+ * The fast gradient starts at zero at pimd_md_init and carries over between
+ * runs. */
 int pimd_md_run(pimd_ctx *ctx, int t0, int nsteps, const double *noise, uint64_t seed,
                 double *Upot, double *virial, double *virialc, double *uk);
 

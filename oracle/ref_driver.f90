@@ -13,6 +13,7 @@
 !   mode 6  funcs      erfc/gammq2_3/PBCsmear*/self* (math.f90, smear.f90)
 !   mode 7  setup      init_pot + ewald_set tables (potential.f90:43-199)
 !   mode 8  monomer    pot_nasa + dms_nasa         (nasa.f90:2-293)
+!   mode 9  contracted contracted_MD step order    (MD.f90:276-449)
 program ref_driver
 use consts
 use system_mod
@@ -38,6 +39,9 @@ double precision, allocatable :: xs(:), drsq(:), apol(:), hist(:,:,:,:), noise(:
 integer, allocatable :: seedst(:)
 integer :: chain2, LNH, per_bead_hist, nseed
 logical :: have_hist
+double precision, allocatable :: dRRfast(:,:,:), Upostep(:)
+double precision :: deltfast, delt2fast, virialmon, virialcmon, roh1(3), roh2(3)
+integer :: intra, tintra, iw, iO, iH1, iH2
 
 call get_command_argument(1, fin)
 call get_command_argument(2, fout)
@@ -210,6 +214,115 @@ case (5)
    write(11) RRt, PPt, dRRt, RRc
    if (THERMOSTAT) write(11) vxi_global
    if (BEADTHERMOSTAT .and. ih(12) == 2) write(11) vxi_beads
+
+case (9)
+   ! contracted_MD (MD.f90:276-449): fast intramolecular forces on the beads
+   ! in intra inner steps, intermolecular forces of the centroid with
+   ! CONTRACTION on (pot_ttm.f90:92-132) every outer step.  dRRfast is kept
+   ! from one outer step to the next (the reference's automatic array is set
+   ! to zero only at t = 1, MD.f90:293).
+   call init_pot
+   allocate(RRt(3,Natoms,nb), PPt(3,Natoms,nb), dRRt(3,Natoms,nb), RRc(3,Natoms), PPc(3,Natoms))
+   allocate(dRRfast(3,Natoms,nb), Upostep(nb))
+   read(10) RRt
+   read(10) PPt
+   read(10) dRRt
+   intra = ih(16)
+   temp = dh(12); delt = dh(11)/1000.d0; delt2 = delt/2.d0; setNMfreq = dh(14)
+   deltfast = delt/intra; delt2fast = deltfast/2.d0
+   tau_centroid = dh(13); tau = dh(15)
+   imassO = 1.d0/massO; imassH = 1.d0/massH
+   omegan = KB_amuA2ps2perK*temp*Nbeads/hbar
+   nsteps = ih(8)
+   BEADTHERMOSTAT = (ih(12) /= 0)
+   CENTROIDTHERMOSTAT = (ih(9) /= 0)
+   THERMOSTAT = .false.
+   bead_thermostat_type = 'none'
+   if (ih(12) == 1) bead_thermostat_type = 'Langevin'
+   s = 1.d0; sbead = 1.d0
+   call InitNormalModes(Nbeads, omegan, deltfast, setNMfreq, 6)
+   if (ih(12) == 1) call Init_Langevin_NM(delt2, CENTROIDTHERMOSTAT, tau_centroid, Nbeads, Nbeads*temp)
+   call seed_rng(ih(10))
+   n = 3*Natoms*Nbeads
+   allocate(noise(n), dRR(3,Natoms), dipI(3,Nwaters), dipE(3,Nwaters), chgb(Natoms))
+   dRRfast = 0.d0
+   call calc_uk
+   do it = 1, nsteps
+      Upostep = 0.d0
+      if (BEADTHERMOSTAT) then
+         call record_noise(noise, n)
+         call bead_thermostat
+         write(11) noise
+      endif
+      PPt = PPt - MASSCON*dRRt*delt2
+      RRc = sum(RRt,3)/Nbeads
+      PPc = sum(PPt,3)/Nbeads
+      call PBCs(RRt, RRc)
+      do tintra = 1, intra
+         PPt = PPt - MASSCON*dRRfast*delt2fast
+         if (nb > 1) then
+            do ia = 1, Nwaters
+               call EvolveRing(RRt(:,3*ia-2,:), PPt(:,3*ia-2,:), Nbeads, massO)
+               call EvolveRing(RRt(:,3*ia-1,:), PPt(:,3*ia-1,:), Nbeads, massH)
+               call EvolveRing(RRt(:,3*ia-0,:), PPt(:,3*ia-0,:), Nbeads, massH)
+            enddo
+         else
+            do ia = 1, Nwaters
+               RRt(:,3*ia-2,1) = RRt(:,3*ia-2,1) + imassO*PPt(:,3*ia-2,1)*deltfast
+               RRt(:,3*ia-1,1) = RRt(:,3*ia-1,1) + imassH*PPt(:,3*ia-1,1)*deltfast
+               RRt(:,3*ia-0,1) = RRt(:,3*ia-0,1) + imassH*PPt(:,3*ia-0,1)*deltfast
+            enddo
+         endif
+         virialmon = 0.d0
+         virialcmon = 0.d0
+         do ib = 1, nb
+            do iw = 1, Nwaters
+               iO = 3*iw-2; iH1 = 3*iw-1; iH2 = 3*iw
+               r1(1:3,1:3) = RRt(1:3, (/iO, iH1, iH2/), ib)
+               call pot_nasa(r1, dr1, e1, box, boxi)
+               dRRfast(1:3, (/iO, iH1, iH2/), ib) = dr1
+               if (tintra == intra) then
+                  Upostep(ib) = Upostep(ib) + e1
+                  roh1 = RRc(1:3, iH1) - RRc(1:3, iO)
+                  roh1 = roh1 - box*anint(roh1*boxi)
+                  roh2 = RRc(1:3, iH2) - RRc(1:3, iO)
+                  roh2 = roh2 - box*anint(roh2*boxi)
+                  virialcmon = virialcmon + dot_product(roh1, dr1(:,2))
+                  virialcmon = virialcmon + dot_product(roh2, dr1(:,3))
+                  roh1 = RRt(1:3, iH1, ib) - RRt(1:3, iO, ib)
+                  roh1 = roh1 - box*anint(roh1*boxi)
+                  roh2 = RRt(1:3, iH2, ib) - RRt(1:3, iO, ib)
+                  roh2 = roh2 - box*anint(roh2*boxi)
+                  virialmon = virialmon + dot_product(roh1, dr1(:,2))
+                  virialmon = virialmon + dot_product(roh2, dr1(:,3))
+               endif
+            enddo
+         enddo
+         PPt = PPt - MASSCON*dRRfast*delt2fast
+      enddo
+      RRc = sum(RRt,3)/Nbeads
+      PPc = sum(PPt,3)/Nbeads
+      call PBCs(RRt, RRc)
+      CONTRACTION = .true.
+      call pot_ttm(RRc, RRc, En, dRR, vtmp, vc, dipI, dipE, chgb, it)
+      CONTRACTION = .false.
+      Upostep = Upostep + En
+      do ib = 1, nb
+         dRRt(:,:,ib) = dRR
+      enddo
+      write(11) sum(Upostep), virialmon + vtmp(1,1) + vtmp(2,2) + vtmp(3,3), &
+                virialcmon/Nbeads + vc
+      call calc_uk
+      PPt = PPt - MASSCON*dRRt*delt2
+      if (BEADTHERMOSTAT) then
+         call record_noise(noise, n)
+         call bead_thermostat
+         write(11) noise
+      endif
+      call calc_uk
+      write(11) uk
+   enddo
+   write(11) RRt, PPt, dRRt, RRc
 
 case (6)
    call init_pot

@@ -23,10 +23,10 @@ def available() -> bool:
 
 def _header(p, mode, t=1, nsteps=0, centroid=0, seed=1, nfunc=0, bead_tt=0, glob_t=0,
             nh_L=0, per_bead_hist=0, delt_fs=0.25, temp=300.0, tau_centroid=0.1, setnm=0.0,
-            tau=0.1, uk=0.0, vxi=(0.0, 0.0)):
+            tau=0.1, uk=0.0, vxi=(0.0, 0.0), intra=0):
     ih = np.zeros(16, dtype=np.int32)
-    ih[:15] = [mode, p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip), t,
-               nsteps, centroid, seed, nfunc, bead_tt, glob_t, nh_L, per_bead_hist]
+    ih[:16] = [mode, p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip), t,
+               nsteps, centroid, seed, nfunc, bead_tt, glob_t, nh_L, per_bead_hist, intra]
     dh = np.zeros(32, dtype=np.float64)
     dh[0:3] = p.box
     dh[3:18] = [p.rc, p.rc1, p.eps_ewald, p.polar_sor, p.polar_eps, p.mass_o, p.mass_h, delt_fs,
@@ -185,3 +185,29 @@ def trajectory(p, RRt, PPt, dRRt, nsteps, temp, delt_fs, bead_thermostat=1, cent
     if global_nh:
         res["vxi_global"] = r.take(2)
     return res
+
+
+def contracted_trajectory(p, RRt, PPt, dRRt, nsteps, intra, temp, delt_fs, bead_thermostat=1,
+                          centroid=False, tau_centroid=0.1, seed=4242):
+    """contracted_MD steps (MD.f90:276-449): per step the recorded normals of
+    the two thermostat calls, (sum of bead energies, virial, centroid virial)
+    and the kinetic energy; then the final state."""
+    P, na = p.nbeads, p.natoms
+    blob = _header(p, 9, nsteps=nsteps, temp=temp, delt_fs=delt_fs, bead_tt=bead_thermostat,
+                   centroid=int(centroid), tau_centroid=tau_centroid, seed=seed, intra=intra)
+    for a in (RRt, PPt, dRRt):
+        blob += np.ascontiguousarray(a, dtype=np.float64).tobytes()
+    out, iters, _ = _run(blob)
+    r = _Reader(out)
+    steps = []
+    for _ in range(nsteps):
+        st = {}
+        if bead_thermostat == 1:
+            st["noise1"] = r.take(na, P, 3)
+        st["rec"] = r.take(3)
+        if bead_thermostat == 1:
+            st["noise2"] = r.take(na, P, 3)
+        st["uk"] = r.take()
+        steps.append(st)
+    return {"steps": steps, "RRt": r.take(P, na, 3), "PPt": r.take(P, na, 3),
+            "dRRt": r.take(P, na, 3), "RRc": r.take(na, 3), "iters": np.array(iters)}

@@ -50,6 +50,10 @@ struct pimd_ctx {
   SysConst c{};
   DevBufs d{};
   hipStream_t stream = nullptr;
+  bool stream_borrowed = false;  // the stream belongs to the context that owns this one
+  // contracted MD: the centroid's intermolecular forces (one bead, no
+  // monomer PES), computed on this context's stream
+  pimd_ctx *cen = nullptr;
   // second stream: the real-space dipole field of each SOR iteration runs on
   // it beside the reciprocal-space structure factor (both read the same
   // dipoles); fork/join through the two events
@@ -138,12 +142,17 @@ int dalloc(T **p, size_t n) {
 
 void free_all(pimd_ctx *x) {
   DevBufs &d = x->d;
+  if (x->cen) {
+    free_all(x->cen);
+    delete x->cen;
+    x->cen = nullptr;
+  }
   void *ptrs[] = {d.RR,   d.RRc,   d.site,   d.Rcs,    d.grdq,   d.phi,   d.Efq,   d.dR,
                   d.dip,  d.dip_prev, d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.nbrn,  d.tab,
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj,   x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -153,7 +162,7 @@ void free_all(pimd_ctx *x) {
     (void)hipEventDestroy(q.b);
   }
   for (auto e : x->pool) (void)hipEventDestroy(e);
-  if (x->stream) (void)hipStreamDestroy(x->stream);
+  if (x->stream && !x->stream_borrowed) (void)hipStreamDestroy(x->stream);
   if (x->stream2) (void)hipStreamDestroy(x->stream2);
   if (x->d_rec) (void)hipFree(x->d_rec);
   if (x->ev_fork) (void)hipEventDestroy(x->ev_fork);
@@ -264,6 +273,7 @@ int setup_system(pimd_ctx *x) {
   c.maxiter = p.polar_maxiter;
   c.guess = p.guess_initdip;
   c.mixed = p.precision == PIMD_MIXED ? 1 : 0;
+  c.contraction = p.contraction;
   c.sor = p.polar_sor;
   c.peps = p.polar_eps;
   c.stath = DEBYE / CHARGECON / std::sqrt((double)(3 * c.N));
@@ -552,6 +562,8 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
     return fail(PIMD_EINVAL, "cut-off %.6f exceeds half the box (%.6f)", p.rc, lmin / 2.0);
   if (!(p.eps_ewald > 0.0 && p.eps_ewald < 1.0)) return fail(PIMD_EINVAL, "eps_ewald must be in (0,1)");
   if (p.polar_maxiter < 1) return fail(PIMD_EINVAL, "polar_maxiter must be >= 1");
+  if (p.contraction != 0 && p.contraction != 1)
+    return fail(PIMD_EINVAL, "contraction must be 0 or 1, got %d", p.contraction);
   if (p.precision != PIMD_FP64 && p.precision != PIMD_MIXED)
     return fail(PIMD_EINVAL, "precision must be PIMD_FP64 (0) or PIMD_MIXED (1), got %d", p.precision);
   if (p.mass_o <= 0.0) p.mass_o = (double)f32(15.994);
@@ -800,12 +812,47 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
   if (!(md->delt_fs > 0.0) || !(md->temp > 0.0)) return fail(PIMD_EINVAL, "delt_fs and temp must be > 0");
   if (md->bead_thermostat != 0 && md->bead_thermostat != 1)
     return fail(PIMD_EINVAL, "bead_thermostat must be 0 (none) or 1 (Langevin)");
+  if (md->intra_timesteps < 0 || md->intra_timesteps > 1000)
+    return fail(PIMD_EINVAL, "intra_timesteps must be 0 (plain PIMD) or 1..1000, got %d",
+                md->intra_timesteps);
+  if (md->intra_timesteps > 0 && x->p.contraction)
+    return fail(PIMD_EINVAL, "contracted MD needs a context created with contraction = 0");
+  HIPCHK(hipSetDevice(x->p.device));
   x->md = *md;
   const int P = x->c.P;
   const double delt = md->delt_fs / 1000.0, delt2 = delt / 2.0;
   const double omegan = KB_AMU * md->temp * P / HBAR;  // InputOutput.f90:140
-  int rc = pimd_nm_init(x, omegan, delt, md->setNMfreq);
+  // contracted MD propagates the ring polymer with the fast step
+  // delt / intra_timesteps (InputOutput.f90:379-385)
+  const double delt_nm = md->intra_timesteps > 0 ? delt / md->intra_timesteps : delt;
+  int rc = pimd_nm_init(x, omegan, delt_nm, md->setNMfreq);
   if (rc != PIMD_OK) return rc;
+  if (x->cen) {
+    (void)hipStreamSynchronize(x->stream);
+    free_all(x->cen);
+    delete x->cen;
+    x->cen = nullptr;
+  }
+  if (md->intra_timesteps > 0) {
+    // the centroid's intermolecular forces: one bead, monomer PES off
+    // (pot_ttm.f90:92-132), same model, cut-offs, Ewald and SOR settings
+    pimd_params pc = x->p;
+    pc.nbeads = 1;
+    pc.contraction = 1;
+    pimd_ctx *cen = nullptr;
+    if ((rc = pimd_ctx_create(&pc, &cen)) != PIMD_OK) return rc;
+    (void)hipStreamDestroy(cen->stream);
+    cen->stream = x->stream;
+    cen->stream_borrowed = true;
+    x->cen = cen;
+    const size_t n = (size_t)P * x->natoms * 3;
+    if (!x->d.dRRfast) {
+      if ((rc = dalloc(&x->d.dRRfast, n)) != PIMD_OK) return rc;
+      if ((rc = dalloc(&x->d.mol, (size_t)P * x->c.N * 3)) != PIMD_OK) return rc;
+    }
+    // the fast forces start from zero (MD.f90:293)
+    HIPCHK(hipMemset(x->d.dRRfast, 0, sizeof(double) * n));
+  }
   if (md->bead_thermostat == 1) {
     rc = pimd_langevin_init(x, delt2, md->centroid_thermostat, md->tau_centroid, P * md->temp);
     if (rc != PIMD_OK) return rc;
@@ -898,7 +945,7 @@ static void md_begin(pimd_ctx *x) {
 // Enqueues PIMD_VV step st (MD step t0 + st, MD.f90:35-110) on the context's
 // stream and records Upot, virial, virialc, uk into record slot st.  Does not
 // synchronise.  noise: NULL or the normals of this step, [2][Natoms][P][3].
-static int md_step(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
+static int md_step_vv(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
   const int P = x->c.P, na = x->natoms;
   const size_t n = (size_t)P * na * 3;
   const pimd_md_params &md = x->md;
@@ -945,6 +992,83 @@ static int md_step(pimd_ctx *x, int t0, int st, const double *noise, uint64_t se
                      x->d_rec + (size_t)st * 4);
   HIPCHK(hipGetLastError());
   return PIMD_OK;
+}
+
+// Enqueues contracted_MD step st (MD step t0 + st, MD.f90:276-449): the
+// monomer PES moves every bead in intra_timesteps inner steps of the ring
+// polymer, the intermolecular forces of the centroid act on every bead once
+// per step.  Records sum of bead energies, virial, centroid virial and uk.
+// The fast gradient is kept from one step to the next.
+static int md_step_contracted(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
+  const int P = x->c.P, na = x->natoms, N = x->c.N;
+  const size_t n = (size_t)P * na * 3;
+  const pimd_md_params &md = x->md;
+  const int intra = md.intra_timesteps;
+  const double delt = md.delt_fs / 1000.0, delt2 = delt / 2.0;
+  const double deltfast = delt / intra, delt2fast = deltfast / 2.0;
+  const int L = 3 * na * P;
+  DevBufs &d = x->d;
+  pimd_ctx *cen = x->cen;
+  hipStream_t s = x->stream;
+  auto thermostat = [&](int half) {
+    if (md.bead_thermostat != 1) return;
+    const double *nz = nullptr;
+    if (noise) {
+      (void)hipMemcpyAsync(d.noise, noise + (size_t)half * n, sizeof(double) * n,
+                           hipMemcpyHostToDevice, s);
+      nz = d.noise;
+    }
+    x->timed("langevin", [&] {
+      launch_langevin(P, na, d, d.PP, nz, seed, (uint64_t)(t0 + st) * 2 + half, x->p.mass_o,
+                      x->p.mass_h, s);
+    });
+  };
+  auto centroid = [&] {
+    x->timed("centroid_pbc",
+             [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
+  };
+  thermostat(0);
+  if (md.global_nh) {
+    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
+    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
+  }
+  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+  centroid();
+  for (int ti = 1; ti <= intra; ++ti) {
+    x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
+    if (P > 1)
+      x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s); });
+    else
+      x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, deltfast, s); });
+    x->timed("pes_fast", [&] { launch_pes_fast(x->c, d, d.RR, d.RRc, d.dRRfast, d.mol, ti == intra, s); });
+    x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
+  }
+  centroid();
+  int rc = PIMD_OK;
+  DevBufs &dc = cen->d;
+  x->timed("centroid_forces", [&] {
+    rc = run_forces(cen, d.RRc, d.RRc, t0 + st, dc.Upot, dc.dRR, dc.virt, dc.virialc, dc.dipI,
+                    dc.dipE, dc.chg, dc.iters_out);
+  });
+  if (rc != PIMD_OK) return rc;
+  x->timed("broadcast", [&] { launch_broadcast_beads(P, na * 3, dc.dRR, d.dRR, s); });
+  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+  thermostat(1);
+  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  if (md.global_nh) {
+    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
+    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
+  }
+  launch_contract_record(P, N, d.mol, dc.Upot, dc.virt, dc.virialc, d.uk,
+                         x->d_rec + (size_t)st * 4, s);
+  HIPCHK(hipGetLastError());
+  return PIMD_OK;
+}
+
+static int md_step(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
+  return x->md.intra_timesteps > 0 ? md_step_contracted(x, t0, st, noise, seed)
+                                   : md_step_vv(x, t0, st, noise, seed);
 }
 
 // Copies nsteps records to the host arrays (any may be NULL); the stream must
@@ -1035,6 +1159,13 @@ static int time_forces(pimd_ctx *x, double *scratch, hipEvent_t e0, hipEvent_t e
 // restored around the trials.
 static int autotune_ctx(pimd_ctx *x, double *best_ms) {
   HIPCHK(hipSetDevice(x->p.device));
+  if (x->cen) {
+    // contracted MD: the force call of a step is the centroid's
+    const size_t nc = (size_t)x->natoms * 3;
+    HIPCHK(hipMemcpyAsync(x->cen->d.RR, x->d.RRc, sizeof(double) * nc, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(x->cen->d.RRc, x->d.RRc, sizeof(double) * nc, hipMemcpyDeviceToDevice, x->stream));
+    x = x->cen;
+  }
   DevBufs &d = x->d;
   const SysConst &c = x->c;
   const size_t nh = (size_t)c.P * 4 * c.N * c.nd * 3, ng = (size_t)c.P * x->natoms * 3;
@@ -1120,7 +1251,8 @@ extern "C" int pimd_batch_autotune(pimd_batch *b, double *ms) {
 
 extern "C" int pimd_batch_shape(pimd_batch *b, int i, int *shape) {
   if (!b || !shape || i < 0 || i >= (int)b->ctx.size()) return fail(PIMD_EINVAL, "bad argument");
-  const Shape sh = get_shape(b->ctx[i]);
+  const pimd_ctx *x = b->ctx[i];
+  const Shape sh = get_shape(x->cen ? x->cen : x);  // contracted: the centroid's force call
   shape[0] = sh.g_qq;
   shape[1] = sh.g_dd;
   shape[2] = sh.g_fin;

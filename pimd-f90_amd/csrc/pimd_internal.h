@@ -42,6 +42,7 @@ struct SysConst {
   int hmax, kmax, lmax;
   int maxiter, guess;
   int mixed;               // PIMD_MIXED: FP32 real-space pair arithmetic
+  int contraction;         // no monomer PES in the force call (pot_ttm.f90:92-132)
   int nbr_cap;
   int kzmax;               // longest kz-column
   int nseg, segmax;        // kz-columns cut into segments of at most segmax k
@@ -117,6 +118,7 @@ struct DevBufs {
   double *radius = nullptr;                     // [P][N] site radius about O
   double *pes = nullptr;                        // [PES_NPARAM] PES parameters
   double *adj = nullptr;                        // adjoint inputs and partial sums
+  double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
   double *red = nullptr;                        // scratch for reductions
 };
 
@@ -149,6 +151,12 @@ enum SfacMode { SF_Q = 0, SF_D = 1, SF_V = 2 };
 enum FieldMode { FD_Q = 0, FD_D = 1, FD_QD = 2 };
 
 // Ring polymer launchers (rp_kernels.hip).
+void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
+                     double *dRR, double *mol, bool record, hipStream_t s);
+void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStream_t s);
+void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
+                            const double *vtcen, const double *vccen, const double *uk,
+                            double *rec, hipStream_t s);
 void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s);
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,

@@ -220,6 +220,8 @@ __device__ __forceinline__ void dd3_apply(const R r[3], R ts1, R ts2, const R v[
 // the order (H1, H2, O), as the reference's gradq.
 // pes: the fitted PES parameters, [0, 245) the combined coefficients c5z of
 // nasa.f90:45, then deohA, alphaoh, roh, phh1A, phh2 (PES_* indices).
+// DMS = false: the potential energy surface only (q3, gq untouched).
+template <bool DMS = true>
 __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], double &e1,
                                   double dr1[3][3], double q3[3], double gq[3][3][3],
                                   const double *__restrict__ pes) {
@@ -288,6 +290,7 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
     dr1[2][k] = g2 * .00285914375100642899;
     dr1[0][k] = -(g1 + g2) * .00285914375100642899;
   }
+  if (!DMS) return;
   // ---- dipole moment surface
   const double ath0 = 1.82400520401572996557;
   const double B2A = 0.529177249;

@@ -131,6 +131,11 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
 
   double e1, dr1[3][3], q3[3], gq[3][3][3];
   ps_monomer(c, r, e1, dr1, q3, gq, pes);
+  if (c.contraction) {  // pot_ttm.f90:92-99: no monomer energy, gradient or virial
+    e1 = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 3; ++k) dr1[a][k] = 0.0;
+  }
   for (int a = 0; a < 3; ++a) q3[a] = q3[a] * CHARGECON;
   if (chg_out)
     for (int a = 0; a < 3; ++a) chg_out[(size_t)b * 3 * N + 3 * w + a] = q3[a];
@@ -220,6 +225,102 @@ void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const doubl
   int n = c.P * c.N;
   hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.site, b.Rcs,
                      b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius, b.pes);
+}
+
+// ---------------------------------------------------------------------------
+// Contracted MD (MD.f90:276-449).
+// ---------------------------------------------------------------------------
+
+// Fast (intramolecular) forces: the PES gradient of every monomer of every
+// bead into dRR (MD.f90:339-353).  With record, also per molecule the energy,
+// the bead virial and the centroid virial (MD.f90:355-372) into mol[P][N][3].
+__global__ __launch_bounds__(64) void k_pes_fast(SysConst c, const double *__restrict__ RR,
+                                                const double *__restrict__ RRc,
+                                                const double *__restrict__ pes, double *dRR,
+                                                double *mol, int record) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= c.P * c.N) return;
+  const int b = gid / c.N, w = gid % c.N, N = c.N;
+  const double *rr = RR + ((size_t)b * 3 * N + 3 * w) * 3;
+  double r[3][3], e1, dr1[3][3], q3[3], gq[3][3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) r[a][k] = rr[3 * a + k];
+  ps_monomer<false>(c, r, e1, dr1, q3, gq, pes);
+  double *out = dRR + ((size_t)b * 3 * N + 3 * w) * 3;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) out[3 * a + k] = dr1[a][k];
+  if (!record) return;
+  double vb = 0.0, vc = 0.0;
+  for (int h = 1; h <= 2; ++h) {
+    double rb[3], rcv[3];
+    for (int k = 0; k < 3; ++k) {
+      rcv[k] = RRc[(3 * w + h) * 3 + k] - RRc[(3 * w) * 3 + k];
+      rb[k] = r[h][k] - r[0][k];
+    }
+    minimg(c, rcv[0], rcv[1], rcv[2]);
+    minimg(c, rb[0], rb[1], rb[2]);
+    vc += rcv[0] * dr1[h][0] + rcv[1] * dr1[h][1] + rcv[2] * dr1[h][2];
+    vb += rb[0] * dr1[h][0] + rb[1] * dr1[h][1] + rb[2] * dr1[h][2];
+  }
+  double *m = mol + ((size_t)b * N + w) * 3;
+  m[0] = e1;
+  m[1] = vb;
+  m[2] = vc;
+}
+
+void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
+                     double *dRR, double *mol, bool record, hipStream_t s) {
+  const int n = c.P * c.N;
+  hipLaunchKernelGGL(k_pes_fast, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.pes, dRR,
+                     mol, record ? 1 : 0);
+}
+
+// The centroid's gradient for every bead (MD.f90:400-403).
+__global__ void k_broadcast_beads(int P, int n, const double *__restrict__ g, double *dRR) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)P * n) dRR[i] = g[i % n];
+}
+
+void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStream_t s) {
+  const size_t m = (size_t)P * n;
+  hipLaunchKernelGGL(k_broadcast_beads, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, P, n,
+                     g, dRR);
+}
+
+// Step record of a contracted step (MD.f90:405-418): sum of the bead
+// energies (monomers of every bead + P times the centroid energy), virial,
+// centroid virial, kinetic energy; molecule terms summed in a fixed order.
+__global__ __launch_bounds__(256) void k_contract_record(int P, int N,
+                                                        const double *__restrict__ mol,
+                                                        const double *__restrict__ Ucen,
+                                                        const double *__restrict__ vtcen,
+                                                        const double *__restrict__ vccen,
+                                                        const double *__restrict__ uk,
+                                                        double *rec) {
+  __shared__ double red[3][256];
+  double s[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < P * N; i += 256)
+    for (int k = 0; k < 3; ++k) s[k] += mol[(size_t)i * 3 + k];
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    rec[0] = red[0][0] + (double)P * Ucen[0];
+    rec[1] = red[1][0] + (vtcen[0] + vtcen[4] + vtcen[8]);
+    rec[2] = red[2][0] / (double)P + vccen[0];
+    rec[3] = uk[0];
+  }
+}
+
+void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
+                            const double *vtcen, const double *vccen, const double *uk,
+                            double *rec, hipStream_t s) {
+  hipLaunchKernelGGL(k_contract_record, dim3(1), dim3(256), 0, s, P, N, mol, Ucen, vtcen, vccen,
+                     uk, rec);
 }
 
 // ---------------------------------------------------------------------------

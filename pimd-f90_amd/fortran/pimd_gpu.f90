@@ -42,11 +42,13 @@ contains
     gpu_ready = c_associated(ctx_all)
   end function gpu_ready
 
+  ! precision (optional): PIMD_FP64 (default) or PIMD_MIXED
   subroutine gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
-                          polar_sor, polar_eps, guess_initdip, massO, massH)
+                          polar_sor, polar_eps, guess_initdip, massO, massH, precision)
     integer, intent(in) :: pot_model, Natoms, Nbeads, polar_maxiter
     double precision, intent(in) :: box(3), Rc, rc1, eps_ewald, polar_sor, polar_eps, massO, massH
     logical, intent(in) :: guess_initdip
+    integer, intent(in), optional :: precision
     call gpu_finalize()
     g_params%pot_model = pot_model
     g_params%nwaters = Natoms / 3
@@ -54,6 +56,7 @@ contains
     g_params%polar_maxiter = polar_maxiter
     g_params%guess_initdip = merge(1, 0, guess_initdip)
     g_params%precision = PIMD_FP64
+    if (present(precision)) g_params%precision = precision
     g_params%device = 0
     g_params%box = box
     g_params%rc = Rc

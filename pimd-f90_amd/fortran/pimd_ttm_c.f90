@@ -10,7 +10,8 @@ module pimd_ttm_c
 
   integer(c_int), parameter, public :: PIMD_OK = 0, PIMD_EINVAL = 1, PIMD_EDEVICE = 2, &
                                        PIMD_ENOMEM = 3, PIMD_ESTATE = 4
-  integer(c_int), parameter, public :: PIMD_TTM21F = 2, PIMD_TTM3F = 3, PIMD_FP64 = 0
+  integer(c_int), parameter, public :: PIMD_TTM21F = 2, PIMD_TTM3F = 3, PIMD_FP64 = 0, &
+                                       PIMD_MIXED = 1
 
   type, bind(C), public :: pimd_params
     integer(c_int) :: pot_model = 3
@@ -20,7 +21,7 @@ module pimd_ttm_c
     integer(c_int) :: guess_initdip = 0
     integer(c_int) :: precision = 0
     integer(c_int) :: device = 0
-    integer(c_int) :: reserved0 = 0
+    integer(c_int) :: contraction = 0
     real(c_double) :: box(3) = 0.d0
     real(c_double) :: rc = -1.d0
     real(c_double) :: rc1 = -1.d0
@@ -38,7 +39,7 @@ module pimd_ttm_c
     integer(c_int) :: centroid_thermostat = 0
     real(c_double) :: tau_centroid = real(0.1_c_float, c_double)
     integer(c_int) :: global_nh = 0
-    integer(c_int) :: reserved0 = 0
+    integer(c_int) :: intra_timesteps = 0
     real(c_double) :: tau = real(0.1_c_float, c_double)
     real(c_double) :: setNMfreq = 0.d0
   end type pimd_md_params

@@ -37,6 +37,7 @@ class TTMParams:
     mass_o: float = field(default_factory=lambda: f32(15.994))
     mass_h: float = field(default_factory=lambda: f32(1.008))
     precision: int = 0                  # 0 = FP64 (the reference's arithmetic), 1 = mixed
+    contraction: bool = False           # pot_ttm's CONTRACTION branch: no monomer PES
 
     def resolved(self) -> "TTMParams":
         """Copy with the -1 sentinels replaced by the reference's derived values."""

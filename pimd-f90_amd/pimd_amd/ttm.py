@@ -44,7 +44,7 @@ PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
 class PimdParams(C.Structure):
     _fields_ = [("pot_model", C.c_int), ("nwaters", C.c_int), ("nbeads", C.c_int),
                 ("polar_maxiter", C.c_int), ("guess_initdip", C.c_int), ("precision", C.c_int),
-                ("device", C.c_int), ("reserved0", C.c_int), ("box", C.c_double * 3),
+                ("device", C.c_int), ("contraction", C.c_int), ("box", C.c_double * 3),
                 ("rc", C.c_double), ("rc1", C.c_double), ("eps_ewald", C.c_double),
                 ("polar_sor", C.c_double), ("polar_eps", C.c_double), ("mass_o", C.c_double),
                 ("mass_h", C.c_double)]
@@ -53,7 +53,7 @@ class PimdParams(C.Structure):
 class MDParams(C.Structure):
     _fields_ = [("delt_fs", C.c_double), ("temp", C.c_double), ("bead_thermostat", C.c_int),
                 ("centroid_thermostat", C.c_int), ("tau_centroid", C.c_double),
-                ("global_nh", C.c_int), ("reserved0", C.c_int), ("tau", C.c_double),
+                ("global_nh", C.c_int), ("intra_timesteps", C.c_int), ("tau", C.c_double),
                 ("setNMfreq", C.c_double)]
 
 
@@ -141,7 +141,8 @@ class TTMForceField:
         self.params = p
         L = load_library()
         cp = PimdParams(p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip),
-                        int(p.precision), device, 0, (C.c_double * 3)(*p.box), p.rc, p.rc1, p.eps_ewald,
+                        int(p.precision), device, int(p.contraction), (C.c_double * 3)(*p.box), p.rc,
+                        p.rc1, p.eps_ewald,
                         p.polar_sor, p.polar_eps, p.mass_o, p.mass_h)
         h = C.c_void_p()
         _check(L.pimd_ctx_create(C.byref(cp), C.byref(h)))
@@ -304,10 +305,12 @@ class PIMDIntegrator:
 
     def __init__(self, ff: TTMForceField, delt_fs=0.25, temp=300.0, bead_thermostat=1,
                  centroid_thermostat=False, tau_centroid=0.1, global_nh=False, tau=0.1,
-                 setNMfreq=0.0):
+                 setNMfreq=0.0, intra_timesteps=0):
+        """intra_timesteps >= 1 runs contracted_MD (MD.f90:276-449) with that many
+        inner steps of the monomer forces per outer step."""
         self.ff = ff
         md = MDParams(delt_fs, temp, bead_thermostat, int(centroid_thermostat), tau_centroid,
-                      int(global_nh), 0, tau, setNMfreq)
+                      int(global_nh), int(intra_timesteps), tau, setNMfreq)
         _check(load_library().pimd_md_init(ff.handle, C.byref(md)))
         self.P, self.natoms = ff.params.nbeads, ff.params.natoms
 

@@ -114,6 +114,25 @@ def traj_fixture():
              **params_dict(p))
 
 
+def contracted_fixture():
+    """contracted_MD: monomer forces in inner steps, centroid TTM forces once per step."""
+    for name, model, N, P, intra, lat in [("contracted_ttm3f_n8_p4_intra3", 3, 8, 4, 3, (2, 2, 2)),
+                                          ("contracted_ttm21f_n8_p1_intra4", 2, 8, 1, 4, (2, 2, 2))]:
+        RRt, PPt, RRc, L = make_system(N, P, seed=70 + N + P, lattice=lat)
+        p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
+                      eps_ewald=1e-6, mass_o=15.995, mass_h=1.008).resolved()
+        dR0 = np.zeros_like(RRt)
+        r = refrun.contracted_trajectory(p, RRt, PPt, dR0, nsteps=5, intra=intra, temp=300.0,
+                                         delt_fs=0.5, bead_thermostat=1, centroid=True,
+                                         tau_centroid=0.05, seed=900 + P)
+        noise = np.array([[s["noise1"], s["noise2"]] for s in r["steps"]])
+        rec = np.array([s["rec"] for s in r["steps"]])
+        save(name, RRt=RRt, PPt=PPt, noise=noise, intra=np.asarray(intra),
+             Upot=rec[:, 0], virial=rec[:, 1], virialc=rec[:, 2],
+             uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
+             dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], **params_dict(p))
+
+
 def ring_fixtures():
     for P, setnm in [(8, 0.0), (7, 0.0), (6, 1500.0)]:
         N = 4
@@ -172,5 +191,6 @@ if __name__ == "__main__":
     force_fixtures()
     predictor_fixture()
     traj_fixture()
+    contracted_fixture()
     ring_fixtures()
     scalar_fixtures()

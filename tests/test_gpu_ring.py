@@ -104,3 +104,33 @@ def test_trajectory_vs_reference(name):
     assert errs["R"] < 1e-12 and errs["P"] < 1e-10 and errs["dR"] < 1e-8
     if "RRc_end" in g:
         assert max_abs(st["RRc"], g["RRc_end"]) < 1e-12
+
+
+@pytest.mark.parametrize("name", golden_names("contracted_"))
+def test_contracted_vs_reference(name):
+    """contracted_MD (MD.f90:276-449) against the reference's own step loop:
+    monomer forces on every bead in intra inner steps, the centroid's TTM
+    forces (CONTRACTION branch, pot_ttm.f90:92-132) broadcast to every bead,
+    driven by the normals the reference's thermostat drew."""
+    from pimd_amd.ttm import PIMDIntegrator
+    g = load_golden(name)
+    p = params_from(g)
+    nsteps = g["Upot"].shape[0]
+    print("case", name, flush=True)
+    ff = _ff(p)
+    integ = PIMDIntegrator(ff, delt_fs=0.5, temp=300.0, bead_thermostat=1,
+                           centroid_thermostat=True, tau_centroid=0.05,
+                           intra_timesteps=int(g["intra"]))
+    integ.set_state(g["RRt"], g["PPt"], None)
+    rec = integ.run(nsteps, t0=1, noise=g["noise"])
+    st = integ.get_state()
+    errs = {k: rel_err(rec[k], g[k]) for k in ("Upot", "virial", "virialc", "uk")}
+    errs.update({"R": rel_err(st["RRt"], g["RRt_end"]), "P": rel_err(st["PPt"], g["PPt_end"]),
+                 "dR": max_abs(st["dRRt"], g["dRRt_end"]), "Rc": max_abs(st["RRc"], g["RRc_end"])})
+    print(name, errs, flush=True)
+    assert errs["Upot"] < 1e-10 and errs["uk"] < 1e-10
+    assert errs["virial"] < 1e-9 and errs["virialc"] < 1e-9
+    assert errs["R"] < 1e-12 and errs["P"] < 1e-10 and errs["dR"] < 1e-8 and errs["Rc"] < 1e-12
+    # a second run continues from the state, the fast gradient included
+    rec2 = integ.run(1, t0=nsteps + 1)
+    assert np.all(np.isfinite(rec2["Upot"]))

@@ -4,6 +4,8 @@ The fixtures were produced by the reference Fortran routines built from source
 (tests/golden/gen_golden.py).  The restatement follows the reference's
 statement order, so agreement is expected to rounding (<= 1e-13 relative).
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -167,3 +169,94 @@ def test_trajectory_vs_reference(oracle_lib, name):
     assert rel_err(R, g["RRt_end"]) < 1e-12
     assert rel_err(Pm, g["PPt_end"]) < 1e-11
     assert max_abs(dR, g["dRRt_end"]) < 1e-10
+
+
+def _pes_all(oracle_lib, p, R):
+    """pot_nasa of every monomer of every bead: energies [P][N], gradients [P][Natoms][3]."""
+    box = np.asarray(p.box, dtype=np.float64)
+    e = np.zeros((R.shape[0], p.nwaters))
+    d = np.zeros_like(R)
+    for b in range(R.shape[0]):
+        for w in range(p.nwaters):
+            e[b, w], d[b, 3 * w:3 * w + 3], _, _ = oracle_lib.monomer(R[b, 3 * w:3 * w + 3], box)
+    return e, d
+
+
+def _oh_virial(p, R, d):
+    """Sum over molecules of roh1 . dr1(H1) + roh2 . dr1(H2) with minimum image."""
+    box = np.asarray(p.box)
+    v = 0.0
+    for w in range(p.nwaters):
+        for h in (1, 2):
+            r = R[3 * w + h] - R[3 * w]
+            r = r - box * np.round(r / box)
+            v += float(r @ d[3 * w + h])
+    return v
+
+
+def oracle_contracted(oracle_lib, p, RRt, PPt, nsteps, intra, temp, delt_fs, noise,
+                      tau_centroid=0.05):
+    """contracted_MD (MD.f90:276-449) composed from the oracle's pieces.  The
+    centroid's CONTRACTION forces (pot_ttm.f90:92-132) are the full TTM forces
+    of the centroid minus its monomer PES terms, which is what the branch
+    leaves out."""
+    P, na = p.nbeads, p.natoms
+    dt = delt_fs / 1000.0
+    dt2, dtf = dt / 2, dt / intra
+    dt2f = dtf / 2
+    nm = oracle_lib.nm_init(P, KB_AMUA2PS2PERK * temp * P / HBAR, dtf)
+    c1, c2 = oracle_lib.langevin_init(P, dt2, True, tau_centroid, P * temp, nm["omegalist"])
+    masses = atom_masses(p.nwaters, p.mass_o, p.mass_h)
+    cen = oracle_lib.System(dataclasses.replace(p, nbeads=1))
+    R, Pm = RRt.copy(), PPt.copy()
+    dR, dRf = np.zeros_like(R), np.zeros_like(R)
+
+    def centroid(R):
+        Rc = R.sum(axis=0) / P
+        sh = p.box[0] * np.round(Rc / p.box[0])
+        return R - sh[None], Rc - sh
+
+    out = []
+    for st in range(nsteps):
+        Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 0], Pm)
+        Pm = Pm - MASSCON * dR * dt2
+        R, Rc = centroid(R)
+        for ti in range(1, intra + 1):
+            Pm = Pm - MASSCON * dRf * dt2f
+            if P > 1:
+                R, Pm = oracle_lib.nm_evolve(nm, masses, R, Pm)
+            else:
+                R = R + (1.0 / masses)[None, :, None] * Pm * dtf
+            e1, dRf = _pes_all(oracle_lib, p, R)
+            if ti == intra:
+                umon = e1.sum()
+                vmon = sum(_oh_virial(p, R[b], dRf[b]) for b in range(P))
+                vcmon = sum(_oh_virial(p, Rc, dRf[b]) for b in range(P))
+            Pm = Pm - MASSCON * dRf * dt2f
+        R, Rc = centroid(R)
+        f = cen.forces(Rc[None], Rc, t=st + 1)
+        ec, dc = _pes_all(oracle_lib, p, Rc[None])
+        vc_mon = _oh_virial(p, Rc, dc[0])
+        ucen = f["En"][0] - ec.sum()
+        dR = np.broadcast_to(f["dRR"][0] - dc[0], R.shape).copy()
+        vir = vmon + np.trace(f["virt"][0]) - vc_mon
+        virc = vcmon / P + f["virialc"][0] - vc_mon
+        Pm = Pm - MASSCON * dR * dt2
+        Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 1], Pm)
+        uk = 0.5 * np.sum((1.0 / masses)[None, :, None] * Pm ** 2)
+        out.append((umon + P * ucen, vir, virc, uk))
+    return R, Pm, dR, Rc, np.array(out)
+
+
+@pytest.mark.parametrize("name", golden_names("contracted_"))
+def test_contracted_vs_reference(oracle_lib, name):
+    g = load_golden(name)
+    p = params_from(g)
+    R, Pm, dR, Rc, rec = oracle_contracted(oracle_lib, p, g["RRt"], g["PPt"], g["Upot"].shape[0],
+                                           int(g["intra"]), 300.0, 0.5, g["noise"])
+    for i, k in enumerate(("Upot", "virial", "virialc", "uk")):
+        assert rel_err(rec[:, i], g[k]) < 1e-11, k
+    assert rel_err(R, g["RRt_end"]) < 1e-12
+    assert rel_err(Pm, g["PPt_end"]) < 1e-11
+    assert max_abs(dR, g["dRRt_end"]) < 1e-10
+    assert max_abs(Rc, g["RRc_end"]) < 1e-12
