@@ -215,6 +215,24 @@ int pimd_pes_params_set(pimd_ctx *ctx, const double *theta);
 int pimd_pes_param_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
                         double *grad);
 
+/* ---- TTM model parameters and their derivatives ------------------------
+ * No counterpart in the reference, whose TTM2.1-F / TTM3-F parameters are
+ * constants (pot_mod.f90:7-26, selected in potential.f90:84-113).
+ * theta[PIMD_TTM_NPARAM]: vdwA, vdwB, vdwC, vdwD, vdwE, polarO, polarH,
+ * polarM, polfacO, polfacH, polfacM, aDD, aCCaCD, dms_param1, dms_param2,
+ * dms_param3.  A context starts with its model's values; later force calls
+ * use the values set.  The M-site fraction gammaM is not among them. */
+#define PIMD_TTM_NPARAM 16
+int pimd_ttm_params_get(pimd_ctx *ctx, double *theta);
+int pimd_ttm_params_set(pimd_ctx *ctx, const double *theta);
+/* The loss gradient of pimd_pes_param_grad with respect to the theta above,
+ * every configuration evaluated as a first MD step (no dipole predictor):
+ *   grad[j] = sum_b ew_b dE_b/dtheta_j + sum gw . d(dE_b/dr)/dtheta_j.
+ * E[P] and dEdtheta[P][PIMD_TTM_NPARAM] (either may be NULL) receive the
+ * energies and their derivatives. */
+int pimd_ttm_param_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
+                        double *grad, double *E, double *dEdtheta);
+
 /* ---- instrumentation --------------------------------------------------- */
 
 /* Enables per-kernel HIP-event timing on the context stream. */

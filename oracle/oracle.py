@@ -35,6 +35,8 @@ def lib():
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.POINTER(OrcParams)]
         L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_get_ttm_params.argtypes = [C.c_void_p, dp]
+        L.orc_set_ttm_params.argtypes = [C.c_void_p, dp]
         L.orc_nkvec.restype = C.c_int
         L.orc_nkvec.argtypes = [C.c_void_p]
         L.orc_consts.argtypes = [C.c_void_p, dp]
@@ -77,6 +79,15 @@ class System:
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_destroy(self.h)
+
+    def ttm_params(self):
+        """vdwA..vdwE, polarO/H/M, polfacO/H/M, aDD, aCCaCD, dms_param1..3."""
+        th = np.zeros(16)
+        lib().orc_get_ttm_params(self.h, th)
+        return th
+
+    def set_ttm_params(self, theta):
+        lib().orc_set_ttm_params(self.h, np.ascontiguousarray(theta, dtype=np.float64))
 
     def consts(self):
         o = np.zeros(11)

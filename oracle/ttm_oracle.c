@@ -32,6 +32,7 @@ struct orc_sys {
   double vdwA, vdwB, vdwC, vdwD, vdwE, aDD, aCC, polarO, polarH, polarM, gammaM;
   double dms1, dms2, dms3;
   double polfac[4], shiftA, shiftB, Uvdw_lrc, aewald, cts1, cts2, cts3;
+  double polfac_in[3]; /* polfacO, polfacH, polfacM as given (potential.f90:88,99) */
   int fd, ld, fdI, ldI; /* dipole site range [fd, ld] and type range */
   int hmax, kmax, lmax, nk;
   double *kvec, *Ak;
@@ -431,6 +432,37 @@ void orc_monomer(const double *r1, double *e1, double *dr1, double *q3, double *
 
 /* ---------------- set-up (potential.f90:43-199, ewald.f90:293-378) ---------------- */
 
+/* The vdW long-range correction and polfac = (polfac^(1/6))^-3
+ * (potential.f90:152-160) from the model parameters. */
+static void derive_params(orc_sys *s) {
+  const double Rc = s->p.rc;
+  s->Uvdw_lrc = TWOPI_ * (double)(s->N * s->N) / s->vol *
+                (s->vdwA / (9.0 * pow(Rc, 9)) + s->vdwB / (7.0 * pow(Rc, 7)) +
+                 s->vdwC / (3.0 * (Rc * Rc * Rc)));
+  double pf[4] = {pow(s->polfac_in[0], 1.0 / 6.0), pow(s->polfac_in[1], 1.0 / 6.0),
+                  pow(s->polfac_in[1], 1.0 / 6.0), pow(s->polfac_in[2], 1.0 / 6.0)};
+  for (int i = 0; i < 4; ++i) s->polfac[i] = 1.0 / (pf[i] * pf[i] * pf[i]);
+}
+
+/* Model parameters in the order vdwA..vdwE, polarO, polarH, polarM, polfacO,
+ * polfacH, polfacM, aDD, aCCaCD, dms_param1..3 (the values of pot_mod.f90:7-26
+ * unless set), for finite-difference checks of parameter derivatives. */
+void orc_get_ttm_params(const orc_sys *s, double *th) {
+  const double v[16] = {s->vdwA, s->vdwB, s->vdwC, s->vdwD, s->vdwE, s->polarO, s->polarH,
+                        s->polarM, s->polfac_in[0], s->polfac_in[1], s->polfac_in[2], s->aDD,
+                        s->aCC, s->dms1, s->dms2, s->dms3};
+  memcpy(th, v, sizeof v);
+}
+
+void orc_set_ttm_params(orc_sys *s, const double *th) {
+  s->vdwA = th[0]; s->vdwB = th[1]; s->vdwC = th[2]; s->vdwD = th[3]; s->vdwE = th[4];
+  s->polarO = th[5]; s->polarH = th[6]; s->polarM = th[7];
+  s->polfac_in[0] = th[8]; s->polfac_in[1] = th[9]; s->polfac_in[2] = th[10];
+  s->aDD = th[11]; s->aCC = th[12];
+  s->dms1 = th[13]; s->dms2 = th[14]; s->dms3 = th[15];
+  derive_params(s);
+}
+
 orc_sys *orc_create(const orc_params *p) {
   orc_sys *s = calloc(1, sizeof *s);
   s->p = *p;
@@ -472,12 +504,10 @@ orc_sys *orc_create(const orc_params *p) {
       s->shiftB = (9 * Rc - 7 * rc1) / (Rc8 * ((Rc - rc1) * (Rc - rc1) * (Rc - rc1)));
     }
   }
-  s->Uvdw_lrc = TWOPI_ * (double)(s->N * s->N) / s->vol *
-                (s->vdwA / (9.0 * pow(Rc, 9)) + s->vdwB / (7.0 * pow(Rc, 7)) +
-                 s->vdwC / (3.0 * (Rc * Rc * Rc)));
-  double pf[4] = {pow(polfacO, 1.0 / 6.0), pow(polfacH, 1.0 / 6.0), pow(polfacH, 1.0 / 6.0),
-                  pow(polfacM, 1.0 / 6.0)};
-  for (int i = 0; i < 4; ++i) s->polfac[i] = 1.0 / (pf[i] * pf[i] * pf[i]);
+  s->polfac_in[0] = polfacO;
+  s->polfac_in[1] = polfacH;
+  s->polfac_in[2] = polfacM;
+  derive_params(s);
   /* ewald_set(.true.), ewald.f90:308-377 */
   double PP = -log(p->eps_ewald);
   s->aewald = sqrt(PP) / Rc;

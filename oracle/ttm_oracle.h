@@ -22,6 +22,8 @@ typedef struct orc_sys orc_sys;
 
 orc_sys *orc_create(const orc_params *p);
 void orc_destroy(orc_sys *s);
+void orc_get_ttm_params(const orc_sys *s, double *theta16);
+void orc_set_ttm_params(orc_sys *s, const double *theta16);
 int orc_nkvec(const orc_sys *s);
 /* aewald, Uvdw_lrc, shiftA, shiftB, const_ts1..3, polfac[4] (10 doubles) */
 void orc_consts(const orc_sys *s, double *out);

@@ -11,33 +11,10 @@
 // r -> dEmon/dtheta(r), which equals w . d(dE/dr)/dtheta.  It is evaluated
 // exactly with dual numbers (value, derivative along w) through the closed
 // forms of dEmon/dtheta.
+#include "dual.h"
 #include "ttm_device.h"
 
 namespace pimd {
-
-// Dual number a + b eps with eps^2 = 0.
-struct Dual {
-  double v, d;
-};
-__device__ __forceinline__ Dual dmk(double v, double d = 0.0) { return Dual{v, d}; }
-__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual{a.v + b.v, a.d + b.d}; }
-__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual{a.v - b.v, a.d - b.d}; }
-__device__ __forceinline__ Dual operator*(Dual a, Dual b) {
-  return Dual{a.v * b.v, a.d * b.v + a.v * b.d};
-}
-__device__ __forceinline__ Dual operator*(double s, Dual a) { return Dual{s * a.v, s * a.d}; }
-__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
-  const double q = a.v / b.v;
-  return Dual{q, (a.d - q * b.d) / b.v};
-}
-__device__ __forceinline__ Dual dexp(Dual a) {
-  const double e = exp(a.v);
-  return Dual{e, e * a.d};
-}
-__device__ __forceinline__ Dual dsqrt(Dual a) {
-  const double r = sqrt(a.v);
-  return Dual{r, 0.5 * a.d / r};
-}
 
 constexpr double KCAL = 0.00285914375100642899;  // cm-1 -> kcal/mol (nasa.f90:101)
 constexpr int ADJ_BLOCK = 64;

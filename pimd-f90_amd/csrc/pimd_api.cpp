@@ -63,6 +63,7 @@ struct pimd_ctx {
   int natoms = 0;
   std::vector<double> kvec, Ak, p4vAk;
   double polfac[4] = {0, 0, 0, 0};
+  double ttm[TTM_NPARAM] = {};  // model parameters (vdW, polarisabilities, Thole, DMS)
   std::vector<int> hkl;
   // ring polymer
   bool nm_ready = false, lang_ready = false, md_ready = false, state_ready = false;
@@ -152,7 +153,7 @@ void free_all(pimd_ctx *x) {
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -171,6 +172,38 @@ void free_all(pimd_ctx *x) {
 
 float f32(double v) { return (float)v; }
 
+// The parameter-dependent constants of init_pot (potential.f90:84-113,
+// 152-160) from the context's theta: vdW terms and their long-range
+// correction, polarisabilities per site type (O, H, H, M), the Thole products
+// aCCaCD or aDD * polfac_i * polfac_j with polfac = (polfac^(1/6))^-3, and the
+// DMS corrections.
+void apply_ttm_params(pimd_ctx *x) {
+  SysConst &c = x->c;
+  const double *th = x->ttm, Rc = x->p.rc;
+  c.vdwA = th[TP_VDWA]; c.vdwB = th[TP_VDWB]; c.vdwC = th[TP_VDWC];
+  c.vdwD = th[TP_VDWD]; c.vdwE = th[TP_VDWE];
+  c.polar[0] = th[TP_POLARO]; c.polar[1] = th[TP_POLARH]; c.polar[2] = th[TP_POLARH];
+  c.polar[3] = th[TP_POLARM];
+  c.dms1 = th[TP_DMS1]; c.dms2 = th[TP_DMS2]; c.dms3 = th[TP_DMS3];
+  c.Uvdw_lrc = TWOPI * (double)(c.N * c.N) / c.vol *
+               (c.vdwA / (9.0 * std::pow(Rc, 9)) + c.vdwB / (7.0 * std::pow(Rc, 7)) +
+                c.vdwC / (3.0 * (Rc * Rc * Rc)));
+  const double pf6[4] = {std::pow(th[TP_POLFACO], 1.0 / 6.0), std::pow(th[TP_POLFACH], 1.0 / 6.0),
+                         std::pow(th[TP_POLFACH], 1.0 / 6.0), std::pow(th[TP_POLFACM], 1.0 / 6.0)};
+  const double aDD = th[TP_ADD], aCC = th[TP_ACC];
+  double pf[4];
+  for (int i = 0; i < 4; ++i) pf[i] = 1.0 / (pf6[i] * pf6[i] * pf6[i]);
+  for (int i = 0; i < 4; ++i) x->polfac[i] = pf[i];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      c.apolCC[i][j] = aCC * pf[i] * pf[j];
+      c.raCC[i][j] = std::pow(c.apolCC[i][j], 1.0 / 3.0);
+      c.apolDD[i][j] = aDD * pf[i] * pf[j];
+      c.apol3DD[i][j] = c.apolDD[i][j];
+    }
+  c.same_a = std::fabs(aCC - aDD) < 1e-12;
+}
+
 // init_pot + ewald_set (potential.f90:43-199, ewald.f90:293-378) on the host.
 int setup_system(pimd_ctx *x) {
   const pimd_params &p = x->p;
@@ -186,23 +219,21 @@ int setup_system(pimd_ctx *x) {
   const double Rc = p.rc, rc1 = p.rc1;
   c.rc2 = Rc * Rc;
   c.rc1 = rc1;
-  double pfO, pfH, pfM, aDD, aCC;
+  double *th = x->ttm;
   if (p.pot_model == PIMD_TTM21F) {  // pot_mod.f90:7-14
-    c.vdwA = -1.329565985e+6; c.vdwB = 3.632560798e+5; c.vdwC = -2.147141323e+3;
-    c.vdwD = 1e13; c.vdwE = 13.2;
-    pfO = 0.837; pfH = 0.496; pfM = 0.837;
-    c.polar[0] = 0.837; c.polar[1] = 0.496; c.polar[2] = 0.496; c.polar[3] = 0.837;
-    c.gammaM = 0.426706882; aDD = 0.3; aCC = 0.2;
-    c.dms1 = c.dms2 = c.dms3 = 0.0;
+    const double v[TTM_NPARAM] = {-1.329565985e+6, 3.632560798e+5, -2.147141323e+3, 1e13, 13.2,
+                                  0.837, 0.496, 0.837, 0.837, 0.496, 0.837, 0.3, 0.2,
+                                  0.0, 0.0, 0.0};
+    std::copy(v, v + TTM_NPARAM, th);
+    c.gammaM = 0.426706882;
     c.nd = 3; c.tdip = 0;
     c.shiftA = c.shiftB = 0.0;
   } else {  // pot_mod.f90:19-26
-    c.vdwA = 0.0; c.vdwB = 0.0; c.vdwC = -0.72298855e+03; c.vdwD = 0.10211829e+06;
-    c.vdwE = 0.37170376e+01;
-    pfO = 0.837; pfH = 0.496; pfM = 0.837;
-    c.polar[0] = 0.0; c.polar[1] = 0.0; c.polar[2] = 0.0; c.polar[3] = 1.444;
-    c.gammaM = 0.46; aDD = 0.175; aCC = 0.175;
-    c.dms1 = 0.5; c.dms2 = 0.9578; c.dms3 = 0.012;
+    const double v[TTM_NPARAM] = {0.0, 0.0, -0.72298855e+03, 0.10211829e+06, 0.37170376e+01,
+                                  0.0, 0.0, 1.444, 0.837, 0.496, 0.837, 0.175, 0.175,
+                                  0.5, 0.9578, 0.012};
+    std::copy(v, v + TTM_NPARAM, th);
+    c.gammaM = 0.46;
     c.nd = 1; c.tdip = 3;
     if (Rc == rc1) {
       c.shiftA = c.shiftB = 0.0;
@@ -212,22 +243,7 @@ int setup_system(pimd_ctx *x) {
       c.shiftB = (9 * Rc - 7 * rc1) / (Rc8 * ((Rc - rc1) * (Rc - rc1) * (Rc - rc1)));
     }
   }
-  c.Uvdw_lrc = TWOPI * (double)(c.N * c.N) / c.vol *
-               (c.vdwA / (9.0 * std::pow(Rc, 9)) + c.vdwB / (7.0 * std::pow(Rc, 7)) +
-                c.vdwC / (3.0 * (Rc * Rc * Rc)));
-  double pf6[4] = {std::pow(pfO, 1.0 / 6.0), std::pow(pfH, 1.0 / 6.0), std::pow(pfH, 1.0 / 6.0),
-                   std::pow(pfM, 1.0 / 6.0)};
-  double pf[4];
-  for (int i = 0; i < 4; ++i) pf[i] = 1.0 / (pf6[i] * pf6[i] * pf6[i]);
-  for (int i = 0; i < 4; ++i) x->polfac[i] = pf[i];
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      c.apolCC[i][j] = aCC * pf[i] * pf[j];
-      c.raCC[i][j] = std::pow(c.apolCC[i][j], 1.0 / 3.0);
-      c.apolDD[i][j] = aDD * pf[i] * pf[j];
-      c.apol3DD[i][j] = c.apolDD[i][j];
-    }
-  c.same_a = std::fabs(aCC - aDD) < 1e-12;
+  apply_ttm_params(x);
   // ewald_set(.true.)
   const double PP = -std::log(p.eps_ewald);
   c.aewald = std::sqrt(PP) / Rc;
@@ -585,7 +601,6 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
     rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
   if (rc == PIMD_OK) rc = alloc_system(x);
   if (rc == PIMD_OK) {
-    set_constant_tables();
     x->group = pick_group(x->c);
   }
   if (rc != PIMD_OK) {
@@ -845,6 +860,8 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
     cen->stream = x->stream;
     cen->stream_borrowed = true;
     x->cen = cen;
+    std::copy(x->ttm, x->ttm + TTM_NPARAM, cen->ttm);  // parameters set on this context
+    apply_ttm_params(cen);
     const size_t n = (size_t)P * x->natoms * 3;
     if (!x->d.dRRfast) {
       if ((rc = dalloc(&x->d.dRRfast, n)) != PIMD_OK) return rc;
@@ -1322,6 +1339,91 @@ extern "C" int pimd_pes_param_grad(pimd_ctx *x, const double *RRt, const double 
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(grad, dgrad, sizeof(double) * PES_NPARAM, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  return PIMD_OK;
+}
+
+extern "C" int pimd_ttm_params_get(pimd_ctx *x, double *theta) {
+  if (!x || !theta) return fail(PIMD_EINVAL, "null argument");
+  std::copy(x->ttm, x->ttm + TTM_NPARAM, theta);
+  return PIMD_OK;
+}
+
+extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
+  if (!x || !theta) return fail(PIMD_EINVAL, "null argument");
+  for (int i = 0; i < TTM_NPARAM; ++i)
+    if (!std::isfinite(theta[i])) return fail(PIMD_EINVAL, "theta[%d] is not finite", i);
+  for (int i : {TP_POLFACO, TP_POLFACH, TP_POLFACM})
+    if (!(theta[i] > 0.0)) return fail(PIMD_EINVAL, "polfac parameters must be > 0 (theta[%d])", i);
+  for (int i : {TP_POLARO, TP_POLARH, TP_POLARM, TP_ADD, TP_ACC})
+    if (theta[i] < 0.0) return fail(PIMD_EINVAL, "polarisabilities and Thole factors must be >= 0 (theta[%d])", i);
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));  // launches already queued keep the old values
+  for (pimd_ctx *y : {x, x->cen}) {
+    if (!y) continue;
+    std::copy(theta, theta + TTM_NPARAM, y->ttm);
+    apply_ttm_params(y);
+  }
+  return PIMD_OK;
+}
+
+extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double *ew,
+                                   const double *gw, double *grad, double *E, double *dEdtheta) {
+  if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
+  if (x->p.contraction)
+    return fail(PIMD_EINVAL, "parameter derivatives need a context with contraction = 0");
+  HIPCHK(hipSetDevice(x->p.device));
+  const SysConst &c = x->c;
+  DevBufs &d = x->d;
+  hipStream_t s = x->stream;
+  const size_t P = (size_t)c.P, n = P * x->natoms * 3, nout = P * TTM_NPARAM * 4;
+  const size_t ws = tangent_ws_duals(c) * P * TTM_NPARAM * 2, cfg = tangent_cfg_doubles(c) * P;
+  const size_t bytes = sizeof(double) * (ws + cfg + 2 * n + P + nout);
+  if (bytes > ((size_t)32 << 30))
+    return fail(PIMD_EINVAL, "parameter derivatives of %zu configurations need %.1f GB of workspace; "
+                "split them into smaller contexts", P, bytes / 1e9);
+  if (bytes > d.tan_bytes) {
+    if (d.tan) (void)hipFree(d.tan);
+    d.tan = nullptr;
+    d.tan_bytes = 0;
+    int rc = dalloc((char **)&d.tan, bytes);
+    if (rc != PIMD_OK) return rc;
+    d.tan_bytes = bytes;
+  }
+  TanLaunch L;
+  L.ws = d.tan;  // dual workspace first: 16-byte aligned
+  L.cfg = d.tan + ws;
+  double *dRR = L.cfg + cfg, *dgw = dRR + n, *dew = dgw + n;
+  L.out = dew + P;
+  L.RR = dRR;
+  L.gw = dgw;
+  L.ew = dew;
+  L.pes = d.pes;
+  L.kv = d.kv;
+  L.khkl = d.khkl;
+  L.dG = nullptr;
+  std::copy(x->ttm, x->ttm + TTM_NPARAM, L.theta);
+  const double Rc = x->p.rc, pre = TWOPI * (double)(c.N * c.N) / c.vol;
+  L.lrc_coef[0] = pre / (9.0 * std::pow(Rc, 9));
+  L.lrc_coef[1] = pre / (7.0 * std::pow(Rc, 7));
+  L.lrc_coef[2] = pre / (3.0 * (Rc * Rc * Rc));
+  HIPCHK(hipMemcpyAsync(dRR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dgw, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dew, ew, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  x->timed("ttm_tangent", [&] { launch_ttm_tangent(c, L, s); });
+  HIPCHK(hipGetLastError());
+  std::vector<double> out(nout);
+  HIPCHK(hipMemcpyAsync(out.data(), L.out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int j = 0; j < TTM_NPARAM; ++j) {
+    double g = 0.0;
+    for (size_t b = 0; b < P; ++b) g += out[(b * TTM_NPARAM + j) * 4 + 2];
+    grad[j] = g;
+  }
+  for (size_t b = 0; b < P; ++b) {
+    if (E) E[b] = out[(b * TTM_NPARAM) * 4];
+    if (dEdtheta)
+      for (int j = 0; j < TTM_NPARAM; ++j) dEdtheta[b * TTM_NPARAM + j] = out[(b * TTM_NPARAM + j) * 4 + 1];
+  }
   return PIMD_OK;
 }
 

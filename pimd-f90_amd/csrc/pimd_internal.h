@@ -119,6 +119,8 @@ struct DevBufs {
   double *pes = nullptr;                        // [PES_NPARAM] PES parameters
   double *adj = nullptr;                        // adjoint inputs and partial sums
   double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
+  double *tan = nullptr;                        // TTM parameter tangents: inputs, tables, workspace
+  size_t tan_bytes = 0;
   double *red = nullptr;                        // scratch for reductions
 };
 
@@ -140,7 +142,6 @@ void launch_finalize(const SysConst &c, DevBufs &b, int t, double *dRR, double *
                      double *dipE, hipStream_t s);
 void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
                         double *virialc, int *iters, hipStream_t s);
-void set_constant_tables();
 void pes_default_params(double *theta);
 void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
                         const double *gw, double *partial, int nblocks, double *grad,
@@ -151,6 +152,27 @@ enum SfacMode { SF_Q = 0, SF_D = 1, SF_V = 2 };
 enum FieldMode { FD_Q = 0, FD_D = 1, FD_QD = 2 };
 
 // Ring polymer launchers (rp_kernels.hip).
+// TTM model parameters for fitting, in the order of pimd_ttm_params_get.
+constexpr int TTM_NPARAM = 16;
+enum TtmParam { TP_VDWA = 0, TP_VDWB, TP_VDWC, TP_VDWD, TP_VDWE, TP_POLARO, TP_POLARH, TP_POLARM,
+                TP_POLFACO, TP_POLFACH, TP_POLFACM, TP_ADD, TP_ACC, TP_DMS1, TP_DMS2, TP_DMS3 };
+
+// Buffers and parameters of one tangent launch (ttm_tangent.hip).
+struct TanLaunch {
+  const double *RR, *pes;
+  const double4 *kv;
+  const int4 *khkl;
+  double *cfg;
+  void *ws;
+  const double *ew, *gw;
+  double *out, *dG;
+  double theta[TTM_NPARAM];
+  double lrc_coef[3];
+};
+size_t tangent_cfg_doubles(const SysConst &c);
+size_t tangent_ws_duals(const SysConst &c);
+void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s);
+
 void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
                      double *dRR, double *mol, bool record, hipStream_t s);
 void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStream_t s);

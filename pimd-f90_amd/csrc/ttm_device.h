@@ -9,11 +9,30 @@
 
 namespace pimd {
 
-// Exponent tables of the PES and DMS bases and the DMS coefficients, uploaded
-// once to constant memory (the PES coefficients are per context).
-extern __constant__ int c_idx[PS_NTERM][3];
-extern __constant__ double c_coefd[PS_NDMS];
-extern __constant__ int c_idxd[PS_NDMS][3];
+// Exponent tables of the PES and DMS bases and the DMS coefficients, built
+// at compile time into the constant memory of each unit that uses them (the
+// PES coefficients are per context).
+struct PsTables {
+  int idx[PS_NTERM][3];
+  int idxd[PS_NDMS][3];
+  double coefd[PS_NDMS];
+};
+constexpr PsTables make_ps_tables() {
+  PsTables t{};
+  for (int j = 0; j < PS_NTERM; ++j) {
+    t.idx[j][0] = PS_IDX1[j];
+    t.idx[j][1] = PS_IDX2[j];
+    t.idx[j][2] = PS_IDX3[j];
+  }
+  for (int j = 0; j < PS_NDMS; ++j) {
+    t.idxd[j][0] = PS_IDXD1[j];
+    t.idxd[j][1] = PS_IDXD2[j];
+    t.idxd[j][2] = PS_IDXD3[j];
+    t.coefd[j] = PS_COEFD[j];
+  }
+  return t;
+}
+static __constant__ PsTables c_ps = make_ps_tables();
 
 // Nearest integer for the minimum image.  The reference's anint rounds halves
 // away from zero; rint rounds them to even.  The two differ only when r/box is
@@ -47,6 +66,7 @@ struct RM<double> {
   static __device__ __forceinline__ double div(double a, double b) { return a / b; }
   static __device__ __forceinline__ double exp_(double x) { return exp(x); }
   static __device__ __forceinline__ double log_(double x) { return log(x); }
+  static __device__ __forceinline__ double abs_(double x) { return fabs(x); }
 };
 template <>
 struct RM<float> {
@@ -55,6 +75,7 @@ struct RM<float> {
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float exp_(float x) { return expf(x); }
   static __device__ __forceinline__ float log_(float x) { return logf(x); }
+  static __device__ __forceinline__ float abs_(float x) { return fabsf(x); }
 };
 
 // Abramowitz-Stegun 7.1.26 complementary error function (math.f90:287-314)
@@ -79,7 +100,7 @@ __device__ inline R gammq23(R x) {
       ap = ap + (R)1.0;
       del = RM<R>::div(del * x, ap);
       sum = sum + del;
-      if (fabs(del) < fabs(sum) * EPS) break;
+      if (RM<R>::abs_(del) < RM<R>::abs_(sum) * EPS) break;
     }
     return (R)1.0 - sum * RM<R>::exp_(-x + a * RM<R>::log_(x) - gln);
   }
@@ -88,13 +109,13 @@ __device__ inline R gammq23(R x) {
     R an = -((R)i * ((R)i - a));
     b = b + (R)2.0;
     d = an * d + b;
-    if (fabs(d) < FPMIN) d = FPMIN;
+    if (RM<R>::abs_(d) < FPMIN) d = FPMIN;
     cc = b + RM<R>::div(an, cc);
-    if (fabs(cc) < FPMIN) cc = FPMIN;
+    if (RM<R>::abs_(cc) < FPMIN) cc = FPMIN;
     d = RM<R>::rcp(d);
     R del = d * cc;
     h = h * del;
-    if (fabs(del - (R)1.0) < EPS) break;
+    if (RM<R>::abs_(del - (R)1.0) < EPS) break;
   }
   return RM<R>::exp_(-x + a * RM<R>::log_(x) - gln) * h;
 }
@@ -266,7 +287,7 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   double efac = exp(-PS_B1 * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (int j = 1; j < PS_NTERM; ++j) {
-    int I = c_idx[j][0], J = c_idx[j][1], K = c_idx[j][2];
+    int I = c_ps.idx[j][0], J = c_ps.idx[j][1], K = c_ps.idx[j][2];
     double cz = pes[j];
     double fI1 = f1[I], fJ2 = f2[J], fJ1 = f1[J], fI2 = f2[I], fK3 = f3[K];
     s0 = s0 + cz * (fI1 * fJ2 + fJ1 * fI2) * fK3;
@@ -299,8 +320,8 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   double P1 = 0, P2 = 0, PL1 = costh, PL2 = 0.5 * (3.0 * PL1 * PL1 - 1.0);
   double p1r1 = 0, p1r2 = 0, p1c = 0, p2r1 = 0, p2r2 = 0, p2c = 0;
   for (int j = 1; j < PS_NDMS; ++j) {
-    int I = c_idxd[j][0], J = c_idxd[j][1], K = c_idxd[j][2];
-    double cj = c_coefd[j];
+    int I = c_ps.idxd[j][0], J = c_ps.idxd[j][1], K = c_ps.idxd[j][2];
+    double cj = c_ps.coefd[j];
     P1 = P1 + cj * f1[I] * f2[J] * f3[K];
     P2 = P2 + cj * f1[J] * f2[I] * f3[K];
     p1r1 = p1r1 + cj * ((double)(I - 1) * f1[I - 1] * f2[J] * f3[K]);
@@ -328,8 +349,8 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   p2r1 = p2r1 * efd + P2 * defdr1 + dpc0dr1;
   p2r2 = p2r2 * efd + P2 * defdr2 + dpc0dr2;
   p2c = p2c * efd + dpc0dc;
-  P1 = c_coefd[0] + P1 * efd + PC0 * B2A;
-  P2 = c_coefd[0] + P2 * efd + PC0 * B2A;
+  P1 = c_ps.coefd[0] + P1 * efd + PC0 * B2A;
+  P2 = c_ps.coefd[0] + P2 * efd + PC0 * B2A;
   q3[0] = -(P1 + P2);
   q3[1] = P1;
   q3[2] = P2;

@@ -21,10 +21,6 @@
 
 namespace pimd {
 
-__constant__ int c_idx[PS_NTERM][3];
-__constant__ double c_coefd[PS_NDMS];
-__constant__ int c_idxd[PS_NDMS][3];
-
 // c5z = f5z*c5zA + fbasis*cbasis + fcore*ccore + frest*crest (nasa.f90:45).
 // The reference's PES parameters in the order of the pes arrays.
 void pes_default_params(double *theta) {
@@ -38,25 +34,6 @@ void pes_default_params(double *theta) {
   theta[PES_PHH2] = PS_PHH2;
 }
 
-void set_constant_tables() {
-  static bool done = false;
-  if (done) return;
-  int idx[PS_NTERM][3], idxd[PS_NDMS][3];
-  for (int j = 0; j < PS_NTERM; ++j) {
-    idx[j][0] = PS_IDX1[j];
-    idx[j][1] = PS_IDX2[j];
-    idx[j][2] = PS_IDX3[j];
-  }
-  for (int j = 0; j < PS_NDMS; ++j) {
-    idxd[j][0] = PS_IDXD1[j];
-    idxd[j][1] = PS_IDXD2[j];
-    idxd[j][2] = PS_IDXD3[j];
-  }
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_idx), idx, sizeof idx);
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_coefd), PS_COEFD, sizeof(double) * PS_NDMS);
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_idxd), idxd, sizeof idxd);
-  done = true;
-}
 
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {

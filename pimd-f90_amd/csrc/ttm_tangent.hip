@@ -1,0 +1,661 @@
+// Derivatives of the TTM energy and gradient with respect to the model's vdW,
+// polarisation, Thole and charge-surface parameters, for fitting.  No
+// counterpart in the reference, whose parameters are fixed in pot_mod.f90.
+//
+// Forward (tangent) mode: one workgroup evaluates pot_ttm (pot_ttm.f90:3-630)
+// for one configuration with dual numbers along one parameter direction, so
+// the value parts reproduce the energy and gradient of the force path and the
+// derivative parts are their exact derivatives along that parameter, through
+// the SOR dipole iterations as executed (the same iteration count, branch
+// decisions on the values).  With ew and gw the loss weights of the energy
+// and of the gradient of each configuration, the loss derivative along
+// parameter k is sum_b ew_b dE_b/dtheta_k + gw_b . dG_b/dtheta_k.
+//
+// Parameters (theta, PIMD_TTM_NPARAM = 16): vdwA..vdwE, polarO, polarH,
+// polarM, polfacO, polfacH, polfacM, aDD, aCCaCD, dms_param1..3
+// (potential.f90:84-113, 152-160).  Geometry (sites, k-space phases) does not
+// depend on them and stays plain FP64; charges, dipoles, fields, potentials
+// and the screened interaction tensors carry the derivative.
+//
+// Work layout: grid (configuration, direction), one 256-lane workgroup each;
+// lanes stride over sites, k-vectors or molecules per phase, each lane owns
+// what it writes (every site-pair term is evaluated from both ends), so no
+// atomics.  Per configuration a preparatory kernel writes the sites and the
+// per-axis phase tables once for all directions.
+#include "dual.h"
+#include "ttm_device.h"
+
+namespace pimd {
+
+template <>
+struct RM<Dual> {
+  static __device__ __forceinline__ Dual sqrt_(Dual x) { return dsqrt(x); }
+  static __device__ __forceinline__ Dual rcp(Dual x) { return Dual(1.0) / x; }
+  static __device__ __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
+  static __device__ __forceinline__ Dual exp_(Dual x) { return dexp(x); }
+  static __device__ __forceinline__ Dual log_(Dual x) { return dlog(x); }
+  static __device__ __forceinline__ Dual abs_(Dual x) { return dabs(x); }
+};
+
+constexpr int TAN_BLOCK = 256;
+
+// Per-configuration FP64 data: sites [4N][3] (O, H1, H2, M blocks, as
+// pot_ttm.f90:56-80), then the phase tables exp(2 pi i m x / L) per axis,
+// [m][site] as (cos, sin): m = 0..hmax, -kmax..kmax, -lmax..lmax.
+size_t tangent_cfg_doubles(const SysConst &c) {
+  const size_t ns = 4 * (size_t)c.N;
+  return ns * 3 + 2 * ns * (size_t)((c.hmax + 1) + (2 * c.kmax + 1) + (2 * c.lmax + 1));
+}
+
+// Per (configuration, direction) dual workspace: charge, phi [4N], Efq, dR
+// [4N][3], dip, olddip, Efd [nd N][3], grdq [N][27], S(k) of the charges and
+// of the dipoles [nk][2].
+size_t tangent_ws_duals(const SysConst &c) {
+  const size_t ns = 4 * (size_t)c.N, nds = (size_t)c.nd * c.N;
+  return 8 * ns + 9 * nds + 27 * (size_t)c.N + 4 * (size_t)c.nk;
+}
+
+__global__ __launch_bounds__(TAN_BLOCK) void k_tangent_prep(SysConst c, const double *__restrict__ RR,
+                                                           double *__restrict__ cfg, size_t stride) {
+  const int b = blockIdx.x, N = c.N, ns = 4 * N;
+  double *Rs = cfg + (size_t)b * stride;
+  const double *rr = RR + (size_t)b * 9 * N;
+  const double gM = c.gammaM;
+  for (int w = threadIdx.x; w < N; w += TAN_BLOCK) {
+    double roh1[3], roh2[3];
+    for (int k = 0; k < 3; ++k) {
+      Rs[(0 * N + w) * 3 + k] = rr[9 * w + k];
+      Rs[(1 * N + w) * 3 + k] = rr[9 * w + 3 + k];
+      Rs[(2 * N + w) * 3 + k] = rr[9 * w + 6 + k];
+      roh1[k] = rr[9 * w + 3 + k] - rr[9 * w + k];
+      roh2[k] = rr[9 * w + 6 + k] - rr[9 * w + k];
+    }
+    minimg(c, roh1[0], roh1[1], roh1[2]);
+    minimg(c, roh2[0], roh2[1], roh2[2]);
+    for (int k = 0; k < 3; ++k) Rs[(3 * N + w) * 3 + k] = 0.5 * gM * (roh1[k] + roh2[k]) + rr[9 * w + k];
+  }
+  __syncthreads();
+  double2 *tab = (double2 *)(Rs + (size_t)ns * 3);
+  const int lens[3] = {c.hmax + 1, 2 * c.kmax + 1, 2 * c.lmax + 1};
+  const int offs[3] = {0, c.kmax, c.lmax};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int rows = lens[ax];
+    for (int e = threadIdx.x; e < rows * ns; e += TAN_BLOCK) {
+      const int m = e / ns - offs[ax], s = e % ns;
+      double sv, cv;
+      sincos(TWOPI * c.boxi[ax] * Rs[s * 3 + ax] * (double)m, &sv, &cv);
+      tab[e] = make_double2(cv, sv);
+    }
+    tab += (size_t)rows * ns;
+  }
+}
+
+// Model parameters as duals along one direction.
+struct TanPar {
+  Dual vdw[5], polar[4], apCC[4][4], raCC[4][4], apDD[4][4], lrc;
+};
+
+struct TanArgs {
+  const double *RR;      // [P][Natoms][3]
+  const double *pes;     // monomer PES parameters
+  const double4 *kv;     // [nk] kx, ky, kz, 4 pi Ak / volume
+  const int4 *khkl;      // [nk] h, k, l
+  const double *cfg;     // per-configuration sites and tables
+  size_t cfg_stride;
+  Dual *ws;              // per (configuration, direction) workspace
+  size_t ws_stride;
+  const double *ew;      // [P] energy weights
+  const double *gw;      // [P][Natoms][3] gradient weights
+  double *out;           // [P][dirs][4]: E, dE/dtheta, loss term, SOR iterations
+  double *dG;            // NULL or [P][dirs][Natoms][3] dG/dtheta
+  double theta[TTM_NPARAM];
+  double lrc_coef[3];    // dUvdw_lrc / d(vdwA, vdwB, vdwC)
+};
+
+// The derivative of the DMS charges and their gradients with respect to
+// dms_param1..3 (which = 0..2), the linear corrections of nasa.f90:245-291,
+// in the layout of ps_monomer (gq[disp][charge][xyz], order H1, H2, O).
+__device__ inline void dms_tangent(const SysConst &c, const double r[3][3], int which,
+                                   double dq3[3], double dgq[3][3][3]) {
+  double R1[3], R2[3];
+  for (int k = 0; k < 3; ++k) {
+    R1[k] = r[1][k] - r[0][k];
+    R2[k] = r[2][k] - r[0][k];
+  }
+  minimg(c, R1[0], R1[1], R1[2]);
+  minimg(c, R2[0], R2[1], R2[2]);
+  const double d1 = sqrt(R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2]);
+  const double d2 = sqrt(R2[0] * R2[0] + R2[1] * R2[1] + R2[2] * R2[2]);
+  const double costh = (R1[0] * R2[0] + R1[1] * R2[1] + R1[2] * R2[2]) / (d1 * d2);
+  const double ax = R1[1] * R2[2] - R1[2] * R2[1];
+  const double ay = -R1[0] * R2[2] + R1[2] * R2[0];
+  const double az = R1[0] * R2[1] - R1[1] * R2[0];
+  const double sinth = sqrt(ax * ax + ay * ay + az * az) / (d1 * d2);
+  const double ang = atan2(sinth, costh), ath0 = 1.82400520401572996557;
+  double P1 = 0, P2 = 0, p1r1 = 0, p2r2 = 0, p1c = 0, p2c = 0;
+  if (which == 0) {
+    P1 = d1 - c.dms2;
+    P2 = d2 - c.dms2;
+    p1r1 = 1.0;
+    p2r2 = 1.0;
+  } else if (which == 1) {
+    P1 = -c.dms1;
+    P2 = -c.dms1;
+  } else {
+    P1 = ang - ath0;
+    P2 = ang - ath0;
+    p1c = -1.0 / sinth;
+    p2c = -1.0 / sinth;
+  }
+  dq3[0] = -(P1 + P2);
+  dq3[1] = P1;
+  dq3[2] = P2;
+  const double f1q1r13 = (p1r1 - (p1c * costh / d1)) / d1, f1q1r23 = p1c / (d1 * d2);
+  const double f2q1r23 = -(p1c * costh / d2) / d2, f2q1r13 = p1c / (d2 * d1);
+  const double f1q2r13 = -(p2c * costh / d1) / d1, f1q2r23 = p2c / (d1 * d2);
+  const double f2q2r23 = (p2r2 - (p2c * costh / d2)) / d2, f2q2r13 = p2c / (d2 * d1);
+  for (int k = 0; k < 3; ++k) {
+    dgq[0][0][k] = f1q1r13 * R1[k] + f1q1r23 * R2[k];
+    dgq[1][0][k] = f2q1r13 * R1[k] + f2q1r23 * R2[k];
+    dgq[2][0][k] = -(dgq[0][0][k] + dgq[1][0][k]);
+    dgq[0][1][k] = f1q2r13 * R1[k] + f1q2r23 * R2[k];
+    dgq[1][1][k] = f2q2r13 * R1[k] + f2q2r23 * R2[k];
+    dgq[2][1][k] = -(dgq[0][1][k] + dgq[1][1][k]);
+    dgq[0][2][k] = -(dgq[0][0][k] + dgq[0][1][k]);
+    dgq[1][2][k] = -(dgq[1][0][k] + dgq[1][1][k]);
+    dgq[2][2][k] = -(dgq[2][0][k] + dgq[2][1][k]);
+  }
+}
+
+// exp(i k . r_s) from the per-axis tables (the products of ssincos,
+// ewald.f90:403-437 and ewald.f90:20-33).
+struct Phases {
+  const double2 *tx, *ty, *tz;
+  int ns, kmax, lmax;
+  __device__ __forceinline__ void at(int s, int4 h, double &ck, double &sk) const {
+    const double2 a = tx[(size_t)h.x * ns + s], b = ty[(size_t)(h.y + kmax) * ns + s],
+                  z = tz[(size_t)(h.z + lmax) * ns + s];
+    const double cxy = a.x * b.x - a.y * b.y, sxy = a.y * b.x + a.x * b.y;
+    ck = cxy * z.x - sxy * z.y;
+    sk = sxy * z.x + cxy * z.y;
+  }
+};
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = TAN_BLOCK / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// (3 t2 r r^T - t1) v, get_dd3 applied (pot_ttm.f90:633-647).
+__device__ __forceinline__ void dd3d(const double r[3], Dual t1, Dual t2, const Dual v[3], Dual out[3]) {
+  const Dual t = 3.0 * t2;
+  const Dual rv = v[0] * r[0] + v[1] * r[1] + v[2] * r[2];
+  for (int k = 0; k < 3; ++k) out[k] = t * r[k] * rv - t1 * v[k];
+}
+
+__global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a) {
+  const int b = blockIdx.x, dir = blockIdx.y, ndir = gridDim.y, tid = threadIdx.x;
+  const int N = c.N, ns = 4 * N, nd = c.nd, tdip = c.tdip, nds = nd * N, nk = c.nk;
+  const int tend = tdip + nd;  // dipole site types [tdip, tend): M (TTM3-F) or O, H1, H2 (TTM2.1-F)
+  const double *Rs = a.cfg + (size_t)b * a.cfg_stride;
+  Phases ph;
+  ph.ns = ns;
+  ph.kmax = c.kmax;
+  ph.lmax = c.lmax;
+  ph.tx = (const double2 *)(Rs + (size_t)ns * 3);
+  ph.ty = ph.tx + (size_t)(c.hmax + 1) * ns;
+  ph.tz = ph.ty + (size_t)(2 * c.kmax + 1) * ns;
+  Dual *q = a.ws + ((size_t)b * ndir + dir) * a.ws_stride;
+  Dual *phi = q + ns, *Efq = phi + ns, *dR = Efq + 3 * ns, *dip = dR + 3 * ns;
+  Dual *old = dip + 3 * nds, *Efd = old + 3 * nds, *grdq = Efd + 3 * nds, *Sq = grdq + 27 * N;
+  Dual *Sm = Sq + 2 * nk;
+
+  __shared__ TanPar P;
+  __shared__ double red[TAN_BLOCK];
+  if (tid == 0) {
+    auto par = [&](int i) { return Dual(a.theta[i], i == dir ? 1.0 : 0.0); };
+    for (int i = 0; i < 5; ++i) P.vdw[i] = par(i);
+    P.polar[0] = par(5);
+    P.polar[1] = par(6);
+    P.polar[2] = par(6);
+    P.polar[3] = par(7);
+    // polfac = (polfac^(1/6))^-3 (potential.f90:157-160)
+    const Dual f6[4] = {dpow(par(8), 1.0 / 6.0), dpow(par(9), 1.0 / 6.0), dpow(par(9), 1.0 / 6.0),
+                        dpow(par(10), 1.0 / 6.0)};
+    Dual pf[4];
+    for (int i = 0; i < 4; ++i) pf[i] = Dual(1.0) / (f6[i] * f6[i] * f6[i]);
+    const Dual aDD = par(11), aCC = par(12);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        P.apCC[i][j] = aCC * pf[i] * pf[j];
+        P.raCC[i][j] = P.apCC[i][j].v > 0.0 ? dpow(P.apCC[i][j], 1.0 / 3.0) : Dual(0.0);
+        P.apDD[i][j] = aDD * pf[i] * pf[j];
+      }
+    P.lrc = Dual(c.Uvdw_lrc, dir < 3 ? a.lrc_coef[dir] : 0.0);
+  }
+  __syncthreads();
+  const double gM = c.gammaM, tmpq = 0.5 * gM / (1.0 - gM);
+
+  // ---- monomers: DMS charges, charge gradients, PES gradient (pot_ttm.f90:84-151)
+  double umon = 0.0;
+  for (int w = tid; w < N; w += TAN_BLOCK) {
+    double r[3][3], e1, dr1[3][3], q3[3], gq[3][3][3], dq3[3] = {0, 0, 0}, dgq[3][3][3] = {};
+    const double *rr = a.RR + ((size_t)b * 3 * N + 3 * w) * 3;
+    for (int at = 0; at < 3; ++at)
+      for (int k = 0; k < 3; ++k) r[at][k] = rr[3 * at + k];
+    ps_monomer<true>(c, r, e1, dr1, q3, gq, a.pes);
+    umon += e1;
+    if (dir >= TP_DMS1) dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
+    Dual qd[3];
+    for (int i = 0; i < 3; ++i) qd[i] = Dual(q3[i] * CHARGECON, dq3[i] * CHARGECON);
+    q[w] = Dual(0.0);
+    q[N + w] = qd[1] + tmpq * (qd[1] + qd[2]);
+    q[2 * N + w] = qd[2] + tmpq * (qd[1] + qd[2]);
+    q[3 * N + w] = qd[0] / (1.0 - gM);
+    Dual *g = grdq + 27 * w;
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 3; ++k) {
+        const Dual g0 = Dual(gq[d][0][k] * CHARGECON, dgq[d][0][k] * CHARGECON);
+        const Dual g1 = Dual(gq[d][1][k] * CHARGECON, dgq[d][1][k] * CHARGECON);
+        const Dual g2 = Dual(gq[d][2][k] * CHARGECON, dgq[d][2][k] * CHARGECON);
+        g[d * 9 + 0 * 3 + k] = g0 + tmpq * (g0 + g1);
+        g[d * 9 + 1 * 3 + k] = g1 + tmpq * (g0 + g1);
+        g[d * 9 + 2 * 3 + k] = g2 - 2.0 * tmpq * (g0 + g1);
+      }
+    for (int k = 0; k < 3; ++k) {
+      dR[(0 * N + w) * 3 + k] = Dual(dr1[0][k]);
+      dR[(1 * N + w) * 3 + k] = Dual(dr1[1][k]);
+      dR[(2 * N + w) * 3 + k] = Dual(dr1[2][k]);
+      dR[(3 * N + w) * 3 + k] = Dual(0.0);
+    }
+    for (int t = 0; t < 4; ++t) {
+      phi[t * N + w] = Dual(0.0);
+      for (int k = 0; k < 3; ++k) Efq[(t * N + w) * 3 + k] = Dual(0.0);
+    }
+  }
+  __syncthreads();
+
+  // ---- real space: vdW and charge-charge terms (pot_ttm.f90:159-228), per site
+  Dual uvdw = 0.0;
+  for (int s = tid; s < ns; s += TAN_BLOCK) {
+    const int iw = s % N, ti = s / N;
+    const double *Ri = Rs + 3 * s, *ROi = Rs + 3 * iw;
+    Dual pv = 0.0, ef[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+    for (int jw = 0; jw < N; ++jw) {
+      if (jw == iw) continue;
+      const double *ROj = Rs + 3 * jw;
+      double Rij[3] = {ROi[0] - ROj[0], ROi[1] - ROj[1], ROi[2] - ROj[2]};
+      minimg(c, Rij[0], Rij[1], Rij[2]);
+      const double d2 = norm2_exact(Rij[0], Rij[1], Rij[2]);
+      if (!(d2 < c.rc2)) continue;
+      if (ti == 0) {
+        const double dRij = sqrt(d2), R2i = 1.0 / d2, R4i = R2i * R2i, R6i = R4i * R2i,
+                     R8i = R6i * R2i, x = dRij - c.rc1;
+        const Dual ex = P.vdw[3] * dexp(-(P.vdw[4] * dRij));
+        const Dual uij = R6i * ((P.vdw[0] * R6i + P.vdw[1] * R4i) + P.vdw[2]) + ex;
+        const Dual fsc = -(R8i * 6.0 * P.vdw[2] + ex * P.vdw[4] / dRij +
+                           Dual(c.shiftA * (x * x) + c.shiftB * (x * x * x)));
+        for (int k = 0; k < 3; ++k) gv[k] += fsc * Rij[k];
+        if (iw < jw) uvdw += uij;
+      }
+      double Ra[3];
+      for (int k = 0; k < 3; ++k) Ra[k] = Ri[k] + (Rij[k] - ROi[k] + ROj[k]);
+      for (int tj = ti == 0 ? 1 : 0; tj < 4; ++tj) {
+        const int sj = tj * N + jw;
+        double r[3] = {Ra[0] - Rs[3 * sj], Ra[1] - Rs[3 * sj + 1], Ra[2] - Rs[3 * sj + 2]};
+        minimg(c, r[0], r[1], r[2]);
+        const double d2s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        Dual ts0, ts1;
+        smear01<true, Dual>(c, Dual(d2s), P.apCC[ti][tj], P.raCC[ti][tj], ts0, ts1);
+        const Dual qj = q[sj];
+        pv += ts0 * qj;
+        const Dual f = ts1 * qj;
+        for (int k = 0; k < 3; ++k) ef[k] += f * r[k];
+      }
+    }
+    phi[s] += pv;
+    for (int k = 0; k < 3; ++k) {
+      Efq[3 * s + k] += ef[k];
+      dR[3 * s + k] += gv[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- Ewald self energy and intramolecular exclusions (pot_ttm.f90:232-261)
+  for (int w = tid; w < N; w += TAN_BLOCK) {
+    for (int t = 1; t < 4; ++t) phi[t * N + w] -= c.self_fac * q[t * N + w];
+    for (int ti = 0; ti < 3; ++ti)
+      for (int tj = ti + 1; tj < 4; ++tj) {
+        const int si = ti * N + w, sj = tj * N + w;
+        double r[3] = {Rs[3 * si] - Rs[3 * sj], Rs[3 * si + 1] - Rs[3 * sj + 1],
+                       Rs[3 * si + 2] - Rs[3 * sj + 2]};
+        minimg(c, r[0], r[1], r[2]);
+        double ts0, ts1;
+        self1(c, r[0] * r[0] + r[1] * r[1] + r[2] * r[2], ts0, ts1);
+        phi[si] += ts0 * q[sj];
+        phi[sj] += ts0 * q[si];
+        for (int k = 0; k < 3; ++k) {
+          Efq[3 * si + k] += ts1 * q[sj] * r[k];
+          Efq[3 * sj + k] -= ts1 * q[si] * r[k];
+        }
+      }
+  }
+  __syncthreads();
+
+  // ---- reciprocal space, charges (ewald_std_qq, ewald.f90:1-52)
+  for (int k = tid; k < nk; k += TAN_BLOCK) {
+    const int4 h = a.khkl[k];
+    Dual rQ = 0.0, iQ = 0.0;
+    for (int s = N; s < ns; ++s) {
+      double ck, sk;
+      ph.at(s, h, ck, sk);
+      rQ += q[s] * ck;
+      iQ += q[s] * sk;
+    }
+    Sq[2 * k] = rQ;
+    Sq[2 * k + 1] = iQ;
+  }
+  __syncthreads();
+  for (int s = tid; s < ns; s += TAN_BLOCK) {
+    Dual ef[3] = {0.0, 0.0, 0.0}, pv = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      const int4 h = a.khkl[k];
+      const double4 kv = a.kv[k];
+      const double p4vAk = kv.w;
+      double ck, sk;
+      ph.at(s, h, ck, sk);
+      const Dual rQ = Sq[2 * k], iQ = Sq[2 * k + 1];
+      const Dual t = p4vAk * (ck * iQ - sk * rQ);
+      ef[0] -= t * kv.x;
+      ef[1] -= t * kv.y;
+      ef[2] -= t * kv.z;
+      pv += p4vAk * (ck * rQ + sk * iQ);
+    }
+    for (int k = 0; k < 3; ++k) Efq[3 * s + k] += ef[k];
+    if (s >= N) phi[s] += pv;
+  }
+  __syncthreads();
+
+  // ---- Uelec, the charge-field gradient (pot_ttm.f90:273-276), initial dipoles
+  Dual uel = 0.0;
+  for (int s = N + tid; s < ns; s += TAN_BLOCK) {
+    uel += q[s] * phi[s];
+    for (int k = 0; k < 3; ++k) dR[3 * s + k] -= q[s] * Efq[3 * s + k];
+  }
+  for (int i = tid; i < nds; i += TAN_BLOCK) {
+    const int t = tdip + i / N, s = t * N + i % N;
+    for (int k = 0; k < 3; ++k) {
+      dip[3 * i + k] = P.polar[t] * Efq[3 * s + k];
+      old[3 * i + k] = Dual(0.0);
+    }
+  }
+  __syncthreads();
+
+  // ---- SOR dipole iterations (pot_ttm.f90:306-392)
+  int iters = c.maxiter;
+  const double om = c.sor;
+  for (int it = 1; it <= c.maxiter; ++it) {
+    for (int i = tid; i < nds; i += TAN_BLOCK) {
+      const int ti = tdip + i / N, iw = i % N, s = ti * N + iw;
+      const double *Ri = Rs + 3 * s, *ROi = Rs + 3 * iw;
+      Dual e[3] = {0.0, 0.0, 0.0};
+      for (int jw = 0; jw < N; ++jw) {
+        if (jw == iw) continue;
+        const double *ROj = Rs + 3 * jw;
+        double Rij[3] = {ROi[0] - ROj[0], ROi[1] - ROj[1], ROi[2] - ROj[2]};
+        minimg(c, Rij[0], Rij[1], Rij[2]);
+        if (!(norm2_exact(Rij[0], Rij[1], Rij[2]) < c.rc2)) continue;
+        double R0[3] = {Rij[0] - ROi[0] + ROj[0], Rij[1] - ROi[1] + ROj[1], Rij[2] - ROi[2] + ROj[2]};
+        minimg(c, R0[0], R0[1], R0[2]);
+        for (int tj = tdip; tj < tend; ++tj) {
+          const int sj = tj * N + jw, j = (tj - tdip) * N + jw;
+          double r[3];
+          for (int k = 0; k < 3; ++k) r[k] = Ri[k] + R0[k] - Rs[3 * sj + k];
+          minimg(c, r[0], r[1], r[2]);
+          Dual t1, t2, o[3];
+          smear2<true, Dual>(c, Dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
+          dd3d(r, t1, t2, dip + 3 * j, o);
+          for (int k = 0; k < 3; ++k) e[k] += o[k];
+        }
+      }
+      // dipole sites of the same molecule (TTM2.1-F, pot_ttm.f90:344-365)
+      for (int tj = tdip; tj < tend; ++tj) {
+        if (tj == ti) continue;
+        const int sj = tj * N + iw, j = (tj - tdip) * N + iw;
+        double r[3] = {Ri[0] - Rs[3 * sj], Ri[1] - Rs[3 * sj + 1], Ri[2] - Rs[3 * sj + 2]};
+        minimg(c, r[0], r[1], r[2]);
+        Dual t1, t2, o[3];
+        smear2<true, Dual>(c, Dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
+        dd3d(r, t1, t2, dip + 3 * j, o);
+        for (int k = 0; k < 3; ++k) e[k] += o[k];
+      }
+      for (int k = 0; k < 3; ++k) Efd[3 * i + k] = e[k];
+    }
+    // structure factor of the dipoles (ewald_std_dd, ewald.f90:56-119)
+    for (int k = tid; k < nk; k += TAN_BLOCK) {
+      const int4 h = a.khkl[k];
+      const double4 kv = a.kv[k];
+      Dual rQ = 0.0, iQ = 0.0;
+      for (int i = 0; i < nds; ++i) {
+        const int s = (tdip + i / N) * N + i % N;
+        double ck, sk;
+        ph.at(s, h, ck, sk);
+        const Dual dK = dip[3 * i] * kv.x + dip[3 * i + 1] * kv.y + dip[3 * i + 2] * kv.z;
+        rQ -= dK * sk;
+        iQ += dK * ck;
+      }
+      Sm[2 * k] = rQ;
+      Sm[2 * k + 1] = iQ;
+    }
+    __syncthreads();
+    double ss = 0.0;
+    for (int i = tid; i < nds; i += TAN_BLOCK) {
+      const int ti = tdip + i / N, s = ti * N + i % N;
+      Dual e[3] = {Efd[3 * i], Efd[3 * i + 1], Efd[3 * i + 2]};
+      for (int k = 0; k < nk; ++k) {
+        const int4 h = a.khkl[k];
+        const double4 kv = a.kv[k];
+        double ck, sk;
+        ph.at(s, h, ck, sk);
+        const Dual t = kv.w * (ck * Sm[2 * k + 1] - sk * Sm[2 * k]);
+        e[0] -= kv.x * t;
+        e[1] -= kv.y * t;
+        e[2] -= kv.z * t;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const Dual mu = dip[3 * i + k];
+        const Dual ek = e[k] + Efq[3 * s + k] + c.fac_dd * mu;
+        const Dual nu = (1.0 - om) * mu + (om * P.polar[ti]) * ek;
+        const double df = nu.v - old[3 * i + k].v;
+        ss += df * df;
+        dip[3 * i + k] = nu;
+      }
+    }
+    const double sum = block_sum(ss, red);
+    if (c.stath * sqrt(sum) < c.peps) {
+      iters = it;
+      break;
+    }
+    for (int i = tid; i < 3 * nds; i += TAN_BLOCK) old[i] = dip[i];
+    __syncthreads();
+  }
+  __syncthreads();
+
+  // ---- Uind (pot_ttm.f90:390) and the final real-space forces (pot_ttm.f90:399-542)
+  Dual uind = 0.0;
+  for (int i = tid; i < nds; i += TAN_BLOCK) {
+    const int s = (tdip + i / N) * N + i % N;
+    for (int k = 0; k < 3; ++k) uind += dip[3 * i + k] * Efq[3 * s + k];
+  }
+  for (int s = tid; s < ns; s += TAN_BLOCK) {
+    const int iw = s % N, ti = s / N;
+    const bool adip = ti >= tdip && ti < tend;
+    const double *Ri = Rs + 3 * s, *ROi = Rs + 3 * iw;
+    const Dual qi = q[s];
+    Dual di[3] = {0.0, 0.0, 0.0};
+    if (adip)
+      for (int k = 0; k < 3; ++k) di[k] = dip[3 * ((ti - tdip) * N + iw) + k];
+    Dual g[3] = {0.0, 0.0, 0.0}, pv = 0.0;
+    for (int jw = 0; jw < N; ++jw) {
+      const bool same = jw == iw;
+      double R0[3] = {0.0, 0.0, 0.0};
+      if (!same) {
+        const double *ROj = Rs + 3 * jw;
+        double Rij[3] = {ROi[0] - ROj[0], ROi[1] - ROj[1], ROi[2] - ROj[2]};
+        minimg(c, Rij[0], Rij[1], Rij[2]);
+        if (!(norm2_exact(Rij[0], Rij[1], Rij[2]) < c.rc2)) continue;
+        for (int k = 0; k < 3; ++k) R0[k] = Rij[k] - ROi[k] + ROj[k];
+        minimg(c, R0[0], R0[1], R0[2]);
+      }
+      for (int tj = 0; tj < 4; ++tj) {
+        if (same && tj == ti) continue;
+        const int sj = tj * N + jw;
+        const bool bdip = tj >= tdip && tj < tend;
+        double r[3];
+        for (int k = 0; k < 3; ++k) r[k] = same ? Ri[k] - Rs[3 * sj + k] : Ri[k] + R0[k] - Rs[3 * sj + k];
+        minimg(c, r[0], r[1], r[2]);
+        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const Dual qj = q[sj];
+        Dual dj[3] = {0.0, 0.0, 0.0};
+        if (bdip)
+          for (int k = 0; k < 3; ++k) dj[k] = dip[3 * ((tj - tdip) * N + jw) + k];
+        if (adip && bdip) {
+          // dipole-dipole: d_i . T3(r) . d_j with T3 the third-rank tensor of PBCsmear3
+          Dual t1, t2, t3;
+          smear3<true, Dual>(c, Dual(d2), P.apDD[ti][tj], t1, t2, t3);
+          const Dual dir_ = di[0] * r[0] + di[1] * r[1] + di[2] * r[2];
+          const Dual djr = dj[0] * r[0] + dj[1] * r[1] + dj[2] * r[2];
+          const Dual dd = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+          const Dual a15 = 15.0 * t3 * dir_ * djr, a3 = 3.0 * t2;
+          for (int k = 0; k < 3; ++k) g[k] += a15 * r[k] - a3 * (dir_ * dj[k] + djr * di[k] + dd * r[k]);
+        }
+        Dual qdqd[3];
+        for (int k = 0; k < 3; ++k) qdqd[k] = qj * di[k] - qi * dj[k];
+        Dual ts1 = 0.0, ts2 = 0.0;
+        bool qd = true;
+        if (same) {
+          double s1, s2;
+          self2(c, d2, s1, s2);
+          ts1 = s1;
+          ts2 = s2;
+        } else {
+          const double qsq = qdqd[0].v * qdqd[0].v + qdqd[1].v * qdqd[1].v + qdqd[2].v * qdqd[2].v;
+          qd = qsq > 1e-9;
+          if (qd) smear2<true, Dual>(c, Dual(d2), P.apCC[ti][tj], ts1, ts2);
+        }
+        if (qd) {
+          Dual o[3];
+          dd3d(r, ts1, ts2, qdqd, o);
+          for (int k = 0; k < 3; ++k) g[k] += o[k];
+        }
+        pv += ts1 * (dj[0] * r[0] + dj[1] * r[1] + dj[2] * r[2]);
+      }
+    }
+    for (int k = 0; k < 3; ++k) dR[3 * s + k] += g[k];
+    phi[s] += pv;
+  }
+  __syncthreads();
+
+  // ---- reciprocal space, dipole forces (ewald_std_qd, ewald.f90:123-226); the
+  // dipole structure factor is the one of the last SOR iteration's input
+  for (int s = tid; s < ns; s += TAN_BLOCK) {
+    const int iw = s % N, ti = s / N;
+    const bool sdip = ti >= tdip && ti < tend, schg = s >= N;
+    const Dual qs = q[s];
+    Dual mu[3] = {0.0, 0.0, 0.0};
+    if (sdip)
+      for (int k = 0; k < 3; ++k) mu[k] = dip[3 * ((ti - tdip) * N + iw) + k];
+    Dual g[3] = {0.0, 0.0, 0.0}, pv = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      const int4 h = a.khkl[k];
+      const double4 kv = a.kv[k];
+      const double p4vAk = kv.w;
+      double ck, sk;
+      ph.at(s, h, ck, sk);
+      const Dual rQm = Sm[2 * k], iQm = Sm[2 * k + 1];
+      if (schg) {
+        pv += p4vAk * (ck * rQm + sk * iQm);
+        const Dual t = qs * p4vAk * (ck * iQm - sk * rQm);
+        g[0] += t * kv.x;
+        g[1] += t * kv.y;
+        g[2] += t * kv.z;
+      }
+      if (sdip) {
+        const Dual dK = mu[0] * kv.x + mu[1] * kv.y + mu[2] * kv.z;
+        const Dual t = dK * p4vAk * (ck * (Sq[2 * k] + rQm) + sk * (Sq[2 * k + 1] + iQm));
+        g[0] -= t * kv.x;
+        g[1] -= t * kv.y;
+        g[2] -= t * kv.z;
+      }
+    }
+    for (int k = 0; k < 3; ++k) dR[3 * s + k] += g[k];
+    phi[s] += pv;
+  }
+  __syncthreads();
+
+  // ---- charge chain rule (pot_ttm.f90:556-584), M-site redistribution
+  // (pot_ttm.f90:597-602), loss terms
+  double lossg = 0.0;
+  for (int w = tid; w < N; w += TAN_BLOCK) {
+    const Dual *gq = grdq + 27 * w;
+    const Dual p1 = phi[N + w], p2 = phi[2 * N + w], pm = phi[3 * N + w];
+    Dual G[3][3];  // atoms O, H1, H2
+    for (int k = 0; k < 3; ++k) {
+      const Dual dg1 = gq[0 * 9 + 0 * 3 + k] * p1 + gq[0 * 9 + 1 * 3 + k] * p2 + gq[0 * 9 + 2 * 3 + k] * pm;
+      const Dual dg2 = gq[1 * 9 + 0 * 3 + k] * p1 + gq[1 * 9 + 1 * 3 + k] * p2 + gq[1 * 9 + 2 * 3 + k] * pm;
+      const Dual dg0 = gq[2 * 9 + 0 * 3 + k] * p1 + gq[2 * 9 + 1 * 3 + k] * p2 + gq[2 * 9 + 2 * 3 + k] * pm;
+      const Dual dM = dR[(3 * N + w) * 3 + k];
+      G[1][k] = (dR[(N + w) * 3 + k] + dg1) + 0.5 * gM * dM;
+      G[2][k] = (dR[(2 * N + w) * 3 + k] + dg2) + 0.5 * gM * dM;
+      G[0][k] = (dR[w * 3 + k] + dg0) + (1.0 - gM) * dM;
+    }
+    const size_t a0 = ((size_t)b * 3 * N + 3 * w) * 3;
+    for (int at = 0; at < 3; ++at)
+      for (int k = 0; k < 3; ++k) {
+        lossg += a.gw[a0 + 3 * at + k] * G[at][k].d;
+        if (a.dG) a.dG[((size_t)b * ndir + dir) * 9 * N + 9 * w + 3 * at + k] = G[at][k].d;
+      }
+  }
+  const double Ev = block_sum(umon, red) + block_sum(uvdw.v, red) + 0.5 * block_sum(uel.v, red) -
+                    0.5 * block_sum(uind.v, red) + P.lrc.v;
+  const double Ed = block_sum(uvdw.d, red) + 0.5 * block_sum(uel.d, red) - 0.5 * block_sum(uind.d, red) +
+                    P.lrc.d;
+  const double lg = block_sum(lossg, red);
+  if (tid == 0) {
+    double *o = a.out + ((size_t)b * ndir + dir) * 4;
+    o[0] = Ev;
+    o[1] = Ed;
+    o[2] = a.ew[b] * Ed + lg;
+    o[3] = (double)iters;
+  }
+}
+
+void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s) {
+  hipLaunchKernelGGL(k_tangent_prep, dim3(c.P), dim3(TAN_BLOCK), 0, s, c, L.RR, L.cfg,
+                     tangent_cfg_doubles(c));
+  TanArgs a;
+  a.RR = L.RR;
+  a.pes = L.pes;
+  a.kv = L.kv;
+  a.khkl = L.khkl;
+  a.cfg = L.cfg;
+  a.cfg_stride = tangent_cfg_doubles(c);
+  a.ws = (Dual *)L.ws;
+  a.ws_stride = tangent_ws_duals(c);
+  a.ew = L.ew;
+  a.gw = L.gw;
+  a.out = L.out;
+  a.dG = L.dG;
+  for (int i = 0; i < TTM_NPARAM; ++i) a.theta[i] = L.theta[i];
+  for (int i = 0; i < 3; ++i) a.lrc_coef[i] = L.lrc_coef[i];
+  hipLaunchKernelGGL(k_ttm_tangent, dim3(c.P, TTM_NPARAM), dim3(TAN_BLOCK), 0, s, c, a);
+}
+
+}  // namespace pimd

@@ -1,4 +1,4 @@
-"""Fitting the monomer PES to reference energies and forces.
+"""Fitting the monomer PES and the TTM parameters to reference energies and forces.
 
 A fitting set is a list of configuration groups; each group is a force field
 whose beads hold independent configurations of one size and box (a bead is
@@ -7,16 +7,32 @@ configuration and target gradients per atom.  The loss is
 
     L(theta) = sum_configs (E - E*)^2 + sum_configs,atoms |dE/dr - (dE/dr)*|^2
 
-over the Partridge-Schwenke parameters theta (combined c5z, deohA, alphaoh,
-roh, phh1A, phh2).  Forward values come from the GPU force path; the gradient
-comes from the hand-written adjoint kernel (pimd_pes_param_grad).  PESLoss is
-a torch.autograd.Function over theta, so any torch optimiser can drive a fit.
+over theta: the Partridge-Schwenke parameters (combined c5z, deohA, alphaoh,
+roh, phh1A, phh2; PES_NPARAM values), optionally followed by the TTM
+parameters (vdW, polarisabilities, Thole factors, DMS corrections;
+TTM_NPARAM values, see TTM_PARAM_NAMES).  Forward values come from the GPU
+force path; the gradient from the hand-written derivative kernels
+(pimd_pes_param_grad for the PES, pimd_ttm_param_grad for the TTM
+parameters).  PESLoss is a torch.autograd.Function over theta, so any torch
+optimiser can drive a fit.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .ttm import PES_NPARAM, TTMForceField
+from .ttm import PES_NPARAM, TTM_NPARAM, TTMForceField
+
+NPARAM_ALL = PES_NPARAM + TTM_NPARAM
+
+
+def split_theta(theta):
+    """(PES part, TTM part or None) of a parameter vector of either length."""
+    th = np.asarray(theta, dtype=np.float64)
+    if th.size == PES_NPARAM:
+        return th, None
+    if th.size == NPARAM_ALL:
+        return th[:PES_NPARAM], th[PES_NPARAM:]
+    raise ValueError(f"theta must hold {PES_NPARAM} or {NPARAM_ALL} values, got {th.size}")
 
 
 class FitGroup:
@@ -30,18 +46,25 @@ class FitGroup:
         self.G_target = np.ascontiguousarray(G_target, dtype=np.float64)
 
     def evaluate(self, theta, grad: bool):
-        self.ff.set_pes_params(theta)
+        pes, ttm = split_theta(theta)
+        self.ff.set_pes_params(pes)
+        if ttm is not None:
+            self.ff.set_ttm_params(ttm)
         r = self.ff.full_bead_forces(self.RRt, self.RRc)
         dE = r["Upott"] - self.E_target
         dG = r["dRRt"] - self.G_target
         loss = float(np.sum(dE * dE) + np.sum(dG * dG))
-        g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG) if grad else None
+        g = None
+        if grad:
+            g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)
+            if ttm is not None:
+                g = np.concatenate([g, self.ff.ttm_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0]])
         return loss, g
 
 
 def loss_and_grad(groups, theta, grad: bool = True):
     """Loss over every group and, with grad, its gradient dL/dtheta."""
-    total, g = 0.0, np.zeros(PES_NPARAM) if grad else None
+    total, g = 0.0, np.zeros(np.asarray(theta).size) if grad else None
     for grp in groups:
         l, gg = grp.evaluate(theta, grad)
         total += l
@@ -54,7 +77,8 @@ def _autograd_function():
     import torch
 
     class PESLoss(torch.autograd.Function):
-        """loss = PESLoss.apply(theta, groups): theta a float64 tensor of 250 values."""
+        """loss = PESLoss.apply(theta, groups): theta a float64 tensor of PES_NPARAM or
+        PES_NPARAM + TTM_NPARAM values."""
 
         @staticmethod
         def forward(ctx, theta, groups):

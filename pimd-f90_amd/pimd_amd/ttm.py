@@ -35,10 +35,16 @@ EXPORTS = (
     "pimd_profile_enable", "pimd_profile_count", "pimd_profile_get", "pimd_profile_reset",
     "pimd_sync", "pimd_batch_create", "pimd_batch_destroy", "pimd_batch_autotune",
     "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
-    "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy",
+    "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy", "pimd_ttm_params_get",
+    "pimd_ttm_params_set", "pimd_ttm_param_grad",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
+# vdwA..vdwE, polarO, polarH, polarM, polfacO, polfacH, polfacM, aDD, aCCaCD, dms_param1..3
+TTM_PARAM_NAMES = ("vdwA", "vdwB", "vdwC", "vdwD", "vdwE", "polarO", "polarH", "polarM",
+                   "polfacO", "polfacH", "polfacM", "aDD", "aCCaCD", "dms_param1", "dms_param2",
+                   "dms_param3")
+TTM_NPARAM = len(TTM_PARAM_NAMES)
 
 
 class PimdParams(C.Structure):
@@ -104,6 +110,9 @@ def load_library(path: str = LIB_PATH):
         "pimd_pes_params_get": [vp, dp],
         "pimd_pes_params_set": [vp, dp],
         "pimd_pes_param_grad": [vp, dp, dp, dp, dp],
+        "pimd_ttm_params_get": [vp, dp],
+        "pimd_ttm_params_set": [vp, dp],
+        "pimd_ttm_param_grad": [vp, dp, dp, dp, dp, dp, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
     }
@@ -230,11 +239,35 @@ class TTMForceField:
         _check(load_library().pimd_pes_param_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(out)))
         return out
 
+    def ttm_params(self) -> np.ndarray:
+        """The TTM model parameters in use, in the order of TTM_PARAM_NAMES."""
+        th = np.zeros(TTM_NPARAM)
+        _check(load_library().pimd_ttm_params_get(self._h, _ptr(th)))
+        return th
+
+    def set_ttm_params(self, theta):
+        th = _arr(theta, (TTM_NPARAM,))
+        _check(load_library().pimd_ttm_params_set(self._h, _ptr(th)))
+        if self._single is not None:
+            self._single.set_ttm_params(th)
+
+    def ttm_param_grad(self, RRt, ew, gw):
+        """Tangents of the TTM parameters: (grad, E, dE/dtheta) with
+        grad = sum_b ew_b dE_b/dtheta + sum gw . d(dE/dr)/dtheta (see pimd_ttm_param_grad)."""
+        P, na = self.params.nbeads, self.params.natoms
+        r, e, g = _arr(RRt, (P, na, 3)), _arr(ew, (P,)), _arr(gw, (P, na, 3))
+        grad, E, dE = np.zeros(TTM_NPARAM), np.zeros(P), np.zeros((P, TTM_NPARAM))
+        _check(load_library().pimd_ttm_param_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(grad),
+                                                  _ptr(E), _ptr(dE)))
+        return grad, E, dE
+
     def potential(self, RR, RRc, t: int = 1) -> dict:
         """One bead (force_calc.f90:112-170): the reference's per-rank call."""
         if self._single is None:
             q = TTMParams(**{**self.params.__dict__, "nbeads": 1})
             self._single = TTMForceField(q)
+            self._single.set_pes_params(self.pes_params())
+            self._single.set_ttm_params(self.ttm_params())
         r = self._single.full_bead_forces(np.asarray(RR)[None], RRc, t)
         return {k: (v[0] if isinstance(v, np.ndarray) and v.ndim > 0 else v) for k, v in r.items()}
 

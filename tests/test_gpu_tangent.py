@@ -1,0 +1,252 @@
+"""Derivatives of the TTM energy and gradient with respect to the model's
+vdW, polarisability, Thole and DMS-correction parameters (pimd_ttm_param_grad).
+
+Fitting sets of gas-phase 5-water clusters (in a 40 A box) and periodic
+32-water boxes, with targets from the FP64 forward model at parameters
+perturbed by 2 % (Gaussian, seeded).  Checked against central finite
+differences (h = 1e-6 relative) of the GPU forward model and of the oracle
+with its parameters overridden; the bar is 1e-6 relative.  A central
+difference carries a rounding error of about eps |E| / h, which exceeds that
+bar for a parameter as small as dms_param3 (0.012, so h = 1.2e-8 against
+energies of hundreds of kcal/mol); the h = 1e-6 check allows that bound on
+top, and a five-point difference with h = 1e-4 relative (truncation error
+O(h^4), rounding 100 times smaller) checks the parameters at 1e-7.  The
+five-point check leaves out the parameters in the argument of the Thole
+charge term's Gamma(2/3, x) (polfacO/H/M, aCCaCD): the reference's gammq2_3
+(math.f90:188-285) switches from its series to its continued fraction at
+x = 5/3 and stops each at a data-dependent number of terms, so the function
+jumps by up to ~3e-7 relative along those parameters, and points 4e-4 apart
+straddle such jumps.  The value parts of the tangent evaluation must
+reproduce the forward energies (1e-10).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FD_REL = 1e-6
+FD5_REL = 1e-7
+EPS = np.finfo(np.float64).eps
+
+
+POLAR = (5, 6, 7)   # polarisabilities: must not go negative
+GAMMA_ARG = (8, 9, 10, 12)   # polfacO/H/M, aCCaCD: in the argument of gammq2_3
+# the size of each parameter in the model that uses it, for the steps of
+# parameters that are zero in the other model (vdwA, vdwB of TTM3-F, the DMS
+# corrections of TTM2.1-F)
+SCALE = np.array([1.33e6, 3.63e5, 2.15e3, 1e13, 13.2, 0.837, 0.496, 1.444, 0.837, 0.496, 0.837,
+                  0.3, 0.2, 0.5, 0.9578, 0.012])
+
+
+def _h(theta, j, rel):
+    return rel * (abs(theta[j]) if theta[j] != 0.0 else SCALE[j])
+
+
+def _pm(theta, j):
+    """(theta+, theta-, their distance): central differences with h = 1e-6
+    relative.  A zero polarisability (those of O and H in TTM3-F, which do not
+    enter at all) takes a forward difference instead."""
+    tp, tm = theta.copy(), theta.copy()
+    h = _h(theta, j, 1e-6)
+    if theta[j] == 0.0 and j in POLAR:
+        tp[j] += h
+        return tp, tm, h
+    tp[j] += h
+    tm[j] -= h
+    return tp, tm, 2 * h
+
+
+def _five_point(theta, j):
+    """Points theta + m h e_j (m = 2, 1, -1, -2) with h = 1e-4 relative
+    (forward points for a zero polarisability) for the five-point first
+    derivative."""
+    h = _h(theta, j, 1e-4)
+    ms = (4, 3, 1, 0) if theta[j] == 0.0 and j in POLAR else (2, 1, -1, -2)
+    pts = []
+    for m in ms:
+        t = theta.copy()
+        t[j] += m * h
+        pts.append(t)
+    return pts, h
+
+
+def _groups(model, ncl=6, nbox=4, seed=5000, polar_eps=1e-12):
+    from pimd_amd.fit import FitGroup
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import distorted_box, water_cluster
+    from pimd_amd.ttm import TTMForceField
+    cl = np.array([water_cluster(5, seed=seed + i) for i in range(ncl)])
+    bx = []
+    for i in range(nbox):
+        r, L = distorted_box(32, seed=seed + 500 + i)
+        bx.append(r)
+    bx = np.array(bx)
+    common = dict(pot_model=model, eps_ewald=1e-8, polar_eps=polar_eps, polar_maxiter=100)
+    pcl = TTMParams(nwaters=5, nbeads=ncl, box=(40.0, 40.0, 40.0), **common)
+    pbx = TTMParams(nwaters=32, nbeads=nbox, box=(L, L, L), rc=4.9, **common)
+    ffc, ffb = TTMForceField(pcl), TTMForceField(pbx)
+    theta0 = ffc.ttm_params()
+    rng = np.random.default_rng(seed)
+    theta_t = theta0 * (1.0 + 0.02 * rng.standard_normal(theta0.size))
+    pes0 = ffc.pes_params()
+    groups = []
+    for ff, R in ((ffc, cl), (ffb, bx)):
+        ff.set_ttm_params(theta_t)
+        r = ff.full_bead_forces(R, R[0])
+        groups.append(FitGroup(ff, R, r["Upott"], r["dRRt"]))
+        ff.set_ttm_params(theta0)
+    return groups, theta0, pes0
+
+
+@pytest.fixture(scope="module", params=[3, 2], ids=["ttm3f", "ttm21f"])
+def fitset(request):
+    from pimd_amd.fit import loss_and_grad
+    groups, theta0, pes0 = _groups(request.param)
+    full = np.concatenate([pes0, theta0])
+    L0, g = loss_and_grad(groups, full)
+    return request.param, groups, theta0, pes0, L0, g[pes0.size:]
+
+
+def _forward(groups, theta):
+    out = []
+    for grp in groups:
+        grp.ff.set_ttm_params(theta)
+        r = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        out.append((r["Upott"], r["dRRt"]))
+    for grp in groups:
+        grp.ff.set_ttm_params(theta)
+    return out
+
+
+def _central(plus, minus, groups, h2):
+    """[L(theta+) - L(theta-)] / h2 summed as (a+ - a-)(a+ + a- - 2 target),
+    and a bound on its rounding error: each a carries about eps |a| (a few
+    eps for sums of many terms)."""
+    tot, noise = 0.0, 0.0
+    for (Ep, Gp), (Em, Gm), grp in zip(plus, minus, groups):
+        wE, wG = Ep + Em - 2 * grp.E_target, Gp + Gm - 2 * grp.G_target
+        tot += np.sum((Ep - Em) * wE)
+        tot += np.sum((Gp - Gm) * wG)
+        noise += np.sum(np.abs(Ep) * np.abs(wE)) + np.sum(np.abs(Gp) * np.abs(wG))
+    return tot / h2, 8 * EPS * noise / h2
+
+
+def test_tangent_values_match_forward(fitset):
+    model, groups, theta0, pes0, L0, g = fitset
+    for grp in groups:
+        grp.ff.set_ttm_params(theta0)
+        r = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        P, na = grp.RRt.shape[:2]
+        _, E, dE = grp.ff.ttm_param_grad(grp.RRt, np.zeros(P), np.zeros((P, na, 3)))
+        rel = np.max(np.abs(E - r["Upott"]) / np.abs(r["Upott"]))
+        print(f"model {model} N {na // 3}: tangent energy vs forward rel {rel:.2e}", flush=True)
+        assert rel < 1e-10
+
+
+def test_energy_derivatives_vs_fd(fitset):
+    """dE_b/dtheta_j of every configuration against central differences of the
+    forward energies."""
+    model, groups, theta0, pes0, L0, g = fitset
+    for grp in groups:
+        P, na = grp.RRt.shape[:2]
+        grp.ff.set_ttm_params(theta0)
+        _, _, dE = grp.ff.ttm_param_grad(grp.RRt, np.zeros(P), np.zeros((P, na, 3)))
+        for j in range(theta0.size):
+            tp, tm, h2 = _pm(theta0, j)
+            grp.ff.set_ttm_params(tp)
+            Ep = grp.ff.full_bead_forces(grp.RRt, grp.RRc)["Upott"]
+            grp.ff.set_ttm_params(tm)
+            Em = grp.ff.full_bead_forces(grp.RRt, grp.RRc)["Upott"]
+            fd = (Ep - Em) / h2
+            E5 = []
+            pts, h = _five_point(theta0, j)
+            for t in pts:
+                grp.ff.set_ttm_params(t)
+                E5.append(grp.ff.full_bead_forces(grp.RRt, grp.RRc)["Upott"])
+            fd5 = (8 * (E5[1] - E5[2]) - (E5[0] - E5[3])) / (12 * h)
+            scale = np.max(np.abs(fd))
+            err = np.max(np.abs(fd - dE[:, j]))
+            err5 = np.max(np.abs(fd5 - dE[:, j]))
+            noise = 4 * EPS * np.max(np.abs(Ep)) / h2
+            noise5 = 6 * EPS * np.max(np.abs(Ep)) / h
+            print(f"model {model} N {na // 3} param {j:2d} |dE| {scale:.3e} err {err:.2e} "
+                  f"(rounding bound {noise:.1e}) five-point err {err5:.2e}", flush=True)
+            if scale == 0.0:
+                assert np.all(dE[:, j] == 0.0) and np.all(fd5 == 0.0), j
+            else:
+                assert err < FD_REL * scale + noise, j
+                if j not in GAMMA_ARG:
+                    assert err5 < FD5_REL * scale + noise5, j
+        grp.ff.set_ttm_params(theta0)
+
+
+def test_loss_gradient_vs_fd_of_gpu_forward(fitset):
+    model, groups, theta0, pes0, L0, g = fitset
+    assert L0 > 0 and np.all(np.isfinite(g))
+    for j in range(theta0.size):
+        tp, tm, h2 = _pm(theta0, j)
+        fd, noise = _central(_forward(groups, tp), _forward(groups, tm), groups, h2)
+        _forward(groups, theta0)
+        err = abs(fd - g[j])
+        print(f"model {model} param {j:2d} tangent {g[j]: .10e} fd {fd: .10e} "
+              f"rel {err / max(abs(g[j]), 1e-300):.2e} (rounding bound {noise:.1e})", flush=True)
+        if fd == 0.0 and g[j] == 0.0:
+            continue
+        assert err < FD_REL * abs(g[j]) + noise, j
+
+
+def test_loss_gradient_vs_fd_of_oracle(fitset):
+    """The same loss gradient against central differences of the oracle (the
+    C restatement of the reference) with its parameters overridden, on the
+    cluster group."""
+    import oracle
+    model, groups, theta0, pes0, L0, g = fitset
+    grp = groups[0]
+    sys_ = oracle.System(grp.ff.params)
+    assert np.array_equal(sys_.ttm_params(), theta0)
+    P, na = grp.RRt.shape[:2]
+    grp.ff.set_ttm_params(theta0)
+    gc = grp.ff.ttm_param_grad(grp.RRt, *_weights(grp))[0]
+
+    def fwd(theta):
+        sys_.set_ttm_params(theta)
+        o = sys_.forces(grp.RRt, grp.RRc, t=1)
+        return [(o["En"], o["dRR"])]
+
+    for j in range(theta0.size):
+        tp, tm, h2 = _pm(theta0, j)
+        fd, noise = _central(fwd(tp), fwd(tm), [grp], h2)
+        err = abs(fd - gc[j])
+        print(f"oracle model {model} param {j:2d} tangent {gc[j]: .10e} fd {fd: .10e} "
+              f"(rounding bound {noise:.1e})", flush=True)
+        if fd == 0.0 and gc[j] == 0.0:
+            continue
+        assert err < FD_REL * abs(gc[j]) + noise, j
+
+
+def _weights(grp):
+    r = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+    return 2.0 * (r["Upott"] - grp.E_target), 2.0 * (r["dRRt"] - grp.G_target)
+
+
+def test_fit_step_reduces_loss(fitset):
+    """One gradient step of the torch driver over PES and TTM parameters
+    together lowers the loss."""
+    import torch
+    from pimd_amd.fit import loss_and_grad, pes_loss
+    model, groups, theta0, pes0, L0, g = fitset
+    full = np.concatenate([pes0, theta0])
+    th = torch.tensor(full, dtype=torch.float64, requires_grad=True)
+    loss = pes_loss(th, groups)
+    loss.backward()
+    gall = th.grad.numpy()
+    assert np.allclose(gall[pes0.size:], g, rtol=1e-12, atol=0)
+    # step in the TTM parameters only, scaled per parameter
+    d = np.zeros_like(full)
+    d[pes0.size:] = -g * theta0 ** 2
+    slope = float(np.dot(gall, d))
+    assert slope < 0
+    L1, _ = loss_and_grad(groups, full + 1e-3 * L0 / abs(slope) * d, grad=False)
+    loss_and_grad(groups, full, grad=False)
+    print("loss", L0, "->", L1, flush=True)
+    assert L1 < L0

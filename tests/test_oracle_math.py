@@ -224,3 +224,36 @@ def test_pes_parameter_form(oracle_lib):
         e2, d2 = oracle_lib.pes_monomer(r1, box, th0 + 2 * v)
         assert abs((e2 - e1) - (e1 - e)) < 1e-9 * max(1.0, abs(e2 - e))
         assert np.max(np.abs((d2 - d1) - (d1 - d))) < 1e-9 * max(1.0, np.max(np.abs(d2 - d)))
+
+
+def test_ttm_parameter_override(oracle_lib):
+    """The oracle's parameter override: the defaults reproduce the built-in
+    model; the energy is affine in vdwC and vdwD (Uvdw and its long-range
+    correction are linear in them) and the gradient does not depend on
+    polarO / polarH in TTM3-F (no dipoles on O and H)."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    RRt, _, RRc, L = make_system(16, 2, seed=91, lattice=(4, 2, 2))
+    p = TTMParams(pot_model=3, nwaters=16, nbeads=2, box=(L, L, L), rc=-1.0,
+                  eps_ewald=1e-6).resolved()
+    s = oracle_lib.System(p)
+    th0 = s.ttm_params()
+    assert np.allclose(th0[[2, 3, 7, 11, 12, 13]], [-0.72298855e+03, 0.10211829e+06, 1.444, 0.175,
+                                                   0.175, 0.5])
+    f0 = s.forces(RRt, RRc)
+    s.set_ttm_params(th0)
+    f1 = s.forces(RRt, RRc)
+    assert np.array_equal(f0["En"], f1["En"]) and np.array_equal(f0["dRR"], f1["dRR"])
+    for j, d in ((2, 5.0), (3, 1e3)):
+        e = []
+        for m in range(3):
+            th = th0.copy()
+            th[j] += m * d
+            s.set_ttm_params(th)
+            e.append(s.forces(RRt, RRc)["En"])
+        assert np.max(np.abs(e[2] - 2 * e[1] + e[0])) < 1e-9 * np.max(np.abs(e[1] - e[0])), j
+    th = th0.copy()
+    th[5], th[6] = 0.7, 0.4
+    s.set_ttm_params(th)
+    f2 = s.forces(RRt, RRc)
+    assert np.array_equal(f2["dRR"], f0["dRR"]) and np.array_equal(f2["En"], f0["En"])
