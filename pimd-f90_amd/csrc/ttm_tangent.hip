@@ -193,6 +193,24 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return r;
 }
 
+// Lanes per site of a loop over nsites sites whose per-site work is a sum
+// over k-vectors: the largest power of two (at most a wave) such that the
+// groups of all sites fit in the workgroup.  Each group splits the k-vectors
+// of its site and combines its partial sums with group_sum.
+__device__ __forceinline__ int site_lanes(int nsites) {
+  int g = 1;
+  while (g < 64 && 2 * g * nsites <= TAN_BLOCK) g *= 2;
+  return g;
+}
+
+__device__ __forceinline__ Dual group_sum(Dual x, int G) {
+  for (int o = G >> 1; o > 0; o >>= 1) {
+    x.v += __shfl_xor(x.v, o);
+    x.d += __shfl_xor(x.d, o);
+  }
+  return x;
+}
+
 // (3 t2 r r^T - t1) v, get_dd3 applied (pot_ttm.f90:633-647).
 __device__ __forceinline__ void dd3d(const double r[3], Dual t1, Dual t2, const Dual v[3], Dual out[3]) {
   const Dual t = 3.0 * t2;
@@ -363,9 +381,10 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
     Sq[2 * k + 1] = iQ;
   }
   __syncthreads();
-  for (int s = tid; s < ns; s += TAN_BLOCK) {
+  const int Gs = site_lanes(ns), Gd = site_lanes(nds);
+  for (int s = tid / Gs; s < ns; s += TAN_BLOCK / Gs) {
     Dual ef[3] = {0.0, 0.0, 0.0}, pv = 0.0;
-    for (int k = 0; k < nk; ++k) {
+    for (int k = tid % Gs; k < nk; k += Gs) {
       const int4 h = a.khkl[k];
       const double4 kv = a.kv[k];
       const double p4vAk = kv.w;
@@ -378,8 +397,12 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       ef[2] -= t * kv.z;
       pv += p4vAk * (ck * rQ + sk * iQ);
     }
-    for (int k = 0; k < 3; ++k) Efq[3 * s + k] += ef[k];
-    if (s >= N) phi[s] += pv;
+    for (int k = 0; k < 3; ++k) ef[k] = group_sum(ef[k], Gs);
+    pv = group_sum(pv, Gs);
+    if (tid % Gs == 0) {
+      for (int k = 0; k < 3; ++k) Efq[3 * s + k] += ef[k];
+      if (s >= N) phi[s] += pv;
+    }
   }
   __syncthreads();
 
@@ -456,10 +479,10 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
     }
     __syncthreads();
     double ss = 0.0;
-    for (int i = tid; i < nds; i += TAN_BLOCK) {
+    for (int i = tid / Gd; i < nds; i += TAN_BLOCK / Gd) {
       const int ti = tdip + i / N, s = ti * N + i % N;
-      Dual e[3] = {Efd[3 * i], Efd[3 * i + 1], Efd[3 * i + 2]};
-      for (int k = 0; k < nk; ++k) {
+      Dual e[3] = {0.0, 0.0, 0.0};
+      for (int k = tid % Gd; k < nk; k += Gd) {
         const int4 h = a.khkl[k];
         const double4 kv = a.kv[k];
         double ck, sk;
@@ -469,6 +492,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         e[1] -= kv.y * t;
         e[2] -= kv.z * t;
       }
+      for (int k = 0; k < 3; ++k) e[k] = Efd[3 * i + k] + group_sum(e[k], Gd);
+      if (tid % Gd != 0) continue;
       for (int k = 0; k < 3; ++k) {
         const Dual mu = dip[3 * i + k];
         const Dual ek = e[k] + Efq[3 * s + k] + c.fac_dd * mu;
@@ -565,7 +590,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
 
   // ---- reciprocal space, dipole forces (ewald_std_qd, ewald.f90:123-226); the
   // dipole structure factor is the one of the last SOR iteration's input
-  for (int s = tid; s < ns; s += TAN_BLOCK) {
+  for (int s = tid / Gs; s < ns; s += TAN_BLOCK / Gs) {
     const int iw = s % N, ti = s / N;
     const bool sdip = ti >= tdip && ti < tend, schg = s >= N;
     const Dual qs = q[s];
@@ -573,7 +598,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
     if (sdip)
       for (int k = 0; k < 3; ++k) mu[k] = dip[3 * ((ti - tdip) * N + iw) + k];
     Dual g[3] = {0.0, 0.0, 0.0}, pv = 0.0;
-    for (int k = 0; k < nk; ++k) {
+    for (int k = tid % Gs; k < nk; k += Gs) {
       const int4 h = a.khkl[k];
       const double4 kv = a.kv[k];
       const double p4vAk = kv.w;
@@ -595,8 +620,12 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         g[2] -= t * kv.z;
       }
     }
-    for (int k = 0; k < 3; ++k) dR[3 * s + k] += g[k];
-    phi[s] += pv;
+    for (int k = 0; k < 3; ++k) g[k] = group_sum(g[k], Gs);
+    pv = group_sum(pv, Gs);
+    if (tid % Gs == 0) {
+      for (int k = 0; k < 3; ++k) dR[3 * s + k] += g[k];
+      phi[s] += pv;
+    }
   }
   __syncthreads();
 
