@@ -183,6 +183,33 @@ def run_form(p, RRt, PPt, RRc, args):
     return out
 
 
+def run_batched(p, nrep, args, seed=2024):
+    """nrep replicas of the workload (seeds seed, seed + 1, ...) stepped side
+    by side as one batch (pimd_batch_md_run, one stream per replica); whole-job
+    molecule-bead force evaluations per second."""
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PIMDIntegrator, ReplicaBatch, TTMForceField
+    reps = []
+    for r in range(nrep):
+        RRt, PPt, RRc, L = make_system(p.nwaters, p.nbeads, seed=seed + r)
+        ff = TTMForceField(p)
+        integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=1,
+                               centroid_thermostat=True, tau_centroid=0.1)
+        integ.set_state(RRt, PPt, ff.full_bead_forces(RRt, RRc)["dRRt"])
+        reps.append((ff, integ))
+    batch = ReplicaBatch([i for _, i in reps])
+    seeds = [99 + r for r in range(nrep)]
+    batch.run(args.warmup, seeds, t0=1)
+    t0 = time.perf_counter()
+    batch.run(args.steps, seeds, t0=1 + args.warmup)
+    s = (time.perf_counter() - t0) / args.steps
+    batch.close()
+    for ff, _ in reps:
+        ff.close()
+    return {"replicas": nrep, "ms_per_step": round(1000 * s, 4),
+            "value": round(nrep * p.nwaters * p.nbeads / s, 1)}
+
+
 def force_only(ff, integ, p, steps):
     """The all-beads force call alone (full_bead_forces equivalent) on the
     device-resident coordinates of the last MD step, dipole predictor off
@@ -494,6 +521,8 @@ def main():
     ap.add_argument("--precision", default="both", choices=("fp64", "mixed", "both"),
                     help="form(s) to time; with both, value is the mixed form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--batch", type=int, default=2,
+                    help="replicas of the workload in the batched measurement (1 skips it)")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=2000,
                     help="length of the mixed-vs-FP64 trajectory report (0 skips it)")
@@ -556,6 +585,16 @@ def main():
                        "force_only": a["force_only"],
                        "kernels_ms_per_step": _kern(a, args.steps)}
         out["mixed_over_fp64"] = round(m["value"] / a["value"], 3)
+        if args.batch > 1:
+            # several replicas of the same workload side by side: the batched
+            # forms against the plain (one system) FP64 form
+            bat = {}
+            for f in forms:
+                q = TTMParams(**{**p.__dict__, "precision": 1 if f == "mixed" else 0})
+                bat[f] = run_batched(q, args.batch, args)
+            bat["mixed_over_plain_fp64"] = round(bat["mixed"]["value"] / a["value"], 3)
+            bat["fp64_over_plain_fp64"] = round(bat["fp64"]["value"] / a["value"], 3)
+            out["batched"] = bat
         out["mixed_error_vs_fp64"] = {
             "force_rms_rel": float(np.sqrt(np.mean(df ** 2)) / np.sqrt(np.mean(a["f0"]["dRRt"] ** 2))),
             "force_max_rel": float(np.max(np.abs(df)) / np.max(np.abs(a["f0"]["dRRt"]))),

@@ -176,6 +176,17 @@ int pimd_state_forces(pimd_ctx *ctx, int t);
 int pimd_md_run(pimd_ctx *ctx, int t0, int nsteps, const double *noise, uint64_t seed,
                 double *Upot, double *virial, double *virialc, double *uk);
 
+/* Per-step observables of the last pimd_md_run / pimd_batch_md_run, as
+ * write_out computes them after each step's force call
+ * (InputOutput.f90:556-621, estimators.f90:30-96): obs[nsteps][PIMD_NOBS] =
+ * sys_temp (K); quantum virial energy (kcal/mol per molecule) and pressure
+ * (bar); simple (primitive) energy and pressure; total molecular dipole
+ * x, y, z and the sum of molecular dipole magnitudes (Debye, each molecule's
+ * dipole averaged over beads; NaN in contracted MD, which has no per-bead
+ * dipoles). */
+#define PIMD_NOBS 9
+int pimd_md_observables(pimd_ctx *ctx, int nsteps, double *obs);
+
 /* ---- batches of independent systems ------------------------------------
  * No counterpart in the reference, which runs one system per job.  A batch
  * runs several contexts (replicas of any size, model and bead count, each

@@ -14,6 +14,12 @@
 !   mode 7  setup      init_pot + ewald_set tables (potential.f90:43-199)
 !   mode 8  monomer    pot_nasa + dms_nasa         (nasa.f90:2-293)
 !   mode 9  contracted contracted_MD step order    (MD.f90:276-449)
+!
+! Mode 5 also writes, per step, the observables of write_out
+! (InputOutput.f90:556-621) from the reference's estimators
+! (estimators.f90:30-96): temperature, quantum virial and simple energy and
+! pressure, the total molecular dipole and the sum of the molecular dipole
+! magnitudes (Debye).
 program ref_driver
 use consts
 use system_mod
@@ -21,6 +27,7 @@ use pot_mod
 use nasa_mod, only: dms_param1
 use NormalModes
 use Langevin
+use estimators, only: quantum_virial_estimators, simple_quantum_estimators
 use math, only: erfc, gammq2_3, rand_norm
 use main_stuff, only: RRt, PPt, dRRt, RRc, PPc, delt, delt2, uk, imassO, imassH, &
                       temp, tau, tau_centroid, s, sbead, global_chain_length, &
@@ -42,6 +49,8 @@ logical :: have_hist
 double precision, allocatable :: dRRfast(:,:,:), Upostep(:)
 double precision :: deltfast, delt2fast, virialmon, virialcmon, roh1(3), roh2(3)
 integer :: intra, tintra, iw, iO, iH1, iH2
+double precision :: Usum, visum, vcsum, sys_temp, eq, pq, es, ps, dipmag, dipmom(3)
+double precision, allocatable :: dipIt(:,:,:)
 
 call get_command_argument(1, fin)
 call get_command_argument(2, fout)
@@ -157,7 +166,7 @@ case (5)
    call seed_rng(ih(10))
    n = 3*Natoms*Nbeads
    allocate(noise(n), dRR(3,Natoms), dipI(3,Nwaters), dipE(3,Nwaters), chgb(Natoms))
-   allocate(hist(3, fd:ld, 4, nb))
+   allocate(hist(3, fd:ld, 4, nb), dipIt(3, Nwaters, nb))
    hist = 0.d0
    call calc_uk
    do it = 1, nsteps
@@ -189,15 +198,39 @@ case (5)
       PPc = sum(PPt,3)/Nbeads
       call PBCs(RRt, RRc)
       ! force_calc.f90:26-106 (serial order, one bead at a time)
+      Usum = 0.d0; visum = 0.d0; vcsum = 0.d0
       do ib = 1, nb
          if (per_bead_hist /= 0) tx_dip = hist(:,:,:,ib)
          call pot_ttm(RRt(:,:,ib), RRc, En, dRR, vtmp, vc, dipI, dipE, chgb, it)
          if (per_bead_hist /= 0) hist(:,:,:,ib) = tx_dip
          dRRt(:,:,ib) = dRR
+         dipIt(:,:,ib) = dipI
          write(11) En, vtmp(1,1)+vtmp(2,2)+vtmp(3,3), vc
+         ! force_calc.f90:94-97
+         Usum = Usum + En
+         visum = visum + (vtmp(1,1)+vtmp(2,2)+vtmp(3,3))
+         vcsum = vcsum + vc
       enddo
+      vcsum = vcsum/Nbeads
       ! MD.f90:85-106
       call calc_uk
+      ! write_out (InputOutput.f90:556-621)
+      sys_temp = TEMPFACTOR*uk/(Natoms*Nbeads*Nbeads)
+      call quantum_virial_estimators(RRt, visum, vcsum, eq, pq, sys_temp, Usum)
+      call simple_quantum_estimators(RRt, visum, es, ps, sys_temp, Usum)
+      dipIt = dipIt*DEBYE/CHARGECON
+      dipmom = 0.d0
+      dipmag = 0.d0
+      do iw = 1, Nwaters
+         do ia = 1, 3
+            dipI(ia,iw) = sum(dipIt(ia,iw,:))/Nbeads
+         enddo
+      enddo
+      dipmom(:) = sum(dipI(:,:), dim=2)
+      do iw = 1, Nwaters
+         dipmag = dipmag + dsqrt(dot_product(dipI(:,iw), dipI(:,iw)))
+      enddo
+      write(11) sys_temp, eq, pq, es, ps, dipmom, dipmag
       PPt = PPt - MASSCON*dRRt*delt2
       if (BEADTHERMOSTAT) then
          if (ih(12) == 1) call record_noise(noise, n)

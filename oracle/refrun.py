@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "_ref", "ref_driver")
+REF = "/root/reference"   # the reference sources (fixture generation only)
 
 
 def available() -> bool:
@@ -157,7 +158,8 @@ def nose_hoover(uk, vxi, tau, delt_fs, L, temp):
 
 
 def trajectory(p, RRt, PPt, dRRt, nsteps, temp, delt_fs, bead_thermostat=1, centroid=False,
-               tau_centroid=0.1, global_nh=False, tau=0.1, seed=4242, per_bead_hist=True):
+               tau_centroid=0.1, global_nh=False, tau=0.1, seed=4242, per_bead_hist=True,
+               setnm=0.0):
     """PIMD_VV steps (MD.f90:35-110) with forces from pot_ttm, bead by bead.
 
     bead_thermostat: 0 none, 1 Langevin (normals recorded), 2 NM Nose-Hoover.
@@ -165,7 +167,7 @@ def trajectory(p, RRt, PPt, dRRt, nsteps, temp, delt_fs, bead_thermostat=1, cent
     P, na = p.nbeads, p.natoms
     blob = _header(p, 5, nsteps=nsteps, temp=temp, delt_fs=delt_fs, bead_tt=bead_thermostat,
                    centroid=int(centroid), tau_centroid=tau_centroid, glob_t=int(global_nh),
-                   tau=tau, seed=seed, per_bead_hist=int(per_bead_hist))
+                   tau=tau, seed=seed, per_bead_hist=int(per_bead_hist), setnm=setnm)
     for a in (RRt, PPt, dRRt):
         blob += np.ascontiguousarray(a, dtype=np.float64).tobytes()
     out, iters, _ = _run(blob)
@@ -176,6 +178,9 @@ def trajectory(p, RRt, PPt, dRRt, nsteps, temp, delt_fs, bead_thermostat=1, cent
         if bead_thermostat == 1:
             st["noise1"] = r.take(na, P, 3)
         st["Upot"] = np.array([r.take(3) for _ in range(P)])  # En, virial, virialc per bead
+        # sys_temp, virial energy and pressure, simple energy and pressure,
+        # total dipole (3), sum of dipole magnitudes
+        st["obs"] = r.take(9)
         if bead_thermostat == 1:
             st["noise2"] = r.take(na, P, 3)
         st["uk"] = r.take()
@@ -211,3 +216,23 @@ def contracted_trajectory(p, RRt, PPt, dRRt, nsteps, intra, temp, delt_fs, bead_
         steps.append(st)
     return {"steps": steps, "RRt": r.take(P, na, 3), "PPt": r.take(P, na, 3),
             "dRRt": r.take(P, na, 3), "RRc": r.take(na, 3), "iters": np.array(iters)}
+
+
+INPUT_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "ref_input")
+IMG_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "ref_img")
+
+
+def parse_input(path):
+    """The reference's read_inputfile on an input file: {key: value} of every
+    variable it sets, or {"error": message} when it stops on a missing key."""
+    r = subprocess.run([INPUT_BIN, os.path.abspath(path)], capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.abspath(path)))
+    lines = r.stdout.splitlines()
+    if "#parsed" not in lines:
+        return {"error": r.stdout.strip()}
+    out = {"messages": [ln.strip() for ln in lines[:lines.index("#parsed")]]}
+    for ln in lines[lines.index("#parsed") + 1:]:
+        k, _, v = ln.partition(" ")
+        out[k] = v
+    return out
+

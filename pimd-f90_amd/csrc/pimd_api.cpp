@@ -69,9 +69,17 @@ struct pimd_ctx {
   bool nm_ready = false, lang_ready = false, md_ready = false, state_ready = false;
   std::vector<double> C, A11, A12, A21, A22, om, msf, c1, c2;
   pimd_md_params md{};
+  bool cmd = false;       // centroid MD (setNMfreq != 0, P > 1): kinetic energy in mode space
+  // one captured MD step, replayed for every step of a run
+  hipGraphExec_t graph = nullptr;
+  bool graph_ok = false;
+  bool graph_off = false;  // capture failed once: eager launches from then on
+  uint64_t graph_seed = 0;
+  double omegan = 0.0;    // ring frequency kB T P / hbar of the MD temperature
   double *d_vxi = nullptr, *d_s = nullptr, *d_step = nullptr;
   double *d_rec = nullptr;  // per-step records of md runs
   int rec_cap = 0;
+  int rec_steps = 0;        // steps recorded by the last run
   int *h_nactive = nullptr;
   // profiling
   bool prof = false;
@@ -153,7 +161,7 @@ void free_all(pimd_ctx *x) {
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, d.cnt, d.clk_own, x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -163,6 +171,7 @@ void free_all(pimd_ctx *x) {
     (void)hipEventDestroy(q.b);
   }
   for (auto e : x->pool) (void)hipEventDestroy(e);
+  if (x->graph) (void)hipGraphExecDestroy(x->graph);
   if (x->stream && !x->stream_borrowed) (void)hipStreamDestroy(x->stream);
   if (x->stream2) (void)hipStreamDestroy(x->stream2);
   if (x->d_rec) (void)hipFree(x->d_rec);
@@ -466,6 +475,15 @@ int alloc_system(pimd_ctx *x) {
   if ((rc2 = dalloc(&d.segk, segk.size())) != PIMD_OK) return rc2;
   HIPCHK(hipMemcpy(d.segs, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.segk, segk.data(), sizeof(int) * segk.size(), hipMemcpyHostToDevice));
+  // arrival counters of the kernels whose last block per bead finishes the
+  // bead's reduction: structure-factor blocks per (bead, segment block), then
+  // SOR field blocks per bead; every last block resets its counter
+  if ((rc2 = dalloc(&d.clk_own, 2)) != PIMD_OK) return rc2;
+  d.clk = d.clk_own;
+  HIPCHK(hipMemset(d.clk, 0, sizeof(int) * 2));
+  d.ncnt_sf = (size_t)c.P * ((c.nseg + 63) / 64);
+  if ((rc2 = dalloc(&d.cnt, d.ncnt_sf + c.P)) != PIMD_OK) return rc2;
+  HIPCHK(hipMemset(d.cnt, 0, sizeof(int) * (d.ncnt_sf + c.P)));
   std::vector<double> masses(na);
   for (size_t a = 0; a < na; ++a) masses[a] = (a % 3 == 0) ? x->p.mass_o : x->p.mass_h;
   HIPCHK(hipMemcpy(d.masses, masses.data(), sizeof(double) * na, hipMemcpyHostToDevice));
@@ -495,7 +513,9 @@ int dd_beside_sfac(pimd_ctx *x, hipStream_t s) {
 }
 
 // One pot_ttm evaluation per bead on device arrays (pot_ttm.f90:3-630).
-int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *Upot,
+// The MD step t of the call (dipole predictor, history slot) is read from the
+// device clock d.clk, set by the caller.
+int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
                double *dRR, double *virt, double *virialc, double *dipI, double *dipE,
                double *chg, int *iters) {
   const SysConst &c = x->c;
@@ -507,7 +527,7 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *
   x->timed("ewald_tables", [&] { launch_tables(c, d, s); });
   x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, s); });
   x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
-  x->timed("post_qq", [&] { launch_post_qq(c, d, t, s); });
+  x->timed("post_qq", [&] { launch_post_qq(c, d, s); });
   for (int it = 1; it <= c.maxiter; ++it) {
     if (c.overlap) {
       // pair_dd on the second stream, concurrent with the structure factor
@@ -518,8 +538,8 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *
       x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s); });
       x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
     }
+    // the field kernel's last block per bead also runs the convergence test
     x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
-    x->timed("converge", [&] { launch_converge(c, d, it, s); });
     // No host round trip: converged beads are switched off on the device
     // and every kernel of the loop skips them, so the remaining iterations
     // of a converged system are near-empty launches.
@@ -545,7 +565,7 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, int t, double *
   x->timed("sfac_v", [&] { launch_sfac(c, d, SF_V, s); });
   x->timed("field_qd", [&] { launch_field(c, d, FD_QD, 0, s); });
   x->timed("virial_k", [&] { launch_virk(c, d, s); });
-  x->timed("finalize", [&] { launch_finalize(c, d, t, dRR, dipI, dipE, s); });
+  x->timed("finalize", [&] { launch_finalize(c, d, dRR, dipI, dipE, s); });
   x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, iters, s); });
   HIPCHK(hipGetLastError());
   return PIMD_OK;
@@ -642,7 +662,8 @@ int pimd_forces_all_beads_dev(pimd_ctx *x, const double *RR, const double *RRc, 
   if (!x || !RR || !RRc) return fail(PIMD_EINVAL, "null argument");
   HIPCHK(hipSetDevice(x->p.device));
   DevBufs &d = x->d;
-  return run_forces(x, RR, RRc, t, Upot ? Upot : d.Upot, dRR ? dRR : d.dRR,
+  launch_set_clock(d.clk, t, 0, x->stream);
+  return run_forces(x, RR, RRc, Upot ? Upot : d.Upot, dRR ? dRR : d.dRR,
                     virt ? virt : d.virt, virialc ? virialc : d.virialc, dipI ? dipI : d.dipI,
                     dipE ? dipE : d.dipE, chg ? chg : d.chg, iters ? iters : d.iters_out);
 }
@@ -657,7 +678,8 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   hipStream_t s = x->stream;
   HIPCHK(hipMemcpyAsync(d.RR, RRt, sizeof(double) * P * na * 3, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.RRc, RRc, sizeof(double) * na * 3, hipMemcpyHostToDevice, s));
-  int rc = run_forces(x, d.RR, d.RRc, t, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
+  launch_set_clock(d.clk, t, 0, s);
+  int rc = run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
                       d.iters_out);
   if (rc != PIMD_OK) return rc;
   if (Upott) HIPCHK(hipMemcpyAsync(Upott, d.Upot, sizeof(double) * P, hipMemcpyDeviceToHost, s));
@@ -833,10 +855,13 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
   if (md->intra_timesteps > 0 && x->p.contraction)
     return fail(PIMD_EINVAL, "contracted MD needs a context created with contraction = 0");
   HIPCHK(hipSetDevice(x->p.device));
+  x->graph_ok = false;
   x->md = *md;
   const int P = x->c.P;
   const double delt = md->delt_fs / 1000.0, delt2 = delt / 2.0;
   const double omegan = KB_AMU * md->temp * P / HBAR;  // InputOutput.f90:140
+  x->omegan = omegan;
+  x->cmd = md->setNMfreq != 0.0 && P > 1;  // calc_uk's branch (main_stuff.f90:353)
   // contracted MD propagates the ring polymer with the fast step
   // delt / intra_timesteps (InputOutput.f90:379-385)
   const double delt_nm = md->intra_timesteps > 0 ? delt / md->intra_timesteps : delt;
@@ -860,6 +885,7 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
     cen->stream = x->stream;
     cen->stream_borrowed = true;
     x->cen = cen;
+    cen->d.clk = x->d.clk;  // the centroid's force call reads the step of this context's clock
     std::copy(x->ttm, x->ttm + TTM_NPARAM, cen->ttm);  // parameters set on this context
     apply_ttm_params(cen);
     const size_t n = (size_t)P * x->natoms * 3;
@@ -918,8 +944,10 @@ int pimd_state_get(pimd_ctx *x, double *RRt, double *PPt, double *dRRt, double *
 }  // extern "C"
 
 namespace pimd {
+// Upot, virial, virialc of the step's force call (force_calc.f90:94-97).
 __global__ void k_step_record(int P, const double *Upot, const double *virt, const double *virialc,
-                              const double *uk, double *out) {
+                              double *rec0, const int *clk) {
+  double *out = rec0 + (size_t)clk[1] * MD_REC;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double u = 0.0, v = 0.0, vc = 0.0;
   for (int b = 0; b < P; ++b) {
@@ -930,7 +958,11 @@ __global__ void k_step_record(int P, const double *Upot, const double *virt, con
   out[0] = u;
   out[1] = v;
   out[2] = vc / P;
-  out[3] = uk[0];
+}
+
+// The kinetic energy at the end of the step.
+__global__ void k_record_uk(const double *uk, double *rec0, const int *clk) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) rec0[(size_t)clk[1] * MD_REC + 3] = uk[0];
 }
 }  // namespace pimd
 
@@ -948,21 +980,47 @@ static int md_records(pimd_ctx *x, int nsteps) {
   if (x->d_rec) (void)hipFree(x->d_rec);
   x->d_rec = nullptr;
   x->rec_cap = 0;
-  int rc = dalloc(&x->d_rec, (size_t)nsteps * 4);
+  x->graph_ok = false;  // captured steps hold the record buffer's address
+  int rc = dalloc(&x->d_rec, (size_t)nsteps * MD_REC);
   if (rc == PIMD_OK) x->rec_cap = nsteps;
   return rc;
 }
 
-// Kinetic energy of the loaded state (needed by the first thermostat call).
-static void md_begin(pimd_ctx *x) {
+// calc_uk (main_stuff.f90:349-388) into d.uk.
+static void md_kinetic(pimd_ctx *x) {
   DevBufs &d = x->d;
-  x->timed("kinetic", [&] { launch_kinetic(x->c.P, x->natoms, d.masses, d.PP, d.uk, d.ukpart, x->stream); });
+  x->timed("kinetic", [&] {
+    if (x->cmd)
+      launch_kinetic_nm(x->c.P, x->natoms, d, d.PP, d.uk, d.ukpart, x->stream);
+    else
+      launch_kinetic(x->c.P, x->natoms, d.masses, d.PP, d.uk, d.ukpart, x->stream);
+  });
 }
 
-// Enqueues PIMD_VV step st (MD step t0 + st, MD.f90:35-110) on the context's
-// stream and records Upot, virial, virialc, uk into record slot st.  Does not
-// synchronise.  noise: NULL or the normals of this step, [2][Natoms][P][3].
-static int md_step_vv(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
+// Start of a run at MD step t0: the clock, and the kinetic energy of the
+// loaded state (needed by the first thermostat call).
+static void md_begin(pimd_ctx *x, int t0) {
+  launch_set_clock(x->d.clk, t0, 0, x->stream);
+  md_kinetic(x);
+}
+
+// write_out's observables of step st from its force record and the kinetic
+// energy after the force call; dipI NULL when the step has no bead dipoles.
+static void md_observables(pimd_ctx *x, const double *dipI) {
+  DevBufs &d = x->d;
+  const ObsArgs a{x->c.P, x->natoms, x->c.vol, x->omegan};
+  x->timed("observables", [&] {
+    launch_observables(a, d.RR, dipI, d.masses, d.uk, x->d_rec, d.clk, x->stream);
+  });
+}
+
+// Enqueues one PIMD_VV step (MD.f90:35-110) on the context's stream: the MD
+// step and the record slot come from the device clock, which the step
+// advances at its end, so the same launches serve every step (and can be
+// captured once).  Records Upot, virial, virialc, uk and the observables.
+// Does not synchronise.  noise: NULL or the normals of this step,
+// [2][Natoms][P][3].
+static int md_step_vv(pimd_ctx *x, const double *noise, uint64_t seed) {
   const int P = x->c.P, na = x->natoms;
   const size_t n = (size_t)P * na * 3;
   const pimd_md_params &md = x->md;
@@ -979,8 +1037,7 @@ static int md_step_vv(pimd_ctx *x, int t0, int st, const double *noise, uint64_t
       nz = d.noise;
     }
     x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, (uint64_t)(t0 + st) * 2 + half, x->p.mass_o,
-                      x->p.mass_h, s);
+      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half);
     });
   };
   thermostat(0);
@@ -994,29 +1051,32 @@ static int md_step_vv(pimd_ctx *x, int t0, int st, const double *noise, uint64_t
   else
     x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, delt, s); });
   x->timed("centroid_pbc", [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
-  int rc = run_forces(x, d.RR, d.RRc, t0 + st, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE,
-                      d.chg, d.iters_out);
+  int rc = run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
+                      d.iters_out);
   if (rc != PIMD_OK) return rc;
-  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  md_kinetic(x);
+  hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, s, P, d.Upot, d.virt, d.virialc,
+                     x->d_rec, d.clk);
+  md_observables(x, d.dipI);
   x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
   thermostat(1);
-  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  md_kinetic(x);
   if (md.global_nh) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
     x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
-  hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, s, P, d.Upot, d.virt, d.virialc, d.uk,
-                     x->d_rec + (size_t)st * 4);
+  hipLaunchKernelGGL(k_record_uk, dim3(1), dim3(64), 0, s, d.uk, x->d_rec, d.clk);
+  launch_advance_clock(d.clk, s);
   HIPCHK(hipGetLastError());
   return PIMD_OK;
 }
 
-// Enqueues contracted_MD step st (MD step t0 + st, MD.f90:276-449): the
+// Enqueues one contracted_MD step (MD.f90:276-449; clock as md_step_vv): the
 // monomer PES moves every bead in intra_timesteps inner steps of the ring
 // polymer, the intermolecular forces of the centroid act on every bead once
 // per step.  Records sum of bead energies, virial, centroid virial and uk.
 // The fast gradient is kept from one step to the next.
-static int md_step_contracted(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
+static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
   const int P = x->c.P, na = x->natoms, N = x->c.N;
   const size_t n = (size_t)P * na * 3;
   const pimd_md_params &md = x->md;
@@ -1036,8 +1096,7 @@ static int md_step_contracted(pimd_ctx *x, int t0, int st, const double *noise, 
       nz = d.noise;
     }
     x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, (uint64_t)(t0 + st) * 2 + half, x->p.mass_o,
-                      x->p.mass_h, s);
+      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half);
     });
   };
   auto centroid = [&] {
@@ -1064,43 +1123,111 @@ static int md_step_contracted(pimd_ctx *x, int t0, int st, const double *noise, 
   int rc = PIMD_OK;
   DevBufs &dc = cen->d;
   x->timed("centroid_forces", [&] {
-    rc = run_forces(cen, d.RRc, d.RRc, t0 + st, dc.Upot, dc.dRR, dc.virt, dc.virialc, dc.dipI,
-                    dc.dipE, dc.chg, dc.iters_out);
+    rc = run_forces(cen, d.RRc, d.RRc, dc.Upot, dc.dRR, dc.virt, dc.virialc, dc.dipI, dc.dipE,
+                    dc.chg, dc.iters_out);
   });
   if (rc != PIMD_OK) return rc;
   x->timed("broadcast", [&] { launch_broadcast_beads(P, na * 3, dc.dRR, d.dRR, s); });
-  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  md_kinetic(x);
+  launch_contract_record(P, N, d.mol, dc.Upot, dc.virt, dc.virialc, d.uk, x->d_rec, d.clk, s);
+  md_observables(x, nullptr);  // no per-bead dipoles in contracted MD
   x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
   thermostat(1);
-  x->timed("kinetic", [&] { launch_kinetic(P, na, d.masses, d.PP, d.uk, d.ukpart, s); });
+  md_kinetic(x);
   if (md.global_nh) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
     x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
-  launch_contract_record(P, N, d.mol, dc.Upot, dc.virt, dc.virialc, d.uk,
-                         x->d_rec + (size_t)st * 4, s);
+  hipLaunchKernelGGL(k_record_uk, dim3(1), dim3(64), 0, s, d.uk, x->d_rec, d.clk);
+  launch_advance_clock(d.clk, s);
   HIPCHK(hipGetLastError());
   return PIMD_OK;
 }
 
-static int md_step(pimd_ctx *x, int t0, int st, const double *noise, uint64_t seed) {
-  return x->md.intra_timesteps > 0 ? md_step_contracted(x, t0, st, noise, seed)
-                                   : md_step_vv(x, t0, st, noise, seed);
+static int md_step_eager(pimd_ctx *x, const double *noise, uint64_t seed) {
+  return x->md.intra_timesteps > 0 ? md_step_contracted(x, noise, seed)
+                                   : md_step_vv(x, noise, seed);
+}
+
+// A step with device normals can run as one replay of a captured graph of
+// md_step_eager, rebuilt when the launch arguments it bakes in change (seed,
+// launch shapes, parameters, buffers).  Host noise, polling, per-kernel
+// timing and the two-stream overlap keep the eager launches.
+static bool graph_eligible(const pimd_ctx *x, const double *noise) {
+  const int on = env_int("PIMD_MD_GRAPH", 1, 0, 1);  // read per step: a test switches it
+  const SysConst &c = x->c;
+  const bool cen_ok = !x->cen || (!x->cen->c.poll && !x->cen->c.overlap && !x->cen->prof);
+  return on && !noise && !x->prof && !c.poll && !c.overlap && cen_ok;
+}
+
+static int md_graph_build(pimd_ctx *x, uint64_t seed) {
+  if (x->graph) (void)hipGraphExecDestroy(x->graph);
+  x->graph = nullptr;
+  x->graph_ok = false;
+  hipStream_t s = x->stream;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = md_step_eager(x, nullptr, seed);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != PIMD_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PIMD_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  const hipError_t e2 = hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    x->graph = nullptr;
+    return fail(PIMD_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+  }
+  x->graph_ok = true;
+  x->graph_seed = seed;
+  return PIMD_OK;
+}
+
+static int md_step(pimd_ctx *x, const double *noise, uint64_t seed) {
+  if (x->graph_off || !graph_eligible(x, noise)) return md_step_eager(x, noise, seed);
+  if (!x->graph_ok || x->graph_seed != seed) {
+    if (md_graph_build(x, seed) != PIMD_OK) {
+      // capture is not available here: launch eagerly from now on (an error
+      // of the step itself shows again in the eager launch)
+      x->graph_off = true;
+      (void)hipGetLastError();
+      return md_step_eager(x, noise, seed);
+    }
+  }
+  HIPCHK(hipGraphLaunch(x->graph, x->stream));
+  return PIMD_OK;
 }
 
 // Copies nsteps records to the host arrays (any may be NULL); the stream must
 // be synchronised.
 static int md_copy_records(pimd_ctx *x, int nsteps, double *Upot, double *virial, double *virialc,
                            double *uk) {
+  x->rec_steps = std::max(nsteps, 0);
   if (nsteps <= 0) return PIMD_OK;
-  std::vector<double> rec((size_t)nsteps * 4);
-  HIPCHK(hipMemcpy(rec.data(), x->d_rec, sizeof(double) * nsteps * 4, hipMemcpyDeviceToHost));
+  std::vector<double> rec((size_t)nsteps * MD_REC);
+  HIPCHK(hipMemcpy(rec.data(), x->d_rec, sizeof(double) * nsteps * MD_REC, hipMemcpyDeviceToHost));
   for (int st = 0; st < nsteps; ++st) {
-    if (Upot) Upot[st] = rec[st * 4];
-    if (virial) virial[st] = rec[st * 4 + 1];
-    if (virialc) virialc[st] = rec[st * 4 + 2];
-    if (uk) uk[st] = rec[st * 4 + 3];
+    if (Upot) Upot[st] = rec[st * MD_REC];
+    if (virial) virial[st] = rec[st * MD_REC + 1];
+    if (virialc) virialc[st] = rec[st * MD_REC + 2];
+    if (uk) uk[st] = rec[st * MD_REC + 3];
   }
+  return PIMD_OK;
+}
+
+extern "C" int pimd_md_observables(pimd_ctx *x, int nsteps, double *obs) {
+  if (!x || !obs) return fail(PIMD_EINVAL, "null argument");
+  if (nsteps < 0 || nsteps > x->rec_steps)
+    return fail(PIMD_EINVAL, "the last run recorded %d steps, asked for %d", x->rec_steps, nsteps);
+  if (nsteps == 0) return PIMD_OK;
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  std::vector<double> rec((size_t)nsteps * MD_REC);
+  HIPCHK(hipMemcpy(rec.data(), x->d_rec, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+  for (int st = 0; st < nsteps; ++st)
+    for (int k = 0; k < PIMD_NOBS; ++k) obs[(size_t)st * PIMD_NOBS + k] = rec[(size_t)st * MD_REC + 4 + k];
   return PIMD_OK;
 }
 
@@ -1112,9 +1239,9 @@ extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise,
   HIPCHK(hipSetDevice(x->p.device));
   if ((rc = md_records(x, nsteps)) != PIMD_OK) return rc;
   const size_t n = (size_t)x->c.P * x->natoms * 3;
-  md_begin(x);
+  md_begin(x, t0);
   for (int st = 0; st < nsteps; ++st)
-    if ((rc = md_step(x, t0, st, noise ? noise + (size_t)st * 2 * n : nullptr, seed)) != PIMD_OK)
+    if ((rc = md_step(x, noise ? noise + (size_t)st * 2 * n : nullptr, seed)) != PIMD_OK)
       return rc;
   HIPCHK(hipStreamSynchronize(x->stream));
   return md_copy_records(x, nsteps, Upot, virial, virialc, uk);
@@ -1141,6 +1268,7 @@ static Shape get_shape(const pimd_ctx *x) {
   return Shape{x->c.g_qq, x->c.g_dd, x->c.g_fin, x->c.fd_cg, x->c.sf_sites};
 }
 static void set_shape(pimd_ctx *x, const Shape &sh) {
+  x->graph_ok = false;
   x->c.g_qq = sh.g_qq;
   x->c.g_dd = sh.g_dd;
   x->c.g_fin = sh.g_fin;
@@ -1154,12 +1282,13 @@ static int time_forces(pimd_ctx *x, double *scratch, hipEvent_t e0, hipEvent_t e
                        double &ms) {
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
-  int rc = run_forces(x, d.RR, d.RRc, 1, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
+  launch_set_clock(d.clk, 1, 0, s);
+  int rc = run_forces(x, d.RR, d.RRc, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
                       d.chg, d.iters_out);
   if (rc != PIMD_OK) return rc;
   HIPCHK(hipEventRecord(e0, s));
   for (int r = 0; r < reps; ++r)
-    if ((rc = run_forces(x, d.RR, d.RRc, 1, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
+    if ((rc = run_forces(x, d.RR, d.RRc, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
                          d.chg, d.iters_out)) != PIMD_OK)
       return rc;
   HIPCHK(hipEventRecord(e1, s));
@@ -1176,6 +1305,7 @@ static int time_forces(pimd_ctx *x, double *scratch, hipEvent_t e0, hipEvent_t e
 // restored around the trials.
 static int autotune_ctx(pimd_ctx *x, double *best_ms) {
   HIPCHK(hipSetDevice(x->p.device));
+  x->graph_ok = false;  // the shapes of a contracted context's centroid change below
   if (x->cen) {
     // contracted MD: the force call of a step is the centroid's
     const size_t nc = (size_t)x->natoms * 3;
@@ -1289,10 +1419,10 @@ extern "C" int pimd_batch_md_run(pimd_batch *b, int t0, int nsteps, const uint64
     if ((rc = md_check(b->ctx[r])) != PIMD_OK) return rc;
     if ((rc = md_records(b->ctx[r], nsteps)) != PIMD_OK) return rc;
   }
-  for (int r = 0; r < n; ++r) md_begin(b->ctx[r]);
+  for (int r = 0; r < n; ++r) md_begin(b->ctx[r], t0);
   for (int st = 0; st < nsteps; ++st)
     for (int r = 0; r < n; ++r)
-      if ((rc = md_step(b->ctx[r], t0, st, nullptr, seeds[r])) != PIMD_OK) return rc;
+      if ((rc = md_step(b->ctx[r], nullptr, seeds[r])) != PIMD_OK) return rc;
   for (int r = 0; r < n; ++r) HIPCHK(hipStreamSynchronize(b->ctx[r]->stream));
   for (int r = 0; r < n; ++r)
     if ((rc = md_copy_records(b->ctx[r], nsteps, Upot ? Upot + (size_t)r * nsteps : nullptr,
@@ -1363,6 +1493,7 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
     std::copy(theta, theta + TTM_NPARAM, y->ttm);
     apply_ttm_params(y);
   }
+  x->graph_ok = false;  // captured launches hold the model constants
   return PIMD_OK;
 }
 
@@ -1432,7 +1563,8 @@ extern "C" int pimd_state_forces(pimd_ctx *x, int t) {
   if (rc != PIMD_OK) return rc;
   HIPCHK(hipSetDevice(x->p.device));
   DevBufs &d = x->d;
-  return run_forces(x, d.RR, d.RRc, t, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
+  launch_set_clock(d.clk, t, 0, x->stream);
+  return run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
                     d.iters_out);
 }
 

@@ -21,6 +21,10 @@ constexpr double TWOPI = 6.28318530717958647688;
 constexpr double FOURPI = 12.56637061435917295376;
 constexpr double TSP = 1.12837916709551257390;   // 2/sqrt(pi)
 constexpr double SQPI = 1.77245385090551602729;
+constexpr double TEMPFACTOR = 0.80181484527;   // amu A^2/ps^2 -> K (consts.f90:31)
+constexpr double PRESSCON2 = 69477.0;          // kcal/mol/A^3 -> bar (consts.f90:26)
+constexpr double KB_KCAL = 0.0019872041;       // kcal/mol/K (consts.f90:32)
+constexpr double MASSCONI = 1.0 / 418.4;       // consts.f90:25
 
 // Fitted Partridge-Schwenke PES parameters of a context: the 245 combined
 // coefficients c5z (nasa.f90:45), then these five (nasa_mod.f90 values).
@@ -119,6 +123,10 @@ struct DevBufs {
   double *pes = nullptr;                        // [PES_NPARAM] PES parameters
   double *adj = nullptr;                        // adjoint inputs and partial sums
   double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
+  int *clk = nullptr;                           // device step clock {t, record slot}
+  int *clk_own = nullptr;                       // this context's clock (clk may be its owner's)
+  int *cnt = nullptr;                           // arrival counters: [P][sfac blocks], then [P]
+  size_t ncnt_sf = 0;
   double *tan = nullptr;                        // TTM parameter tangents: inputs, tables, workspace
   size_t tan_bytes = 0;
   double *red = nullptr;                        // scratch for reductions
@@ -132,13 +140,12 @@ void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s);
 void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s);
 void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s);
-void launch_post_qq(const SysConst &c, DevBufs &b, int t, hipStream_t s);
+void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s);
 void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s);
 void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s);
 void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_finalize(const SysConst &c, DevBufs &b, int t, double *dRR, double *dipI,
+void launch_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI,
                      double *dipE, hipStream_t s);
 void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
                         double *virialc, int *iters, hipStream_t s);
@@ -173,16 +180,30 @@ size_t tangent_cfg_doubles(const SysConst &c);
 size_t tangent_ws_duals(const SysConst &c);
 void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s);
 
+// Per-step observables (write_out): sizes and constants of the system.
+struct ObsArgs {
+  int P, natoms;
+  double volume, omegan;
+};
+constexpr int MD_REC = 13;   // per-step record: Upot, virial, virialc, uk, then 9 observables
+void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, double *uk,
+                       double *part, hipStream_t s);
+void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
+                        const double *masses, const double *uk, double *rec0, const int *clk,
+                        hipStream_t s);
+void launch_set_clock(int *clk, int t, int slot, hipStream_t s);
+void launch_advance_clock(int *clk, hipStream_t s);
+
 void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
                      double *dRR, double *mol, bool record, hipStream_t s);
 void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStream_t s);
 void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
                             const double *vtcen, const double *vccen, const double *uk,
-                            double *rec, hipStream_t s);
+                            double *rec0, const int *clk, hipStream_t s);
 void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s);
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
-                     hipStream_t s);
+                     hipStream_t s, const int *clk = nullptr, int half = 0);
 void launch_kick(int n, double *PP, const double *dRR, double f, hipStream_t s);
 void launch_scale(int n, double *PP, const double *sc, hipStream_t s);
 void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,

@@ -124,8 +124,11 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
                                                  const double *__restrict__ c2, double massO,
                                                  double massH, double *PP,
                                                  const double *__restrict__ noise, uint64_t seed,
-                                                 uint64_t counter) {
+                                                 uint64_t counter, const int *__restrict__ clk,
+                                                 int half) {
   extern __shared__ double lds[];
+  // in MD steps the counter follows the device step clock: step t, half 0 / 1
+  if (clk) counter = (uint64_t)clk[0] * 2 + half;
   const int CB = nm_cols(P);
   double *Cs = lds, *xp = Cs + P * P, *yp = xp + P * CB;
   const int ncols = natoms * 3, col0 = blockIdx.x * CB;
@@ -169,11 +172,33 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
 
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
-                     hipStream_t s) {
+                     hipStream_t s, const int *clk, int half) {
   const int CB = nm_cols(P), n = natoms * 3;
   const size_t lds = sizeof(double) * ((size_t)P * P + 2 * (size_t)P * CB);
   hipLaunchKernelGGL(k_langevin, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
-                     b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter);
+                     b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter, clk, half);
+}
+
+// The device step clock {MD step t, record slot} read by the force and
+// thermostat kernels of an MD step, so that one captured step replays for
+// every step.
+__global__ void k_set_clock(int *clk, int t, int slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clk[0] = t;
+    clk[1] = slot;
+  }
+}
+__global__ void k_advance_clock(int *clk) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clk[0] += 1;
+    clk[1] += 1;
+  }
+}
+void launch_set_clock(int *clk, int t, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, s, clk, t, slot);
+}
+void launch_advance_clock(int *clk, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance_clock, dim3(1), dim3(64), 0, s, clk);
 }
 
 // PPt = PPt - MASSCON*dRRt*delt2 (MD.f90:49, 94).
@@ -274,6 +299,131 @@ void launch_kinetic(int P, int natoms, const double *masses, const double *PP, d
   const int nb = (int)std::min<size_t>(KIN_BLOCKS, (n + 255) / 256);
   hipLaunchKernelGGL(k_kinetic_part, dim3(nb), dim3(256), 0, s, P, natoms, masses, PP, part);
   hipLaunchKernelGGL(k_kinetic_final, dim3(1), dim3(256), 0, s, part, nb, uk);
+}
+
+// calc_uk with centroid-MD mass scaling (main_stuff.f90:366-385): per atom
+// the normal-mode momenta p~_j = sum_k C[j][k] p_k (real2NM,
+// NormalModes.f90:308-322), uk = 1/2 sum_atoms sum_j |p~_j|^2 / (m msf_j).
+// One thread per atom, per-block partials, then the fixed-order final sum.
+__global__ __launch_bounds__(256) void k_kinetic_nm_part(int P, int natoms,
+                                                        const double *__restrict__ Cg,
+                                                        const double *__restrict__ msf,
+                                                        const double *__restrict__ masses,
+                                                        const double *__restrict__ PP, double *part) {
+  __shared__ double red[256];
+  const size_t stride = (size_t)natoms * 3;
+  double s = 0.0;
+  for (int a = blockIdx.x * 256 + threadIdx.x; a < natoms; a += gridDim.x * 256) {
+    const double im = 1.0 / masses[a];
+    for (int j = 0; j < P; ++j) {
+      double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+      for (int k = 0; k < P; ++k) {
+        const double cj = Cg[j * P + k];
+        const double *p = PP + k * stride + 3 * (size_t)a;
+        q0 = q0 + cj * p[0];
+        q1 = q1 + cj * p[1];
+        q2 = q2 + cj * p[2];
+      }
+      s += im * (q0 * q0 + q1 * q1 + q2 * q2) / msf[j];
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, double *uk,
+                       double *part, hipStream_t s) {
+  const int nb = std::min(KIN_BLOCKS, (natoms + 255) / 256);
+  hipLaunchKernelGGL(k_kinetic_nm_part, dim3(nb), dim3(256), 0, s, P, natoms, b.C, b.msf,
+                     b.masses, PP, part);
+  hipLaunchKernelGGL(k_kinetic_final, dim3(1), dim3(256), 0, s, part, nb, uk);
+}
+
+// Per-step observables of write_out (InputOutput.f90:556-621) from the step
+// record rec[0..2] = (Upot, virial, virialc) of the force call and the
+// kinetic energy after it:
+//   obs[0] sys_temp = TEMPFACTOR uk / (Natoms P^2)
+//   obs[1..2] quantum_virial_estimators energy, pressure (estimators.f90:30-49)
+//   obs[3..4] simple_quantum_estimators energy, pressure (estimators.f90:55-94)
+//   obs[5..7] total molecular dipole, obs[8] sum of molecular dipole
+//   magnitudes, Debye, molecular dipoles averaged over beads
+//   (InputOutput.f90:593-607); NaN when dipI is NULL.
+// One block; fixed-order sums.
+__global__ __launch_bounds__(256) void k_observables(ObsArgs a, const double *__restrict__ RR,
+                                                    const double *__restrict__ dipI,
+                                                    const double *__restrict__ masses,
+                                                    const double *__restrict__ uk,
+                                                    double *rec0, const int *__restrict__ clk) {
+  double *rec = rec0 + (size_t)clk[1] * MD_REC;
+  __shared__ double red[5][256];
+  const int P = a.P, na = a.natoms, N = na / 3;
+  const size_t stride = (size_t)na * 3;
+  // spring term K0 = sum_k sum_atoms m |R_k - R_(k-1)|^2, bead 0 against P-1
+  double k0 = 0.0;
+  for (size_t e = threadIdx.x; e < (size_t)P * na; e += 256) {
+    const int k = (int)(e / na), at = (int)(e % na);
+    const int km = k == 0 ? P - 1 : k - 1;
+    const double *r = RR + k * stride + 3 * (size_t)at, *rm = RR + km * stride + 3 * (size_t)at;
+    const double m = masses[at];
+    for (int c = 0; c < 3; ++c) {
+      const double d = r[c] - rm[c];
+      k0 += m * (d * d);
+    }
+  }
+  double dx = 0.0, dy = 0.0, dz = 0.0, dm = 0.0;
+  if (dipI) {
+    const double f = DEBYE / CHARGECON;
+    for (int w = threadIdx.x; w < N; w += 256) {
+      double v[3] = {0.0, 0.0, 0.0};
+      for (int b = 0; b < P; ++b)
+        for (int c = 0; c < 3; ++c) v[c] += dipI[((size_t)b * N + w) * 3 + c] * f;
+      for (int c = 0; c < 3; ++c) v[c] = v[c] / P;
+      dx += v[0];
+      dy += v[1];
+      dz += v[2];
+      dm += sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+  }
+  red[0][threadIdx.x] = k0;
+  red[1][threadIdx.x] = dx;
+  red[2][threadIdx.x] = dy;
+  red[3][threadIdx.x] = dz;
+  red[4][threadIdx.x] = dm;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int v = 0; v < 5; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double Upot = rec[0], virial = rec[1], virialc = rec[2];
+  const double temp = TEMPFACTOR * uk[0] / (double)(na * P * P);
+  const double p3v = PRESSCON2 * (1.0 / (3.0 * a.volume));
+  const double ke = (double)(1.5f * (float)na) * KB_KCAL * temp;
+  double *o = rec + 4;
+  o[0] = temp;
+  o[1] = (ke + 0.5 * (virial - virialc) / P + Upot / P) / N;
+  o[2] = p3v * (2 * ke - virialc / P) / N;
+  const double kes = (double)(1.5f * (float)na * (float)P) * KB_KCAL * temp * P;
+  const double K0 = 0.5 * red[0][0] * (a.omegan * a.omegan) * MASSCONI;
+  o[3] = (kes - K0 + Upot) / (double)(N * P);
+  o[4] = p3v * (2 * (kes - K0) - virial) / N;
+  const double nan = __builtin_nan("");
+  o[5] = dipI ? red[1][0] : nan;
+  o[6] = dipI ? red[2][0] : nan;
+  o[7] = dipI ? red[3][0] : nan;
+  o[8] = dipI ? red[4][0] : nan;
+}
+
+void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
+                        const double *masses, const double *uk, double *rec0, const int *clk,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_observables, dim3(1), dim3(256), 0, s, a, RR, dipI, masses, uk, rec0, clk);
 }
 
 // Nose_Hoover half step, chain length 2 (Nose_Hoover.f90:158-198), scalars on device.

@@ -273,7 +273,8 @@ __global__ __launch_bounds__(256) void k_contract_record(int P, int N,
                                                         const double *__restrict__ vtcen,
                                                         const double *__restrict__ vccen,
                                                         const double *__restrict__ uk,
-                                                        double *rec) {
+                                                        double *rec0, const int *__restrict__ clk) {
+  double *rec = rec0 + (size_t)clk[1] * MD_REC;
   __shared__ double red[3][256];
   double s[3] = {0, 0, 0};
   for (int i = threadIdx.x; i < P * N; i += 256)
@@ -295,9 +296,9 @@ __global__ __launch_bounds__(256) void k_contract_record(int P, int N,
 
 void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
                             const double *vtcen, const double *vccen, const double *uk,
-                            double *rec, hipStream_t s) {
+                            double *rec0, const int *clk, hipStream_t s) {
   hipLaunchKernelGGL(k_contract_record, dim3(1), dim3(256), 0, s, P, N, mol, Ucen, vtcen, vccen,
-                     uk, rec);
+                     uk, rec0, clk);
 }
 
 // ---------------------------------------------------------------------------
@@ -605,8 +606,9 @@ __device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, doub
 // registers.  A block takes a chunk of
 // sites of one bead; their phase tables and weights are staged in LDS (rows
 // padded so that lanes reading different rows of one site hit different
-// banks).  Per-chunk partial sums are combined in a fixed order by
-// k_sfac_reduce.  SF_Q and SF_D as above.
+// banks).  The last block of a (bead, segment block) to finish, found with
+// an arrival counter, sums the per-chunk partials of its segments in chunk
+// order into the structure factor.  SF_Q and SF_D as above.
 template <int MODE, int KZ>
 __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__restrict__ site,
                                                  const double2 *__restrict__ tab,
@@ -614,7 +616,8 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
                                                  const int4 *__restrict__ cols,
                                                  const int *__restrict__ colk,
                                                  const int *__restrict__ active,
-                                                 double *__restrict__ part, int sc, int scp) {
+                                                 double *part, int sc, int scp, int *cnt,
+                                                 double *__restrict__ out) {
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
   extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
@@ -642,10 +645,10 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
   }
   __syncthreads();
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= c.nseg) return;
-  const int4 cl = cols[col];
-  const int n = cl.w;
-  const int k0 = colk[col];
+  const bool live = col < c.nseg;
+  const int4 cl = cols[live ? col : 0];
+  const int n = live ? cl.w : 0;
+  const int k0 = colk[live ? col : 0];
   const int rowx = cl.x, rowy = c.hmax + 1 + abs(cl.y), rowz = c.hmax + c.kmax + 2 + abs(cl.z);
   const int rowz1 = c.hmax + c.kmax + 3;
   const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
@@ -697,6 +700,27 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
       o[2 * i] = are[i];
       o[2 * i + 1] = aim[i];
     }
+  // arrival: every lane publishes its partials, one lane counts the block
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  int *ctr = cnt + (size_t)b * gridDim.x + blockIdx.x;
+  if (threadIdx.x == 0) s_last = atomicAdd(ctr, 1) == nch - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const double *pp = part + (size_t)b * nch * c.nk * 2;
+  for (int i = 0; i < n; ++i) {
+    const size_t e = (size_t)(k0 + i) * 2;
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q < nch; ++q) {
+      re += pp[(size_t)q * c.nk * 2 + e];
+      im += pp[(size_t)q * c.nk * 2 + e + 1];
+    }
+    out[(size_t)b * c.nk * 2 + e] = re;
+    out[(size_t)b * c.nk * 2 + e + 1] = im;
+  }
+  if (threadIdx.x == 0) *ctr = 0;
 }
 
 // Structure factors with one k-vector per lane.  A block holds 256 k of one
@@ -818,11 +842,10 @@ static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s)
   sfac_chunking(c, M, nch, sc, scp);
   const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
   dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
+  // grid.x * P arrival counters fit the (bead, 64-segment block) allocation
   hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), sfac_lds(c, MODE, sc, scp), s, c,
-                     b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
-  dim3 g2((2 * c.nk + 255) / 256, c.P);
-  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, 2, b.active,
-                     MODE == SF_D ? 1 : 0, MODE == SF_Q ? b.Sq : b.Sm);
+                     b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp, b.cnt,
+                     MODE == SF_Q ? b.Sq : b.Sm);
 }
 
 template <int MODE>
@@ -895,8 +918,8 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
                                                    double *__restrict__ dip,
                                                    double *__restrict__ dip_prev,
                                                    const double *__restrict__ Efd,
-                                                   double *__restrict__ diff2,
-                                                   const int *__restrict__ active, int iter) {
+                                                   double *diff2, int *active, int *iters,
+                                                   int *nactive, int *cnt, int iter) {
   constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
   // [T][64] phase tables of the block's sites (later reused for the
   // reduction), then with KL the bead's k data: [nk] (w Re S, w Im S, kz, w)
@@ -1036,7 +1059,7 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
   red[(g * 4 + 2) * FD_SB + lane] = e2;
   red[(g * 4 + 3) * FD_SB + lane] = ph;
   __syncthreads();
-  if (g != 0 || !valid) return;
+  if (g == 0 && valid) {
   for (int w = 1; w < CG; ++w) {
     e0 += red[(w * 4 + 0) * FD_SB + lane];
     e1 += red[(w * 4 + 1) * FD_SB + lane];
@@ -1068,6 +1091,41 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
     }
     diff2[(size_t)b * c.N * c.nd + m] = d2;
   }
+  }
+  if (MODE != FD_D || iter > c.maxiter) return;
+  // Per-bead convergence test (pot_ttm.f90:380-385) by the bead's last block
+  // to finish: deltadip = DEBYE/CHARGECON/sqrt(Natoms) * ||mu_new - mu_old||,
+  // the squared differences summed in a fixed order.
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(cnt + b, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double sum = 0.0;
+  for (int i = threadIdx.x; i < M; i += 64 * CG) sum += diff2[(size_t)b * M + i];
+  double *rd = reinterpret_cast<double *>(lds_tab);  // [64 CG]
+  rd[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 32 * CG; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) rd[threadIdx.x] += rd[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double delta = c.stath * sqrt(rd[0]);
+    if (delta < c.peps) {
+      active[b] = 0;
+      iters[b] = iter;
+    } else if (iter >= c.maxiter) {
+      active[b] = 0;
+      iters[b] = -iter;
+    } else {
+      iters[b] = iter;
+      atomicAdd(&nactive[iter], 1);
+    }
+    cnt[b] = 0;
+  }
 }
 
 // Shared memory of k_field: the phase tables (reused for the CG-way
@@ -1083,11 +1141,13 @@ static void launch_field_cg(const SysConst &c, DevBufs &b, int iter, dim3 grid, 
   if (kl)
     hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), field_lds(c, MO, true), s,
                        c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
-                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
+                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, b.iters, b.nactive,
+                       b.cnt + b.ncnt_sf, iter);
   else
     hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), field_lds(c, MO, false),
                        s, c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
-                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
+                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, b.iters, b.nactive,
+                       b.cnt + b.ncnt_sf, iter);
 }
 
 template <int MO, int ML>
@@ -1124,9 +1184,10 @@ __global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restri
                                                const double *__restrict__ Efq, double *dR,
                                                double *dip, const double *__restrict__ hist,
                                                double *tally, int *active, int *iters,
-                                               int *nactive, int t) {
+                                               int *nactive, const int *__restrict__ clk) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
+  const int t = clk[0];  // MD step of this force call
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const size_t base = ((size_t)b * N + w) * 4;
   double ue = 0.0;
@@ -1166,10 +1227,10 @@ __global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restri
     for (int i = 0; i <= c.maxiter; ++i) nactive[i] = 0;
 }
 
-void launch_post_qq(const SysConst &c, DevBufs &b, int t, hipStream_t s) {
+void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s) {
   int n = c.P * c.N;
   hipLaunchKernelGGL(k_post_qq, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.phi, b.Efq,
-                     b.dR, b.dip, b.hist, b.tally, b.active, b.iters, b.nactive, t);
+                     b.dR, b.dip, b.hist, b.tally, b.active, b.iters, b.nactive, b.clk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1415,42 +1476,6 @@ void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s) {
                      b.dip_prev, b.dip, b.active);
 }
 
-// Per-bead convergence test (pot_ttm.f90:380-385): deltadip =
-// DEBYE/CHARGECON/sqrt(Natoms) * ||mu_new - mu_old||, summed in a fixed order.
-__global__ __launch_bounds__(256) void k_converge(SysConst c, const double *__restrict__ diff2,
-                                                 int *active, int *iters, int *nactive,
-                                                 int iter) {
-  __shared__ double red[256];
-  const int b = blockIdx.x;
-  if (!active[b]) return;
-  const int M = c.N * c.nd;
-  double s = 0.0;
-  for (int m = threadIdx.x; m < M; m += 256) s += diff2[(size_t)b * M + m];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double delta = c.stath * sqrt(red[0]);
-    if (delta < c.peps) {
-      active[b] = 0;
-      iters[b] = iter;
-    } else if (iter >= c.maxiter) {
-      active[b] = 0;
-      iters[b] = -iter;
-    } else {
-      iters[b] = iter;
-      atomicAdd(&nactive[iter], 1);
-    }
-  }
-}
-
-void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
-  hipLaunchKernelGGL(k_converge, dim3(c.P), dim3(256), 0, s, c, b.diff2, b.active, b.iters,
-                     b.nactive, iter);
-}
 
 // ---------------------------------------------------------------------------
 // Real-space forces of the induced dipoles (pot_ttm.f90:399-478): dipole-dipole
@@ -1708,11 +1733,12 @@ __global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restr
                                                 const double *__restrict__ Efq, double *phi,
                                                 double *dR, double *hist, double *tally,
                                                 double *dRR, double *dipI, double *dipE,
-                                                int t) {
+                                                const int *__restrict__ clk) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
+  const int t = clk[0];  // MD step of this force call
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const size_t base = ((size_t)b * N + w) * 4;
   Site A[4];
@@ -1836,15 +1862,15 @@ __global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restr
   vadd9(tl + T_VIRT, vt);
 }
 
-void launch_finalize(const SysConst &c, DevBufs &b, int t, double *dRR, double *dipI,
-                     double *dipE, hipStream_t s) {
+void launch_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI, double *dipE,
+                     hipStream_t s) {
   int n = c.P * c.N;
   if (c.model == 2)
     hipLaunchKernelGGL(k_finalize<2>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
-                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, t);
+                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, b.clk);
   else
     hipLaunchKernelGGL(k_finalize<3>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
-                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, t);
+                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, b.clk);
 }
 
 void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s) {

@@ -11,7 +11,7 @@ module pimd_ttm_c
   integer(c_int), parameter, public :: PIMD_OK = 0, PIMD_EINVAL = 1, PIMD_EDEVICE = 2, &
                                        PIMD_ENOMEM = 3, PIMD_ESTATE = 4
   integer(c_int), parameter, public :: PIMD_TTM21F = 2, PIMD_TTM3F = 3, PIMD_FP64 = 0, &
-                                       PIMD_MIXED = 1
+                                       PIMD_MIXED = 1, PIMD_NOBS = 9
 
   type, bind(C), public :: pimd_params
     integer(c_int) :: pot_model = 3
@@ -48,7 +48,7 @@ module pimd_ttm_c
             pimd_forces_all_beads, pimd_energy_terms, pimd_nm_init, pimd_nm_get, &
             pimd_nm_evolve, pimd_langevin_init, pimd_langevin, pimd_nose_hoover, &
             pimd_md_init, pimd_state_set, pimd_state_get, pimd_md_run, pimd_sync, &
-            pimd_error_message
+            pimd_md_observables, pimd_error_message
 
   interface
     integer(c_int) function pimd_ctx_create(p, ctx) bind(C, name="pimd_ctx_create")
@@ -147,6 +147,14 @@ module pimd_ttm_c
       type(c_ptr), value :: noise
       integer(c_int64_t), value :: seed
       real(c_double), intent(out) :: Upot(*), virial(*), virialc(*), uk(*)
+    end function
+    ! obs(PIMD_NOBS, nsteps): sys_temp, virial energy and pressure, simple
+    ! energy and pressure, total dipole (3), sum of dipole magnitudes
+    integer(c_int) function pimd_md_observables(ctx, nsteps, obs) bind(C, name="pimd_md_observables")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: obs(*)
     end function
     integer(c_int) function pimd_sync(ctx) bind(C, name="pimd_sync")
       import :: c_int, c_ptr

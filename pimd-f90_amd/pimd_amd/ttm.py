@@ -36,7 +36,7 @@ EXPORTS = (
     "pimd_sync", "pimd_batch_create", "pimd_batch_destroy", "pimd_batch_autotune",
     "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
     "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy", "pimd_ttm_params_get",
-    "pimd_ttm_params_set", "pimd_ttm_param_grad",
+    "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -45,6 +45,9 @@ TTM_PARAM_NAMES = ("vdwA", "vdwB", "vdwC", "vdwD", "vdwE", "polarO", "polarH", "
                    "polfacO", "polfacH", "polfacM", "aDD", "aCCaCD", "dms_param1", "dms_param2",
                    "dms_param3")
 TTM_NPARAM = len(TTM_PARAM_NAMES)
+# per-step observables of write_out (pimd_md_observables)
+OBS_NAMES = ("sys_temp", "virial_energy", "virial_press", "simple_energy", "simple_press",
+             "dip_x", "dip_y", "dip_z", "dip_mag_sum")
 
 
 class PimdParams(C.Structure):
@@ -113,6 +116,7 @@ def load_library(path: str = LIB_PATH):
         "pimd_ttm_params_get": [vp, dp],
         "pimd_ttm_params_set": [vp, dp],
         "pimd_ttm_param_grad": [vp, dp, dp, dp, dp, dp, dp],
+        "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
     }
@@ -373,6 +377,12 @@ class PIMDIntegrator:
         _check(load_library().pimd_md_run(self.ff.handle, int(t0), int(nsteps), _ptr(nz), seed,
                                           _ptr(o["Upot"]), _ptr(o["virial"]), _ptr(o["virialc"]),
                                           _ptr(o["uk"])))
+        return o
+
+    def observables(self, nsteps: int) -> np.ndarray:
+        """[nsteps][9] observables of the last run's steps, in the order of OBS_NAMES."""
+        o = np.zeros((nsteps, len(OBS_NAMES)))
+        _check(load_library().pimd_md_observables(self.ff.handle, int(nsteps), _ptr(o)))
         return o
 
 

@@ -91,23 +91,30 @@ def predictor_fixture():
                           bead_thermostat=0, per_bead_hist=True)
     save("traj_ttm3f_predictor_n16_p2", RRt=RRt, PPt=PPt,
          Upot=np.array([s["Upot"] for s in r["steps"]]), uk=np.array([s["uk"] for s in r["steps"]]),
+         obs=np.array([s["obs"] for s in r["steps"]]),
          RRt_end=r["RRt"], PPt_end=r["PPt"], dRRt_end=r["dRRt"], iters=r["iters"],
          **params_dict(p))
 
 
 def traj_fixture():
-    """PIMD_VV with the NM Langevin thermostat; the normals it consumed are recorded."""
-    for name, model, N, P, lat, glob in [("traj_ttm3f_langevin_n8_p4", 3, 8, 4, (2, 2, 2), False),
-                                         ("traj_ttm21f_langevin_nh_n8_p2", 2, 8, 2, (2, 2, 2), True)]:
+    """PIMD_VV with the NM Langevin thermostat; the normals it consumed are
+    recorded.  The last case is centroid MD (setNMfreq 1500 cm^-1: scaled
+    normal-mode masses, mode-space kinetic energy) with the global
+    Nose-Hoover chain."""
+    for name, model, N, P, lat, glob, setnm in [
+            ("traj_ttm3f_langevin_n8_p4", 3, 8, 4, (2, 2, 2), False, 0.0),
+            ("traj_ttm21f_langevin_nh_n8_p2", 2, 8, 2, (2, 2, 2), True, 0.0),
+            ("traj_ttm3f_cmd_nh_n8_p4", 3, 8, 4, (2, 2, 2), True, 1500.0)]:
         RRt, PPt, RRc, L = make_system(N, P, seed=31 + N + P, lattice=lat)
         p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
                       eps_ewald=1e-6, mass_o=15.995, mass_h=1.008).resolved()
         dR0 = np.zeros_like(RRt)
         r = refrun.trajectory(p, RRt, PPt, dR0, nsteps=6, temp=300.0, delt_fs=0.25,
                               bead_thermostat=1, centroid=True, tau_centroid=0.05,
-                              global_nh=glob, tau=0.1, per_bead_hist=True)
+                              global_nh=glob, tau=0.1, per_bead_hist=True, setnm=setnm)
         noise = np.array([[s["noise1"], s["noise2"]] for s in r["steps"]])
-        save(name, RRt=RRt, PPt=PPt, noise=noise,
+        save(name, RRt=RRt, PPt=PPt, noise=noise, setnm=np.asarray(setnm),
+             obs=np.array([s["obs"] for s in r["steps"]]),
              Upot=np.array([s["Upot"] for s in r["steps"]]),
              uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
              dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], global_nh=np.asarray(glob),
@@ -131,6 +138,51 @@ def contracted_fixture():
              Upot=rec[:, 0], virial=rec[:, 1], virialc=rec[:, 2],
              uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
              dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], **params_dict(p))
+
+
+def io_fixtures():
+    """The reference's input-file reader on its own example inputs and on a
+    crafted one (tests/golden/inputs/quirks.inp), and its checkpoint writer and
+    reader on a seeded bead configuration."""
+    import json
+    import shutil
+    import subprocess
+    from pimd_amd.params import f32
+    d = os.path.join(OUT, "inputs")
+    os.makedirs(d, exist_ok=True)
+    for src in ("input32.inp", "input_new_64.inp", "test_input.inp",
+                "input_files/input4nm.inp", "input_files/inputdimer.inp"):
+        dst = os.path.join(d, os.path.basename(src))
+        shutil.copyfile(os.path.join(refrun.REF, src), dst)
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".inp"):
+            continue
+        r = refrun.parse_input(os.path.join(d, f))
+        with open(os.path.join(d, f[:-4] + ".json"), "w") as fh:
+            json.dump(r, fh, indent=1, sort_keys=True)
+        print("wrote inputs/" + f[:-4] + ".json")
+    # checkpoint: the reference's save_checkpoint text and load_checkpoint result
+    RRt, PPt, RRc, L = make_system(8, 3, seed=81, lattice=(2, 2, 2))
+    RRt = RRt + np.array([4.0, -3.1, 0.2])       # centroids outside the box
+    mO, mH = f32(15.994), f32(1.008)
+    tmp = os.path.join(d, "_tmp.bin")
+    with open(tmp, "wb") as fh:
+        fh.write(np.array([24, 3], np.int32).tobytes())
+        fh.write(np.array([mO, mH, L, L, L, 0.5, 0.25e-3, 40.0]).tobytes())
+        fh.write(np.ascontiguousarray(RRt).tobytes())
+        fh.write(np.ascontiguousarray(PPt).tobytes())
+    img = os.path.join(d, "beads_n8_p3.img")
+    subprocess.run([refrun.IMG_BIN, "write", tmp, img], check=True)
+    out = os.path.join(d, "_out.bin")
+    subprocess.run([refrun.IMG_BIN, "read", tmp, img, out], check=True)
+    o = np.fromfile(out)
+    n = 3 * 24 * 3
+    save("inputs/beads_n8_p3", RRt=RRt, PPt=PPt, box=np.array([L, L, L]), t=np.asarray(40),
+         delt_ps=np.asarray(0.25e-3), init_time=np.asarray(0.5), mass_o=np.asarray(mO),
+         mass_h=np.asarray(mH), RRt_loaded=o[:n].reshape(3, 24, 3),
+         PPt_loaded=o[n:2 * n].reshape(3, 24, 3), RRc_loaded=o[2 * n:].reshape(24, 3))
+    os.remove(tmp)
+    os.remove(out)
 
 
 def ring_fixtures():
@@ -192,5 +244,6 @@ if __name__ == "__main__":
     predictor_fixture()
     traj_fixture()
     contracted_fixture()
+    io_fixtures()
     ring_fixtures()
     scalar_fixtures()

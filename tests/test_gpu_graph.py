@@ -1,0 +1,77 @@
+"""MD steps replayed from a captured HIP graph against the same steps launched
+one kernel at a time (PIMD_MD_GRAPH=0): every replay, with host reads and
+synchronisations between the runs, must give the eager records, observables
+and state bit for bit (plain, centroid-MD and contracted steps, and a batch)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _integrator(model, N, P, seed, **md):
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PIMDIntegrator, TTMForceField
+    RRt, PPt, RRc, L = make_system(N, P, seed=seed)
+    p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0, eps_ewald=1e-6,
+                  guess_initdip=True)
+    ff = TTMForceField(p)
+    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=1,
+                           centroid_thermostat=True, tau_centroid=0.1, **md)
+    integ.set_state(RRt, PPt, ff.full_bead_forces(RRt, RRc)["dRRt"])
+    return integ
+
+
+def _runs(integ, graph: bool, chunks=(3, 2, 4)):
+    os.environ["PIMD_MD_GRAPH"] = "1" if graph else "0"
+    out, t0 = [], 1
+    try:
+        for n in chunks:
+            rec = integ.run(n, t0=t0, seed=1234)
+            obs = integ.observables(n)
+            st = integ.get_state()          # synchronises and reads back between replays
+            out.append((rec, obs, st))
+            t0 += n
+    finally:
+        os.environ.pop("PIMD_MD_GRAPH", None)
+    return out
+
+
+@pytest.mark.parametrize("case", ["ttm3f", "ttm21f_cmd_nh", "contracted"])
+def test_graph_replay_matches_eager(case):
+    kw = {"ttm3f": (3, 64, 8, {}),
+          "ttm21f_cmd_nh": (2, 32, 4, {"setNMfreq": 1500.0, "global_nh": True}),
+          "contracted": (3, 32, 4, {"intra_timesteps": 3})}[case]
+    model, N, P, md = kw
+    print("case", case, flush=True)
+    a = _runs(_integrator(model, N, P, 77, **md), graph=False)
+    b = _runs(_integrator(model, N, P, 77, **md), graph=True)
+    for (ra, oa, sa), (rb, ob, sb) in zip(a, b):
+        for k in ("Upot", "virial", "virialc", "uk"):
+            assert np.array_equal(ra[k], rb[k]), k
+        assert np.array_equal(oa, ob, equal_nan=True)
+        for k in ("RRt", "PPt", "dRRt", "RRc"):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_batch_graph_matches_eager():
+    from pimd_amd.ttm import ReplicaBatch
+    res = []
+    for graph in (False, True):
+        os.environ["PIMD_MD_GRAPH"] = "1" if graph else "0"
+        try:
+            integs = [_integrator(3, 32, 4, 90 + r) for r in range(3)]
+            batch = ReplicaBatch(integs)
+            r1 = batch.run(3, [5, 6, 7], t0=1)
+            r2 = batch.run(2, [5, 6, 7], t0=4)
+            res.append((r1, r2, [i.get_state()["RRt"] for i in integs]))
+            batch.close()
+        finally:
+            os.environ.pop("PIMD_MD_GRAPH", None)
+    (a1, a2, sa), (b1, b2, sb) = res
+    for k in ("Upot", "uk"):
+        assert np.array_equal(a1[k], b1[k]) and np.array_equal(a2[k], b2[k]), k
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)

@@ -90,11 +90,14 @@ def test_trajectory_vs_reference(name):
     nsteps = g["Upot"].shape[0]
     print("case", name, flush=True)
     ff = _ff(p)
+    setnm = float(g.get("setnm", 0.0))
     integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0,
                            bead_thermostat=1 if noise is not None else 0,
-                           centroid_thermostat=True, tau_centroid=0.05, global_nh=glob, tau=0.1)
+                           centroid_thermostat=True, tau_centroid=0.05, global_nh=glob, tau=0.1,
+                           setNMfreq=setnm)
     integ.set_state(g["RRt"], g["PPt"], None)
     rec = integ.run(nsteps, t0=1, noise=noise)
+    obs = integ.observables(nsteps)
     st = integ.get_state()
     errs = {"Upot": rel_err(rec["Upot"], g["Upot"][:, :, 0].sum(axis=1)),
             "uk": rel_err(rec["uk"], g["uk"]), "R": rel_err(st["RRt"], g["RRt_end"]),
@@ -102,6 +105,10 @@ def test_trajectory_vs_reference(name):
     print(name, errs, flush=True)
     assert errs["Upot"] < 1e-10 and errs["uk"] < 1e-10
     assert errs["R"] < 1e-12 and errs["P"] < 1e-10 and errs["dR"] < 1e-8
+    # write_out's observables (temperature, virial and simple estimators, dipoles)
+    oerr = [rel_err(obs[:, i], g["obs"][:, i]) for i in range(obs.shape[1])]
+    print(name, "observables rel err", np.max(oerr), flush=True)
+    assert max(oerr[:5]) < 1e-10 and max(oerr[5:]) < 1e-8
     if "RRc_end" in g:
         assert max_abs(st["RRc"], g["RRc_end"]) < 1e-12
 

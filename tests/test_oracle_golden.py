@@ -110,13 +110,38 @@ def test_nose_hoover_vs_reference(oracle_lib):
         assert max_abs(r["vxi"], [w0, w1]) < 1e-14
 
 
+TEMPFACTOR, PRESSCON2, KB_KCAL, DEBYE, CHARGECON = (0.80181484527, 69477.0, 0.0019872041,
+                                                   4.8033324, 18.22234397655801030455)
+
+
+def observables(p, R, dipI, Upot, virial, virialc, uk, omegan, masses):
+    """write_out's per-step observables (InputOutput.f90:556-621) from the
+    reference's estimators (estimators.f90:30-96), restated in numpy."""
+    P, na, N = p.nbeads, p.natoms, p.nwaters
+    vol = p.box[0] * p.box[1] * p.box[2]
+    temp = TEMPFACTOR * uk / (na * P * P)
+    ke = 1.5 * na * KB_KCAL * temp
+    qe = (ke + 0.5 * (virial - virialc) / P + Upot / P) / N
+    qp = PRESSCON2 * (1 / (3 * vol)) * (2 * ke - virialc / P) / N
+    d = R - np.roll(R, 1, axis=0)       # bead k against bead k-1, bead 0 against P-1
+    k0 = np.sum(masses[None, :, None] * d * d)
+    kes = 1.5 * na * P * KB_KCAL * temp * P
+    k0 = 0.5 * k0 * omegan ** 2 / 418.4
+    se = (kes - k0 + Upot) / (N * P)
+    sp = PRESSCON2 * (1 / (3 * vol)) * (2 * (kes - k0) - virial) / N
+    mol = (dipI * DEBYE / CHARGECON).sum(axis=0) / P
+    return np.array([temp, qe, qp, se, sp, *mol.sum(axis=0), np.sum(np.linalg.norm(mol, axis=1))])
+
+
 def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=None,
-                   centroid=True, tau_centroid=0.05, global_nh=False, tau=0.1):
-    """PIMD_VV (MD.f90:35-110) composed from the oracle's pieces."""
+                   centroid=True, tau_centroid=0.05, global_nh=False, tau=0.1, setnm=0.0):
+    """PIMD_VV (MD.f90:35-110) composed from the oracle's pieces, with the
+    per-step observables of write_out."""
     P, na = p.nbeads, p.natoms
     dt = delt_fs / 1000.0
     dt2 = dt / 2
-    nm = oracle_lib.nm_init(P, KB_AMUA2PS2PERK * temp * P / HBAR, dt)
+    omegan = KB_AMUA2PS2PERK * temp * P / HBAR
+    nm = oracle_lib.nm_init(P, omegan, dt, setnm)
     c1, c2 = oracle_lib.langevin_init(P, dt2, centroid, tau_centroid, P * temp, nm["omegalist"])
     masses = atom_masses(p.nwaters, p.mass_o, p.mass_h)
     sys_ = oracle_lib.System(p)
@@ -125,6 +150,10 @@ def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=N
     L = 3 * na * P
 
     def uk_of(Pm):
+        if setnm != 0.0 and P > 1:   # calc_uk in mode space (main_stuff.f90:366-385)
+            q = np.einsum("jk,kac->jac", nm["C"], Pm)
+            return 0.5 * np.sum((1.0 / masses)[None, :, None] * q ** 2
+                                / nm["MassScaleFactor"][:, None, None])
         return 0.5 * np.sum((1.0 / masses)[None, :, None] * Pm ** 2)
 
     uk = uk_of(Pm)
@@ -144,6 +173,8 @@ def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=N
         R = R - sh[None]
         f = sys_.forces(R, Rc, t=st + 1)
         dR = f["dRR"]
+        obs = observables(p, R, f["dipI"], f["En"].sum(), np.trace(f["virt"], axis1=1, axis2=2).sum(),
+                          f["virialc"].sum() / P, uk_of(Pm), omegan, masses)
         Pm = Pm - MASSCON * dR * dt2
         if noise is not None:
             Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 1], Pm)
@@ -152,7 +183,7 @@ def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=N
             r = oracle_lib.nose_hoover(uk, vxi, tau, dt2, L, temp * P)
             vxi, uk = r["vxi"], r["uk"]
             Pm = Pm * r["s"]
-        out.append((f["En"].sum(), uk))
+        out.append((f["En"].sum(), uk, *obs))
     return R, Pm, dR, np.array(out)
 
 
@@ -164,8 +195,12 @@ def test_trajectory_vs_reference(oracle_lib, name):
     glob = bool(g.get("global_nh", False))
     nsteps = g["Upot"].shape[0]
     R, Pm, dR, rec = oracle_pimd_vv(oracle_lib, p, g["RRt"], g["PPt"], np.zeros_like(g["RRt"]),
-                                    nsteps, 300.0, 0.25, noise=noise, global_nh=glob)
+                                    nsteps, 300.0, 0.25, noise=noise, global_nh=glob,
+                                    setnm=float(g.get("setnm", 0.0)))
     assert rel_err(rec[:, 0], g["Upot"][:, :, 0].sum(axis=1)) < 1e-12
+    assert rel_err(rec[:, 1], g["uk"]) < 1e-12
+    for i in range(9):
+        assert rel_err(rec[:, 2 + i], g["obs"][:, i]) < 1e-11, i
     assert rel_err(R, g["RRt_end"]) < 1e-12
     assert rel_err(Pm, g["PPt_end"]) < 1e-11
     assert max_abs(dR, g["dRRt_end"]) < 1e-10
