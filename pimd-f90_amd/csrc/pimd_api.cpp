@@ -161,7 +161,7 @@ void free_all(pimd_ctx *x) {
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, d.cnt, d.clk_own, x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, d.clk_own, d.obs_part, x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -475,15 +475,11 @@ int alloc_system(pimd_ctx *x) {
   if ((rc2 = dalloc(&d.segk, segk.size())) != PIMD_OK) return rc2;
   HIPCHK(hipMemcpy(d.segs, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.segk, segk.data(), sizeof(int) * segk.size(), hipMemcpyHostToDevice));
-  // arrival counters of the kernels whose last block per bead finishes the
-  // bead's reduction: structure-factor blocks per (bead, segment block), then
-  // SOR field blocks per bead; every last block resets its counter
+  // the device step clock {t, record slot}
+  if ((rc2 = dalloc(&d.obs_part, 64 * 5)) != PIMD_OK) return rc2;
   if ((rc2 = dalloc(&d.clk_own, 2)) != PIMD_OK) return rc2;
   d.clk = d.clk_own;
   HIPCHK(hipMemset(d.clk, 0, sizeof(int) * 2));
-  d.ncnt_sf = (size_t)c.P * ((c.nseg + 63) / 64);
-  if ((rc2 = dalloc(&d.cnt, d.ncnt_sf + c.P)) != PIMD_OK) return rc2;
-  HIPCHK(hipMemset(d.cnt, 0, sizeof(int) * (d.ncnt_sf + c.P)));
   std::vector<double> masses(na);
   for (size_t a = 0; a < na; ++a) masses[a] = (a % 3 == 0) ? x->p.mass_o : x->p.mass_h;
   HIPCHK(hipMemcpy(d.masses, masses.data(), sizeof(double) * na, hipMemcpyHostToDevice));
@@ -538,8 +534,8 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
       x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s); });
       x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
     }
-    // the field kernel's last block per bead also runs the convergence test
     x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
+    x->timed("converge", [&] { launch_converge(c, d, it, s); });
     // No host round trip: converged beads are switched off on the device
     // and every kernel of the loop skips them, so the remaining iterations
     // of a converged system are near-empty launches.
@@ -1010,7 +1006,7 @@ static void md_observables(pimd_ctx *x, const double *dipI) {
   DevBufs &d = x->d;
   const ObsArgs a{x->c.P, x->natoms, x->c.vol, x->omegan};
   x->timed("observables", [&] {
-    launch_observables(a, d.RR, dipI, d.masses, d.uk, x->d_rec, d.clk, x->stream);
+    launch_observables(a, d.RR, dipI, d.masses, d.uk, d.obs_part, x->d_rec, d.clk, x->stream);
   });
 }
 

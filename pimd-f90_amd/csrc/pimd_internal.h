@@ -124,9 +124,8 @@ struct DevBufs {
   double *adj = nullptr;                        // adjoint inputs and partial sums
   double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
   int *clk = nullptr;                           // device step clock {t, record slot}
+  double *obs_part = nullptr;                   // [64][5] partial sums of the observables
   int *clk_own = nullptr;                       // this context's clock (clk may be its owner's)
-  int *cnt = nullptr;                           // arrival counters: [P][sfac blocks], then [P]
-  size_t ncnt_sf = 0;
   double *tan = nullptr;                        // TTM parameter tangents: inputs, tables, workspace
   size_t tan_bytes = 0;
   double *red = nullptr;                        // scratch for reductions
@@ -139,6 +138,7 @@ void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s);
 void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s);
+void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s);
 void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s);
 void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s);
@@ -189,8 +189,8 @@ constexpr int MD_REC = 13;   // per-step record: Upot, virial, virialc, uk, then
 void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, double *uk,
                        double *part, hipStream_t s);
 void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
-                        const double *masses, const double *uk, double *rec0, const int *clk,
-                        hipStream_t s);
+                        const double *masses, const double *uk, double *part, double *rec0,
+                        const int *clk, hipStream_t s);
 void launch_set_clock(int *clk, int t, int slot, hipStream_t s);
 void launch_advance_clock(int *clk, hipStream_t s);
 

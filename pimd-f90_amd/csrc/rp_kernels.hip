@@ -353,19 +353,21 @@ void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, do
 //   obs[5..7] total molecular dipole, obs[8] sum of molecular dipole
 //   magnitudes, Debye, molecular dipoles averaged over beads
 //   (InputOutput.f90:593-607); NaN when dipI is NULL.
-// One block; fixed-order sums.
-__global__ __launch_bounds__(256) void k_observables(ObsArgs a, const double *__restrict__ RR,
-                                                    const double *__restrict__ dipI,
-                                                    const double *__restrict__ masses,
-                                                    const double *__restrict__ uk,
-                                                    double *rec0, const int *__restrict__ clk) {
-  double *rec = rec0 + (size_t)clk[1] * MD_REC;
+// k_obs_part: per block, partial sums of the spring term
+// K0 = sum_k sum_atoms m |R_k - R_(k-1)|^2 (bead 0 against P-1) and of the
+// dipole sums over a slice of molecules; k_obs_final adds the partials in
+// block order and forms the observables.
+constexpr int OBS_BLOCKS = 64;
+__global__ __launch_bounds__(256) void k_obs_part(ObsArgs a, const double *__restrict__ RR,
+                                                 const double *__restrict__ dipI,
+                                                 const double *__restrict__ masses,
+                                                 double *__restrict__ part) {
   __shared__ double red[5][256];
   const int P = a.P, na = a.natoms, N = na / 3;
   const size_t stride = (size_t)na * 3;
-  // spring term K0 = sum_k sum_atoms m |R_k - R_(k-1)|^2, bead 0 against P-1
+  const size_t gsz = (size_t)gridDim.x * 256, g0 = (size_t)blockIdx.x * 256 + threadIdx.x;
   double k0 = 0.0;
-  for (size_t e = threadIdx.x; e < (size_t)P * na; e += 256) {
+  for (size_t e = g0; e < (size_t)P * na; e += gsz) {
     const int k = (int)(e / na), at = (int)(e % na);
     const int km = k == 0 ? P - 1 : k - 1;
     const double *r = RR + k * stride + 3 * (size_t)at, *rm = RR + km * stride + 3 * (size_t)at;
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a, const double *__
   double dx = 0.0, dy = 0.0, dz = 0.0, dm = 0.0;
   if (dipI) {
     const double f = DEBYE / CHARGECON;
-    for (int w = threadIdx.x; w < N; w += 256) {
+    for (size_t w = g0; w < (size_t)N; w += gsz) {
       double v[3] = {0.0, 0.0, 0.0};
       for (int b = 0; b < P; ++b)
         for (int c = 0; c < 3; ++c) v[c] += dipI[((size_t)b * N + w) * 3 + c] * f;
@@ -400,7 +402,18 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a, const double *__
       for (int v = 0; v < 5; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_obs_final(ObsArgs a, const double *__restrict__ part, int nb, int has_dip,
+                            const double *__restrict__ uk, double *rec0,
+                            const int *__restrict__ clk) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < nb; ++i)
+    for (int v = 0; v < 5; ++v) s[v] += part[i * 5 + v];
+  double *rec = rec0 + (size_t)clk[1] * MD_REC;
+  const int P = a.P, na = a.natoms, N = na / 3;
   const double Upot = rec[0], virial = rec[1], virialc = rec[2];
   const double temp = TEMPFACTOR * uk[0] / (double)(na * P * P);
   const double p3v = PRESSCON2 * (1.0 / (3.0 * a.volume));
@@ -410,20 +423,21 @@ __global__ __launch_bounds__(256) void k_observables(ObsArgs a, const double *__
   o[1] = (ke + 0.5 * (virial - virialc) / P + Upot / P) / N;
   o[2] = p3v * (2 * ke - virialc / P) / N;
   const double kes = (double)(1.5f * (float)na * (float)P) * KB_KCAL * temp * P;
-  const double K0 = 0.5 * red[0][0] * (a.omegan * a.omegan) * MASSCONI;
+  const double K0 = 0.5 * s[0] * (a.omegan * a.omegan) * MASSCONI;
   o[3] = (kes - K0 + Upot) / (double)(N * P);
   o[4] = p3v * (2 * (kes - K0) - virial) / N;
   const double nan = __builtin_nan("");
-  o[5] = dipI ? red[1][0] : nan;
-  o[6] = dipI ? red[2][0] : nan;
-  o[7] = dipI ? red[3][0] : nan;
-  o[8] = dipI ? red[4][0] : nan;
+  for (int c = 0; c < 4; ++c) o[5 + c] = has_dip ? s[1 + c] : nan;
 }
 
 void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
-                        const double *masses, const double *uk, double *rec0, const int *clk,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_observables, dim3(1), dim3(256), 0, s, a, RR, dipI, masses, uk, rec0, clk);
+                        const double *masses, const double *uk, double *part, double *rec0,
+                        const int *clk, hipStream_t s) {
+  const size_t n = (size_t)a.P * a.natoms;
+  const int nb = (int)std::min<size_t>(OBS_BLOCKS, (n + 255) / 256);
+  hipLaunchKernelGGL(k_obs_part, dim3(nb), dim3(256), 0, s, a, RR, dipI, masses, part);
+  hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(64), 0, s, a, part, nb, dipI ? 1 : 0, uk, rec0,
+                     clk);
 }
 
 // Nose_Hoover half step, chain length 2 (Nose_Hoover.f90:158-198), scalars on device.

@@ -606,9 +606,8 @@ __device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, doub
 // registers.  A block takes a chunk of
 // sites of one bead; their phase tables and weights are staged in LDS (rows
 // padded so that lanes reading different rows of one site hit different
-// banks).  The last block of a (bead, segment block) to finish, found with
-// an arrival counter, sums the per-chunk partials of its segments in chunk
-// order into the structure factor.  SF_Q and SF_D as above.
+// banks).  Per-chunk partial sums are combined in a fixed order by
+// k_sfac_reduce.  SF_Q and SF_D as above.
 template <int MODE, int KZ>
 __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__restrict__ site,
                                                  const double2 *__restrict__ tab,
@@ -616,8 +615,7 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
                                                  const int4 *__restrict__ cols,
                                                  const int *__restrict__ colk,
                                                  const int *__restrict__ active,
-                                                 double *part, int sc, int scp, int *cnt,
-                                                 double *__restrict__ out) {
+                                                 double *__restrict__ part, int sc, int scp) {
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
   extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
@@ -645,10 +643,10 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
   }
   __syncthreads();
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = col < c.nseg;
-  const int4 cl = cols[live ? col : 0];
-  const int n = live ? cl.w : 0;
-  const int k0 = colk[live ? col : 0];
+  if (col >= c.nseg) return;
+  const int4 cl = cols[col];
+  const int n = cl.w;
+  const int k0 = colk[col];
   const int rowx = cl.x, rowy = c.hmax + 1 + abs(cl.y), rowz = c.hmax + c.kmax + 2 + abs(cl.z);
   const int rowz1 = c.hmax + c.kmax + 3;
   const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
@@ -700,27 +698,6 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
       o[2 * i] = are[i];
       o[2 * i + 1] = aim[i];
     }
-  // arrival: every lane publishes its partials, one lane counts the block
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  int *ctr = cnt + (size_t)b * gridDim.x + blockIdx.x;
-  if (threadIdx.x == 0) s_last = atomicAdd(ctr, 1) == nch - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const double *pp = part + (size_t)b * nch * c.nk * 2;
-  for (int i = 0; i < n; ++i) {
-    const size_t e = (size_t)(k0 + i) * 2;
-    double re = 0.0, im = 0.0;
-    for (int q = 0; q < nch; ++q) {
-      re += pp[(size_t)q * c.nk * 2 + e];
-      im += pp[(size_t)q * c.nk * 2 + e + 1];
-    }
-    out[(size_t)b * c.nk * 2 + e] = re;
-    out[(size_t)b * c.nk * 2 + e + 1] = im;
-  }
-  if (threadIdx.x == 0) *ctr = 0;
 }
 
 // Structure factors with one k-vector per lane.  A block holds 256 k of one
@@ -842,10 +819,11 @@ static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s)
   sfac_chunking(c, M, nch, sc, scp);
   const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
   dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
-  // grid.x * P arrival counters fit the (bead, 64-segment block) allocation
   hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), sfac_lds(c, MODE, sc, scp), s, c,
-                     b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp, b.cnt,
-                     MODE == SF_Q ? b.Sq : b.Sm);
+                     b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
+  dim3 g2((2 * c.nk + 255) / 256, c.P);
+  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, 2, b.active,
+                     MODE == SF_D ? 1 : 0, MODE == SF_Q ? b.Sq : b.Sm);
 }
 
 template <int MODE>
@@ -918,8 +896,8 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
                                                    double *__restrict__ dip,
                                                    double *__restrict__ dip_prev,
                                                    const double *__restrict__ Efd,
-                                                   double *diff2, int *active, int *iters,
-                                                   int *nactive, int *cnt, int iter) {
+                                                   double *__restrict__ diff2,
+                                                   const int *__restrict__ active, int iter) {
   constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
   // [T][64] phase tables of the block's sites (later reused for the
   // reduction), then with KL the bead's k data: [nk] (w Re S, w Im S, kz, w)
@@ -1059,7 +1037,7 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
   red[(g * 4 + 2) * FD_SB + lane] = e2;
   red[(g * 4 + 3) * FD_SB + lane] = ph;
   __syncthreads();
-  if (g == 0 && valid) {
+  if (g != 0 || !valid) return;
   for (int w = 1; w < CG; ++w) {
     e0 += red[(w * 4 + 0) * FD_SB + lane];
     e1 += red[(w * 4 + 1) * FD_SB + lane];
@@ -1091,41 +1069,6 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
     }
     diff2[(size_t)b * c.N * c.nd + m] = d2;
   }
-  }
-  if (MODE != FD_D || iter > c.maxiter) return;
-  // Per-bead convergence test (pot_ttm.f90:380-385) by the bead's last block
-  // to finish: deltadip = DEBYE/CHARGECON/sqrt(Natoms) * ||mu_new - mu_old||,
-  // the squared differences summed in a fixed order.
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(cnt + b, 1) == (int)gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double sum = 0.0;
-  for (int i = threadIdx.x; i < M; i += 64 * CG) sum += diff2[(size_t)b * M + i];
-  double *rd = reinterpret_cast<double *>(lds_tab);  // [64 CG]
-  rd[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 32 * CG; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) rd[threadIdx.x] += rd[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double delta = c.stath * sqrt(rd[0]);
-    if (delta < c.peps) {
-      active[b] = 0;
-      iters[b] = iter;
-    } else if (iter >= c.maxiter) {
-      active[b] = 0;
-      iters[b] = -iter;
-    } else {
-      iters[b] = iter;
-      atomicAdd(&nactive[iter], 1);
-    }
-    cnt[b] = 0;
-  }
 }
 
 // Shared memory of k_field: the phase tables (reused for the CG-way
@@ -1141,13 +1084,11 @@ static void launch_field_cg(const SysConst &c, DevBufs &b, int iter, dim3 grid, 
   if (kl)
     hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), field_lds(c, MO, true), s,
                        c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
-                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, b.iters, b.nactive,
-                       b.cnt + b.ncnt_sf, iter);
+                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
   else
     hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), field_lds(c, MO, false),
                        s, c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
-                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, b.iters, b.nactive,
-                       b.cnt + b.ncnt_sf, iter);
+                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
 }
 
 template <int MO, int ML>
@@ -1476,6 +1417,43 @@ void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s) {
                      b.dip_prev, b.dip, b.active);
 }
 
+
+// Per-bead convergence test (pot_ttm.f90:380-385): deltadip =
+// DEBYE/CHARGECON/sqrt(Natoms) * ||mu_new - mu_old||, summed in a fixed order.
+__global__ __launch_bounds__(256) void k_converge(SysConst c, const double *__restrict__ diff2,
+                                                 int *active, int *iters, int *nactive,
+                                                 int iter) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  if (!active[b]) return;
+  const int M = c.N * c.nd;
+  double s = 0.0;
+  for (int m = threadIdx.x; m < M; m += 256) s += diff2[(size_t)b * M + m];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double delta = c.stath * sqrt(red[0]);
+    if (delta < c.peps) {
+      active[b] = 0;
+      iters[b] = iter;
+    } else if (iter >= c.maxiter) {
+      active[b] = 0;
+      iters[b] = -iter;
+    } else {
+      iters[b] = iter;
+      atomicAdd(&nactive[iter], 1);
+    }
+  }
+}
+
+void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
+  hipLaunchKernelGGL(k_converge, dim3(c.P), dim3(256), 0, s, c, b.diff2, b.active, b.iters,
+                     b.nactive, iter);
+}
 
 // ---------------------------------------------------------------------------
 // Real-space forces of the induced dipoles (pot_ttm.f90:399-478): dipole-dipole
