@@ -939,29 +939,6 @@ int pimd_state_get(pimd_ctx *x, double *RRt, double *PPt, double *dRRt, double *
 
 }  // extern "C"
 
-namespace pimd {
-// Upot, virial, virialc of the step's force call (force_calc.f90:94-97).
-__global__ void k_step_record(int P, const double *Upot, const double *virt, const double *virialc,
-                              double *rec0, const int *clk) {
-  double *out = rec0 + (size_t)clk[1] * MD_REC;
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double u = 0.0, v = 0.0, vc = 0.0;
-  for (int b = 0; b < P; ++b) {
-    u += Upot[b];
-    v += virt[b * 9 + 0] + virt[b * 9 + 4] + virt[b * 9 + 8];
-    vc += virialc[b];
-  }
-  out[0] = u;
-  out[1] = v;
-  out[2] = vc / P;
-}
-
-// The kinetic energy at the end of the step.
-__global__ void k_record_uk(const double *uk, double *rec0, const int *clk) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) rec0[(size_t)clk[1] * MD_REC + 3] = uk[0];
-}
-}  // namespace pimd
-
 // Checks that a context is ready for device-resident MD steps.
 static int md_check(pimd_ctx *x) {
   if (!x) return fail(PIMD_EINVAL, "null argument");
@@ -1000,13 +977,17 @@ static void md_begin(pimd_ctx *x, int t0) {
   md_kinetic(x);
 }
 
-// write_out's observables of step st from its force record and the kinetic
-// energy after the force call; dipI NULL when the step has no bead dipoles.
-static void md_observables(pimd_ctx *x, const double *dipI) {
+// write_out's observables of the step from its force record and the kinetic
+// energy after the force call.  dipI is NULL when the step has no bead
+// dipoles; Upot (with virt, virialc) is NULL when the step has written its
+// record's first three entries already, else the kernel forms them.
+static void md_observables(pimd_ctx *x, const double *dipI, const double *Upot,
+                           const double *virt, const double *virialc) {
   DevBufs &d = x->d;
   const ObsArgs a{x->c.P, x->natoms, x->c.vol, x->omegan};
   x->timed("observables", [&] {
-    launch_observables(a, d.RR, dipI, d.masses, d.uk, d.obs_part, x->d_rec, d.clk, x->stream);
+    launch_observables(a, d.RR, dipI, d.masses, d.uk, Upot, virt, virialc, d.obs_part, x->d_rec,
+                       d.clk, x->stream);
   });
 }
 
@@ -1051,9 +1032,7 @@ static int md_step_vv(pimd_ctx *x, const double *noise, uint64_t seed) {
                       d.iters_out);
   if (rc != PIMD_OK) return rc;
   md_kinetic(x);
-  hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, s, P, d.Upot, d.virt, d.virialc,
-                     x->d_rec, d.clk);
-  md_observables(x, d.dipI);
+  md_observables(x, d.dipI, d.Upot, d.virt, d.virialc);  // also the step record
   x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
   thermostat(1);
   md_kinetic(x);
@@ -1061,8 +1040,7 @@ static int md_step_vv(pimd_ctx *x, const double *noise, uint64_t seed) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
     x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
-  hipLaunchKernelGGL(k_record_uk, dim3(1), dim3(64), 0, s, d.uk, x->d_rec, d.clk);
-  launch_advance_clock(d.clk, s);
+  launch_advance_clock(d.clk, s, d.uk, x->d_rec);  // with the record's kinetic energy
   HIPCHK(hipGetLastError());
   return PIMD_OK;
 }
@@ -1126,7 +1104,7 @@ static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
   x->timed("broadcast", [&] { launch_broadcast_beads(P, na * 3, dc.dRR, d.dRR, s); });
   md_kinetic(x);
   launch_contract_record(P, N, d.mol, dc.Upot, dc.virt, dc.virialc, d.uk, x->d_rec, d.clk, s);
-  md_observables(x, nullptr);  // no per-bead dipoles in contracted MD
+  md_observables(x, nullptr, nullptr, nullptr, nullptr);  // no per-bead dipoles here
   x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
   thermostat(1);
   md_kinetic(x);
@@ -1134,8 +1112,7 @@ static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
     x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
-  hipLaunchKernelGGL(k_record_uk, dim3(1), dim3(64), 0, s, d.uk, x->d_rec, d.clk);
-  launch_advance_clock(d.clk, s);
+  launch_advance_clock(d.clk, s, d.uk, x->d_rec);  // with the record's kinetic energy
   HIPCHK(hipGetLastError());
   return PIMD_OK;
 }

@@ -188,11 +188,15 @@ struct ObsArgs {
 constexpr int MD_REC = 13;   // per-step record: Upot, virial, virialc, uk, then 9 observables
 void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, double *uk,
                        double *part, hipStream_t s);
+// Upot, virt, virialc: the per-bead outputs of the step's force call, or NULL
+// when the step record rec[0..2] has been written already.
 void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
-                        const double *masses, const double *uk, double *part, double *rec0,
+                        const double *masses, const double *uk, const double *Upot,
+                        const double *virt, const double *virialc, double *part, double *rec0,
                         const int *clk, hipStream_t s);
 void launch_set_clock(int *clk, int t, int slot, hipStream_t s);
-void launch_advance_clock(int *clk, hipStream_t s);
+void launch_advance_clock(int *clk, hipStream_t s, const double *uk = nullptr,
+                          double *rec0 = nullptr);
 
 void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
                      double *dRR, double *mol, bool record, hipStream_t s);

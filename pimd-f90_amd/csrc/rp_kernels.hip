@@ -188,8 +188,11 @@ __global__ void k_set_clock(int *clk, int t, int slot) {
     clk[1] = slot;
   }
 }
-__global__ void k_advance_clock(int *clk) {
+// End of an MD step: its record takes the final kinetic energy (rec[3]) when
+// rec0 is non-NULL, then the clock moves to the next step and record slot.
+__global__ void k_advance_clock(int *clk, const double *__restrict__ uk, double *rec0) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (rec0) rec0[(size_t)clk[1] * MD_REC + 3] = uk[0];
     clk[0] += 1;
     clk[1] += 1;
   }
@@ -197,8 +200,8 @@ __global__ void k_advance_clock(int *clk) {
 void launch_set_clock(int *clk, int t, int slot, hipStream_t s) {
   hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, s, clk, t, slot);
 }
-void launch_advance_clock(int *clk, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance_clock, dim3(1), dim3(64), 0, s, clk);
+void launch_advance_clock(int *clk, hipStream_t s, const double *uk, double *rec0) {
+  hipLaunchKernelGGL(k_advance_clock, dim3(1), dim3(64), 0, s, clk, uk, rec0);
 }
 
 // PPt = PPt - MASSCON*dRRt*delt2 (MD.f90:49, 94).
@@ -241,18 +244,28 @@ __global__ void k_centroid_pbc(int P, int natoms, double box0, double boxi0, dou
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= natoms * 3) return;
   const size_t stride = (size_t)natoms * 3;
+  // beads summed in order 1..P; loads issued 8 at a time
   double s = 0.0;
-  for (int k = 0; k < P; ++k) s = s + RR[k * stride + id];
+  int k = 0;
+  for (; k + 8 <= P; k += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = RR[(k + u) * stride + id];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = s + v[u];
+  }
+  for (; k < P; ++k) s = s + RR[k * stride + id];
   const double c = s / P;
   const double sh = box0 * round(c * boxi0);
   RRc[id] = c - sh;
-  for (int k = 0; k < P; ++k) RR[k * stride + id] = RR[k * stride + id] - sh;
+  if (sh == 0.0) return;  // x - 0 = x
+  for (k = 0; k < P; ++k) RR[k * stride + id] = RR[k * stride + id] - sh;
 }
 
 void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR, double *RRc,
                          hipStream_t s) {
   const int n = natoms * 3;
-  hipLaunchKernelGGL(k_centroid_pbc, dim3((n + 255) / 256), dim3(256), 0, s, P, natoms, box0,
+  hipLaunchKernelGGL(k_centroid_pbc, dim3((n + 63) / 64), dim3(64), 0, s, P, natoms, box0,
                      boxi0, RR, RRc);
 }
 
@@ -405,39 +418,65 @@ __global__ __launch_bounds__(256) void k_obs_part(ObsArgs a, const double *__res
   if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_obs_final(ObsArgs a, const double *__restrict__ part, int nb, int has_dip,
-                            const double *__restrict__ uk, double *rec0,
-                            const int *__restrict__ clk) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[5] = {0, 0, 0, 0, 0};
-  for (int i = 0; i < nb; ++i)
-    for (int v = 0; v < 5; ++v) s[v] += part[i * 5 + v];
+// With Upot non-NULL it first forms the step record rec[0..2] from the
+// per-bead outputs (force_calc.f90:94-97); otherwise rec[0..2] is already set.
+__global__ __launch_bounds__(64) void k_obs_final(ObsArgs a, const double *__restrict__ part,
+                                                 int nb, int has_dip,
+                                                 const double *__restrict__ uk,
+                                                 const double *__restrict__ Upot,
+                                                 const double *__restrict__ virt,
+                                                 const double *__restrict__ virialc, double *rec0,
+                                                 const int *__restrict__ clk) {
+  __shared__ double red[8][64];
+  const int P = a.P, na = a.natoms, N = na / 3, tid = threadIdx.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = tid; i < nb; i += 64)
+    for (int k = 0; k < 5; ++k) v[k] += part[i * 5 + k];
+  if (Upot)
+    for (int b = tid; b < P; b += 64) {
+      v[5] += Upot[b];
+      v[6] += virt[b * 9 + 0] + virt[b * 9 + 4] + virt[b * 9 + 8];
+      v[7] += virialc[b];
+    }
+  for (int k = 0; k < 8; ++k) red[k][tid] = v[k];
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    if (tid < off)
+      for (int k = 0; k < 8; ++k) red[k][tid] += red[k][tid + off];
+    __syncthreads();
+  }
+  if (tid != 0) return;
   double *rec = rec0 + (size_t)clk[1] * MD_REC;
-  const int P = a.P, na = a.natoms, N = na / 3;
-  const double Upot = rec[0], virial = rec[1], virialc = rec[2];
+  if (Upot) {
+    rec[0] = red[5][0];
+    rec[1] = red[6][0];
+    rec[2] = red[7][0] / P;
+  }
+  const double Up = rec[0], virial = rec[1], virialc_ = rec[2];
   const double temp = TEMPFACTOR * uk[0] / (double)(na * P * P);
   const double p3v = PRESSCON2 * (1.0 / (3.0 * a.volume));
   const double ke = (double)(1.5f * (float)na) * KB_KCAL * temp;
   double *o = rec + 4;
   o[0] = temp;
-  o[1] = (ke + 0.5 * (virial - virialc) / P + Upot / P) / N;
-  o[2] = p3v * (2 * ke - virialc / P) / N;
+  o[1] = (ke + 0.5 * (virial - virialc_) / P + Up / P) / N;
+  o[2] = p3v * (2 * ke - virialc_ / P) / N;
   const double kes = (double)(1.5f * (float)na * (float)P) * KB_KCAL * temp * P;
-  const double K0 = 0.5 * s[0] * (a.omegan * a.omegan) * MASSCONI;
-  o[3] = (kes - K0 + Upot) / (double)(N * P);
+  const double K0 = 0.5 * red[0][0] * (a.omegan * a.omegan) * MASSCONI;
+  o[3] = (kes - K0 + Up) / (double)(N * P);
   o[4] = p3v * (2 * (kes - K0) - virial) / N;
   const double nan = __builtin_nan("");
-  for (int c = 0; c < 4; ++c) o[5 + c] = has_dip ? s[1 + c] : nan;
+  for (int c = 0; c < 4; ++c) o[5 + c] = has_dip ? red[1 + c][0] : nan;
 }
 
 void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
-                        const double *masses, const double *uk, double *part, double *rec0,
+                        const double *masses, const double *uk, const double *Upot,
+                        const double *virt, const double *virialc, double *part, double *rec0,
                         const int *clk, hipStream_t s) {
   const size_t n = (size_t)a.P * a.natoms;
   const int nb = (int)std::min<size_t>(OBS_BLOCKS, (n + 255) / 256);
   hipLaunchKernelGGL(k_obs_part, dim3(nb), dim3(256), 0, s, a, RR, dipI, masses, part);
-  hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(64), 0, s, a, part, nb, dipI ? 1 : 0, uk, rec0,
-                     clk);
+  hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(64), 0, s, a, part, nb, dipI ? 1 : 0, uk, Upot,
+                     virt, virialc, rec0, clk);
 }
 
 // Nose_Hoover half step, chain length 2 (Nose_Hoover.f90:158-198), scalars on device.
