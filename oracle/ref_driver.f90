@@ -6,6 +6,7 @@
 ! Output (stream): real*8 values, layout per mode (see oracle/refrun.py).
 !
 !   mode 1  forces     pot_ttm per bead            (pot_ttm.f90:3-630)
+!                      (pot_spc for pot_model 4, 5: pot_spc.f90:1-277)
 !   mode 2  nm         InitNormalModes+EvolveRing  (NormalModes.f90:34-218)
 !   mode 3  langevin   Init_Langevin_NM+Langevin_NM(Langevin.f90:30-98)
 !   mode 4  nosehoover Nose_Hoover                 (Nose_Hoover.f90:33-112)
@@ -94,6 +95,14 @@ case (1)
    allocate(hist(3, fd:ld, 4, nb))
    hist = 0.d0
    do ib = 1, nb
+      if (pot_model >= 4) then
+         ! pot_spc (pot_spc.f90:1-277) sets neither virialc nor Edip_mom,
+         ! and has no induction energy: written as zeros
+         call pot_spc(RRt(:,:,ib), En, dRR, vtmp, dipI, chgb)
+         vc = 0.d0; Uind = 0.d0; dipE = 0.d0
+         write(11) En, Umon, Uvdw, Uelec, Uind, vc, vtmp, dRR, dipI, dipE, chgb
+         cycle
+      endif
       if (per_bead_hist /= 0) tx_dip = hist(:,:,:,ib)
       call pot_ttm(RRt(:,:,ib), RRc, En, dRR, vtmp, vc, dipI, dipE, chgb, ih(7))
       if (per_bead_hist /= 0) hist(:,:,:,ib) = tx_dip

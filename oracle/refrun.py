@@ -80,7 +80,8 @@ def forces(p, RRt, RRc, t=1, per_bead_hist=False):
         res["dipE"].append(r.take(N, 3))
         res["chg"].append(r.take(na))
     res = {k: np.array(v) for k, v in res.items()}
-    res["iters"] = np.array(iters)
+    # the SPC models have no SOR loop and report no iterations
+    res["iters"] = np.array(iters) if iters else np.zeros(P, dtype=int)
     return res
 
 

@@ -33,6 +33,8 @@ struct orc_sys {
   double dms1, dms2, dms3;
   double polfac[4], shiftA, shiftB, Uvdw_lrc, aewald, cts1, cts2, cts3;
   double polfac_in[3]; /* polfacO, polfacH, polfacM as given (potential.f90:88,99) */
+  /* q-SPC/Fw and SPC/F (pot_mod.f90:31-44): charges, bond and angle terms */
+  double qO, qH, roh0, th0, harmkb, harmka, spcA, spcB, spcC, spcD;
   int fd, ld, fdI, ldI; /* dipole site range [fd, ld] and type range */
   int hmax, kmax, lmax, nk;
   double *kvec, *Ak;
@@ -474,7 +476,26 @@ orc_sys *orc_create(const orc_params *p) {
   s->rc2 = p->rc * p->rc;
   double polfacO, polfacH, polfacM;
   const double Rc = p->rc, rc1 = p->rc1;
-  if (p->pot_model == 2) { /* TTM2.1-F, pot_mod.f90:7-14, potential.f90:84-94 */
+  if (p->pot_model == 4 || p->pot_model == 5) {
+    /* q-SPC/Fw (pot_mod.f90:31-35, potential.f90:121-127) and SPC/F
+     * (pot_mod.f90:40-44, potential.f90:140-145): three charged sites per
+     * water (potential.f90:65-71), no dipoles, no vdW shift */
+    s->ns = 3 * s->N;
+    s->fd = 0; s->ld = -1; s->fdI = 0; s->ldI = -1;
+    s->vdwB = 0.0; s->vdwD = 0.0; s->vdwE = 0.0;
+    if (p->pot_model == 4) {
+      s->vdwA = 629326.57249877361701443513; s->vdwC = -625.50206521141523463400;
+      s->qO = -0.84; s->qH = 0.42; s->roh0 = 1.012; s->th0 = 1.95476876223364912614;
+      s->harmkb = 529.581; s->harmka = 37.95;
+    } else {
+      s->vdwA = 628213.72817108343399678768; s->vdwC = -624.97858519194071637260;
+      s->qO = -0.82; s->qH = 0.41; s->roh0 = 1.0; s->th0 = 1.91061193215819258784;
+      s->spcA = 671.288262912; s->spcB = 164.376; s->spcC = -211.536; s->spcD = 111.744;
+    }
+    polfacO = polfacH = polfacM = 1.0; /* not used by these models */
+    s->shiftA = 0.0;
+    s->shiftB = 0.0;
+  } else if (p->pot_model == 2) { /* TTM2.1-F, pot_mod.f90:7-14, potential.f90:84-94 */
     s->fd = 0;
     s->ld = 3 * s->N - 1;
     s->vdwA = -1.329565985e+6; s->vdwB = 3.632560798e+5; s->vdwC = -2.147141323e+3;
@@ -1354,6 +1375,270 @@ static int pot_ttm_bead(const orc_sys *s, scratch_t *w, const double *RR, const 
   return iter;
 }
 
+/* ---------------- one bead: pot_spc (pot_ttm/pot_spc.f90:1-277) ---------------- */
+
+/* Minimum image with a division, as pot_spc.f90 writes it for intramolecular
+ * vectors (r - box*anint(r/box)). */
+static void minimg_div(const orc_sys *s, double *r) {
+  for (int c = 0; c < 3; ++c) r[c] = r[c] - s->p.box[c] * anint(r[c] / s->p.box[c]);
+}
+
+/* ewald_pw_std_qq, ewald.f90:233-289: reciprocal-space potential and field of
+ * the charges of all sites, and the reciprocal virial vir[col][row]. */
+static void ewald_pw_qq(const orc_sys *s, scratch_t *w, double *vir) {
+  const int ns = s->ns;
+  memset(vir, 0, 9 * sizeof(double));
+  ssincos(s, w);
+  int lastkx = -1000, lastky = -1000;
+  for (int k = 0; k < s->nk; ++k) {
+    int kx = s->hkl[3 * k], ky = s->hkl[3 * k + 1], kz = s->hkl[3 * k + 2];
+    const double *kv = &s->kvec[3 * k];
+    double p4vAkk = FOURPI_ * s->Ak[k] / s->vol;
+    if (!(kx == lastkx && ky == lastky)) {
+      for (int i = 0; i < ns; ++i) {
+        double tmp = SX(i, kx) * CY(i, ky) + CX(i, kx) * SY(i, ky);
+        w->cxy[i] = CX(i, kx) * CY(i, ky) - SX(i, kx) * SY(i, ky);
+        w->sxy[i] = tmp;
+      }
+    }
+    lastkx = kx;
+    lastky = ky;
+    for (int i = 0; i < ns; ++i) {
+      double tmp = w->sxy[i] * CZ(i, kz) + w->cxy[i] * SZ(i, kz);
+      w->ckr[i] = w->cxy[i] * CZ(i, kz) - w->sxy[i] * SZ(i, kz);
+      w->skr[i] = tmp;
+    }
+    double rQ = 0.0, iQ = 0.0;
+    for (int i = 0; i < ns; ++i) rQ += w->charge[i] * w->ckr[i];
+    for (int i = 0; i < ns; ++i) iQ += w->charge[i] * w->skr[i];
+    for (int i = 0; i < ns; ++i) {
+      double tmp = p4vAkk * (w->ckr[i] * iQ - w->skr[i] * rQ);
+      for (int c = 0; c < 3; ++c) w->Efq[i][c] = w->Efq[i][c] - tmp * kv[c];
+    }
+    for (int i = 0; i < ns; ++i) w->phi[i] = w->phi[i] + p4vAkk * (w->ckr[i] * rQ + w->skr[i] * iQ);
+    double ksq = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    double coeff2 = 2.0 / ksq + 0.5 / (s->aewald * s->aewald);
+    double U = -0.5 * p4vAkk * (rQ * rQ + iQ * iQ);
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r)
+        vir[c * 3 + r] = vir[c * 3 + r] + (r == c ? U : 0.0) - U * coeff2 * kv[r] * kv[c];
+  }
+}
+
+static void pot_spc_bead(const orc_sys *s, scratch_t *w, const double *RR, double *En,
+                         double *terms, double *dRR, double *virt, double *dipI, double *chg) {
+  const int N = s->N, ns = s->ns;
+  const double *box = s->p.box, *boxi = s->boxi;
+  double Umon = 0.0, Uvdw = 0.0, Uelec = 0.0;
+  memset(virt, 0, 9 * sizeof(double));
+  memset(w->phi, 0, ns * sizeof(double));
+  memset(w->Efq, 0, ns * sizeof *w->Efq);
+  memset(w->dR, 0, ns * sizeof *w->dR);
+  /* OHH -> type-major sites (pot_spc.f90:55-60); charges of init_pot
+   * (potential.f90:186-192) */
+  for (int iw = 0; iw < N; ++iw) {
+    int iO = iw, iH1 = iw + N, iH2 = iw + 2 * N;
+    for (int c = 0; c < 3; ++c) {
+      w->R[iO][c] = RR[9 * iw + c];
+      w->R[iH1][c] = RR[9 * iw + 3 + c];
+      w->R[iH2][c] = RR[9 * iw + 6 + c];
+    }
+    w->charge[iO] = s->qO * CHARGECON;
+    w->charge[iH1] = s->qH * CHARGECON;
+    w->charge[iH2] = s->qH * CHARGECON;
+    chg[3 * iw] = w->charge[iO];
+    chg[3 * iw + 1] = w->charge[iH1];
+    chg[3 * iw + 2] = w->charge[iH2];
+  }
+  if (s->p.pot_model == 4) {
+    /* harmonic bonds and angle (pot_spc.f90:69-106) */
+    for (int iw = 0; iw < N; ++iw) {
+      int iO = iw, iH1 = iw + N, iH2 = iw + 2 * N;
+      double roh1[3], roh2[3], AxB[3], A1[3], A2[3], dh1[3], dh2[3];
+      for (int c = 0; c < 3; ++c) {
+        roh1[c] = w->R[iH1][c] - w->R[iO][c];
+        roh2[c] = w->R[iH2][c] - w->R[iO][c];
+      }
+      minimg_div(s, roh1);
+      minimg_div(s, roh2);
+      double d1 = sqrt(dot3(roh1, roh1)), d2 = sqrt(dot3(roh2, roh2));
+      AxB[0] = roh1[1] * roh2[2] - roh1[2] * roh2[1];
+      AxB[1] = -roh1[0] * roh2[2] + roh1[2] * roh2[0];
+      AxB[2] = roh1[0] * roh2[1] - roh1[1] * roh2[0];
+      double dAxB = sqrt(dot3(AxB, AxB));
+      double costh = dot3(roh1, roh2) / (d1 * d2);
+      double sinth = dAxB / (d1 * d2);
+      double ang = atan2(sinth, costh);
+      double e1 = s->harmkb * ((d1 - s->roh0) * (d1 - s->roh0) + (d2 - s->roh0) * (d2 - s->roh0)) +
+                  s->harmka * ((ang - s->th0) * (ang - s->th0));
+      double tmp1 = 2.0 * s->harmkb * (d1 - s->roh0) / d1;
+      double tmp2 = 2.0 * s->harmkb * (d2 - s->roh0) / d2;
+      double tmp = 2.0 * s->harmka * (ang - s->th0);
+      for (int c = 0; c < 3; ++c) {
+        A1[c] = -(roh2[c] / (d1 * d2) - costh * roh1[c] / (d1 * d1)) / sinth;
+        A2[c] = -(roh1[c] / (d1 * d2) - costh * roh2[c] / (d2 * d2)) / sinth;
+      }
+      for (int c = 0; c < 3; ++c) {
+        dh1[c] = tmp1 * roh1[c] + tmp * A1[c];
+        dh2[c] = tmp2 * roh2[c] + tmp * A2[c];
+      }
+      for (int c = 0; c < 3; ++c) {
+        w->dR[iH1][c] = dh1[c];
+        w->dR[iH2][c] = dh2[c];
+        w->dR[iO][c] = -dh1[c] - dh2[c];
+      }
+      Umon = Umon + e1;
+      for (int iy = 0; iy < 3; ++iy)
+        for (int ix = 0; ix < 3; ++ix)
+          virt[iy * 3 + ix] = virt[iy * 3 + ix] + roh1[ix] * dh1[iy] + roh2[ix] * dh2[iy];
+    }
+  } else {
+    /* SPC/F bond, bond and H-H terms (pot_spc.f90:107-137) */
+    double rhh0 = 2.0 * sin(0.5 * s->th0) * s->roh0;
+    for (int iw = 0; iw < N; ++iw) {
+      int iO = iw, iH1 = iw + N, iH2 = iw + 2 * N;
+      double roh1[3], roh2[3], rhh[3], dr1[3][3];
+      for (int c = 0; c < 3; ++c) {
+        roh1[c] = w->R[iO][c] - w->R[iH1][c];
+        roh2[c] = w->R[iO][c] - w->R[iH2][c];
+        rhh[c] = w->R[iH1][c] - w->R[iH2][c];
+      }
+      minimg_div(s, roh1);
+      minimg_div(s, roh2);
+      minimg_div(s, rhh);
+      double d1 = sqrt(roh1[0] * roh1[0] + roh1[1] * roh1[1] + roh1[2] * roh1[2]);
+      double d2 = sqrt(roh2[0] * roh2[0] + roh2[1] * roh2[1] + roh2[2] * roh2[2]);
+      double d3 = sqrt(rhh[0] * rhh[0] + rhh[1] * rhh[1] + rhh[2] * rhh[2]);
+      double dd1 = d1 - s->roh0, dd2 = d2 - s->roh0, dd3 = d3 - rhh0;
+      double e1 = s->spcA * (dd1 * dd1 + dd2 * dd2) + s->spcB * dd3 * dd3 +
+                  s->spcC * (dd1 + dd2) * dd3 + s->spcD * dd1 * dd2;
+      double v1 = 2.0 * s->spcA * dd1 + s->spcC * dd3 + s->spcD * dd2;
+      double v2 = 2.0 * s->spcA * dd2 + s->spcC * dd3 + s->spcD * dd1;
+      double v3 = 2.0 * s->spcB * dd3 + s->spcC * (dd1 + dd2);
+      for (int c = 0; c < 3; ++c) {
+        dr1[0][c] = v1 * roh1[c] / d1 + v2 * roh2[c] / d2;
+        dr1[1][c] = -v1 * roh1[c] / d1 + v3 * rhh[c] / d3;
+        dr1[2][c] = -v2 * roh2[c] / d2 - v3 * rhh[c] / d3;
+      }
+      Umon = Umon + e1;
+      for (int c = 0; c < 3; ++c) {
+        w->dR[iO][c] = dr1[0][c];
+        w->dR[iH1][c] = dr1[1][c];
+        w->dR[iH2][c] = dr1[2][c];
+      }
+      for (int iy = 0; iy < 3; ++iy)
+        for (int ix = 0; ix < 3; ++ix)
+          virt[iy * 3 + ix] = virt[iy * 3 + ix] - roh1[ix] * dr1[1][iy] - roh2[ix] * dr1[2][iy];
+    }
+  }
+  /* vdW and real-space charge pairs over O-O neighbours (pot_spc.f90:142-191) */
+  for (int i = 0; i < N; ++i) {
+    find_neigh(s, i, w->R, &w->nb);
+    for (int n = 0; n < w->nb.n; ++n) {
+      int jO = w->nb.j[n];
+      double Rij[3], dij[3], rij0[3];
+      memcpy(Rij, &w->nb.Rij[3 * n], sizeof Rij);
+      double dRsq = w->nb.R2[n];
+      double R2i = 1.0 / dRsq, R6i = R2i * R2i * R2i, R8i = R6i * R2i;
+      double uij = R6i * (s->vdwA * R6i + s->vdwC);
+      for (int c = 0; c < 3; ++c) dij[c] = -R8i * (12.0 * s->vdwA * R6i + 6.0 * s->vdwC) * Rij[c];
+      Uvdw = Uvdw + uij;
+      for (int c = 0; c < 3; ++c) {
+        w->dR[i][c] = w->dR[i][c] + dij[c];
+        w->dR[jO][c] = w->dR[jO][c] - dij[c];
+      }
+      for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) VADD(virt, col, row, dij[col] * Rij[row]);
+      for (int c = 0; c < 3; ++c) rij0[c] = Rij[c] - w->R[i][c] + w->R[jO][c];
+      for (int isp = 0; isp < 3; ++isp) {
+        int iat = i + isp * N;
+        double Ri[3], qi = w->charge[iat];
+        for (int c = 0; c < 3; ++c) Ri[c] = w->R[iat][c] + rij0[c];
+        for (int jsp = 0; jsp < 3; ++jsp) {
+          int jat = jO + jsp * N;
+          double qj = w->charge[jat], r[3];
+          for (int c = 0; c < 3; ++c) r[c] = Ri[c] - w->R[jat][c];
+          for (int c = 0; c < 3; ++c) r[c] = r[c] - box[c] * anint(r[c] * boxi[c]);
+          double drsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+          double dd = sqrt(drsq);
+          double expon = exp(-(s->aewald * s->aewald * drsq));
+          double ts0 = orc_erfc(s->aewald * dd) / dd;
+          double ts1 = (ts0 + s->cts1 * expon) / drsq;
+          w->phi[iat] = w->phi[iat] + ts0 * qj;
+          w->phi[jat] = w->phi[jat] + ts0 * qi;
+          Uelec = Uelec + ts0 * qi * qj;
+          for (int c = 0; c < 3; ++c) {
+            w->Efq[iat][c] = w->Efq[iat][c] + ts1 * qj * r[c];
+            w->Efq[jat][c] = w->Efq[jat][c] - ts1 * qi * r[c];
+            dij[c] = -ts1 * qi * qj * r[c];
+          }
+          for (int col = 0; col < 3; ++col)
+            for (int row = 0; row < 3; ++row) VADD(virt, col, row, dij[col] * r[row]);
+        }
+      }
+    }
+  }
+  /* Ewald self energy and the screened intramolecular pairs (pot_spc.f90:202-233) */
+  for (int iat = 0; iat < ns; ++iat) w->phi[iat] = w->phi[iat] - s->aewald * TSP_ * w->charge[iat];
+  for (int iw = 0; iw < N; ++iw)
+    for (int isp = 0; isp < 2; ++isp) {
+      int iat = iw + isp * N;
+      for (int jsp = isp + 1; jsp < 3; ++jsp) {
+        int jat = iw + jsp * N;
+        double r[3], dij[3];
+        for (int c = 0; c < 3; ++c) r[c] = w->R[iat][c] - w->R[jat][c];
+        minimg_div(s, r);
+        double drsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        double dd = sqrt(drsq);
+        double expon = exp(-(s->aewald * s->aewald * drsq));
+        double kr0 = orc_erfc(s->aewald * dd) / dd;
+        double kr1 = (kr0 + s->cts1 * expon) / drsq;
+        double ch0 = -1.0 / dd, ch1 = ch0 / drsq;
+        double ts0 = ch0 + kr0, ts1 = ch1 + kr1;
+        w->phi[iat] = w->phi[iat] + ts0 * w->charge[jat];
+        w->phi[jat] = w->phi[jat] + ts0 * w->charge[iat];
+        for (int c = 0; c < 3; ++c) {
+          w->Efq[iat][c] = w->Efq[iat][c] + ts1 * w->charge[jat] * r[c];
+          w->Efq[jat][c] = w->Efq[jat][c] - ts1 * w->charge[iat] * r[c];
+          dij[c] = -ts1 * w->charge[iat] * w->charge[jat] * r[c];
+        }
+        for (int col = 0; col < 3; ++col)
+          for (int row = 0; row < 3; ++row) VADD(virt, col, row, dij[col] * r[row]);
+      }
+    }
+  /* reciprocal space (pot_spc.f90:239-246) */
+  double vir_rec[9];
+  ewald_pw_qq(s, w, vir_rec);
+  for (int i = 0; i < 9; ++i) virt[i] = virt[i] + vir_rec[i];
+  Uelec = 0.0;
+  for (int iat = 0; iat < ns; ++iat) Uelec += w->charge[iat] * w->phi[iat];
+  Uelec = 0.5 * Uelec;
+  for (int iat = 0; iat < ns; ++iat)
+    for (int c = 0; c < 3; ++c) w->dR[iat][c] = w->dR[iat][c] - w->charge[iat] * w->Efq[iat][c];
+  virt[0] -= s->Uvdw_lrc;
+  virt[4] -= s->Uvdw_lrc;
+  virt[8] -= s->Uvdw_lrc;
+  /* molecular dipoles (pot_spc.f90:257-268) and the gradient in OHH order */
+  for (int iw = 0; iw < N; ++iw) {
+    int iO = iw, iH1 = iw + N, iH2 = iw + 2 * N;
+    double a[3], b[3];
+    for (int c = 0; c < 3; ++c) {
+      a[c] = w->R[iH1][c] - w->R[iO][c];
+      b[c] = w->R[iH2][c] - w->R[iO][c];
+    }
+    minimg_div(s, a);
+    minimg_div(s, b);
+    for (int c = 0; c < 3; ++c) {
+      dipI[3 * iw + c] = w->charge[iH1] * a[c] + w->charge[iH2] * b[c];
+      dRR[9 * iw + c] = w->dR[iO][c];
+      dRR[9 * iw + 3 + c] = w->dR[iH1][c];
+      dRR[9 * iw + 6 + c] = w->dR[iH2][c];
+    }
+  }
+  *En = Umon + Uvdw + Uelec + s->Uvdw_lrc;
+  terms[0] = Umon; terms[1] = Uvdw; terms[2] = Uelec; terms[3] = 0.0; terms[4] = s->Uvdw_lrc;
+}
+
 int orc_forces(orc_sys *s, const double *RRt, const double *RRc, int t, double *En,
                double *terms, double *dRRt, double *virt, double *virialc, double *dipI,
                double *dipE, double *chg, int *iters, double *hist) {
@@ -1361,6 +1646,16 @@ int orc_forces(orc_sys *s, const double *RRt, const double *RRc, int t, double *
   scratch_alloc(s, &w);
   int na = s->natoms, N = s->N, ndip = s->ld - s->fd + 1;
   for (int b = 0; b < s->p.nbeads; ++b) {
+    if (s->p.pot_model >= 4) {
+      /* pot_spc sets neither virialc nor Edip_mom (pot_spc.f90:1-13): zero here */
+      pot_spc_bead(s, &w, RRt + (size_t)b * na * 3, &En[b], &terms[5 * b],
+                   dRRt + (size_t)b * na * 3, virt + 9 * b, dipI + (size_t)b * N * 3,
+                   chg + (size_t)b * na);
+      virialc[b] = 0.0;
+      memset(dipE + (size_t)b * N * 3, 0, sizeof(double) * N * 3);
+      iters[b] = 0;
+      continue;
+    }
     iters[b] = pot_ttm_bead(s, &w, RRt + (size_t)b * na * 3, RRc, t, &En[b], &terms[5 * b],
                             dRRt + (size_t)b * na * 3, virt + 9 * b, &virialc[b],
                             dipI + (size_t)b * N * 3, dipE + (size_t)b * N * 3,

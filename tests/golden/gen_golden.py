@@ -48,11 +48,18 @@ FORCE_CASES = [
      {"polar_maxiter": 4, "polar_eps": 1e-12}),
     ("forces_ttm21f_n27_p2_eps1e5", 2, 27, 2, (3, 3, 3), None, -1.0, 1e-5, 18,
      {"polar_maxiter": 40, "polar_eps": 1e-9}),
+    # the fixed-charge models of pot_spc: q-SPC/Fw (4) and SPC/F (5)
+    ("forces_qspcfw_n32_p2_rc49", 4, 32, 2, None, None, 4.9, 1e-8, 19, {}),
+    ("forces_spcf_n32_p2_rchalf", 5, 32, 2, None, None, -1.0, 1e-6, 20, {}),
+    ("forces_qspcfw_n16_p2_ortho", 4, 16, 2, (4, 2, 2), (12.0, 8.0, 8.0), 3.9, 1e-7, 22, {}),
+    ("forces_spcf_n64_p1_rchalf", 5, 64, 1, None, None, -1.0, 1e-6, 23, {}),
 ]
 
 
-def force_fixtures():
+def force_fixtures(only=None):
     for name, model, N, P, lat, box, rc, eps, seed, extra in FORCE_CASES:
+        if only and name not in only:
+            continue
         if box is None:
             RRt, PPt, RRc, L = make_system(N, P, seed=seed, lattice=lat)
             bx = (L, L, L)
@@ -240,6 +247,9 @@ def scalar_fixtures():
 if __name__ == "__main__":
     if not refrun.available():
         sys.exit("build the reference first: make -C oracle ref")
+    if len(sys.argv) > 1:   # named force fixtures only
+        force_fixtures(set(sys.argv[1:]))
+        sys.exit(0)
     force_fixtures()
     predictor_fixture()
     traj_fixture()
