@@ -32,6 +32,8 @@ extern "C" {
 
 #define PIMD_TTM21F 2
 #define PIMD_TTM3F 3
+#define PIMD_QSPCFW 4   /* q-SPC/Fw, fixed charges (pot_spc.f90:69-106)       */
+#define PIMD_SPCF 5     /* SPC/F, fixed charges (pot_spc.f90:107-137)         */
 
 #define PIMD_FP64 0     /* plain double precision, the reference's arithmetic */
 #define PIMD_MIXED 1    /* real-space pair terms and the real-space dipole field
@@ -46,7 +48,8 @@ typedef struct pimd_ctx pimd_ctx;
  * (InputOutput.f90:118-125): rc <= 0 -> min(box)/2; rc1 < 0 -> 0.8*rc with
  * 0.8 a single-precision literal. */
 typedef struct {
-  int pot_model;      /* PIMD_TTM21F or PIMD_TTM3F (potential.f90:84-113)    */
+  int pot_model;      /* PIMD_TTM21F or PIMD_TTM3F (potential.f90:84-113), or
+                         PIMD_QSPCFW or PIMD_SPCF (potential.f90:121-145)    */
   int nwaters;        /* Natoms = 3*nwaters                                  */
   int nbeads;         /* P                                                   */
   int polar_maxiter;  /* input_file_reader.f90:79, default 15                */

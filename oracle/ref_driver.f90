@@ -209,9 +209,15 @@ case (5)
       ! force_calc.f90:26-106 (serial order, one bead at a time)
       Usum = 0.d0; visum = 0.d0; vcsum = 0.d0
       do ib = 1, nb
-         if (per_bead_hist /= 0) tx_dip = hist(:,:,:,ib)
-         call pot_ttm(RRt(:,:,ib), RRc, En, dRR, vtmp, vc, dipI, dipE, chgb, it)
-         if (per_bead_hist /= 0) hist(:,:,:,ib) = tx_dip
+         if (pot_model >= 4) then
+            ! pot_spc leaves virialc unset (pot_spc.f90:1-13): taken as zero
+            call pot_spc(RRt(:,:,ib), En, dRR, vtmp, dipI, chgb)
+            vc = 0.d0
+         else
+            if (per_bead_hist /= 0) tx_dip = hist(:,:,:,ib)
+            call pot_ttm(RRt(:,:,ib), RRc, En, dRR, vtmp, vc, dipI, dipE, chgb, it)
+            if (per_bead_hist /= 0) hist(:,:,:,ib) = tx_dip
+         endif
          dRRt(:,:,ib) = dRR
          dipIt(:,:,ib) = dipI
          write(11) En, vtmp(1,1)+vtmp(2,2)+vtmp(3,3), vc

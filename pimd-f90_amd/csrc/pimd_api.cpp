@@ -237,6 +237,28 @@ int setup_system(pimd_ctx *x) {
     c.gammaM = 0.426706882;
     c.nd = 3; c.tdip = 0;
     c.shiftA = c.shiftB = 0.0;
+  } else if (p.pot_model >= PIMD_QSPCFW) {
+    // q-SPC/Fw (pot_mod.f90:31-35) and SPC/F (pot_mod.f90:40-44): vdwA and
+    // vdwC only; polfac set to 1 (no polarisation in these models)
+    const bool q = p.pot_model == PIMD_QSPCFW;
+    const double v[TTM_NPARAM] = {q ? 629326.57249877361701443513 : 628213.72817108343399678768,
+                                  0.0, q ? -625.50206521141523463400 : -624.97858519194071637260,
+                                  0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    std::copy(v, v + TTM_NPARAM, th);
+    c.gammaM = 0.0;
+    c.nd = 0; c.tdip = 3;
+    c.shiftA = c.shiftB = 0.0;
+    c.spc_q[0] = (q ? -0.84 : -0.82) * CHARGECON;  // potential.f90:186-192
+    c.spc_q[1] = (q ? 0.42 : 0.41) * CHARGECON;
+    c.spc_roh0 = q ? 1.012 : 1.0;
+    c.spc_th0 = q ? 1.95476876223364912614 : 1.91061193215819258784;
+    c.spc_rhh0 = 2.0 * std::sin(0.5 * c.spc_th0) * c.spc_roh0;  // pot_spc.f90:108
+    c.spc_kb = q ? 529.581 : 0.0;
+    c.spc_ka = q ? 37.95 : 0.0;
+    c.spc_A = q ? 0.0 : 671.288262912;
+    c.spc_B = q ? 0.0 : 164.376;
+    c.spc_C = q ? 0.0 : -211.536;
+    c.spc_D = q ? 0.0 : 111.744;
   } else {  // pot_mod.f90:19-26
     const double v[TTM_NPARAM] = {0.0, 0.0, -0.72298855e+03, 0.10211829e+06, 0.37170376e+01,
                                   0.0, 0.0, 1.444, 0.837, 0.496, 0.837, 0.175, 0.175,
@@ -508,6 +530,11 @@ int dd_beside_sfac(pimd_ctx *x, hipStream_t s) {
   return PIMD_OK;
 }
 
+__global__ void k_zero_int(int *v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0;
+}
+
 // One pot_ttm evaluation per bead on device arrays (pot_ttm.f90:3-630).
 // The MD step t of the call (dipole predictor, history slot) is read from the
 // device clock d.clk, set by the caller.
@@ -517,6 +544,21 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
   const SysConst &c = x->c;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
+  if (c.model >= PIMD_QSPCFW) {  // pot_spc (pot_spc.f90:1-277)
+    x->timed("monomer", [&] { launch_spc_monomer(c, d, RR, chg, s); });
+    x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
+    x->timed("pair_qq", [&] { launch_spc_pair(c, d, s); });
+    x->timed("ewald_tables", [&] { launch_tables(c, d, s); });
+    x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, s); });
+    x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
+    x->timed("virial_k", [&] { launch_virk(c, d, s); });
+    x->timed("finalize", [&] { launch_spc_finalize(c, d, dRR, dipI, dipE, s); });
+    x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, nullptr, s); });
+    // no SOR loop: zero iterations
+    hipLaunchKernelGGL(k_zero_int, dim3((c.P + 63) / 64), dim3(64), 0, s, iters, c.P);
+    HIPCHK(hipGetLastError());
+    return PIMD_OK;
+  }
   x->timed("monomer", [&] { launch_monomer(c, d, RR, RRc, chg, s); });
   x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
   x->timed("pair_qq", [&] { launch_pair_qq(c, d, x->group, s); });
@@ -581,8 +623,9 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
   if (!params || !out) return fail(PIMD_EINVAL, "null argument");
   *out = nullptr;
   pimd_params p = *params;
-  if (p.pot_model != PIMD_TTM21F && p.pot_model != PIMD_TTM3F)
-    return fail(PIMD_EINVAL, "pot_model must be 2 (TTM2.1-F) or 3 (TTM3-F), got %d", p.pot_model);
+  if (p.pot_model < PIMD_TTM21F || p.pot_model > PIMD_SPCF)
+    return fail(PIMD_EINVAL, "pot_model must be 2 (TTM2.1-F), 3 (TTM3-F), 4 (q-SPC/Fw) or "
+                "5 (SPC/F), got %d", p.pot_model);
   if (p.nwaters < 2) return fail(PIMD_EINVAL, "need at least 2 waters, got %d", p.nwaters);
   if (p.nbeads < 1 || p.nbeads > 64) return fail(PIMD_EINVAL, "nbeads must be 1..64, got %d", p.nbeads);
   for (int k = 0; k < 3; ++k)
@@ -596,6 +639,10 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
   if (p.polar_maxiter < 1) return fail(PIMD_EINVAL, "polar_maxiter must be >= 1");
   if (p.contraction != 0 && p.contraction != 1)
     return fail(PIMD_EINVAL, "contraction must be 0 or 1, got %d", p.contraction);
+  // contracted MD takes its fast forces from the PS monomer surface
+  // (MD.f90:318-395), which only the TTM models share
+  if (p.contraction && p.pot_model >= PIMD_QSPCFW)
+    return fail(PIMD_EINVAL, "contraction needs a TTM model (pot_model 2 or 3)");
   if (p.precision != PIMD_FP64 && p.precision != PIMD_MIXED)
     return fail(PIMD_EINVAL, "precision must be PIMD_FP64 (0) or PIMD_MIXED (1), got %d", p.precision);
   if (p.mass_o <= 0.0) p.mass_o = (double)f32(15.994);
@@ -1427,6 +1474,8 @@ extern "C" int pimd_pes_params_set(pimd_ctx *x, const double *theta) {
 extern "C" int pimd_pes_param_grad(pimd_ctx *x, const double *RRt, const double *ew,
                                    const double *gw, double *grad) {
   if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
+  if (x->p.pot_model >= PIMD_QSPCFW)
+    return fail(PIMD_EINVAL, "the PS monomer surface belongs to the TTM models (pot_model 2, 3)");
   HIPCHK(hipSetDevice(x->p.device));
   const SysConst &c = x->c;
   DevBufs &d = x->d;
@@ -1475,6 +1524,8 @@ extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double 
   if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
   if (x->p.contraction)
     return fail(PIMD_EINVAL, "parameter derivatives need a context with contraction = 0");
+  if (x->p.pot_model >= PIMD_QSPCFW)
+    return fail(PIMD_EINVAL, "TTM parameter derivatives need a TTM model (pot_model 2, 3)");
   HIPCHK(hipSetDevice(x->p.device));
   const SysConst &c = x->c;
   DevBufs &d = x->d;

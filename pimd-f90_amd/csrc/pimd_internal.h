@@ -40,7 +40,7 @@ struct __align__(32) Site {
 
 // Everything the kernels need about the model and the box, passed by value.
 struct SysConst {
-  int N, P, model;        // waters, beads, 2 or 3
+  int N, P, model;        // waters, beads, pot_model (2, 3 TTM; 4, 5 fixed-charge SPC)
   int nd, tdip;           // dipole sites per molecule and the first dipole type
   int nk, ncol, T;        // k-vectors, kz-columns, phase-table rows per site
   int hmax, kmax, lmax;
@@ -72,6 +72,9 @@ struct SysConst {
   double sor, stath, peps;
   double Uvdw_lrc;
   double p4v;             // FOURPI / volume
+  // q-SPC/Fw and SPC/F (pot_mod.f90:31-44): charges qO, qH (times CHARGECON),
+  // equilibrium geometry and force constants
+  double spc_q[2], spc_roh0, spc_th0, spc_rhh0, spc_kb, spc_ka, spc_A, spc_B, spc_C, spc_D;
 };
 
 // Tally slots per (bead, molecule): scalar partial sums reduced per bead.
@@ -149,6 +152,11 @@ void launch_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI,
                      double *dipE, hipStream_t s);
 void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
                         double *virialc, int *iters, hipStream_t s);
+void launch_spc_monomer(const SysConst &c, DevBufs &b, const double *RR, double *chg_out,
+                        hipStream_t s);
+void launch_spc_pair(const SysConst &c, DevBufs &b, hipStream_t s);
+void launch_spc_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI, double *dipE,
+                         hipStream_t s);
 void pes_default_params(double *theta);
 void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
                         const double *gw, double *partial, int nblocks, double *grad,

@@ -48,6 +48,18 @@ __device__ __forceinline__ void minimg(const SysConst &c, double &x, double &y, 
   z = z - c.box[2] * anint(z * c.boxi[2]);
 }
 
+// Sum over the G lanes of a lane group (G a power of two, groups aligned).
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+__device__ __forceinline__ void vadd9(double *t, const double *v) {
+  for (int i = 0; i < 9; ++i) t[i] += v[i];
+}
+
 // x*x + y*y + z*z rounded exactly as the reference (no fused multiply-add), so
 // cut-off tests (r^2 < Rc^2, r^2 < 25) decide identically.
 __device__ __forceinline__ double norm2_exact(double x, double y, double z) {

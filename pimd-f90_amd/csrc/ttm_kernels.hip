@@ -35,17 +35,6 @@ void pes_default_params(double *theta) {
 }
 
 
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
-  return v;
-}
-
-__device__ __forceinline__ void vadd9(double *t, const double *v) {
-  for (int i = 0; i < 9; ++i) t[i] += v[i];
-}
-
 // Largest distance of H1, H2, M from O in a molecule record.
 __device__ __forceinline__ double site_radius(const SysConst &c, const Site *m) {
   double r2 = 0.0;
@@ -1674,7 +1663,11 @@ __global__ __launch_bounds__(256) void k_virk(SysConst c, const double4 *__restr
     const double kx[3] = {kk.x, kk.y, kk.z};
     const double A = Ak[k];
     const size_t o = (size_t)b * c.nk + k;
-    const double rqqm = Sq[o * 2] + Sm[o * 2], iqqm = Sq[o * 2 + 1] + Sm[o * 2 + 1];
+    // models without induced dipoles (nd = 0) have the charge part only
+    // (ewald.f90:267-279)
+    const bool dip = c.nd > 0;
+    const double rqqm = dip ? Sq[o * 2] + Sm[o * 2] : Sq[o * 2];
+    const double iqqm = dip ? Sq[o * 2 + 1] + Sm[o * 2 + 1] : Sq[o * 2 + 1];
     const double ksq = kx[0] * kx[0] + kx[1] * kx[1] + kx[2] * kx[2];
     const double coeff2 = 2.0 / ksq + 0.5 / (c.aewald * c.aewald);
     const double U = A * (rqqm * rqqm + iqqm * iqqm);
@@ -1682,7 +1675,7 @@ __global__ __launch_bounds__(256) void k_virk(SysConst c, const double4 *__restr
       for (int row = 0; row < 3; ++row)
         v[col * 3 + row] = (row == col ? U : 0.0) - U * coeff2 * kx[row] * kx[col];
     const double *sv = Sv + o * 6;  // sum mu cos (3), sum mu sin (3)
-    for (int ix = 0; ix < 3; ++ix) {
+    for (int ix = 0; ix < 3 && dip; ++ix) {
       const double t = 2.0 * A * kx[ix];
       for (int iy = 0; iy < 3; ++iy)
         v[9 + iy * 3 + ix] = t * (rqqm * (-sv[3 + iy]) + iqqm * sv[iy]);
@@ -1890,7 +1883,9 @@ __global__ __launch_bounds__(256) void k_reduce_bead(SysConst c, const double *_
     vt[0] -= c.Uvdw_lrc;
     vt[4] -= c.Uvdw_lrc;
     vt[8] -= c.Uvdw_lrc;
-    const double vcc = red[T_VIRC][0] + vrec[0] + vrec[4] + vrec[8] - 3 * c.Uvdw_lrc;
+    // the fixed-charge models leave virialc unset (pot_spc.f90:1-13): zero
+    const double vcc =
+        c.model >= PIMD_QSPCFW ? 0.0 : red[T_VIRC][0] + vrec[0] + vrec[4] + vrec[8] - 3 * c.Uvdw_lrc;
     if (Upot) Upot[b] = umon + uvdw + uelec + uind + c.Uvdw_lrc;
     if (virt)
       for (int i = 0; i < 9; ++i) virt[b * 9 + i] = vt[i];

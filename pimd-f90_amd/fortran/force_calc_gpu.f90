@@ -38,7 +38,9 @@ contains
     virialc = sum(virialct) / Nbeads
   end subroutine full_bead_forces
 
-  ! force_calc.f90:112-170 for the TTM models (pot_model 2 and 3).
+  ! force_calc.f90:112-170 for the TTM models (pot_model 2 and 3) and the
+  ! fixed-charge models of pot_spc (4 and 5; virialc and Edip_mom, which
+  ! pot_spc leaves unset, come back as zeros).
   subroutine potential(RR, RRc, Upot, dRR, virt, virialc, dip_momI, Edip_mom, chg, t, BAROSTAT, sys_lab)
     use system_mod
     implicit none
@@ -52,7 +54,8 @@ contains
     logical, intent(in) :: BAROSTAT
     character(len=*), intent(in) :: sys_lab
     double precision :: e5(5)
-    if (pot_model /= 2 .and. pot_model /= 3) error stop 'force_calc (GPU): only TTM2.1-F and TTM3-F'
+    if (pot_model < 2 .or. pot_model > 5) &
+      error stop 'force_calc (GPU): only TTM2.1-F, TTM3-F, q-SPC/Fw and SPC/F'
     if (BAROSTAT) error stop 'force_calc (GPU): the barostat path is not supported'
     if (.not. gpu_ready()) then
       call gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &

@@ -10,8 +10,9 @@ module pimd_ttm_c
 
   integer(c_int), parameter, public :: PIMD_OK = 0, PIMD_EINVAL = 1, PIMD_EDEVICE = 2, &
                                        PIMD_ENOMEM = 3, PIMD_ESTATE = 4
-  integer(c_int), parameter, public :: PIMD_TTM21F = 2, PIMD_TTM3F = 3, PIMD_FP64 = 0, &
-                                       PIMD_MIXED = 1, PIMD_NOBS = 9
+  integer(c_int), parameter, public :: PIMD_TTM21F = 2, PIMD_TTM3F = 3, PIMD_QSPCFW = 4, &
+                                       PIMD_SPCF = 5, PIMD_FP64 = 0, PIMD_MIXED = 1, &
+                                       PIMD_NOBS = 9
 
   type, bind(C), public :: pimd_params
     integer(c_int) :: pot_model = 3

@@ -4,7 +4,7 @@ The compute lives in libpimd_ttm.so (HIP kernels for gfx950 behind the C ABI
 of include/pimd_ttm.h); this package is the Python host mirror of the
 reference's force_calc / NormalModes / Langevin interface.
 """
-from .params import TTMParams, TTM21F, TTM3F, f32  # noqa: F401
+from .params import TTMParams, TTM21F, TTM3F, QSPCFW, SPCF, f32  # noqa: F401
 from . import systems  # noqa: F401
 
 

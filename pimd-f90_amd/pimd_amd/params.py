@@ -19,11 +19,13 @@ def f32(x: float) -> float:
 
 TTM21F = 2
 TTM3F = 3
+QSPCFW = 4     # q-SPC/Fw, fixed charges (pot_spc.f90:69-106)
+SPCF = 5       # SPC/F, fixed charges (pot_spc.f90:107-137)
 
 
 @dataclass
 class TTMParams:
-    pot_model: int = TTM3F              # 2 = TTM2.1-F, 3 = TTM3-F (potential.f90:84-113)
+    pot_model: int = TTM3F              # 2 TTM2.1-F, 3 TTM3-F, 4 q-SPC/Fw, 5 SPC/F (potential.f90:84-145)
     nwaters: int = 32
     nbeads: int = 1
     box: tuple = (0.0, 0.0, 0.0)

@@ -284,8 +284,9 @@ def run_setup(inp: dict, box, natoms: int):
     """(TTMParams, integrator keyword arguments) of a run described by an
     input dict, for pimd_amd.ttm.TTMForceField and PIMDIntegrator."""
     model = inp["pot_model"]
-    if model not in (2, 3):
-        raise InputError(f"pot_model {model} is not built (TTM2.1-F = 2 and TTM3-F = 3 are)")
+    if model not in (2, 3, 4, 5):
+        raise InputError(f"pot_model {model} is not built (TTM2.1-F = 2, TTM3-F = 3, "
+                         "q-SPC/Fw = 4 and SPC/F = 5 are)")
     if inp["BAROSTAT"]:
         raise InputError("the barostat is not built")
     bt = inp["bead_thermostat_type"].strip()

@@ -103,7 +103,7 @@ def predictor_fixture():
          **params_dict(p))
 
 
-def traj_fixture():
+def traj_fixture(only=None):
     """PIMD_VV with the NM Langevin thermostat; the normals it consumed are
     recorded.  The last case is centroid MD (setNMfreq 1500 cm^-1: scaled
     normal-mode masses, mode-space kinetic energy) with the global
@@ -111,7 +111,11 @@ def traj_fixture():
     for name, model, N, P, lat, glob, setnm in [
             ("traj_ttm3f_langevin_n8_p4", 3, 8, 4, (2, 2, 2), False, 0.0),
             ("traj_ttm21f_langevin_nh_n8_p2", 2, 8, 2, (2, 2, 2), True, 0.0),
-            ("traj_ttm3f_cmd_nh_n8_p4", 3, 8, 4, (2, 2, 2), True, 1500.0)]:
+            ("traj_ttm3f_cmd_nh_n8_p4", 3, 8, 4, (2, 2, 2), True, 1500.0),
+            ("traj_qspcfw_langevin_n8_p4", 4, 8, 4, (2, 2, 2), False, 0.0),
+            ("traj_spcf_langevin_nh_n8_p2", 5, 8, 2, (2, 2, 2), True, 0.0)]:
+        if only and name not in only:
+            continue
         RRt, PPt, RRc, L = make_system(N, P, seed=31 + N + P, lattice=lat)
         p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
                       eps_ewald=1e-6, mass_o=15.995, mass_h=1.008).resolved()
@@ -247,8 +251,9 @@ def scalar_fixtures():
 if __name__ == "__main__":
     if not refrun.available():
         sys.exit("build the reference first: make -C oracle ref")
-    if len(sys.argv) > 1:   # named force fixtures only
+    if len(sys.argv) > 1:   # named force and trajectory fixtures only
         force_fixtures(set(sys.argv[1:]))
+        traj_fixture(set(sys.argv[1:]))
         sys.exit(0)
     force_fixtures()
     predictor_fixture()

@@ -70,7 +70,8 @@ def _select(r, beads):
 
 
 @pytest.mark.parametrize("model,N,P,rc,eps", [(2, 128, 4, -1.0, 1e-6), (3, 128, 3, -1.0, 1e-8),
-                                              (3, 256, 2, 9.0, 1e-6)])
+                                              (3, 256, 2, 9.0, 1e-6), (4, 128, 4, -1.0, 1e-8),
+                                              (5, 256, 2, 9.0, 1e-6)])
 def test_forces_vs_oracle_medium(model, N, P, rc, eps):
     from pimd_amd.params import TTMParams
     from pimd_amd.systems import make_system

@@ -1,7 +1,8 @@
 """MD steps replayed from a captured HIP graph against the same steps launched
 one kernel at a time (PIMD_MD_GRAPH=0): every replay, with host reads and
 synchronisations between the runs, must give the eager records, observables
-and state bit for bit (plain, centroid-MD and contracted steps, and a batch)."""
+and state bit for bit (plain, centroid-MD and contracted steps, a fixed-charge
+q-SPC/Fw system, and a batch)."""
 import os
 
 import numpy as np
@@ -39,11 +40,12 @@ def _runs(integ, graph: bool, chunks=(3, 2, 4)):
     return out
 
 
-@pytest.mark.parametrize("case", ["ttm3f", "ttm21f_cmd_nh", "contracted"])
+@pytest.mark.parametrize("case", ["ttm3f", "ttm21f_cmd_nh", "contracted", "qspcfw"])
 def test_graph_replay_matches_eager(case):
     kw = {"ttm3f": (3, 64, 8, {}),
           "ttm21f_cmd_nh": (2, 32, 4, {"setNMfreq": 1500.0, "global_nh": True}),
-          "contracted": (3, 32, 4, {"intra_timesteps": 3})}[case]
+          "contracted": (3, 32, 4, {"intra_timesteps": 3}),
+          "qspcfw": (4, 64, 8, {})}[case]
     model, N, P, md = kw
     print("case", case, flush=True)
     a = _runs(_integrator(model, N, P, 77, **md), graph=False)

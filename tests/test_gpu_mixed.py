@@ -91,7 +91,8 @@ def test_mixed_deterministic(big_pair):
         assert np.array_equal(again[k], rmx[k]), k
 
 
-@pytest.mark.parametrize("model,N,P,rc,eps", [(2, 128, 8, -1.0, 1e-6), (3, 32, 8, 4.9, 1e-8)])
+@pytest.mark.parametrize("model,N,P,rc,eps", [(2, 128, 8, -1.0, 1e-6), (3, 32, 8, 4.9, 1e-8),
+                                              (4, 256, 4, -1.0, 1e-6), (5, 128, 4, -1.0, 1e-8)])
 def test_mixed_vs_fp64_other_systems(model, N, P, rc, eps):
     from pimd_amd.params import TTMParams
     from pimd_amd.systems import make_system

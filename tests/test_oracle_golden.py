@@ -187,7 +187,7 @@ def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=N
     return R, Pm, dR, np.array(out)
 
 
-@pytest.mark.parametrize("name", golden_names("traj_ttm") )
+@pytest.mark.parametrize("name", golden_names("traj_"))
 def test_trajectory_vs_reference(oracle_lib, name):
     g = load_golden(name)
     p = params_from(g)
