@@ -45,7 +45,10 @@ WORKLOADS = {
     "h2o512_p32_ttm3f": (3, 512, 32, -1.0, 1e-6),
     "h2o128_p32_ttm21f": (2, 128, 32, -1.0, 1e-6),
     "h2o32_p32_ttm3f_rc49": (3, 32, 32, 4.9, 1e-8),
+    # the fixed-charge q-SPC/Fw model of pot_spc on the same system
+    "h2o512_p32_qspcfw": (4, 512, 32, -1.0, 1e-6),
 }
+MODEL_NAMES = {2: "TTM2.1-F", 3: "TTM3-F", 4: "q-SPC/Fw", 5: "SPC/F"}
 
 
 def flop_model(kernel, c):
@@ -54,6 +57,9 @@ def flop_model(kernel, c):
     inner loop; see the kernel-table of the design notes."""
     N, P, nk, nnb, nd = c["N"], c["P"], c["nk"], c["nnb"], c["nd"]
     pairs = P * N * nnb  # ordered molecule pairs (each side evaluated once)
+    if c.get("model", 3) >= 4 and kernel == "pair_qq":
+        # fixed charges: vdW ~30 + per site pair: vectors 17, erfc-screened ts0/ts1 ~26, field 9
+        return pairs * (30 + 9 * 52)
     if kernel == "field_d_sor":   # per (dipole site, k): phase step 6 + field 8
         return P * N * nd * nk * 14
     if kernel == "sfac_d":        # per (dipole site, k): phase 12 + k.mu 5 + acc 4
@@ -119,7 +125,7 @@ def cpu_baseline(p, RRt, RRc, budget_s=15.0):
     dt = time.perf_counter() - t0
     return {"value": p.nwaters * nb / dt, "unit": "molecule-bead force evals/s", "cores": 1,
             "kind": kind,
-            "sample": f"pot_ttm on {nb} of the {p.nbeads} beads of the same system "
+            "sample": f"{'pot_spc' if p.pot_model >= 4 else 'pot_ttm'} on {nb} of the {p.nbeads} beads of the same system "
                       f"({'reference Fortran, amdflang -O2' if kind == 'reference' else 'C restatement, gcc -O2'}), "
                       f"serial, {dt:.1f} s"}
 
@@ -163,7 +169,7 @@ def run_form(p, RRt, PPt, RRc, args):
     out["kernels"] = kernels
     nnb = (4.0 / 3.0) * np.pi * p.rc ** 3 / (p.box[0] * p.box[1] * p.box[2]) * p.nwaters
     c = {"N": p.nwaters, "P": p.nbeads, "nk": info["nkvec"], "nnb": min(nnb, p.nwaters - 1),
-         "nd": 1 if p.pot_model == 3 else 3}
+         "nd": {2: 3, 3: 1}.get(p.pot_model, 0), "model": p.pot_model}
     tot = {k: v[1] for k, v in kernels.items()}
     dom = max(tot, key=tot.get)
     cnt, tms = kernels[dom]
@@ -562,7 +568,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64" if main_form == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": args.workload, "nwaters": p.nwaters, "nbeads": p.nbeads,
-                   "model": "TTM3-F" if p.pot_model == 3 else "TTM2.1-F", "precision": main_form,
+                   "model": MODEL_NAMES[p.pot_model], "precision": main_form,
                    "rc": round(p.rc, 4), "eps_ewald": p.eps_ewald, "nkvec": r["info"]["nkvec"],
                    "dt_fs": 0.25, "thermostat": "NM Langevin (PILE), 300 K",
                    "polar_maxiter": p.polar_maxiter, "polar_eps": p.polar_eps,

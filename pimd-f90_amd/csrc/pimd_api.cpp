@@ -83,6 +83,10 @@ struct pimd_ctx {
   int *h_nactive = nullptr;
   // profiling
   bool prof = false;
+  // inside a graph capture: independent real- and reciprocal-space kernels
+  // of a force call go on two streams (graph branches; in eager launches the
+  // event fork and join cost more than the overlap gains)
+  bool branches = false;
   std::vector<std::string> names;
   std::map<std::string, int> name_id;
   std::vector<long> launches;
@@ -161,7 +165,7 @@ void free_all(pimd_ctx *x) {
                   d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
                   d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
                   d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, d.clk_own, d.obs_part, x->d_vxi, x->d_s,
+                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, d.clk_own, d.obs_part, d.kE, d.kphi, x->d_vxi, x->d_s,
                   x->d_step};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -352,6 +356,8 @@ int alloc_system(pimd_ctx *x) {
   ALLOC(d.grdq, P * N * 27);
   ALLOC(d.phi, P * S);
   ALLOC(d.Efq, P * S * 3);
+  ALLOC(d.kE, P * S * 3);
+  ALLOC(d.kphi, P * S);
   ALLOC(d.dR, P * S * 3);
   ALLOC(d.dip, P * N * nd * 3);
   ALLOC(d.dip_prev, P * N * nd * 3);
@@ -518,16 +524,23 @@ int pick_group(const SysConst &c) {
   return 16;
 }
 
+// a(stream2) beside b(s); s waits for both.
+template <class A, class B>
+int beside(pimd_ctx *x, hipStream_t s, A &&a, B &&b) {
+  HIPCHK(hipEventRecord(x->ev_fork, s));
+  HIPCHK(hipStreamWaitEvent(x->stream2, x->ev_fork, 0));
+  a(x->stream2);
+  HIPCHK(hipEventRecord(x->ev_join, x->stream2));
+  b(s);
+  HIPCHK(hipStreamWaitEvent(s, x->ev_join, 0));
+  return PIMD_OK;
+}
+
 // The real-space dipole field on the second stream beside the dipole
 // structure factor on the first; the first stream waits for both.
 int dd_beside_sfac(pimd_ctx *x, hipStream_t s) {
-  HIPCHK(hipEventRecord(x->ev_fork, s));
-  HIPCHK(hipStreamWaitEvent(x->stream2, x->ev_fork, 0));
-  launch_pair_dd(x->c, x->d, x->group, false, x->stream2);
-  HIPCHK(hipEventRecord(x->ev_join, x->stream2));
-  launch_sfac(x->c, x->d, SF_D, s);
-  HIPCHK(hipStreamWaitEvent(s, x->ev_join, 0));
-  return PIMD_OK;
+  return beside(x, s, [&](hipStream_t s2) { launch_pair_dd(x->c, x->d, x->group, false, s2); },
+                [&](hipStream_t s1) { launch_sfac(x->c, x->d, SF_D, s1); });
 }
 
 __global__ void k_zero_int(int *v, int n) {
@@ -544,13 +557,25 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
   const SysConst &c = x->c;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
+  const bool two = x->branches && !x->prof;
+  // the reciprocal-space charge pass, beside the real-space pair pass when
+  // two streams are in use (it writes kE / kphi, read after the join)
+  auto kspace_q = [&](hipStream_t st) {
+    x->timed("ewald_tables", [&] { launch_tables(c, d, st); });
+    x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, st); });
+    x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, st); });
+  };
   if (c.model >= PIMD_QSPCFW) {  // pot_spc (pot_spc.f90:1-277)
     x->timed("monomer", [&] { launch_spc_monomer(c, d, RR, chg, s); });
     x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
-    x->timed("pair_qq", [&] { launch_spc_pair(c, d, s); });
-    x->timed("ewald_tables", [&] { launch_tables(c, d, s); });
-    x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, s); });
-    x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
+    auto pair = [&](hipStream_t st) { x->timed("pair_qq", [&] { launch_spc_pair(c, d, st); }); };
+    if (two) {
+      const int err = beside(x, s, pair, kspace_q);
+      if (err != PIMD_OK) return err;
+    } else {
+      pair(s);
+      kspace_q(s);
+    }
     x->timed("virial_k", [&] { launch_virk(c, d, s); });
     x->timed("finalize", [&] { launch_spc_finalize(c, d, dRR, dipI, dipE, s); });
     x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, nullptr, s); });
@@ -561,13 +586,19 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
   }
   x->timed("monomer", [&] { launch_monomer(c, d, RR, RRc, chg, s); });
   x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
-  x->timed("pair_qq", [&] { launch_pair_qq(c, d, x->group, s); });
-  x->timed("ewald_tables", [&] { launch_tables(c, d, s); });
-  x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, s); });
-  x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, s); });
+  {
+    auto pair = [&](hipStream_t st) { x->timed("pair_qq", [&] { launch_pair_qq(c, d, x->group, st); }); };
+    if (two) {
+      const int err = beside(x, s, pair, kspace_q);
+      if (err != PIMD_OK) return err;
+    } else {
+      pair(s);
+      kspace_q(s);
+    }
+  }
   x->timed("post_qq", [&] { launch_post_qq(c, d, s); });
   for (int it = 1; it <= c.maxiter; ++it) {
-    if (c.overlap) {
+    if (c.overlap || two) {
       // pair_dd on the second stream, concurrent with the structure factor
       int err = PIMD_OK;
       x->timed("dd_sfac_d", [&] { err = dd_beside_sfac(x, s); });
@@ -599,10 +630,22 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
       launch_field(c, d, FD_D, c.maxiter + 1, s);
     });
   }
-  x->timed("pair_final", [&] { launch_pair_final(c, d, x->group, s); });
-  x->timed("sfac_v", [&] { launch_sfac(c, d, SF_V, s); });
-  x->timed("field_qd", [&] { launch_field(c, d, FD_QD, 0, s); });
-  x->timed("virial_k", [&] { launch_virk(c, d, s); });
+  {
+    // the final real-space pass beside the reciprocal-space forces and virial
+    auto pair = [&](hipStream_t st) { x->timed("pair_final", [&] { launch_pair_final(c, d, x->group, st); }); };
+    auto kspace = [&](hipStream_t st) {
+      x->timed("sfac_v", [&] { launch_sfac(c, d, SF_V, st); });
+      x->timed("field_qd", [&] { launch_field(c, d, FD_QD, 0, st); });
+      x->timed("virial_k", [&] { launch_virk(c, d, st); });
+    };
+    if (two) {
+      const int err = beside(x, s, pair, kspace);
+      if (err != PIMD_OK) return err;
+    } else {
+      pair(s);
+      kspace(s);
+    }
+  }
   x->timed("finalize", [&] { launch_finalize(c, d, dRR, dipI, dipE, s); });
   x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, iters, s); });
   HIPCHK(hipGetLastError());
@@ -1176,8 +1219,8 @@ static int md_step_eager(pimd_ctx *x, const double *noise, uint64_t seed) {
 static bool graph_eligible(const pimd_ctx *x, const double *noise) {
   const int on = env_int("PIMD_MD_GRAPH", 1, 0, 1);  // read per step: a test switches it
   const SysConst &c = x->c;
-  const bool cen_ok = !x->cen || (!x->cen->c.poll && !x->cen->c.overlap && !x->cen->prof);
-  return on && !noise && !x->prof && !c.poll && !c.overlap && cen_ok;
+  const bool cen_ok = !x->cen || (!x->cen->c.poll && !x->cen->prof);
+  return on && !noise && !x->prof && !c.poll && cen_ok;
 }
 
 static int md_graph_build(pimd_ctx *x, uint64_t seed) {
@@ -1185,8 +1228,17 @@ static int md_graph_build(pimd_ctx *x, uint64_t seed) {
   x->graph = nullptr;
   x->graph_ok = false;
   hipStream_t s = x->stream;
+  // PIMD_GRAPH_BRANCHES=1: force calls inside the step split into two graph
+  // branches where their real- and reciprocal-space kernels are independent.
+  // Off by default: on MI355X it gained 4 % on one FP64 system, nothing in
+  // the mixed form, and slowed batches of replicas by a third.
+  const bool br = env_int("PIMD_GRAPH_BRANCHES", 0, 0, 1) != 0;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  x->branches = br;
+  if (x->cen) x->cen->branches = br;
   const int rc = md_step_eager(x, nullptr, seed);
+  x->branches = false;
+  if (x->cen) x->cen->branches = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &g);
   if (rc != PIMD_OK) {

@@ -132,6 +132,10 @@ struct DevBufs {
   double *tan = nullptr;                        // TTM parameter tangents: inputs, tables, workspace
   size_t tan_bytes = 0;
   double *red = nullptr;                        // scratch for reductions
+  // reciprocal-space field, potential (FD_Q) and gradient, potential (FD_QD)
+  // of the last field pass, [P][N][4][3] and [P][N][4]; their consumers add
+  // them to Efq / dR and phi, so the real-space pair pass may run beside
+  double *kE = nullptr, *kphi = nullptr;
 };
 
 // Host-side launchers (ttm_kernels.hip / rp_kernels.hip).

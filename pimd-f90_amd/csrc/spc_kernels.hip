@@ -307,6 +307,8 @@ void launch_spc_pair(const SysConst &c, DevBufs &b, hipStream_t s) {
 __global__ __launch_bounds__(64) void k_spc_finalize(SysConst c, const Site *__restrict__ site,
                                                     const double *__restrict__ phi,
                                                     const double *__restrict__ Efq,
+                                                    const double *__restrict__ kphi,
+                                                    const double *__restrict__ kE,
                                                     const double *__restrict__ dR, double *tally,
                                                     double *dRR, double *dipI, double *dipE) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -315,9 +317,14 @@ __global__ __launch_bounds__(64) void k_spc_finalize(SysConst c, const Site *__r
   const size_t base = ((size_t)b * N + w) * 4;
   double ue = 0.0, g[4][3];
   for (int t = 1; t < 4; ++t) {
+    // real-space sums plus the reciprocal-space part (ewald.f90:270-276)
     const double q = site[base + t].q;
-    ue += q * phi[base + t];
-    for (int k = 0; k < 3; ++k) g[t][k] = dR[(base + t) * 3 + k] - q * Efq[(base + t) * 3 + k];
+    const double pt = phi[base + t] + kphi[base + t];
+    ue += q * pt;
+    for (int k = 0; k < 3; ++k) {
+      const double e = Efq[(base + t) * 3 + k] + kE[(base + t) * 3 + k];
+      g[t][k] = dR[(base + t) * 3 + k] - q * e;
+    }
   }
   double *out = dRR + ((size_t)b * 3 * N + 3 * w) * 3;
   for (int k = 0; k < 3; ++k) {
@@ -340,7 +347,7 @@ void launch_spc_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dip
                          hipStream_t s) {
   const int n = c.P * c.N;
   hipLaunchKernelGGL(k_spc_finalize, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.phi, b.Efq,
-                     b.dR, b.tally, dRR, dipI, dipE);
+                     b.kphi, b.kE, b.dR, b.tally, dRR, dipI, dipE);
 }
 
 }  // namespace pimd

@@ -1034,16 +1034,14 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
     ph += red[(w * 4 + 3) * FD_SB + lane];
   }
   const size_t ss = (size_t)b * S + s;
-  if (MODE == FD_Q) {
-    Efq[ss * 3 + 0] += e0;
-    Efq[ss * 3 + 1] += e1;
-    Efq[ss * 3 + 2] += e2;
-    if (has_q) phi[ss] += ph;
-  } else if (MODE == FD_QD) {
-    dR[ss * 3 + 0] += e0;
-    dR[ss * 3 + 1] += e1;
-    dR[ss * 3 + 2] += e2;
-    if (has_q) phi[ss] += ph;
+  if (MODE == FD_Q || MODE == FD_QD) {
+    // into their own buffers (kE, kphi): the consumer adds them to the
+    // real-space sums, which the pair kernels may be forming meanwhile
+    double *ko = MODE == FD_Q ? Efq : dR;
+    ko[ss * 3 + 0] = e0;
+    ko[ss * 3 + 1] = e1;
+    ko[ss * 3 + 2] = e2;
+    phi[ss] = has_q ? ph : 0.0;
   } else {
     const double e[3] = {e0, e1, e2};
     const double om = c.sor, pol = c.polar[type];
@@ -1070,13 +1068,17 @@ static size_t field_lds(const SysConst &c, int mode, bool kl) {
 template <int MO, int ML, int CG>
 static void launch_field_cg(const SysConst &c, DevBufs &b, int iter, dim3 grid, hipStream_t s) {
   const bool kl = c.fd_klds && field_lds(c, MO, true) <= 160 * 1024;
+  // FD_Q and FD_QD write the reciprocal-space parts to kE / kphi
+  double *phi = MO == FD_D ? b.phi : b.kphi;
+  double *Efq = MO == FD_Q ? b.kE : b.Efq;
+  double *dR = MO == FD_QD ? b.kE : b.dR;
   if (kl)
     hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), field_lds(c, MO, true), s,
-                       c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
+                       c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, phi, Efq, dR,
                        b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
   else
     hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), field_lds(c, MO, false),
-                       s, c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, b.phi, b.Efq, b.dR,
+                       s, c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, phi, Efq, dR,
                        b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
 }
 
@@ -1110,8 +1112,9 @@ void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t
 // initial induced dipoles (pot_ttm.f90:273-297).  One thread per (bead, molecule).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restrict__ site,
-                                               const double *__restrict__ phi,
-                                               const double *__restrict__ Efq, double *dR,
+                                               double *phi, double *Efq,
+                                               const double *__restrict__ kE,
+                                               const double *__restrict__ kphi, double *dR,
                                                double *dip, const double *__restrict__ hist,
                                                double *tally, int *active, int *iters,
                                                int *nactive, const int *__restrict__ clk) {
@@ -1120,6 +1123,13 @@ __global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restri
   const int t = clk[0];  // MD step of this force call
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const size_t base = ((size_t)b * N + w) * 4;
+  // real-space sums plus the reciprocal-space part (ewald.f90:45-49): the
+  // field of every site of the charge pass (all four for TTM2.1-F, where O
+  // holds a dipole), the potential of the charged sites
+  for (int ty = c.model == 2 ? 0 : 1; ty < 4; ++ty) {
+    for (int k = 0; k < 3; ++k) Efq[(base + ty) * 3 + k] = Efq[(base + ty) * 3 + k] + kE[(base + ty) * 3 + k];
+    if (ty > 0) phi[base + ty] = phi[base + ty] + kphi[base + ty];
+  }
   double ue = 0.0;
   for (int ty = 1; ty < 4; ++ty) {
     const double q = site[base + ty].q;
@@ -1160,7 +1170,8 @@ __global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restri
 void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s) {
   int n = c.P * c.N;
   hipLaunchKernelGGL(k_post_qq, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.phi, b.Efq,
-                     b.dR, b.dip, b.hist, b.tally, b.active, b.iters, b.nactive, b.clk);
+                     b.kE, b.kphi, b.dR, b.dip, b.hist, b.tally, b.active, b.iters, b.nactive,
+                     b.clk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1702,9 +1713,10 @@ __global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restr
                                                 const double *__restrict__ grdq,
                                                 const double *__restrict__ dip,
                                                 const double *__restrict__ Efq, double *phi,
-                                                double *dR, double *hist, double *tally,
-                                                double *dRR, double *dipI, double *dipE,
-                                                const int *__restrict__ clk) {
+                                                double *dR, const double *__restrict__ kE,
+                                                const double *__restrict__ kphi, double *hist,
+                                                double *tally, double *dRR, double *dipI,
+                                                double *dipE, const int *__restrict__ clk) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1721,6 +1733,13 @@ __global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restr
       mu[ty][k] = is_dip<MODEL>(ty) ? dip[(((size_t)b * N + w) * ND + (ty - T0)) * 3 + k] : 0.0;
       cen[ty][k] = Rcs[(w * 4 + ty) * 3 + k];
       fr[ty][k] = dR[(base + ty) * 3 + k];
+    }
+    // plus the reciprocal-space part of the last field pass (ewald.f90:184-193):
+    // every site of the pass (all four for TTM2.1-F), the potential of the
+    // charged sites
+    if (MODEL == 2 || ty > 0) {
+      for (int k = 0; k < 3; ++k) fr[ty][k] = fr[ty][k] + kE[(base + ty) * 3 + k];
+      if (ty > 0) ph[ty] = ph[ty] + kphi[base + ty];
     }
   }
   double vc = 0.0, vt[9] = {0};
@@ -1838,10 +1857,12 @@ void launch_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI, d
   int n = c.P * c.N;
   if (c.model == 2)
     hipLaunchKernelGGL(k_finalize<2>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
-                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, b.clk);
+                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.kE, b.kphi, b.hist, b.tally, dRR, dipI,
+                       dipE, b.clk);
   else
     hipLaunchKernelGGL(k_finalize<3>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
-                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.hist, b.tally, dRR, dipI, dipE, b.clk);
+                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.kE, b.kphi, b.hist, b.tally, dRR, dipI,
+                       dipE, b.clk);
 }
 
 void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s) {

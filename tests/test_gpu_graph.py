@@ -25,8 +25,9 @@ def _integrator(model, N, P, seed, **md):
     return integ
 
 
-def _runs(integ, graph: bool, chunks=(3, 2, 4)):
+def _runs(integ, graph: bool, chunks=(3, 2, 4), branches=False):
     os.environ["PIMD_MD_GRAPH"] = "1" if graph else "0"
+    os.environ["PIMD_GRAPH_BRANCHES"] = "1" if branches else "0"
     out, t0 = [], 1
     try:
         for n in chunks:
@@ -37,19 +38,28 @@ def _runs(integ, graph: bool, chunks=(3, 2, 4)):
             t0 += n
     finally:
         os.environ.pop("PIMD_MD_GRAPH", None)
+        os.environ.pop("PIMD_GRAPH_BRANCHES", None)
     return out
 
 
-@pytest.mark.parametrize("case", ["ttm3f", "ttm21f_cmd_nh", "contracted", "qspcfw"])
+@pytest.mark.parametrize("case", ["ttm3f", "ttm21f_cmd_nh", "contracted", "qspcfw",
+                                  "ttm3f_branches", "ttm21f_branches", "contracted_branches",
+                                  "qspcfw_branches"])
 def test_graph_replay_matches_eager(case):
+    """*_branches: the graph's force calls split into a real-space and a
+    reciprocal-space branch on two streams (PIMD_GRAPH_BRANCHES=1)."""
     kw = {"ttm3f": (3, 64, 8, {}),
           "ttm21f_cmd_nh": (2, 32, 4, {"setNMfreq": 1500.0, "global_nh": True}),
           "contracted": (3, 32, 4, {"intra_timesteps": 3}),
-          "qspcfw": (4, 64, 8, {})}[case]
+          "qspcfw": (4, 64, 8, {}),
+          "ttm3f_branches": (3, 64, 8, {}),
+          "ttm21f_branches": (2, 32, 4, {}),
+          "contracted_branches": (3, 32, 4, {"intra_timesteps": 3}),
+          "qspcfw_branches": (4, 64, 8, {})}[case]
     model, N, P, md = kw
     print("case", case, flush=True)
     a = _runs(_integrator(model, N, P, 77, **md), graph=False)
-    b = _runs(_integrator(model, N, P, 77, **md), graph=True)
+    b = _runs(_integrator(model, N, P, 77, **md), graph=True, branches=case.endswith("_branches"))
     for (ra, oa, sa), (rb, ob, sb) in zip(a, b):
         for k in ("Upot", "virial", "virialc", "uk"):
             assert np.array_equal(ra[k], rb[k]), k
