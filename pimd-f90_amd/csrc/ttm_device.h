@@ -48,6 +48,25 @@ __device__ __forceinline__ void minimg(const SysConst &c, double &x, double &y, 
   z = z - c.box[2] * anint(z * c.boxi[2]);
 }
 
+// Split sums over the MONO_W waves of a block (one part per wave, one
+// molecule per lane): partials through LDS, added in part order by every wave
+// alike.  Every thread of the block must call it.
+template <int W>
+struct WaveSplit {
+  double *buf;  // [W][8][64] doubles of LDS
+  __device__ void operator()(double *v, int n) const {
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    for (int i = 0; i < n; ++i) buf[(g * 8 + i) * 64 + lane] = v[i];
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+      double s = buf[i * 64 + lane];
+      for (int h = 1; h < W; ++h) s += buf[(h * 8 + i) * 64 + lane];
+      v[i] = s;
+    }
+    __syncthreads();  // the buffer is reused by the next call
+  }
+};
+
 // Sum over the G lanes of a lane group (G a power of two, groups aligned).
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {
@@ -254,10 +273,17 @@ __device__ __forceinline__ void dd3_apply(const R r[3], R ts1, R ts2, const R v[
 // pes: the fitted PES parameters, [0, 245) the combined coefficients c5z of
 // nasa.f90:45, then deohA, alphaoh, roh, phh1A, phh2 (PES_* indices).
 // DMS = false: the potential energy surface only (q3, gq untouched).
-template <bool DMS = true>
+// Split sums: with a reducer, part `part` of L takes every L-th polynomial
+// term and red(v, n) must replace v[0..n) by the sums over the L parts, the
+// same on every part (NoSplit: one part, the terms in the reference's order).
+struct NoSplit {
+  __device__ void operator()(double *, int) const {}
+};
+template <bool DMS = true, int L = 1, class Red = NoSplit>
 __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], double &e1,
                                   double dr1[3][3], double q3[3], double gq[3][3][3],
-                                  const double *__restrict__ pes) {
+                                  const double *__restrict__ pes, int part = 0,
+                                  const Red &red = Red()) {
   const double deohA = pes[PES_DEOHA], alphaoh = pes[PES_ALPHAOH], roh = pes[PES_ROH],
                phh1A = pes[PES_PHH1A], phh2 = pes[PES_PHH2];
   double R1[3], R2[3], RH[3];
@@ -298,7 +324,11 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   double dVb = -phh2 * Vb / dh;
   double efac = exp(-PS_B1 * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  // the loop is unrolled, so the table indices are constants; a part skips
+  // the other parts' terms with a branch that is uniform across the wave
+#pragma unroll
   for (int j = 1; j < PS_NTERM; ++j) {
+    if (L > 1 && (j - 1) % L != part) continue;
     int I = c_ps.idx[j][0], J = c_ps.idx[j][1], K = c_ps.idx[j][2];
     double cz = pes[j];
     double fI1 = f1[I], fJ2 = f2[J], fJ1 = f1[J], fI2 = f2[I], fK3 = f3[K];
@@ -306,6 +336,14 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
     s1 = s1 + cz * ((double)(I - 1) * f1[I - 1] * fJ2 + (double)(J - 1) * f1[J - 1] * fI2) * fK3;
     s2 = s2 + cz * ((double)(J - 1) * fI1 * f2[J - 1] + (double)(I - 1) * fJ1 * f2[I - 1]) * fK3;
     s3 = s3 + cz * (fI1 * fJ2 + fJ1 * fI2) * (double)(K - 1) * f3[K - 1];
+  }
+  if (L > 1) {
+    double v[4] = {s0, s1, s2, s3};
+    red(v, 4);
+    s0 = v[0];
+    s1 = v[1];
+    s2 = v[2];
+    s3 = v[3];
   }
   double Vc = 2.0 * pes[0] + efac * s0;
   double e = Va + Vb + Vc;
@@ -331,7 +369,9 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   double efd = exp(-DMS_B1D * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
   double P1 = 0, P2 = 0, PL1 = costh, PL2 = 0.5 * (3.0 * PL1 * PL1 - 1.0);
   double p1r1 = 0, p1r2 = 0, p1c = 0, p2r1 = 0, p2r2 = 0, p2c = 0;
+#pragma unroll
   for (int j = 1; j < PS_NDMS; ++j) {
+    if (L > 1 && (j - 1) % L != part) continue;
     int I = c_ps.idxd[j][0], J = c_ps.idxd[j][1], K = c_ps.idxd[j][2];
     double cj = c_ps.coefd[j];
     P1 = P1 + cj * f1[I] * f2[J] * f3[K];
@@ -342,6 +382,18 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
     p2r1 = p2r1 + cj * ((double)(J - 1) * f1[J - 1] * f2[I] * f3[K]);
     p2r2 = p2r2 + cj * ((double)(I - 1) * f1[J] * f2[I - 1] * f3[K]);
     p2c = p2c + cj * ((double)(K - 1) * f1[J] * f2[I] * f3[K - 1]);
+  }
+  if (L > 1) {
+    double v[8] = {P1, P2, p1r1, p1r2, p1c, p2r1, p2r2, p2c};
+    red(v, 8);
+    P1 = v[0];
+    P2 = v[1];
+    p1r1 = v[2];
+    p1r2 = v[3];
+    p1c = v[4];
+    p2r1 = v[5];
+    p2r2 = v[6];
+    p2c = v[7];
   }
   p1r1 = p1r1 / PS_REOH * B2A;
   p1r2 = p1r2 / PS_REOH * B2A;

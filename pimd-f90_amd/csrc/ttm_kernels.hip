@@ -52,22 +52,32 @@ __device__ __forceinline__ double site_radius(const SysConst &c, const Site *m) 
 // self energy and the screened intramolecular charge pairs.  One thread per
 // (bead, molecule).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__restrict__ RR,
-                                               const double *__restrict__ RRc, Site *site,
-                                               double *Rcs, double *grdq, double *phi,
-                                               double *Efq, double *dR, double *tally,
-                                               double *chg_out, double *radius,
-                                               const double *__restrict__ pes) {
-  int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= c.P * c.N) return;
+// A block holds 64 (bead, molecule) pairs, one per lane, in MONO_W waves:
+// wave g sums the g-th share of the PS and DMS polynomial terms (so the term
+// index, and the table reads it drives, stay uniform across a wave) and the
+// shares are added through LDS; wave 0 writes the results.
+constexpr int MONO_W = 4;
+__global__ __launch_bounds__(64 * MONO_W) void k_monomer(SysConst c, const double *__restrict__ RR,
+                                                        const double *__restrict__ RRc, Site *site,
+                                                        double *Rcs, double *grdq, double *phi,
+                                                        double *Efq, double *dR, double *tally,
+                                                        double *chg_out, double *radius,
+                                                        const double *__restrict__ pes) {
+  __shared__ double lds_part[MONO_W * 8 * 64];
+  const int wv = threadIdx.x >> 6;
+  const int gid0 = blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool valid = gid0 < c.P * c.N;
+  const int gid = valid ? gid0 : 0;  // spare lanes take part in the sums, write nothing
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const double *rr = RR + ((size_t)b * 3 * N + 3 * w) * 3;
   double r[3][3], rc[3][3];
   for (int a = 0; a < 3; ++a)
-    for (int k = 0; k < 3; ++k) {
-      r[a][k] = rr[3 * a + k];
-      rc[a][k] = RRc[(3 * w + a) * 3 + k];
-    }
+    for (int k = 0; k < 3; ++k) r[a][k] = rr[3 * a + k];
+  double e1, dr1[3][3], q3[3], gq[3][3][3];
+  ps_monomer<true, MONO_W>(c, r, e1, dr1, q3, gq, pes, wv, WaveSplit<MONO_W>{lds_part});
+  if (wv != 0 || !valid) return;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) rc[a][k] = RRc[(3 * w + a) * 3 + k];
   const double gM = c.gammaM;
   double pos[4][3], cen[4][3];
   {
@@ -95,8 +105,6 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
     for (int t = 0; t < 4; ++t)
       for (int k = 0; k < 3; ++k) Rcs[(w * 4 + t) * 3 + k] = cen[t][k];
 
-  double e1, dr1[3][3], q3[3], gq[3][3][3];
-  ps_monomer(c, r, e1, dr1, q3, gq, pes);
   if (c.contraction) {  // pot_ttm.f90:92-99: no monomer energy, gradient or virial
     e1 = 0.0;
     for (int a = 0; a < 3; ++a)
@@ -188,9 +196,9 @@ __global__ __launch_bounds__(64) void k_monomer(SysConst c, const double *__rest
 
 void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
                     double *chg_out, hipStream_t s) {
-  int n = c.P * c.N;
-  hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.site, b.Rcs,
-                     b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius, b.pes);
+  const int n = c.P * c.N;
+  hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64 * MONO_W), 0, s, c, RR, RRc, b.site,
+                     b.Rcs, b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius, b.pes);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,18 +208,22 @@ void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const doubl
 // Fast (intramolecular) forces: the PES gradient of every monomer of every
 // bead into dRR (MD.f90:339-353).  With record, also per molecule the energy,
 // the bead virial and the centroid virial (MD.f90:355-372) into mol[P][N][3].
-__global__ __launch_bounds__(64) void k_pes_fast(SysConst c, const double *__restrict__ RR,
-                                                const double *__restrict__ RRc,
-                                                const double *__restrict__ pes, double *dRR,
-                                                double *mol, int record) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= c.P * c.N) return;
+__global__ __launch_bounds__(64 * MONO_W) void k_pes_fast(SysConst c, const double *__restrict__ RR,
+                                                         const double *__restrict__ RRc,
+                                                         const double *__restrict__ pes,
+                                                         double *dRR, double *mol, int record) {
+  __shared__ double lds_part[MONO_W * 8 * 64];
+  const int wv = threadIdx.x >> 6;
+  const int gid0 = blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool valid = gid0 < c.P * c.N;
+  const int gid = valid ? gid0 : 0;  // spare lanes take part in the sums, write nothing
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const double *rr = RR + ((size_t)b * 3 * N + 3 * w) * 3;
   double r[3][3], e1, dr1[3][3], q3[3], gq[3][3][3];
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 3; ++k) r[a][k] = rr[3 * a + k];
-  ps_monomer<false>(c, r, e1, dr1, q3, gq, pes);
+  ps_monomer<false, MONO_W>(c, r, e1, dr1, q3, gq, pes, wv, WaveSplit<MONO_W>{lds_part});
+  if (wv != 0 || !valid) return;
   double *out = dRR + ((size_t)b * 3 * N + 3 * w) * 3;
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 3; ++k) out[3 * a + k] = dr1[a][k];
@@ -237,8 +249,8 @@ __global__ __launch_bounds__(64) void k_pes_fast(SysConst c, const double *__res
 void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
                      double *dRR, double *mol, bool record, hipStream_t s) {
   const int n = c.P * c.N;
-  hipLaunchKernelGGL(k_pes_fast, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, RRc, b.pes, dRR,
-                     mol, record ? 1 : 0);
+  hipLaunchKernelGGL(k_pes_fast, dim3((n + 63) / 64), dim3(64 * MONO_W), 0, s, c, RR, RRc, b.pes,
+                     dRR, mol, record ? 1 : 0);
 }
 
 // The centroid's gradient for every bead (MD.f90:400-403).
