@@ -597,6 +597,10 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
     }
   }
   x->timed("post_qq", [&] { launch_post_qq(c, d, s); });
+  // the convergence test of iteration it - 1 runs at the start of the
+  // dipole pair kernel of iteration it, where that kernel can hold it (and
+  // nothing runs beside it); the last iteration's test stays a launch
+  const bool fold = pair_dd_tests(c) && !c.overlap && !two && !c.poll && !x->prof;
   for (int it = 1; it <= c.maxiter; ++it) {
     if (c.overlap || two) {
       // pair_dd on the second stream, concurrent with the structure factor
@@ -604,11 +608,11 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
       x->timed("dd_sfac_d", [&] { err = dd_beside_sfac(x, s); });
       if (err != PIMD_OK) return err;
     } else {
-      x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s); });
+      x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s, fold && it > 1 ? it - 1 : 0); });
       x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
     }
     x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
-    x->timed("converge", [&] { launch_converge(c, d, it, s); });
+    if (!fold || it == c.maxiter) x->timed("converge", [&] { launch_converge(c, d, it, s); });
     // No host round trip: converged beads are switched off on the device
     // and every kernel of the loop skips them, so the remaining iterations
     // of a converged system are near-empty launches.

@@ -148,7 +148,11 @@ void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s);
 void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s);
 void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s);
 void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s);
+// test_iter > 0: the kernel first runs the convergence test of SOR iteration
+// test_iter (in place of launch_converge); only where pair_dd_tests(c).
+void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s,
+                    int test_iter = 0);
+bool pair_dd_tests(const SysConst &c);
 void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s);
 void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s);
 void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s);

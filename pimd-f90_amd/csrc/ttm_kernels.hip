@@ -305,53 +305,88 @@ void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
 // ---------------------------------------------------------------------------
 // Neighbour lists (find_neigh.f90:14-120): for every bead and molecule i, all
 // molecules j != i whose minimum-image O-O distance satisfies r^2 < Rc^2, in
-// ascending j.  One wavefront per (bead, i); ballot + mbcnt compaction.
+// ascending j.  One wavefront per (bead, i) at a time; ballot + mbcnt
+// compaction.
 // ---------------------------------------------------------------------------
 // Each list holds near partners from the front (ascending j) and far
 // partners from the back of its capacity: a far partner's O lies at least
 // 5 A + rad_i + rad_j away, so none of its site pairs is inside the 5 A Thole
 // radius (smear.f90:23,57,92) and the damping branch can be compiled out.
 // nbrn = number of near, nbrc = number of far partners.
+// A block takes NB_I molecules i of one bead, a wave at a time; with LDS the
+// bead's O positions and radii are staged there once per block.
+constexpr int NB_I = 16;
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__restrict__ site,
                                                   const double *__restrict__ radius, int *nbr,
                                                   int *nbrc, int *nbrn) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= c.P * c.N) return;
-  const int b = wave / c.N, i = wave % c.N, N = c.N;
-  const Site Oi = site[((size_t)b * N + i) * 4];
-  const double radi = radius[(size_t)b * N + i];
-  int *out = nbr + (size_t)wave * c.nbr_cap;
-  int nnear = 0, nfar = 0;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int j0 = 0; j0 < N; j0 += 64) {
-    const int j = j0 + lane;
-    bool in = false, near = false;
-    if (j < N && j != i) {
-      const Site Oj = site[((size_t)b * N + j) * 4];
-      double x = Oi.x - Oj.x, y = Oi.y - Oj.y, z = Oi.z - Oj.z;
-      minimg(c, x, y, z);
-      const double r2 = norm2_exact(x, y, z);
-      in = r2 < c.rc2;
-      const double lim = 5.0 + radi + radius[(size_t)b * N + j] + 1e-6;
-      near = r2 < lim * lim;
+  extern __shared__ double lds_o[];  // [N][4]: x, y, z, radius
+  const int b = blockIdx.y, N = c.N;
+  if (LDS) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      const Site o = site[((size_t)b * N + j) * 4];
+      lds_o[4 * j + 0] = o.x;
+      lds_o[4 * j + 1] = o.y;
+      lds_o[4 * j + 2] = o.z;
+      lds_o[4 * j + 3] = radius[(size_t)b * N + j];
     }
-    const unsigned long long mn = __ballot(in && near), mf = __ballot(in && !near);
-    if (in && near) out[nnear + __popcll(mn & below)] = j;
-    if (in && !near) out[c.nbr_cap - 1 - (nfar + __popcll(mf & below))] = j;
-    nnear += __popcll(mn);
-    nfar += __popcll(mf);
+    __syncthreads();
   }
-  if (lane == 0) {
-    nbrn[wave] = nnear;
-    nbrc[wave] = nfar;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int ii = wv; ii < NB_I; ii += blockDim.x >> 6) {
+    const int i = blockIdx.x * NB_I + ii;
+    if (i >= N) break;
+    double xi, yi, zi, radi;
+    if (LDS) {
+      xi = lds_o[4 * i]; yi = lds_o[4 * i + 1]; zi = lds_o[4 * i + 2]; radi = lds_o[4 * i + 3];
+    } else {
+      const Site o = site[((size_t)b * N + i) * 4];
+      xi = o.x; yi = o.y; zi = o.z; radi = radius[(size_t)b * N + i];
+    }
+    const size_t row = (size_t)b * N + i;
+    int *out = nbr + row * c.nbr_cap;
+    int nnear = 0, nfar = 0;
+    for (int j0 = 0; j0 < N; j0 += 64) {
+      const int j = j0 + lane;
+      bool in = false, near = false;
+      if (j < N && j != i) {
+        double xj, yj, zj, radj;
+        if (LDS) {
+          xj = lds_o[4 * j]; yj = lds_o[4 * j + 1]; zj = lds_o[4 * j + 2]; radj = lds_o[4 * j + 3];
+        } else {
+          const Site o = site[((size_t)b * N + j) * 4];
+          xj = o.x; yj = o.y; zj = o.z; radj = radius[(size_t)b * N + j];
+        }
+        double x = xi - xj, y = yi - yj, z = zi - zj;
+        minimg(c, x, y, z);
+        const double r2 = norm2_exact(x, y, z);
+        in = r2 < c.rc2;
+        const double lim = 5.0 + radi + radj + 1e-6;
+        near = r2 < lim * lim;
+      }
+      const unsigned long long mn = __ballot(in && near), mf = __ballot(in && !near);
+      if (in && near) out[nnear + __popcll(mn & below)] = j;
+      if (in && !near) out[c.nbr_cap - 1 - (nfar + __popcll(mf & below))] = j;
+      nnear += __popcll(mn);
+      nfar += __popcll(mf);
+    }
+    if (lane == 0) {
+      nbrn[row] = nnear;
+      nbrc[row] = nfar;
+    }
   }
 }
 
 void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
-  long threads = (long)c.P * c.N * 64;
-  hipLaunchKernelGGL(k_neighbors, dim3((threads + 255) / 256), dim3(256), 0, s, c, b.site,
-                     b.radius, b.nbr, b.nbrc, b.nbrn);
+  dim3 grid((c.N + NB_I - 1) / NB_I, c.P);
+  const size_t lds = sizeof(double) * 4 * c.N;
+  if (lds <= 64 * 1024)
+    hipLaunchKernelGGL(k_neighbors<true>, grid, dim3(256), lds, s, c, b.site, b.radius, b.nbr,
+                       b.nbrc, b.nbrn);
+  else
+    hipLaunchKernelGGL(k_neighbors<false>, grid, dim3(256), 0, s, c, b.site, b.radius, b.nbr,
+                       b.nbrc, b.nbrn);
 }
 
 // ---------------------------------------------------------------------------
@@ -1271,19 +1306,69 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
 // the neighbour loop reads LDS instead of gathering scattered records from
 // L2.  Arithmetic and summation order are those of k_pair_dd.
 constexpr int DD_BLOCK = 1024;
+// The SOR convergence test of iteration `iter` for bead b (pot_ttm.f90:306,
+// 380-386), as k_converge forms it: the first 256 threads of the block add
+// the bead's squared dipole changes in the same order.  All blocks of the
+// bead reach the same outcome; block `writer` records it.  Returns whether
+// the bead is still active; every thread of the block must call it.
+__device__ bool sor_test(const SysConst &c, const double *__restrict__ diff2, int *active,
+                         int *iters, int *nactive, int b, int iter, bool writer, double *red,
+                         int &act) {
+  const int tid = threadIdx.x;
+  if (tid == 0) act = active[b];  // one read per block: the writer block may change it
+  __syncthreads();
+  if (!act) return false;
+  const int M = c.N * c.nd;
+  if (tid < 256) {
+    double s = 0.0;
+    for (int m = tid; m < M; m += 256) s += diff2[(size_t)b * M + m];
+    red[tid] = s;
+  }
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) red[tid] += red[tid + off];
+    __syncthreads();
+  }
+  const double delta = c.stath * sqrt(red[0]);
+  const bool conv = delta < c.peps, last = iter >= c.maxiter;
+  if (writer && tid == 0) {
+    if (conv) {
+      active[b] = 0;
+      iters[b] = iter;
+    } else if (last) {
+      active[b] = 0;
+      iters[b] = -iter;
+    } else {
+      iters[b] = iter;
+      atomicAdd(&nactive[iter], 1);
+    }
+  }
+  return !(conv || last);
+}
+
 template <int MODEL, int G, typename R>
 __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site *__restrict__ site,
                                                          const double *__restrict__ dip,
                                                          const int *__restrict__ nbr,
                                                          const int *__restrict__ nbrc,
                                                          const int *__restrict__ nbrn,
-                                                         const int *__restrict__ active,
-                                                         double *__restrict__ Efd) {
+                                                         int *active, double *__restrict__ Efd,
+                                                         const double *__restrict__ diff2,
+                                                         int *iters, int *nactive, int test_iter) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
   extern __shared__ double lds_dd[];  // [N][ND][6]: x, y, z, mu_x, mu_y, mu_z
+  __shared__ double red[256];
+  __shared__ int act;
   const int b = blockIdx.y;
-  if (!active[b]) return;
+  if (test_iter > 0) {
+    // the convergence test of the previous iteration, in place of a
+    // k_converge launch
+    if (!sor_test(c, diff2, active, iters, nactive, b, test_iter, blockIdx.x == 0, red, act))
+      return;
+  } else if (!active[b]) {
+    return;
+  }
   const int N = c.N;
   for (int e = threadIdx.x; e < N * ND; e += blockDim.x) {
     const int j = e / ND, u = e % ND;
@@ -1361,26 +1446,30 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
 static size_t dd_lds(const SysConst &c) { return sizeof(double) * 6 * c.N * c.nd; }
 
 template <int G, typename R>
-static bool launch_dd_lds_g(const SysConst &c, DevBufs &b, hipStream_t s) {
+static bool launch_dd_lds_g(const SysConst &c, DevBufs &b, int test_iter, hipStream_t s) {
   if (dd_lds(c) > 64 * 1024) return false;
   dim3 grid((c.N + DD_BLOCK / G - 1) / (DD_BLOCK / G), c.P);
   if (c.model == 2)
     hipLaunchKernelGGL((k_pair_dd_lds<2, G, R>), grid, dim3(DD_BLOCK), dd_lds(c), s, c, b.site,
-                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd);
+                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd, b.diff2, b.iters,
+                       b.nactive, test_iter);
   else
     hipLaunchKernelGGL((k_pair_dd_lds<3, G, R>), grid, dim3(DD_BLOCK), dd_lds(c), s, c, b.site,
-                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd);
+                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd, b.diff2, b.iters,
+                       b.nactive, test_iter);
   return true;
 }
 
 template <typename R>
-static bool launch_dd_lds_r(const SysConst &c, DevBufs &b, hipStream_t s) {
+static bool launch_dd_lds_r(const SysConst &c, DevBufs &b, int test_iter, hipStream_t s) {
   const int g = c.g_dd;
-  if (g == 64) return launch_dd_lds_g<64, R>(c, b, s);
-  if (g == 32) return launch_dd_lds_g<32, R>(c, b, s);
-  if (g == 8) return launch_dd_lds_g<8, R>(c, b, s);
-  return launch_dd_lds_g<16, R>(c, b, s);
+  if (g == 64) return launch_dd_lds_g<64, R>(c, b, test_iter, s);
+  if (g == 32) return launch_dd_lds_g<32, R>(c, b, test_iter, s);
+  if (g == 8) return launch_dd_lds_g<8, R>(c, b, test_iter, s);
+  return launch_dd_lds_g<16, R>(c, b, test_iter, s);
 }
+
+bool pair_dd_tests(const SysConst &c) { return c.g_dd > 0 && dd_lds(c) <= 64 * 1024; }
 
 template <int G, typename R>
 static void launch_dd_g(const SysConst &c, DevBufs &b, hipStream_t s) {
@@ -1405,9 +1494,11 @@ static void launch_dd_r(const SysConst &c, DevBufs &b, int group, hipStream_t s)
 }
 
 // fp64 forces the FP64 form (the refinement pass of the mixed solve).
-void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s) {
+void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s,
+                    int test_iter) {
   const bool f32 = c.mixed && !fp64;
-  if (c.g_dd > 0 && (f32 ? launch_dd_lds_r<float>(c, b, s) : launch_dd_lds_r<double>(c, b, s)))
+  if (c.g_dd > 0 && (f32 ? launch_dd_lds_r<float>(c, b, test_iter, s)
+                         : launch_dd_lds_r<double>(c, b, test_iter, s)))
     return;
   if (f32) launch_dd_r<float>(c, b, group, s);
   else launch_dd_r<double>(c, b, group, s);
