@@ -527,7 +527,7 @@ def main():
     ap.add_argument("--precision", default="both", choices=("fp64", "mixed", "both"),
                     help="form(s) to time; with both, value is the mixed form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--batch", type=int, default=2,
+    ap.add_argument("--batch", type=int, default=3,
                     help="replicas of the workload in the batched measurement (1 skips it)")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=2000,

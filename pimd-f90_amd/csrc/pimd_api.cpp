@@ -54,9 +54,9 @@ struct pimd_ctx {
   // contracted MD: the centroid's intermolecular forces (one bead, no
   // monomer PES), computed on this context's stream
   pimd_ctx *cen = nullptr;
-  // second stream: the real-space dipole field of each SOR iteration runs on
-  // it beside the reciprocal-space structure factor (both read the same
-  // dipoles); fork/join through the two events
+  // second stream (made when first needed): real-space pair kernels run on
+  // it beside the reciprocal-space kernels they do not depend on (SOR
+  // overlap, graph branches); fork/join through the two events
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int group = 64;
@@ -524,9 +524,19 @@ int pick_group(const SysConst &c) {
   return 16;
 }
 
+// The second stream, made on first use (outside any capture: by the caller
+// before a capture begins).
+int ensure_stream2(pimd_ctx *x) {
+  if (x->stream2) return PIMD_OK;
+  if (hipStreamCreateWithFlags(&x->stream2, hipStreamNonBlocking) != hipSuccess)
+    return fail(PIMD_EDEVICE, "hipStreamCreate failed");
+  return PIMD_OK;
+}
+
 // a(stream2) beside b(s); s waits for both.
 template <class A, class B>
 int beside(pimd_ctx *x, hipStream_t s, A &&a, B &&b) {
+  if (!x->stream2) return fail(PIMD_ESTATE, "second stream missing");
   HIPCHK(hipEventRecord(x->ev_fork, s));
   HIPCHK(hipStreamWaitEvent(x->stream2, x->ev_fork, 0));
   a(x->stream2);
@@ -558,6 +568,13 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
   const bool two = x->branches && !x->prof;
+  if (c.overlap && !x->stream2) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return fail(PIMD_ESTATE, "second stream missing");
+    const int err = ensure_stream2(x);
+    if (err != PIMD_OK) return err;
+  }
   // the reciprocal-space charge pass, beside the real-space pair pass when
   // two streams are in use (it writes kE / kphi, read after the join)
   auto kspace_q = [&](hipStream_t st) {
@@ -703,8 +720,9 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
   x->p = p;
   x->natoms = 3 * p.nwaters;
   int rc = setup_system(x);
-  if (rc == PIMD_OK && (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess ||
-                        hipStreamCreateWithFlags(&x->stream2, hipStreamNonBlocking) != hipSuccess))
+  // one stream per context (the second is made when first needed): each
+  // context of a batch then gets a hardware queue of its own
+  if (rc == PIMD_OK && hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(PIMD_EDEVICE, "hipStreamCreate failed");
   if (rc == PIMD_OK && (hipEventCreateWithFlags(&x->ev_fork, hipEventDisableTiming) != hipSuccess ||
                         hipEventCreateWithFlags(&x->ev_join, hipEventDisableTiming) != hipSuccess))
@@ -1218,8 +1236,8 @@ static int md_step_eager(pimd_ctx *x, const double *noise, uint64_t seed) {
 
 // A step with device normals can run as one replay of a captured graph of
 // md_step_eager, rebuilt when the launch arguments it bakes in change (seed,
-// launch shapes, parameters, buffers).  Host noise, polling, per-kernel
-// timing and the two-stream overlap keep the eager launches.
+// launch shapes, parameters, buffers).  Host noise, polling and per-kernel
+// timing keep the eager launches.
 static bool graph_eligible(const pimd_ctx *x, const double *noise) {
   const int on = env_int("PIMD_MD_GRAPH", 1, 0, 1);  // read per step: a test switches it
   const SysConst &c = x->c;
@@ -1237,6 +1255,11 @@ static int md_graph_build(pimd_ctx *x, uint64_t seed) {
   // Off by default: on MI355X it gained 4 % on one FP64 system, nothing in
   // the mixed form, and slowed batches of replicas by a third.
   const bool br = env_int("PIMD_GRAPH_BRANCHES", 0, 0, 1) != 0;
+  for (pimd_ctx *y : {x, x->cen}) {
+    if (!y || !(br || y->c.overlap)) continue;
+    const int err = ensure_stream2(y);
+    if (err != PIMD_OK) return err;
+  }
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   x->branches = br;
   if (x->cen) x->cen->branches = br;
