@@ -254,8 +254,8 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
     }
   };
   // near and far partners alike: there is no damping radius here
-  for (int n = lane; n < nn; n += G) visit(list[n]);
-  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n]);
+  walk_list<G>(list, lane, nn, visit);
+  walk_list<G>(list, c.nbr_cap - nf + lane, c.nbr_cap, visit);
   for (int t = 1; t < 4; ++t) ph[t] = group_sum<G>(ph[t]);
   for (int t = 1; t < 4; ++t)
     for (int k = 0; k < 3; ++k) ef[t][k] = group_sum<G>(ef[t][k]);

@@ -75,6 +75,20 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+// Visits list[first], list[first + G], ... below `end` in order, loading
+// each next index while the current one is visited (the list is in global
+// memory; its load latency would otherwise stall every step).
+template <int G, class F>
+__device__ __forceinline__ void walk_list(const int *__restrict__ list, int first, int end, F &&f) {
+  if (first >= end) return;
+  int j = list[first];
+  for (int n = first; n < end; n += G) {
+    const int jn = n + G < end ? list[n + G] : 0;
+    f(j);
+    j = jn;
+  }
+}
+
 __device__ __forceinline__ void vadd9(double *t, const double *v) {
   for (int i = 0; i < 9; ++i) t[i] += v[i];
 }

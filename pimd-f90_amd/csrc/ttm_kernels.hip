@@ -499,8 +499,8 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
       }
     }
   };
-  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
-  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
+  walk_list<G>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
+  walk_list<G>(list, c.nbr_cap - nf + lane, c.nbr_cap, [&](int j) { visit(j, std::false_type{}); });
   for (int t = 1; t < 4; ++t) ph[t] = group_sum<G>(ph[t]);
   for (int t = SA0; t < 4; ++t)
     for (int k = 0; k < 3; ++k) ef[t][k] = group_sum<G>(ef[t][k]);
@@ -1272,8 +1272,8 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
       }
     }
   };
-  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
-  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
+  walk_list<G>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
+  walk_list<G>(list, c.nbr_cap - nf + lane, c.nbr_cap, [&](int j) { visit(j, std::false_type{}); });
   for (int u = 0; u < ND; ++u)
     for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
   if (lane == 0) {
@@ -1306,6 +1306,9 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
 // the neighbour loop reads LDS instead of gathering scattered records from
 // L2.  Arithmetic and summation order are those of k_pair_dd.
 constexpr int DD_BLOCK = 1024;
+// doubles per staged dipole site: 6 used, one of padding so that lanes
+// gathering different sites spread over the LDS banks
+constexpr int DD_REC = 7;
 // The SOR convergence test of iteration `iter` for bead b (pot_ttm.f90:306,
 // 380-386), as k_converge forms it: the first 256 threads of the block add
 // the bead's squared dipole changes in the same order.  All blocks of the
@@ -1357,7 +1360,7 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
                                                          int *iters, int *nactive, int test_iter) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
-  extern __shared__ double lds_dd[];  // [N][ND][6]: x, y, z, mu_x, mu_y, mu_z
+  extern __shared__ double lds_dd[];  // [N][ND][DD_REC]: x, y, z, mu_x, mu_y, mu_z, pad
   __shared__ double red[256];
   __shared__ int act;
   const int b = blockIdx.y;
@@ -1374,7 +1377,7 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
     const int j = e / ND, u = e % ND;
     const Site sj = site[((size_t)b * N + j) * 4 + T0 + u];
     const double *mu = dip + (((size_t)b * N + j) * ND + u) * 3;
-    double *o = lds_dd + (size_t)e * 6;
+    double *o = lds_dd + (size_t)e * DD_REC;
     o[0] = sj.x;
     o[1] = sj.y;
     o[2] = sj.z;
@@ -1389,7 +1392,7 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
   const long grp = (long)b * N + a;
   double A[ND][3];
   for (int u = 0; u < ND; ++u)
-    for (int k = 0; k < 3; ++k) A[u][k] = lds_dd[((size_t)a * ND + u) * 6 + k];
+    for (int k = 0; k < 3; ++k) A[u][k] = lds_dd[((size_t)a * ND + u) * DD_REC + k];
   double ef[ND][3];
   for (int u = 0; u < ND; ++u) ef[u][0] = ef[u][1] = ef[u][2] = 0.0;
   const int nf = nbrc[grp], nn = nbrn[grp];
@@ -1402,7 +1405,7 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
 #pragma unroll
       for (int ub = 0; ub < ND; ++ub) {
         const int sa = T0 + ua, sb = T0 + ub;
-        const double *B = lds_dd + ((size_t)j * ND + ub) * 6;
+        const double *B = lds_dd + ((size_t)j * ND + ub) * DD_REC;
         double v[3] = {A[ua][0] - B[0], A[ua][1] - B[1], A[ua][2] - B[2]};
         minimg(c, v[0], v[1], v[2]);
         const double d2 = norm2_exact(v[0], v[1], v[2]);
@@ -1416,8 +1419,8 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
       }
     }
   };
-  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
-  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
+  walk_list<G>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
+  walk_list<G>(list, c.nbr_cap - nf + lane, c.nbr_cap, [&](int j) { visit(j, std::false_type{}); });
   for (int u = 0; u < ND; ++u)
     for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
   if (lane == 0) {
@@ -1430,8 +1433,8 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
           const double d2 = norm2_exact(r[0], r[1], r[2]);
           double ts1, ts2, o[3];
           smear2(c, d2, c.apolDD[sa][sb], ts1, ts2);
-          const double *mu_v = lds_dd + ((size_t)a * ND + v2) * 6 + 3;
-          const double *mu_u = lds_dd + ((size_t)a * ND + u) * 6 + 3;
+          const double *mu_v = lds_dd + ((size_t)a * ND + v2) * DD_REC + 3;
+          const double *mu_u = lds_dd + ((size_t)a * ND + u) * DD_REC + 3;
           dd3_apply(r, ts1, ts2, mu_v, o);
           for (int k = 0; k < 3; ++k) ef[u][k] += o[k];
           dd3_apply(r, ts1, ts2, mu_u, o);
@@ -1443,7 +1446,7 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
   }
 }
 
-static size_t dd_lds(const SysConst &c) { return sizeof(double) * 6 * c.N * c.nd; }
+static size_t dd_lds(const SysConst &c) { return sizeof(double) * DD_REC * c.N * c.nd; }
 
 template <int G, typename R>
 static bool launch_dd_lds_g(const SysConst &c, DevBufs &b, int test_iter, hipStream_t s) {
@@ -1694,8 +1697,8 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
       }
     }
   };
-  for (int n = lane; n < nn; n += G) visit(list[n], std::true_type{});
-  for (int n = c.nbr_cap - nf + lane; n < c.nbr_cap; n += G) visit(list[n], std::false_type{});
+  walk_list<G>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
+  walk_list<G>(list, c.nbr_cap - nf + lane, c.nbr_cap, [&](int j) { visit(j, std::false_type{}); });
   for (int t = 0; t < 4; ++t) {
     ph[t] = group_sum<G>(ph[t]);
     for (int k = 0; k < 3; ++k) fr[t][k] = group_sum<G>(fr[t][k]);
