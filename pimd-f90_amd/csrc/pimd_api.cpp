@@ -75,6 +75,11 @@ struct pimd_ctx {
   bool graph_ok = false;
   bool graph_off = false;  // capture failed once: eager launches from then on
   uint64_t graph_seed = 0;
+  // the force call of the all-beads entry points, captured for its buffer
+  // addresses fkey and replayed while they and the model stay the same
+  hipGraphExec_t fgraph = nullptr;
+  bool fgraph_ok = false, fgraph_off = false;
+  const void *fkey[10] = {};
   double omegan = 0.0;    // ring frequency kB T P / hbar of the MD temperature
   double *d_vxi = nullptr, *d_s = nullptr, *d_step = nullptr;
   double *d_rec = nullptr;  // per-step records of md runs
@@ -144,6 +149,13 @@ struct pimd_ctx {
 
 namespace {
 
+// Captured launches hold launch shapes, model constants and buffer addresses:
+// after a change to any of them both graphs are captured again.
+void invalidate_graphs(pimd_ctx *x) {
+  x->graph_ok = false;
+  x->fgraph_ok = false;
+}
+
 template <class T>
 int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
@@ -176,6 +188,7 @@ void free_all(pimd_ctx *x) {
   }
   for (auto e : x->pool) (void)hipEventDestroy(e);
   if (x->graph) (void)hipGraphExecDestroy(x->graph);
+  if (x->fgraph) (void)hipGraphExecDestroy(x->fgraph);
   if (x->stream && !x->stream_borrowed) (void)hipStreamDestroy(x->stream);
   if (x->stream2) (void)hipStreamDestroy(x->stream2);
   if (x->d_rec) (void)hipFree(x->d_rec);
@@ -673,6 +686,48 @@ int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
   return PIMD_OK;
 }
 
+
+// run_forces as one replay of a captured graph, captured again when the
+// buffer addresses change (or after invalidate_graphs).  Per-kernel timing,
+// polling and the two-stream overlap keep eager launches, as does a device
+// where capture fails.  PIMD_FORCE_GRAPH=0 turns replay off.
+int force_call(pimd_ctx *x, const double *RR, const double *RRc, double *Upot, double *dRR,
+               double *virt, double *virialc, double *dipI, double *dipE, double *chg,
+               int *iters) {
+  const SysConst &c = x->c;
+  const bool on = env_int("PIMD_FORCE_GRAPH", 1, 0, 1) != 0;
+  if (!on || x->fgraph_off || x->prof || c.poll || c.overlap)
+    return run_forces(x, RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters);
+  const void *key[10] = {RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters};
+  if (!x->fgraph_ok || !std::equal(key, key + 10, x->fkey)) {
+    if (x->fgraph) (void)hipGraphExecDestroy(x->fgraph);
+    x->fgraph = nullptr;
+    x->fgraph_ok = false;
+    hipStream_t s = x->stream;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = run_forces(x, RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != PIMD_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    hipError_t e2 = e;
+    if (e == hipSuccess) e2 = hipGraphInstantiate(&x->fgraph, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      // capture is not available here: launch eagerly from now on
+      x->fgraph = nullptr;
+      x->fgraph_off = true;
+      (void)hipGetLastError();
+      return run_forces(x, RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters);
+    }
+    std::copy(key, key + 10, x->fkey);
+    x->fgraph_ok = true;
+  }
+  HIPCHK(hipGraphLaunch(x->fgraph, x->stream));
+  return PIMD_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -771,7 +826,7 @@ int pimd_forces_all_beads_dev(pimd_ctx *x, const double *RR, const double *RRc, 
   HIPCHK(hipSetDevice(x->p.device));
   DevBufs &d = x->d;
   launch_set_clock(d.clk, t, 0, x->stream);
-  return run_forces(x, RR, RRc, Upot ? Upot : d.Upot, dRR ? dRR : d.dRR,
+  return force_call(x, RR, RRc, Upot ? Upot : d.Upot, dRR ? dRR : d.dRR,
                     virt ? virt : d.virt, virialc ? virialc : d.virialc, dipI ? dipI : d.dipI,
                     dipE ? dipE : d.dipE, chg ? chg : d.chg, iters ? iters : d.iters_out);
 }
@@ -787,7 +842,7 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   HIPCHK(hipMemcpyAsync(d.RR, RRt, sizeof(double) * P * na * 3, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.RRc, RRc, sizeof(double) * na * 3, hipMemcpyHostToDevice, s));
   launch_set_clock(d.clk, t, 0, s);
-  int rc = run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
+  int rc = force_call(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
                       d.iters_out);
   if (rc != PIMD_OK) return rc;
   if (Upott) HIPCHK(hipMemcpyAsync(Upott, d.Upot, sizeof(double) * P, hipMemcpyDeviceToHost, s));
@@ -963,7 +1018,7 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
   if (md->intra_timesteps > 0 && x->p.contraction)
     return fail(PIMD_EINVAL, "contracted MD needs a context created with contraction = 0");
   HIPCHK(hipSetDevice(x->p.device));
-  x->graph_ok = false;
+  invalidate_graphs(x);
   x->md = *md;
   const int P = x->c.P;
   const double delt = md->delt_fs / 1000.0, delt2 = delt / 2.0;
@@ -1065,7 +1120,7 @@ static int md_records(pimd_ctx *x, int nsteps) {
   if (x->d_rec) (void)hipFree(x->d_rec);
   x->d_rec = nullptr;
   x->rec_cap = 0;
-  x->graph_ok = false;  // captured steps hold the record buffer's address
+  invalidate_graphs(x);  // captured steps hold the record buffer's address
   int rc = dalloc(&x->d_rec, (size_t)nsteps * MD_REC);
   if (rc == PIMD_OK) x->rec_cap = nsteps;
   return rc;
@@ -1367,7 +1422,7 @@ static Shape get_shape(const pimd_ctx *x) {
   return Shape{x->c.g_qq, x->c.g_dd, x->c.g_fin, x->c.fd_cg, x->c.sf_sites};
 }
 static void set_shape(pimd_ctx *x, const Shape &sh) {
-  x->graph_ok = false;
+  invalidate_graphs(x);
   x->c.g_qq = sh.g_qq;
   x->c.g_dd = sh.g_dd;
   x->c.g_fin = sh.g_fin;
@@ -1404,7 +1459,7 @@ static int time_forces(pimd_ctx *x, double *scratch, hipEvent_t e0, hipEvent_t e
 // restored around the trials.
 static int autotune_ctx(pimd_ctx *x, double *best_ms) {
   HIPCHK(hipSetDevice(x->p.device));
-  x->graph_ok = false;  // the shapes of a contracted context's centroid change below
+  invalidate_graphs(x);  // the shapes of a contracted context's centroid change below
   if (x->cen) {
     // contracted MD: the force call of a step is the centroid's
     const size_t nc = (size_t)x->natoms * 3;
@@ -1594,7 +1649,7 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
     std::copy(theta, theta + TTM_NPARAM, y->ttm);
     apply_ttm_params(y);
   }
-  x->graph_ok = false;  // captured launches hold the model constants
+  invalidate_graphs(x);  // captured launches hold the model constants
   return PIMD_OK;
 }
 

@@ -87,3 +87,40 @@ def test_batch_graph_matches_eager():
         assert np.array_equal(a1[k], b1[k]) and np.array_equal(a2[k], b2[k]), k
     for x, y in zip(sa, sb):
         assert np.array_equal(x, y)
+
+
+def test_force_graph_matches_eager():
+    """The all-beads force call replayed from a captured graph (the default)
+    against eager launches (PIMD_FORCE_GRAPH=0), bit for bit: calls at several
+    MD steps on moving coordinates (the dipole predictor reads the step and
+    its history), then after a change of model parameters, for TTM3-F,
+    TTM2.1-F and q-SPC/Fw."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import TTMForceField
+    RRt, PPt, RRc, L = make_system(64, 8, seed=5)
+    res = {}
+    for graph in ("0", "1"):
+        os.environ["PIMD_FORCE_GRAPH"] = graph
+        try:
+            for model in (3, 2, 4):
+                print("case force graph", graph, "model", model, flush=True)
+                p = TTMParams(pot_model=model, nwaters=64, nbeads=8, box=(L, L, L), rc=-1.0,
+                              eps_ewald=1e-6, guess_initdip=True)
+                ff = TTMForceField(p)
+                out = []
+                for t in range(1, 8):
+                    R = RRt + 0.002 * t
+                    out.append(ff.full_bead_forces(R, R.mean(axis=0), t=t))
+                th = ff.ttm_params()
+                th[2] *= 1.01
+                ff.set_ttm_params(th)
+                out.append(ff.full_bead_forces(RRt, RRc, t=8))
+                ff.close()
+                res[(graph, model)] = out
+        finally:
+            os.environ.pop("PIMD_FORCE_GRAPH", None)
+    for model in (3, 2, 4):
+        for ra, rb in zip(res[("0", model)], res[("1", model)]):
+            for k in ("Upott", "dRRt", "virt_cm", "virialct", "dip_momIt", "chg", "iters"):
+                assert np.array_equal(ra[k], rb[k]), (model, k)
