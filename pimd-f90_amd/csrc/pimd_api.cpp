@@ -86,6 +86,9 @@ struct pimd_ctx {
   int rec_cap = 0;
   int rec_steps = 0;        // steps recorded by the last run
   int *h_nactive = nullptr;
+  // pinned staging of the host-array force call: coordinates in, all outputs
+  // out (made on the first call)
+  double *h_stage = nullptr;
   // profiling
   bool prof = false;
   // inside a graph capture: independent real- and reciprocal-space kernels
@@ -182,6 +185,7 @@ void free_all(pimd_ctx *x) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (x->h_nactive) (void)hipHostFree(x->h_nactive);
+  if (x->h_stage) (void)hipHostFree(x->h_stage);
   for (auto &q : x->pending) {
     (void)hipEventDestroy(q.a);
     (void)hipEventDestroy(q.b);
@@ -839,21 +843,40 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   const size_t P = x->c.P, na = x->natoms, N = x->c.N;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
-  HIPCHK(hipMemcpyAsync(d.RR, RRt, sizeof(double) * P * na * 3, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(d.RRc, RRc, sizeof(double) * na * 3, hipMemcpyHostToDevice, s));
+  // the caller's arrays go through a pinned staging block: DMA copies at
+  // full link speed instead of the runtime's pageable path
+  const size_t nR = P * na * 3, nc = na * 3;
+  const size_t o_U = nR + nc, o_dR = o_U + P, o_vt = o_dR + nR, o_vc = o_vt + P * 9,
+               o_dI = o_vc + P, o_dE = o_dI + P * N * 3, o_ch = o_dE + P * N * 3,
+               o_it = o_ch + P * na, n_stage = o_it + P;
+  if (!x->h_stage)
+    HIPCHK(hipHostMalloc((void **)&x->h_stage, sizeof(double) * n_stage, hipHostMallocDefault));
+  double *h = x->h_stage;
+  std::memcpy(h, RRt, sizeof(double) * nR);
+  std::memcpy(h + nR, RRc, sizeof(double) * nc);
+  HIPCHK(hipMemcpyAsync(d.RR, h, sizeof(double) * nR, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.RRc, h + nR, sizeof(double) * nc, hipMemcpyHostToDevice, s));
   launch_set_clock(d.clk, t, 0, s);
   int rc = force_call(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
                       d.iters_out);
   if (rc != PIMD_OK) return rc;
-  if (Upott) HIPCHK(hipMemcpyAsync(Upott, d.Upot, sizeof(double) * P, hipMemcpyDeviceToHost, s));
-  if (dRRt) HIPCHK(hipMemcpyAsync(dRRt, d.dRR, sizeof(double) * P * na * 3, hipMemcpyDeviceToHost, s));
-  if (virt) HIPCHK(hipMemcpyAsync(virt, d.virt, sizeof(double) * P * 9, hipMemcpyDeviceToHost, s));
-  if (virialct) HIPCHK(hipMemcpyAsync(virialct, d.virialc, sizeof(double) * P, hipMemcpyDeviceToHost, s));
-  if (dipI) HIPCHK(hipMemcpyAsync(dipI, d.dipI, sizeof(double) * P * N * 3, hipMemcpyDeviceToHost, s));
-  if (dipE) HIPCHK(hipMemcpyAsync(dipE, d.dipE, sizeof(double) * P * N * 3, hipMemcpyDeviceToHost, s));
-  if (chg) HIPCHK(hipMemcpyAsync(chg, d.chg, sizeof(double) * P * na, hipMemcpyDeviceToHost, s));
-  if (iters) HIPCHK(hipMemcpyAsync(iters, d.iters_out, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  struct Out {
+    void *dst;
+    const void *src;
+    size_t off, bytes;
+  } outs[] = {{Upott, d.Upot, o_U, sizeof(double) * P},
+              {dRRt, d.dRR, o_dR, sizeof(double) * nR},
+              {virt, d.virt, o_vt, sizeof(double) * P * 9},
+              {virialct, d.virialc, o_vc, sizeof(double) * P},
+              {dipI, d.dipI, o_dI, sizeof(double) * P * N * 3},
+              {dipE, d.dipE, o_dE, sizeof(double) * P * N * 3},
+              {chg, d.chg, o_ch, sizeof(double) * P * na},
+              {iters, d.iters_out, o_it, sizeof(int) * P}};
+  for (const Out &o : outs)
+    if (o.dst) HIPCHK(hipMemcpyAsync(h + o.off, o.src, o.bytes, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  for (const Out &o : outs)
+    if (o.dst) std::memcpy(o.dst, h + o.off, o.bytes);
   return PIMD_OK;
 }
 
