@@ -1195,8 +1195,11 @@ static int md_step_vv(pimd_ctx *x, const double *noise, uint64_t seed) {
   const int L = 3 * na * P;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
-  auto thermostat = [&](int half) {
-    if (md.bead_thermostat != 1) return;
+  // the half kicks ride on the kernel that next reads the momenta: the free
+  // ring-polymer step and the second Langevin half step
+  const bool lang = md.bead_thermostat == 1;
+  auto thermostat = [&](int half, const double *kick) {
+    if (!lang) return;
     const double *nz = nullptr;
     if (noise) {
       (void)hipMemcpyAsync(d.noise, noise + (size_t)half * n, sizeof(double) * n,
@@ -1204,27 +1207,31 @@ static int md_step_vv(pimd_ctx *x, const double *noise, uint64_t seed) {
       nz = d.noise;
     }
     x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half);
+      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half, kick,
+                      delt2);
     });
   };
-  thermostat(0);
+  thermostat(0, nullptr);
   if (md.global_nh) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
     x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
-  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-  if (P > 1)
-    x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s); });
-  else
+  if (P > 1) {
+    x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s, d.dRR, delt2); });
+  } else {
+    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
     x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, delt, s); });
+  }
   x->timed("centroid_pbc", [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
   int rc = run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
                       d.iters_out);
   if (rc != PIMD_OK) return rc;
   md_kinetic(x);
   md_observables(x, d.dipI, d.Upot, d.virt, d.virialc);  // also the step record
-  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-  thermostat(1);
+  if (lang)
+    thermostat(1, d.dRR);
+  else
+    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
   md_kinetic(x);
   if (md.global_nh) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });

@@ -220,10 +220,13 @@ void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStrea
 void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
                             const double *vtcen, const double *vccen, const double *uk,
                             double *rec0, const int *clk, hipStream_t s);
-void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s);
+// kick (optional): PP first takes the half kick PP - MASSCON*kick*delt2.
+void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
+                      const double *kick = nullptr, double delt2 = 0.0);
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
-                     hipStream_t s, const int *clk = nullptr, int half = 0);
+                     hipStream_t s, const int *clk = nullptr, int half = 0,
+                     const double *kick = nullptr, double delt2 = 0.0);
 void launch_kick(int n, double *PP, const double *dRR, double f, hipStream_t s);
 void launch_scale(int n, double *PP, const double *sc, hipStream_t s);
 void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,

@@ -26,7 +26,8 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
                                                   const double *__restrict__ A,
                                                   const double *__restrict__ msf,
                                                   const double *__restrict__ masses, double *RR,
-                                                  double *PP) {
+                                                  double *PP, const double *__restrict__ kick,
+                                                  double delt2) {
   extern __shared__ double lds[];
   const int CB = nm_cols(P);
   double *Cs = lds, *xr = Cs + P * P, *xp = xr + P * CB, *yr = xp + P * CB, *yp = yr + P * CB;
@@ -36,7 +37,11 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
   for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
     const int k = e / CB, cc = e % CB, col = col0 + cc;
     xr[e] = col < ncols ? RR[k * stride + col] : 0.0;
-    xp[e] = col < ncols ? PP[k * stride + col] : 0.0;
+    double p = col < ncols ? PP[k * stride + col] : 0.0;
+    // with kick: the half kick PPt - MASSCON*dRRt*delt2 before the step
+    // (MD.f90:49), as k_kick forms it
+    if (kick && col < ncols) p = p - MASSCON * kick[k * stride + col] * delt2;
+    xp[e] = p;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
@@ -65,11 +70,12 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
   }
 }
 
-void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s) {
+void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
+                      const double *kick, double delt2) {
   const int CB = nm_cols(P), n = natoms * 3;
   const size_t lds = sizeof(double) * ((size_t)P * P + 4 * (size_t)P * CB);
   hipLaunchKernelGGL(k_nm_evolve, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
-                     b.A, b.msf, b.masses, RR, PP);
+                     b.A, b.msf, b.masses, RR, PP, kick, delt2);
 }
 
 // Philox4x32-10 counter-based generator (J. K. Salmon et al., "Parallel random
@@ -125,7 +131,8 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
                                                  double massH, double *PP,
                                                  const double *__restrict__ noise, uint64_t seed,
                                                  uint64_t counter, const int *__restrict__ clk,
-                                                 int half) {
+                                                 int half, const double *__restrict__ kick,
+                                                 double delt2) {
   extern __shared__ double lds[];
   // in MD steps the counter follows the device step clock: step t, half 0 / 1
   if (clk) counter = (uint64_t)clk[0] * 2 + half;
@@ -136,7 +143,10 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
   for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
   for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
     const int k = e / CB, cc = e % CB, col = col0 + cc;
-    xp[e] = col < ncols ? PP[k * stride + col] : 0.0;
+    double p = col < ncols ? PP[k * stride + col] : 0.0;
+    // with kick: the half kick PPt - MASSCON*dRRt*delt2 first (MD.f90:94)
+    if (kick && col < ncols) p = p - MASSCON * kick[k * stride + col] * delt2;
+    xp[e] = p;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
@@ -172,11 +182,12 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
 
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
-                     hipStream_t s, const int *clk, int half) {
+                     hipStream_t s, const int *clk, int half, const double *kick, double delt2) {
   const int CB = nm_cols(P), n = natoms * 3;
   const size_t lds = sizeof(double) * ((size_t)P * P + 2 * (size_t)P * CB);
   hipLaunchKernelGGL(k_langevin, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
-                     b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter, clk, half);
+                     b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter, clk, half, kick,
+                     delt2);
 }
 
 // The device step clock {MD step t, record slot} read by the force and
