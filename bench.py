@@ -216,6 +216,30 @@ def run_batched(p, nrep, args, seed=2024):
             "value": round(nrep * p.nwaters * p.nbeads / s, 1)}
 
 
+def time_md(p, args, delt_fs=0.25, **md):
+    """Whole PIMD steps of a variant of the workload (another model, or
+    contracted MD with md['intra_timesteps']), from the same synthetic system:
+    ms per step, molecule-bead force evaluations (or, for contracted MD,
+    molecule-bead steps) per second, ns/day at delt_fs."""
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PIMDIntegrator, TTMForceField, load_library
+    RRt, PPt, RRc, L = make_system(p.nwaters, p.nbeads, seed=2024)
+    ff = TTMForceField(p)
+    integ = PIMDIntegrator(ff, delt_fs=delt_fs, temp=300.0, bead_thermostat=1,
+                           centroid_thermostat=True, tau_centroid=0.1, **md)
+    integ.set_state(RRt, PPt, ff.full_bead_forces(RRt, RRc)["dRRt"])
+    integ.run(args.warmup, t0=1, seed=99)
+    lib = load_library()
+    lib.pimd_sync(ff.handle)
+    t0 = time.perf_counter()
+    integ.run(args.steps, t0=1 + args.warmup, seed=99)
+    lib.pimd_sync(ff.handle)
+    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    ff.close()
+    return {"ms_per_step": round(ms, 4), "value": round(p.nwaters * p.nbeads / (ms / 1000.0), 1),
+            "ns_per_day": round((1000.0 / ms) * delt_fs * 1e-6 * 86400.0, 3), "dt_fs": delt_fs}
+
+
 def force_only(ff, integ, p, steps):
     """The all-beads force call alone (full_bead_forces equivalent) on the
     device-resident coordinates of the last MD step, dipole predictor off
@@ -601,6 +625,19 @@ def main():
             bat["mixed_over_plain_fp64"] = round(bat["mixed"]["value"] / a["value"], 3)
             bat["fp64_over_plain_fp64"] = round(bat["fp64"]["value"] / a["value"], 3)
             out["batched"] = bat
+        # the other built rows on the same system: contracted MD (the
+        # reference's default intra_timesteps 10, inner step 0.25 fs) and the
+        # fixed-charge q-SPC/Fw model
+        extra = {"contracted_md": {"intra_timesteps": 10, "outer_dt_fs": 2.5},
+                 "qspcfw": {"model": "q-SPC/Fw"}}
+        for f in forms:
+            q = TTMParams(**{**p.__dict__, "precision": 1 if f == "mixed" else 0})
+            extra["contracted_md"][f] = time_md(q, args, delt_fs=2.5, intra_timesteps=10)
+            qs = TTMParams(**{**q.__dict__, "pot_model": 4, "guess_initdip": False})
+            extra["qspcfw"][f] = time_md(qs, args)
+        extra["contracted_md"]["note"] = ("value = molecule-bead steps/s (one centroid TTM call and "
+                                          "10 monomer passes per step)")
+        out["other_rows"] = extra
         out["mixed_error_vs_fp64"] = {
             "force_rms_rel": float(np.sqrt(np.mean(df ** 2)) / np.sqrt(np.mean(a["f0"]["dRRt"] ** 2))),
             "force_max_rel": float(np.max(np.abs(df)) / np.max(np.abs(a["f0"]["dRRt"]))),
