@@ -495,15 +495,27 @@ int alloc_system(pimd_ctx *x) {
   // site chunks of at most 64 sites keep the staged tables within LDS
   c.sf_chunks_max = std::max(32, (3 * c.N + 63) / 64);
   // launch-shape tuning (results do not depend on them)
-  c.sf_sites = env_int("PIMD_SFAC_SITES", 32, 1, 64);
+  // default launch shapes by size: a system of few molecule-beads (a
+  // contracted run's one-bead centroid, small replicas) gives each molecule
+  // more lanes and the structure factor more site chunks, so that its
+  // kernels still spread over the device
+  const long mb = (long)c.P * c.N;
+  auto pow2_clamp = [](long v, int lo, int hi) {
+    int g = lo;
+    while (g < hi && g < v) g <<= 1;
+    return g;
+  };
+  const int dflt_qq = pow2_clamp(131072 / std::max(mb, 1L), 8, 32);
+  const int dflt_dd = pow2_clamp(262144 / std::max(mb, 1L), 16, 64);
+  c.sf_sites = env_int("PIMD_SFAC_SITES", mb >= 8192 ? 32 : 8, 1, 64);
   c.fd_cg = env_int("PIMD_FIELD_CG", 0, 0, 16);
   c.sf_kind = env_int("PIMD_SFAC_KIND", 1, 0, 1);
   c.fd_klds = env_int("PIMD_FIELD_KLDS", 1, 0, 1);
   // phase tables of 64 sites, or the 16-group reduction if larger (in double2)
   c.fd_tab_d2 = std::max(c.T * 64, 16 * 4 * 64 / 2);
-  c.g_dd = env_int("PIMD_G_DD", 16, 0, 64);
-  c.g_qq = env_int("PIMD_G_QQ", 8, 0, 32);
-  c.g_fin = env_int("PIMD_G_FIN", 8, 0, 32);
+  c.g_dd = env_int("PIMD_G_DD", dflt_dd, 0, 64);
+  c.g_qq = env_int("PIMD_G_QQ", dflt_qq, 0, 32);
+  c.g_fin = env_int("PIMD_G_FIN", dflt_qq, 0, 32);
   c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
   c.overlap = env_int("PIMD_SOR_OVERLAP", 0, 0, 1);
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
@@ -1516,10 +1528,12 @@ static int autotune_ctx(pimd_ctx *x, double *best_ms) {
   double tbest = 0.0;
   if (rc == PIMD_OK) rc = time_forces(x, scratch, e0, e1, 2, tbest);
   // candidate values per knob, in the order of Shape
-  const int cand[NSHAPE][3] = {{4, 8, 16}, {16, 32, 64}, {4, 8, 16}, {4, 8, 16}, {16, 32, 64}};
+  // (0: no further candidate)
+  const int cand[NSHAPE][5] = {{4, 8, 16, 32, 0}, {8, 16, 32, 64, 0}, {4, 8, 16, 32, 0},
+                               {4, 8, 16, 0, 0}, {4, 8, 16, 32, 64}};
   for (int knob = 0; knob < NSHAPE && rc == PIMD_OK; ++knob) {
     const int *vals = cand[knob];
-    for (int v = 0; v < 3 && rc == PIMD_OK; ++v) {
+    for (int v = 0; v < 5 && vals[v] != 0 && rc == PIMD_OK; ++v) {
       Shape sh = best;
       int *fields[NSHAPE] = {&sh.g_qq, &sh.g_dd, &sh.g_fin, &sh.fd_cg, &sh.sf_sites};
       int &field = *fields[knob];
