@@ -1270,8 +1270,9 @@ static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
   DevBufs &d = x->d;
   pimd_ctx *cen = x->cen;
   hipStream_t s = x->stream;
-  auto thermostat = [&](int half) {
-    if (md.bead_thermostat != 1) return;
+  const bool lang = md.bead_thermostat == 1;
+  auto thermostat = [&](int half, const double *kick) {
+    if (!lang) return;
     const double *nz = nullptr;
     if (noise) {
       (void)hipMemcpyAsync(d.noise, noise + (size_t)half * n, sizeof(double) * n,
@@ -1279,29 +1280,40 @@ static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
       nz = d.noise;
     }
     x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half);
+      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half, kick,
+                      delt2);
     });
   };
   auto centroid = [&] {
     x->timed("centroid_pbc",
              [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
   };
-  thermostat(0);
+  thermostat(0, nullptr);
   if (md.global_nh) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
     x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
   }
-  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-  centroid();
+  centroid();  // reads and moves coordinates only: the kicks below commute with it
   for (int ti = 1; ti <= intra; ++ti) {
-    x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
-    if (P > 1)
-      x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s); });
-    else
+    if (P > 1) {
+      // the kicks before each free ring-polymer step ride on it: the outer
+      // half kick and the first fast one, then the previous inner step's
+      // closing fast kick and this step's opening one (the same gradient)
+      x->timed("nm_evolve", [&] {
+        if (ti == 1)
+          launch_nm_evolve(P, na, d, d.RR, d.PP, s, d.dRR, delt2, d.dRRfast, delt2fast);
+        else
+          launch_nm_evolve(P, na, d, d.RR, d.PP, s, d.dRRfast, delt2fast, d.dRRfast, delt2fast);
+      });
+    } else {
+      if (ti == 1) x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
+      else x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
+      x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
       x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, deltfast, s); });
+    }
     x->timed("pes_fast", [&] { launch_pes_fast(x->c, d, d.RR, d.RRc, d.dRRfast, d.mol, ti == intra, s); });
-    x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
   }
+  x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
   centroid();
   int rc = PIMD_OK;
   DevBufs &dc = cen->d;
@@ -1314,8 +1326,10 @@ static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
   md_kinetic(x);
   launch_contract_record(P, N, d.mol, dc.Upot, dc.virt, dc.virialc, d.uk, x->d_rec, d.clk, s);
   md_observables(x, nullptr, nullptr, nullptr, nullptr);  // no per-bead dipoles here
-  x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-  thermostat(1);
+  if (lang)
+    thermostat(1, d.dRR);  // with the outer half kick
+  else
+    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
   md_kinetic(x);
   if (md.global_nh) {
     x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });

@@ -220,9 +220,11 @@ void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStrea
 void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
                             const double *vtcen, const double *vccen, const double *uk,
                             double *rec0, const int *clk, hipStream_t s);
-// kick (optional): PP first takes the half kick PP - MASSCON*kick*delt2.
+// kick, kick2 (optional): PP first takes the half kicks PP - MASSCON*kick*delt2,
+// then - MASSCON*kick2*delt2b, each rounded as a k_kick launch would.
 void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
-                      const double *kick = nullptr, double delt2 = 0.0);
+                      const double *kick = nullptr, double delt2 = 0.0,
+                      const double *kick2 = nullptr, double delt2b = 0.0);
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
                      hipStream_t s, const int *clk = nullptr, int half = 0,

@@ -27,7 +27,8 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
                                                   const double *__restrict__ msf,
                                                   const double *__restrict__ masses, double *RR,
                                                   double *PP, const double *__restrict__ kick,
-                                                  double delt2) {
+                                                  double delt2, const double *__restrict__ kick2,
+                                                  double delt2b) {
   extern __shared__ double lds[];
   const int CB = nm_cols(P);
   double *Cs = lds, *xr = Cs + P * P, *xp = xr + P * CB, *yr = xp + P * CB, *yp = yr + P * CB;
@@ -41,6 +42,7 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
     // with kick: the half kick PPt - MASSCON*dRRt*delt2 before the step
     // (MD.f90:49), as k_kick forms it
     if (kick && col < ncols) p = p - MASSCON * kick[k * stride + col] * delt2;
+    if (kick2 && col < ncols) p = p - MASSCON * kick2[k * stride + col] * delt2b;
     xp[e] = p;
   }
   __syncthreads();
@@ -71,11 +73,11 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
 }
 
 void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
-                      const double *kick, double delt2) {
+                      const double *kick, double delt2, const double *kick2, double delt2b) {
   const int CB = nm_cols(P), n = natoms * 3;
   const size_t lds = sizeof(double) * ((size_t)P * P + 4 * (size_t)P * CB);
   hipLaunchKernelGGL(k_nm_evolve, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
-                     b.A, b.msf, b.masses, RR, PP, kick, delt2);
+                     b.A, b.msf, b.masses, RR, PP, kick, delt2, kick2, delt2b);
 }
 
 // Philox4x32-10 counter-based generator (J. K. Salmon et al., "Parallel random
