@@ -510,6 +510,7 @@ int alloc_system(pimd_ctx *x) {
   c.sf_sites = env_int("PIMD_SFAC_SITES", mb >= 8192 ? 32 : 8, 1, 64);
   c.fd_cg = env_int("PIMD_FIELD_CG", 0, 0, 16);
   c.sf_kind = env_int("PIMD_SFAC_KIND", 1, 0, 1);
+  c.sfv_col = env_int("PIMD_SFV_COL", 1, 0, 1);
   c.fd_klds = env_int("PIMD_FIELD_KLDS", 1, 0, 1);
   // phase tables of 64 sites, or the 16-group reduction if larger (in double2)
   c.fd_tab_d2 = std::max(c.T * 64, 16 * 4 * 64 / 2);

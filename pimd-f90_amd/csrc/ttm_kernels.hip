@@ -654,6 +654,7 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
                                                  double *__restrict__ part, int sc, int scp) {
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
+  constexpr int NV = MODE == SF_V ? 6 : 2;  // values per k
   extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
   double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
   const int b = blockIdx.z;
@@ -688,9 +689,12 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
   const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
   const double kvx = TWOPI * c.boxi[0] * (double)cl.x, kvy = TWOPI * c.boxi[1] * (double)cl.y;
   const double tpz = TWOPI * c.boxi[2];
-  double are[KZ], aim[KZ];
+  // SF_Q, SF_D: (re, im) per k; SF_V: (sum mu cos, sum mu sin) per component
+  double acc[NV][KZ];
 #pragma unroll
-  for (int i = 0; i < KZ; ++i) are[i] = aim[i] = 0.0;
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int i = 0; i < KZ; ++i) acc[v][i] = 0.0;
   #pragma unroll 2
   for (int e = 0; e < nt; ++e) {
     const double2 x = lds_sf[rowx * scp + e];
@@ -704,7 +708,7 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
     double w = 0.0, wxy = 0.0, mz = 0.0;
     if (MODE == SF_Q) {
       w = lw[e];
-    } else {
+    } else if (MODE == SF_D) {
       wxy = kvx * lw[e * 3 + 0] + kvy * lw[e * 3 + 1];
       mz = lw[e * 3 + 2];
     }
@@ -717,23 +721,28 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
           ck = nc;
         }
         if (MODE == SF_Q) {
-          are[i] += w * ck;
-          aim[i] += w * sk;
-        } else {
+          acc[0][i] += w * ck;
+          acc[1][i] += w * sk;
+        } else if (MODE == SF_D) {
           const double dk = wxy + (tpz * (double)(cl.z + i)) * mz;
-          are[i] -= dk * sk;
-          aim[i] += dk * ck;
+          acc[0][i] -= dk * sk;
+          acc[1][i] += dk * ck;
+        } else {
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            acc[cc][i] += lw[e * 3 + cc] * ck;
+            acc[3 + cc][i] += lw[e * 3 + cc] * sk;
+          }
         }
       }
     }
   }
-  double *o = part + (((size_t)b * nch + chunk) * c.nk + k0) * 2;
+  double *o = part + (((size_t)b * nch + chunk) * c.nk + k0) * NV;
 #pragma unroll
   for (int i = 0; i < KZ; ++i)
-    if (i < n) {
-      o[2 * i] = are[i];
-      o[2 * i + 1] = aim[i];
-    }
+    if (i < n)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) o[NV * i + v] = acc[v][i];
 }
 
 // Structure factors with one k-vector per lane.  A block holds 256 k of one
@@ -851,15 +860,17 @@ static size_t sfac_lds(const SysConst &c, int mode, int sc, int scp) {
 
 template <int MODE, int KZ>
 static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
+  constexpr int NV = MODE == SF_V ? 6 : 2;
   int nch, sc, scp;
   sfac_chunking(c, M, nch, sc, scp);
   const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
   dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
   hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), sfac_lds(c, MODE, sc, scp), s, c,
                      b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
-  dim3 g2((2 * c.nk + 255) / 256, c.P);
-  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, 2, b.active,
-                     MODE == SF_D ? 1 : 0, MODE == SF_Q ? b.Sq : b.Sm);
+  dim3 g2((NV * c.nk + 255) / 256, c.P);
+  double *out = MODE == SF_Q ? b.Sq : MODE == SF_D ? b.Sm : b.Sv;
+  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, NV, b.active,
+                     MODE == SF_D ? 1 : 0, out);
 }
 
 template <int MODE>
@@ -893,9 +904,10 @@ static void launch_sfac_k(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
 
 void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
   const int M = mode == SF_Q ? 3 * c.N : c.nd * c.N;
-  if (c.sf_kind == 1 && mode != SF_V) {
-    const bool ok = mode == SF_Q ? sfac_col_dispatch<SF_Q>(c, b, M, s)
-                                 : sfac_col_dispatch<SF_D>(c, b, M, s);
+  if (c.sf_kind == 1 && (mode != SF_V || c.sfv_col)) {
+    const bool ok = mode == SF_Q   ? sfac_col_dispatch<SF_Q>(c, b, M, s)
+                    : mode == SF_D ? sfac_col_dispatch<SF_D>(c, b, M, s)
+                                   : sfac_col_dispatch<SF_V>(c, b, M, s);
     if (ok) return;
   }
   if (mode == SF_Q) launch_sfac_k<SF_Q>(c, b, M, s);
