@@ -11,7 +11,10 @@ from helpers import rel_err
 
 pytestmark = pytest.mark.gpu
 
-REPLICAS = [(3, 32, 8, 250.0), (2, 64, 4, 300.0), (3, 128, 16, 350.0)]
+# (model, waters, beads, T): TTM3-F, TTM2.1-F and the fixed-charge q-SPC/Fw
+# and SPC/F models side by side
+REPLICAS = [(3, 32, 8, 250.0), (2, 64, 4, 300.0), (3, 128, 16, 350.0), (4, 48, 8, 300.0),
+            (5, 32, 2, 280.0)]
 
 
 def _make(model, N, P, T, seed):
@@ -31,7 +34,7 @@ def _fresh():
     return [_make(m, N, P, T, 900 + i) for i, (m, N, P, T) in enumerate(REPLICAS)]
 
 
-SEEDS = [11, 22, 33]
+SEEDS = [11, 22, 33, 44, 55]
 NSTEPS = 6
 
 

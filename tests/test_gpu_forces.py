@@ -139,3 +139,27 @@ def test_gradient_sums_to_small_net_force(big_system):
     scale = np.abs(r["dRRt"]).max()
     print("net gradient", net, "max component", scale, flush=True)
     assert net < 1e-5 * scale
+
+
+def test_fixed_charge_models_refuse_ttm_only_paths():
+    """Contraction (its fast forces are the TTM models' PS surface) and the
+    TTM / PES parameter derivatives are refused for q-SPC/Fw and SPC/F; the
+    force call itself works on one bead as on many."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PimdError, TTMForceField
+    RRt, _, RRc, L = make_system(16, 1, seed=3, lattice=(4, 2, 2))
+    for model in (4, 5):
+        kw = dict(pot_model=model, nwaters=16, nbeads=1, box=(L, L, L))
+        with pytest.raises(PimdError, match="contraction"):
+            TTMForceField(TTMParams(**kw, contraction=True))
+        ff = TTMForceField(TTMParams(**kw))
+        r = ff.full_bead_forces(RRt, RRc)
+        assert np.all(np.isfinite(r["dRRt"])) and np.all(r["iters"] == 0)
+        assert np.all(r["virialct"] == 0.0) and np.all(r["dip_momEt"] == 0.0)
+        na = 48
+        with pytest.raises(PimdError, match="TTM"):
+            ff.ttm_param_grad(RRt, np.zeros(1), np.zeros((1, na, 3)))
+        with pytest.raises(PimdError, match="TTM"):
+            ff.pes_param_grad(RRt, np.zeros(1), np.zeros((1, na, 3)))
+        ff.close()

@@ -89,3 +89,14 @@ def test_run_setup_from_reference_input():
     inp2 = runinput.read_input(os.path.join(HERE, "input_new_64.inp"), warn=lambda m: None)
     with pytest.raises(runinput.InputError, match="pot_model 6"):
         runinput.run_setup(inp2, (12.4, 12.4, 12.4), 192)
+
+
+def test_run_setup_fixed_charge_models():
+    """pot_model 4 (q-SPC/Fw) and 5 (SPC/F) set up as the other models;
+    6 (SIESTA) is refused."""
+    for model in (4, 5):
+        inp = runinput.read_input(os.path.join(HERE, "quirks.inp"), warn=lambda m: None)
+        inp["pot_model"] = model
+        inp["Nbeads"] = 2
+        p, md = runinput.run_setup(inp, (9.865, 9.865, 9.865), 96)
+        assert p.pot_model == model and p.nwaters == 32 and p.nbeads == 2
