@@ -8,6 +8,7 @@
 // The mode transforms work on LDS slabs of P beads x CB (atom, component)
 // columns with C resident in LDS.
 #include <algorithm>
+#include <cstdlib>
 
 #include "pimd_internal.h"
 
@@ -72,12 +73,188 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
   }
 }
 
+static bool nm_mfma();
+static bool langevin_mfma();
+static void launch_nm_evolve_mfma(int P, int natoms, DevBufs &b, double *RR, double *PP,
+                                  hipStream_t s, const double *kick, double delt2,
+                                  const double *kick2, double delt2b);
+
 void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
                       const double *kick, double delt2, const double *kick2, double delt2b) {
+  if (nm_mfma() && P <= 64) {
+    launch_nm_evolve_mfma(P, natoms, b, RR, PP, s, kick, delt2, kick2, delt2b);
+    return;
+  }
   const int CB = nm_cols(P), n = natoms * 3;
   const size_t lds = sizeof(double) * ((size_t)P * P + 4 * (size_t)P * CB);
   hipLaunchKernelGGL(k_nm_evolve, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
                      b.A, b.msf, b.masses, RR, PP, kick, delt2, kick2, delt2b);
+}
+
+// ---------------------------------------------------------------------------
+// The same transforms on FP64 matrix cores.  v_mfma_f64_16x16x4_f64 adds
+// A (16 x 4) B (4 x 16) to D (16 x 16) across one wave: lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16], and D[l / 16 + 4 i][l % 16] in
+// element i (rows interleaved over the four lane groups).  A block is one wave and 16 (atom, component) columns; P is
+// padded with zeros to PT (a multiple of 16) modes and beads, C sits in LDS
+// with rows padded by one double (no bank conflicts between the 16 rows a
+// wave reads at once), and the mode-space slab goes through LDS between the
+// two transforms (the D layout of one product is not the B layout of the
+// next).  Products are summed in the matrix cores' order, not the
+// reference's; the difference is rounding.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d d) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+}
+
+constexpr int NM_TILE = 16;  // columns per block
+
+// C (P x P) into LDS as PT x PT with row stride PT + 1, zero padded.
+template <int PT>
+__device__ __forceinline__ void stage_C(double *Cs, const double *__restrict__ Cg, int P) {
+  for (int e = threadIdx.x; e < PT * PT; e += blockDim.x) {
+    const int m = e / PT, k = e % PT;
+    Cs[m * (PT + 1) + k] = (m < P && k < P) ? Cg[m * P + k] : 0.0;
+  }
+}
+
+// Y[rt] = C[rt rows] X for the RT row tiles, X given as B operands x[kk]
+// (bead kk*4 + l/16, column l%16).
+template <int PT>
+__device__ __forceinline__ void mfma_fwd(const double *Cs, const double *x, v4d *y) {
+  constexpr int KS = PT / 4, RT = PT / 16;
+  const int j = threadIdx.x & 15, q = (threadIdx.x >> 4) & 3;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    v4d d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) d = mfma_f64(Cs[(rt * 16 + j) * (PT + 1) + kk * 4 + q], x[kk], d);
+    y[rt] = d;
+  }
+}
+
+// X[rt] = C^T[rt rows] Y with Y a PT x 16 slab in LDS (row stride 17).
+template <int PT>
+__device__ __forceinline__ void mfma_bwd(const double *Cs, const double *Ys, v4d *xo) {
+  constexpr int KS = PT / 4, RT = PT / 16;
+  const int j = threadIdx.x & 15, q = (threadIdx.x >> 4) & 3;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    v4d d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk)
+      d = mfma_f64(Cs[(kk * 4 + q) * (PT + 1) + rt * 16 + j], Ys[(kk * 4 + q) * 17 + j], d);
+    xo[rt] = d;
+  }
+}
+
+template <int PT>
+__global__ __launch_bounds__(64) void k_nm_evolve_mfma(int P, int natoms,
+                                                       const double *__restrict__ Cg,
+                                                       const double *__restrict__ A,
+                                                       const double *__restrict__ msf,
+                                                       const double *__restrict__ masses,
+                                                       double *RR, double *PP,
+                                                       const double *__restrict__ kick,
+                                                       double delt2,
+                                                       const double *__restrict__ kick2,
+                                                       double delt2b) {
+  constexpr int KS = PT / 4, RT = PT / 16;
+  __shared__ double Cs[PT * (PT + 1)], Yr[PT * 17], Yp[PT * 17];
+  const int ncols = natoms * 3;
+  const size_t stride = (size_t)ncols;
+  const int j = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const int col = blockIdx.x * NM_TILE + j;
+  const bool vc = col < ncols;
+  stage_C<PT>(Cs, Cg, P);
+  double xr[KS], xp[KS];
+#pragma unroll
+  for (int kk = 0; kk < KS; ++kk) {
+    const int bead = kk * 4 + q;
+    const bool in = vc && bead < P;
+    const size_t o = (size_t)bead * stride + col;
+    xr[kk] = in ? RR[o] : 0.0;
+    double p = in ? PP[o] : 0.0;
+    // the half kicks before the step (MD.f90:49), rounded as k_kick does
+    if (kick && in) p = p - MASSCON * kick[o] * delt2;
+    if (kick2 && in) p = p - MASSCON * kick2[o] * delt2b;
+    xp[kk] = p;
+  }
+  __syncthreads();
+  v4d yr[RT], yp[RT];
+  mfma_fwd<PT>(Cs, xr, yr);
+  mfma_fwd<PT>(Cs, xp, yp);
+  // exact harmonic propagator of each mode (NormalModes.f90:34-63)
+  const double m = vc ? masses[col / 3] : 1.0;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int mode = rt * 16 + q + 4 * i;
+      double nr = 0.0, np = 0.0;
+      if (mode < P) {
+        const double vr = yr[rt][i], vp = yp[rt][i];
+        nr = A[mode] * vr + A[P + mode] * vp / (m * msf[mode]);
+        np = A[2 * P + mode] * m * msf[mode] * vr + A[3 * P + mode] * vp;
+      }
+      Yr[mode * 17 + j] = nr;
+      Yp[mode * 17 + j] = np;
+    }
+  __syncthreads();
+  v4d orr[RT], opp[RT];
+  mfma_bwd<PT>(Cs, Yr, orr);
+  mfma_bwd<PT>(Cs, Yp, opp);
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int bead = rt * 16 + q + 4 * i;
+      if (vc && bead < P) {
+        RR[(size_t)bead * stride + col] = orr[rt][i];
+        PP[(size_t)bead * stride + col] = opp[rt][i];
+      }
+    }
+}
+
+// PT: P rounded up to a multiple of 16 (P <= 64).
+static int nm_pt(int P) { return ((P + 15) / 16) * 16; }
+
+// The free ring-polymer step runs on the FP64 matrix cores unless
+// PIMD_NM_MFMA=0 (13 us against 21 us at 512 x 32).  The Langevin step keeps
+// the FMA slabs unless PIMD_LANGEVIN_MFMA=1: its time goes to the normal
+// deviates, which the slab layout spreads over twice as many lanes (18 us
+// against 20 us).
+static int env_flag(const char *name, int dflt) {
+  const char *v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
+}
+static bool nm_mfma() { return env_flag("PIMD_NM_MFMA", 1) != 0; }
+static bool langevin_mfma() { return env_flag("PIMD_LANGEVIN_MFMA", 0) != 0; }
+
+static void launch_nm_evolve_mfma(int P, int natoms, DevBufs &b, double *RR, double *PP,
+                                  hipStream_t s, const double *kick, double delt2,
+                                  const double *kick2, double delt2b) {
+  const int n = natoms * 3;
+  const dim3 grid((n + NM_TILE - 1) / NM_TILE);
+  switch (nm_pt(P)) {
+    case 16:
+      hipLaunchKernelGGL(k_nm_evolve_mfma<16>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
+                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_nm_evolve_mfma<32>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
+                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
+      break;
+    case 48:
+      hipLaunchKernelGGL(k_nm_evolve_mfma<48>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
+                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
+      break;
+    default:
+      hipLaunchKernelGGL(k_nm_evolve_mfma<64>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
+                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
+      break;
+  }
 }
 
 // Philox4x32-10 counter-based generator (J. K. Salmon et al., "Parallel random
@@ -182,9 +359,97 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
   }
 }
 
+// k_langevin on FP64 matrix cores (layout as k_nm_evolve_mfma).
+template <int PT>
+__global__ __launch_bounds__(64) void k_langevin_mfma(int P, int natoms,
+                                                      const double *__restrict__ Cg,
+                                                      const double *__restrict__ msf,
+                                                      const double *__restrict__ c1,
+                                                      const double *__restrict__ c2, double massO,
+                                                      double massH, double *PP,
+                                                      const double *__restrict__ noise,
+                                                      uint64_t seed, uint64_t counter,
+                                                      const int *__restrict__ clk, int half,
+                                                      const double *__restrict__ kick,
+                                                      double delt2) {
+  constexpr int KS = PT / 4, RT = PT / 16;
+  __shared__ double Cs[PT * (PT + 1)], Yp[PT * 17];
+  if (clk) counter = (uint64_t)clk[0] * 2 + half;
+  const int ncols = natoms * 3;
+  const size_t stride = (size_t)ncols;
+  const int j = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const int col = blockIdx.x * NM_TILE + j;
+  const bool vc = col < ncols;
+  stage_C<PT>(Cs, Cg, P);
+  double xp[KS];
+#pragma unroll
+  for (int kk = 0; kk < KS; ++kk) {
+    const int bead = kk * 4 + q;
+    const bool in = vc && bead < P;
+    const size_t o = (size_t)bead * stride + col;
+    double p = in ? PP[o] : 0.0;
+    // with kick: the half kick PPt - MASSCON*dRRt*delt2 first (MD.f90:94)
+    if (kick && in) p = p - MASSCON * kick[o] * delt2;
+    xp[kk] = p;
+  }
+  __syncthreads();
+  v4d yp[RT];
+  mfma_fwd<PT>(Cs, xp, yp);
+  const int a = col / 3, comp = col % 3;
+  const double m = (a % 3 == 0) ? massO : massH;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int mode = rt * 16 + q + 4 * i;
+      double v = 0.0;
+      if (mode < P) {
+        v = yp[rt][i];
+        if (vc) {
+          const uint64_t n = ((uint64_t)a * P + mode) * 3 + comp;
+          double xi;
+          if (noise) {
+            xi = noise[n];
+          } else {
+            double z0, z1;
+            normal2(seed, counter, n >> 1, z0, z1);
+            xi = (n & 1) ? z1 : z0;
+          }
+          const double sm = sqrt(m * msf[mode]);
+          v = c1[mode] * v + c2[mode] * (sm * xi);
+        }
+      }
+      Yp[mode * 17 + j] = v;
+    }
+  __syncthreads();
+  v4d op[RT];
+  mfma_bwd<PT>(Cs, Yp, op);
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int bead = rt * 16 + q + 4 * i;
+      if (vc && bead < P) PP[(size_t)bead * stride + col] = op[rt][i];
+    }
+}
+
 void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
                      uint64_t seed, uint64_t counter, double massO, double massH,
                      hipStream_t s, const int *clk, int half, const double *kick, double delt2) {
+  if (langevin_mfma() && P <= 64) {
+    const dim3 grid((natoms * 3 + NM_TILE - 1) / NM_TILE);
+#define PIMD_LANG_MFMA(PT)                                                                    \
+  hipLaunchKernelGGL(k_langevin_mfma<PT>, grid, dim3(64), 0, s, P, natoms, b.C, b.msf, b.c1, \
+                     b.c2, massO, massH, PP, noise, seed, counter, clk, half, kick, delt2)
+    switch (nm_pt(P)) {
+      case 16: PIMD_LANG_MFMA(16); break;
+      case 32: PIMD_LANG_MFMA(32); break;
+      case 48: PIMD_LANG_MFMA(48); break;
+      default: PIMD_LANG_MFMA(64); break;
+    }
+#undef PIMD_LANG_MFMA
+    return;
+  }
   const int CB = nm_cols(P), n = natoms * 3;
   const size_t lds = sizeof(double) * ((size_t)P * P + 2 * (size_t)P * CB);
   hipLaunchKernelGGL(k_langevin, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,

@@ -141,3 +141,18 @@ def test_contracted_vs_reference(name):
     # a second run continues from the state, the fast gradient included
     rec2 = integ.run(1, t0=nsteps + 1)
     assert np.all(np.isfinite(rec2["Upot"]))
+
+
+@pytest.mark.parametrize("path", ["nm_fma", "langevin_mfma"])
+def test_other_transform_kernels_vs_reference(path, monkeypatch):
+    """The normal-mode step on FMA slabs (PIMD_NM_MFMA=0) and the Langevin
+    step on the FP64 matrix cores (PIMD_LANGEVIN_MFMA=1), the forms not
+    taken by default, against the reference's outputs."""
+    if path == "nm_fma":
+        monkeypatch.setenv("PIMD_NM_MFMA", "0")
+        for name in golden_names("nm_"):
+            test_normal_modes_vs_reference(name)
+    else:
+        monkeypatch.setenv("PIMD_LANGEVIN_MFMA", "1")
+        for name in golden_names("langevin_"):
+            test_langevin_vs_reference(name)
