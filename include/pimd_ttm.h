@@ -81,7 +81,8 @@ int pimd_last_error(char *buf, size_t len);
 
 /* Derived set-up values: out[0]=aewald, out[1]=Uvdw_lrc, out[2]=shiftA,
  * out[3]=shiftB, out[4..7]=polfac(1:4), out[8]=nkvec, out[9]=hmax,
- * out[10]=kmax, out[11]=lmax (12 doubles). */
+ * out[10]=kmax, out[11]=lmax, out[12]=1 when the neighbour search uses a
+ * cell list (13 doubles). */
 int pimd_ctx_info(const pimd_ctx *ctx, double *out);
 
 /* ---- forces: force_calc.f90:26-106 full_bead_forces ------------------- */

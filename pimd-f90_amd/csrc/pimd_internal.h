@@ -48,6 +48,8 @@ struct SysConst {
   int mixed;               // PIMD_MIXED: FP32 real-space pair arithmetic
   int contraction;         // no monomer PES in the force call (pot_ttm.f90:92-132)
   int nbr_cap;
+  int cells;               // 1: neighbour search through a cell list
+  int ncell[3];            // cells per axis (>= 3 each when cells = 1), width >= Rc
   int kzmax;               // longest kz-column
   int nseg, segmax;        // kz-columns cut into segments of at most segmax k
   int sf_chunks_max;       // site chunks of the column structure-factor kernel
@@ -99,6 +101,7 @@ struct DevBufs {
   double *hist = nullptr;                       // [P][4][N*nd][3]
   int *nbr = nullptr, *nbrc = nullptr;          // [P][N][cap], [P][N]
   int *nbrn = nullptr;                          // [P][N] near partners (listed first)
+  int *cell_start = nullptr, *cell_mol = nullptr;  // [P][ncells + 1], [P][N] cell list
   double2 *tab = nullptr;                       // [P][T][4N]
   double *Sq = nullptr, *Sm = nullptr;          // [P][nk][2]
   double *Sv = nullptr;                         // [P][nk][6]

@@ -378,7 +378,132 @@ __global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__res
   }
 }
 
+// Cell list of each bead (one block per bead): the cell of every oxygen
+// (its position wrapped into the box), a counting sort into cell_mol with
+// cell_start[c] the first entry of cell c, and each cell's molecules put in
+// ascending order so that the lists do not depend on the order of atomics.
+__global__ __launch_bounds__(256) void k_cells(SysConst c, const Site *__restrict__ site,
+                                              int *cell_start, int *cell_mol) {
+  extern __shared__ int lds_cells[];  // [ncells] counts, then fill pointers
+  const int b = blockIdx.x, N = c.N;
+  const int nx = c.ncell[0], ny = c.ncell[1], nz = c.ncell[2], nc = nx * ny * nz;
+  int *cnt = lds_cells, *fill = lds_cells + nc;
+  int *start = cell_start + (size_t)b * (nc + 1);
+  int *mol = cell_mol + (size_t)b * N;
+  auto cell_of = [&](int m) {
+    const Site o = site[((size_t)b * N + m) * 4];
+    const double r[3] = {o.x, o.y, o.z};
+    int id[3];
+    for (int k = 0; k < 3; ++k) {
+      const double w = r[k] - c.box[k] * floor(r[k] * c.boxi[k]);  // in [0, L]
+      id[k] = min((int)(w * c.boxi[k] * c.ncell[k]), c.ncell[k] - 1);
+      id[k] = max(id[k], 0);
+    }
+    return (id[0] * ny + id[1]) * nz + id[2];
+  };
+  for (int e = threadIdx.x; e < nc; e += blockDim.x) cnt[e] = 0;
+  __syncthreads();
+  for (int m = threadIdx.x; m < N; m += blockDim.x) atomicAdd(&cnt[cell_of(m)], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int e = 0; e < nc; ++e) {
+      start[e] = acc;
+      fill[e] = acc;
+      acc += cnt[e];
+    }
+    start[nc] = acc;
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < N; m += blockDim.x) mol[atomicAdd(&fill[cell_of(m)], 1)] = m;
+  __syncthreads();
+  // ascending molecule order inside each cell (a few molecules each)
+  for (int e = threadIdx.x; e < nc; e += blockDim.x) {
+    const int lo = start[e], hi = start[e + 1];
+    for (int i = lo + 1; i < hi; ++i) {
+      const int v = mol[i];
+      int k = i - 1;
+      while (k >= lo && mol[k] > v) {
+        mol[k + 1] = mol[k];
+        --k;
+      }
+      mol[k + 1] = v;
+    }
+  }
+}
+
+// The neighbour search of k_neighbors over the 27 cells around molecule i's
+// cell (cells at least Rc wide, at least three per axis, so every partner
+// inside Rc is among them once).  Partners come in cell order, ascending
+// inside each cell.
+__global__ __launch_bounds__(256) void k_neighbors_cells(SysConst c, const Site *__restrict__ site,
+                                                        const double *__restrict__ radius,
+                                                        const int *__restrict__ cell_start,
+                                                        const int *__restrict__ cell_mol,
+                                                        int *nbr, int *nbrc, int *nbrn) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= c.P * c.N) return;
+  const int b = wave / c.N, i = wave % c.N, N = c.N;
+  const int nx = c.ncell[0], ny = c.ncell[1], nz = c.ncell[2], nc = nx * ny * nz;
+  const int *start = cell_start + (size_t)b * (nc + 1);
+  const int *mol = cell_mol + (size_t)b * N;
+  const Site Oi = site[((size_t)b * N + i) * 4];
+  const double radi = radius[(size_t)b * N + i];
+  int id[3];
+  {
+    const double r[3] = {Oi.x, Oi.y, Oi.z};
+    for (int k = 0; k < 3; ++k) {
+      const double w = r[k] - c.box[k] * floor(r[k] * c.boxi[k]);
+      id[k] = max(min((int)(w * c.boxi[k] * c.ncell[k]), c.ncell[k] - 1), 0);
+    }
+  }
+  int *out = nbr + (size_t)wave * c.nbr_cap;
+  int nnear = 0, nfar = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int cx = (id[0] + dx + nx) % nx, cy = (id[1] + dy + ny) % ny,
+                  cz = (id[2] + dz + nz) % nz;
+        const int cell = (cx * ny + cy) * nz + cz;
+        const int lo = start[cell], hi = start[cell + 1];
+        for (int e0 = lo; e0 < hi; e0 += 64) {
+          const int e = e0 + lane;
+          const int j = e < hi ? mol[e] : -1;
+          bool in = false, near = false;
+          if (j >= 0 && j != i) {
+            const Site Oj = site[((size_t)b * N + j) * 4];
+            double x = Oi.x - Oj.x, y = Oi.y - Oj.y, z = Oi.z - Oj.z;
+            minimg(c, x, y, z);
+            const double r2 = norm2_exact(x, y, z);
+            in = r2 < c.rc2;
+            const double lim = 5.0 + radi + radius[(size_t)b * N + j] + 1e-6;
+            near = r2 < lim * lim;
+          }
+          const unsigned long long mn = __ballot(in && near), mf = __ballot(in && !near);
+          if (in && near) out[nnear + __popcll(mn & below)] = j;
+          if (in && !near) out[c.nbr_cap - 1 - (nfar + __popcll(mf & below))] = j;
+          nnear += __popcll(mn);
+          nfar += __popcll(mf);
+        }
+      }
+  if (lane == 0) {
+    nbrn[wave] = nnear;
+    nbrc[wave] = nfar;
+  }
+}
+
 void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
+  if (c.cells) {
+    const int nc = c.ncell[0] * c.ncell[1] * c.ncell[2];
+    hipLaunchKernelGGL(k_cells, dim3(c.P), dim3(256), sizeof(int) * 2 * nc, s, c, b.site,
+                       b.cell_start, b.cell_mol);
+    const long threads = (long)c.P * c.N * 64;
+    hipLaunchKernelGGL(k_neighbors_cells, dim3((threads + 255) / 256), dim3(256), 0, s, c, b.site,
+                       b.radius, b.cell_start, b.cell_mol, b.nbr, b.nbrc, b.nbrn);
+    return;
+  }
   dim3 grid((c.N + NB_I - 1) / NB_I, c.P);
   const size_t lds = sizeof(double) * 4 * c.N;
   if (lds <= 64 * 1024)

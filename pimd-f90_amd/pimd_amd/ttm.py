@@ -181,11 +181,11 @@ class TTMForceField:
         return self._h
 
     def info(self) -> dict:
-        o = np.zeros(12)
+        o = np.zeros(13)
         _check(load_library().pimd_ctx_info(self._h, _ptr(o)))
         return {"aewald": o[0], "Uvdw_lrc": o[1], "shiftA": o[2], "shiftB": o[3],
                 "polfac": o[4:8].copy(), "nkvec": int(o[8]), "hmax": int(o[9]),
-                "kmax": int(o[10]), "lmax": int(o[11])}
+                "kmax": int(o[10]), "lmax": int(o[11]), "cells": int(o[12])}
 
     def full_bead_forces(self, RRt, RRc, t: int = 1) -> dict:
         """pot_ttm for every bead; returns Upott, dRRt, virt, virialct, dipoles, chg, iters.

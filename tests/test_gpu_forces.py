@@ -71,15 +71,22 @@ def _select(r, beads):
 
 @pytest.mark.parametrize("model,N,P,rc,eps", [(2, 128, 4, -1.0, 1e-6), (3, 128, 3, -1.0, 1e-8),
                                               (3, 256, 2, 9.0, 1e-6), (4, 128, 4, -1.0, 1e-8),
-                                              (5, 256, 2, 9.0, 1e-6)])
-def test_forces_vs_oracle_medium(model, N, P, rc, eps):
+                                              (5, 256, 2, 9.0, 1e-6), (3, 256, 2, 6.0, 1e-6),
+                                              (2, 512, 1, 7.0, 1e-6), (4, 256, 2, 6.0, 1e-6),
+                                              (3, 512, 1, 6.0, 1e-6)])
+def test_forces_vs_oracle_medium(model, N, P, rc, eps, monkeypatch):
+    """PIMD_CELLS=2: the cases with Rc = 6 and 7 A (three cells per axis)
+    search neighbours through the cell list, the others over all molecules."""
     from pimd_amd.params import TTMParams
     from pimd_amd.systems import make_system
     RRt, _, RRc, L = make_system(N, P, seed=100 + N + model)
     p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=rc,
                   eps_ewald=eps).resolved()
     print("case", model, N, P, rc, eps, flush=True)
-    r = _ff(p).full_bead_forces(RRt, RRc)
+    monkeypatch.setenv("PIMD_CELLS", "2")
+    ff = _ff(p)
+    assert ff.info()["cells"] == (rc in (6.0, 7.0))
+    r = ff.full_bead_forces(RRt, RRc)
     beads = list(range(P))
     check_against(r, _oracle_forces(p, RRt, RRc, beads), p, f"ttm{model} N{N} P{P}")
 
@@ -163,3 +170,37 @@ def test_fixed_charge_models_refuse_ttm_only_paths():
         with pytest.raises(PimdError, match="TTM"):
             ff.pes_param_grad(RRt, np.zeros(1), np.zeros((1, na, 3)))
         ff.close()
+
+
+@pytest.mark.parametrize("model,N,P,rc", [(3, 256, 3, 6.0), (2, 512, 2, 7.0), (5, 256, 2, 6.0),
+                                          (3, 1024, 1, 6.0)])
+def test_cell_list_matches_full_search(model, N, P, rc, monkeypatch):
+    """With at least three cells of width >= Rc per axis the neighbour search
+    can go through a cell list (by default from 108 cells on, PIMD_CELLS=2
+    from 27); the forces equal those of the full search (PIMD_CELLS=0) up to
+    the order of the pair sums."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    RRt, _, RRc, L = make_system(N, P, seed=300 + N + model)
+    p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=rc,
+                  eps_ewald=1e-6).resolved()
+    print("case cells", model, N, P, rc, flush=True)
+    ncell = int(np.floor(L / rc)) ** 3
+    ff = _ff(p)
+    assert ff.info()["cells"] == (ncell >= 108)
+    ff.close()
+    monkeypatch.setenv("PIMD_CELLS", "2")
+    ff = _ff(p)
+    assert ff.info()["cells"] == 1
+    rc_ = ff.full_bead_forces(RRt, RRc)
+    ff.close()
+    monkeypatch.setenv("PIMD_CELLS", "0")
+    ff = _ff(p)
+    assert ff.info()["cells"] == 0
+    rf = ff.full_bead_forces(RRt, RRc)
+    ff.close()
+    _report(f"cells vs full ttm{model} N{N}", En=rel_err(rc_["Upott"], rf["Upott"]),
+            dRR=max_abs(rc_["dRRt"], rf["dRRt"]))
+    assert rel_err(rc_["Upott"], rf["Upott"]) < 1e-12
+    assert max_abs(rc_["dRRt"], rf["dRRt"]) < 1e-10
+    assert np.array_equal(rc_["iters"], rf["iters"])
