@@ -1270,10 +1270,12 @@ template <int MO, int ML>
 static void launch_field_m(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
   const int M = MO == FD_D ? c.nd * c.N : (ML == 2 ? 4 * c.N : 3 * c.N);
   dim3 grid((M + FD_SB - 1) / FD_SB, c.P);
-  // enough waves to fill the device: about 16 per CU
-  const long blocks = (long)grid.x * grid.y;
+  // as many column groups as the columns keep busy: more resident waves hide
+  // the table and structure-factor reads (512 H2O x 32 beads: the charge
+  // fields at 16 groups instead of 8 take 1.68 instead of 1.73 ms per force
+  // call)
   int cg = c.fd_cg;
-  if (cg == 0) cg = (blocks * 4 >= 4096 || c.ncol < 8) ? 4 : (blocks * 8 >= 4096 || c.ncol < 16) ? 8 : 16;
+  if (cg == 0) cg = c.ncol < 8 ? 4 : c.ncol < 16 ? 8 : 16;
   if (cg == 4) launch_field_cg<MO, ML, 4>(c, b, iter, grid, s);
   else if (cg == 8) launch_field_cg<MO, ML, 8>(c, b, iter, grid, s);
   else launch_field_cg<MO, ML, 16>(c, b, iter, grid, s);
