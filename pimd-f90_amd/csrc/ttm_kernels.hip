@@ -531,6 +531,10 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
                                                 const int *__restrict__ nbrn, double *phi,
                                                 double *Efq, double *dR, double *tally) {
   constexpr int SA0 = MODEL == 2 ? 0 : 1;  // TTM3-F: O carries neither charge nor dipole
+  // sites of j: unrolled in the FP32 form (more independent work per lane
+  // at the same occupancy), one at a time in the FP64 form (fewer live
+  // registers: two waves per SIMD instead of one)
+  constexpr int SB_UNROLL = sizeof(R) == 4 ? 3 : 1;
   extern __shared__ Site lds_site[];
   const int N = c.N;
   long grp;
@@ -554,22 +558,23 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
   // molecule records of this bead: LDS or global
   const Site *bead_sites = LDS ? lds_site : site + (size_t)b * N * 4;
   Site A[4];
-  double RcA[4][3];
-  for (int t = 0; t < 4; ++t) {
-    A[t] = bead_sites[a * 4 + t];
-    for (int k = 0; k < 3; ++k) RcA[t][k] = Rcs[(a * 4 + t) * 3 + k];
-  }
+  for (int t = 0; t < 4; ++t) A[t] = bead_sites[a * 4 + t];
+  const double RcO[3] = {Rcs[a * 12 + 0], Rcs[a * 12 + 1], Rcs[a * 12 + 2]};
   double ph[4] = {0, 0, 0, 0}, ef[4][3] = {{0}}, dro[3] = {0, 0, 0};
+  // The energy and virial terms of a pair are the same seen from either
+  // molecule, so every lane adds them for all its pairs and the sums are
+  // halved at the end (no branch on which molecule is the lower one).  The
+  // centroid virial's Rc(a, site) . F part is formed once per site at the end
+  // from the site's field: its pair forces are -q(a, site) times the field terms.
   double uvdw = 0.0, vc = 0.0, vt[9] = {0};
   const int nf = valid ? nbrc[grp] : 0, nn = valid ? nbrn[grp] : 0;
   const int *list = nbr + (size_t)grp * c.nbr_cap;
   auto visit = [&](const int j, auto thole_tag) {
     constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
-    Site B[4];
-    for (int t = 0; t < 4; ++t) B[t] = bead_sites[j * 4 + t];
+    const Site B0 = bead_sites[j * 4];
     // vdW between oxygens
-    double r[3] = {A[0].x - B[0].x, A[0].y - B[0].y, A[0].z - B[0].z};
+    double r[3] = {A[0].x - B0.x, A[0].y - B0.y, A[0].z - B0.z};
     minimg(c, r[0], r[1], r[2]);
     const double r2 = norm2_exact(r[0], r[1], r[2]);
     const R r2r = (R)r2;
@@ -583,41 +588,45 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
     const double d[3] = {(double)(fsc * (R)r[0]), (double)(fsc * (R)r[1]),
                          (double)(fsc * (R)r[2])};
     for (int k = 0; k < 3; ++k) dro[k] += d[k];
-    if (lo) {
-      uvdw += (double)uij;
-      double cx = RcA[0][0] - Rcs[(j * 4) * 3 + 0], cy = RcA[0][1] - Rcs[(j * 4) * 3 + 1],
-             cz = RcA[0][2] - Rcs[(j * 4) * 3 + 2];
+    uvdw += (double)uij;
+    {
+      double cx = RcO[0] - Rcs[j * 12 + 0], cy = RcO[1] - Rcs[j * 12 + 1],
+             cz = RcO[2] - Rcs[j * 12 + 2];
       minimg(c, cx, cy, cz);
       vc += cx * d[0] + cy * d[1] + cz * d[2];
-      for (int col = 0; col < 3; ++col)
-        for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
     }
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
     // O-O image shift of the reference's charge loop, Rij - R(iO) + R(jO)
     // (pot_ttm.f90:193); it enters the centroid virial of the lower molecule.
-    const double s0[3] = {r[0] - A[0].x + B[0].x, r[1] - A[0].y + B[0].y,
-                          r[2] - A[0].z + B[0].z};
+    const double s0[3] = {r[0] - A[0].x + B0.x, r[1] - A[0].y + B0.y, r[2] - A[0].z + B0.z};
+    // one site of j at a time (its record read in the loop), against every
+    // site of a: each accumulator of a site of a still takes its terms in
+    // the order (j, site of j)
+#pragma unroll SB_UNROLL
+    for (int sb = 1; sb < 4; ++sb) {
+      const Site Bs = bead_sites[j * 4 + sb];
+      const R qb = (R)Bs.q;
+      // the pair-side part of the centroid virial's lever arm
+      const double w[3] = {s0[0] - Rcs[(j * 4 + sb) * 3 + 0], s0[1] - Rcs[(j * 4 + sb) * 3 + 1],
+                           s0[2] - Rcs[(j * 4 + sb) * 3 + 2]};
 #pragma unroll
-    for (int sa = SA0; sa < 4; ++sa) {
-#pragma unroll
-      for (int sb = 1; sb < 4; ++sb) {
+      for (int sa = SA0; sa < 4; ++sa) {
         // site-site minimum image, oriented from a to b (pot_ttm.f90:206-207)
-        double v[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
+        double v[3] = {A[sa].x - Bs.x, A[sa].y - Bs.y, A[sa].z - Bs.z};
         minimg(c, v[0], v[1], v[2]);
         const double d2 = norm2_exact(v[0], v[1], v[2]);
         const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
         R ts0, ts1;
         smear01<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]),
                        (R)(lo ? c.raCC[sa][sb] : c.raCC[sb][sa]), ts0, ts1);
-        const R qb = (R)B[sb].q;
         if (sa > 0) ph[sa] += (double)(ts0 * qb);
         for (int k = 0; k < 3; ++k) ef[sa][k] += (double)(ts1 * qb * vr[k]);
-        if (lo && sa > 0) {
+        if (sa > 0) {
           const R qa = (R)A[sa].q;
           double dd[3];
           for (int k = 0; k < 3; ++k) dd[k] = (double)(-(ts1 * qa * qb * vr[k]));
-          double rcv[3];
-          for (int k = 0; k < 3; ++k) rcv[k] = RcA[sa][k] - Rcs[(j * 4 + sb) * 3 + k] + s0[k];
-          vc += rcv[0] * dd[0] + rcv[1] * dd[1] + rcv[2] * dd[2];
+          vc += w[0] * dd[0] + w[1] * dd[1] + w[2] * dd[2];
           for (int col = 0; col < 3; ++col)
             for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dd[col] * v[row];
         }
@@ -639,9 +648,16 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
     for (int t = SA0; t < 4; ++t)
       for (int k = 0; k < 3; ++k) Efq[(base + t) * 3 + k] += ef[t][k];
     for (int k = 0; k < 3; ++k) dR[base * 3 + k] += dro[k];
+    // Rc(a, site) . (sum of the site's pair forces), the forces being
+    // -q(a, site) times its field terms
+    for (int t = 1; t < 4; ++t) {
+      const double *rc = Rcs + (a * 4 + t) * 3;
+      vc -= A[t].q * (rc[0] * ef[t][0] + rc[1] * ef[t][1] + rc[2] * ef[t][2]);
+    }
+    for (int i = 0; i < 9; ++i) vt[i] *= 0.5;
     double *tl = tally + ((size_t)b * N + a) * T_COUNT;
-    tl[T_UVDW] += uvdw;
-    tl[T_VIRC] += vc;
+    tl[T_UVDW] += 0.5 * uvdw;
+    tl[T_VIRC] += 0.5 * vc;
     vadd9(tl + T_VIRT, vt);
   }
 }
