@@ -89,6 +89,24 @@ __device__ __forceinline__ void walk_list(const int *__restrict__ list, int firs
   }
 }
 
+// walk_list with D indices in flight: for visits too short to cover one
+// global load, the next D indices are requested ahead.  Same visiting order.
+template <int G, int D, class F>
+__device__ __forceinline__ void walk_list_deep(const int *__restrict__ list, int first, int end,
+                                               F &&f) {
+  int q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = first + d * G < end ? list[first + d * G] : 0;
+  for (int n = first; n < end; n += D * G) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int j = q[d], at = n + d * G, ahead = at + D * G;
+      q[d] = ahead < end ? list[ahead] : 0;
+      if (at < end) f(j);
+    }
+  }
+}
+
 __device__ __forceinline__ void vadd9(double *t, const double *v) {
   for (int i = 0; i < 9; ++i) t[i] += v[i];
 }

@@ -1464,6 +1464,8 @@ constexpr int DD_BLOCK = 1024;
 // doubles per staged dipole site: 6 used, one of padding so that lanes
 // gathering different sites spread over the LDS banks
 constexpr int DD_REC = 7;
+// neighbour indices requested ahead by the TTM3-F dipole pass
+constexpr int DD_AHEAD = 4;
 // The SOR convergence test of iteration `iter` for bead b (pot_ttm.f90:306,
 // 380-386), as k_converge forms it: the first 256 threads of the block add
 // the bead's squared dipole changes in the same order.  All blocks of the
@@ -1574,8 +1576,12 @@ __global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site
       }
     }
   };
-  walk_list<G>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
-  walk_list<G>(list, c.nbr_cap - nf + lane, c.nbr_cap, [&](int j) { visit(j, std::false_type{}); });
+  // a TTM3-F visit is one short dipole-tensor product: keep DD_AHEAD
+  // neighbour indices in flight (TTM2.1-F's nine products cover one load)
+  constexpr int AHEAD = MODEL == 2 ? 1 : DD_AHEAD;
+  walk_list_deep<G, AHEAD>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
+  walk_list_deep<G, AHEAD>(list, c.nbr_cap - nf + lane, c.nbr_cap,
+                           [&](int j) { visit(j, std::false_type{}); });
   for (int u = 0; u < ND; ++u)
     for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
   if (lane == 0) {
