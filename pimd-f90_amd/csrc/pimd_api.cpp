@@ -529,10 +529,20 @@ int alloc_system(pimd_ctx *x) {
   // the FP32 dipole pass runs lighter lanes: twice the lanes per molecule
   // (512 H2O x 32 beads: 32 instead of 16, 1.685 -> 1.657 ms per force call)
   const int dflt_dd = pow2_clamp((c.mixed ? 524288 : 262144) / std::max(mb, 1L), 16, 64);
-  c.sf_sites = env_int("PIMD_SFAC_SITES", mb >= 8192 ? 32 : 8, 1, 64);
+  // structure-factor site chunks: about 8 per bead from 4096 molecule-beads
+  // on (with two lane sets per block; 512 H2O x 32 beads: 1.66 -> 1.61 ms per
+  // mixed force call, TTM2.1-F 128 x 32: 1.75 -> 1.66 ms FP64), the staged
+  // tables of a chunk within 60 KB; 8 sites below, to spread small systems
+  int dflt_sites = 8;
+  if (mb >= 4096) {
+    const int lds_cap = std::max(8, 60 * 1024 / (16 * c.T) - 17);
+    dflt_sites = std::max(8, std::min({(c.nd * c.N + 7) / 8, 128, lds_cap}));
+  }
+  c.sf_sites = env_int("PIMD_SFAC_SITES", dflt_sites, 1, 128);
   c.fd_cg = env_int("PIMD_FIELD_CG", 0, 0, 16);
   c.sf_kind = env_int("PIMD_SFAC_KIND", 1, 0, 1);
   c.sfv_col = env_int("PIMD_SFV_COL", 1, 0, 1);
+  c.sf_split = env_int("PIMD_SFAC_SPLIT", 1, 0, 1);
   c.fd_klds = env_int("PIMD_FIELD_KLDS", 1, 0, 1);
   // phase tables of 64 sites, or the 16-group reduction if larger (in double2)
   c.fd_tab_d2 = std::max(c.T * 64, 16 * 4 * 64 / 2);

@@ -59,6 +59,7 @@ struct SysConst {
   int fd_tab_d2;           // double2 slots of the field kernel's table / reduction area
   int sf_kind;             // structure-factor kernel: 0 one k per lane, 1 kz-columns
   int sfv_col;             // 1: the 6-component SF_V by kz-columns too
+  int sf_split;            // 1: column structure factors with two lane sets per block
   int g_dd;                // lanes per molecule of the LDS-staged dipole kernel (0: gather kernel)
   int g_qq, g_fin;         // the same for the charge and final pair kernels
   int poll;                // 1: read the active-bead count back after every SOR iteration

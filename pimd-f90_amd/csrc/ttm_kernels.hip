@@ -785,8 +785,11 @@ __device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, doub
 // padded so that lanes reading different rows of one site hit different
 // banks).  Per-chunk partial sums are combined in a fixed order by
 // k_sfac_reduce.  SF_Q and SF_D as above.
-template <int MODE, int KZ>
-__global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__restrict__ site,
+// SPLIT = 2: the block holds two lane sets for the same segments, each over
+// half of the chunk's sites; the second hands its sums to the first through
+// LDS, which adds them and writes (twice the waves for the same partials).
+template <int MODE, int KZ, int SPLIT = 1>
+__global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__restrict__ site,
                                                  const double2 *__restrict__ tab,
                                                  const double *__restrict__ dip,
                                                  const int4 *__restrict__ cols,
@@ -820,11 +823,14 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
     }
   }
   __syncthreads();
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= c.nseg) return;
-  const int4 cl = cols[col];
+  const int nl = blockDim.x / SPLIT, half_id = threadIdx.x / nl;
+  const int col = blockIdx.x * nl + threadIdx.x % nl;
+  const bool live = col < c.nseg;
+  const int hn = (nt + SPLIT - 1) / SPLIT;
+  const int e0 = half_id * hn, e1 = min(nt, e0 + hn);
+  const int4 cl = cols[live ? col : 0];
   const int n = cl.w;
-  const int k0 = colk[col];
+  const int k0 = colk[live ? col : 0];
   const int rowx = cl.x, rowy = c.hmax + 1 + abs(cl.y), rowz = c.hmax + c.kmax + 2 + abs(cl.z);
   const int rowz1 = c.hmax + c.kmax + 3;
   const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
@@ -837,7 +843,7 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
 #pragma unroll
     for (int i = 0; i < KZ; ++i) acc[v][i] = 0.0;
   #pragma unroll 2
-  for (int e = 0; e < nt; ++e) {
+  for (int e = live ? e0 : e1; e < e1; ++e) {
     const double2 x = lds_sf[rowx * scp + e];
     double2 y = lds_sf[rowy * scp + e];
     double2 z = lds_sf[rowz * scp + e];
@@ -878,6 +884,22 @@ __global__ __launch_bounds__(256) void k_sfac_col(SysConst c, const Site *__rest
       }
     }
   }
+  if (SPLIT == 2) {
+    // after the weights, 16-byte aligned: [NV][KZ][nl] sums of the second set
+    double *red = lw + ((sc * NW + 1) & ~1);
+    if (half_id == 1 && live)
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int i = 0; i < KZ; ++i) red[(v * KZ + i) * nl + threadIdx.x % nl] = acc[v][i];
+    __syncthreads();
+    if (half_id == 1) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int i = 0; i < KZ; ++i) acc[v][i] += red[(v * KZ + i) * nl + threadIdx.x];
+  }
+  if (!live) return;
   double *o = part + (((size_t)b * nch + chunk) * c.nk + k0) * NV;
 #pragma unroll
   for (int i = 0; i < KZ; ++i)
@@ -1006,8 +1028,16 @@ static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s)
   sfac_chunking(c, M, nch, sc, scp);
   const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
   dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
-  hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), sfac_lds(c, MODE, sc, scp), s, c,
-                     b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
+  const size_t lds = sfac_lds(c, MODE, sc, scp);
+  // two lane sets per block where the second set's sums fit beside the tables
+  const size_t red = sizeof(double) * ((size_t)NV * KZ * threads + 2);
+  if (MODE != SF_V && c.sf_split && lds + red <= 96 * 1024) {
+    hipLaunchKernelGGL((k_sfac_col<MODE, KZ, 2>), grid, dim3(2 * threads), lds + red, s, c,
+                       b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
+  } else {
+    hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), lds, s, c, b.site, b.tab,
+                       b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
+  }
   dim3 g2((NV * c.nk + 255) / 256, c.P);
   double *out = MODE == SF_Q ? b.Sq : MODE == SF_D ? b.Sm : b.Sv;
   hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, NV, b.active,
