@@ -1009,8 +1009,11 @@ __global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *_
 
 // Site chunk of the column structure-factor kernel and its padded LDS row
 // (a multiple of 16 plus one double2, so rows start 4 banks apart).
-static void sfac_chunking(const SysConst &c, int M, int &nch, int &sc, int &scp) {
-  nch = std::max(1, std::min(c.sf_chunks_max, (M + c.sf_sites - 1) / c.sf_sites));
+static void sfac_chunking(const SysConst &c, int mode, int M, int &nch, int &sc, int &scp) {
+  // the dipole structure factor takes c.sf_sites per chunk; the charge and
+  // virial ones at most 32, which keeps their blocks spread over the device
+  const int sites = mode == SF_D ? c.sf_sites : std::min(c.sf_sites, 32);
+  nch = std::max(1, std::min(c.sf_chunks_max, (M + sites - 1) / sites));
   sc = (M + nch - 1) / nch;
   nch = (M + sc - 1) / sc;
   scp = ((sc + 15) / 16) * 16 + 1;
@@ -1025,7 +1028,7 @@ template <int MODE, int KZ>
 static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
   constexpr int NV = MODE == SF_V ? 6 : 2;
   int nch, sc, scp;
-  sfac_chunking(c, M, nch, sc, scp);
+  sfac_chunking(c, MODE, M, nch, sc, scp);
   const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
   dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
   const size_t lds = sfac_lds(c, MODE, sc, scp);
@@ -1047,7 +1050,7 @@ static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s)
 template <int MODE>
 static bool sfac_col_dispatch(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
   int nch, sc, scp;
-  sfac_chunking(c, M, nch, sc, scp);
+  sfac_chunking(c, MODE, M, nch, sc, scp);
   if (sfac_lds(c, MODE, sc, scp) > 64 * 1024) return false;
   if (c.segmax <= 4) launch_sfac_col<MODE, 4>(c, b, M, s);
   else if (c.segmax <= 8) launch_sfac_col<MODE, 8>(c, b, M, s);
@@ -1063,7 +1066,7 @@ template <int MODE>
 static void launch_sfac_k(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
   constexpr int NV = MODE == SF_V ? 6 : 2;
   int nch, sc, scp;
-  sfac_chunking(c, M, nch, sc, scp);
+  sfac_chunking(c, MODE, M, nch, sc, scp);
   dim3 grid((c.nk + 255) / 256, nch, c.P);
   hipLaunchKernelGGL((k_sfac_k<MODE>), grid, dim3(256), sfac_lds(c, MODE, sc, scp), s, c, b.site,
                      b.tab, b.dip, b.kv, b.khkl, b.active, b.part, sc, scp);
