@@ -207,11 +207,12 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
   Site A[4];
   for (int t = 0; t < 4; ++t) A[t] = recs[a * 4 + t];
   double ph[4] = {0, 0, 0, 0}, ef[4][3] = {{0}}, dro[3] = {0, 0, 0};
+  // the energy and virial terms of a pair are the same from either
+  // molecule: every lane adds them for all its pairs, halved at the end
   double uvdw = 0.0, vt[9] = {0};
   const int nf = valid ? nbrc[grp] : 0, nn = valid ? nbrn[grp] : 0;
   const int *list = nbr + (size_t)grp * c.nbr_cap;
   auto visit = [&](const int j) {
-    const bool lo = a < j;
     Site B[4];
     for (int t = 0; t < 4; ++t) B[t] = recs[j * 4 + t];
     double r[3] = {A[0].x - B[0].x, A[0].y - B[0].y, A[0].z - B[0].z};
@@ -224,11 +225,9 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
     const double d[3] = {(double)(fsc * (R)r[0]), (double)(fsc * (R)r[1]),
                          (double)(fsc * (R)r[2])};
     for (int k = 0; k < 3; ++k) dro[k] += d[k];
-    if (lo) {
-      uvdw += (double)uij;
-      for (int col = 0; col < 3; ++col)
-        for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
-    }
+    uvdw += (double)uij;
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
 #pragma unroll
     for (int sa = 1; sa < 4; ++sa) {
 #pragma unroll
@@ -243,7 +242,7 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
         const R qb = (R)B[sb].q;
         ph[sa] += (double)(ts0 * qb);
         for (int k = 0; k < 3; ++k) ef[sa][k] += (double)(ts1 * qb * vr[k]);
-        if (lo) {
+        {
           const R qa = (R)A[sa].q;
           double dd[3];
           for (int k = 0; k < 3; ++k) dd[k] = (double)(-(ts1 * qa * qb * vr[k]));
@@ -269,8 +268,9 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
       for (int k = 0; k < 3; ++k) Efq[(base + t) * 3 + k] += ef[t][k];
     }
     for (int k = 0; k < 3; ++k) dR[base * 3 + k] += dro[k];
+    for (int i = 0; i < 9; ++i) vt[i] *= 0.5;
     double *tl = tally + ((size_t)bb * N + a) * T_COUNT;
-    tl[T_UVDW] += uvdw;
+    tl[T_UVDW] += 0.5 * uvdw;
     vadd9(tl + T_VIRT, vt);
   }
 }
