@@ -1775,6 +1775,7 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
                                                    double *dR, double *tally) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
+  constexpr int FIN_SB_UNROLL = MODEL == 2 && sizeof(R) == 8 ? 1 : 4;
   extern __shared__ Site lds_site[];  // [4N] sites, then [N][ND][3] dipoles
   double *lds_dip = reinterpret_cast<double *>(lds_site + 4 * c.N);
   const int N = c.N;
@@ -1814,34 +1815,37 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
   auto visit = [&](const int j, auto thole_tag) {
     constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
-    Site B[4];
-    double mub[4][3];
-    for (int t = 0; t < 4; ++t) {
-      B[t] = bead_sites[j * 4 + t];
+    // one site of j at a time (read in the loop) against every site of a:
+    // each accumulator of a site of a still takes its terms in the order
+    // (j, site of j).  Unrolled except in the FP64 form of TTM2.1-F, where one
+    // site at a time keeps it at two waves per SIMD (TTM2.1-F 128 x 32: 2 %
+    // faster force calls; TTM3-F FP64 is faster unrolled).
+#pragma unroll FIN_SB_UNROLL
+    for (int sb = MODEL == 3 ? 1 : 0; sb < 4; ++sb) {
+      const bool db = is_dip<MODEL>(sb);
+      const Site Bs = bead_sites[j * 4 + sb];
+      double mubs[3];
       for (int k = 0; k < 3; ++k)
-        mub[t][k] = is_dip<MODEL>(t) ? bead_dip[((size_t)j * ND + (t - T0)) * 3 + k] : 0.0;
-    }
+        mubs[k] = db ? bead_dip[((size_t)j * ND + max(sb - T0, 0)) * 3 + k] : 0.0;
 #pragma unroll
-    for (int sa = 0; sa < 4; ++sa) {
-#pragma unroll
-      for (int sb = 0; sb < 4; ++sb) {
-        const bool da = is_dip<MODEL>(sa), db = is_dip<MODEL>(sb);
+      for (int sa = 0; sa < 4; ++sa) {
+        const bool da = is_dip<MODEL>(sa);
         // Pairs whose force and live potentials vanish identically: neither
         // site carries a dipole, or the only coupling is a zero charge.
         if (!da && !db) continue;
-        if (MODEL == 3 && (sa == 0 || sb == 0)) continue;
+        if (MODEL == 3 && sa == 0) continue;
         // site-site minimum image oriented from a to b (pot_ttm.f90:419-420);
         // v is the reference's orientation (lower molecule first)
-        double rd[3] = {A[sa].x - B[sb].x, A[sa].y - B[sb].y, A[sa].z - B[sb].z};
+        double rd[3] = {A[sa].x - Bs.x, A[sa].y - Bs.y, A[sa].z - Bs.z};
         minimg(c, rd[0], rd[1], rd[2]);
         const double *v = rd;
         const double d2 = norm2_exact(rd[0], rd[1], rd[2]);
         const R r[3] = {(R)rd[0], (R)rd[1], (R)rd[2]};
-        const R qa = (R)A[sa].q, qb = (R)B[sb].q;
+        const R qa = (R)A[sa].q, qb = (R)Bs.q;
         R di[3], dj[3];
         for (int k = 0; k < 3; ++k) {
           di[k] = (R)mua[sa][k];
-          dj[k] = (R)mub[sb][k];
+          dj[k] = (R)mubs[k];
         }
         R qd[3];
         for (int k = 0; k < 3; ++k) qd[k] = qb * di[k] - qa * dj[k];
