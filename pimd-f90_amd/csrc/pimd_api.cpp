@@ -527,8 +527,10 @@ int alloc_system(pimd_ctx *x) {
   };
   const int dflt_qq = pow2_clamp(131072 / std::max(mb, 1L), 8, 32);
   // the FP32 dipole pass runs lighter lanes: twice the lanes per molecule
-  // (512 H2O x 32 beads: 32 instead of 16, 1.685 -> 1.657 ms per force call)
-  const int dflt_dd = pow2_clamp((c.mixed ? 524288 : 262144) / std::max(mb, 1L), 16, 64);
+  // (512 H2O x 32 beads: 32 instead of 16, 1.685 -> 1.657 ms per force call);
+  // TTM2.1-F's three dipole sites at most 32 (128 x 32: 1.50 -> 1.37 ms FP64)
+  const int dflt_dd = pow2_clamp((c.mixed ? 524288 : 262144) / std::max(mb, 1L), 16,
+                                 c.model == 2 ? 32 : 64);
   // structure-factor site chunks: about 8 per bead from 4096 molecule-beads
   // on (with two lane sets per block; 512 H2O x 32 beads: 1.66 -> 1.61 ms per
   // mixed force call, TTM2.1-F 128 x 32: 1.75 -> 1.66 ms FP64), the staged

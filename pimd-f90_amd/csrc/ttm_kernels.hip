@@ -1494,6 +1494,12 @@ __global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restr
 // the neighbour loop reads LDS instead of gathering scattered records from
 // L2.  Arithmetic and summation order are those of k_pair_dd.
 constexpr int DD_BLOCK = 1024;
+// TTM2.1-F (three dipole sites, nine products per neighbour) in blocks of 256:
+// at 1024 lanes a block's registers are capped at 128 per lane and it spills
+template <int MODEL>
+constexpr int dd_block() {
+  return MODEL == 2 ? 256 : DD_BLOCK;
+}
 // doubles per staged dipole site: 6 used, one of padding so that lanes
 // gathering different sites spread over the LDS banks
 constexpr int DD_REC = 7;
@@ -1540,7 +1546,7 @@ __device__ bool sor_test(const SysConst &c, const double *__restrict__ diff2, in
 }
 
 template <int MODEL, int G, typename R>
-__global__ __launch_bounds__(DD_BLOCK) void k_pair_dd_lds(SysConst c, const Site *__restrict__ site,
+__global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(SysConst c, const Site *__restrict__ site,
                                                          const double *__restrict__ dip,
                                                          const int *__restrict__ nbr,
                                                          const int *__restrict__ nbrc,
@@ -1645,13 +1651,15 @@ static size_t dd_lds(const SysConst &c) { return sizeof(double) * DD_REC * c.N *
 template <int G, typename R>
 static bool launch_dd_lds_g(const SysConst &c, DevBufs &b, int test_iter, hipStream_t s) {
   if (dd_lds(c) > 64 * 1024) return false;
-  dim3 grid((c.N + DD_BLOCK / G - 1) / (DD_BLOCK / G), c.P);
+  constexpr int B2 = dd_block<2>(), B3 = dd_block<3>();
   if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_dd_lds<2, G, R>), grid, dim3(DD_BLOCK), dd_lds(c), s, c, b.site,
+    hipLaunchKernelGGL((k_pair_dd_lds<2, G, R>), dim3((c.N + B2 / G - 1) / (B2 / G), c.P), dim3(B2),
+                       dd_lds(c), s, c, b.site,
                        b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd, b.diff2, b.iters,
                        b.nactive, test_iter);
   else
-    hipLaunchKernelGGL((k_pair_dd_lds<3, G, R>), grid, dim3(DD_BLOCK), dd_lds(c), s, c, b.site,
+    hipLaunchKernelGGL((k_pair_dd_lds<3, G, R>), dim3((c.N + B3 / G - 1) / (B3 / G), c.P), dim3(B3),
+                       dd_lds(c), s, c, b.site,
                        b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd, b.diff2, b.iters,
                        b.nactive, test_iter);
   return true;
