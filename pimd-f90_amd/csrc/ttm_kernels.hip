@@ -799,6 +799,9 @@ __global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__rest
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
   constexpr int NV = MODE == SF_V ? 6 : 2;  // values per k
+  // sites per unrolled step: the grid, not the registers, limits the waves
+  // per SIMD, so more independent phase chains per wave hide FP64 latency
+  constexpr int SITE_UNROLL = MODE == SF_V ? 2 : 4;
   extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
   double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
   const int b = blockIdx.z;
@@ -842,7 +845,7 @@ __global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__rest
   for (int v = 0; v < NV; ++v)
 #pragma unroll
     for (int i = 0; i < KZ; ++i) acc[v][i] = 0.0;
-  #pragma unroll 2
+  #pragma unroll SITE_UNROLL
   for (int e = live ? e0 : e1; e < e1; ++e) {
     const double2 x = lds_sf[rowx * scp + e];
     double2 y = lds_sf[rowy * scp + e];
