@@ -756,14 +756,19 @@ void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s) {
 
 // Site index of the m-th member of a site set.  SET_CHG: H1, H2, M of every
 // molecule; SET_DIP: the dipole sites; SET_ALL: all four.
-enum SiteSet { SET_CHG = 0, SET_DIP = 1, SET_ALL = 2 };
+// SET_CHG_T / SET_ALL_T: the same sites as SET_CHG / SET_ALL, type-major
+// (all H1, then all H2, ...), so that a wave of the final reciprocal pass holds
+// one site type and its dipole-site branch is wave-uniform.
+enum SiteSet { SET_CHG = 0, SET_DIP = 1, SET_ALL = 2, SET_CHG_T = 3, SET_ALL_T = 4 };
 
 __device__ __forceinline__ int set_size(const SysConst &c, int set) {
-  return set == SET_CHG ? 3 * c.N : set == SET_DIP ? c.nd * c.N : 4 * c.N;
+  return set == SET_CHG || set == SET_CHG_T ? 3 * c.N : set == SET_DIP ? c.nd * c.N : 4 * c.N;
 }
 __device__ __forceinline__ int set_site(const SysConst &c, int set, int m) {
   if (set == SET_CHG) return (m / 3) * 4 + 1 + m % 3;
   if (set == SET_DIP) return (m / c.nd) * 4 + c.tdip + m % c.nd;
+  if (set == SET_CHG_T) return (m % c.N) * 4 + 1 + m / c.N;
+  if (set == SET_ALL_T) return (m % c.N) * 4 + m / c.N;
   return m;
 }
 
@@ -1123,7 +1128,9 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
                                                    const double *__restrict__ Efd,
                                                    double *__restrict__ diff2,
                                                    const int *__restrict__ active, int iter) {
-  constexpr int SET = MODE == FD_D ? SET_DIP : (MODEL == 2 ? SET_ALL : SET_CHG);
+  constexpr int SET = MODE == FD_D    ? SET_DIP
+                      : MODE == FD_QD ? (MODEL == 2 ? SET_ALL_T : SET_CHG_T)
+                                      : (MODEL == 2 ? SET_ALL : SET_CHG);
   // [T][64] phase tables of the block's sites (later reused for the
   // reduction), then with KL the bead's k data: [nk] (w Re S, w Im S, kz, w)
   // with w = 4 pi A_k / V, and for FD_QD [nk] (w Re(Sq+Sm), w Im(Sq+Sm))
