@@ -142,7 +142,10 @@ struct RM<float> {
   static __device__ __forceinline__ float exp_(float x) {
     return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
   }
-  static __device__ __forceinline__ float log_(float x) { return logf(x); }
+  // ln x = log2(x) ln 2 on the hardware log2 (x > 0 here: Q(2/3, x) for x > 0)
+  static __device__ __forceinline__ float log_(float x) {
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+  }
   static __device__ __forceinline__ float abs_(float x) { return fabsf(x); }
 };
 
