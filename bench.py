@@ -387,25 +387,37 @@ def replica_specs(n=64, seed=3000):
     return specs
 
 
+def replica_params(sp, precision=0):
+    """Model parameters of one replica of replica_specs()."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    L = make_system(sp["nwaters"], 1, seed=sp["seed"])[3]
+    return TTMParams(pot_model=sp["model"], nwaters=sp["nwaters"], nbeads=sp["nbeads"],
+                     box=(L, L, L), rc=-1.0, eps_ewald=1e-8, mass_o=15.995, mass_h=1.008,
+                     guess_initdip=True, precision=precision)
+
+
+def make_replica(sp, precision=0):
+    """One replica of replica_specs() loaded with its initial state and
+    gradient: (integrator, force field, RRt, PPt, dRRt)."""
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PIMDIntegrator, TTMForceField
+    RRt, PPt, RRc, _ = make_system(sp["nwaters"], sp["nbeads"], temp=sp["temp"], seed=sp["seed"])
+    ff = TTMForceField(replica_params(sp, precision))
+    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=sp["temp"], bead_thermostat=1,
+                           centroid_thermostat=True, tau_centroid=0.1)
+    f0 = ff.full_bead_forces(RRt, RRc)
+    integ.set_state(RRt, PPt, f0["dRRt"])
+    return integ, ff, RRt, PPt, f0["dRRt"]
+
+
 def run_replicas(args):
     """Aggregate throughput of 64 uneven replicas: run one after another with
     the default launch shapes, then side by side as one batch, untuned and
     autotuned.  Every run starts each replica from the same state."""
-    from pimd_amd.params import TTMParams
-    from pimd_amd.systems import make_system
-    from pimd_amd.ttm import PIMDIntegrator, ReplicaBatch, TTMForceField, load_library
+    from pimd_amd.ttm import ReplicaBatch, load_library
     specs = replica_specs()
-    reps = []
-    for sp in specs:
-        RRt, PPt, RRc, L = make_system(sp["nwaters"], sp["nbeads"], temp=sp["temp"], seed=sp["seed"])
-        p = TTMParams(pot_model=sp["model"], nwaters=sp["nwaters"], nbeads=sp["nbeads"],
-                      box=(L, L, L), rc=-1.0, eps_ewald=1e-8, mass_o=15.995, mass_h=1.008,
-                      guess_initdip=True, precision=1 if args.precision == "mixed" else 0)
-        ff = TTMForceField(p)
-        integ = PIMDIntegrator(ff, delt_fs=0.25, temp=sp["temp"], bead_thermostat=1,
-                               centroid_thermostat=True, tau_centroid=0.1)
-        f0 = ff.full_bead_forces(RRt, RRc)
-        reps.append((integ, ff, RRt, PPt, f0["dRRt"]))
+    reps = [make_replica(sp, 1 if args.precision == "mixed" else 0) for sp in specs]
     work = sum(sp["nwaters"] * sp["nbeads"] for sp in specs)
     seeds = [sp["seed"] for sp in specs]
     L = load_library()

@@ -94,13 +94,16 @@ int pimd_ctx_info(const pimd_ctx *ctx, double *out);
  *   virialct[P], dip_momIt[P][N][3], dip_momEt[P][N][3], chg[P][Natoms],
  *   iters[P] (SOR iterations run; negative when polar_maxiter was reached
  *   without convergence)                         out, any may be NULL
- * t is the MD step index passed to pot_ttm (drives the dipole predictor). */
+ * t is the MD step index passed to pot_ttm (drives the dipole predictor).
+ * Non-finite coordinates are refused with PIMD_EINVAL before anything runs. */
 int pimd_forces_all_beads(pimd_ctx *ctx, const double *RRt, const double *RRc, int t,
                           double *Upott, double *dRRt, double *virt, double *virialct,
                           double *dip_momIt, double *dip_momEt, double *chg, int *iters);
 
 /* Same computation on device-resident arrays (same shapes), enqueued on the
- * context's stream.  Outputs may be NULL.  Does not synchronise. */
+ * context's stream.  Outputs may be NULL.  Does not synchronise: non-finite
+ * coordinates (seen by the kernels) are reported as PIMD_EINVAL by the next
+ * pimd_sync, pimd_forces_all_beads or pimd_md_run of the context. */
 int pimd_forces_all_beads_dev(pimd_ctx *ctx, const double *d_RRt, const double *d_RRc, int t,
                               double *d_Upott, double *d_dRRt, double *d_virt,
                               double *d_virialct, double *d_dip_momIt, double *d_dip_momEt,
@@ -143,8 +146,10 @@ int pimd_nose_hoover(double *s, double *uk, double *vxi, double tau, double delt
 typedef struct {
   double delt_fs;        /* time step (fs)                                  */
   double temp;           /* K                                                */
-  int bead_thermostat;   /* 0 none, 1 Langevin (NM, PILE)                    */
-  int centroid_thermostat;
+  int bead_thermostat;   /* 0 none, 1 Langevin (NM, PILE, Langevin.f90:30-98),
+                            2 Nose-Hoover chains in normal-mode space
+                            (main_stuff.f90:145-171)                          */
+  int centroid_thermostat; /* thermostat the centroid mode too              */
   double tau_centroid;   /* ps                                               */
   int global_nh;         /* global Nose-Hoover chain (MD.f90:43-46)          */
   int intra_timesteps;   /* 0: PIMD_VV (MD.f90:35-110); n >= 1: contracted_MD
@@ -159,7 +164,9 @@ typedef struct {
  * reference's derived values (omegan = kB*T*P/hbar, delt2 = delt/2). */
 int pimd_md_init(pimd_ctx *ctx, const pimd_md_params *md);
 /* Loads RRt, PPt, dRRt [P][Natoms][3] into device state (dRRt may be NULL ->
- * zero, as the reference starts). */
+ * zero, as the reference starts).  Non-finite values: PIMD_EINVAL.  A run
+ * whose coordinates stop being finite (pimd_md_run, pimd_batch_md_run)
+ * returns PIMD_EINVAL at its end. */
 int pimd_state_set(pimd_ctx *ctx, const double *RRt, const double *PPt, const double *dRRt);
 /* Copies the device state out (any may be NULL); RRc [Natoms][3]. */
 int pimd_state_get(pimd_ctx *ctx, double *RRt, double *PPt, double *dRRt, double *RRc);
@@ -261,7 +268,8 @@ int pimd_profile_reset(pimd_ctx *ctx);
  * of known HBM traffic (bytes read, bytes written) for calibrating
  * counter-based byte counts; ms receives its duration. */
 int pimd_profile_copy(pimd_ctx *ctx, size_t bytes, double *ms);
-/* Blocks until all work queued on the context stream is done. */
+/* Blocks until all work queued on the context stream is done; PIMD_EINVAL
+ * if a force call since the last check saw non-finite coordinates. */
 int pimd_sync(pimd_ctx *ctx);
 
 #ifdef __cplusplus

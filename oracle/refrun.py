@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "_ref", "ref_driver")
+PIMD_BIN = os.path.join(HERE, "_ref", "PIMD_ref.x")   # the reference program, unmodified
 REF = "/root/reference"   # the reference sources (fixture generation only)
 
 
@@ -190,6 +191,9 @@ def trajectory(p, RRt, PPt, dRRt, nsteps, temp, delt_fs, bead_thermostat=1, cent
            "dRRt": r.take(P, na, 3), "RRc": r.take(na, 3), "iters": np.array(iters)}
     if global_nh:
         res["vxi_global"] = r.take(2)
+    if bead_thermostat == 2:
+        # vxi_beads(2, Natoms, Nbeads, 3) column-major, as [atom][bead][component][chain]
+        res["vxi_beads"] = r.take(3, P, na, 2).transpose(2, 1, 0, 3).copy()
     return res
 
 

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,17 +50,18 @@ struct pimd_ctx {
   pimd_params p{};
   SysConst c{};
   DevBufs d{};
+  void *arena = nullptr;  // the device block that holds the buffers of d
   hipStream_t stream = nullptr;
   bool stream_borrowed = false;  // the stream belongs to the context that owns this one
   // contracted MD: the centroid's intermolecular forces (one bead, no
   // monomer PES), computed on this context's stream
   pimd_ctx *cen = nullptr;
-  // second stream (made when first needed): real-space pair kernels run on
-  // it beside the reciprocal-space kernels they do not depend on (SOR
-  // overlap, graph branches); fork/join through the two events
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int group = 64;
+  // this context as a replica: its device record (in the arena), the
+  // ring-polymer constants of its MD step, and the group of one replica
+  // that every launch of a call on the context covers
+  Rep *rep_dev = nullptr;
+  MDConst m{};
+  Group self;
   int natoms = 0;
   std::vector<double> kvec, Ak, p4vAk;
   double polfac[4] = {0, 0, 0, 0};
@@ -75,13 +77,11 @@ struct pimd_ctx {
   bool graph_ok = false;
   bool graph_off = false;  // capture failed once: eager launches from then on
   uint64_t graph_seed = 0;
-  // the force call of the all-beads entry points, captured for its buffer
-  // addresses fkey and replayed while they and the model stay the same
+  // the force call of the all-beads entry points, captured once and
+  // replayed while the model stays the same
   hipGraphExec_t fgraph = nullptr;
   bool fgraph_ok = false, fgraph_off = false;
-  const void *fkey[10] = {};
   double omegan = 0.0;    // ring frequency kB T P / hbar of the MD temperature
-  double *d_vxi = nullptr, *d_s = nullptr, *d_step = nullptr;
   double *d_rec = nullptr;  // per-step records of md runs
   int rec_cap = 0;
   int rec_steps = 0;        // steps recorded by the last run
@@ -91,10 +91,6 @@ struct pimd_ctx {
   double *h_stage = nullptr;
   // profiling
   bool prof = false;
-  // inside a graph capture: independent real- and reciprocal-space kernels
-  // of a force call go on two streams (graph branches; in eager launches the
-  // event fork and join cost more than the overlap gains)
-  bool branches = false;
   std::vector<std::string> names;
   std::map<std::string, int> name_id;
   std::vector<long> launches;
@@ -175,15 +171,12 @@ void free_all(pimd_ctx *x) {
     delete x->cen;
     x->cen = nullptr;
   }
-  void *ptrs[] = {d.RR,   d.RRc,   d.site,   d.Rcs,    d.grdq,   d.phi,   d.Efq,   d.dR,
-                  d.dip,  d.dip_prev, d.Efd,   d.diff2,  d.tally,  d.hist,   d.nbr,   d.nbrc,  d.nbrn,  d.tab,
-                  d.Sq,   d.Sm,    d.Sv,     d.virk,   d.part,   d.active, d.iters, d.nactive, d.kv,
-                  d.khkl, d.cols,  d.colk,   d.segs,   d.segk,   d.Upot,   d.virt,   d.virialc, d.terms, d.dRR,
-                  d.dipI, d.dipE,  d.chg,    d.iters_out, d.C,  d.A,     d.msf,   d.c1,
-                  d.c2,   d.masses, d.PP,    d.noise,  d.uk,  d.ukpart, d.radius,     d.red, d.pes, d.adj, d.dRRfast, d.mol, d.tan, d.clk_own, d.obs_part, d.kE, d.kphi, d.cell_start, d.cell_mol, x->d_vxi, x->d_s,
-                  x->d_step};
+  // one block holds the buffers of layout_system; these are allocated apart
+  void *ptrs[] = {x->arena, d.dRRfast, d.mol, d.tan};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  x->arena = nullptr;
+  d = DevBufs{};
   if (x->h_nactive) (void)hipHostFree(x->h_nactive);
   if (x->h_stage) (void)hipHostFree(x->h_stage);
   for (auto &q : x->pending) {
@@ -194,10 +187,24 @@ void free_all(pimd_ctx *x) {
   if (x->graph) (void)hipGraphExecDestroy(x->graph);
   if (x->fgraph) (void)hipGraphExecDestroy(x->fgraph);
   if (x->stream && !x->stream_borrowed) (void)hipStreamDestroy(x->stream);
-  if (x->stream2) (void)hipStreamDestroy(x->stream2);
   if (x->d_rec) (void)hipFree(x->d_rec);
-  if (x->ev_fork) (void)hipEventDestroy(x->ev_fork);
-  if (x->ev_join) (void)hipEventDestroy(x->ev_join);
+}
+
+// The context's replica record (model constants, buffers, MD constants) onto
+// the device, after any change to them; waits for the context's queued work,
+// which reads the record.
+int sync_rep(pimd_ctx *x) {
+  Rep h;
+  h.c = x->c;
+  h.b = x->d;
+  h.m = x->m;
+  x->self.hc.assign(1, x->c);
+  x->self.hm.assign(1, x->m);
+  x->self.d = x->rep_dev;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpyAsync(x->rep_dev, &h, sizeof(Rep), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return PIMD_OK;
 }
 
 float f32(double v) { return (float)v; }
@@ -375,87 +382,28 @@ int env_int(const char *name, int dflt, int lo, int hi) {
   return n < lo ? lo : n > hi ? hi : n;
 }
 
-int alloc_system(pimd_ctx *x) {
+// Host-side tables of a system: k-vectors, kz-columns and their segments.
+struct HostTables {
+  std::vector<double4> kv;
+  std::vector<int4> kh, cols, segs;
+  std::vector<int> colk, segk;
+};
+
+// The k-vector tables, kz-columns and segments and the default launch shapes
+// of a set-up system (no device work).
+void plan_system(pimd_ctx *x, HostTables &t) {
   SysConst &c = x->c;
-  DevBufs &d = x->d;
-  const size_t P = c.P, N = c.N, S = 4 * N, na = 3 * N, nk = c.nk, nd = c.nd;
-  int rc;
-#define ALLOC(ptr, n) \
-  if ((rc = dalloc(&(ptr), (n))) != PIMD_OK) return rc
-  ALLOC(d.RR, P * na * 3);
-  ALLOC(d.RRc, na * 3);
-  ALLOC(d.site, P * S);
-  ALLOC(d.Rcs, S * 3);
-  ALLOC(d.grdq, P * N * 27);
-  ALLOC(d.phi, P * S);
-  ALLOC(d.Efq, P * S * 3);
-  ALLOC(d.kE, P * S * 3);
-  ALLOC(d.kphi, P * S);
-  ALLOC(d.dR, P * S * 3);
-  ALLOC(d.dip, P * N * nd * 3);
-  ALLOC(d.dip_prev, P * N * nd * 3);
-  ALLOC(d.Efd, P * N * nd * 3);
-  ALLOC(d.diff2, P * N * nd);
-  ALLOC(d.tally, P * N * T_COUNT);
-  ALLOC(d.hist, P * 4 * N * nd * 3);
-  ALLOC(d.nbr, P * N * (size_t)c.nbr_cap);
-  ALLOC(d.nbrc, P * N);
-  ALLOC(d.nbrn, P * N);
-  ALLOC(d.tab, P * (size_t)c.T * S);
-  ALLOC(d.Sq, P * nk * 2);
-  ALLOC(d.Sm, P * nk * 2);
-  ALLOC(d.Sv, P * nk * 6);
-  ALLOC(d.virk, P * ((nk + 255) / 256) * 18);
-  ALLOC(d.active, P);
-  ALLOC(d.iters, P);
-  ALLOC(d.nactive, (size_t)c.maxiter + 1);
-  ALLOC(d.kv, nk);
-  ALLOC(d.khkl, nk);
-  ALLOC(d.red, nk);  // A_k
-  ALLOC(d.Upot, P);
-  ALLOC(d.virt, P * 9);
-  ALLOC(d.virialc, P);
-  ALLOC(d.terms, P * 5);
-  ALLOC(d.dRR, P * na * 3);
-  ALLOC(d.dipI, P * N * 3);
-  ALLOC(d.dipE, P * N * 3);
-  ALLOC(d.chg, P * na);
-  ALLOC(d.iters_out, P);
-  ALLOC(d.C, P * P);
-  ALLOC(d.A, 4 * P);
-  ALLOC(d.msf, P);
-  ALLOC(d.c1, P);
-  ALLOC(d.c2, P);
-  ALLOC(d.masses, na);
-  ALLOC(d.PP, P * na * 3);
-  ALLOC(d.noise, 2 * na * P * 3);
-  ALLOC(d.uk, 1);
-  ALLOC(d.ukpart, 256);
-  ALLOC(d.radius, P * N);
-  if (c.cells) {
-    ALLOC(d.cell_start, P * ((size_t)c.ncell[0] * c.ncell[1] * c.ncell[2] + 1));
-    ALLOC(d.cell_mol, P * N);
-  }
-  ALLOC(d.pes, PES_NPARAM);
-  ALLOC(d.adj, 2 * P * na * 3 + P + (size_t)pes_adjoint_blocks(c) * PES_NPARAM + PES_NPARAM);
-  ALLOC(x->d_vxi, 2);
-  ALLOC(x->d_s, 1);
-  ALLOC(x->d_step, 4);
-#undef ALLOC
-  HIPCHK(hipMemset(d.hist, 0, sizeof(double) * P * 4 * N * nd * 3));
-  {
-    double theta[PES_NPARAM];
-    pes_default_params(theta);
-    HIPCHK(hipMemcpy(d.pes, theta, sizeof theta, hipMemcpyHostToDevice));
-  }
+  const size_t nk = c.nk;
   // k-vectors and kz-columns
-  std::vector<double4> kv(nk);
-  std::vector<int4> kh(nk);
-  std::vector<int4> cols;
-  std::vector<int> colk;
+  t.kv.resize(nk);
+  t.kh.resize(nk);
+  std::vector<int4> &cols = t.cols;
+  std::vector<int> &colk = t.colk;
+  cols.clear();
+  colk.clear();
   for (size_t k = 0; k < nk; ++k) {
-    kv[k] = make_double4(x->kvec[3 * k], x->kvec[3 * k + 1], x->kvec[3 * k + 2], x->p4vAk[k]);
-    kh[k] = make_int4(x->hkl[3 * k], x->hkl[3 * k + 1], x->hkl[3 * k + 2], 0);
+    t.kv[k] = make_double4(x->kvec[3 * k], x->kvec[3 * k + 1], x->kvec[3 * k + 2], x->p4vAk[k]);
+    t.kh[k] = make_int4(x->hkl[3 * k], x->hkl[3 * k + 1], x->hkl[3 * k + 2], 0);
     const bool cont = k > 0 && x->hkl[3 * k] == x->hkl[3 * k - 3] &&
                       x->hkl[3 * k + 1] == x->hkl[3 * k - 2] &&
                       x->hkl[3 * k + 2] == x->hkl[3 * k - 1] + 1;
@@ -487,8 +435,10 @@ int alloc_system(pimd_ctx *x) {
   for (auto &cl : cols) c.kzmax = std::max(c.kzmax, cl.w);
   // segments for the structure-factor kernel: the columns cut into pieces of
   // at most segmax consecutive kz, longest first
-  std::vector<int4> segs;
-  std::vector<int> segk;
+  std::vector<int4> &segs = t.segs;
+  std::vector<int> &segk = t.segk;
+  segs.clear();
+  segk.clear();
   {
     const int seg = env_int("PIMD_SFAC_SEG", 8, 1, 32);
     for (size_t i = 0; i < cols.size(); ++i)
@@ -552,28 +502,143 @@ int alloc_system(pimd_ctx *x) {
   c.g_qq = env_int("PIMD_G_QQ", dflt_qq, 0, 32);
   c.g_fin = env_int("PIMD_G_FIN", dflt_qq, 0, 32);
   c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
-  c.overlap = env_int("PIMD_SOR_OVERLAP", 0, 0, 1);
+  c.overlap = 0;
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
-  int rc2;
-  if ((rc2 = dalloc(&d.part, P * (size_t)c.sf_chunks_max * nk * 6)) != PIMD_OK) return rc2;
-  if ((rc2 = dalloc(&d.cols, cols.size())) != PIMD_OK) return rc2;
-  if ((rc2 = dalloc(&d.colk, colk.size())) != PIMD_OK) return rc2;
-  HIPCHK(hipMemcpy(d.kv, kv.data(), sizeof(double4) * nk, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d.khkl, kh.data(), sizeof(int4) * nk, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d.red, x->Ak.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d.cols, cols.data(), sizeof(int4) * cols.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d.colk, colk.data(), sizeof(int) * colk.size(), hipMemcpyHostToDevice));
-  if ((rc2 = dalloc(&d.segs, segs.size())) != PIMD_OK) return rc2;
-  if ((rc2 = dalloc(&d.segk, segk.size())) != PIMD_OK) return rc2;
-  HIPCHK(hipMemcpy(d.segs, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d.segk, segk.data(), sizeof(int) * segk.size(), hipMemcpyHostToDevice));
-  // the device step clock {t, record slot}
-  if ((rc2 = dalloc(&d.obs_part, 64 * 5)) != PIMD_OK) return rc2;
-  if ((rc2 = dalloc(&d.clk_own, 2)) != PIMD_OK) return rc2;
+}
+
+// Device buffers of a context laid out in one block, each 256-byte aligned.
+// With base == nullptr only the size is counted.
+struct Arena {
+  char *base = nullptr;
+  size_t off = 0;
+  template <class T>
+  void take(T **p, size_t n) {
+    off = (off + 255) & ~(size_t)255;
+    if (base) *p = reinterpret_cast<T *>(base + off);
+    off += std::max<size_t>(n, 1) * sizeof(T);
+  }
+};
+
+// full: every buffer of a context (force call, ring polymer, MD, adjoint);
+// otherwise only what one force call on caller-provided coordinates uses.
+void layout_system(pimd_ctx *x, Arena &a, bool full) {
+  SysConst &c = x->c;
+  DevBufs &d = x->d;
+  const size_t P = c.P, N = c.N, S = 4 * N, na = 3 * N, nk = c.nk, nd = c.nd;
+  a.take(&d.RR, P * na * 3);
+  a.take(&d.RRc, na * 3);
+  a.take(&d.site, P * S);
+  a.take(&d.Rcs, S * 3);
+  a.take(&d.grdq, P * N * 27);
+  a.take(&d.phi, P * S);
+  a.take(&d.Efq, P * S * 3);
+  a.take(&d.kE, P * S * 3);
+  a.take(&d.kphi, P * S);
+  a.take(&d.dR, P * S * 3);
+  a.take(&d.dip, P * N * nd * 3);
+  a.take(&d.dip_prev, P * N * nd * 3);
+  a.take(&d.Efd, P * N * nd * 3);
+  a.take(&d.diff2, P * N * nd);
+  a.take(&d.tally, P * N * T_COUNT);
+  a.take(&d.hist, P * 4 * N * nd * 3);
+  a.take(&d.nbr, P * N * (size_t)c.nbr_cap);
+  a.take(&d.nbrc, P * N);
+  a.take(&d.nbrn, P * N);
+  a.take(&d.tab, P * (size_t)c.T * S);
+  a.take(&d.Sq, P * nk * 2);
+  a.take(&d.Sm, P * nk * 2);
+  a.take(&d.Sv, P * nk * 6);
+  a.take(&d.virk, P * ((nk + 255) / 256) * 18);
+  a.take(&d.part, P * (size_t)c.sf_chunks_max * nk * 6);
+  a.take(&d.active, P);
+  a.take(&d.iters, P);
+  a.take(&d.nactive, (size_t)c.maxiter + 1);
+  a.take(&d.kv, nk);
+  a.take(&d.khkl, nk);
+  a.take(&d.red, nk);  // A_k
+  a.take(&d.cols, c.ncol);
+  a.take(&d.colk, c.ncol);
+  a.take(&d.segs, c.nseg);
+  a.take(&d.segk, c.nseg);
+  a.take(&d.Upot, P);
+  a.take(&d.virt, P * 9);
+  a.take(&d.virialc, P);
+  a.take(&d.terms, P * 5);
+  a.take(&d.dRR, P * na * 3);
+  a.take(&d.dipI, P * N * 3);
+  a.take(&d.dipE, P * N * 3);
+  a.take(&d.chg, P * na);
+  a.take(&d.iters_out, P);
+  a.take(&d.radius, P * N);
+  if (c.cells) {
+    a.take(&d.cell_start, P * ((size_t)c.ncell[0] * c.ncell[1] * c.ncell[2] + 1));
+    a.take(&d.cell_mol, P * N);
+  }
+  a.take(&d.pes, PES_NPARAM);
+  a.take(&d.clk_own, 2);
+  a.take(&d.flags, 1);
+  a.take(&x->rep_dev, 1);
+  if (!full) return;
+  a.take(&d.C, P * P);
+  a.take(&d.A, 4 * P);
+  a.take(&d.msf, P);
+  a.take(&d.c1, P);
+  a.take(&d.c2, P);
+  a.take(&d.masses, na);
+  a.take(&d.PP, P * na * 3);
+  a.take(&d.noise, 2 * na * P * 3);
+  a.take(&d.uk, 1);
+  a.take(&d.ukpart, 256);
+  a.take(&d.adj, 2 * P * na * 3 + P + (size_t)pes_adjoint_blocks(c) * PES_NPARAM + PES_NPARAM);
+  a.take(&d.obs_part, 64 * 5);
+  a.take(&d.vxi_beads, na * P * 3 * 2);
+  a.take(&d.nh_tau, P);
+  a.take(&d.nh_vxi, 2);
+  a.take(&d.nh_s, 1);
+}
+
+// Copies the host tables and the reference's PES parameters into the laid-out
+// buffers on stream s and sets the step clock to zero.
+int upload_system(pimd_ctx *x, const HostTables &t, hipStream_t s) {
+  const SysConst &c = x->c;
+  DevBufs &d = x->d;
+  const size_t nk = c.nk;
+  double theta[PES_NPARAM];
+  pes_default_params(theta);
+  HIPCHK(hipMemcpyAsync(d.pes, theta, sizeof theta, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.kv, t.kv.data(), sizeof(double4) * nk, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.khkl, t.kh.data(), sizeof(int4) * nk, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.red, x->Ak.data(), sizeof(double) * nk, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.cols, t.cols.data(), sizeof(int4) * t.cols.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.colk, t.colk.data(), sizeof(int) * t.colk.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.segs, t.segs.data(), sizeof(int4) * t.segs.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.segk, t.segk.data(), sizeof(int) * t.segk.size(), hipMemcpyHostToDevice, s));
   d.clk = d.clk_own;
-  HIPCHK(hipMemset(d.clk, 0, sizeof(int) * 2));
+  HIPCHK(hipMemsetAsync(d.clk_own, 0, 2 * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(d.flags, 0, sizeof(int), s));
+  HIPCHK(hipGetLastError());
+  // the host tables are pageable: done before they go out of scope
+  HIPCHK(hipStreamSynchronize(s));
+  return PIMD_OK;
+}
+
+int alloc_system(pimd_ctx *x) {
+  SysConst &c = x->c;
+  DevBufs &d = x->d;
+  HostTables t;
+  plan_system(x, t);
+  Arena a;
+  layout_system(x, a, true);
+  int rc;
+  if ((rc = dalloc((char **)&x->arena, a.off)) != PIMD_OK) return rc;
+  a.base = static_cast<char *>(x->arena);
+  a.off = 0;
+  layout_system(x, a, true);
+  const size_t P = c.P, N = c.N, na = 3 * N, nd = c.nd;
+  if ((rc = upload_system(x, t, x->stream)) != PIMD_OK) return rc;
+  HIPCHK(hipMemset(d.hist, 0, sizeof(double) * P * 4 * N * nd * 3));
   std::vector<double> masses(na);
-  for (size_t a = 0; a < na; ++a) masses[a] = (a % 3 == 0) ? x->p.mass_o : x->p.mass_h;
+  for (size_t i = 0; i < na; ++i) masses[i] = (i % 3 == 0) ? x->p.mass_o : x->p.mass_h;
   HIPCHK(hipMemcpy(d.masses, masses.data(), sizeof(double) * na, hipMemcpyHostToDevice));
   HIPCHK(hipHostMalloc((void **)&x->h_nactive, sizeof(int) * (c.maxiter + 1), hipHostMallocDefault));
   return PIMD_OK;
@@ -588,192 +653,125 @@ int pick_group(const SysConst &c) {
   return 16;
 }
 
-// The second stream, made on first use (outside any capture: by the caller
-// before a capture begins).
-int ensure_stream2(pimd_ctx *x) {
-  if (x->stream2) return PIMD_OK;
-  if (hipStreamCreateWithFlags(&x->stream2, hipStreamNonBlocking) != hipSuccess)
-    return fail(PIMD_EDEVICE, "hipStreamCreate failed");
-  return PIMD_OK;
+// Per-kernel timing hook: px (may be NULL) records each launch's time when
+// its profiling is on.
+template <class F>
+void timed(pimd_ctx *px, const char *name, F &&f) {
+  if (px) px->timed(name, f);
+  else f();
 }
 
-// a(stream2) beside b(s); s waits for both.
-template <class A, class B>
-int beside(pimd_ctx *x, hipStream_t s, A &&a, B &&b) {
-  if (!x->stream2) return fail(PIMD_ESTATE, "second stream missing");
-  HIPCHK(hipEventRecord(x->ev_fork, s));
-  HIPCHK(hipStreamWaitEvent(x->stream2, x->ev_fork, 0));
-  a(x->stream2);
-  HIPCHK(hipEventRecord(x->ev_join, x->stream2));
-  b(s);
-  HIPCHK(hipStreamWaitEvent(s, x->ev_join, 0));
-  return PIMD_OK;
-}
-
-// The real-space dipole field on the second stream beside the dipole
-// structure factor on the first; the first stream waits for both.
-int dd_beside_sfac(pimd_ctx *x, hipStream_t s) {
-  return beside(x, s, [&](hipStream_t s2) { launch_pair_dd(x->c, x->d, x->group, false, s2); },
-                [&](hipStream_t s1) { launch_sfac(x->c, x->d, SF_D, s1); });
-}
-
-__global__ void k_zero_int(int *v, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = 0;
-}
-
-// One pot_ttm evaluation per bead on device arrays (pot_ttm.f90:3-630).
-// The MD step t of the call (dipole predictor, history slot) is read from the
-// device clock d.clk, set by the caller.
-int run_forces(pimd_ctx *x, const double *RR, const double *RRc, double *Upot,
-               double *dRR, double *virt, double *virialc, double *dipI, double *dipE,
-               double *chg, int *iters) {
-  const SysConst &c = x->c;
-  DevBufs &d = x->d;
-  hipStream_t s = x->stream;
-  const bool two = x->branches && !x->prof;
-  if (c.overlap && !x->stream2) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    HIPCHK(hipStreamIsCapturing(s, &cs));
-    if (cs != hipStreamCaptureStatusNone) return fail(PIMD_ESTATE, "second stream missing");
-    const int err = ensure_stream2(x);
-    if (err != PIMD_OK) return err;
+// One pot_ttm evaluation per bead of every replica of the group
+// (pot_ttm.f90:3-630; pot_spc.f90:1-277 for the fixed-charge models), each
+// from its coordinates RR, RRc into its output buffers.  The MD step t of the
+// call (dipole predictor, history slot) comes from each replica's device
+// clock, set by the caller.  px: the context whose profiling and SOR polling
+// apply (a group of that one replica), or NULL.
+int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
+  const int n = g.n();
+  bool any_ttm = false, all_fold = true, any_mixed = false;
+  int maxiter = 0;
+  for (const SysConst &c : g.hc) {
+    if (c.model > 3) continue;
+    any_ttm = true;
+    maxiter = std::max(maxiter, c.maxiter);
+    all_fold = all_fold && pair_dd_tests(c);
+    any_mixed = any_mixed || c.mixed;
   }
-  // the reciprocal-space charge pass, beside the real-space pair pass when
-  // two streams are in use (it writes kE / kphi, read after the join)
-  auto kspace_q = [&](hipStream_t st) {
-    x->timed("ewald_tables", [&] { launch_tables(c, d, st); });
-    x->timed("sfac_q", [&] { launch_sfac(c, d, SF_Q, st); });
-    x->timed("field_q", [&] { launch_field(c, d, FD_Q, 0, st); });
-  };
-  if (c.model >= PIMD_QSPCFW) {  // pot_spc (pot_spc.f90:1-277)
-    x->timed("monomer", [&] { launch_spc_monomer(c, d, RR, chg, s); });
-    x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
-    auto pair = [&](hipStream_t st) { x->timed("pair_qq", [&] { launch_spc_pair(c, d, st); }); };
-    if (two) {
-      const int err = beside(x, s, pair, kspace_q);
-      if (err != PIMD_OK) return err;
-    } else {
-      pair(s);
-      kspace_q(s);
+  const bool prof = px && px->prof;
+  const bool poll = px && (px->c.poll || prof) && n == 1;
+  timed(px, "monomer", [&] {
+    launch_monomer(g, s);
+    launch_spc_monomer(g, s);
+  });
+  timed(px, "neighbors", [&] { launch_neighbors(g, s); });
+  timed(px, "pair_qq", [&] {
+    launch_pair_qq(g, s);
+    launch_spc_pair(g, s);
+  });
+  // the reciprocal-space charge pass (it writes kE / kphi, added by their consumers)
+  timed(px, "ewald_tables", [&] { launch_tables(g, s); });
+  timed(px, "sfac_q", [&] { launch_sfac(g, SF_Q, s); });
+  timed(px, "field_q", [&] { launch_field(g, FD_Q, 0, s); });
+  if (any_ttm) {
+    timed(px, "post_qq", [&] { launch_post_qq(g, s); });
+    // the convergence test of iteration it - 1 runs at the start of the
+    // dipole pair kernel of iteration it, where every replica's dipole
+    // kernel can hold it; the last iteration's test stays a launch
+    const bool fold = all_fold && !poll;
+    for (int it = 1; it <= maxiter; ++it) {
+      timed(px, "pair_dd", [&] { launch_pair_dd(g, false, s, fold && it > 1 ? it - 1 : 0); });
+      timed(px, "sfac_d", [&] { launch_sfac(g, SF_D, s); });
+      timed(px, "field_d_sor", [&] { launch_field(g, FD_D, it, s); });
+      if (!fold || it == maxiter) timed(px, "converge", [&] { launch_converge(g, it, s); });
+      // No host round trip: converged beads are switched off on the device
+      // and every kernel of the loop skips them, so the remaining iterations
+      // of a converged system are near-empty launches.  (With per-kernel
+      // timing on, poll so only live iterations are launched and timed.)
+      if (poll && it < maxiter) {
+        HIPCHK(hipMemcpyAsync(px->h_nactive + it, px->d.nactive + it, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (px->h_nactive[it] == 0) break;
+      }
     }
-    x->timed("virial_k", [&] { launch_virk(c, d, s); });
-    x->timed("finalize", [&] { launch_spc_finalize(c, d, dRR, dipI, dipE, s); });
-    x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, nullptr, s); });
-    // no SOR loop: zero iterations
-    hipLaunchKernelGGL(k_zero_int, dim3((c.P + 63) / 64), dim3(64), 0, s, iters, c.P);
-    HIPCHK(hipGetLastError());
-    return PIMD_OK;
-  }
-  x->timed("monomer", [&] { launch_monomer(c, d, RR, RRc, chg, s); });
-  x->timed("neighbors", [&] { launch_neighbors(c, d, s); });
-  {
-    auto pair = [&](hipStream_t st) { x->timed("pair_qq", [&] { launch_pair_qq(c, d, x->group, st); }); };
-    if (two) {
-      const int err = beside(x, s, pair, kspace_q);
-      if (err != PIMD_OK) return err;
-    } else {
-      pair(s);
-      kspace_q(s);
+    if (any_mixed) {
+      // Iterative refinement of the mixed form: redo each bead's last SOR
+      // step in FP64 from the dipoles it started from.  The reciprocal-space
+      // dipole structure factor of that step is still in place (beads stop
+      // updating it once converged).
+      timed(px, "refine_dd", [&] {
+        launch_refine_prep(g, s);
+        launch_pair_dd(g, true, s, 0, true);
+        launch_field(g, FD_D, -1, s, true);
+      });
     }
+    timed(px, "pair_final", [&] { launch_pair_final(g, s); });
+    timed(px, "sfac_v", [&] { launch_sfac(g, SF_V, s); });
+    timed(px, "field_qd", [&] { launch_field(g, FD_QD, 0, s); });
   }
-  x->timed("post_qq", [&] { launch_post_qq(c, d, s); });
-  // the convergence test of iteration it - 1 runs at the start of the
-  // dipole pair kernel of iteration it, where that kernel can hold it (and
-  // nothing runs beside it); the last iteration's test stays a launch
-  const bool fold = pair_dd_tests(c) && !c.overlap && !two && !c.poll && !x->prof;
-  for (int it = 1; it <= c.maxiter; ++it) {
-    if (c.overlap || two) {
-      // pair_dd on the second stream, concurrent with the structure factor
-      int err = PIMD_OK;
-      x->timed("dd_sfac_d", [&] { err = dd_beside_sfac(x, s); });
-      if (err != PIMD_OK) return err;
-    } else {
-      x->timed("pair_dd", [&] { launch_pair_dd(c, d, x->group, false, s, fold && it > 1 ? it - 1 : 0); });
-      x->timed("sfac_d", [&] { launch_sfac(c, d, SF_D, s); });
-    }
-    x->timed("field_d_sor", [&] { launch_field(c, d, FD_D, it, s); });
-    if (!fold || it == c.maxiter) x->timed("converge", [&] { launch_converge(c, d, it, s); });
-    // No host round trip: converged beads are switched off on the device
-    // and every kernel of the loop skips them, so the remaining iterations
-    // of a converged system are near-empty launches.
-    // (with per-kernel timing on, poll so only live iterations are launched
-    // and timed)
-    if ((c.poll || x->prof) && it < c.maxiter) {
-      HIPCHK(hipMemcpyAsync(x->h_nactive + it, d.nactive + it, sizeof(int), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      if (x->h_nactive[it] == 0) break;
-    }
-  }
-  if (c.mixed) {
-    // Iterative refinement: redo each bead's last SOR step in FP64 from the
-    // dipoles it started from.  The reciprocal-space dipole structure factor
-    // of that step is still in place (beads stop updating it once converged).
-    x->timed("refine_dd", [&] {
-      launch_refine_prep(c, d, s);
-      launch_pair_dd(c, d, x->group, true, s);
-      launch_field(c, d, FD_D, c.maxiter + 1, s);
-    });
-  }
-  {
-    // the final real-space pass beside the reciprocal-space forces and virial
-    auto pair = [&](hipStream_t st) { x->timed("pair_final", [&] { launch_pair_final(c, d, x->group, st); }); };
-    auto kspace = [&](hipStream_t st) {
-      x->timed("sfac_v", [&] { launch_sfac(c, d, SF_V, st); });
-      x->timed("field_qd", [&] { launch_field(c, d, FD_QD, 0, st); });
-      x->timed("virial_k", [&] { launch_virk(c, d, st); });
-    };
-    if (two) {
-      const int err = beside(x, s, pair, kspace);
-      if (err != PIMD_OK) return err;
-    } else {
-      pair(s);
-      kspace(s);
-    }
-  }
-  x->timed("finalize", [&] { launch_finalize(c, d, dRR, dipI, dipE, s); });
-  x->timed("reduce_bead", [&] { launch_reduce_bead(c, d, Upot, virt, virialc, iters, s); });
+  timed(px, "virial_k", [&] { launch_virk(g, s); });
+  timed(px, "finalize", [&] {
+    launch_finalize(g, s);
+    launch_spc_finalize(g, s);
+  });
+  timed(px, "reduce_bead", [&] { launch_reduce_bead(g, s); });
   HIPCHK(hipGetLastError());
   return PIMD_OK;
 }
 
+int run_forces(pimd_ctx *x) { return run_forces(x->self, x->stream, x); }
 
-// run_forces as one replay of a captured graph, captured again when the
-// buffer addresses change (or after invalidate_graphs).  Per-kernel timing,
-// polling and the two-stream overlap keep eager launches, as does a device
-// where capture fails.  PIMD_FORCE_GRAPH=0 turns replay off.
-int force_call(pimd_ctx *x, const double *RR, const double *RRc, double *Upot, double *dRR,
-               double *virt, double *virialc, double *dipI, double *dipE, double *chg,
-               int *iters) {
+// run_forces of a context as one replay of a captured graph, captured again
+// after invalidate_graphs.  Per-kernel timing and polling keep eager
+// launches, as does a device where capture fails.  PIMD_FORCE_GRAPH=0 turns
+// replay off.
+int force_call(pimd_ctx *x) {
   const SysConst &c = x->c;
   const bool on = env_int("PIMD_FORCE_GRAPH", 1, 0, 1) != 0;
-  if (!on || x->fgraph_off || x->prof || c.poll || c.overlap)
-    return run_forces(x, RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters);
-  const void *key[10] = {RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters};
-  if (!x->fgraph_ok || !std::equal(key, key + 10, x->fkey)) {
+  if (!on || x->fgraph_off || x->prof || c.poll) return run_forces(x);
+  if (!x->fgraph_ok) {
     if (x->fgraph) (void)hipGraphExecDestroy(x->fgraph);
     x->fgraph = nullptr;
-    x->fgraph_ok = false;
     hipStream_t s = x->stream;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int rc = run_forces(x, RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &g);
+    const int rc = run_forces(x);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &gr);
     if (rc != PIMD_OK) {
-      if (g) (void)hipGraphDestroy(g);
+      if (gr) (void)hipGraphDestroy(gr);
       return rc;
     }
     hipError_t e2 = e;
-    if (e == hipSuccess) e2 = hipGraphInstantiate(&x->fgraph, g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
+    if (e == hipSuccess) e2 = hipGraphInstantiate(&x->fgraph, gr, nullptr, nullptr, 0);
+    if (gr) (void)hipGraphDestroy(gr);
     if (e2 != hipSuccess) {
       // capture is not available here: launch eagerly from now on
       x->fgraph = nullptr;
       x->fgraph_off = true;
       (void)hipGetLastError();
-      return run_forces(x, RR, RRc, Upot, dRR, virt, virialc, dipI, dipE, chg, iters);
+      return run_forces(x);
     }
-    std::copy(key, key + 10, x->fkey);
     x->fgraph_ok = true;
   }
   HIPCHK(hipGraphLaunch(x->fgraph, x->stream));
@@ -781,18 +779,11 @@ int force_call(pimd_ctx *x, const double *RR, const double *RRc, double *Upot, d
 }
 }  // namespace
 
-extern "C" {
-
-int pimd_last_error(char *buf, size_t len) {
-  if (!buf || len == 0) return PIMD_EINVAL;
-  snprintf(buf, len, "%s", g_err.c_str());
-  return PIMD_OK;
-}
-
-int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
-  if (!params || !out) return fail(PIMD_EINVAL, "null argument");
-  *out = nullptr;
-  pimd_params p = *params;
+namespace {
+// Checks run parameters and fills the reference's derived defaults
+// (InputOutput.f90:118-125, input_file_reader.f90:104-105); makes the
+// device current.
+int check_params(pimd_params &p) {
   if (p.pot_model < PIMD_TTM21F || p.pot_model > PIMD_SPCF)
     return fail(PIMD_EINVAL, "pot_model must be 2 (TTM2.1-F), 3 (TTM3-F), 4 (q-SPC/Fw) or "
                 "5 (SPC/F), got %d", p.pot_model);
@@ -822,6 +813,50 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
     return fail(PIMD_EDEVICE, "no HIP device available");
   if (p.device < 0 || p.device >= ndev) return fail(PIMD_EINVAL, "device %d out of range", p.device);
   if (hipSetDevice(p.device) != hipSuccess) return fail(PIMD_EDEVICE, "hipSetDevice failed");
+  return PIMD_OK;
+}
+
+}  // namespace
+
+namespace {
+// Non-finite coordinates in host arrays: refused before anything runs.
+int check_finite(const double *a, size_t n, const char *what) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return fail(PIMD_EINVAL, "%s[%zu] is not finite", what, i);
+  return PIMD_OK;
+}
+
+// The device flag of non-finite coordinates in force calls since the last
+// check (the monomer kernels set it), read and cleared; the context's stream
+// must be synchronised.
+int take_flags(pimd_ctx *x) {
+  for (pimd_ctx *y : {x, x->cen}) {
+    if (!y || !y->d.flags) continue;
+    int f = 0;
+    HIPCHK(hipMemcpy(&f, y->d.flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (f) {
+      HIPCHK(hipMemset(y->d.flags, 0, sizeof(int)));
+      return fail(PIMD_EINVAL, "non-finite coordinates entered a force call");
+    }
+  }
+  return PIMD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pimd_last_error(char *buf, size_t len) {
+  if (!buf || len == 0) return PIMD_EINVAL;
+  snprintf(buf, len, "%s", g_err.c_str());
+  return PIMD_OK;
+}
+
+int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
+  if (!params || !out) return fail(PIMD_EINVAL, "null argument");
+  *out = nullptr;
+  pimd_params p = *params;
+  int rc0 = check_params(p);
+  if (rc0 != PIMD_OK) return rc0;
   pimd_ctx *x = new pimd_ctx();
   x->p = p;
   x->natoms = 3 * p.nwaters;
@@ -830,12 +865,16 @@ int pimd_ctx_create(const pimd_params *params, pimd_ctx **out) {
   // context of a batch then gets a hardware queue of its own
   if (rc == PIMD_OK && hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(PIMD_EDEVICE, "hipStreamCreate failed");
-  if (rc == PIMD_OK && (hipEventCreateWithFlags(&x->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                        hipEventCreateWithFlags(&x->ev_join, hipEventDisableTiming) != hipSuccess))
-    rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
-  if (rc == PIMD_OK) rc = alloc_system(x);
   if (rc == PIMD_OK) {
-    x->group = pick_group(x->c);
+    x->c.g_nolds = pick_group(x->c);
+    rc = alloc_system(x);
+  }
+  if (rc == PIMD_OK) {
+    x->m.massO = p.mass_o;
+    x->m.massH = p.mass_h;
+    x->m.box0 = p.box[0];
+    x->m.boxi0 = 1.0 / p.box[0];
+    rc = sync_rep(x);
   }
   if (rc != PIMD_OK) {
     free_all(x);
@@ -877,10 +916,30 @@ int pimd_forces_all_beads_dev(pimd_ctx *x, const double *RR, const double *RRc, 
   if (!x || !RR || !RRc) return fail(PIMD_EINVAL, "null argument");
   HIPCHK(hipSetDevice(x->p.device));
   DevBufs &d = x->d;
-  launch_set_clock(d.clk, t, 0, x->stream);
-  return force_call(x, RR, RRc, Upot ? Upot : d.Upot, dRR ? dRR : d.dRR,
-                    virt ? virt : d.virt, virialc ? virialc : d.virialc, dipI ? dipI : d.dipI,
-                    dipE ? dipE : d.dipE, chg ? chg : d.chg, iters ? iters : d.iters_out);
+  hipStream_t s = x->stream;
+  const size_t P = x->c.P, na = x->natoms, N = x->c.N;
+  // the call works on the context's own coordinate and output buffers (those
+  // of its MD state); the caller's arrays are copied in and out on the stream
+  if (RR != d.RR) HIPCHK(hipMemcpyAsync(d.RR, RR, sizeof(double) * P * na * 3, hipMemcpyDeviceToDevice, s));
+  if (RRc != d.RRc) HIPCHK(hipMemcpyAsync(d.RRc, RRc, sizeof(double) * na * 3, hipMemcpyDeviceToDevice, s));
+  launch_set_clock(x->self, t, 0, s);
+  int rc = force_call(x);
+  if (rc != PIMD_OK) return rc;
+  struct Out {
+    void *dst;
+    const void *src;
+    size_t bytes;
+  } outs[] = {{Upot, d.Upot, sizeof(double) * P},
+              {dRR, d.dRR, sizeof(double) * P * na * 3},
+              {virt, d.virt, sizeof(double) * P * 9},
+              {virialc, d.virialc, sizeof(double) * P},
+              {dipI, d.dipI, sizeof(double) * P * N * 3},
+              {dipE, d.dipE, sizeof(double) * P * N * 3},
+              {chg, d.chg, sizeof(double) * P * na},
+              {iters, d.iters_out, sizeof(int) * P}};
+  for (const Out &o : outs)
+    if (o.dst && o.dst != o.src) HIPCHK(hipMemcpyAsync(o.dst, o.src, o.bytes, hipMemcpyDeviceToDevice, s));
+  return PIMD_OK;
 }
 
 int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int t,
@@ -889,6 +948,10 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   if (!x || !RRt || !RRc) return fail(PIMD_EINVAL, "null argument");
   HIPCHK(hipSetDevice(x->p.device));
   const size_t P = x->c.P, na = x->natoms, N = x->c.N;
+  int rc0;
+  if ((rc0 = check_finite(RRt, P * na * 3, "RRt")) != PIMD_OK ||
+      (rc0 = check_finite(RRc, na * 3, "RRc")) != PIMD_OK)
+    return rc0;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
   // the caller's arrays go through a pinned staging block: DMA copies at
@@ -904,9 +967,8 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   std::memcpy(h + nR, RRc, sizeof(double) * nc);
   HIPCHK(hipMemcpyAsync(d.RR, h, sizeof(double) * nR, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.RRc, h + nR, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-  launch_set_clock(d.clk, t, 0, s);
-  int rc = force_call(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
-                      d.iters_out);
+  launch_set_clock(x->self, t, 0, s);
+  int rc = force_call(x);
   if (rc != PIMD_OK) return rc;
   struct Out {
     void *dst;
@@ -925,7 +987,7 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   HIPCHK(hipStreamSynchronize(s));
   for (const Out &o : outs)
     if (o.dst) std::memcpy(o.dst, h + o.off, o.bytes);
-  return PIMD_OK;
+  return take_flags(x);
 }
 
 int pimd_energy_terms(pimd_ctx *x, double *terms) {
@@ -1012,7 +1074,7 @@ int pimd_nm_evolve(pimd_ctx *x, double *RRt, double *PPt, const double *masses) 
   if (masses) HIPCHK(hipMemcpyAsync(d.masses, masses, sizeof(double) * na, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.RR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.PP, PPt, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  x->timed("nm_evolve", [&] { launch_nm_evolve((int)P, (int)na, d, d.RR, d.PP, s); });
+  x->timed("nm_evolve", [&] { launch_nm_evolve(x->self, s, KICK_NONE, KICK_NONE, true); });
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(RRt, d.RR, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(PPt, d.PP, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1050,8 +1112,7 @@ int pimd_langevin(pimd_ctx *x, double *PPt, const double *noise, uint64_t seed, 
   HIPCHK(hipMemcpyAsync(d.PP, PPt, sizeof(double) * n, hipMemcpyHostToDevice, s));
   if (noise) HIPCHK(hipMemcpyAsync(d.noise, noise, sizeof(double) * n, hipMemcpyHostToDevice, s));
   x->timed("langevin", [&] {
-    launch_langevin((int)P, (int)na, d, d.PP, noise ? d.noise : nullptr, seed, counter,
-                    x->p.mass_o, x->p.mass_h, s);
+    launch_langevin(x->self, s, true, noise ? 1 : 0, 0, seed, counter, 0, KICK_NONE);
   });
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(PPt, d.PP, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1081,8 +1142,10 @@ int pimd_nose_hoover(double *s, double *uk, double *vxi, double tau, double delt
 int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
   if (!x || !md) return fail(PIMD_EINVAL, "null argument");
   if (!(md->delt_fs > 0.0) || !(md->temp > 0.0)) return fail(PIMD_EINVAL, "delt_fs and temp must be > 0");
-  if (md->bead_thermostat != 0 && md->bead_thermostat != 1)
-    return fail(PIMD_EINVAL, "bead_thermostat must be 0 (none) or 1 (Langevin)");
+  if (md->bead_thermostat < 0 || md->bead_thermostat > 2)
+    return fail(PIMD_EINVAL, "bead_thermostat must be 0 (none), 1 (Langevin) or 2 (Nose-Hoover)");
+  if (md->bead_thermostat == 2 && md->centroid_thermostat && !(md->tau_centroid > 0.0))
+    return fail(PIMD_EINVAL, "the centroid's Nose-Hoover chain needs tau_centroid > 0");
   if (md->intra_timesteps < 0 || md->intra_timesteps > 1000)
     return fail(PIMD_EINVAL, "intra_timesteps must be 0 (plain PIMD) or 1..1000, got %d",
                 md->intra_timesteps);
@@ -1107,6 +1170,7 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
     delete x->cen;
     x->cen = nullptr;
   }
+  x->d.cen_Upot = x->d.cen_virt = x->d.cen_virialc = x->d.cen_dRR = nullptr;
   if (md->intra_timesteps > 0) {
     // the centroid's intermolecular forces: one bead, monomer PES off
     // (pot_ttm.f90:92-132), same model, cut-offs, Ewald and SOR settings
@@ -1119,9 +1183,18 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
     cen->stream = x->stream;
     cen->stream_borrowed = true;
     x->cen = cen;
-    cen->d.clk = x->d.clk;  // the centroid's force call reads the step of this context's clock
+    // its coordinates are this context's centroids (potential(RRc, RRc, ...),
+    // MD.f90:398) and its step the step of this context's clock
+    cen->d.RR = x->d.RRc;
+    cen->d.RRc = x->d.RRc;
+    cen->d.clk = x->d.clk;
     std::copy(x->ttm, x->ttm + TTM_NPARAM, cen->ttm);  // parameters set on this context
     apply_ttm_params(cen);
+    if ((rc = sync_rep(cen)) != PIMD_OK) return rc;
+    x->d.cen_Upot = cen->d.Upot;
+    x->d.cen_virt = cen->d.virt;
+    x->d.cen_virialc = cen->d.virialc;
+    x->d.cen_dRR = cen->d.dRR;
     const size_t n = (size_t)P * x->natoms * 3;
     if (!x->d.dRRfast) {
       if ((rc = dalloc(&x->d.dRRfast, n)) != PIMD_OK) return rc;
@@ -1134,9 +1207,32 @@ int pimd_md_init(pimd_ctx *x, const pimd_md_params *md) {
     rc = pimd_langevin_init(x, delt2, md->centroid_thermostat, md->tau_centroid, P * md->temp);
     if (rc != PIMD_OK) return rc;
   }
+  if (md->bead_thermostat == 2) {
+    // bead_thermostat (main_stuff.f90:145-171): tau_centroid for the centroid
+    // mode, 1/omega_j for the others; chain velocities start at zero
+    // (InputOutput.f90:402-403)
+    std::vector<double> tau(P);
+    for (int j = 0; j < P; ++j) tau[j] = j == 0 ? md->tau_centroid : 1.0 / x->om[j];
+    HIPCHK(hipMemcpy(x->d.nh_tau, tau.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(x->d.vxi_beads, 0, sizeof(double) * (size_t)x->natoms * P * 3 * 2));
+  }
   const double vxi0[2] = {1.0, 1.0}, one = 1.0;  // vxi_global = 1 (InputOutput.f90:399)
-  HIPCHK(hipMemcpy(x->d_vxi, vxi0, sizeof vxi0, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(x->d_s, &one, sizeof one, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->d.nh_vxi, vxi0, sizeof vxi0, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->d.nh_s, &one, sizeof one, hipMemcpyHostToDevice));
+  // the replica's MD constants, read by the ring-polymer and thermostat kernels
+  MDConst &m = x->m;
+  m.delt = delt;
+  m.delt2 = delt2;
+  m.deltf = md->intra_timesteps > 0 ? delt / md->intra_timesteps : delt;
+  m.delt2f = m.deltf / 2.0;
+  m.tempP = P * md->temp;
+  m.omegan = omegan;
+  m.tau = md->tau;
+  m.bead_tt = md->bead_thermostat;
+  m.centroid = md->centroid_thermostat ? 1 : 0;
+  m.global_nh = md->global_nh ? 1 : 0;
+  m.cmd = x->cmd ? 1 : 0;
+  if ((rc = sync_rep(x)) != PIMD_OK) return rc;
   x->md_ready = true;
   return PIMD_OK;
 }
@@ -1145,6 +1241,11 @@ int pimd_state_set(pimd_ctx *x, const double *RRt, const double *PPt, const doub
   if (!x || !RRt || !PPt) return fail(PIMD_EINVAL, "null argument");
   HIPCHK(hipSetDevice(x->p.device));
   const size_t n = (size_t)x->c.P * x->natoms * 3;
+  int rc0;
+  if ((rc0 = check_finite(RRt, n, "RRt")) != PIMD_OK ||
+      (rc0 = check_finite(PPt, n, "PPt")) != PIMD_OK ||
+      (dRRt && (rc0 = check_finite(dRRt, n, "dRRt")) != PIMD_OK))
+    return rc0;
   DevBufs &d = x->d;
   HIPCHK(hipMemcpy(d.RR, RRt, sizeof(double) * n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.PP, PPt, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1177,6 +1278,118 @@ int pimd_state_get(pimd_ctx *x, double *RRt, double *PPt, double *dRRt, double *
 
 }  // extern "C"
 
+// Which replicas of a group have the bead thermostats, the global chain,
+// one bead; used to leave out launches no replica needs.
+struct GroupFlags {
+  bool langevin = false, bead_nh = false, global_nh = false, p1 = false, multi = false,
+       no_bead_tt = false;
+};
+static GroupFlags group_flags(const Group &g) {
+  GroupFlags f;
+  for (int i = 0; i < g.n(); ++i) {
+    f.langevin |= g.hm[i].bead_tt == 1;
+    f.bead_nh |= g.hm[i].bead_tt == 2;
+    f.no_bead_tt |= g.hm[i].bead_tt == 0;
+    f.global_nh |= g.hm[i].global_nh != 0;
+    f.p1 |= g.hc[i].P == 1;
+    f.multi |= g.hc[i].P > 1;
+  }
+  return f;
+}
+
+// bead_thermostat (main_stuff.f90:132-188) of an MD step: the NM-space
+// Langevin thermostat (half 0 / 1 of the step) and the NM-space Nose-Hoover
+// chains of the replicas that have them; with kick = KICK_DRR the half kick
+// -MASSCON dRRt delt2 comes first, and replicas without a bead thermostat
+// take it as a kick of their own.  load_noise(half): loads host normals into
+// the noise buffer first (one replica, eager steps), or NULL.
+template <class LoadNoise>
+static void bead_thermostat(const Group &g, const GroupFlags &f, hipStream_t s, pimd_ctx *px,
+                            int half, int kick, LoadNoise load_noise) {
+  const bool host = load_noise(half);
+  if (f.langevin)
+    timed(px, "langevin",
+          [&] { launch_langevin(g, s, false, host ? 1 : 0, 1, 0, 0, half, kick); });
+  if (f.bead_nh) timed(px, "bead_nh", [&] { launch_bead_nh(g, kick, s); });
+  if (kick != KICK_NONE && f.no_bead_tt)
+    timed(px, "kick", [&] { launch_kick(g, kick, SEL_NO_BEAD_TT, s); });
+}
+
+// Enqueues one PIMD_VV step (MD.f90:35-110) of every replica of g on stream
+// s: the MD step and the record slot come from each replica's device clock,
+// which the step advances at its end, so the same launches serve every step
+// (and can be captured once).  Records Upot, virial, virialc, uk and the
+// observables.  Does not synchronise.
+template <class LoadNoise>
+static int step_vv(const Group &g, hipStream_t s, pimd_ctx *px, LoadNoise load_noise) {
+  const GroupFlags f = group_flags(g);
+  bead_thermostat(g, f, s, px, 0, KICK_NONE, load_noise);
+  if (f.global_nh) timed(px, "nose_hoover", [&] { launch_global_nh(g, s); });
+  // the first half kick rides on the free ring-polymer step
+  if (f.multi) timed(px, "nm_evolve", [&] { launch_nm_evolve(g, s, KICK_DRR); });
+  if (f.p1) {
+    timed(px, "kick", [&] { launch_kick(g, KICK_DRR, SEL_P1, s); });
+    timed(px, "free_flight", [&] { launch_free_flight(g, false, s); });
+  }
+  timed(px, "centroid_pbc", [&] { launch_centroid_pbc(g, s); });
+  int rc = run_forces(g, s, px);
+  if (rc != PIMD_OK) return rc;
+  timed(px, "kinetic", [&] { launch_kinetic(g, s); });
+  timed(px, "observables", [&] { launch_observables(g, true, true, s); });  // also the step record
+  // the second half kick rides on the second bead-thermostat half step
+  bead_thermostat(g, f, s, px, 1, KICK_DRR, load_noise);
+  timed(px, "kinetic", [&] { launch_kinetic(g, s); });
+  if (f.global_nh) timed(px, "nose_hoover", [&] { launch_global_nh(g, s); });
+  launch_advance_clock(g, true, s);  // with the record's kinetic energy
+  HIPCHK(hipGetLastError());
+  return PIMD_OK;
+}
+
+// Enqueues one contracted_MD step (MD.f90:276-449; clock as step_vv) of the
+// replicas of g, all with intra inner steps, whose centroids form the group
+// cen: the monomer PES moves every bead in intra inner steps of the ring
+// polymer, the intermolecular forces of the centroid act on every bead once
+// per step.  Records sum of bead energies, virial, centroid virial and uk.
+// The fast gradient is kept from one step to the next.
+template <class LoadNoise>
+static int step_contracted(const Group &g, const Group &cen, int intra, hipStream_t s,
+                           pimd_ctx *px, LoadNoise load_noise) {
+  const GroupFlags f = group_flags(g);
+  bead_thermostat(g, f, s, px, 0, KICK_NONE, load_noise);
+  if (f.global_nh) timed(px, "nose_hoover", [&] { launch_global_nh(g, s); });
+  // reads and moves coordinates only: the kicks below commute with it
+  timed(px, "centroid_pbc", [&] { launch_centroid_pbc(g, s); });
+  for (int ti = 1; ti <= intra; ++ti) {
+    // the kicks before each free ring-polymer step ride on it: the outer
+    // half kick and the first fast one, then the previous inner step's
+    // closing fast kick and this step's opening one (the same gradient)
+    if (f.multi)
+      timed(px, "nm_evolve",
+            [&] { launch_nm_evolve(g, s, ti == 1 ? KICK_DRR : KICK_FAST, KICK_FAST); });
+    if (f.p1) {
+      timed(px, "kick", [&] { launch_kick(g, ti == 1 ? KICK_DRR : KICK_FAST, SEL_P1, s); });
+      timed(px, "kick_fast", [&] { launch_kick(g, KICK_FAST, SEL_P1, s); });
+      timed(px, "free_flight", [&] { launch_free_flight(g, true, s); });
+    }
+    timed(px, "pes_fast", [&] { launch_pes_fast(g, ti == intra, s); });
+  }
+  timed(px, "kick_fast", [&] { launch_kick(g, KICK_FAST, SEL_ALL, s); });
+  timed(px, "centroid_pbc", [&] { launch_centroid_pbc(g, s); });
+  int rc = PIMD_OK;
+  timed(px, "centroid_forces", [&] { rc = run_forces(cen, s, nullptr); });
+  if (rc != PIMD_OK) return rc;
+  timed(px, "broadcast", [&] { launch_broadcast_beads(g, s); });
+  timed(px, "kinetic", [&] { launch_kinetic(g, s); });
+  launch_contract_record(g, s);
+  timed(px, "observables", [&] { launch_observables(g, false, false, s); });  // no bead dipoles
+  bead_thermostat(g, f, s, px, 1, KICK_DRR, load_noise);  // with the outer half kick
+  timed(px, "kinetic", [&] { launch_kinetic(g, s); });
+  if (f.global_nh) timed(px, "nose_hoover", [&] { launch_global_nh(g, s); });
+  launch_advance_clock(g, true, s);  // with the record's kinetic energy
+  HIPCHK(hipGetLastError());
+  return PIMD_OK;
+}
+
 // Checks that a context is ready for device-resident MD steps.
 static int md_check(pimd_ctx *x) {
   if (!x) return fail(PIMD_EINVAL, "null argument");
@@ -1188,203 +1401,36 @@ static int md_check(pimd_ctx *x) {
 // Device buffer for the per-step records of nsteps steps (grown, never shrunk).
 static int md_records(pimd_ctx *x, int nsteps) {
   if (nsteps <= x->rec_cap) return PIMD_OK;
+  HIPCHK(hipStreamSynchronize(x->stream));
   if (x->d_rec) (void)hipFree(x->d_rec);
   x->d_rec = nullptr;
   x->rec_cap = 0;
-  invalidate_graphs(x);  // captured steps hold the record buffer's address
   int rc = dalloc(&x->d_rec, (size_t)nsteps * MD_REC);
-  if (rc == PIMD_OK) x->rec_cap = nsteps;
-  return rc;
-}
-
-// calc_uk (main_stuff.f90:349-388) into d.uk.
-static void md_kinetic(pimd_ctx *x) {
-  DevBufs &d = x->d;
-  x->timed("kinetic", [&] {
-    if (x->cmd)
-      launch_kinetic_nm(x->c.P, x->natoms, d, d.PP, d.uk, d.ukpart, x->stream);
-    else
-      launch_kinetic(x->c.P, x->natoms, d.masses, d.PP, d.uk, d.ukpart, x->stream);
-  });
-}
-
-// Start of a run at MD step t0: the clock, and the kinetic energy of the
-// loaded state (needed by the first thermostat call).
-static void md_begin(pimd_ctx *x, int t0) {
-  launch_set_clock(x->d.clk, t0, 0, x->stream);
-  md_kinetic(x);
-}
-
-// write_out's observables of the step from its force record and the kinetic
-// energy after the force call.  dipI is NULL when the step has no bead
-// dipoles; Upot (with virt, virialc) is NULL when the step has written its
-// record's first three entries already, else the kernel forms them.
-static void md_observables(pimd_ctx *x, const double *dipI, const double *Upot,
-                           const double *virt, const double *virialc) {
-  DevBufs &d = x->d;
-  const ObsArgs a{x->c.P, x->natoms, x->c.vol, x->omegan};
-  x->timed("observables", [&] {
-    launch_observables(a, d.RR, dipI, d.masses, d.uk, Upot, virt, virialc, d.obs_part, x->d_rec,
-                       d.clk, x->stream);
-  });
-}
-
-// Enqueues one PIMD_VV step (MD.f90:35-110) on the context's stream: the MD
-// step and the record slot come from the device clock, which the step
-// advances at its end, so the same launches serve every step (and can be
-// captured once).  Records Upot, virial, virialc, uk and the observables.
-// Does not synchronise.  noise: NULL or the normals of this step,
-// [2][Natoms][P][3].
-static int md_step_vv(pimd_ctx *x, const double *noise, uint64_t seed) {
-  const int P = x->c.P, na = x->natoms;
-  const size_t n = (size_t)P * na * 3;
-  const pimd_md_params &md = x->md;
-  const double delt = md.delt_fs / 1000.0, delt2 = delt / 2.0;
-  const int L = 3 * na * P;
-  DevBufs &d = x->d;
-  hipStream_t s = x->stream;
-  // the half kicks ride on the kernel that next reads the momenta: the free
-  // ring-polymer step and the second Langevin half step
-  const bool lang = md.bead_thermostat == 1;
-  auto thermostat = [&](int half, const double *kick) {
-    if (!lang) return;
-    const double *nz = nullptr;
-    if (noise) {
-      (void)hipMemcpyAsync(d.noise, noise + (size_t)half * n, sizeof(double) * n,
-                           hipMemcpyHostToDevice, s);
-      nz = d.noise;
-    }
-    x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half, kick,
-                      delt2);
-    });
-  };
-  thermostat(0, nullptr);
-  if (md.global_nh) {
-    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
-    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
-  }
-  if (P > 1) {
-    x->timed("nm_evolve", [&] { launch_nm_evolve(P, na, d, d.RR, d.PP, s, d.dRR, delt2); });
-  } else {
-    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-    x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, delt, s); });
-  }
-  x->timed("centroid_pbc", [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
-  int rc = run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
-                      d.iters_out);
   if (rc != PIMD_OK) return rc;
-  md_kinetic(x);
-  md_observables(x, d.dipI, d.Upot, d.virt, d.virialc);  // also the step record
-  if (lang)
-    thermostat(1, d.dRR);
-  else
-    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-  md_kinetic(x);
-  if (md.global_nh) {
-    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
-    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
-  }
-  launch_advance_clock(d.clk, s, d.uk, x->d_rec);  // with the record's kinetic energy
-  HIPCHK(hipGetLastError());
-  return PIMD_OK;
+  x->rec_cap = nsteps;
+  x->m.rec = x->d_rec;
+  return sync_rep(x);
 }
 
-// Enqueues one contracted_MD step (MD.f90:276-449; clock as md_step_vv): the
-// monomer PES moves every bead in intra_timesteps inner steps of the ring
-// polymer, the intermolecular forces of the centroid act on every bead once
-// per step.  Records sum of bead energies, virial, centroid virial and uk.
-// The fast gradient is kept from one step to the next.
-static int md_step_contracted(pimd_ctx *x, const double *noise, uint64_t seed) {
-  const int P = x->c.P, na = x->natoms, N = x->c.N;
-  const size_t n = (size_t)P * na * 3;
-  const pimd_md_params &md = x->md;
-  const int intra = md.intra_timesteps;
-  const double delt = md.delt_fs / 1000.0, delt2 = delt / 2.0;
-  const double deltfast = delt / intra, delt2fast = deltfast / 2.0;
-  const int L = 3 * na * P;
-  DevBufs &d = x->d;
-  pimd_ctx *cen = x->cen;
-  hipStream_t s = x->stream;
-  const bool lang = md.bead_thermostat == 1;
-  auto thermostat = [&](int half, const double *kick) {
-    if (!lang) return;
-    const double *nz = nullptr;
-    if (noise) {
-      (void)hipMemcpyAsync(d.noise, noise + (size_t)half * n, sizeof(double) * n,
-                           hipMemcpyHostToDevice, s);
-      nz = d.noise;
-    }
-    x->timed("langevin", [&] {
-      launch_langevin(P, na, d, d.PP, nz, seed, 0, x->p.mass_o, x->p.mass_h, s, d.clk, half, kick,
-                      delt2);
-    });
+// One MD step of a context.  noise: NULL or the normals of this step,
+// [2][Natoms][P][3] (host), copied into the noise buffer before each half.
+static int md_step_eager(pimd_ctx *x, const double *noise) {
+  const size_t n = (size_t)x->c.P * x->natoms * 3;
+  auto load = [&](int half) {
+    if (!noise) return false;
+    (void)hipMemcpyAsync(x->d.noise, noise + (size_t)half * n, sizeof(double) * n,
+                         hipMemcpyHostToDevice, x->stream);
+    return true;
   };
-  auto centroid = [&] {
-    x->timed("centroid_pbc",
-             [&] { launch_centroid_pbc(P, na, x->p.box[0], 1.0 / x->p.box[0], d.RR, d.RRc, s); });
-  };
-  thermostat(0, nullptr);
-  if (md.global_nh) {
-    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
-    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
-  }
-  centroid();  // reads and moves coordinates only: the kicks below commute with it
-  for (int ti = 1; ti <= intra; ++ti) {
-    if (P > 1) {
-      // the kicks before each free ring-polymer step ride on it: the outer
-      // half kick and the first fast one, then the previous inner step's
-      // closing fast kick and this step's opening one (the same gradient)
-      x->timed("nm_evolve", [&] {
-        if (ti == 1)
-          launch_nm_evolve(P, na, d, d.RR, d.PP, s, d.dRR, delt2, d.dRRfast, delt2fast);
-        else
-          launch_nm_evolve(P, na, d, d.RR, d.PP, s, d.dRRfast, delt2fast, d.dRRfast, delt2fast);
-      });
-    } else {
-      if (ti == 1) x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-      else x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
-      x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
-      x->timed("free_flight", [&] { launch_free_flight((int)n, d.RR, d.PP, d.masses, deltfast, s); });
-    }
-    x->timed("pes_fast", [&] { launch_pes_fast(x->c, d, d.RR, d.RRc, d.dRRfast, d.mol, ti == intra, s); });
-  }
-  x->timed("kick_fast", [&] { launch_kick((int)n, d.PP, d.dRRfast, delt2fast, s); });
-  centroid();
-  int rc = PIMD_OK;
-  DevBufs &dc = cen->d;
-  x->timed("centroid_forces", [&] {
-    rc = run_forces(cen, d.RRc, d.RRc, dc.Upot, dc.dRR, dc.virt, dc.virialc, dc.dipI, dc.dipE,
-                    dc.chg, dc.iters_out);
-  });
-  if (rc != PIMD_OK) return rc;
-  x->timed("broadcast", [&] { launch_broadcast_beads(P, na * 3, dc.dRR, d.dRR, s); });
-  md_kinetic(x);
-  launch_contract_record(P, N, d.mol, dc.Upot, dc.virt, dc.virialc, d.uk, x->d_rec, d.clk, s);
-  md_observables(x, nullptr, nullptr, nullptr, nullptr);  // no per-bead dipoles here
-  if (lang)
-    thermostat(1, d.dRR);  // with the outer half kick
-  else
-    x->timed("kick", [&] { launch_kick((int)n, d.PP, d.dRR, delt2, s); });
-  md_kinetic(x);
-  if (md.global_nh) {
-    x->timed("nose_hoover", [&] { launch_nose_hoover(d.uk, x->d_vxi, x->d_s, md.tau, delt2, L, md.temp * P, s); });
-    x->timed("scale", [&] { launch_scale((int)n, d.PP, x->d_s, s); });
-  }
-  launch_advance_clock(d.clk, s, d.uk, x->d_rec);  // with the record's kinetic energy
-  HIPCHK(hipGetLastError());
-  return PIMD_OK;
-}
-
-static int md_step_eager(pimd_ctx *x, const double *noise, uint64_t seed) {
-  return x->md.intra_timesteps > 0 ? md_step_contracted(x, noise, seed)
-                                   : md_step_vv(x, noise, seed);
+  if (x->md.intra_timesteps > 0)
+    return step_contracted(x->self, x->cen->self, x->md.intra_timesteps, x->stream, x, load);
+  return step_vv(x->self, x->stream, x, load);
 }
 
 // A step with device normals can run as one replay of a captured graph of
-// md_step_eager, rebuilt when the launch arguments it bakes in change (seed,
-// launch shapes, parameters, buffers).  Host noise, polling and per-kernel
-// timing keep the eager launches.
+// md_step_eager, rebuilt after invalidate_graphs (launch shapes, parameters,
+// buffers).  Host noise, polling and per-kernel timing keep the eager
+// launches.
 static bool graph_eligible(const pimd_ctx *x, const double *noise) {
   const int on = env_int("PIMD_MD_GRAPH", 1, 0, 1);  // read per step: a test switches it
   const SysConst &c = x->c;
@@ -1392,27 +1438,13 @@ static bool graph_eligible(const pimd_ctx *x, const double *noise) {
   return on && !noise && !x->prof && !c.poll && cen_ok;
 }
 
-static int md_graph_build(pimd_ctx *x, uint64_t seed) {
+static int md_graph_build(pimd_ctx *x) {
   if (x->graph) (void)hipGraphExecDestroy(x->graph);
   x->graph = nullptr;
   x->graph_ok = false;
   hipStream_t s = x->stream;
-  // PIMD_GRAPH_BRANCHES=1: force calls inside the step split into two graph
-  // branches where their real- and reciprocal-space kernels are independent.
-  // Off by default: on MI355X it gained 4 % on one FP64 system, nothing in
-  // the mixed form, and slowed batches of replicas by a third.
-  const bool br = env_int("PIMD_GRAPH_BRANCHES", 0, 0, 1) != 0;
-  for (pimd_ctx *y : {x, x->cen}) {
-    if (!y || !(br || y->c.overlap)) continue;
-    const int err = ensure_stream2(y);
-    if (err != PIMD_OK) return err;
-  }
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  x->branches = br;
-  if (x->cen) x->cen->branches = br;
-  const int rc = md_step_eager(x, nullptr, seed);
-  x->branches = false;
-  if (x->cen) x->cen->branches = false;
+  const int rc = md_step_eager(x, nullptr);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &g);
   if (rc != PIMD_OK) {
@@ -1427,19 +1459,18 @@ static int md_graph_build(pimd_ctx *x, uint64_t seed) {
     return fail(PIMD_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e2));
   }
   x->graph_ok = true;
-  x->graph_seed = seed;
   return PIMD_OK;
 }
 
-static int md_step(pimd_ctx *x, const double *noise, uint64_t seed) {
-  if (x->graph_off || !graph_eligible(x, noise)) return md_step_eager(x, noise, seed);
-  if (!x->graph_ok || x->graph_seed != seed) {
-    if (md_graph_build(x, seed) != PIMD_OK) {
+static int md_step(pimd_ctx *x, const double *noise) {
+  if (x->graph_off || !graph_eligible(x, noise)) return md_step_eager(x, noise);
+  if (!x->graph_ok) {
+    if (md_graph_build(x) != PIMD_OK) {
       // capture is not available here: launch eagerly from now on (an error
       // of the step itself shows again in the eager launch)
       x->graph_off = true;
       (void)hipGetLastError();
-      return md_step_eager(x, noise, seed);
+      return md_step_eager(x, noise);
     }
   }
   HIPCHK(hipGraphLaunch(x->graph, x->stream));
@@ -1484,23 +1515,158 @@ extern "C" int pimd_md_run(pimd_ctx *x, int t0, int nsteps, const double *noise,
   if (nsteps < 0) return fail(PIMD_EINVAL, "nsteps must be >= 0");
   HIPCHK(hipSetDevice(x->p.device));
   if ((rc = md_records(x, nsteps)) != PIMD_OK) return rc;
+  if (x->m.seed != seed) {  // the device normals' key, read from the replica record
+    x->m.seed = seed;
+    if ((rc = sync_rep(x)) != PIMD_OK) return rc;
+  }
   const size_t n = (size_t)x->c.P * x->natoms * 3;
-  md_begin(x, t0);
+  // the clock, and the kinetic energy of the loaded state (needed by the
+  // first thermostat call)
+  launch_set_clock(x->self, t0, 0, x->stream);
+  x->timed("kinetic", [&] { launch_kinetic(x->self, x->stream); });
   for (int st = 0; st < nsteps; ++st)
-    if ((rc = md_step(x, noise ? noise + (size_t)st * 2 * n : nullptr, seed)) != PIMD_OK)
-      return rc;
+    if ((rc = md_step(x, noise ? noise + (size_t)st * 2 * n : nullptr)) != PIMD_OK) return rc;
   HIPCHK(hipStreamSynchronize(x->stream));
-  return md_copy_records(x, nsteps, Upot, virial, virialc, uk);
+  if ((rc = md_copy_records(x, nsteps, Upot, virial, virialc, uk)) != PIMD_OK) return rc;
+  return take_flags(x);
 }
 
 // ---------------------------------------------------------------------------
-// Batches of independent systems (replicas) run side by side: each context
-// keeps its own stream, the host enqueues step st of every replica before
-// step st + 1 of any, and small replicas fill the device together.
+// Batches of independent systems (replicas).  Every kernel of a batch step
+// covers all replicas at once (one grid per kernel variant, see launch_over):
+// the plain PIMD_VV replicas form one group, the contracted ones one group
+// per intra_timesteps with their centroids as another.  The whole step of
+// the batch is captured once as a graph and replayed.
 // ---------------------------------------------------------------------------
 struct pimd_batch {
   std::vector<pimd_ctx *> ctx;
+  hipStream_t stream = nullptr;
+  Rep *table = nullptr;  // device replica records of every group, group by group
+  int table_cap = 0;
+  Group vv;
+  struct Con {
+    int intra;
+    Group outer, cen;
+  };
+  std::vector<Con> con;
+  hipGraphExec_t graph = nullptr;
+  bool graph_ok = false;
+  std::vector<unsigned char> sig;  // what the captured launches depend on
 };
+
+static Rep host_rep(const pimd_ctx *x) {
+  Rep h;
+  h.c = x->c;
+  h.b = x->d;
+  h.m = x->m;
+  return h;
+}
+
+// The batch's groups and device replica table from the contexts' current
+// records (uploaded on the batch stream); a change to anything a captured
+// launch depends on (model constants, launch shapes, bead counts, MD
+// options) drops the captured step.
+static int batch_tables(pimd_batch *b) {
+  std::vector<Rep> host;
+  std::vector<unsigned char> sig;
+  auto add_sig = [&](const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    sig.insert(sig.end(), c, c + n);
+  };
+  auto push = [&](Group &g, const pimd_ctx *x) {
+    const Rep h = host_rep(x);
+    host.push_back(h);
+    g.hc.push_back(h.c);
+    g.hm.push_back(h.m);
+    add_sig(&h.c, sizeof(SysConst));
+    const int flags[5] = {h.m.bead_tt, h.m.centroid, h.m.global_nh, h.m.cmd, x->md.intra_timesteps};
+    add_sig(flags, sizeof flags);
+  };
+  b->vv = Group{};
+  b->con.clear();
+  for (pimd_ctx *x : b->ctx)
+    if (x->md.intra_timesteps == 0) push(b->vv, x);
+  std::vector<int> intras;
+  for (pimd_ctx *x : b->ctx)
+    if (x->md.intra_timesteps > 0 &&
+        std::find(intras.begin(), intras.end(), x->md.intra_timesteps) == intras.end())
+      intras.push_back(x->md.intra_timesteps);
+  for (int in : intras) {
+    pimd_batch::Con cn;
+    cn.intra = in;
+    for (pimd_ctx *x : b->ctx)
+      if (x->md.intra_timesteps == in) push(cn.outer, x);
+    for (pimd_ctx *x : b->ctx)
+      if (x->md.intra_timesteps == in) push(cn.cen, x->cen);
+    b->con.push_back(cn);
+  }
+  const int n = (int)host.size();
+  if (n > b->table_cap) {
+    if (b->table) (void)hipFree(b->table);
+    b->table = nullptr;
+    b->table_cap = 0;
+    int rc = dalloc(&b->table, (size_t)n);
+    if (rc != PIMD_OK) return rc;
+    b->table_cap = n;
+    b->graph_ok = false;
+  }
+  int off = 0;
+  b->vv.d = b->table + off;
+  off += b->vv.n();
+  for (auto &cn : b->con) {
+    cn.outer.d = b->table + off;
+    off += cn.outer.n();
+    cn.cen.d = b->table + off;
+    off += cn.cen.n();
+  }
+  HIPCHK(hipMemcpyAsync(b->table, host.data(), sizeof(Rep) * n, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (sig != b->sig) {
+    b->sig.swap(sig);
+    b->graph_ok = false;
+  }
+  return PIMD_OK;
+}
+
+static int batch_step(pimd_batch *b) {
+  auto no_noise = [](int) { return false; };
+  int rc = PIMD_OK;
+  if (b->vv.n() && (rc = step_vv(b->vv, b->stream, nullptr, no_noise)) != PIMD_OK) return rc;
+  for (auto &cn : b->con)
+    if ((rc = step_contracted(cn.outer, cn.cen, cn.intra, b->stream, nullptr, no_noise)) != PIMD_OK)
+      return rc;
+  return PIMD_OK;
+}
+
+// The batch step as one replay of a captured graph (captured again when the
+// tables' signature changes); eager launches where capture fails or with
+// PIMD_MD_GRAPH=0.
+static int batch_md_step(pimd_batch *b) {
+  if (env_int("PIMD_MD_GRAPH", 1, 0, 1) == 0) return batch_step(b);
+  if (!b->graph_ok) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_step(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != PIMD_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    hipError_t e2 = e;
+    if (e == hipSuccess) e2 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      b->graph = nullptr;
+      (void)hipGetLastError();
+      return batch_step(b);
+    }
+    b->graph_ok = true;
+  }
+  HIPCHK(hipGraphLaunch(b->graph, b->stream));
+  return PIMD_OK;
+}
 
 // Launch shapes of one context: lanes per molecule of the pair kernels and of
 // the LDS-staged dipole kernel, column groups of the field kernels and sites
@@ -1513,103 +1679,50 @@ constexpr int NSHAPE = 5;
 static Shape get_shape(const pimd_ctx *x) {
   return Shape{x->c.g_qq, x->c.g_dd, x->c.g_fin, x->c.fd_cg, x->c.sf_sites};
 }
-static void set_shape(pimd_ctx *x, const Shape &sh) {
+static int set_shape(pimd_ctx *x, const Shape &sh) {
   invalidate_graphs(x);
   x->c.g_qq = sh.g_qq;
   x->c.g_dd = sh.g_dd;
   x->c.g_fin = sh.g_fin;
   x->c.fd_cg = sh.fd_cg;
   x->c.sf_sites = sh.sf_sites;
+  return sync_rep(x);
 }
 
-// Mean time of one all-beads force evaluation on the loaded coordinates (ms),
-// writing the gradient to scratch so the MD state is untouched.
-static int time_forces(pimd_ctx *x, double *scratch, hipEvent_t e0, hipEvent_t e1, int reps,
-                       double &ms) {
-  DevBufs &d = x->d;
-  hipStream_t s = x->stream;
-  launch_set_clock(d.clk, 1, 0, s);
-  int rc = run_forces(x, d.RR, d.RRc, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
-                      d.chg, d.iters_out);
-  if (rc != PIMD_OK) return rc;
-  HIPCHK(hipEventRecord(e0, s));
-  for (int r = 0; r < reps; ++r)
-    if ((rc = run_forces(x, d.RR, d.RRc, d.Upot, scratch, d.virt, d.virialc, d.dipI, d.dipE,
-                         d.chg, d.iters_out)) != PIMD_OK)
-      return rc;
-  HIPCHK(hipEventRecord(e1, s));
-  HIPCHK(hipEventSynchronize(e1));
-  float f = 0.f;
-  HIPCHK(hipEventElapsedTime(&f, e0, e1));
-  ms = (double)f / reps;
-  return PIMD_OK;
-}
-
-// Coordinate search over the launch shapes of one context, timing force
-// evaluations of its current coordinates.  Results do not depend on the
-// shapes beyond summation order.  The dipole predictor history is saved and
-// restored around the trials.
-static int autotune_ctx(pimd_ctx *x, double *best_ms) {
-  HIPCHK(hipSetDevice(x->p.device));
-  invalidate_graphs(x);  // the shapes of a contracted context's centroid change below
-  if (x->cen) {
-    // contracted MD: the force call of a step is the centroid's
-    const size_t nc = (size_t)x->natoms * 3;
-    HIPCHK(hipMemcpyAsync(x->cen->d.RR, x->d.RRc, sizeof(double) * nc, hipMemcpyDeviceToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(x->cen->d.RRc, x->d.RRc, sizeof(double) * nc, hipMemcpyDeviceToDevice, x->stream));
-    x = x->cen;
-  }
-  DevBufs &d = x->d;
-  const SysConst &c = x->c;
-  const size_t nh = (size_t)c.P * 4 * c.N * c.nd * 3, ng = (size_t)c.P * x->natoms * 3;
-  double *hist_save = nullptr, *scratch = nullptr;
-  int rc = dalloc(&hist_save, nh);
-  if (rc == PIMD_OK) rc = dalloc(&scratch, ng);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (rc == PIMD_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
-    rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
-  const bool prof = x->prof;
-  x->prof = false;
-  if (rc == PIMD_OK &&
-      hipMemcpyAsync(hist_save, d.hist, sizeof(double) * nh, hipMemcpyDeviceToDevice, x->stream) !=
-          hipSuccess)
-    rc = fail(PIMD_EDEVICE, "hipMemcpyAsync failed");
-  Shape best = get_shape(x);
-  double tbest = 0.0;
-  if (rc == PIMD_OK) rc = time_forces(x, scratch, e0, e1, 2, tbest);
-  // candidate values per knob, in the order of Shape
-  // (0: no further candidate)
-  const int cand[NSHAPE][5] = {{4, 8, 16, 32, 0}, {8, 16, 32, 64, 0}, {4, 8, 16, 32, 0},
-                               {4, 8, 16, 0, 0}, {4, 8, 16, 32, 64}};
-  for (int knob = 0; knob < NSHAPE && rc == PIMD_OK; ++knob) {
-    const int *vals = cand[knob];
-    for (int v = 0; v < 5 && vals[v] != 0 && rc == PIMD_OK; ++v) {
-      Shape sh = best;
-      int *fields[NSHAPE] = {&sh.g_qq, &sh.g_dd, &sh.g_fin, &sh.fd_cg, &sh.sf_sites};
-      int &field = *fields[knob];
-      if (field == vals[v]) continue;
-      field = vals[v];
-      set_shape(x, sh);
-      double t = 0.0;
-      rc = time_forces(x, scratch, e0, e1, 2, t);
-      if (rc == PIMD_OK && t < tbest) {
-        tbest = t;
-        best = sh;
-      }
+// The contexts whose force calls a batch step makes (the replicas of plain
+// PIMD, the centroids of contracted MD), in classes that share launch
+// shapes: one model and arithmetic each.
+static std::vector<std::vector<pimd_ctx *>> force_classes(const pimd_batch *b) {
+  std::vector<std::vector<pimd_ctx *>> cls;
+  std::vector<std::pair<int, int>> keys;
+  for (pimd_ctx *x : b->ctx) {
+    pimd_ctx *f = x->cen ? x->cen : x;
+    const std::pair<int, int> k{f->c.model, f->c.mixed};
+    auto it = std::find(keys.begin(), keys.end(), k);
+    if (it == keys.end()) {
+      keys.push_back(k);
+      cls.push_back({f});
+    } else {
+      cls[it - keys.begin()].push_back(f);
     }
   }
-  set_shape(x, best);
-  x->prof = prof;
-  if (hist_save) {
-    (void)hipMemcpyAsync(d.hist, hist_save, sizeof(double) * nh, hipMemcpyDeviceToDevice, x->stream);
-    (void)hipStreamSynchronize(x->stream);
+  return cls;
+}
+
+// A class's members take the launch shapes of its largest member (in
+// molecule-beads), so that each kernel of a batch step is one launch per class.
+static int unify_shapes(pimd_batch *b) {
+  for (auto &cl : force_classes(b)) {
+    pimd_ctx *big = cl[0];
+    for (pimd_ctx *f : cl)
+      if ((long)f->c.P * f->c.N > (long)big->c.P * big->c.N) big = f;
+    const Shape sh = get_shape(big);
+    for (pimd_ctx *f : cl) {
+      const int rc = set_shape(f, sh);
+      if (rc != PIMD_OK) return rc;
+    }
   }
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  if (hist_save) (void)hipFree(hist_save);
-  if (scratch) (void)hipFree(scratch);
-  if (best_ms) *best_ms = tbest;
-  return rc;
+  return PIMD_OK;
 }
 
 extern "C" int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out) {
@@ -1622,26 +1735,146 @@ extern "C" int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out)
     for (int j = 0; j < i; ++j)
       if (ctxs[j] == ctxs[i]) return fail(PIMD_EINVAL, "context %d appears twice", i);
   }
+  HIPCHK(hipSetDevice(ctxs[0]->p.device));
   pimd_batch *b = new pimd_batch();
   b->ctx.assign(ctxs, ctxs + n);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(PIMD_EDEVICE, "hipStreamCreate failed");
+  }
+  int rc = unify_shapes(b);
+  if (rc != PIMD_OK) {
+    (void)pimd_batch_destroy(b);
+    return rc;
+  }
   *out = b;
   return PIMD_OK;
 }
 
 extern "C" int pimd_batch_destroy(pimd_batch *b) {
+  if (!b) return PIMD_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->table) (void)hipFree(b->table);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return PIMD_OK;
 }
 
+// Mean time (ms) of one force call of every member of a class, all in one
+// group, on the members' loaded coordinates.
+static int time_class(pimd_batch *b, const std::vector<pimd_ctx *> &cl, Rep *tab,
+                      hipEvent_t e0, hipEvent_t e1, int reps, double &ms) {
+  Group g;
+  std::vector<Rep> host;
+  for (pimd_ctx *f : cl) {
+    host.push_back(host_rep(f));
+    g.hc.push_back(f->c);
+    g.hm.push_back(f->m);
+  }
+  g.d = tab;
+  HIPCHK(hipMemcpyAsync(tab, host.data(), sizeof(Rep) * host.size(), hipMemcpyHostToDevice, b->stream));
+  for (pimd_ctx *f : cl) launch_set_clock(f->self, 1, 0, b->stream);
+  int rc = run_forces(g, b->stream, nullptr);
+  if (rc != PIMD_OK) return rc;
+  HIPCHK(hipEventRecord(e0, b->stream));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = run_forces(g, b->stream, nullptr)) != PIMD_OK) return rc;
+  HIPCHK(hipEventRecord(e1, b->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, e0, e1));
+  ms = (double)f / reps;
+  return PIMD_OK;
+}
+
+// Coordinate search over the launch shapes of each class, timing the class's
+// force calls together on the members' current coordinates.  Results do not
+// depend on the shapes beyond summation order.  What the force calls
+// overwrite of the MD state (the gradient, the dipole predictor history, the
+// centroid of contracted MD) is saved and restored around the trials.
 extern "C" int pimd_batch_autotune(pimd_batch *b, double *ms) {
   if (!b) return fail(PIMD_EINVAL, "null argument");
-  for (size_t i = 0; i < b->ctx.size(); ++i) {
-    pimd_ctx *x = b->ctx[i];
-    if (!x->state_ready) return fail(PIMD_ESTATE, "replica %zu: pimd_state_set has not been called", i);
-    int rc = autotune_ctx(x, ms ? ms + i : nullptr);
-    if (rc != PIMD_OK) return rc;
+  for (size_t i = 0; i < b->ctx.size(); ++i)
+    if (!b->ctx[i]->state_ready)
+      return fail(PIMD_ESTATE, "replica %zu: pimd_state_set has not been called", i);
+  HIPCHK(hipSetDevice(b->ctx[0]->p.device));
+  // saved state: gradient and history of the plain replicas
+  size_t nsave = 0;
+  for (pimd_ctx *x : b->ctx) {
+    const size_t P = x->c.P, N = x->c.N;
+    nsave += P * x->natoms * 3 + P * 4 * N * x->c.nd * 3;
+    HIPCHK(hipStreamSynchronize(x->stream));
   }
-  return PIMD_OK;
+  double *save = nullptr;
+  Rep *tab = nullptr;
+  int rc = dalloc(&save, nsave);
+  if (rc == PIMD_OK) rc = dalloc(&tab, b->ctx.size());
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == PIMD_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(PIMD_EDEVICE, "hipEventCreate failed");
+  auto each_state = [&](bool to_save) {
+    size_t o = 0;
+    for (pimd_ctx *x : b->ctx) {
+      const size_t ng = (size_t)x->c.P * x->natoms * 3, nh = (size_t)x->c.P * 4 * x->c.N * x->c.nd * 3;
+      double *a = save + o, *h = save + o + ng;
+      if (to_save) {
+        (void)hipMemcpyAsync(a, x->d.dRR, sizeof(double) * ng, hipMemcpyDeviceToDevice, b->stream);
+        (void)hipMemcpyAsync(h, x->d.hist, sizeof(double) * nh, hipMemcpyDeviceToDevice, b->stream);
+      } else {
+        (void)hipMemcpyAsync(x->d.dRR, a, sizeof(double) * ng, hipMemcpyDeviceToDevice, b->stream);
+        (void)hipMemcpyAsync(x->d.hist, h, sizeof(double) * nh, hipMemcpyDeviceToDevice, b->stream);
+      }
+      o += ng + nh;
+    }
+  };
+  if (rc == PIMD_OK) each_state(true);
+  // candidate values per knob, in the order of Shape
+  const int cand[NSHAPE][5] = {{4, 8, 16, 32, 0}, {8, 16, 32, 64, 0}, {4, 8, 16, 32, 0},
+                               {4, 8, 16, 0, 0}, {4, 8, 16, 32, 64}};
+  const auto classes = force_classes(b);
+  std::map<const pimd_ctx *, double> best_ms;
+  for (const auto &cl : classes) {
+    if (rc != PIMD_OK) break;
+    Shape best = get_shape(cl[0]);
+    double tbest = 0.0;
+    rc = time_class(b, cl, tab, e0, e1, 2, tbest);
+    for (int knob = 0; knob < NSHAPE && rc == PIMD_OK; ++knob) {
+      for (int v = 0; v < 5 && cand[knob][v] != 0 && rc == PIMD_OK; ++v) {
+        Shape sh = best;
+        int *fields[NSHAPE] = {&sh.g_qq, &sh.g_dd, &sh.g_fin, &sh.fd_cg, &sh.sf_sites};
+        if (*fields[knob] == cand[knob][v]) continue;
+        *fields[knob] = cand[knob][v];
+        for (pimd_ctx *f : cl)
+          if ((rc = set_shape(f, sh)) != PIMD_OK) break;
+        double t = 0.0;
+        if (rc == PIMD_OK) rc = time_class(b, cl, tab, e0, e1, 2, t);
+        if (rc == PIMD_OK && t < tbest) {
+          tbest = t;
+          best = sh;
+        }
+      }
+    }
+    for (pimd_ctx *f : cl) {
+      if (rc == PIMD_OK) rc = set_shape(f, best);
+      best_ms[f] = tbest;
+    }
+  }
+  if (ms)
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+      const pimd_ctx *f = b->ctx[i]->cen ? b->ctx[i]->cen : b->ctx[i];
+      ms[i] = best_ms.count(f) ? best_ms[f] : 0.0;
+    }
+  if (save) {
+    each_state(false);
+    (void)hipStreamSynchronize(b->stream);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (save) (void)hipFree(save);
+  if (tab) (void)hipFree(tab);
+  b->graph_ok = false;
+  return rc;
 }
 
 extern "C" int pimd_batch_shape(pimd_batch *b, int i, int *shape) {
@@ -1664,17 +1897,31 @@ extern "C" int pimd_batch_md_run(pimd_batch *b, int t0, int nsteps, const uint64
   int rc;
   HIPCHK(hipSetDevice(b->ctx[0]->p.device));
   for (int r = 0; r < n; ++r) {
-    if ((rc = md_check(b->ctx[r])) != PIMD_OK) return rc;
-    if ((rc = md_records(b->ctx[r], nsteps)) != PIMD_OK) return rc;
+    pimd_ctx *x = b->ctx[r];
+    if ((rc = md_check(x)) != PIMD_OK) return rc;
+    if ((rc = md_records(x, nsteps)) != PIMD_OK) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->m.seed = seeds[r];  // uploaded with the batch's replica records
   }
-  for (int r = 0; r < n; ++r) md_begin(b->ctx[r], t0);
+  if ((rc = batch_tables(b)) != PIMD_OK) return rc;
+  // the clocks, and the kinetic energies of the loaded states (needed by the
+  // first thermostat call)
+  for (Group *g : {&b->vv}) {
+    if (!g->n()) continue;
+    launch_set_clock(*g, t0, 0, b->stream);
+    launch_kinetic(*g, b->stream);
+  }
+  for (auto &cn : b->con) {
+    launch_set_clock(cn.outer, t0, 0, b->stream);
+    launch_kinetic(cn.outer, b->stream);
+  }
   for (int st = 0; st < nsteps; ++st)
-    for (int r = 0; r < n; ++r)
-      if ((rc = md_step(b->ctx[r], nullptr, seeds[r])) != PIMD_OK) return rc;
-  for (int r = 0; r < n; ++r) HIPCHK(hipStreamSynchronize(b->ctx[r]->stream));
+    if ((rc = batch_md_step(b)) != PIMD_OK) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
   for (int r = 0; r < n; ++r)
     if ((rc = md_copy_records(b->ctx[r], nsteps, Upot ? Upot + (size_t)r * nsteps : nullptr,
-                              nullptr, nullptr, uk ? uk + (size_t)r * nsteps : nullptr)) != PIMD_OK)
+                              nullptr, nullptr, uk ? uk + (size_t)r * nsteps : nullptr)) != PIMD_OK ||
+        (rc = take_flags(b->ctx[r])) != PIMD_OK)
       return rc;
   return PIMD_OK;
 }
@@ -1742,6 +1989,8 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
     if (!y) continue;
     std::copy(theta, theta + TTM_NPARAM, y->ttm);
     apply_ttm_params(y);
+    const int rc = sync_rep(y);
+    if (rc != PIMD_OK) return rc;
   }
   invalidate_graphs(x);  // captured launches hold the model constants
   return PIMD_OK;
@@ -1814,10 +2063,8 @@ extern "C" int pimd_state_forces(pimd_ctx *x, int t) {
   int rc = md_check(x);
   if (rc != PIMD_OK) return rc;
   HIPCHK(hipSetDevice(x->p.device));
-  DevBufs &d = x->d;
-  launch_set_clock(d.clk, t, 0, x->stream);
-  return run_forces(x, d.RR, d.RRc, d.Upot, d.dRR, d.virt, d.virialc, d.dipI, d.dipE, d.chg,
-                    d.iters_out);
+  launch_set_clock(x->self, t, 0, x->stream);
+  return run_forces(x);
 }
 
 __global__ void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b, size_t n) {
@@ -1888,8 +2135,9 @@ int pimd_profile_reset(pimd_ctx *x) {
 
 int pimd_sync(pimd_ctx *x) {
   if (!x) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
   HIPCHK(hipStreamSynchronize(x->stream));
-  return PIMD_OK;
+  return take_flags(x);
 }
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -62,6 +63,7 @@ struct SysConst {
   int sf_split;            // 1: column structure factors with two lane sets per block
   int g_dd;                // lanes per molecule of the LDS-staged dipole kernel (0: gather kernel)
   int g_qq, g_fin;         // the same for the charge and final pair kernels
+  int g_nolds;             // lanes per molecule of the pair kernels that do not stage in LDS
   int poll;                // 1: read the active-bead count back after every SOR iteration
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   double box[3], boxi[3], vol;
@@ -133,6 +135,13 @@ struct DevBufs {
   double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
   int *clk = nullptr;                           // device step clock {t, record slot}
   double *obs_part = nullptr;                   // [64][5] partial sums of the observables
+  int *flags = nullptr;                         // [1] bit 0: non-finite coordinates seen
+  double *vxi_beads = nullptr;                  // [Natoms][P][3][2] bead Nose-Hoover chains
+  double *nh_vxi = nullptr, *nh_s = nullptr;    // global chain velocities [2], its scale [1]
+  // contracted MD: the outputs of the centroid's force call (one bead)
+  const double *cen_Upot = nullptr, *cen_virt = nullptr, *cen_virialc = nullptr;
+  const double *cen_dRR = nullptr;
+  double *nh_tau = nullptr;                     // [P] their time constants per mode
   int *clk_own = nullptr;                       // this context's clock (clk may be its owner's)
   double *tan = nullptr;                        // TTM parameter tangents: inputs, tables, workspace
   size_t tan_bytes = 0;
@@ -143,33 +152,151 @@ struct DevBufs {
   double *kE = nullptr, *kphi = nullptr;
 };
 
-// Host-side launchers (ttm_kernels.hip / rp_kernels.hip).
-void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
-                    double *chg_out, hipStream_t s);
-void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s);
-void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s);
-void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s);
-void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s);
-void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s);
+// ---------------------------------------------------------------------------
+// Replica dimension.  Every kernel of an MD step takes a table of replicas
+// (independent systems of any size, model and bead count) and covers several
+// of them in one grid: the blocks of replica i of a launch are off[i] ..
+// off[i+1]-1, numbered x-fastest over that replica's own nx x ny x nz grid
+// (the grid it would get alone), so each replica's arithmetic and summation
+// order do not depend on its companions.  One system alone is a table of one.
+// ---------------------------------------------------------------------------
+constexpr int MAXREP = 64;   // replicas per launch; more are cut into several launches
+
+// Ring-polymer and thermostat constants of a replica's MD step.
+struct MDConst {
+  double delt, delt2;         // step and half step (ps)
+  double deltf, delt2f;       // contracted MD: inner step and its half
+  double tempP;               // bead temperature P T (K)
+  double massO, massH;
+  double omegan, tau;         // ring frequency kB T P / hbar; global chain tau (ps)
+  double box0, boxi0;         // box(1), 1 / box(1) of the centroid wrap (main_stuff.f90:115)
+  uint64_t seed;              // device normals of the Langevin thermostat
+  int bead_tt, centroid, global_nh, cmd;
+  double *rec;                // per-step records [steps][MD_REC]
+};
+
+struct Rep {
+  SysConst c;
+  DevBufs b;
+  MDConst m;
+};
+
+struct BlockMap {
+  int n;
+  int off[MAXREP + 1];
+  int nx[MAXREP], ny[MAXREP];
+  int rep[MAXREP];            // entry of the Rep table
+};
+
+struct Blk {
+  int r, x, y, z;             // Rep table entry; block coordinates in its own grid
+  int nx, ny;                 // its own grid's x and y extents
+};
+
+__device__ __forceinline__ Blk block_of(const BlockMap &m) {
+  const int g = blockIdx.x;
+  int lo = 0, hi = m.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const int l = g - m.off[lo], nx = m.nx[lo], ny = m.ny[lo];
+  return Blk{m.rep[lo], l % nx, (l / nx) % ny, l / (nx * ny), nx, ny};
+}
+
+// Host side: the replicas of one launch sequence.  hc[i] mirrors the SysConst
+// of Rep table entry i (launch decisions and grids are made from it).
+struct Group {
+  std::vector<SysConst> hc;
+  std::vector<MDConst> hm;
+  Rep *d = nullptr;
+  int n() const { return (int)hc.size(); }
+};
+
+// A replica's part of a launch: the kernel variant it takes (< 0: none), its
+// own grid and its dynamic LDS.
+struct Plan {
+  int variant;
+  int nx, ny, nz;
+  size_t lds;
+};
+
+// Launches one grid per variant (and per MAXREP replicas of it), replicas in
+// table order.  plan(const SysConst &, const MDConst &) -> Plan;
+// launch(int variant, dim3 grid, const BlockMap &, size_t lds).
+template <class PlanFn, class LaunchFn>
+void launch_over(const Group &g, PlanFn plan, LaunchFn launch) {
+  const int n = g.n();
+  std::vector<Plan> pl(n);
+  for (int i = 0; i < n; ++i) pl[i] = plan(g.hc[i], g.hm[i]);
+  std::vector<char> done(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (done[i]) continue;
+    if (pl[i].variant < 0 || (long)pl[i].nx * pl[i].ny * pl[i].nz == 0) {
+      done[i] = 1;
+      continue;
+    }
+    BlockMap m{};
+    size_t lds = 0;
+    long total = 0;
+    for (int j = i; j < n && m.n < MAXREP; ++j) {
+      if (done[j] || pl[j].variant != pl[i].variant) continue;
+      const long nb = (long)pl[j].nx * pl[j].ny * pl[j].nz;
+      if (nb == 0) {
+        done[j] = 1;
+        continue;
+      }
+      m.off[m.n] = (int)total;
+      m.nx[m.n] = pl[j].nx;
+      m.ny[m.n] = pl[j].ny;
+      m.rep[m.n] = j;
+      ++m.n;
+      total += nb;
+      lds = std::max(lds, pl[j].lds);
+      done[j] = 1;
+    }
+    m.off[m.n] = (int)total;
+    launch(pl[i].variant, dim3((unsigned)total), m, lds);
+  }
+}
+
+// Host-side launchers of the force path (ttm_kernels.hip, spc_kernels.hip).
+// Each covers every replica of the group it applies to; a force call reads
+// each replica's coordinates RR, RRc and writes its outputs (Upot, dRR,
+// virt, virialc, dipI, dipE, chg, iters_out) in its own buffers.
+void launch_monomer(const Group &g, hipStream_t s);
+void launch_neighbors(const Group &g, hipStream_t s);
+void launch_pair_qq(const Group &g, hipStream_t s);
+void launch_tables(const Group &g, hipStream_t s);
+void launch_sfac(const Group &g, int mode, hipStream_t s);
+int sfac_chunks_used(const SysConst &c);
+void launch_converge(const Group &g, int iter, hipStream_t s);
+// iter: the SOR iteration of FD_D, -1 for the FP64 refinement (maxiter + 1);
+// sel_mixed: only the replicas of the mixed form
+void launch_field(const Group &g, int mode, int iter, hipStream_t s, bool sel_mixed = false);
+void launch_post_qq(const Group &g, hipStream_t s);
 // test_iter > 0: the kernel first runs the convergence test of SOR iteration
-// test_iter (in place of launch_converge); only where pair_dd_tests(c).
-void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s,
-                    int test_iter = 0);
+// test_iter (in place of launch_converge); only where pair_dd_tests(c) holds
+// for every replica.
+void launch_pair_dd(const Group &g, bool fp64, hipStream_t s, int test_iter = 0,
+                    bool sel_mixed = false);
 bool pair_dd_tests(const SysConst &c);
-void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s);
-void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI,
-                     double *dipE, hipStream_t s);
-void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
-                        double *virialc, int *iters, hipStream_t s);
-void launch_spc_monomer(const SysConst &c, DevBufs &b, const double *RR, double *chg_out,
-                        hipStream_t s);
-void launch_spc_pair(const SysConst &c, DevBufs &b, hipStream_t s);
-void launch_spc_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI, double *dipE,
-                         hipStream_t s);
+void launch_refine_prep(const Group &g, hipStream_t s);
+void launch_pair_final(const Group &g, hipStream_t s);
+void launch_virk(const Group &g, hipStream_t s);
+void launch_finalize(const Group &g, hipStream_t s);
+void launch_reduce_bead(const Group &g, hipStream_t s);
+void launch_spc_monomer(const Group &g, hipStream_t s);
+void launch_spc_pair(const Group &g, hipStream_t s);
+void launch_spc_finalize(const Group &g, hipStream_t s);
+// contracted MD (MD.f90:276-449): monomer forces of every bead into the fast
+// gradient (with record, also the energies and virials of the step record),
+// the centroid's gradient to every bead, and the step record
+void launch_pes_fast(const Group &g, bool record, hipStream_t s);
+void launch_broadcast_beads(const Group &g, hipStream_t s);
+void launch_contract_record(const Group &g, hipStream_t s);
+
 void pes_default_params(double *theta);
 void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
                         const double *gw, double *partial, int nblocks, double *grad,
@@ -179,7 +306,6 @@ int pes_adjoint_blocks(const SysConst &c);
 enum SfacMode { SF_Q = 0, SF_D = 1, SF_V = 2 };
 enum FieldMode { FD_Q = 0, FD_D = 1, FD_QD = 2 };
 
-// Ring polymer launchers (rp_kernels.hip).
 // TTM model parameters for fitting, in the order of pimd_ttm_params_get.
 constexpr int TTM_NPARAM = 16;
 enum TtmParam { TP_VDWA = 0, TP_VDWB, TP_VDWC, TP_VDWD, TP_VDWE, TP_POLARO, TP_POLARH, TP_POLARM,
@@ -201,48 +327,36 @@ size_t tangent_cfg_doubles(const SysConst &c);
 size_t tangent_ws_duals(const SysConst &c);
 void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s);
 
-// Per-step observables (write_out): sizes and constants of the system.
-struct ObsArgs {
-  int P, natoms;
-  double volume, omegan;
-};
+// Ring-polymer launchers (rp_kernels.hip), over every replica of the group.
 constexpr int MD_REC = 13;   // per-step record: Upot, virial, virialc, uk, then 9 observables
-void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, double *uk,
-                       double *part, hipStream_t s);
-// Upot, virt, virialc: the per-bead outputs of the step's force call, or NULL
-// when the step record rec[0..2] has been written already.
-void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
-                        const double *masses, const double *uk, const double *Upot,
-                        const double *virt, const double *virialc, double *part, double *rec0,
-                        const int *clk, hipStream_t s);
-void launch_set_clock(int *clk, int t, int slot, hipStream_t s);
-void launch_advance_clock(int *clk, hipStream_t s, const double *uk = nullptr,
-                          double *rec0 = nullptr);
-
-void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
-                     double *dRR, double *mol, bool record, hipStream_t s);
-void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStream_t s);
-void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
-                            const double *vtcen, const double *vccen, const double *uk,
-                            double *rec0, const int *clk, hipStream_t s);
-// kick, kick2 (optional): PP first takes the half kicks PP - MASSCON*kick*delt2,
-// then - MASSCON*kick2*delt2b, each rounded as a k_kick launch would.
-void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
-                      const double *kick = nullptr, double delt2 = 0.0,
-                      const double *kick2 = nullptr, double delt2b = 0.0);
-void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
-                     uint64_t seed, uint64_t counter, double massO, double massH,
-                     hipStream_t s, const int *clk = nullptr, int half = 0,
-                     const double *kick = nullptr, double delt2 = 0.0);
-void launch_kick(int n, double *PP, const double *dRR, double f, hipStream_t s);
-void launch_scale(int n, double *PP, const double *sc, hipStream_t s);
-void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,
-                         double *RRc, hipStream_t s);
-void launch_kinetic(int P, int natoms, const double *masses, const double *PP, double *uk,
-                    double *part, hipStream_t s);
-void launch_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
-                        double temp, hipStream_t s);
-void launch_free_flight(int n, double *RR, const double *PP, const double *masses,
-                        double delt, hipStream_t s);
+// Kicks folded into a ring-polymer kernel: PP - MASSCON*src*dt2 of the
+// replica's gradient (KICK_DRR: dRR and the half step; KICK_FAST: the fast
+// gradient of contracted MD and the inner half step).
+enum KickSrc { KICK_NONE = 0, KICK_DRR = 1, KICK_FAST = 2 };
+// with_p1: one-bead replicas too (in MD steps their free flight is a kick and
+// k_free_flight, MD.f90:60-68)
+void launch_nm_evolve(const Group &g, hipStream_t s, int kick = KICK_NONE, int kick2 = KICK_NONE,
+                      bool with_p1 = false);
+// Langevin_NM of the replicas with the Langevin bead thermostat (all of them
+// with all = true).  host_noise: the replica's noise buffer holds the normals;
+// else drawn from (seed, counter), with use_clk the replica's seed and its
+// step clock.
+void launch_langevin(const Group &g, hipStream_t s, bool all, int host_noise, int use_clk,
+                     uint64_t seed, uint64_t counter, int half, int kick);
+void launch_bead_nh(const Group &g, int kick, hipStream_t s);
+void launch_global_nh(const Group &g, hipStream_t s);
+// Replica selections of launch_kick: all, the one-bead ones (whose free flight
+// replaces the ring-polymer step), those without a bead thermostat (whose
+// second half kick no thermostat kernel carries).
+enum KickSel { SEL_ALL = 0, SEL_P1 = 1, SEL_NO_BEAD_TT = 2 };
+void launch_kick(const Group &g, int kick, int sel, hipStream_t s);
+void launch_free_flight(const Group &g, bool fast, hipStream_t s);
+void launch_centroid_pbc(const Group &g, hipStream_t s);
+void launch_kinetic(const Group &g, hipStream_t s);
+// with_forces: form the step record's Upot, virial, virialc from the force
+// call's per-bead outputs first
+void launch_observables(const Group &g, bool has_dip, bool with_forces, hipStream_t s);
+void launch_set_clock(const Group &g, int t, int slot, hipStream_t s);
+void launch_advance_clock(const Group &g, bool with_uk, hipStream_t s);
 
 }  // namespace pimd

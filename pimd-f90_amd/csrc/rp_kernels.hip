@@ -23,27 +23,33 @@ __host__ __device__ constexpr int nm_cols(int P) { return 1024 / P; }
 // columns: X (P x CB) is staged in LDS, Y = C X, the 2x2 map is applied per
 // mode, and X = C^T Y is written back.  Sums run over beads in the
 // reference's order.
-__global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const double *__restrict__ Cg,
-                                                  const double *__restrict__ A,
-                                                  const double *__restrict__ msf,
-                                                  const double *__restrict__ masses, double *RR,
-                                                  double *PP, const double *__restrict__ kick,
-                                                  double delt2, const double *__restrict__ kick2,
-                                                  double delt2b) {
+__device__ __forceinline__ double kicked(const Rep &R, int kick, size_t o, double p) {
+  if (kick == KICK_DRR) return p - MASSCON * R.b.dRR[o] * R.m.delt2;
+  if (kick == KICK_FAST) return p - MASSCON * R.b.dRRfast[o] * R.m.delt2f;
+  return p;
+}
+
+__global__ __launch_bounds__(256) void k_nm_evolve(const Rep *__restrict__ reps, BlockMap bm,
+                                                  int kick, int kick2) {
   extern __shared__ double lds[];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ Cg = R.b.C, *__restrict__ A = R.b.A, *__restrict__ msf = R.b.msf;
+  const double *__restrict__ masses = R.b.masses;
+  double *RR = R.b.RR, *PP = R.b.PP;
   const int CB = nm_cols(P);
   double *Cs = lds, *xr = Cs + P * P, *xp = xr + P * CB, *yr = xp + P * CB, *yp = yr + P * CB;
-  const int ncols = natoms * 3, col0 = blockIdx.x * CB;
+  const int ncols = natoms * 3, col0 = bk.x * CB;
   const size_t stride = (size_t)natoms * 3;
   for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
   for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
     const int k = e / CB, cc = e % CB, col = col0 + cc;
     xr[e] = col < ncols ? RR[k * stride + col] : 0.0;
     double p = col < ncols ? PP[k * stride + col] : 0.0;
-    // with kick: the half kick PPt - MASSCON*dRRt*delt2 before the step
+    // with kicks: the half kick PPt - MASSCON*dRRt*delt2 before the step
     // (MD.f90:49), as k_kick forms it
-    if (kick && col < ncols) p = p - MASSCON * kick[k * stride + col] * delt2;
-    if (kick2 && col < ncols) p = p - MASSCON * kick2[k * stride + col] * delt2b;
+    if (col < ncols) p = kicked(R, kick2, k * stride + col, kicked(R, kick, k * stride + col, p));
     xp[e] = p;
   }
   __syncthreads();
@@ -71,24 +77,6 @@ __global__ __launch_bounds__(256) void k_nm_evolve(int P, int natoms, const doub
     RR[j * stride + col] = vr;
     PP[j * stride + col] = vp;
   }
-}
-
-static bool nm_mfma();
-static bool langevin_mfma();
-static void launch_nm_evolve_mfma(int P, int natoms, DevBufs &b, double *RR, double *PP,
-                                  hipStream_t s, const double *kick, double delt2,
-                                  const double *kick2, double delt2b);
-
-void launch_nm_evolve(int P, int natoms, DevBufs &b, double *RR, double *PP, hipStream_t s,
-                      const double *kick, double delt2, const double *kick2, double delt2b) {
-  if (nm_mfma() && P <= 64) {
-    launch_nm_evolve_mfma(P, natoms, b, RR, PP, s, kick, delt2, kick2, delt2b);
-    return;
-  }
-  const int CB = nm_cols(P), n = natoms * 3;
-  const size_t lds = sizeof(double) * ((size_t)P * P + 4 * (size_t)P * CB);
-  hipLaunchKernelGGL(k_nm_evolve, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
-                     b.A, b.msf, b.masses, RR, PP, kick, delt2, kick2, delt2b);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,22 +138,20 @@ __device__ __forceinline__ void mfma_bwd(const double *Cs, const double *Ys, v4d
 }
 
 template <int PT>
-__global__ __launch_bounds__(64) void k_nm_evolve_mfma(int P, int natoms,
-                                                       const double *__restrict__ Cg,
-                                                       const double *__restrict__ A,
-                                                       const double *__restrict__ msf,
-                                                       const double *__restrict__ masses,
-                                                       double *RR, double *PP,
-                                                       const double *__restrict__ kick,
-                                                       double delt2,
-                                                       const double *__restrict__ kick2,
-                                                       double delt2b) {
+__global__ __launch_bounds__(64) void k_nm_evolve_mfma(const Rep *__restrict__ reps, BlockMap bm,
+                                                       int kick, int kick2) {
   constexpr int KS = PT / 4, RT = PT / 16;
   __shared__ double Cs[PT * (PT + 1)], Yr[PT * 17], Yp[PT * 17];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ Cg = R.b.C, *__restrict__ A = R.b.A, *__restrict__ msf = R.b.msf;
+  const double *__restrict__ masses = R.b.masses;
+  double *RR = R.b.RR, *PP = R.b.PP;
   const int ncols = natoms * 3;
   const size_t stride = (size_t)ncols;
   const int j = threadIdx.x & 15, q = threadIdx.x >> 4;
-  const int col = blockIdx.x * NM_TILE + j;
+  const int col = bk.x * NM_TILE + j;
   const bool vc = col < ncols;
   stage_C<PT>(Cs, Cg, P);
   double xr[KS], xp[KS];
@@ -177,8 +163,7 @@ __global__ __launch_bounds__(64) void k_nm_evolve_mfma(int P, int natoms,
     xr[kk] = in ? RR[o] : 0.0;
     double p = in ? PP[o] : 0.0;
     // the half kicks before the step (MD.f90:49), rounded as k_kick does
-    if (kick && in) p = p - MASSCON * kick[o] * delt2;
-    if (kick2 && in) p = p - MASSCON * kick2[o] * delt2b;
+    if (in) p = kicked(R, kick2, o, kicked(R, kick, o, p));
     xp[kk] = p;
   }
   __syncthreads();
@@ -232,29 +217,38 @@ static int env_flag(const char *name, int dflt) {
 static bool nm_mfma() { return env_flag("PIMD_NM_MFMA", 1) != 0; }
 static bool langevin_mfma() { return env_flag("PIMD_LANGEVIN_MFMA", 0) != 0; }
 
-static void launch_nm_evolve_mfma(int P, int natoms, DevBufs &b, double *RR, double *PP,
-                                  hipStream_t s, const double *kick, double delt2,
-                                  const double *kick2, double delt2b) {
-  const int n = natoms * 3;
-  const dim3 grid((n + NM_TILE - 1) / NM_TILE);
-  switch (nm_pt(P)) {
-    case 16:
-      hipLaunchKernelGGL(k_nm_evolve_mfma<16>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
-                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
-      break;
-    case 32:
-      hipLaunchKernelGGL(k_nm_evolve_mfma<32>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
-                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
-      break;
-    case 48:
-      hipLaunchKernelGGL(k_nm_evolve_mfma<48>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
-                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
-      break;
-    default:
-      hipLaunchKernelGGL(k_nm_evolve_mfma<64>, grid, dim3(64), 0, s, P, natoms, b.C, b.A, b.msf,
-                         b.masses, RR, PP, kick, delt2, kick2, delt2b);
-      break;
+// The free ring-polymer step of the replicas of the group with more than one
+// bead (all of them with with_p1; the 3 Natoms EvolveRing calls of
+// MD.f90:52-58), with the half kicks kick, kick2 first.  On the matrix cores
+// one launch per padded bead count PT.
+void launch_nm_evolve(const Group &g, hipStream_t s, int kick, int kick2, bool with_p1) {
+  auto on = [&](const SysConst &c) { return with_p1 || c.P > 1; };
+  if (nm_mfma()) {
+    launch_over(
+        g,
+        [&](const SysConst &c, const MDConst &) {
+          return Plan{on(c) ? nm_pt(c.P) : -1, (3 * 3 * c.N + NM_TILE - 1) / NM_TILE, 1, 1, 0};
+        },
+        [&](int pt, dim3 grid, const BlockMap &m, size_t) {
+          switch (pt) {
+            case 16: hipLaunchKernelGGL(k_nm_evolve_mfma<16>, grid, dim3(64), 0, s, g.d, m, kick, kick2); break;
+            case 32: hipLaunchKernelGGL(k_nm_evolve_mfma<32>, grid, dim3(64), 0, s, g.d, m, kick, kick2); break;
+            case 48: hipLaunchKernelGGL(k_nm_evolve_mfma<48>, grid, dim3(64), 0, s, g.d, m, kick, kick2); break;
+            default: hipLaunchKernelGGL(k_nm_evolve_mfma<64>, grid, dim3(64), 0, s, g.d, m, kick, kick2); break;
+          }
+        });
+    return;
   }
+  launch_over(
+      g,
+      [&](const SysConst &c, const MDConst &) {
+        const int CB = nm_cols(c.P), n = 3 * 3 * c.N;
+        return Plan{on(c) ? 0 : -1, (n + CB - 1) / CB, 1, 1,
+                    sizeof(double) * ((size_t)c.P * c.P + 4 * (size_t)c.P * CB)};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t lds) {
+        hipLaunchKernelGGL(k_nm_evolve, grid, dim3(256), lds, s, g.d, m, kick, kick2);
+      });
 }
 
 // Philox4x32-10 counter-based generator (J. K. Salmon et al., "Parallel random
@@ -301,30 +295,41 @@ __device__ __forceinline__ void normal2(uint64_t seed, uint64_t counter, uint64_
 }
 
 // Langevin_NM: p~_j <- c1_j p~_j + c2_j sqrt(m s_j) xi (Langevin.f90:41-60),
-// on the same LDS slabs as k_nm_evolve.  noise [Natoms][P][3] (the
-// reference's consumption order) or drawn from (seed, counter).
-__global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const double *__restrict__ Cg,
-                                                 const double *__restrict__ msf,
-                                                 const double *__restrict__ c1,
-                                                 const double *__restrict__ c2, double massO,
-                                                 double massH, double *PP,
-                                                 const double *__restrict__ noise, uint64_t seed,
-                                                 uint64_t counter, const int *__restrict__ clk,
-                                                 int half, const double *__restrict__ kick,
-                                                 double delt2) {
+// on the same LDS slabs as k_nm_evolve.  Normals: host_noise, the replica's
+// noise buffer [Natoms][P][3] (the reference's consumption order); otherwise
+// drawn from (seed, counter): with use_clk the replica's own seed and the
+// counter 2 t + half of its device step clock (MD steps), else the given ones.
+__device__ __forceinline__ double langevin_xi(const Rep &R, int host_noise, int use_clk,
+                                              uint64_t seed, uint64_t counter, uint64_t n) {
+  if (host_noise) return R.b.noise[n];
+  double z0, z1;
+  normal2(use_clk ? R.m.seed : seed, counter, n >> 1, z0, z1);
+  return (n & 1) ? z1 : z0;
+}
+
+__global__ __launch_bounds__(256) void k_langevin(const Rep *__restrict__ reps, BlockMap bm,
+                                                 int host_noise, int use_clk, uint64_t seed,
+                                                 uint64_t counter, int half, int kick) {
   extern __shared__ double lds[];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ Cg = R.b.C, *__restrict__ msf = R.b.msf;
+  const double *__restrict__ c1 = R.b.c1, *__restrict__ c2 = R.b.c2;
+  const double massO = R.m.massO, massH = R.m.massH;
+  double *PP = R.b.PP;
   // in MD steps the counter follows the device step clock: step t, half 0 / 1
-  if (clk) counter = (uint64_t)clk[0] * 2 + half;
+  if (use_clk) counter = (uint64_t)R.b.clk[0] * 2 + half;
   const int CB = nm_cols(P);
   double *Cs = lds, *xp = Cs + P * P, *yp = xp + P * CB;
-  const int ncols = natoms * 3, col0 = blockIdx.x * CB;
+  const int ncols = natoms * 3, col0 = bk.x * CB;
   const size_t stride = (size_t)natoms * 3;
   for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
   for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
     const int k = e / CB, cc = e % CB, col = col0 + cc;
     double p = col < ncols ? PP[k * stride + col] : 0.0;
     // with kick: the half kick PPt - MASSCON*dRRt*delt2 first (MD.f90:94)
-    if (kick && col < ncols) p = p - MASSCON * kick[k * stride + col] * delt2;
+    if (col < ncols) p = kicked(R, kick, k * stride + col, p);
     xp[e] = p;
   }
   __syncthreads();
@@ -336,14 +341,7 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
       const int a = col / 3, comp = col % 3;
       const double m = (a % 3 == 0) ? massO : massH;
       const uint64_t n = ((uint64_t)a * P + j) * 3 + comp;
-      double xi;
-      if (noise) {
-        xi = noise[n];
-      } else {
-        double z0, z1;
-        normal2(seed, counter, n >> 1, z0, z1);
-        xi = (n & 1) ? z1 : z0;
-      }
+      const double xi = langevin_xi(R, host_noise, use_clk, seed, counter, n);
       const double sm = sqrt(m * msf[j]);
       v = c1[j] * v + c2[j] * (sm * xi);
     }
@@ -361,24 +359,23 @@ __global__ __launch_bounds__(256) void k_langevin(int P, int natoms, const doubl
 
 // k_langevin on FP64 matrix cores (layout as k_nm_evolve_mfma).
 template <int PT>
-__global__ __launch_bounds__(64) void k_langevin_mfma(int P, int natoms,
-                                                      const double *__restrict__ Cg,
-                                                      const double *__restrict__ msf,
-                                                      const double *__restrict__ c1,
-                                                      const double *__restrict__ c2, double massO,
-                                                      double massH, double *PP,
-                                                      const double *__restrict__ noise,
-                                                      uint64_t seed, uint64_t counter,
-                                                      const int *__restrict__ clk, int half,
-                                                      const double *__restrict__ kick,
-                                                      double delt2) {
+__global__ __launch_bounds__(64) void k_langevin_mfma(const Rep *__restrict__ reps, BlockMap bm,
+                                                      int host_noise, int use_clk, uint64_t seed,
+                                                      uint64_t counter, int half, int kick) {
   constexpr int KS = PT / 4, RT = PT / 16;
   __shared__ double Cs[PT * (PT + 1)], Yp[PT * 17];
-  if (clk) counter = (uint64_t)clk[0] * 2 + half;
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ Cg = R.b.C, *__restrict__ msf = R.b.msf;
+  const double *__restrict__ c1 = R.b.c1, *__restrict__ c2 = R.b.c2;
+  const double massO = R.m.massO, massH = R.m.massH;
+  double *PP = R.b.PP;
+  if (use_clk) counter = (uint64_t)R.b.clk[0] * 2 + half;
   const int ncols = natoms * 3;
   const size_t stride = (size_t)ncols;
   const int j = threadIdx.x & 15, q = threadIdx.x >> 4;
-  const int col = blockIdx.x * NM_TILE + j;
+  const int col = bk.x * NM_TILE + j;
   const bool vc = col < ncols;
   stage_C<PT>(Cs, Cg, P);
   double xp[KS];
@@ -389,7 +386,7 @@ __global__ __launch_bounds__(64) void k_langevin_mfma(int P, int natoms,
     const size_t o = (size_t)bead * stride + col;
     double p = in ? PP[o] : 0.0;
     // with kick: the half kick PPt - MASSCON*dRRt*delt2 first (MD.f90:94)
-    if (kick && in) p = p - MASSCON * kick[o] * delt2;
+    if (in) p = kicked(R, kick, o, p);
     xp[kk] = p;
   }
   __syncthreads();
@@ -407,14 +404,7 @@ __global__ __launch_bounds__(64) void k_langevin_mfma(int P, int natoms,
         v = yp[rt][i];
         if (vc) {
           const uint64_t n = ((uint64_t)a * P + mode) * 3 + comp;
-          double xi;
-          if (noise) {
-            xi = noise[n];
-          } else {
-            double z0, z1;
-            normal2(seed, counter, n >> 1, z0, z1);
-            xi = (n & 1) ? z1 : z0;
-          }
+          const double xi = langevin_xi(R, host_noise, use_clk, seed, counter, n);
           const double sm = sqrt(m * msf[mode]);
           v = c1[mode] * v + c2[mode] * (sm * xi);
         }
@@ -433,93 +423,146 @@ __global__ __launch_bounds__(64) void k_langevin_mfma(int P, int natoms,
     }
 }
 
-void launch_langevin(int P, int natoms, DevBufs &b, double *PP, const double *noise,
-                     uint64_t seed, uint64_t counter, double massO, double massH,
-                     hipStream_t s, const int *clk, int half, const double *kick, double delt2) {
-  if (langevin_mfma() && P <= 64) {
-    const dim3 grid((natoms * 3 + NM_TILE - 1) / NM_TILE);
-#define PIMD_LANG_MFMA(PT)                                                                    \
-  hipLaunchKernelGGL(k_langevin_mfma<PT>, grid, dim3(64), 0, s, P, natoms, b.C, b.msf, b.c1, \
-                     b.c2, massO, massH, PP, noise, seed, counter, clk, half, kick, delt2)
-    switch (nm_pt(P)) {
-      case 16: PIMD_LANG_MFMA(16); break;
-      case 32: PIMD_LANG_MFMA(32); break;
-      case 48: PIMD_LANG_MFMA(48); break;
-      default: PIMD_LANG_MFMA(64); break;
-    }
+// Langevin_NM of the replicas whose bead thermostat is Langevin (every
+// replica with all = true, as the pimd_langevin entry point asks).
+void launch_langevin(const Group &g, hipStream_t s, bool all, int host_noise, int use_clk,
+                     uint64_t seed, uint64_t counter, int half, int kick) {
+  auto on = [&](const MDConst &m) { return all || m.bead_tt == 1; };
+  if (langevin_mfma()) {
+    launch_over(
+        g,
+        [&](const SysConst &c, const MDConst &m) {
+          return Plan{on(m) ? nm_pt(c.P) : -1, (3 * 3 * c.N + NM_TILE - 1) / NM_TILE, 1, 1, 0};
+        },
+        [&](int pt, dim3 grid, const BlockMap &m, size_t) {
+#define PIMD_LANG_MFMA(PT)                                                                 \
+  hipLaunchKernelGGL(k_langevin_mfma<PT>, grid, dim3(64), 0, s, g.d, m, host_noise, use_clk, \
+                     seed, counter, half, kick)
+          switch (pt) {
+            case 16: PIMD_LANG_MFMA(16); break;
+            case 32: PIMD_LANG_MFMA(32); break;
+            case 48: PIMD_LANG_MFMA(48); break;
+            default: PIMD_LANG_MFMA(64); break;
+          }
 #undef PIMD_LANG_MFMA
+        });
     return;
   }
-  const int CB = nm_cols(P), n = natoms * 3;
-  const size_t lds = sizeof(double) * ((size_t)P * P + 2 * (size_t)P * CB);
-  hipLaunchKernelGGL(k_langevin, dim3((n + CB - 1) / CB), dim3(256), lds, s, P, natoms, b.C,
-                     b.msf, b.c1, b.c2, massO, massH, PP, noise, seed, counter, clk, half, kick,
-                     delt2);
+  launch_over(
+      g,
+      [&](const SysConst &c, const MDConst &m) {
+        const int CB = nm_cols(c.P), n = 3 * 3 * c.N;
+        return Plan{on(m) ? 0 : -1, (n + CB - 1) / CB, 1, 1,
+                    sizeof(double) * ((size_t)c.P * c.P + 2 * (size_t)c.P * CB)};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t lds) {
+        hipLaunchKernelGGL(k_langevin, grid, dim3(256), lds, s, g.d, m, host_noise, use_clk, seed,
+                           counter, half, kick);
+      });
 }
 
 // The device step clock {MD step t, record slot} read by the force and
 // thermostat kernels of an MD step, so that one captured step replays for
-// every step.
-__global__ void k_set_clock(int *clk, int t, int slot) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    clk[0] = t;
-    clk[1] = slot;
-  }
+// every step.  One thread per replica.
+__global__ void k_set_clock(const Rep *__restrict__ reps, BlockMap bm, int t, int slot) {
+  const Blk bk = block_of(bm);
+  if (threadIdx.x != 0) return;
+  int *clk = reps[bk.r].b.clk;
+  clk[0] = t;
+  clk[1] = slot;
 }
-// End of an MD step: its record takes the final kinetic energy (rec[3]) when
-// rec0 is non-NULL, then the clock moves to the next step and record slot.
-__global__ void k_advance_clock(int *clk, const double *__restrict__ uk, double *rec0) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (rec0) rec0[(size_t)clk[1] * MD_REC + 3] = uk[0];
-    clk[0] += 1;
-    clk[1] += 1;
-  }
-}
-void launch_set_clock(int *clk, int t, int slot, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, s, clk, t, slot);
-}
-void launch_advance_clock(int *clk, hipStream_t s, const double *uk, double *rec0) {
-  hipLaunchKernelGGL(k_advance_clock, dim3(1), dim3(64), 0, s, clk, uk, rec0);
+// End of an MD step: with with_uk its record takes the final kinetic energy
+// (rec[3]), then the clock moves to the next step and record slot.
+__global__ void k_advance_clock(const Rep *__restrict__ reps, BlockMap bm, int with_uk) {
+  const Blk bk = block_of(bm);
+  if (threadIdx.x != 0) return;
+  const Rep &R = reps[bk.r];
+  int *clk = R.b.clk;
+  if (with_uk) R.m.rec[(size_t)clk[1] * MD_REC + 3] = R.b.uk[0];
+  clk[0] += 1;
+  clk[1] += 1;
 }
 
-// PPt = PPt - MASSCON*dRRt*delt2 (MD.f90:49, 94).
-__global__ void k_kick(int n, double *PP, const double *__restrict__ dRR, double delt2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) PP[i] = PP[i] - MASSCON * dRR[i] * delt2;
+void launch_set_clock(const Group &g, int t, int slot, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &, const MDConst &) { return Plan{0, 1, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_set_clock, grid, dim3(64), 0, s, g.d, m, t, slot);
+      });
+}
+void launch_advance_clock(const Group &g, bool with_uk, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &, const MDConst &) { return Plan{0, 1, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_advance_clock, grid, dim3(64), 0, s, g.d, m, with_uk ? 1 : 0);
+      });
 }
 
-void launch_kick(int n, double *PP, const double *dRR, double delt2, hipStream_t s) {
-  hipLaunchKernelGGL(k_kick, dim3((n + 255) / 256), dim3(256), 0, s, n, PP, dRR, delt2);
+// PPt = PPt - MASSCON*dRRt*delt2 (MD.f90:49, 94) with the replica's gradient
+// of kind `kick` (KICK_DRR or KICK_FAST).
+__global__ void k_kick(const Rep *__restrict__ reps, BlockMap bm, int kick) {
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const size_t n = (size_t)R.c.P * 3 * R.c.N * 3;
+  const size_t i = (size_t)bk.x * blockDim.x + threadIdx.x;
+  if (i < n) R.b.PP[i] = kicked(R, kick, i, R.b.PP[i]);
 }
 
-// PPt = PPt*s with s on the device (MD.f90:45, 105).
-__global__ void k_scale(int n, double *PP, const double *__restrict__ sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) PP[i] = PP[i] * sc[0];
+void launch_kick(const Group &g, int kick, int sel, hipStream_t s) {
+  launch_over(
+      g,
+      [&](const SysConst &c, const MDConst &m) {
+        const long n = (long)c.P * 3 * c.N * 3;
+        const bool on = sel == SEL_ALL || (sel == SEL_P1 && c.P == 1) ||
+                        (sel == SEL_NO_BEAD_TT && m.bead_tt == 0);
+        return Plan{on ? 0 : -1, (int)((n + 255) / 256), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_kick, grid, dim3(256), 0, s, g.d, m, kick);
+      });
 }
 
-void launch_scale(int n, double *PP, const double *sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, PP, sc);
+// PPt = PPt*s with the global chain's s on the device (MD.f90:45, 105).
+__global__ void k_scale(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const size_t n = (size_t)R.c.P * 3 * R.c.N * 3;
+  const size_t i = (size_t)bk.x * blockDim.x + threadIdx.x;
+  if (i < n) R.b.PP[i] = R.b.PP[i] * R.b.nh_s[0];
 }
 
-// Free flight for P = 1 (MD.f90:60-68).
-__global__ void k_free_flight(int n, double *RR, const double *__restrict__ PP,
-                              const double *__restrict__ masses, double delt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) RR[i] = RR[i] + (1.0 / masses[i / 3]) * PP[i] * delt;
+// Free flight of one-bead replicas (MD.f90:60-68), over the step (fast = 0)
+// or the inner step of contracted MD (fast = 1).
+__global__ void k_free_flight(const Rep *__restrict__ reps, BlockMap bm, int fast) {
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const size_t n = (size_t)R.c.P * 3 * R.c.N * 3;
+  const size_t i = (size_t)bk.x * blockDim.x + threadIdx.x;
+  const double delt = fast ? R.m.deltf : R.m.delt;
+  if (i < n) R.b.RR[i] = R.b.RR[i] + (1.0 / R.b.masses[i / 3]) * R.b.PP[i] * delt;
 }
 
-void launch_free_flight(int n, double *RR, const double *PP, const double *masses, double delt,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_free_flight, dim3((n + 255) / 256), dim3(256), 0, s, n, RR, PP, masses,
-                     delt);
+void launch_free_flight(const Group &g, bool fast, hipStream_t s) {
+  launch_over(
+      g,
+      [&](const SysConst &c, const MDConst &) {
+        const long n = (long)c.P * 3 * c.N * 3;
+        return Plan{c.P == 1 ? 0 : -1, (int)((n + 255) / 256), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_free_flight, grid, dim3(256), 0, s, g.d, m, fast ? 1 : 0);
+      });
 }
 
 // RRc = sum(RRt,3)/Nbeads, then the centroid wrap of PBCs (MD.f90:70;
 // main_stuff.f90:107-125): shifts use box(1) on every axis.
-__global__ void k_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR,
-                               double *RRc) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_centroid_pbc(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double box0 = R.m.box0, boxi0 = R.m.boxi0;
+  double *RR = R.b.RR, *RRc = R.b.RRc;
+  const int id = bk.x * blockDim.x + threadIdx.x;
   if (id >= natoms * 3) return;
   const size_t stride = (size_t)natoms * 3;
   // beads summed in order 1..P; loads issued 8 at a time
@@ -540,23 +583,31 @@ __global__ void k_centroid_pbc(int P, int natoms, double box0, double boxi0, dou
   for (k = 0; k < P; ++k) RR[k * stride + id] = RR[k * stride + id] - sh;
 }
 
-void launch_centroid_pbc(int P, int natoms, double box0, double boxi0, double *RR, double *RRc,
-                         hipStream_t s) {
-  const int n = natoms * 3;
-  hipLaunchKernelGGL(k_centroid_pbc, dim3((n + 63) / 64), dim3(64), 0, s, P, natoms, box0,
-                     boxi0, RR, RRc);
+void launch_centroid_pbc(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, (9 * c.N + 63) / 64, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_centroid_pbc, grid, dim3(64), 0, s, g.d, m);
+      });
 }
 
 // uk = 1/2 sum_beads sum_atoms |p|^2 / m (calc_uk, RPMD branch): per-block
 // partial sums, then one block adds them in a fixed order.
 constexpr int KIN_BLOCKS = 256;
-__global__ __launch_bounds__(256) void k_kinetic_part(int P, int natoms,
-                                                     const double *__restrict__ masses,
-                                                     const double *__restrict__ PP, double *part) {
+__host__ __device__ inline int kin_blocks(const SysConst &c, int cmd) {
+  const long n = cmd ? 3L * c.N : (long)c.P * 3 * c.N;
+  return (int)std::min<long>(KIN_BLOCKS, (n + 255) / 256);
+}
+
+__global__ __launch_bounds__(256) void k_kinetic_part(const Rep *__restrict__ reps, BlockMap bm) {
   __shared__ double red[256];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ masses = R.b.masses, *__restrict__ PP = R.b.PP;
   const size_t n = (size_t)P * natoms;
   double s = 0.0;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+  for (size_t i = (size_t)bk.x * 256 + threadIdx.x; i < n; i += (size_t)bk.nx * 256) {
     const int a = (int)(i % natoms);
     const double *p = PP + i * 3;
     s += (1.0 / masses[a]) * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
@@ -567,44 +618,23 @@ __global__ __launch_bounds__(256) void k_kinetic_part(int P, int natoms,
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(256) void k_kinetic_final(const double *__restrict__ part, int nb,
-                                                      double *uk) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) uk[0] = 0.5 * red[0];
-}
-
-void launch_kinetic(int P, int natoms, const double *masses, const double *PP, double *uk,
-                    double *part, hipStream_t s) {
-  const size_t n = (size_t)P * natoms;
-  const int nb = (int)std::min<size_t>(KIN_BLOCKS, (n + 255) / 256);
-  hipLaunchKernelGGL(k_kinetic_part, dim3(nb), dim3(256), 0, s, P, natoms, masses, PP, part);
-  hipLaunchKernelGGL(k_kinetic_final, dim3(1), dim3(256), 0, s, part, nb, uk);
+  if (threadIdx.x == 0) R.b.ukpart[bk.x] = red[0];
 }
 
 // calc_uk with centroid-MD mass scaling (main_stuff.f90:366-385): per atom
 // the normal-mode momenta p~_j = sum_k C[j][k] p_k (real2NM,
 // NormalModes.f90:308-322), uk = 1/2 sum_atoms sum_j |p~_j|^2 / (m msf_j).
 // One thread per atom, per-block partials, then the fixed-order final sum.
-__global__ __launch_bounds__(256) void k_kinetic_nm_part(int P, int natoms,
-                                                        const double *__restrict__ Cg,
-                                                        const double *__restrict__ msf,
-                                                        const double *__restrict__ masses,
-                                                        const double *__restrict__ PP, double *part) {
+__global__ __launch_bounds__(256) void k_kinetic_nm_part(const Rep *__restrict__ reps, BlockMap bm) {
   __shared__ double red[256];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ Cg = R.b.C, *__restrict__ msf = R.b.msf;
+  const double *__restrict__ masses = R.b.masses, *__restrict__ PP = R.b.PP;
   const size_t stride = (size_t)natoms * 3;
   double s = 0.0;
-  for (int a = blockIdx.x * 256 + threadIdx.x; a < natoms; a += gridDim.x * 256) {
+  for (int a = bk.x * 256 + threadIdx.x; a < natoms; a += bk.nx * 256) {
     const double im = 1.0 / masses[a];
     for (int j = 0; j < P; ++j) {
       double q0 = 0.0, q1 = 0.0, q2 = 0.0;
@@ -624,15 +654,40 @@ __global__ __launch_bounds__(256) void k_kinetic_nm_part(int P, int natoms,
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) R.b.ukpart[bk.x] = red[0];
 }
 
-void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, double *uk,
-                       double *part, hipStream_t s) {
-  const int nb = std::min(KIN_BLOCKS, (natoms + 255) / 256);
-  hipLaunchKernelGGL(k_kinetic_nm_part, dim3(nb), dim3(256), 0, s, P, natoms, b.C, b.msf,
-                     b.masses, PP, part);
-  hipLaunchKernelGGL(k_kinetic_final, dim3(1), dim3(256), 0, s, part, nb, uk);
+__global__ __launch_bounds__(256) void k_kinetic_final(const Rep *__restrict__ reps, BlockMap bm) {
+  __shared__ double red[256];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int nb = kin_blocks(R.c, R.m.cmd);
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) s += R.b.ukpart[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) R.b.uk[0] = 0.5 * red[0];
+}
+
+// calc_uk (main_stuff.f90:349-388) of every replica into its uk: mode-space
+// for centroid MD (MDConst::cmd), bead-space otherwise.
+void launch_kinetic(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &m) { return Plan{m.cmd ? 1 : 0, kin_blocks(c, m.cmd), 1, 1, 0}; },
+      [&](int v, dim3 grid, const BlockMap &m, size_t) {
+        if (v) hipLaunchKernelGGL(k_kinetic_nm_part, grid, dim3(256), 0, s, g.d, m);
+        else hipLaunchKernelGGL(k_kinetic_part, grid, dim3(256), 0, s, g.d, m);
+      });
+  launch_over(
+      g, [](const SysConst &, const MDConst &) { return Plan{0, 1, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_kinetic_final, grid, dim3(256), 0, s, g.d, m);
+      });
 }
 
 // Per-step observables of write_out (InputOutput.f90:556-621) from the step
@@ -643,20 +698,26 @@ void launch_kinetic_nm(int P, int natoms, const DevBufs &b, const double *PP, do
 //   obs[3..4] simple_quantum_estimators energy, pressure (estimators.f90:55-94)
 //   obs[5..7] total molecular dipole, obs[8] sum of molecular dipole
 //   magnitudes, Debye, molecular dipoles averaged over beads
-//   (InputOutput.f90:593-607); NaN when dipI is NULL.
+//   (InputOutput.f90:593-607); NaN without bead dipoles (has_dip = 0).
 // k_obs_part: per block, partial sums of the spring term
 // K0 = sum_k sum_atoms m |R_k - R_(k-1)|^2 (bead 0 against P-1) and of the
 // dipole sums over a slice of molecules; k_obs_final adds the partials in
 // block order and forms the observables.
 constexpr int OBS_BLOCKS = 64;
-__global__ __launch_bounds__(256) void k_obs_part(ObsArgs a, const double *__restrict__ RR,
-                                                 const double *__restrict__ dipI,
-                                                 const double *__restrict__ masses,
-                                                 double *__restrict__ part) {
+__host__ __device__ inline int obs_blocks(const SysConst &c) {
+  return (int)std::min<long>(OBS_BLOCKS, ((long)c.P * 3 * c.N + 255) / 256);
+}
+
+__global__ __launch_bounds__(256) void k_obs_part(const Rep *__restrict__ reps, BlockMap bm,
+                                                 int has_dip) {
   __shared__ double red[5][256];
-  const int P = a.P, na = a.natoms, N = na / 3;
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, na = 3 * R.c.N, N = R.c.N;
+  const double *__restrict__ RR = R.b.RR, *__restrict__ masses = R.b.masses;
+  const double *__restrict__ dipI = has_dip ? R.b.dipI : nullptr;
   const size_t stride = (size_t)na * 3;
-  const size_t gsz = (size_t)gridDim.x * 256, g0 = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t gsz = (size_t)bk.nx * 256, g0 = (size_t)bk.x * 256 + threadIdx.x;
   double k0 = 0.0;
   for (size_t e = g0; e < (size_t)P * na; e += gsz) {
     const int k = (int)(e / na), at = (int)(e % na);
@@ -693,28 +754,28 @@ __global__ __launch_bounds__(256) void k_obs_part(ObsArgs a, const double *__res
       for (int v = 0; v < 5; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 5) R.b.obs_part[bk.x * 5 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// With Upot non-NULL it first forms the step record rec[0..2] from the
-// per-bead outputs (force_calc.f90:94-97); otherwise rec[0..2] is already set.
-__global__ __launch_bounds__(64) void k_obs_final(ObsArgs a, const double *__restrict__ part,
-                                                 int nb, int has_dip,
-                                                 const double *__restrict__ uk,
-                                                 const double *__restrict__ Upot,
-                                                 const double *__restrict__ virt,
-                                                 const double *__restrict__ virialc, double *rec0,
-                                                 const int *__restrict__ clk) {
+// With with_forces it first forms the step record rec[0..2] from the
+// per-bead outputs of the force call (force_calc.f90:94-97); otherwise
+// rec[0..2] is already set.
+__global__ __launch_bounds__(64) void k_obs_final(const Rep *__restrict__ reps, BlockMap bm,
+                                                 int has_dip, int with_forces) {
   __shared__ double red[8][64];
-  const int P = a.P, na = a.natoms, N = na / 3, tid = threadIdx.x;
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, na = 3 * R.c.N, N = R.c.N, tid = threadIdx.x;
+  const int nb = obs_blocks(R.c);
+  const double *__restrict__ part = R.b.obs_part;
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = tid; i < nb; i += 64)
     for (int k = 0; k < 5; ++k) v[k] += part[i * 5 + k];
-  if (Upot)
+  if (with_forces)
     for (int b = tid; b < P; b += 64) {
-      v[5] += Upot[b];
-      v[6] += virt[b * 9 + 0] + virt[b * 9 + 4] + virt[b * 9 + 8];
-      v[7] += virialc[b];
+      v[5] += R.b.Upot[b];
+      v[6] += R.b.virt[b * 9 + 0] + R.b.virt[b * 9 + 4] + R.b.virt[b * 9 + 8];
+      v[7] += R.b.virialc[b];
     }
   for (int k = 0; k < 8; ++k) red[k][tid] = v[k];
   __syncthreads();
@@ -724,47 +785,50 @@ __global__ __launch_bounds__(64) void k_obs_final(ObsArgs a, const double *__res
     __syncthreads();
   }
   if (tid != 0) return;
-  double *rec = rec0 + (size_t)clk[1] * MD_REC;
-  if (Upot) {
+  double *rec = R.m.rec + (size_t)R.b.clk[1] * MD_REC;
+  if (with_forces) {
     rec[0] = red[5][0];
     rec[1] = red[6][0];
     rec[2] = red[7][0] / P;
   }
   const double Up = rec[0], virial = rec[1], virialc_ = rec[2];
-  const double temp = TEMPFACTOR * uk[0] / (double)(na * P * P);
-  const double p3v = PRESSCON2 * (1.0 / (3.0 * a.volume));
+  const double temp = TEMPFACTOR * R.b.uk[0] / (double)(na * P * P);
+  const double p3v = PRESSCON2 * (1.0 / (3.0 * R.c.vol));
   const double ke = (double)(1.5f * (float)na) * KB_KCAL * temp;
   double *o = rec + 4;
   o[0] = temp;
   o[1] = (ke + 0.5 * (virial - virialc_) / P + Up / P) / N;
   o[2] = p3v * (2 * ke - virialc_ / P) / N;
   const double kes = (double)(1.5f * (float)na * (float)P) * KB_KCAL * temp * P;
-  const double K0 = 0.5 * red[0][0] * (a.omegan * a.omegan) * MASSCONI;
+  const double K0 = 0.5 * red[0][0] * (R.m.omegan * R.m.omegan) * MASSCONI;
   o[3] = (kes - K0 + Up) / (double)(N * P);
   o[4] = p3v * (2 * (kes - K0) - virial) / N;
   const double nan = __builtin_nan("");
   for (int c = 0; c < 4; ++c) o[5 + c] = has_dip ? red[1 + c][0] : nan;
 }
 
-void launch_observables(const ObsArgs &a, const double *RR, const double *dipI,
-                        const double *masses, const double *uk, const double *Upot,
-                        const double *virt, const double *virialc, double *part, double *rec0,
-                        const int *clk, hipStream_t s) {
-  const size_t n = (size_t)a.P * a.natoms;
-  const int nb = (int)std::min<size_t>(OBS_BLOCKS, (n + 255) / 256);
-  hipLaunchKernelGGL(k_obs_part, dim3(nb), dim3(256), 0, s, a, RR, dipI, masses, part);
-  hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(64), 0, s, a, part, nb, dipI ? 1 : 0, uk, Upot,
-                     virt, virialc, rec0, clk);
+void launch_observables(const Group &g, bool has_dip, bool with_forces, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, obs_blocks(c), 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_obs_part, grid, dim3(256), 0, s, g.d, m, has_dip ? 1 : 0);
+      });
+  launch_over(
+      g, [](const SysConst &, const MDConst &) { return Plan{0, 1, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_obs_final, grid, dim3(64), 0, s, g.d, m, has_dip ? 1 : 0,
+                           with_forces ? 1 : 0);
+      });
 }
 
-// Nose_Hoover half step, chain length 2 (Nose_Hoover.f90:158-198), scalars on device.
-__global__ void k_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
-                              double temp) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One Nose_Hoover call with a chain of length 2 (Nose_Hoover.f90:33-112):
+// advances the chain velocities v0, v1 by half a step, scales u (2 x kinetic
+// energy / 2) by s^2 and returns the momentum scale s.
+__device__ __forceinline__ double nh_chain2(double &u, double &v0, double &v1, double tau,
+                                            double delt2, double L, double temp) {
   const double delt4 = delt2 / 2.0, delt8 = delt2 / 4.0;
   const double kT = KB_AMU * temp;
   const double M1 = L * kT * (tau * tau), M2 = M1;
-  double v0 = vxi[0], v1 = vxi[1], u = uk[0];
   v1 = v1 + (M1 * (v0 * v0) - kT) * delt4 / M2;
   v0 = v0 * exp(-v1 * delt8);
   v0 = v0 + (2 * u - L * kT) * delt4 / M1;
@@ -775,15 +839,113 @@ __global__ void k_nose_hoover(double *uk, double *vxi, double *sc, double tau, d
   v0 = v0 + (2 * u - L * kT) * delt4 / M1;
   v0 = v0 * exp(-v1 * delt8);
   v1 = v1 + (M1 * (v0 * v0) - kT) * delt4 / M2;
-  vxi[0] = v0;
-  vxi[1] = v1;
-  uk[0] = u;
-  sc[0] = s;
+  return s;
 }
 
-void launch_nose_hoover(double *uk, double *vxi, double *sc, double tau, double delt2, int L,
-                        double temp, hipStream_t s) {
-  hipLaunchKernelGGL(k_nose_hoover, dim3(1), dim3(64), 0, s, uk, vxi, sc, tau, delt2, L, temp);
+// The global chain (MD.f90:43-46, 99-102) of each replica, scalars on
+// device: L = 3 Natoms P degrees of freedom at P T.
+__global__ void k_nose_hoover(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  if (threadIdx.x != 0) return;
+  const Rep &R = reps[bk.r];
+  double *vxi = R.b.nh_vxi;
+  double v0 = vxi[0], v1 = vxi[1], u = R.b.uk[0];
+  const int L = 3 * 3 * R.c.N * R.c.P;
+  const double s = nh_chain2(u, v0, v1, R.m.tau, R.m.delt2, (double)L, R.m.tempP);
+  vxi[0] = v0;
+  vxi[1] = v1;
+  R.b.uk[0] = u;
+  R.b.nh_s[0] = s;
+}
+
+// The global chain, then PPt = PPt*s, for the replicas that have it.
+void launch_global_nh(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &, const MDConst &m) { return Plan{m.global_nh ? 0 : -1, 1, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_nose_hoover, grid, dim3(64), 0, s, g.d, m);
+      });
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &m) {
+        const long n = (long)c.P * 3 * c.N * 3;
+        return Plan{m.global_nh ? 0 : -1, (int)((n + 255) / 256), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_scale, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+// Nose-Hoover bead thermostat in normal-mode space (main_stuff.f90:145-171):
+// p~ = C p per (atom, component) (real2NM); every mode momentum p~_j has a
+// chain of its own (vxi [Natoms][P][3][2], Nose_Hoover with L = 1 at the bead
+// temperature P T): the centroid (j = 0) only with the centroid thermostat,
+// tau = tau_centroid and u = p~^2 / 2m; the other modes tau = 1 / omega_j and
+// u = p~^2 / (2 m MassScaleFactor_j); p~_j is scaled by the chain's s and
+// p = C^T p~ (NM2real).  LDS slabs and sum orders as k_langevin; with kick,
+// the half kick PPt - MASSCON dRRt delt2 comes first (MD.f90:94).
+__global__ __launch_bounds__(256) void k_bead_nh(const Rep *__restrict__ reps, BlockMap bm,
+                                                int kick) {
+  extern __shared__ double lds[];
+  const Blk bk = block_of(bm);
+  const Rep &R = reps[bk.r];
+  const int P = R.c.P, natoms = 3 * R.c.N;
+  const double *__restrict__ Cg = R.b.C, *__restrict__ msf = R.b.msf;
+  const double *__restrict__ tau_mode = R.b.nh_tau;
+  const int centroid = R.m.centroid;
+  const double massO = R.m.massO, massH = R.m.massH, delt2 = R.m.delt2, tempP = R.m.tempP;
+  double *vxi = R.b.vxi_beads, *PP = R.b.PP;
+  const int CB = nm_cols(P);
+  double *Cs = lds, *xp = Cs + P * P, *yp = xp + P * CB;
+  const int ncols = natoms * 3, col0 = bk.x * CB;
+  const size_t stride = (size_t)natoms * 3;
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) Cs[e] = Cg[e];
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int k = e / CB, cc = e % CB, col = col0 + cc;
+    double p = col < ncols ? PP[k * stride + col] : 0.0;
+    if (col < ncols) p = kicked(R, kick, k * stride + col, p);
+    xp[e] = p;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int j = e / CB, cc = e % CB, col = col0 + cc;
+    double v = 0.0;
+    for (int k = 0; k < P; ++k) v = v + Cs[j * P + k] * xp[k * CB + cc];
+    if (col < ncols && (j > 0 || centroid)) {
+      const int a = col / 3, comp = col % 3;
+      const double im = 1.0 / ((a % 3 == 0) ? massO : massH);
+      double u = 0.5 * im * (v * v);
+      if (j > 0) u = u / msf[j];
+      double *ch = vxi + (((size_t)a * P + j) * 3 + comp) * 2;
+      double v0 = ch[0], v1 = ch[1];
+      const double sc = nh_chain2(u, v0, v1, tau_mode[j], delt2, 1.0, tempP);
+      ch[0] = v0;
+      ch[1] = v1;
+      v = v * sc;
+    }
+    yp[e] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * CB; e += blockDim.x) {
+    const int j = e / CB, cc = e % CB, col = col0 + cc;
+    if (col >= ncols) continue;
+    double v = 0.0;
+    for (int k = 0; k < P; ++k) v = v + Cs[k * P + j] * yp[k * CB + cc];
+    PP[j * stride + col] = v;
+  }
+}
+
+void launch_bead_nh(const Group &g, int kick, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &m) {
+        const int CB = nm_cols(c.P), n = 3 * 3 * c.N;
+        return Plan{m.bead_tt == 2 ? 0 : -1, (n + CB - 1) / CB, 1, 1,
+                    sizeof(double) * ((size_t)c.P * c.P + 2 * (size_t)c.P * CB)};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t lds) {
+        hipLaunchKernelGGL(k_bead_nh, grid, dim3(256), lds, s, g.d, m, kick);
+      });
 }
 
 }  // namespace pimd

@@ -93,17 +93,23 @@ __device__ inline void spc_intra(const SysConst &c, const double r[3][3], double
 // energy and the screened intramolecular charge pairs (pot_spc.f90:55-137,
 // 202-233).  One thread per (bead, molecule).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_spc_monomer(SysConst c, const double *__restrict__ RR,
-                                                   Site *site, double *phi, double *Efq,
-                                                   double *dR, double *tally, double *chg_out,
-                                                   double *radius) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_spc_monomer(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ RR = Rp.b.RR;
+  Site *site = Rp.b.site;
+  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR, *tally = Rp.b.tally;
+  double *chg_out = Rp.b.chg, *radius = Rp.b.radius;
+  int *flags = Rp.b.flags;
+  const int gid = bk.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const double *rr = RR + ((size_t)b * 3 * N + 3 * w) * 3;
   double r[3][3];
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 3; ++k) r[a][k] = rr[3 * a + k];
+  flag_nonfinite(flags, r);
   const double qO = c.spc_q[0], qH = c.spc_q[1];
   if (chg_out) {
     double *co = chg_out + (size_t)b * 3 * N + 3 * w;
@@ -167,11 +173,15 @@ __global__ __launch_bounds__(64) void k_spc_monomer(SysConst c, const double *__
   for (int i = 0; i < 9; ++i) tl[T_VIRT + i] = vt[i];
 }
 
-void launch_spc_monomer(const SysConst &c, DevBufs &b, const double *RR, double *chg_out,
-                        hipStream_t s) {
-  const int n = c.P * c.N;
-  hipLaunchKernelGGL(k_spc_monomer, dim3((n + 63) / 64), dim3(64), 0, s, c, RR, b.site, b.phi,
-                     b.Efq, b.dR, b.tally, chg_out, b.radius);
+void launch_spc_monomer(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return Plan{c.model >= PIMD_QSPCFW ? 0 : -1, (c.P * c.N + 63) / 64, 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_spc_monomer, grid, dim3(64), 0, s, g.d, m);
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -185,13 +195,15 @@ void launch_spc_monomer(const SysConst &c, DevBufs &b, const double *RR, double 
 // and accumulation).
 // ---------------------------------------------------------------------------
 template <int G, typename R, bool LDS>
-__global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__restrict__ site,
-                                                 const int *__restrict__ nbr,
-                                                 const int *__restrict__ nbrc,
-                                                 const int *__restrict__ nbrn, double *phi,
-                                                 double *Efq, double *dR, double *tally) {
+__global__ __launch_bounds__(256) void k_spc_pair(const Rep *__restrict__ reps, BlockMap bm) {
   extern __shared__ Site lds_site[];
-  const int N = c.N, bb = blockIdx.y;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc, *__restrict__ nbrn = Rp.b.nbrn;
+  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR, *tally = Rp.b.tally;
+  const int N = c.N, bb = bk.y;
   if (LDS) {
     for (int e = threadIdx.x; e < 4 * N; e += blockDim.x) lds_site[e] = site[(size_t)bb * 4 * N + e];
     __syncthreads();
@@ -199,7 +211,7 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
   // the bead's molecule records: staged in LDS, or read from global memory
   // when they do not fit
   const Site *recs = LDS ? lds_site : site + (size_t)bb * 4 * N;
-  const int aa = blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int aa = bk.x * (256 / G) + threadIdx.x / G;
   const int lane = threadIdx.x % G;
   const bool valid = aa < N;
   const int a = valid ? aa : 0;
@@ -275,43 +287,56 @@ __global__ __launch_bounds__(256) void k_spc_pair(SysConst c, const Site *__rest
   }
 }
 
-template <int G, typename R>
-static void launch_spc_pair_g(const SysConst &c, DevBufs &b, hipStream_t s) {
-  dim3 grid((c.N + 256 / G - 1) / (256 / G), c.P);
-  const size_t lds = sizeof(Site) * 4 * c.N;
-  if (lds <= 150 * 1024)
-    hipLaunchKernelGGL((k_spc_pair<G, R, true>), grid, dim3(256), lds, s, c, b.site, b.nbr,
-                       b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
-  else
-    hipLaunchKernelGGL((k_spc_pair<G, R, false>), grid, dim3(256), 0, s, c, b.site, b.nbr,
-                       b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
+// Variant: bit 0 FP32 pair arithmetic, bit 1 records staged in LDS, bits 2..
+// the lanes per molecule (4, 8, 16, 32).
+template <typename R, bool LDS>
+static void launch_spc_pair_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const Group &g,
+                              hipStream_t s) {
+  switch (gi) {
+    case 0: hipLaunchKernelGGL((k_spc_pair<4, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 2: hipLaunchKernelGGL((k_spc_pair<16, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 3: hipLaunchKernelGGL((k_spc_pair<32, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    default: hipLaunchKernelGGL((k_spc_pair<8, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+  }
 }
 
-template <typename R>
-static void launch_spc_pair_r(const SysConst &c, DevBufs &b, hipStream_t s) {
-  if (c.g_qq == 4) launch_spc_pair_g<4, R>(c, b, s);
-  else if (c.g_qq == 16) launch_spc_pair_g<16, R>(c, b, s);
-  else if (c.g_qq == 32) launch_spc_pair_g<32, R>(c, b, s);
-  else launch_spc_pair_g<8, R>(c, b, s);
-}
-
-void launch_spc_pair(const SysConst &c, DevBufs &b, hipStream_t s) {
-  if (c.mixed) launch_spc_pair_r<float>(c, b, s);
-  else launch_spc_pair_r<double>(c, b, s);
+void launch_spc_pair(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (c.model < PIMD_QSPCFW) return Plan{-1, 0, 0, 0, 0};
+        const int G = (c.g_qq == 4 || c.g_qq == 16 || c.g_qq == 32) ? c.g_qq : 8;
+        const int gi = G == 4 ? 0 : G == 8 ? 1 : G == 16 ? 2 : 3;
+        const size_t lds = sizeof(Site) * 4 * c.N;
+        const bool in_lds = lds <= 150 * 1024;
+        return Plan{(c.mixed ? 1 : 0) | (in_lds ? 2 : 0) | (gi << 2),
+                    (c.N + 256 / G - 1) / (256 / G), c.P, 1, in_lds ? lds : 0};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        const int gi = v >> 2;
+        switch (v & 3) {
+          case 0: launch_spc_pair_v<double, false>(gi, grid, m, lds, g, s); break;
+          case 1: launch_spc_pair_v<float, false>(gi, grid, m, lds, g, s); break;
+          case 2: launch_spc_pair_v<double, true>(gi, grid, m, lds, g, s); break;
+          default: launch_spc_pair_v<float, true>(gi, grid, m, lds, g, s); break;
+        }
+      });
 }
 
 // ---------------------------------------------------------------------------
 // Per-molecule finish (pot_spc.f90:242-275): Uelec partials, the charge
 // gradient -q Efq, the gradient in OHH order and the molecular dipoles.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_spc_finalize(SysConst c, const Site *__restrict__ site,
-                                                    const double *__restrict__ phi,
-                                                    const double *__restrict__ Efq,
-                                                    const double *__restrict__ kphi,
-                                                    const double *__restrict__ kE,
-                                                    const double *__restrict__ dR, double *tally,
-                                                    double *dRR, double *dipI, double *dipE) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_spc_finalize(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ phi = Rp.b.phi, *__restrict__ Efq = Rp.b.Efq;
+  const double *__restrict__ kphi = Rp.b.kphi, *__restrict__ kE = Rp.b.kE;
+  const double *__restrict__ dR = Rp.b.dR;
+  double *tally = Rp.b.tally, *dRR = Rp.b.dRR, *dipI = Rp.b.dipI, *dipE = Rp.b.dipE;
+  const int gid = bk.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
   const int b = gid / c.N, w = gid % c.N, N = c.N;
   const size_t base = ((size_t)b * N + w) * 4;
@@ -343,11 +368,15 @@ __global__ __launch_bounds__(64) void k_spc_finalize(SysConst c, const Site *__r
   tally[((size_t)b * N + w) * T_COUNT + T_UELEC] = ue;
 }
 
-void launch_spc_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI, double *dipE,
-                         hipStream_t s) {
-  const int n = c.P * c.N;
-  hipLaunchKernelGGL(k_spc_finalize, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.phi, b.Efq,
-                     b.kphi, b.kE, b.dR, b.tally, dRR, dipI, dipE);
+void launch_spc_finalize(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return Plan{c.model >= PIMD_QSPCFW ? 0 : -1, (c.P * c.N + 63) / 64, 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_spc_finalize, grid, dim3(64), 0, s, g.d, m);
+      });
 }
 
 }  // namespace pimd

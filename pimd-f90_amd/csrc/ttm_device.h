@@ -9,6 +9,17 @@
 
 namespace pimd {
 
+// Bit 0 of *flags is set when a molecule's coordinates are not finite; the
+// host reads it at the next synchronising call (the monomer kernels test
+// their molecule as they load it).
+__device__ __forceinline__ void flag_nonfinite(int *flags, const double (&r)[3][3]) {
+  bool ok = true;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) ok = ok && __builtin_isfinite(r[a][k]);
+  if (!ok && flags) atomicOr(flags, 1);
+}
+
+
 // Exponent tables of the PES and DMS bases and the DMS coefficients, built
 // at compile time into the constant memory of each unit that uses them (the
 // PES coefficients are per context).

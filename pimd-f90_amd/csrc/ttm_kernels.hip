@@ -57,15 +57,19 @@ __device__ __forceinline__ double site_radius(const SysConst &c, const Site *m) 
 // index, and the table reads it drives, stay uniform across a wave) and the
 // shares are added through LDS; wave 0 writes the results.
 constexpr int MONO_W = 4;
-__global__ __launch_bounds__(64 * MONO_W) void k_monomer(SysConst c, const double *__restrict__ RR,
-                                                        const double *__restrict__ RRc, Site *site,
-                                                        double *Rcs, double *grdq, double *phi,
-                                                        double *Efq, double *dR, double *tally,
-                                                        double *chg_out, double *radius,
-                                                        const double *__restrict__ pes) {
+__global__ __launch_bounds__(64 * MONO_W) void k_monomer(const Rep *__restrict__ reps, BlockMap bm) {
   __shared__ double lds_part[MONO_W * 8 * 64];
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ RR = Rp.b.RR, *__restrict__ RRc = Rp.b.RRc;
+  Site *site = Rp.b.site;
+  double *Rcs = Rp.b.Rcs, *grdq = Rp.b.grdq, *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR;
+  double *tally = Rp.b.tally, *chg_out = Rp.b.chg, *radius = Rp.b.radius;
+  const double *__restrict__ pes = Rp.b.pes;
+  int *flags = Rp.b.flags;
   const int wv = threadIdx.x >> 6;
-  const int gid0 = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int gid0 = bk.x * 64 + (threadIdx.x & 63);
   const bool valid = gid0 < c.P * c.N;
   const int gid = valid ? gid0 : 0;  // spare lanes take part in the sums, write nothing
   const int b = gid / c.N, w = gid % c.N, N = c.N;
@@ -78,6 +82,8 @@ __global__ __launch_bounds__(64 * MONO_W) void k_monomer(SysConst c, const doubl
   if (wv != 0 || !valid) return;
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 3; ++k) rc[a][k] = RRc[(3 * w + a) * 3 + k];
+  flag_nonfinite(flags, r);
+  if (b == 0) flag_nonfinite(flags, rc);
   const double gM = c.gammaM;
   double pos[4][3], cen[4][3];
   {
@@ -194,12 +200,6 @@ __global__ __launch_bounds__(64 * MONO_W) void k_monomer(SysConst c, const doubl
   for (int i = 0; i < 9; ++i) tl[T_VIRT + i] = vt[i];
 }
 
-void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
-                    double *chg_out, hipStream_t s) {
-  const int n = c.P * c.N;
-  hipLaunchKernelGGL(k_monomer, dim3((n + 63) / 64), dim3(64 * MONO_W), 0, s, c, RR, RRc, b.site,
-                     b.Rcs, b.grdq, b.phi, b.Efq, b.dR, b.tally, chg_out, b.radius, b.pes);
-}
 
 // ---------------------------------------------------------------------------
 // Contracted MD (MD.f90:276-449).
@@ -208,13 +208,17 @@ void launch_monomer(const SysConst &c, DevBufs &b, const double *RR, const doubl
 // Fast (intramolecular) forces: the PES gradient of every monomer of every
 // bead into dRR (MD.f90:339-353).  With record, also per molecule the energy,
 // the bead virial and the centroid virial (MD.f90:355-372) into mol[P][N][3].
-__global__ __launch_bounds__(64 * MONO_W) void k_pes_fast(SysConst c, const double *__restrict__ RR,
-                                                         const double *__restrict__ RRc,
-                                                         const double *__restrict__ pes,
-                                                         double *dRR, double *mol, int record) {
+__global__ __launch_bounds__(64 * MONO_W) void k_pes_fast(const Rep *__restrict__ reps, BlockMap bm,
+                                                         int record) {
   __shared__ double lds_part[MONO_W * 8 * 64];
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ RR = Rp.b.RR, *__restrict__ RRc = Rp.b.RRc;
+  const double *__restrict__ pes = Rp.b.pes;
+  double *dRR = Rp.b.dRRfast, *mol = Rp.b.mol;
   const int wv = threadIdx.x >> 6;
-  const int gid0 = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int gid0 = bk.x * 64 + (threadIdx.x & 63);
   const bool valid = gid0 < c.P * c.N;
   const int gid = valid ? gid0 : 0;  // spare lanes take part in the sums, write nothing
   const int b = gid / c.N, w = gid % c.N, N = c.N;
@@ -246,36 +250,28 @@ __global__ __launch_bounds__(64 * MONO_W) void k_pes_fast(SysConst c, const doub
   m[2] = vc;
 }
 
-void launch_pes_fast(const SysConst &c, DevBufs &b, const double *RR, const double *RRc,
-                     double *dRR, double *mol, bool record, hipStream_t s) {
-  const int n = c.P * c.N;
-  hipLaunchKernelGGL(k_pes_fast, dim3((n + 63) / 64), dim3(64 * MONO_W), 0, s, c, RR, RRc, b.pes,
-                     dRR, mol, record ? 1 : 0);
-}
 
 // The centroid's gradient for every bead (MD.f90:400-403).
-__global__ void k_broadcast_beads(int P, int n, const double *__restrict__ g, double *dRR) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < (size_t)P * n) dRR[i] = g[i % n];
+__global__ void k_broadcast_beads(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const int P = Rp.c.P, n = 9 * Rp.c.N;
+  const size_t i = (size_t)bk.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)P * n) Rp.b.dRR[i] = Rp.b.cen_dRR[i % n];
 }
 
-void launch_broadcast_beads(int P, int n, const double *g, double *dRR, hipStream_t s) {
-  const size_t m = (size_t)P * n;
-  hipLaunchKernelGGL(k_broadcast_beads, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, P, n,
-                     g, dRR);
-}
 
 // Step record of a contracted step (MD.f90:405-418): sum of the bead
 // energies (monomers of every bead + P times the centroid energy), virial,
 // centroid virial, kinetic energy; molecule terms summed in a fixed order.
-__global__ __launch_bounds__(256) void k_contract_record(int P, int N,
-                                                        const double *__restrict__ mol,
-                                                        const double *__restrict__ Ucen,
-                                                        const double *__restrict__ vtcen,
-                                                        const double *__restrict__ vccen,
-                                                        const double *__restrict__ uk,
-                                                        double *rec0, const int *__restrict__ clk) {
-  double *rec = rec0 + (size_t)clk[1] * MD_REC;
+__global__ __launch_bounds__(256) void k_contract_record(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const int P = Rp.c.P, N = Rp.c.N;
+  const double *__restrict__ mol = Rp.b.mol, *__restrict__ uk = Rp.b.uk;
+  const double *__restrict__ Ucen = Rp.b.cen_Upot, *__restrict__ vtcen = Rp.b.cen_virt;
+  const double *__restrict__ vccen = Rp.b.cen_virialc;
+  double *rec = Rp.m.rec + (size_t)Rp.b.clk[1] * MD_REC;
   __shared__ double red[3][256];
   double s[3] = {0, 0, 0};
   for (int i = threadIdx.x; i < P * N; i += 256)
@@ -295,12 +291,6 @@ __global__ __launch_bounds__(256) void k_contract_record(int P, int N,
   }
 }
 
-void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
-                            const double *vtcen, const double *vccen, const double *uk,
-                            double *rec0, const int *clk, hipStream_t s) {
-  hipLaunchKernelGGL(k_contract_record, dim3(1), dim3(256), 0, s, P, N, mol, Ucen, vtcen, vccen,
-                     uk, rec0, clk);
-}
 
 // ---------------------------------------------------------------------------
 // Neighbour lists (find_neigh.f90:14-120): for every bead and molecule i, all
@@ -317,11 +307,15 @@ void launch_contract_record(int P, int N, const double *mol, const double *Ucen,
 // bead's O positions and radii are staged there once per block.
 constexpr int NB_I = 16;
 template <bool LDS>
-__global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__restrict__ site,
-                                                  const double *__restrict__ radius, int *nbr,
-                                                  int *nbrc, int *nbrn) {
+__global__ __launch_bounds__(256) void k_neighbors(const Rep *__restrict__ reps, BlockMap bm) {
   extern __shared__ double lds_o[];  // [N][4]: x, y, z, radius
-  const int b = blockIdx.y, N = c.N;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ radius = Rp.b.radius;
+  int *nbr = Rp.b.nbr, *nbrc = Rp.b.nbrc, *nbrn = Rp.b.nbrn;
+  const int b = bk.y, N = c.N;
   if (LDS) {
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
       const Site o = site[((size_t)b * N + j) * 4];
@@ -335,7 +329,7 @@ __global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__res
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int ii = wv; ii < NB_I; ii += blockDim.x >> 6) {
-    const int i = blockIdx.x * NB_I + ii;
+    const int i = bk.x * NB_I + ii;
     if (i >= N) break;
     double xi, yi, zi, radi;
     if (LDS) {
@@ -382,10 +376,14 @@ __global__ __launch_bounds__(256) void k_neighbors(SysConst c, const Site *__res
 // (its position wrapped into the box), a counting sort into cell_mol with
 // cell_start[c] the first entry of cell c, and each cell's molecules put in
 // ascending order so that the lists do not depend on the order of atomics.
-__global__ __launch_bounds__(256) void k_cells(SysConst c, const Site *__restrict__ site,
-                                              int *cell_start, int *cell_mol) {
+__global__ __launch_bounds__(256) void k_cells(const Rep *__restrict__ reps, BlockMap bm) {
   extern __shared__ int lds_cells[];  // [ncells] counts, then fill pointers
-  const int b = blockIdx.x, N = c.N;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  int *cell_start = Rp.b.cell_start, *cell_mol = Rp.b.cell_mol;
+  const int b = bk.x, N = c.N;
   const int nx = c.ncell[0], ny = c.ncell[1], nz = c.ncell[2], nc = nx * ny * nz;
   int *cnt = lds_cells, *fill = lds_cells + nc;
   int *start = cell_start + (size_t)b * (nc + 1);
@@ -436,12 +434,15 @@ __global__ __launch_bounds__(256) void k_cells(SysConst c, const Site *__restric
 // cell (cells at least Rc wide, at least three per axis, so every partner
 // inside Rc is among them once).  Partners come in cell order, ascending
 // inside each cell.
-__global__ __launch_bounds__(256) void k_neighbors_cells(SysConst c, const Site *__restrict__ site,
-                                                        const double *__restrict__ radius,
-                                                        const int *__restrict__ cell_start,
-                                                        const int *__restrict__ cell_mol,
-                                                        int *nbr, int *nbrc, int *nbrn) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__global__ __launch_bounds__(256) void k_neighbors_cells(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ radius = Rp.b.radius;
+  const int *__restrict__ cell_start = Rp.b.cell_start, *__restrict__ cell_mol = Rp.b.cell_mol;
+  int *nbr = Rp.b.nbr, *nbrc = Rp.b.nbrc, *nbrn = Rp.b.nbrn;
+  const int wave = (bk.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= c.P * c.N) return;
   const int b = wave / c.N, i = wave % c.N, N = c.N;
@@ -494,25 +495,6 @@ __global__ __launch_bounds__(256) void k_neighbors_cells(SysConst c, const Site 
   }
 }
 
-void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
-  if (c.cells) {
-    const int nc = c.ncell[0] * c.ncell[1] * c.ncell[2];
-    hipLaunchKernelGGL(k_cells, dim3(c.P), dim3(256), sizeof(int) * 2 * nc, s, c, b.site,
-                       b.cell_start, b.cell_mol);
-    const long threads = (long)c.P * c.N * 64;
-    hipLaunchKernelGGL(k_neighbors_cells, dim3((threads + 255) / 256), dim3(256), 0, s, c, b.site,
-                       b.radius, b.cell_start, b.cell_mol, b.nbr, b.nbrc, b.nbrn);
-    return;
-  }
-  dim3 grid((c.N + NB_I - 1) / NB_I, c.P);
-  const size_t lds = sizeof(double) * 4 * c.N;
-  if (lds <= 64 * 1024)
-    hipLaunchKernelGGL(k_neighbors<true>, grid, dim3(256), lds, s, c, b.site, b.radius, b.nbr,
-                       b.nbrc, b.nbrn);
-  else
-    hipLaunchKernelGGL(k_neighbors<false>, grid, dim3(256), 0, s, c, b.site, b.radius, b.nbr,
-                       b.nbrc, b.nbrn);
-}
 
 // ---------------------------------------------------------------------------
 // Real-space vdW + charge-charge pass (pot_ttm.f90:159-228).
@@ -524,32 +506,34 @@ void launch_neighbors(const SysConst &c, DevBufs &b, hipStream_t s) {
 // blocks, beads)) and first copies the bead's molecule records into LDS, so
 // neighbour sites are read from LDS instead of gathered from L2.
 template <int MODEL, int G, typename R, bool LDS = false>
-__global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restrict__ site,
-                                                const double *__restrict__ Rcs,
-                                                const int *__restrict__ nbr,
-                                                const int *__restrict__ nbrc,
-                                                const int *__restrict__ nbrn, double *phi,
-                                                double *Efq, double *dR, double *tally) {
+__global__ __launch_bounds__(256) void k_pair_qq(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int SA0 = MODEL == 2 ? 0 : 1;  // TTM3-F: O carries neither charge nor dipole
   // sites of j: unrolled in the FP32 form (more independent work per lane
   // at the same occupancy), one at a time in the FP64 form (fewer live
   // registers: two waves per SIMD instead of one)
   constexpr int SB_UNROLL = sizeof(R) == 4 ? 3 : 1;
   extern __shared__ Site lds_site[];
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ Rcs = Rp.b.Rcs;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc, *__restrict__ nbrn = Rp.b.nbrn;
+  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR, *tally = Rp.b.tally;
   const int N = c.N;
   long grp;
   int lane;
   bool valid;
   if (LDS) {
-    const int bb = blockIdx.y;
+    const int bb = bk.y;
     for (int e = threadIdx.x; e < 4 * N; e += blockDim.x) lds_site[e] = site[(size_t)bb * 4 * N + e];
     __syncthreads();
-    const int aa = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int aa = bk.x * (256 / G) + threadIdx.x / G;
     lane = threadIdx.x % G;
     valid = aa < N;
     grp = (long)bb * N + (valid ? aa : 0);
   } else {
-    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gtid = (long)bk.x * blockDim.x + threadIdx.x;
     grp = gtid / G;
     lane = (int)(gtid % G);
     valid = grp < (long)c.P * N;
@@ -666,62 +650,24 @@ __global__ __launch_bounds__(256) void k_pair_qq(SysConst c, const Site *__restr
 static size_t bead_sites_lds(const SysConst &c) { return sizeof(Site) * 4 * c.N; }
 constexpr size_t PAIR_LDS_MAX = 150 * 1024;
 
-template <int G, typename R>
-static void launch_qq_g(const SysConst &c, DevBufs &b, hipStream_t s) {
-  long threads = (long)c.P * c.N * G;
-  dim3 grid((threads + 255) / 256);
-  if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_qq<2, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr,
-                       b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
-  else
-    hipLaunchKernelGGL((k_pair_qq<3, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.nbr,
-                       b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
-}
 
-template <int G, typename R>
-static void launch_qq_lds_g(const SysConst &c, DevBufs &b, hipStream_t s) {
-  dim3 grid((c.N + 256 / G - 1) / (256 / G), c.P);
-  const size_t lds = bead_sites_lds(c);
-  if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_qq<2, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
-                       b.nbr, b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
-  else
-    hipLaunchKernelGGL((k_pair_qq<3, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
-                       b.nbr, b.nbrc, b.nbrn, b.phi, b.Efq, b.dR, b.tally);
-}
 
-template <typename R>
-static void launch_qq_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
-  if (c.g_qq > 0 && bead_sites_lds(c) <= PAIR_LDS_MAX) {
-    if (c.g_qq == 4) launch_qq_lds_g<4, R>(c, b, s);
-    else if (c.g_qq == 8) launch_qq_lds_g<8, R>(c, b, s);
-    else if (c.g_qq == 32) launch_qq_lds_g<32, R>(c, b, s);
-    else launch_qq_lds_g<16, R>(c, b, s);
-    return;
-  }
-  if (group == 64)
-    launch_qq_g<64, R>(c, b, s);
-  else if (group == 32)
-    launch_qq_g<32, R>(c, b, s);
-  else
-    launch_qq_g<16, R>(c, b, s);
-}
 
-void launch_pair_qq(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
-  if (c.mixed) launch_qq_r<float>(c, b, group, s);
-  else launch_qq_r<double>(c, b, group, s);
-}
 
 // ---------------------------------------------------------------------------
 // Ewald reciprocal space (ewald.f90:1-226, 403-437).
 // ---------------------------------------------------------------------------
 
 // Per-site phase tables, the reference's ssincos recurrence (ewald.f90:403-437).
-__global__ __launch_bounds__(256) void k_tables(SysConst c, const Site *__restrict__ site,
-                                               double2 *tab) {
+__global__ __launch_bounds__(256) void k_tables(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  double2 *tab = Rp.b.tab;
   const int S = 4 * c.N;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
+  const int s = bk.x * blockDim.x + threadIdx.x;
+  const int b = bk.y;
   if (s >= S) return;
   const Site p = site[(size_t)b * S + s];
   const double xyz[3] = {p.x, p.y, p.z};
@@ -749,10 +695,6 @@ __global__ __launch_bounds__(256) void k_tables(SysConst c, const Site *__restri
   }
 }
 
-void launch_tables(const SysConst &c, DevBufs &b, hipStream_t s) {
-  const int S = 4 * c.N;
-  hipLaunchKernelGGL(k_tables, dim3((S + 255) / 256, c.P), dim3(256), 0, s, c, b.site, b.tab);
-}
 
 // Site index of the m-th member of a site set.  SET_CHG: H1, H2, M of every
 // molecule; SET_DIP: the dipole sites; SET_ALL: all four.
@@ -782,6 +724,28 @@ __device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, doub
   sk = sxy * z.x + cxy * z.y;
 }
 
+// Site chunk of the column structure-factor kernel and its padded LDS row
+// (a multiple of 16 plus one double2, so rows start 4 banks apart).
+__host__ __device__ inline void sfac_chunking(const SysConst &c, int mode, int M, int &nch,
+                                              int &sc, int &scp) {
+  // the dipole structure factor takes c.sf_sites per chunk; the charge and
+  // virial ones at most 32, which keeps their blocks spread over the device
+  const int sites = mode == SF_D ? c.sf_sites : (c.sf_sites < 32 ? c.sf_sites : 32);
+  const int want = (M + sites - 1) / sites;
+  nch = want < c.sf_chunks_max ? want : c.sf_chunks_max;
+  if (nch < 1) nch = 1;
+  sc = (M + nch - 1) / nch;
+  if (sc < 1) sc = 1;  // an empty site set (no dipole sites): one empty chunk
+  nch = (M + sc - 1) / sc;
+  if (nch < 1) nch = 1;
+  scp = ((sc + 15) / 16) * 16 + 1;
+}
+
+// Sites of a structure factor's site set.
+__host__ __device__ inline int sfac_sites(const SysConst &c, int mode) {
+  return mode == SF_Q ? 3 * c.N : c.nd * c.N;
+}
+
 // Structure factors with one kz-segment (fixed kx, ky, up to KZ consecutive kz)
 // per lane: the phase of each site is formed once per segment and advanced
 // along kz by e^{i 2 pi z/L}, accumulating every kz of the segment in
@@ -794,13 +758,7 @@ __device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, doub
 // half of the chunk's sites; the second hands its sums to the first through
 // LDS, which adds them and writes (twice the waves for the same partials).
 template <int MODE, int KZ, int SPLIT = 1>
-__global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__restrict__ site,
-                                                 const double2 *__restrict__ tab,
-                                                 const double *__restrict__ dip,
-                                                 const int4 *__restrict__ cols,
-                                                 const int *__restrict__ colk,
-                                                 const int *__restrict__ active,
-                                                 double *__restrict__ part, int sc, int scp) {
+__global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
   constexpr int NV = MODE == SF_V ? 6 : 2;  // values per k
@@ -808,11 +766,23 @@ __global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__rest
   // per SIMD, so more independent phase chains per wave hide FP64 latency
   constexpr int SITE_UNROLL = MODE == SF_V ? 2 : 4;
   extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double2 *__restrict__ tab = Rp.b.tab;
+  const double *__restrict__ dip = Rp.b.dip;
+  const int4 *__restrict__ cols = Rp.b.segs;
+  const int *__restrict__ colk = Rp.b.segk;
+  const int *__restrict__ active = Rp.b.active;
+  double *__restrict__ part = Rp.b.part;
+  int nch, sc, scp;
+  sfac_chunking(c, MODE, sfac_sites(c, MODE), nch, sc, scp);
   double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
-  const int b = blockIdx.z;
+  const int b = bk.z;
   if (MODE == SF_D && !active[b]) return;
   const int S = 4 * c.N, M = set_size(c, SET);
-  const int nch = gridDim.y, chunk = blockIdx.y;
+  const int chunk = bk.y;
   const int m0 = chunk * sc, nt = min(sc, M - m0);
   const double2 *tb = tab + (size_t)b * c.T * S;
   for (int e = threadIdx.x; e < c.T * sc; e += blockDim.x) {
@@ -832,7 +802,7 @@ __global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__rest
   }
   __syncthreads();
   const int nl = blockDim.x / SPLIT, half_id = threadIdx.x / nl;
-  const int col = blockIdx.x * nl + threadIdx.x % nl;
+  const int col = bk.x * nl + threadIdx.x % nl;
   const bool live = col < c.nseg;
   const int hn = (nt + SPLIT - 1) / SPLIT;
   const int e0 = half_id * hn, e1 = min(nt, e0 + hn);
@@ -927,22 +897,28 @@ __global__ __launch_bounds__(512) void k_sfac_col(SysConst c, const Site *__rest
 //   SF_D: (-sum (k.mu) sin, sum (k.mu) cos)           (ewald.f90:99-110)
 //   SF_V: (sum mu cos, sum mu sin) per component      (ewald.f90:208-213)
 template <int MODE>
-__global__ __launch_bounds__(256) void k_sfac_k(SysConst c, const Site *__restrict__ site,
-                                               const double2 *__restrict__ tab,
-                                               const double *__restrict__ dip,
-                                               const double4 *__restrict__ kv,
-                                               const int4 *__restrict__ khkl,
-                                               const int *__restrict__ active,
-                                               double *__restrict__ part, int sc, int scp) {
+__global__ __launch_bounds__(256) void k_sfac_k(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
   constexpr int NV = MODE == SF_V ? 6 : 2;
   extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double2 *__restrict__ tab = Rp.b.tab;
+  const double *__restrict__ dip = Rp.b.dip;
+  const double4 *__restrict__ kv = Rp.b.kv;
+  const int4 *__restrict__ khkl = Rp.b.khkl;
+  const int *__restrict__ active = Rp.b.active;
+  double *__restrict__ part = Rp.b.part;
+  int nch, sc, scp;
+  sfac_chunking(c, MODE, sfac_sites(c, MODE), nch, sc, scp);
   double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
-  const int b = blockIdx.z;
+  const int b = bk.z;
   if (MODE == SF_D && !active[b]) return;
   const int S = 4 * c.N, M = set_size(c, SET);
-  const int nch = gridDim.y, chunk = blockIdx.y;
+  const int chunk = bk.y;
   const int m0 = chunk * sc, nt = min(sc, M - m0);
   const double2 *tb = tab + (size_t)b * c.T * S;
   for (int e = threadIdx.x; e < c.T * sc; e += blockDim.x) {
@@ -960,7 +936,7 @@ __global__ __launch_bounds__(256) void k_sfac_k(SysConst c, const Site *__restri
     }
   }
   __syncthreads();
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = bk.x * blockDim.x + threadIdx.x;
   if (k >= c.nk) return;
   const int4 h = khkl[k];
   const double4 kvec = kv[k];
@@ -1000,13 +976,20 @@ __global__ __launch_bounds__(256) void k_sfac_k(SysConst c, const Site *__restri
   for (int v = 0; v < NV; ++v) o[v] = acc[v];
 }
 
-__global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *__restrict__ part,
-                                                    int nch, int nv,
-                                                    const int *__restrict__ active,
-                                                    int check_active, double *__restrict__ out) {
-  const int b = blockIdx.y;
-  if (check_active && !active[b]) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, component)
+// The per-chunk partial sums of structure factor `mode` added in chunk order.
+__global__ __launch_bounds__(256) void k_sfac_reduce(const Rep *__restrict__ reps, BlockMap bm,
+                                                    int mode) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ part = Rp.b.part;
+  double *__restrict__ out = mode == SF_Q ? Rp.b.Sq : mode == SF_D ? Rp.b.Sm : Rp.b.Sv;
+  const int nv = mode == SF_V ? 6 : 2;
+  int nch, sc, scp;
+  sfac_chunking(c, mode, sfac_sites(c, mode), nch, sc, scp);
+  const int b = bk.y;
+  if (mode == SF_D && !Rp.b.active[b]) return;
+  const int e = bk.x * blockDim.x + threadIdx.x;  // (k, component)
   if (e >= nv * c.nk) return;
   double v = 0.0;
   const double *pp = part + (size_t)b * nch * c.nk * nv + e;
@@ -1015,87 +998,26 @@ __global__ __launch_bounds__(256) void k_sfac_reduce(SysConst c, const double *_
   out[(size_t)b * c.nk * nv + e] = v;
 }
 
-// Site chunk of the column structure-factor kernel and its padded LDS row
-// (a multiple of 16 plus one double2, so rows start 4 banks apart).
-static void sfac_chunking(const SysConst &c, int mode, int M, int &nch, int &sc, int &scp) {
-  // the dipole structure factor takes c.sf_sites per chunk; the charge and
-  // virial ones at most 32, which keeps their blocks spread over the device
-  const int sites = mode == SF_D ? c.sf_sites : std::min(c.sf_sites, 32);
-  nch = std::max(1, std::min(c.sf_chunks_max, (M + sites - 1) / sites));
-  sc = (M + nch - 1) / nch;
-  nch = (M + sc - 1) / sc;
-  scp = ((sc + 15) / 16) * 16 + 1;
-}
-
 static size_t sfac_lds(const SysConst &c, int mode, int sc, int scp) {
   return sizeof(double2) * c.T * scp + sizeof(double) * sc * (mode == SF_Q ? 1 : 3);
 }
 
-
-template <int MODE, int KZ>
-static void launch_sfac_col(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
-  constexpr int NV = MODE == SF_V ? 6 : 2;
-  int nch, sc, scp;
-  sfac_chunking(c, MODE, M, nch, sc, scp);
-  const int threads = std::min(256, ((c.nseg + 63) / 64) * 64);
-  dim3 grid((c.nseg + threads - 1) / threads, nch, c.P);
-  const size_t lds = sfac_lds(c, MODE, sc, scp);
-  // two lane sets per block where the second set's sums fit beside the tables
-  const size_t red = sizeof(double) * ((size_t)NV * KZ * threads + 2);
-  if (MODE != SF_V && c.sf_split && lds + red <= 96 * 1024) {
-    hipLaunchKernelGGL((k_sfac_col<MODE, KZ, 2>), grid, dim3(2 * threads), lds + red, s, c,
-                       b.site, b.tab, b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
-  } else {
-    hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), lds, s, c, b.site, b.tab,
-                       b.dip, b.segs, b.segk, b.active, b.part, sc, scp);
+// The most site chunks any structure factor of a force call uses (the
+// partial sums hold P x this many chunks of nk k-vectors).
+int sfac_chunks_used(const SysConst &c) {
+  int most = 1;
+  for (int mode : {SF_Q, SF_D, SF_V}) {
+    int nch, sc, scp;
+    sfac_chunking(c, mode, sfac_sites(c, mode), nch, sc, scp);
+    most = std::max(most, nch);
   }
-  dim3 g2((NV * c.nk + 255) / 256, c.P);
-  double *out = MODE == SF_Q ? b.Sq : MODE == SF_D ? b.Sm : b.Sv;
-  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, NV, b.active,
-                     MODE == SF_D ? 1 : 0, out);
+  return most;
 }
 
-template <int MODE>
-static bool sfac_col_dispatch(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
-  int nch, sc, scp;
-  sfac_chunking(c, MODE, M, nch, sc, scp);
-  if (sfac_lds(c, MODE, sc, scp) > 64 * 1024) return false;
-  if (c.segmax <= 4) launch_sfac_col<MODE, 4>(c, b, M, s);
-  else if (c.segmax <= 8) launch_sfac_col<MODE, 8>(c, b, M, s);
-  else if (c.segmax <= 12) launch_sfac_col<MODE, 12>(c, b, M, s);
-  else if (c.segmax <= 16) launch_sfac_col<MODE, 16>(c, b, M, s);
-  else if (c.segmax <= 24) launch_sfac_col<MODE, 24>(c, b, M, s);
-  else if (c.segmax <= 32) launch_sfac_col<MODE, 32>(c, b, M, s);
-  else return false;
-  return true;
-}
 
-template <int MODE>
-static void launch_sfac_k(const SysConst &c, DevBufs &b, int M, hipStream_t s) {
-  constexpr int NV = MODE == SF_V ? 6 : 2;
-  int nch, sc, scp;
-  sfac_chunking(c, MODE, M, nch, sc, scp);
-  dim3 grid((c.nk + 255) / 256, nch, c.P);
-  hipLaunchKernelGGL((k_sfac_k<MODE>), grid, dim3(256), sfac_lds(c, MODE, sc, scp), s, c, b.site,
-                     b.tab, b.dip, b.kv, b.khkl, b.active, b.part, sc, scp);
-  dim3 g2((NV * c.nk + 255) / 256, c.P);
-  double *out = MODE == SF_Q ? b.Sq : MODE == SF_D ? b.Sm : b.Sv;
-  hipLaunchKernelGGL(k_sfac_reduce, g2, dim3(256), 0, s, c, b.part, nch, NV, b.active,
-                     MODE == SF_D ? 1 : 0, out);
-}
 
-void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
-  const int M = mode == SF_Q ? 3 * c.N : c.nd * c.N;
-  if (c.sf_kind == 1 && (mode != SF_V || c.sfv_col)) {
-    const bool ok = mode == SF_Q   ? sfac_col_dispatch<SF_Q>(c, b, M, s)
-                    : mode == SF_D ? sfac_col_dispatch<SF_D>(c, b, M, s)
-                                   : sfac_col_dispatch<SF_V>(c, b, M, s);
-    if (ok) return;
-  }
-  if (mode == SF_Q) launch_sfac_k<SF_Q>(c, b, M, s);
-  else if (mode == SF_D) launch_sfac_k<SF_D>(c, b, M, s);
-  else launch_sfac_k<SF_V>(c, b, M, s);
-}
+
+
 
 // Reciprocal-space fields at each site.  A block holds 64 sites of one bead,
 // one per lane, and CG waves; wave g walks kz-columns g, g + CG, ... (columns
@@ -1112,22 +1034,11 @@ void launch_sfac(const SysConst &c, DevBufs &b, int mode, hipStream_t s) {
 //   FD_QD: phi and dR from the stale dipole and the total structure factors
 //          (ewald.f90:184-193)
 constexpr int FD_SB = 64;  // sites per block, one per lane
+// iter: the SOR iteration (FD_D), or -1 for the mixed form's FP64
+// refinement, the step after the replica's last (maxiter + 1).
 template <int MODE, int MODEL, int CG, bool KL>
-__global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__restrict__ site,
-                                                   const double2 *__restrict__ tab,
-                                                   const double4 *__restrict__ kv,
-                                                   const int4 *__restrict__ cols,
-                                                   const int *__restrict__ colk,
-                                                   const double *__restrict__ Sq,
-                                                   const double *__restrict__ Sm,
-                                                   double *__restrict__ phi,
-                                                   double *__restrict__ Efq,
-                                                   double *__restrict__ dR,
-                                                   double *__restrict__ dip,
-                                                   double *__restrict__ dip_prev,
-                                                   const double *__restrict__ Efd,
-                                                   double *__restrict__ diff2,
-                                                   const int *__restrict__ active, int iter) {
+__global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps, BlockMap bm,
+                                                   int iter) {
   constexpr int SET = MODE == FD_D    ? SET_DIP
                       : MODE == FD_QD ? (MODEL == 2 ? SET_ALL_T : SET_CHG_T)
                                       : (MODEL == 2 ? SET_ALL : SET_CHG);
@@ -1135,13 +1046,32 @@ __global__ __launch_bounds__(64 * CG) void k_field(SysConst c, const Site *__res
   // reduction), then with KL the bead's k data: [nk] (w Re S, w Im S, kz, w)
   // with w = 4 pi A_k / V, and for FD_QD [nk] (w Re(Sq+Sm), w Im(Sq+Sm))
   extern __shared__ double2 lds_tab[];
-  const int b = blockIdx.y;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double2 *__restrict__ tab = Rp.b.tab;
+  const double4 *__restrict__ kv = Rp.b.kv;
+  const int4 *__restrict__ cols = Rp.b.cols;
+  const int *__restrict__ colk = Rp.b.colk;
+  const double *__restrict__ Sq = Rp.b.Sq, *__restrict__ Sm = Rp.b.Sm;
+  // FD_Q and FD_QD write their reciprocal-space parts to kE / kphi, added by
+  // their consumers (the real-space pair pass may be forming the rest meanwhile)
+  double *__restrict__ phi = MODE == FD_D ? Rp.b.phi : Rp.b.kphi;
+  double *__restrict__ Efq = MODE == FD_Q ? Rp.b.kE : Rp.b.Efq;
+  double *__restrict__ dR = MODE == FD_QD ? Rp.b.kE : Rp.b.dR;
+  double *__restrict__ dip = Rp.b.dip, *__restrict__ dip_prev = Rp.b.dip_prev;
+  const double *__restrict__ Efd = Rp.b.Efd;
+  double *__restrict__ diff2 = Rp.b.diff2;
+  const int *__restrict__ active = Rp.b.active;
+  if (iter < 0) iter = c.maxiter + 1;
+  const int b = bk.y;
   if (MODE == FD_D && !active[b]) return;
   const int lane = threadIdx.x & 63;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int M = set_size(c, SET);
   const int S = 4 * c.N;
-  const int mbase = blockIdx.x * FD_SB;
+  const int mbase = bk.x * FD_SB;
   const double2 *tb = tab + (size_t)b * c.T * S;
   for (int e = threadIdx.x; e < c.T * FD_SB; e += 64 * CG) {
     const int row = e >> 6, l = e & 63;
@@ -1308,62 +1238,24 @@ static size_t field_lds(const SysConst &c, int mode, bool kl) {
          (kl ? c.nk * (sizeof(double4) + (mode == FD_QD ? sizeof(double2) : 0)) : 0);
 }
 
-template <int MO, int ML, int CG>
-static void launch_field_cg(const SysConst &c, DevBufs &b, int iter, dim3 grid, hipStream_t s) {
-  const bool kl = c.fd_klds && field_lds(c, MO, true) <= 160 * 1024;
-  // FD_Q and FD_QD write the reciprocal-space parts to kE / kphi
-  double *phi = MO == FD_D ? b.phi : b.kphi;
-  double *Efq = MO == FD_Q ? b.kE : b.Efq;
-  double *dR = MO == FD_QD ? b.kE : b.dR;
-  if (kl)
-    hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), field_lds(c, MO, true), s,
-                       c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, phi, Efq, dR,
-                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
-  else
-    hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), field_lds(c, MO, false),
-                       s, c, b.site, b.tab, b.kv, b.cols, b.colk, b.Sq, b.Sm, phi, Efq, dR,
-                       b.dip, b.dip_prev, b.Efd, b.diff2, b.active, iter);
-}
 
-template <int MO, int ML>
-static void launch_field_m(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
-  const int M = MO == FD_D ? c.nd * c.N : (ML == 2 ? 4 * c.N : 3 * c.N);
-  dim3 grid((M + FD_SB - 1) / FD_SB, c.P);
-  // as many column groups as the columns keep busy: more resident waves hide
-  // the table and structure-factor reads (512 H2O x 32 beads: the charge
-  // fields at 16 groups instead of 8 take 1.68 instead of 1.73 ms per force
-  // call)
-  int cg = c.fd_cg;
-  if (cg == 0) cg = c.ncol < 8 ? 4 : c.ncol < 16 ? 8 : 16;
-  if (cg == 4) launch_field_cg<MO, ML, 4>(c, b, iter, grid, s);
-  else if (cg == 8) launch_field_cg<MO, ML, 8>(c, b, iter, grid, s);
-  else launch_field_cg<MO, ML, 16>(c, b, iter, grid, s);
-}
 
-void launch_field(const SysConst &c, DevBufs &b, int mode, int iter, hipStream_t s) {
-  if (c.model == 2) {
-    if (mode == FD_Q) launch_field_m<FD_Q, 2>(c, b, iter, s);
-    else if (mode == FD_D) launch_field_m<FD_D, 2>(c, b, iter, s);
-    else launch_field_m<FD_QD, 2>(c, b, iter, s);
-  } else {
-    if (mode == FD_Q) launch_field_m<FD_Q, 3>(c, b, iter, s);
-    else if (mode == FD_D) launch_field_m<FD_D, 3>(c, b, iter, s);
-    else launch_field_m<FD_QD, 3>(c, b, iter, s);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // After the charge pass: gradient of the charge energy, Uelec partials and the
 // initial induced dipoles (pot_ttm.f90:273-297).  One thread per (bead, molecule).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restrict__ site,
-                                               double *phi, double *Efq,
-                                               const double *__restrict__ kE,
-                                               const double *__restrict__ kphi, double *dR,
-                                               double *dip, const double *__restrict__ hist,
-                                               double *tally, int *active, int *iters,
-                                               int *nactive, const int *__restrict__ clk) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_post_qq(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR, *dip = Rp.b.dip, *tally = Rp.b.tally;
+  const double *__restrict__ kE = Rp.b.kE, *__restrict__ kphi = Rp.b.kphi;
+  const double *__restrict__ hist = Rp.b.hist;
+  int *active = Rp.b.active, *iters = Rp.b.iters, *nactive = Rp.b.nactive;
+  const int *__restrict__ clk = Rp.b.clk;
+  const int gid = bk.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
   const int t = clk[0];  // MD step of this force call
   const int b = gid / c.N, w = gid % c.N, N = c.N;
@@ -1412,26 +1304,23 @@ __global__ __launch_bounds__(64) void k_post_qq(SysConst c, const Site *__restri
     for (int i = 0; i <= c.maxiter; ++i) nactive[i] = 0;
 }
 
-void launch_post_qq(const SysConst &c, DevBufs &b, hipStream_t s) {
-  int n = c.P * c.N;
-  hipLaunchKernelGGL(k_post_qq, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.phi, b.Efq,
-                     b.kE, b.kphi, b.dR, b.dip, b.hist, b.tally, b.active, b.iters, b.nactive,
-                     b.clk);
-}
 
 // ---------------------------------------------------------------------------
 // Real-space dipole field T.mu of one SOR iteration (pot_ttm.f90:311-365).
 // ---------------------------------------------------------------------------
 template <int MODEL, int G, typename R>
-__global__ __launch_bounds__(256) void k_pair_dd(SysConst c, const Site *__restrict__ site,
-                                                const double *__restrict__ dip,
-                                                const int *__restrict__ nbr,
-                                                const int *__restrict__ nbrc,
-                                                const int *__restrict__ nbrn,
-                                                const int *__restrict__ active, double *Efd) {
+__global__ __launch_bounds__(256) void k_pair_dd(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
-  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ dip = Rp.b.dip;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc, *__restrict__ nbrn = Rp.b.nbrn;
+  const int *__restrict__ active = Rp.b.active;
+  double *Efd = Rp.b.Efd;
+  const long gtid = (long)bk.x * blockDim.x + threadIdx.x;
   const long grp = gtid / G;
   const int lane = (int)(gtid % G);
   const int N = c.N;
@@ -1556,24 +1445,27 @@ __device__ bool sor_test(const SysConst &c, const double *__restrict__ diff2, in
 }
 
 template <int MODEL, int G, typename R>
-__global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(SysConst c, const Site *__restrict__ site,
-                                                         const double *__restrict__ dip,
-                                                         const int *__restrict__ nbr,
-                                                         const int *__restrict__ nbrc,
-                                                         const int *__restrict__ nbrn,
-                                                         int *active, double *__restrict__ Efd,
-                                                         const double *__restrict__ diff2,
-                                                         int *iters, int *nactive, int test_iter) {
+__global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(const Rep *__restrict__ reps,
+                                                                   BlockMap bm, int test_iter) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
   extern __shared__ double lds_dd[];  // [N][ND][DD_REC]: x, y, z, mu_x, mu_y, mu_z, pad
   __shared__ double red[256];
   __shared__ int act;
-  const int b = blockIdx.y;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ dip = Rp.b.dip;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc, *__restrict__ nbrn = Rp.b.nbrn;
+  int *active = Rp.b.active, *iters = Rp.b.iters, *nactive = Rp.b.nactive;
+  double *__restrict__ Efd = Rp.b.Efd;
+  const double *__restrict__ diff2 = Rp.b.diff2;
+  const int b = bk.y;
   if (test_iter > 0) {
     // the convergence test of the previous iteration, in place of a
     // k_converge launch
-    if (!sor_test(c, diff2, active, iters, nactive, b, test_iter, blockIdx.x == 0, red, act))
+    if (!sor_test(c, diff2, active, iters, nactive, b, test_iter, bk.x == 0, red, act))
       return;
   } else if (!active[b]) {
     return;
@@ -1592,7 +1484,7 @@ __global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(SysConst c, c
     o[5] = mu[2];
   }
   __syncthreads();
-  const int a = blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+  const int a = bk.x * (blockDim.x / G) + threadIdx.x / G;
   const int lane = threadIdx.x % G;
   if (a >= N) return;
   const long grp = (long)b * N + a;
@@ -1658,91 +1550,42 @@ __global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(SysConst c, c
 
 static size_t dd_lds(const SysConst &c) { return sizeof(double) * DD_REC * c.N * c.nd; }
 
-template <int G, typename R>
-static bool launch_dd_lds_g(const SysConst &c, DevBufs &b, int test_iter, hipStream_t s) {
-  if (dd_lds(c) > 64 * 1024) return false;
-  constexpr int B2 = dd_block<2>(), B3 = dd_block<3>();
-  if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_dd_lds<2, G, R>), dim3((c.N + B2 / G - 1) / (B2 / G), c.P), dim3(B2),
-                       dd_lds(c), s, c, b.site,
-                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd, b.diff2, b.iters,
-                       b.nactive, test_iter);
-  else
-    hipLaunchKernelGGL((k_pair_dd_lds<3, G, R>), dim3((c.N + B3 / G - 1) / (B3 / G), c.P), dim3(B3),
-                       dd_lds(c), s, c, b.site,
-                       b.dip, b.nbr, b.nbrc, b.nbrn, b.active, b.Efd, b.diff2, b.iters,
-                       b.nactive, test_iter);
-  return true;
-}
 
-template <typename R>
-static bool launch_dd_lds_r(const SysConst &c, DevBufs &b, int test_iter, hipStream_t s) {
-  const int g = c.g_dd;
-  if (g == 64) return launch_dd_lds_g<64, R>(c, b, test_iter, s);
-  if (g == 32) return launch_dd_lds_g<32, R>(c, b, test_iter, s);
-  if (g == 8) return launch_dd_lds_g<8, R>(c, b, test_iter, s);
-  return launch_dd_lds_g<16, R>(c, b, test_iter, s);
-}
 
 bool pair_dd_tests(const SysConst &c) { return c.g_dd > 0 && dd_lds(c) <= 64 * 1024; }
 
-template <int G, typename R>
-static void launch_dd_g(const SysConst &c, DevBufs &b, hipStream_t s) {
-  long threads = (long)c.P * c.N * G;
-  dim3 grid((threads + 255) / 256);
-  if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_dd<2, G, R>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr,
-                       b.nbrc, b.nbrn, b.active, b.Efd);
-  else
-    hipLaunchKernelGGL((k_pair_dd<3, G, R>), grid, dim3(256), 0, s, c, b.site, b.dip, b.nbr,
-                       b.nbrc, b.nbrn, b.active, b.Efd);
-}
 
-template <typename R>
-static void launch_dd_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
-  if (group == 64)
-    launch_dd_g<64, R>(c, b, s);
-  else if (group == 32)
-    launch_dd_g<32, R>(c, b, s);
-  else
-    launch_dd_g<16, R>(c, b, s);
-}
 
 // fp64 forces the FP64 form (the refinement pass of the mixed solve).
-void launch_pair_dd(const SysConst &c, DevBufs &b, int group, bool fp64, hipStream_t s,
-                    int test_iter) {
-  const bool f32 = c.mixed && !fp64;
-  if (c.g_dd > 0 && (f32 ? launch_dd_lds_r<float>(c, b, test_iter, s)
-                         : launch_dd_lds_r<double>(c, b, test_iter, s)))
-    return;
-  if (f32) launch_dd_r<float>(c, b, group, s);
-  else launch_dd_r<double>(c, b, group, s);
-}
 
 // Start of the FP64 refinement of the mixed solve: every bead's dipoles go
 // back to the input of its last SOR step and every bead is active again.
-__global__ __launch_bounds__(256) void k_refine_prep(SysConst c, const double *__restrict__ dip_prev,
-                                                    double *__restrict__ dip, int *active) {
+__global__ __launch_bounds__(256) void k_refine_prep(const Rep *__restrict__ reps, BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ dip_prev = Rp.b.dip_prev;
+  double *__restrict__ dip = Rp.b.dip;
+  int *active = Rp.b.active;
   const size_t n = (size_t)c.P * c.N * c.nd * 3;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = (size_t)bk.x * blockDim.x + threadIdx.x;
   if (i < n) dip[i] = dip_prev[i];
   if (i < (size_t)c.P) active[i] = 1;
 }
 
-void launch_refine_prep(const SysConst &c, DevBufs &b, hipStream_t s) {
-  const size_t n = std::max((size_t)c.P * c.N * c.nd * 3, (size_t)c.P);
-  hipLaunchKernelGGL(k_refine_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c,
-                     b.dip_prev, b.dip, b.active);
-}
 
 
 // Per-bead convergence test (pot_ttm.f90:380-385): deltadip =
 // DEBYE/CHARGECON/sqrt(Natoms) * ||mu_new - mu_old||, summed in a fixed order.
-__global__ __launch_bounds__(256) void k_converge(SysConst c, const double *__restrict__ diff2,
-                                                 int *active, int *iters, int *nactive,
+__global__ __launch_bounds__(256) void k_converge(const Rep *__restrict__ reps, BlockMap bm,
                                                  int iter) {
   __shared__ double red[256];
-  const int b = blockIdx.x;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ diff2 = Rp.b.diff2;
+  int *active = Rp.b.active, *iters = Rp.b.iters, *nactive = Rp.b.nactive;
+  const int b = bk.x;
   if (!active[b]) return;
   const int M = c.N * c.nd;
   double s = 0.0;
@@ -1768,10 +1611,6 @@ __global__ __launch_bounds__(256) void k_converge(SysConst c, const double *__re
   }
 }
 
-void launch_converge(const SysConst &c, DevBufs &b, int iter, hipStream_t s) {
-  hipLaunchKernelGGL(k_converge, dim3(c.P), dim3(256), 0, s, c, b.diff2, b.active, b.iters,
-                     b.nactive, iter);
-}
 
 // ---------------------------------------------------------------------------
 // Real-space forces of the induced dipoles (pot_ttm.f90:399-478): dipole-dipole
@@ -1784,34 +1623,35 @@ __device__ __forceinline__ bool is_dip(int t) {
 
 // LDS = true: as k_pair_qq, with the bead's dipoles staged beside the sites.
 template <int MODEL, int G, typename R, bool LDS = false>
-__global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__restrict__ site,
-                                                   const double *__restrict__ Rcs,
-                                                   const double *__restrict__ dip,
-                                                   const int *__restrict__ nbr,
-                                                   const int *__restrict__ nbrc,
-                                                   const int *__restrict__ nbrn, double *phi,
-                                                   double *dR, double *tally) {
+__global__ __launch_bounds__(256) void k_pair_final(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
   constexpr int FIN_SB_UNROLL = MODEL == 2 && sizeof(R) == 8 ? 1 : 4;
   extern __shared__ Site lds_site[];  // [4N] sites, then [N][ND][3] dipoles
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ Rcs = Rp.b.Rcs, *__restrict__ dip = Rp.b.dip;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc, *__restrict__ nbrn = Rp.b.nbrn;
+  double *phi = Rp.b.phi, *dR = Rp.b.dR, *tally = Rp.b.tally;
   double *lds_dip = reinterpret_cast<double *>(lds_site + 4 * c.N);
   const int N = c.N;
   long grp;
   int lane;
   bool valid;
   if (LDS) {
-    const int bb = blockIdx.y;
+    const int bb = bk.y;
     for (int e = threadIdx.x; e < 4 * N; e += blockDim.x) lds_site[e] = site[(size_t)bb * 4 * N + e];
     for (int e = threadIdx.x; e < N * ND * 3; e += blockDim.x)
       lds_dip[e] = dip[(size_t)bb * N * ND * 3 + e];
     __syncthreads();
-    const int aa = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int aa = bk.x * (256 / G) + threadIdx.x / G;
     lane = threadIdx.x % G;
     valid = aa < N;
     grp = (long)bb * N + (valid ? aa : 0);
   } else {
-    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gtid = (long)bk.x * blockDim.x + threadIdx.x;
     grp = gtid / G;
     lane = (int)(gtid % G);
     valid = grp < (long)c.P * N;
@@ -1933,62 +1773,23 @@ __global__ __launch_bounds__(256) void k_pair_final(SysConst c, const Site *__re
   }
 }
 
-template <int G, typename R>
-static void launch_final_g(const SysConst &c, DevBufs &b, hipStream_t s) {
-  long threads = (long)c.P * c.N * G;
-  dim3 grid((threads + 255) / 256);
-  if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_final<2, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
-                       b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
-  else
-    hipLaunchKernelGGL((k_pair_final<3, G, R>), grid, dim3(256), 0, s, c, b.site, b.Rcs, b.dip,
-                       b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
-}
 
-template <int G, typename R>
-static void launch_final_lds_g(const SysConst &c, DevBufs &b, size_t lds, hipStream_t s) {
-  dim3 grid((c.N + 256 / G - 1) / (256 / G), c.P);
-  if (c.model == 2)
-    hipLaunchKernelGGL((k_pair_final<2, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
-                       b.dip, b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
-  else
-    hipLaunchKernelGGL((k_pair_final<3, G, R, true>), grid, dim3(256), lds, s, c, b.site, b.Rcs,
-                       b.dip, b.nbr, b.nbrc, b.nbrn, b.phi, b.dR, b.tally);
-}
 
-template <typename R>
-static void launch_final_r(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
-  const size_t lds = bead_sites_lds(c) + sizeof(double) * c.N * c.nd * 3;
-  if (c.g_fin > 0 && lds <= PAIR_LDS_MAX) {
-    if (c.g_fin == 4) launch_final_lds_g<4, R>(c, b, lds, s);
-    else if (c.g_fin == 8) launch_final_lds_g<8, R>(c, b, lds, s);
-    else if (c.g_fin == 32) launch_final_lds_g<32, R>(c, b, lds, s);
-    else launch_final_lds_g<16, R>(c, b, lds, s);
-    return;
-  }
-  if (group == 64)
-    launch_final_g<64, R>(c, b, s);
-  else if (group == 32)
-    launch_final_g<32, R>(c, b, s);
-  else
-    launch_final_g<16, R>(c, b, s);
-}
 
-void launch_pair_final(const SysConst &c, DevBufs &b, int group, hipStream_t s) {
-  if (c.mixed) launch_final_r<float>(c, b, group, s);
-  else launch_final_r<double>(c, b, group, s);
-}
 
 // Reciprocal-space virial (ewald.f90:194-224), partial sums per block of k:
 // virk[b][blk][0..8] = sum Ureck (delta - coeff2 k k), [9..17] = dipole part.
-__global__ __launch_bounds__(256) void k_virk(SysConst c, const double4 *__restrict__ kv,
-                                             const double *__restrict__ Ak,
-                                             const double *__restrict__ Sq,
-                                             const double *__restrict__ Sm,
-                                             const double *__restrict__ Sv, double *virk) {
+__global__ __launch_bounds__(256) void k_virk(const Rep *__restrict__ reps, BlockMap bm) {
   __shared__ double red[18][256];
-  const int b = blockIdx.y;
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double4 *__restrict__ kv = Rp.b.kv;
+  const double *__restrict__ Ak = Rp.b.red;
+  const double *__restrict__ Sq = Rp.b.Sq, *__restrict__ Sm = Rp.b.Sm, *__restrict__ Sv = Rp.b.Sv;
+  double *virk = Rp.b.virk;
+  const int b = bk.y;
+  const int k = bk.x * 256 + threadIdx.x;
   double v[18];
   for (int i = 0; i < 18; ++i) v[i] = 0.0;
   if (k < c.nk) {
@@ -2021,7 +1822,7 @@ __global__ __launch_bounds__(256) void k_virk(SysConst c, const double4 *__restr
       for (int i = 0; i < 18; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x < 18) virk[((size_t)b * gridDim.x + blockIdx.x) * 18 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 18) virk[((size_t)b * bk.nx + bk.x) * 18 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -2030,18 +1831,20 @@ __global__ __launch_bounds__(256) void k_virk(SysConst c, const double4 *__restr
 // redistribution and molecular dipoles.  One thread per (bead, molecule).
 // ---------------------------------------------------------------------------
 template <int MODEL>
-__global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restrict__ site,
-                                                const double *__restrict__ Rcs,
-                                                const double *__restrict__ grdq,
-                                                const double *__restrict__ dip,
-                                                const double *__restrict__ Efq, double *phi,
-                                                double *dR, const double *__restrict__ kE,
-                                                const double *__restrict__ kphi, double *hist,
-                                                double *tally, double *dRR, double *dipI,
-                                                double *dipE, const int *__restrict__ clk) {
+__global__ __launch_bounds__(64) void k_finalize(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ Rcs = Rp.b.Rcs, *__restrict__ grdq = Rp.b.grdq;
+  const double *__restrict__ dip = Rp.b.dip, *__restrict__ Efq = Rp.b.Efq;
+  const double *__restrict__ kE = Rp.b.kE, *__restrict__ kphi = Rp.b.kphi;
+  double *phi = Rp.b.phi, *dR = Rp.b.dR, *hist = Rp.b.hist, *tally = Rp.b.tally;
+  double *dRR = Rp.b.dRR, *dipI = Rp.b.dipI, *dipE = Rp.b.dipE;
+  const int *__restrict__ clk = Rp.b.clk;
+  const int gid = bk.x * blockDim.x + threadIdx.x;
   if (gid >= c.P * c.N) return;
   const int t = clk[0];  // MD step of this force call
   const int b = gid / c.N, w = gid % c.N, N = c.N;
@@ -2174,32 +1977,20 @@ __global__ __launch_bounds__(64) void k_finalize(SysConst c, const Site *__restr
   vadd9(tl + T_VIRT, vt);
 }
 
-void launch_finalize(const SysConst &c, DevBufs &b, double *dRR, double *dipI, double *dipE,
-                     hipStream_t s) {
-  int n = c.P * c.N;
-  if (c.model == 2)
-    hipLaunchKernelGGL(k_finalize<2>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
-                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.kE, b.kphi, b.hist, b.tally, dRR, dipI,
-                       dipE, b.clk);
-  else
-    hipLaunchKernelGGL(k_finalize<3>, dim3((n + 63) / 64), dim3(64), 0, s, c, b.site, b.Rcs,
-                       b.grdq, b.dip, b.Efq, b.phi, b.dR, b.kE, b.kphi, b.hist, b.tally, dRR, dipI,
-                       dipE, b.clk);
-}
 
-void launch_virk(const SysConst &c, DevBufs &b, hipStream_t s) {
-  dim3 grid((c.nk + 255) / 256, c.P);
-  hipLaunchKernelGGL(k_virk, grid, dim3(256), 0, s, c, b.kv, b.red, b.Sq, b.Sm, b.Sv, b.virk);
-}
 
 // Per-bead totals (pot_ttm.f90:552-553, 588-591, 628; force_calc.f90:48).
-__global__ __launch_bounds__(256) void k_reduce_bead(SysConst c, const double *__restrict__ tally,
-                                                    const double *__restrict__ virk, int nkb,
-                                                    const int *__restrict__ iters_in,
-                                                    double *Upot, double *virt, double *virialc,
-                                                    double *terms, int *iters) {
+__global__ __launch_bounds__(256) void k_reduce_bead(const Rep *__restrict__ reps, BlockMap bm) {
   __shared__ double red[T_COUNT][256];
-  const int b = blockIdx.x;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const double *__restrict__ tally = Rp.b.tally, *__restrict__ virk = Rp.b.virk;
+  const int nkb = (c.nk + 255) / 256;
+  const int *__restrict__ iters_in = Rp.b.iters;
+  double *Upot = Rp.b.Upot, *virt = Rp.b.virt, *virialc = Rp.b.virialc, *terms = Rp.b.terms;
+  int *iters = Rp.b.iters_out;
+  const int b = bk.x;
   double v[T_COUNT];
   for (int i = 0; i < T_COUNT; ++i) v[i] = 0.0;
   for (int w = threadIdx.x; w < c.N; w += 256)
@@ -2240,15 +2031,394 @@ __global__ __launch_bounds__(256) void k_reduce_bead(SysConst c, const double *_
       terms[b * 5 + 3] = uind;
       terms[b * 5 + 4] = c.Uvdw_lrc;
     }
-    if (iters) iters[b] = iters_in[b];
+    // the fixed-charge models have no SOR solve: zero iterations
+    iters[b] = c.model >= PIMD_QSPCFW ? 0 : iters_in[b];
   }
 }
 
-void launch_reduce_bead(const SysConst &c, DevBufs &b, double *Upot, double *virt,
-                        double *virialc, int *iters, hipStream_t s) {
-  const int nkb = (c.nk + 255) / 256;
-  hipLaunchKernelGGL(k_reduce_bead, dim3(c.P), dim3(256), 0, s, c, b.tally, b.virk, nkb,
-                     b.iters, Upot, virt, virialc, b.terms, iters);
+
+
+// ---------------------------------------------------------------------------
+// Launchers.  Each takes a group of replicas and launches one grid per kernel
+// variant present (template instantiation: model, arithmetic, lanes per
+// molecule, LDS staging), each replica with the grid and variant it would
+// have alone.
+// ---------------------------------------------------------------------------
+static bool is_ttm(const SysConst &c) { return c.model == PIMD_TTM21F || c.model == PIMD_TTM3F; }
+static int idiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+void launch_monomer(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return Plan{is_ttm(c) ? 0 : -1, idiv((long)c.P * c.N, 64), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_monomer, grid, dim3(64 * MONO_W), 0, s, g.d, m);
+      });
+}
+
+void launch_pes_fast(const Group &g, bool record, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, idiv((long)c.P * c.N, 64), 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_pes_fast, grid, dim3(64 * MONO_W), 0, s, g.d, m, record ? 1 : 0);
+      });
+}
+
+void launch_broadcast_beads(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, idiv((long)c.P * 9 * c.N, 256), 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_broadcast_beads, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+void launch_contract_record(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &, const MDConst &) { return Plan{0, 1, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_contract_record, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+void launch_neighbors(const Group &g, hipStream_t s) {
+  // variant 0: all molecules through LDS, 1: the same from global memory,
+  // 2: through the cell list
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (c.cells) return Plan{-1, 0, 0, 0, 0};
+        const size_t lds = sizeof(double) * 4 * c.N;
+        return Plan{lds <= 64 * 1024 ? 0 : 1, idiv(c.N, NB_I), c.P, 1, lds <= 64 * 1024 ? lds : 0};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        if (v == 0) hipLaunchKernelGGL(k_neighbors<true>, grid, dim3(256), lds, s, g.d, m);
+        else hipLaunchKernelGGL(k_neighbors<false>, grid, dim3(256), 0, s, g.d, m);
+      });
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        const int nc = c.ncell[0] * c.ncell[1] * c.ncell[2];
+        return Plan{c.cells ? 0 : -1, c.P, 1, 1, sizeof(int) * 2 * (size_t)nc};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t lds) {
+        hipLaunchKernelGGL(k_cells, grid, dim3(256), lds, s, g.d, m);
+      });
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return Plan{c.cells ? 0 : -1, idiv((long)c.P * c.N * 64, 256), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_neighbors_cells, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+// Lanes per molecule as a variant index and back.
+static int gidx(int G) { return G == 4 ? 0 : G == 8 ? 1 : G == 16 ? 2 : G == 32 ? 3 : 4; }
+
+// Variant of the charge / final pair passes: bit 0 TTM3-F, bit 1 FP32 pair
+// arithmetic, bit 2 LDS-staged records, bits 3.. the lanes-per-molecule index.
+static Plan pair_plan(const SysConst &c, int g_lds, size_t lds) {
+  if (!is_ttm(c)) return Plan{-1, 0, 0, 0, 0};
+  const int v0 = (c.model == 3 ? 1 : 0) | (c.mixed ? 2 : 0);
+  if (g_lds > 0 && lds <= PAIR_LDS_MAX) {
+    const int G = (g_lds == 4 || g_lds == 8 || g_lds == 32) ? g_lds : 16;
+    return Plan{v0 | 4 | (gidx(G) << 3), idiv(c.N, 256 / G), c.P, 1, lds};
+  }
+  const int G = c.g_nolds == 64 ? 64 : c.g_nolds == 32 ? 32 : 16;
+  return Plan{v0 | (gidx(G) << 3), idiv((long)c.P * c.N * G, 256), 1, 1, 0};
+}
+
+template <int ML, typename R, bool LDS>
+static void launch_qq_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const Group &g, hipStream_t s) {
+  switch (gi) {
+    case 0: if (LDS) hipLaunchKernelGGL((k_pair_qq<ML, 4, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 1: if (LDS) hipLaunchKernelGGL((k_pair_qq<ML, 8, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 2: hipLaunchKernelGGL((k_pair_qq<ML, 16, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 3: hipLaunchKernelGGL((k_pair_qq<ML, 32, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    default: if (!LDS) hipLaunchKernelGGL((k_pair_qq<ML, 64, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+  }
+}
+
+void launch_pair_qq(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return pair_plan(c, c.g_qq, bead_sites_lds(c)); },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        const int gi = v >> 3;
+        switch (v & 7) {
+          case 0: launch_qq_v<2, double, false>(gi, grid, m, lds, g, s); break;
+          case 1: launch_qq_v<3, double, false>(gi, grid, m, lds, g, s); break;
+          case 2: launch_qq_v<2, float, false>(gi, grid, m, lds, g, s); break;
+          case 3: launch_qq_v<3, float, false>(gi, grid, m, lds, g, s); break;
+          case 4: launch_qq_v<2, double, true>(gi, grid, m, lds, g, s); break;
+          case 5: launch_qq_v<3, double, true>(gi, grid, m, lds, g, s); break;
+          case 6: launch_qq_v<2, float, true>(gi, grid, m, lds, g, s); break;
+          default: launch_qq_v<3, float, true>(gi, grid, m, lds, g, s); break;
+        }
+      });
+}
+
+void launch_tables(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, idiv(4 * c.N, 256), c.P, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_tables, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+// Structure factor `mode`.  Variant: bit 0 the column kernel (else one k per
+// lane), bits 1-3 its segment-length bucket, bit 4 two lane sets per block,
+// bits 5.. the lane count of a set in units of 64.
+static const int KZ_BUCKET[6] = {4, 8, 12, 16, 24, 32};
+
+template <int MODE, int KZ>
+static void launch_sfac_col_v(bool split, int threads, dim3 grid, const BlockMap &m, size_t lds,
+                              const Group &g, hipStream_t s) {
+  if (split) hipLaunchKernelGGL((k_sfac_col<MODE, KZ, 2>), grid, dim3(2 * threads), lds, s, g.d, m);
+  else hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), lds, s, g.d, m);
+}
+
+template <int MODE>
+static void launch_sfac_mode(const Group &g, hipStream_t s) {
+  constexpr int NV = MODE == SF_V ? 6 : 2;
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (MODE != SF_Q && !is_ttm(c)) return Plan{-1, 0, 0, 0, 0};
+        int nch, sc, scp;
+        sfac_chunking(c, MODE, sfac_sites(c, MODE), nch, sc, scp);
+        const size_t lds = sfac_lds(c, MODE, sc, scp);
+        int kzb = -1;
+        for (int i = 0; i < 6; ++i)
+          if (kzb < 0 && c.segmax <= KZ_BUCKET[i]) kzb = i;
+        const bool col = c.sf_kind == 1 && (MODE != SF_V || c.sfv_col) && lds <= 64 * 1024 && kzb >= 0;
+        if (!col) return Plan{0, idiv(c.nk, 256), nch, c.P, lds};
+        const int threads = std::min(256, idiv(c.nseg, 64) * 64);
+        // two lane sets per block where the second set's sums fit beside the tables
+        const size_t red = sizeof(double) * ((size_t)NV * KZ_BUCKET[kzb] * threads + 2);
+        const bool split = MODE != SF_V && c.sf_split && lds + red <= 96 * 1024;
+        return Plan{1 | (kzb << 1) | (split ? 16 : 0) | ((threads / 64) << 5), idiv(c.nseg, threads),
+                    nch, c.P, split ? lds + red : lds};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        if (!(v & 1)) {
+          hipLaunchKernelGGL((k_sfac_k<MODE>), grid, dim3(256), lds, s, g.d, m);
+          return;
+        }
+        const bool split = (v & 16) != 0;
+        const int threads = (v >> 5) * 64;
+        switch ((v >> 1) & 7) {
+          case 0: launch_sfac_col_v<MODE, 4>(split, threads, grid, m, lds, g, s); break;
+          case 1: launch_sfac_col_v<MODE, 8>(split, threads, grid, m, lds, g, s); break;
+          case 2: launch_sfac_col_v<MODE, 12>(split, threads, grid, m, lds, g, s); break;
+          case 3: launch_sfac_col_v<MODE, 16>(split, threads, grid, m, lds, g, s); break;
+          case 4: launch_sfac_col_v<MODE, 24>(split, threads, grid, m, lds, g, s); break;
+          default: launch_sfac_col_v<MODE, 32>(split, threads, grid, m, lds, g, s); break;
+        }
+      });
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (MODE != SF_Q && !is_ttm(c)) return Plan{-1, 0, 0, 0, 0};
+        return Plan{0, idiv((long)NV * c.nk, 256), c.P, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_sfac_reduce, grid, dim3(256), 0, s, g.d, m, MODE);
+      });
+}
+
+void launch_sfac(const Group &g, int mode, hipStream_t s) {
+  if (mode == SF_Q) launch_sfac_mode<SF_Q>(g, s);
+  else if (mode == SF_D) launch_sfac_mode<SF_D>(g, s);
+  else launch_sfac_mode<SF_V>(g, s);
+}
+
+// Reciprocal-space field `mode`.  Variant: bit 0 TTM3-F site sets (the
+// fixed-charge models share them), bits 1-2 the column groups (4, 8, 16),
+// bit 3 k data staged in LDS.  sel_mixed: only the mixed-form replicas.
+template <int MO>
+static void launch_field_mode(const Group &g, int iter, bool sel_mixed, hipStream_t s) {
+  launch_over(
+      g,
+      [&](const SysConst &c, const MDConst &) {
+        if ((MO != FD_Q && !is_ttm(c)) || (sel_mixed && !c.mixed)) return Plan{-1, 0, 0, 0, 0};
+        const int ML = c.model == 2 ? 2 : 3;
+        const int M = MO == FD_D ? c.nd * c.N : (ML == 2 ? 4 * c.N : 3 * c.N);
+        // as many column groups as the columns keep busy: more resident waves hide
+        // the table and structure-factor reads (512 H2O x 32 beads: the charge
+        // fields at 16 groups instead of 8 take 1.68 instead of 1.73 ms per force
+        // call)
+        int cg = c.fd_cg;
+        if (cg == 0) cg = c.ncol < 8 ? 4 : c.ncol < 16 ? 8 : 16;
+        const int cgi = cg == 4 ? 0 : cg == 8 ? 1 : 2;
+        const bool kl = c.fd_klds && field_lds(c, MO, true) <= 160 * 1024;
+        return Plan{(ML == 3 ? 1 : 0) | (cgi << 1) | (kl ? 8 : 0), idiv(M, FD_SB), c.P, 1,
+                    field_lds(c, MO, kl)};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        const bool kl = (v & 8) != 0;
+#define PIMD_FIELD(ML, CG)                                                                     \
+  if (kl) hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), lds, s, g.d, m, iter); \
+  else hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), lds, s, g.d, m, iter)
+        switch (v & 7) {
+          case 0: PIMD_FIELD(2, 4); break;
+          case 1: PIMD_FIELD(3, 4); break;
+          case 2: PIMD_FIELD(2, 8); break;
+          case 3: PIMD_FIELD(3, 8); break;
+          case 4: PIMD_FIELD(2, 16); break;
+          default: PIMD_FIELD(3, 16); break;
+        }
+#undef PIMD_FIELD
+      });
+}
+
+void launch_field(const Group &g, int mode, int iter, hipStream_t s, bool sel_mixed) {
+  if (mode == FD_Q) launch_field_mode<FD_Q>(g, iter, sel_mixed, s);
+  else if (mode == FD_D) launch_field_mode<FD_D>(g, iter, sel_mixed, s);
+  else launch_field_mode<FD_QD>(g, iter, sel_mixed, s);
+}
+
+void launch_post_qq(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return Plan{is_ttm(c) ? 0 : -1, idiv((long)c.P * c.N, 64), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_post_qq, grid, dim3(64), 0, s, g.d, m);
+      });
+}
+
+// Variant of the dipole pass: bit 0 TTM3-F, bit 1 FP32, bit 2 LDS-staged
+// (with the folded convergence test), bits 3.. the lanes-per-molecule index.
+template <int ML, typename R>
+static void launch_dd_v(bool lds_kernel, int gi, dim3 grid, const BlockMap &m, size_t lds,
+                        int test_iter, const Group &g, hipStream_t s) {
+  constexpr int B = dd_block<ML>();
+  if (lds_kernel) {
+    switch (gi) {
+      case 1: hipLaunchKernelGGL((k_pair_dd_lds<ML, 8, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
+      case 3: hipLaunchKernelGGL((k_pair_dd_lds<ML, 32, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
+      case 4: hipLaunchKernelGGL((k_pair_dd_lds<ML, 64, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
+      default: hipLaunchKernelGGL((k_pair_dd_lds<ML, 16, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
+    }
+    return;
+  }
+  switch (gi) {
+    case 3: hipLaunchKernelGGL((k_pair_dd<ML, 32, R>), grid, dim3(256), 0, s, g.d, m); break;
+    case 4: hipLaunchKernelGGL((k_pair_dd<ML, 64, R>), grid, dim3(256), 0, s, g.d, m); break;
+    default: hipLaunchKernelGGL((k_pair_dd<ML, 16, R>), grid, dim3(256), 0, s, g.d, m); break;
+  }
+}
+
+// fp64 forces the FP64 form (the refinement pass of the mixed solve);
+// sel_mixed: only the mixed-form replicas.
+void launch_pair_dd(const Group &g, bool fp64, hipStream_t s, int test_iter, bool sel_mixed) {
+  launch_over(
+      g,
+      [&](const SysConst &c, const MDConst &) {
+        if (!is_ttm(c) || (sel_mixed && !c.mixed)) return Plan{-1, 0, 0, 0, 0};
+        const int v0 = (c.model == 3 ? 1 : 0) | (c.mixed && !fp64 ? 2 : 0);
+        if (pair_dd_tests(c)) {
+          const int G = c.g_dd == 64 ? 64 : c.g_dd == 32 ? 32 : c.g_dd == 8 ? 8 : 16;
+          const int B = c.model == 2 ? dd_block<2>() : dd_block<3>();
+          return Plan{v0 | 4 | (gidx(G) << 3), idiv(c.N, B / G), c.P, 1, dd_lds(c)};
+        }
+        const int G = c.g_nolds == 64 ? 64 : c.g_nolds == 32 ? 32 : 16;
+        return Plan{v0 | (gidx(G) << 3), idiv((long)c.P * c.N * G, 256), 1, 1, 0};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        const bool lk = (v & 4) != 0;
+        const int gi = v >> 3;
+        switch (v & 3) {
+          case 0: launch_dd_v<2, double>(lk, gi, grid, m, lds, test_iter, g, s); break;
+          case 1: launch_dd_v<3, double>(lk, gi, grid, m, lds, test_iter, g, s); break;
+          case 2: launch_dd_v<2, float>(lk, gi, grid, m, lds, test_iter, g, s); break;
+          default: launch_dd_v<3, float>(lk, gi, grid, m, lds, test_iter, g, s); break;
+        }
+      });
+}
+
+void launch_refine_prep(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        const long n = std::max((long)c.P * c.N * c.nd * 3, (long)c.P);
+        return Plan{is_ttm(c) && c.mixed ? 0 : -1, idiv(n, 256), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_refine_prep, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+void launch_converge(const Group &g, int iter, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{is_ttm(c) ? 0 : -1, c.P, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_converge, grid, dim3(256), 0, s, g.d, m, iter);
+      });
+}
+
+template <int ML, typename R, bool LDS>
+static void launch_fin_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const Group &g, hipStream_t s) {
+  switch (gi) {
+    case 0: if (LDS) hipLaunchKernelGGL((k_pair_final<ML, 4, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 1: if (LDS) hipLaunchKernelGGL((k_pair_final<ML, 8, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 2: hipLaunchKernelGGL((k_pair_final<ML, 16, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    case 3: hipLaunchKernelGGL((k_pair_final<ML, 32, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+    default: if (!LDS) hipLaunchKernelGGL((k_pair_final<ML, 64, R, LDS>), grid, dim3(256), lds, s, g.d, m); break;
+  }
+}
+
+void launch_pair_final(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return pair_plan(c, c.g_fin, bead_sites_lds(c) + sizeof(double) * c.N * c.nd * 3);
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        const int gi = v >> 3;
+        switch (v & 7) {
+          case 0: launch_fin_v<2, double, false>(gi, grid, m, lds, g, s); break;
+          case 1: launch_fin_v<3, double, false>(gi, grid, m, lds, g, s); break;
+          case 2: launch_fin_v<2, float, false>(gi, grid, m, lds, g, s); break;
+          case 3: launch_fin_v<3, float, false>(gi, grid, m, lds, g, s); break;
+          case 4: launch_fin_v<2, double, true>(gi, grid, m, lds, g, s); break;
+          case 5: launch_fin_v<3, double, true>(gi, grid, m, lds, g, s); break;
+          case 6: launch_fin_v<2, float, true>(gi, grid, m, lds, g, s); break;
+          default: launch_fin_v<3, float, true>(gi, grid, m, lds, g, s); break;
+        }
+      });
+}
+
+void launch_virk(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, idiv(c.nk, 256), c.P, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_virk, grid, dim3(256), 0, s, g.d, m);
+      });
+}
+
+void launch_finalize(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        return Plan{is_ttm(c) ? c.model : -1, idiv((long)c.P * c.N, 64), 1, 1, 0};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t) {
+        if (v == 2) hipLaunchKernelGGL(k_finalize<2>, grid, dim3(64), 0, s, g.d, m);
+        else hipLaunchKernelGGL(k_finalize<3>, grid, dim3(64), 0, s, g.d, m);
+      });
+}
+
+void launch_reduce_bead(const Group &g, hipStream_t s) {
+  launch_over(
+      g, [](const SysConst &c, const MDConst &) { return Plan{0, c.P, 1, 1, 0}; },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_reduce_bead, grid, dim3(256), 0, s, g.d, m);
+      });
 }
 
 }  // namespace pimd

@@ -18,9 +18,9 @@ contains
   subroutine full_bead_forces
     use main_stuff
     implicit none
-    if (.not. gpu_ready()) then
+    if (.not. gpu_ready() .or. (gpu_contraction() .neqv. CONTRACTION)) then
       call gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
-                        polar_sor, polar_eps, guess_initdip, massO, massH)
+                        polar_sor, polar_eps, guess_initdip, massO, massH, contraction=CONTRACTION)
     endif
     if (.not. allocated(chg_all)) allocate(chg_all(Natoms, Nbeads), virt_all(3, 3, Nbeads), terms(5, Nbeads))
     if (BAROSTAT) error stop 'force_calc (GPU): the barostat path is not supported'
@@ -57,9 +57,11 @@ contains
     if (pot_model < 2 .or. pot_model > 5) &
       error stop 'force_calc (GPU): only TTM2.1-F, TTM3-F, q-SPC/Fw and SPC/F'
     if (BAROSTAT) error stop 'force_calc (GPU): the barostat path is not supported'
-    if (.not. gpu_ready()) then
+    ! the context follows the run's CONTRACTION flag: contracted_MD calls
+    ! potential at the centroid with the monomer PES left out (MD.f90:398)
+    if (.not. gpu_ready() .or. (gpu_contraction() .neqv. CONTRACTION)) then
       call gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
-                        polar_sor, polar_eps, guess_initdip, massO, massH)
+                        polar_sor, polar_eps, guess_initdip, massO, massH, contraction=CONTRACTION)
     endif
     call gpu_potential(RR, RRc, Upot, dRR, virt, virialc, dip_momI, Edip_mom, chg, t, e5)
     Umon = e5(1)

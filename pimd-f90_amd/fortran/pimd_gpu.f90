@@ -20,7 +20,7 @@ module pimd_gpu
   private
   public :: gpu_init_pot, gpu_full_bead_forces, gpu_potential, gpu_energy_terms, &
             gpu_init_normal_modes, gpu_evolve_rings, gpu_init_langevin_nm, gpu_langevin_nm, &
-            gpu_finalize, gpu_ready, gpu_nm_data
+            gpu_finalize, gpu_ready, gpu_nm_data, gpu_contraction
 
   type(c_ptr), save :: ctx_all = c_null_ptr   ! every bead in one launch
   type(c_ptr), save :: ctx_one = c_null_ptr   ! one bead (potential)
@@ -42,13 +42,23 @@ contains
     gpu_ready = c_associated(ctx_all)
   end function gpu_ready
 
-  ! precision (optional): PIMD_FP64 (default) or PIMD_MIXED
+  ! Whether the contexts were made with pot_ttm's CONTRACTION branch.
+  logical function gpu_contraction()
+    gpu_contraction = g_params%contraction /= 0
+  end function gpu_contraction
+
+  ! precision (optional): PIMD_FP64 (default) or PIMD_MIXED.
+  ! contraction (optional): the reference's CONTRACTION flag (system_mod.f90:40);
+  ! when true pot_ttm leaves out the monomer PES energy, gradient and virial
+  ! (pot_ttm.f90:92-132), as contracted_MD's centroid call needs (MD.f90:398).
   subroutine gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
-                          polar_sor, polar_eps, guess_initdip, massO, massH, precision)
+                          polar_sor, polar_eps, guess_initdip, massO, massH, precision, &
+                          contraction)
     integer, intent(in) :: pot_model, Natoms, Nbeads, polar_maxiter
     double precision, intent(in) :: box(3), Rc, rc1, eps_ewald, polar_sor, polar_eps, massO, massH
     logical, intent(in) :: guess_initdip
     integer, intent(in), optional :: precision
+    logical, intent(in), optional :: contraction
     call gpu_finalize()
     g_params%pot_model = pot_model
     g_params%nwaters = Natoms / 3
@@ -58,6 +68,8 @@ contains
     g_params%precision = PIMD_FP64
     if (present(precision)) g_params%precision = precision
     g_params%device = 0
+    g_params%contraction = 0
+    if (present(contraction)) g_params%contraction = merge(1, 0, contraction)
     g_params%box = box
     g_params%rc = Rc
     g_params%rc1 = rc1

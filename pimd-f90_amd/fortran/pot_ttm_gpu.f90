@@ -6,7 +6,8 @@
 !
 ! One predictor history serves every call, as the reference's tx_dip
 ! (pot_mod.f90:53) does in a serial run.  Side effects on system_mod match
-! the reference: Umon, Uvdw, Uelec, Uind (system_mod.f90:30).
+! the reference: Umon, Uvdw, Uelec, Uind (system_mod.f90:30).  The
+! CONTRACTION branch (pot_ttm.f90:92-132) follows system_mod's flag.
 subroutine pot_ttm(RR, RRc, En, dRR, virt, virialc, dip_momI, Edip_mom, chg, t)
   use system_mod
   use pimd_gpu
@@ -19,9 +20,11 @@ subroutine pot_ttm(RR, RRc, En, dRR, virt, virialc, dip_momI, Edip_mom, chg, t)
   double precision, dimension(Natoms), intent(out) :: chg
   integer, intent(in) :: t
   double precision :: e5(5)
-  if (.not. gpu_ready()) then
+  ! the reference's pot_ttm reads CONTRACTION (system_mod.f90:40) on every call
+  ! and then leaves out the monomer PES terms (pot_ttm.f90:92-132)
+  if (.not. gpu_ready() .or. (gpu_contraction() .neqv. CONTRACTION)) then
     call gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
-                      polar_sor, polar_eps, guess_initdip, massO, massH)
+                      polar_sor, polar_eps, guess_initdip, massO, massH, contraction=CONTRACTION)
   endif
   call gpu_potential(RR, RRc, En, dRR, virt, virialc, dip_momI, Edip_mom, chg, t, e5)
   Umon = e5(1)

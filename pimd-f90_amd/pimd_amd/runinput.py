@@ -290,8 +290,9 @@ def run_setup(inp: dict, box, natoms: int):
     if inp["BAROSTAT"]:
         raise InputError("the barostat is not built")
     bt = inp["bead_thermostat_type"].strip()
-    if inp["BEADTHERMOSTAT"] and bt not in ("Langevin",):
-        raise InputError(f"bead thermostat {bt!r} is not built (Langevin is)")
+    # InputOutput.f90:251-256 admits these three; main_stuff.f90:140,145
+    if inp["BEADTHERMOSTAT"] and bt not in ("Langevin", "Nose-Hoover", "none"):
+        raise InputError(f"bead thermostat {bt!r} is not one of 'Nose-Hoover', 'Langevin', 'none'")
     ptype = inp["PIMD_type"].strip()
     if ptype not in ("full", "contracted"):
         raise InputError(f"PIMD_type {ptype!r} is not built (full and contracted are)")
@@ -303,7 +304,8 @@ def run_setup(inp: dict, box, natoms: int):
                   polar_sor=inp["polar_sor"], polar_eps=inp["polar_eps"],
                   guess_initdip=inp["guess_initdip"], mass_o=inp["massO"], mass_h=inp["massH"])
     md = dict(delt_fs=inp["delt"], temp=inp["temp"],
-              bead_thermostat=1 if inp["BEADTHERMOSTAT"] and bt == "Langevin" else 0,
+              bead_thermostat=({"Langevin": 1, "Nose-Hoover": 2}.get(bt, 0)
+                               if inp["BEADTHERMOSTAT"] else 0),
               centroid_thermostat=bool(inp["CENTROIDTHERMOSTAT"]),
               tau_centroid=inp["tau_centroid"], global_nh=bool(inp["THERMOSTAT"]), tau=inp["tau"],
               setNMfreq=inp["setNMfreq"],

@@ -146,6 +146,13 @@ def _arr(x, shape):
     return a
 
 
+def c_params(p: TTMParams, device: int = 0) -> PimdParams:
+    """The C ABI's pimd_params of resolved run parameters."""
+    return PimdParams(p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip),
+                      int(p.precision), device, int(p.contraction), (C.c_double * 3)(*p.box), p.rc,
+                      p.rc1, p.eps_ewald, p.polar_sor, p.polar_eps, p.mass_o, p.mass_h)
+
+
 class TTMForceField:
     """All-beads TTM2.1-F / TTM3-F force evaluation on one GPU."""
 
@@ -153,10 +160,7 @@ class TTMForceField:
         p = params.resolved()
         self.params = p
         L = load_library()
-        cp = PimdParams(p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip),
-                        int(p.precision), device, int(p.contraction), (C.c_double * 3)(*p.box), p.rc,
-                        p.rc1, p.eps_ewald,
-                        p.polar_sor, p.polar_eps, p.mass_o, p.mass_h)
+        cp = c_params(p, device)
         h = C.c_void_p()
         _check(L.pimd_ctx_create(C.byref(cp), C.byref(h)))
         self._h = h

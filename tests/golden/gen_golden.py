@@ -132,6 +132,46 @@ def traj_fixture(only=None):
              **params_dict(p))
 
 
+def config0_fixture():
+    """The CPU-runnable benchmark system as stated: 32 H2O (0.997 g/cm^3,
+    4 x 2 x 4 lattice), TTM3-F, Rc 4.9, eps_ewald 1e-8, P = 32, 100 NVE steps
+    of 0.25 fs from a thermal ring polymer at 300 K; per-step energies,
+    kinetic energy and write_out observables and the final state."""
+    N, P = 32, 32
+    RRt, PPt, RRc, L = make_system(N, P, seed=1000, lattice=(4, 2, 4), temp=300.0)
+    p = TTMParams(pot_model=3, nwaters=N, nbeads=P, box=(L, L, L), rc=4.9, eps_ewald=1e-8,
+                  mass_o=15.995, mass_h=1.008).resolved()
+    r = refrun.trajectory(p, RRt, PPt, np.zeros_like(RRt), nsteps=100, temp=300.0, delt_fs=0.25,
+                          bead_thermostat=0, per_bead_hist=True)
+    save("traj_ttm3f_nve_n32_p32_rc49", RRt=RRt, PPt=PPt,
+         obs=np.array([s["obs"] for s in r["steps"]]),
+         Upot=np.array([s["Upot"] for s in r["steps"]]),
+         uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
+         dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], **params_dict(p))
+
+
+def bead_nh_fixture():
+    """PIMD_VV with the normal-mode-space Nose-Hoover bead thermostat
+    (bead_thermostat_type 'Nose-Hoover', main_stuff.f90:145-171): with and
+    without the centroid chain, one with the global chain as well."""
+    for name, model, N, P, cen, glob, setnm in [
+            ("traj_ttm3f_beadnh_n8_p4", 3, 8, 4, True, False, 0.0),
+            ("traj_ttm21f_beadnh_nocen_n8_p3", 2, 8, 3, False, True, 0.0),
+            ("traj_ttm3f_beadnh_cmd_n8_p4", 3, 8, 4, True, False, 1500.0)]:
+        RRt, PPt, RRc, L = make_system(N, P, seed=60 + N + P, lattice=(2, 2, 2))
+        p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
+                      eps_ewald=1e-6, mass_o=15.995, mass_h=1.008).resolved()
+        r = refrun.trajectory(p, RRt, PPt, np.zeros_like(RRt), nsteps=6, temp=300.0,
+                              delt_fs=0.25, bead_thermostat=2, centroid=cen, tau_centroid=0.05,
+                              global_nh=glob, tau=0.1, per_bead_hist=True, setnm=setnm)
+        save(name, RRt=RRt, PPt=PPt, bead_tt=np.asarray(2), centroid=np.asarray(cen),
+             setnm=np.asarray(setnm), obs=np.array([s["obs"] for s in r["steps"]]),
+             Upot=np.array([s["Upot"] for s in r["steps"]]),
+             uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
+             dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], global_nh=np.asarray(glob),
+             vxi_beads=r["vxi_beads"], **params_dict(p))
+
+
 def contracted_fixture():
     """contracted_MD: monomer forces in inner steps, centroid TTM forces once per step."""
     for name, model, N, P, intra, lat in [("contracted_ttm3f_n8_p4_intra3", 3, 8, 4, 3, (2, 2, 2)),
@@ -149,6 +189,109 @@ def contracted_fixture():
              Upot=rec[:, 0], virial=rec[:, 1], virialc=rec[:, 2],
              uk=np.array([s["uk"] for s in r["steps"]]), RRt_end=r["RRt"], PPt_end=r["PPt"],
              dRRt_end=r["dRRt"], RRc_end=r["RRc"], iters=r["iters"], **params_dict(p))
+
+
+DROPIN_INPUT = """old
+!Input file of the drop-in program test (the reference's positional format)
+
+BASIC INFO
+'dropin_n32_p4.img'    ! input filename - either a centroid .xyz or a full bead configuration
+'dropin'    ! run name - this will be appended to all the output files
+4                    ! Number of beads
+0                    ! number of steps to equilibrate
+10                   ! number of steps to run
+.25                  ! timestep (delt) (fs)
+
+OUTPUT OPTIONS
+.f.                      ! centroid coordinates
+.f.                      ! centroid momenta
+.f.                      ! dipoles
+.f.                      ! polarization dipoles
+.f.                      ! total dipole
+.f.                      ! coordinates for all images
+.f.                      ! dipoles for all images
+.f.                      ! box size running average
+.t.                      ! Temp/Press (.t. = to file, .f. = to terminal)
+.t.                      ! avg. radius of gyration (in Temp/Press file)
+.f.                      ! output dielectric constant (running average in Temp/Press file)
+.f.                      ! write out charges in seperate file?
+.t.                      ! write final configuration of all beads to seperate file?
+20                       ! Total dipole output frequency
+1                        ! Temp/Press output frequency
+2000                     ! all images output frequency
+10                       ! Output frequency for everything else
+
+MD OPTIONS
+3                    ! Model: 2=ttm21f 3=ttm3f 4= qspcfw 5=spcf 6=SIESTA
+'siesta','32h2o', 1                 ! sys_label for SIESTA runs, # SIESTA processors
+4.9d0 3.92d0 1.d-8           ! realspace cutoff (Ang), start of VdW force switch (Ang) (TTM3F only), eps for aewald
+15  0.7  1.d-5 .f.  .f. ! params for dipole iterative procedure: Max iterations, pol factor, accuracy, guess initial?, debug output
+.f.                     ! generate initial velocities?
+.f.                     ! Global thermostat? t/f
+.f.                     ! Thermostat non-centroid modes ?
+.f.                     ! Thermostat centroid mode ?
+'none'                  ! type of bead thermostat - 'Nose-Hoover', 'Langevin' or 'none'
+.5                      ! tau for global thermostat (ps)
+.1                      ! tau for centroid thermostat (ps)
+2              ! global Nose-Hoover chain length
+2              ! bead Nose-Hoover chain length
+300      ! temperature (Kelvin)
+.f.          ! Berendson barostat ?
+.2       ! tau for barostat (ps)
+1            ! reference pressure (bar)
+.f.          ! pressure coupling during equilibration only?
+
+MORE RUN OPTIONS
+0            ! frequency (inverse cm) to set NMs to ( or enter 0 for none/RPMD)
+{ptype}      ! type of PIMD run - "full", "contracted", "monomerPIMD"
+3            ! ratio of slow timestep / fast timestep for contraction scheme
+15.994       ! mass of Oxygen (au)
+1.008        ! mass of Hyrdrogen
+.false.      ! restart? (append to previous files)
+"""
+
+
+def parse_temppress(path):
+    """The '@' lines of a run's Temp/Press file (InputOutput.f90:633-670):
+    time, temperature, running simple pressure, simple energy, its running
+    mean, running potential energy per molecule, mean r_O and r_H, mean
+    molecular dipole."""
+    rows = []
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("@"):
+                rows.append([float(x) for x in ln[1:].split()])
+    return np.array(rows)
+
+
+def dropin_fixture():
+    """The reference program (PIMD.f90, built unmodified into
+    oracle/_ref/PIMD_ref.x) run from a committed input and bead image for 10
+    NVE steps, full and contracted (3 inner steps): its Temp/Press lines and
+    the final bead image it writes.  The GPU test runs the drop-in programs on
+    the same files."""
+    import shutil
+    import subprocess
+    import tempfile
+    d = os.path.join(OUT, "inputs")
+    from pimd_amd import runinput
+    from pimd_amd.params import f32
+    RRt, PPt, RRc, L = make_system(32, 4, seed=1100, lattice=(4, 2, 4), temp=300.0)
+    runinput.write_image(os.path.join(d, "dropin_n32_p4.img"), RRt, PPt, (L, L, L), 0, 0.25e-3,
+                         f32(15.994), f32(1.008))
+    out = {}
+    for ptype in ("full", "contracted"):
+        with open(os.path.join(d, f"dropin_{ptype}.inp"), "w") as f:
+            f.write(DROPIN_INPUT.replace("{ptype}", ptype))
+        with tempfile.TemporaryDirectory() as tmp:
+            shutil.copy(os.path.join(d, f"dropin_{ptype}.inp"), tmp)
+            shutil.copy(os.path.join(d, "dropin_n32_p4.img"), tmp)
+            subprocess.run([refrun.PIMD_BIN, f"dropin_{ptype}.inp"], cwd=tmp, check=True,
+                           stdout=subprocess.DEVNULL)
+            out[f"tp_{ptype}"] = parse_temppress(os.path.join(tmp, "out_dropin_TempPress.dat"))
+            with open(os.path.join(tmp, "out_dropin_finalchk.img")) as f:
+                out[f"final_{ptype}"] = np.array(f.read())
+    save("dropin_ttm3f_n32_p4", **out)
 
 
 def io_fixtures():
@@ -251,14 +394,22 @@ def scalar_fixtures():
 if __name__ == "__main__":
     if not refrun.available():
         sys.exit("build the reference first: make -C oracle ref")
-    if len(sys.argv) > 1:   # named force and trajectory fixtures only
+    if len(sys.argv) > 1:   # named fixture groups or force and trajectory fixtures only
+        groups = {"config0": config0_fixture, "bead_nh": bead_nh_fixture,
+                  "dropin": dropin_fixture}
+        for a in sys.argv[1:]:
+            if a in groups:
+                groups[a]()
         force_fixtures(set(sys.argv[1:]))
         traj_fixture(set(sys.argv[1:]))
         sys.exit(0)
     force_fixtures()
     predictor_fixture()
     traj_fixture()
+    config0_fixture()
+    bead_nh_fixture()
     contracted_fixture()
     io_fixtures()
+    dropin_fixture()
     ring_fixtures()
     scalar_fixtures()

@@ -19,15 +19,15 @@ def _integrator(model, N, P, seed, **md):
     p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0, eps_ewald=1e-6,
                   guess_initdip=True)
     ff = TTMForceField(p)
-    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=1,
-                           centroid_thermostat=True, tau_centroid=0.1, **md)
+    md.setdefault("bead_thermostat", 1)
+    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, centroid_thermostat=True,
+                           tau_centroid=0.1, **md)
     integ.set_state(RRt, PPt, ff.full_bead_forces(RRt, RRc)["dRRt"])
     return integ
 
 
-def _runs(integ, graph: bool, chunks=(3, 2, 4), branches=False):
+def _runs(integ, graph: bool, chunks=(3, 2, 4)):
     os.environ["PIMD_MD_GRAPH"] = "1" if graph else "0"
-    os.environ["PIMD_GRAPH_BRANCHES"] = "1" if branches else "0"
     out, t0 = [], 1
     try:
         for n in chunks:
@@ -38,28 +38,24 @@ def _runs(integ, graph: bool, chunks=(3, 2, 4), branches=False):
             t0 += n
     finally:
         os.environ.pop("PIMD_MD_GRAPH", None)
-        os.environ.pop("PIMD_GRAPH_BRANCHES", None)
     return out
 
 
 @pytest.mark.parametrize("case", ["ttm3f", "ttm21f_cmd_nh", "contracted", "qspcfw",
-                                  "ttm3f_branches", "ttm21f_branches", "contracted_branches",
-                                  "qspcfw_branches"])
+                                  "ttm3f_bead_nh", "contracted_bead_nh"])
 def test_graph_replay_matches_eager(case):
-    """*_branches: the graph's force calls split into a real-space and a
-    reciprocal-space branch on two streams (PIMD_GRAPH_BRANCHES=1)."""
+    """*_bead_nh: the normal-mode Nose-Hoover chains on the beads in place of
+    the Langevin thermostat."""
     kw = {"ttm3f": (3, 64, 8, {}),
           "ttm21f_cmd_nh": (2, 32, 4, {"setNMfreq": 1500.0, "global_nh": True}),
           "contracted": (3, 32, 4, {"intra_timesteps": 3}),
           "qspcfw": (4, 64, 8, {}),
-          "ttm3f_branches": (3, 64, 8, {}),
-          "ttm21f_branches": (2, 32, 4, {}),
-          "contracted_branches": (3, 32, 4, {"intra_timesteps": 3}),
-          "qspcfw_branches": (4, 64, 8, {})}[case]
+          "ttm3f_bead_nh": (3, 64, 8, {"bead_thermostat": 2}),
+          "contracted_bead_nh": (3, 32, 4, {"intra_timesteps": 3, "bead_thermostat": 2})}[case]
     model, N, P, md = kw
     print("case", case, flush=True)
     a = _runs(_integrator(model, N, P, 77, **md), graph=False)
-    b = _runs(_integrator(model, N, P, 77, **md), graph=True, branches=case.endswith("_branches"))
+    b = _runs(_integrator(model, N, P, 77, **md), graph=True)
     for (ra, oa, sa), (rb, ob, sb) in zip(a, b):
         for k in ("Upot", "virial", "virialc", "uk"):
             assert np.array_equal(ra[k], rb[k]), k
@@ -68,16 +64,24 @@ def test_graph_replay_matches_eager(case):
             assert np.array_equal(sa[k], sb[k]), k
 
 
+# a batch of unlike replicas: Langevin and Nose-Hoover beads, CMD with the
+# global chains, a fixed-charge model and contracted MD
+BATCH = [(3, 32, 4, {}), (2, 32, 8, {"bead_thermostat": 2}),
+         (3, 64, 4, {"setNMfreq": 1500.0, "global_nh": True}), (4, 32, 4, {}),
+         (3, 32, 4, {"intra_timesteps": 3})]
+
+
 def test_batch_graph_matches_eager():
     from pimd_amd.ttm import ReplicaBatch
     res = []
+    seeds = [5, 6, 7, 8, 9]
     for graph in (False, True):
         os.environ["PIMD_MD_GRAPH"] = "1" if graph else "0"
         try:
-            integs = [_integrator(3, 32, 4, 90 + r) for r in range(3)]
+            integs = [_integrator(m, N, P, 90 + r, **dict(md)) for r, (m, N, P, md) in enumerate(BATCH)]
             batch = ReplicaBatch(integs)
-            r1 = batch.run(3, [5, 6, 7], t0=1)
-            r2 = batch.run(2, [5, 6, 7], t0=4)
+            r1 = batch.run(3, seeds, t0=1)
+            r2 = batch.run(2, seeds, t0=4)
             res.append((r1, r2, [i.get_state()["RRt"] for i in integs]))
             batch.close()
         finally:

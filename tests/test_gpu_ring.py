@@ -91,10 +91,12 @@ def test_trajectory_vs_reference(name):
     print("case", name, flush=True)
     ff = _ff(p)
     setnm = float(g.get("setnm", 0.0))
-    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0,
-                           bead_thermostat=1 if noise is not None else 0,
-                           centroid_thermostat=True, tau_centroid=0.05, global_nh=glob, tau=0.1,
-                           setNMfreq=setnm)
+    # Langevin with the reference's recorded normals, the NM-space Nose-Hoover
+    # chains (bead_tt 2), or none
+    bead_tt = int(g["bead_tt"]) if "bead_tt" in g else (1 if noise is not None else 0)
+    integ = PIMDIntegrator(ff, delt_fs=0.25, temp=300.0, bead_thermostat=bead_tt,
+                           centroid_thermostat=bool(g.get("centroid", True)), tau_centroid=0.05,
+                           global_nh=glob, tau=0.1, setNMfreq=setnm)
     integ.set_state(g["RRt"], g["PPt"], None)
     rec = integ.run(nsteps, t0=1, noise=noise)
     obs = integ.observables(nsteps)

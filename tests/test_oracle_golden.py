@@ -133,10 +133,53 @@ def observables(p, R, dipI, Upot, virial, virialc, uk, omegan, masses):
     return np.array([temp, qe, qp, se, sp, *mol.sum(axis=0), np.sum(np.linalg.norm(mol, axis=1))])
 
 
+def nh_chain2(u, v0, v1, tau, dt2, L, temp):
+    """Nose_Hoover with a chain of length 2 (Nose_Hoover.f90:33-112),
+    elementwise over arrays: returns (s, u, v0, v1)."""
+    d4, d8 = dt2 / 2.0, dt2 / 4.0
+    kT = KB_AMUA2PS2PERK * temp
+    M1 = L * kT * (tau * tau)
+    M2 = M1
+    v1 = v1 + (M1 * (v0 * v0) - kT) * d4 / M2
+    v0 = v0 * np.exp(-v1 * d8)
+    v0 = v0 + (2 * u - L * kT) * d4 / M1
+    v0 = v0 * np.exp(-v1 * d8)
+    s = np.exp(-v0 * dt2)
+    u = u * s * s
+    v0 = v0 * np.exp(-v1 * d8)
+    v0 = v0 + (2 * u - L * kT) * d4 / M1
+    v0 = v0 * np.exp(-v1 * d8)
+    v1 = v1 + (M1 * (v0 * v0) - kT) * d4 / M2
+    return s, u, v0, v1
+
+
+def bead_nose_hoover(nm, Pm, vxi, masses, centroid, tau_centroid, dt2, temp_beads):
+    """bead_thermostat's Nose-Hoover branch (main_stuff.f90:145-171): real2NM
+    of every atom's momenta, one chain per (atom, mode, component) with L = 1,
+    the centroid only with CENTROIDTHERMOSTAT, NM2real.  vxi [Natoms][P][3][2]
+    is updated in place."""
+    C, om, msf = nm["C"], nm["omegalist"], nm["MassScaleFactor"]
+    P = C.shape[0]
+    q = np.einsum("jk,kac->ajc", C, Pm)           # [atom][mode][component]
+    u = 0.5 * (1.0 / masses)[:, None, None] * q ** 2
+    u[:, 1:, :] = u[:, 1:, :] / msf[None, 1:, None]
+    tau = np.where(np.arange(P) == 0, tau_centroid, 1.0 / np.where(om > 0, om, 1.0))
+    s, _, v0, v1 = nh_chain2(u, vxi[..., 0], vxi[..., 1], tau[None, :, None], dt2, 1.0,
+                             temp_beads)
+    live = np.ones(P, dtype=bool)
+    live[0] = centroid
+    s = np.where(live[None, :, None], s, 1.0)
+    vxi[..., 0] = np.where(live[None, :, None], v0, vxi[..., 0])
+    vxi[..., 1] = np.where(live[None, :, None], v1, vxi[..., 1])
+    return np.einsum("kj,akc->jac", C, q * s)
+
+
 def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=None,
-                   centroid=True, tau_centroid=0.05, global_nh=False, tau=0.1, setnm=0.0):
+                   centroid=True, tau_centroid=0.05, global_nh=False, tau=0.1, setnm=0.0,
+                   bead_nh=False):
     """PIMD_VV (MD.f90:35-110) composed from the oracle's pieces, with the
-    per-step observables of write_out."""
+    per-step observables of write_out.  Bead thermostat: Langevin with the
+    given normals, or with bead_nh the normal-mode Nose-Hoover chains."""
     P, na = p.nbeads, p.natoms
     dt = delt_fs / 1000.0
     dt2 = dt / 2
@@ -157,10 +200,18 @@ def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=N
         return 0.5 * np.sum((1.0 / masses)[None, :, None] * Pm ** 2)
 
     uk = uk_of(Pm)
+    vxi_b = np.zeros((na, P, 3, 2))
+
+    def thermostat(Pm, half):
+        if noise is not None:
+            return oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, half], Pm)
+        if bead_nh:
+            return bead_nose_hoover(nm, Pm, vxi_b, masses, centroid, tau_centroid, dt2, P * temp)
+        return Pm
+
     out = []
     for st in range(nsteps):
-        if noise is not None:
-            Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 0], Pm)
+        Pm = thermostat(Pm, 0)
         if global_nh:
             r = oracle_lib.nose_hoover(uk, vxi, tau, dt2, L, temp * P)
             vxi, uk = r["vxi"], r["uk"]
@@ -176,15 +227,19 @@ def oracle_pimd_vv(oracle_lib, p, RRt, PPt, dRRt, nsteps, temp, delt_fs, noise=N
         obs = observables(p, R, f["dipI"], f["En"].sum(), np.trace(f["virt"], axis1=1, axis2=2).sum(),
                           f["virialc"].sum() / P, uk_of(Pm), omegan, masses)
         Pm = Pm - MASSCON * dR * dt2
-        if noise is not None:
-            Pm = oracle_lib.langevin(nm, c1, c2, p.mass_o, p.mass_h, noise[st, 1], Pm)
+        Pm = thermostat(Pm, 1)
         uk = uk_of(Pm)
         if global_nh:
             r = oracle_lib.nose_hoover(uk, vxi, tau, dt2, L, temp * P)
             vxi, uk = r["vxi"], r["uk"]
             Pm = Pm * r["s"]
         out.append((f["En"].sum(), uk, *obs))
-    return R, Pm, dR, np.array(out)
+    return R, Pm, dR, np.array(out), vxi_b
+
+
+# the 100-step 32-bead run of the CPU-runnable system: its first steps here
+# (the whole run takes minutes on the oracle; the GPU tests run all of it)
+ORACLE_STEPS = {"traj_ttm3f_nve_n32_p32_rc49": 20}
 
 
 @pytest.mark.parametrize("name", golden_names("traj_"))
@@ -193,17 +248,37 @@ def test_trajectory_vs_reference(oracle_lib, name):
     p = params_from(g)
     noise = g.get("noise")
     glob = bool(g.get("global_nh", False))
-    nsteps = g["Upot"].shape[0]
-    R, Pm, dR, rec = oracle_pimd_vv(oracle_lib, p, g["RRt"], g["PPt"], np.zeros_like(g["RRt"]),
-                                    nsteps, 300.0, 0.25, noise=noise, global_nh=glob,
-                                    setnm=float(g.get("setnm", 0.0)))
-    assert rel_err(rec[:, 0], g["Upot"][:, :, 0].sum(axis=1)) < 1e-12
-    assert rel_err(rec[:, 1], g["uk"]) < 1e-12
+    bead_nh = int(g.get("bead_tt", 0)) == 2
+    full = g["Upot"].shape[0]
+    nsteps = ORACLE_STEPS.get(name, full)
+    R, Pm, dR, rec, vxi_b = oracle_pimd_vv(
+        oracle_lib, p, g["RRt"], g["PPt"], np.zeros_like(g["RRt"]), nsteps, 300.0, 0.25,
+        noise=noise, global_nh=glob, setnm=float(g.get("setnm", 0.0)), bead_nh=bead_nh,
+        centroid=bool(g.get("centroid", True)))
+    assert rel_err(rec[:, 0], g["Upot"][:nsteps, :, 0].sum(axis=1)) < 1e-12
+    assert rel_err(rec[:, 1], g["uk"][:nsteps]) < 1e-12
     for i in range(9):
-        assert rel_err(rec[:, 2 + i], g["obs"][:, i]) < 1e-11, i
+        assert rel_err(rec[:, 2 + i], g["obs"][:nsteps, i]) < 1e-11, i
+    if nsteps < full:
+        return
     assert rel_err(R, g["RRt_end"]) < 1e-12
     assert rel_err(Pm, g["PPt_end"]) < 1e-11
     assert max_abs(dR, g["dRRt_end"]) < 1e-10
+    if bead_nh:
+        assert rel_err(vxi_b, g["vxi_beads"]) < 1e-11
+
+
+def test_bead_nose_hoover_restatement_is_pinned(oracle_lib):
+    """A plausible slip in the bead chains (the centroid chain without the
+    centroid thermostat, or unscaled mode masses) fails the reference
+    trajectory that the test above matches."""
+    g = load_golden("traj_ttm3f_beadnh_cmd_n8_p4")
+    p = params_from(g)
+    args = (oracle_lib, p, g["RRt"], g["PPt"], np.zeros_like(g["RRt"]), 6, 300.0, 0.25)
+    good = oracle_pimd_vv(*args, setnm=1500.0, bead_nh=True)[1]
+    assert rel_err(good, g["PPt_end"]) < 1e-11
+    plain = oracle_pimd_vv(*args, setnm=1500.0, bead_nh=False)[1]
+    assert rel_err(plain, g["PPt_end"]) > 1e-6
 
 
 def _pes_all(oracle_lib, p, R):

@@ -17,7 +17,10 @@ import pytest
 from pimd_amd import runinput
 
 HERE = os.path.join(os.path.dirname(__file__), "golden", "inputs")
-CASES = sorted(f[:-4] for f in os.listdir(HERE) if f.endswith(".inp"))
+# inputs with the reference reader's record beside them (the drop-in program
+# inputs in the same folder have none)
+CASES = sorted(f[:-4] for f in os.listdir(HERE)
+               if f.endswith(".inp") and os.path.exists(os.path.join(HERE, f[:-4] + ".json")))
 
 
 def _typed(kind, text):
@@ -86,6 +89,12 @@ def test_run_setup_from_reference_input():
     assert p.rc == 4.9 and p.rc1 == np.float32(0.8) * 4.9     # rc1 default 0.8 Rc
     assert p.mass_h == np.float32(1.008) and p.polar_sor == 0.6
     assert md["bead_thermostat"] == 1 and md["intra_timesteps"] == 4 and md["delt_fs"] == 0.25
+    # the normal-mode Nose-Hoover bead thermostat (main_stuff.f90:145-171)
+    inp["bead_thermostat_type"] = "Nose-Hoover"
+    assert runinput.run_setup(inp, (9.865, 9.865, 9.865), 96)[1]["bead_thermostat"] == 2
+    inp["bead_thermostat_type"] = "Andersen"
+    with pytest.raises(runinput.InputError, match="Andersen"):
+        runinput.run_setup(inp, (9.865, 9.865, 9.865), 96)
     inp2 = runinput.read_input(os.path.join(HERE, "input_new_64.inp"), warn=lambda m: None)
     with pytest.raises(runinput.InputError, match="pot_model 6"):
         runinput.run_setup(inp2, (12.4, 12.4, 12.4), 192)
