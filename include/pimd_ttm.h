@@ -109,6 +109,27 @@ int pimd_forces_all_beads_dev(pimd_ctx *ctx, const double *d_RRt, const double *
                               double *d_virialct, double *d_dip_momIt, double *d_dip_momEt,
                               double *d_chg, int *d_iters);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pimd_forces_all_beads_dev without a context.  The
+ * parameters are checked and set up as pimd_ctx_create does; every
+ * intermediate array of the call lives in one static device workspace of
+ * PIMD_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, and the
+ * call makes no device or pinned allocation.  Systems whose arrays exceed the
+ * workspace are refused with PIMD_EINVAL (128 waters x 32 beads fit at
+ * eps_ewald 1e-6 with Rc = L/2; 512 x 32 does not).  guess_initdip must be 0
+ * (the predictor history would have to persist between calls).  Device
+ * arrays as in pimd_forces_all_beads_dev; outputs may be NULL.  stream: a
+ * hipStream_t, NULL for the default stream; the kernels are enqueued there
+ * and the call returns without waiting for them.  Every call on one GPU
+ * shares that GPU's workspace: calls that overlap on one GPU (on different
+ * streams or host threads) race. */
+#define PIMD_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int pimd_forces_all_beads_noalloc(const pimd_params *params, const double *d_RRt,
+                                  const double *d_RRc, int t, double *d_Upott, double *d_dRRt,
+                                  double *d_virt, double *d_virialct, double *d_dip_momIt,
+                                  double *d_dip_momEt, double *d_chg, int *d_iters, void *stream);
+
 /* Energy terms of the last force call, per bead: [P][5] =
  * Umon, Uvdw, Uelec, Uind, Uvdw_lrc (system_mod.f90:30 module variables). */
 int pimd_energy_terms(pimd_ctx *ctx, double *terms);

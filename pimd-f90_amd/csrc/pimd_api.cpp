@@ -2067,6 +2067,102 @@ extern "C" int pimd_state_forces(pimd_ctx *x, int t) {
   return run_forces(x);
 }
 
+// ---------------------------------------------------------------------------
+// A force call that allocates nothing (pimd_forces_all_beads_noalloc): every
+// intermediate array of the call lives in one static device workspace per
+// GPU, set up at the first call on that device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[PIMD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of this device's workspace (the device must be
+// current); the symbol's own name is only a host-side handle.
+int scratch_on_device(int dev, char **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(PIMD_EINVAL, "device %d out of range", dev);
+  if (!g_scratch_ptr[dev]) {
+    char *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PIMD_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PIMD_OK;
+}
+}  // namespace
+
+extern "C" int pimd_forces_all_beads_noalloc(const pimd_params *params, const double *RR,
+                                             const double *RRc, int t, double *Upot, double *dRR,
+                                             double *virt, double *virialc, double *dipI,
+                                             double *dipE, double *chg, int *iters, void *stream) {
+  if (!params || !RR || !RRc) return fail(PIMD_EINVAL, "null argument");
+  pimd_params p = *params;
+  int rc = check_params(p);
+  if (rc != PIMD_OK) return rc;
+  if (p.guess_initdip)
+    return fail(PIMD_EINVAL, "the dipole predictor keeps a history between calls; "
+                "pimd_forces_all_beads_noalloc needs guess_initdip = 0");
+  char *base = nullptr;
+  if ((rc = scratch_on_device(p.device, &base)) != PIMD_OK) return rc;
+  // a host-side context whose device buffers are carved from the workspace
+  pimd_ctx x;
+  x.p = p;
+  x.natoms = 3 * p.nwaters;
+  x.stream = static_cast<hipStream_t>(stream);
+  if ((rc = setup_system(&x)) != PIMD_OK) return rc;
+  HostTables tabs;
+  plan_system(&x, tabs);
+  // one stream and no host polling: nothing here creates streams or pinned memory
+  x.c.poll = 0;
+  x.c.overlap = 0;
+  // partial sums for the chunks the structure factors use (the same
+  // chunking as a context's, so the results are the same bits)
+  x.c.sf_chunks_max = sfac_chunks_used(x.c);
+  Arena a;
+  layout_system(&x, a, false);
+  if (a.off > sizeof(scratch))
+    return fail(PIMD_EINVAL, "%d waters x %d beads need %zu bytes of device workspace, more than "
+                "the %zu bytes of pimd_forces_all_beads_noalloc", p.nwaters, p.nbeads, a.off,
+                sizeof(scratch));
+  a.base = base;
+  a.off = 0;
+  layout_system(&x, a, false);
+  if ((rc = upload_system(&x, tabs, x.stream)) != PIMD_OK) return rc;
+  x.c.g_nolds = pick_group(x.c);
+  x.m.massO = p.mass_o;
+  x.m.massH = p.mass_h;
+  if ((rc = sync_rep(&x)) != PIMD_OK) return rc;  // the replica record, in the workspace too
+  DevBufs &d = x.d;
+  hipStream_t s = x.stream;
+  const size_t P = p.nbeads, N = p.nwaters, na = 3 * N;
+  HIPCHK(hipMemcpyAsync(d.RR, RR, sizeof(double) * P * na * 3, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.RRc, RRc, sizeof(double) * na * 3, hipMemcpyDeviceToDevice, s));
+  launch_set_clock(x.self, t, 0, s);
+  rc = run_forces(&x);
+  struct Out {
+    void *dst;
+    const void *src;
+    size_t bytes;
+  } outs[] = {{Upot, d.Upot, sizeof(double) * P},
+              {dRR, d.dRR, sizeof(double) * P * na * 3},
+              {virt, d.virt, sizeof(double) * P * 9},
+              {virialc, d.virialc, sizeof(double) * P},
+              {dipI, d.dipI, sizeof(double) * P * N * 3},
+              {dipE, d.dipE, sizeof(double) * P * N * 3},
+              {chg, d.chg, sizeof(double) * P * na},
+              {iters, d.iters_out, sizeof(int) * P}};
+  for (const Out &o : outs)
+    if (rc == PIMD_OK && o.dst)
+      HIPCHK(hipMemcpyAsync(o.dst, o.src, o.bytes, hipMemcpyDeviceToDevice, s));
+  x.d = DevBufs{};  // workspace addresses, not allocations of x
+  return rc;
+}
+
 __global__ void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)

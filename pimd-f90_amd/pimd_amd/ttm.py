@@ -37,6 +37,7 @@ EXPORTS = (
     "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
     "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy", "pimd_ttm_params_get",
     "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
+    "pimd_forces_all_beads_noalloc",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -119,6 +120,8 @@ def load_library(path: str = LIB_PATH):
         "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
+        "pimd_forces_all_beads_noalloc": [C.POINTER(PimdParams), vp, vp, C.c_int, vp, vp, vp, vp,
+                                          vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -151,6 +154,20 @@ def c_params(p: TTMParams, device: int = 0) -> PimdParams:
     return PimdParams(p.pot_model, p.nwaters, p.nbeads, p.polar_maxiter, int(p.guess_initdip),
                       int(p.precision), device, int(p.contraction), (C.c_double * 3)(*p.box), p.rc,
                       p.rc1, p.eps_ewald, p.polar_sor, p.polar_eps, p.mass_o, p.mass_h)
+
+
+def forces_noalloc(params: TTMParams, d_RRt, d_RRc, t: int = 1, outs=None, stream=None,
+                   device: int = 0):
+    """pimd_forces_all_beads_noalloc on device addresses (ints): no context and no
+    device allocation.  outs: dict of device addresses for Upott, dRRt, virt,
+    virialct, dip_momIt, dip_momEt, chg, iters (missing ones stay in the
+    library's workspace).  stream: a hipStream_t handle as an int, or None for
+    the default stream.  Enqueued only: synchronise before reading."""
+    o = outs or {}
+    cp = c_params(params.resolved(), device)
+    keys = ("Upott", "dRRt", "virt", "virialct", "dip_momIt", "dip_momEt", "chg", "iters")
+    _check(load_library().pimd_forces_all_beads_noalloc(
+        C.byref(cp), d_RRt, d_RRc, int(t), *[o.get(k) for k in keys], stream))
 
 
 class TTMForceField:
