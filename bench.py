@@ -52,34 +52,58 @@ MODEL_NAMES = {2: "TTM2.1-F", 3: "TTM3-F", 4: "q-SPC/Fw", 5: "SPC/F"}
 
 
 def flop_model(kernel, c):
-    """Algorithmic FP64 operations per launch (transcendentals, sqrt and
-    divisions count one each), from the operation counts of each kernel's
-    inner loop; see the kernel-table of the design notes."""
+    """Algorithmic FP64/FP32 operations per launch: what the algorithm needs,
+    not what the kernel issues.  A fused multiply-add counts two, a
+    transcendental, square root or division one.  Pair terms are counted once
+    per unordered molecule pair within the cutoff (the kernels evaluate each
+    pair from both molecules, which is not credited); reciprocal-space terms
+    once per (site, k) with one complex product to advance the phase along
+    the kz column:
+      structure factors  q:  phase 6 + accumulate 4                     = 10
+                         d:  phase 6 + k.mu 2 + accumulate 4 + setup 2   = 14
+                         v:  phase 6 + 3 components x 4                  = 18
+      fields             q:  phase 6 + field 6 + potential 3             = 15
+                         d:  phase 6 + field 6                           = 12
+                         qd: phase 6 + charge 7 + dipole 7 + column 3    = 23
+      pairs  qq: O-O vdW 30 + 9 charge-site pairs x 71 (vector and
+                 minimum image 21, damped erfc-screened kernels 40, both
+                 sides' fields 10); fixed charges 30 + 9 x 52
+             dd: per dipole-site pair: vector 21, smeared tensor 45, both
+                 sides' T.mu 18                                          = 84
+             final: 5 site pairs (TTM3-F) x 111."""
     N, P, nk, nnb, nd = c["N"], c["P"], c["nk"], c["nnb"], c["nd"]
-    pairs = P * N * nnb  # ordered molecule pairs (each side evaluated once)
+    pairs = P * N * nnb / 2.0  # unordered molecule pairs within the cutoff
     if c.get("model", 3) >= 4 and kernel == "pair_qq":
-        # fixed charges: vdW ~30 + per site pair: vectors 17, erfc-screened ts0/ts1 ~26, field 9
         return pairs * (30 + 9 * 52)
-    if kernel == "field_d_sor":   # per (dipole site, k): phase step 6 + field 8
-        return P * N * nd * nk * 14
-    if kernel == "sfac_d":        # per (dipole site, k): phase 12 + k.mu 5 + acc 4
-        return P * N * nd * nk * 21
-    if kernel == "pair_dd":       # per ordered pair and dipole-site pair: vectors 21, smear2 ~45, T.mu 18
-        return pairs * nd * nd * 84
-    if kernel == "dd_sfac_d":     # pair_dd and sfac_d run side by side on two streams
-        return pairs * nd * nd * 84 + P * N * nd * nk * 21
-    if kernel == "pair_qq":       # vdW ~30 + per site pair: vectors 21, smear01 ~40 (+Thole), field 10
-        return pairs * (30 + 9 * 71)
-    if kernel == "pair_final":    # 5 site pairs (TTM3-F): smear2/3 ~50, tensors ~40, vectors 21
-        return pairs * 5 * 111
-    if kernel == "field_q":
-        return P * N * 3 * nk * 18
-    if kernel == "field_qd":
-        return P * N * 3 * nk * 24
-    if kernel == "sfac_q":
-        return P * N * 3 * nk * 16
-    if kernel == "sfac_v":
-        return P * N * nd * nk * 24
+    table = {
+        "field_d_sor": P * N * nd * nk * 12,
+        "sfac_d": P * N * nd * nk * 14,
+        "pair_dd": pairs * nd * nd * 84,
+        "pair_qq": pairs * (30 + 9 * 71),
+        "pair_final": pairs * 5 * 111,
+        "field_q": P * N * 3 * nk * 15,
+        "field_qd": P * N * 3 * nk * 23,
+        "sfac_q": P * N * 3 * nk * 10,
+        "sfac_v": P * N * nd * nk * 18,
+    }
+    v = table.get(kernel)
+    return None if v is None else int(v)
+
+
+def measured_valu(form, key):
+    """Issued VALU floating-point operations per launch of a kernel from the
+    committed rocprofv3 SQ_INSTS_VALU_* summary (64 lanes per instruction,
+    an FMA two), or None."""
+    path = os.path.join(ROOT, "profiles", "valu_counters_h2o512_p32_ttm3f.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == form and cells[2] == key:
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
     return None
 
 
@@ -179,19 +203,27 @@ def run_form(p, RRt, PPt, RRc, args):
         peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
         avg_s = tms / cnt / 1000.0
         ach = fl / avg_s / 1e12
-        traffic = measured_traffic(args.workload, "mixed" if p.precision == 1 else "fp64", dom)
-        out["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak,
+        form = "mixed" if p.precision == 1 else "fp64"
+        traffic = measured_traffic(args.workload, form, dom)
+        issued = measured_valu(form, dom) if args.workload == "h2o512_p32_ttm3f" else None
+        out["roofline"] = {"bound": "valu_fp32" if fp32 else "valu_fp64",
+                           "achieved": round(ach, 3), "peak": peak,
                            "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic,
-                           "kernel": dom, "arith": "fp32" if fp32 else "fp64",
-                           "avg_launch_ms": round(tms / cnt, 4), "flops_per_launch": fl,
-                           "traffic_source": "profiles/hbm_traffic_h2o512_p32_ttm3f.md"}
+                           "kernel": dom, "avg_launch_ms": round(tms / cnt, 4),
+                           "flops_per_launch": fl,
+                           "issued_flops_per_launch": issued,
+                           "issued_frac": (round(issued / avg_s / 1e12 / peak, 4)
+                                           if issued else None),
+                           "traffic_source": "profiles/hbm_traffic_h2o512_p32_ttm3f.md",
+                           "issued_source": "profiles/valu_counters_h2o512_p32_ttm3f.md"}
     ff.close()
     return out
 
 
 def run_batched(p, nrep, args, seed=2024):
     """nrep replicas of the workload (seeds seed, seed + 1, ...) stepped side
-    by side as one batch (pimd_batch_md_run, one stream per replica); whole-job
+    by side as one batch (pimd_batch_md_run: each kernel one launch over all
+    the replicas, one captured graph per batch step); whole-job
     molecule-bead force evaluations per second."""
     from pimd_amd.systems import make_system
     from pimd_amd.ttm import PIMDIntegrator, ReplicaBatch, TTMForceField

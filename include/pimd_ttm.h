@@ -130,6 +130,12 @@ int pimd_forces_all_beads_noalloc(const pimd_params *params, const double *d_RRt
                                   double *d_virt, double *d_virialct, double *d_dip_momIt,
                                   double *d_dip_momEt, double *d_chg, int *d_iters, void *stream);
 
+/* calc_monomer_dip_moments (force_calc.f90:205-264): the monomer dipole
+ * moment of every molecule of every bead from its dipole-moment-surface
+ * charges.  RRt [P][Natoms][3] host; dip_momIt [P][Nwaters][3] host.  Does
+ * not change the results of the last force call. */
+int pimd_monomer_dipoles(pimd_ctx *ctx, const double *RRt, double *dip_momIt);
+
 /* Energy terms of the last force call, per bead: [P][5] =
  * Umon, Uvdw, Uelec, Uind, Uvdw_lrc (system_mod.f90:30 module variables). */
 int pimd_energy_terms(pimd_ctx *ctx, double *terms);

@@ -243,6 +243,7 @@ void apply_ttm_params(pimd_ctx *x) {
 
 int env_int(const char *name, int dflt, int lo, int hi);
 
+
 // init_pot + ewald_set (potential.f90:43-199, ewald.f90:293-378) on the host.
 int setup_system(pimd_ctx *x) {
   const pimd_params &p = x->p;
@@ -545,6 +546,8 @@ void layout_system(pimd_ctx *x, Arena &a, bool full) {
   a.take(&d.nbrc, P * N);
   a.take(&d.nbrn, P * N);
   a.take(&d.tab, P * (size_t)c.T * S);
+  a.take(&d.tabd, P * (size_t)c.T * N * nd);
+  if (c.mixed) a.take(&d.tabdf, P * (size_t)c.T * N * nd);
   a.take(&d.Sq, P * nk * 2);
   a.take(&d.Sm, P * nk * 2);
   a.take(&d.Sv, P * nk * 6);
@@ -717,12 +720,12 @@ int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
     }
     if (any_mixed) {
       // Iterative refinement of the mixed form: redo each bead's last SOR
-      // step in FP64 from the dipoles it started from.  The reciprocal-space
-      // dipole structure factor of that step is still in place (beads stop
-      // updating it once converged).
+      // step in FP64 from the dipoles it started from, both halves of the
+      // dipole field (the iterations took them in FP32).
       timed(px, "refine_dd", [&] {
         launch_refine_prep(g, s);
         launch_pair_dd(g, true, s, 0, true);
+        launch_sfac(g, SF_D, s, true);
         launch_field(g, FD_D, -1, s, true);
       });
     }
@@ -994,6 +997,27 @@ int pimd_energy_terms(pimd_ctx *x, double *terms) {
   if (!x || !terms) return fail(PIMD_EINVAL, "null argument");
   HIPCHK(hipSetDevice(x->p.device));
   HIPCHK(hipMemcpyAsync(terms, x->d.terms, sizeof(double) * x->c.P * 5, hipMemcpyDeviceToHost,
+                        x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return PIMD_OK;
+}
+
+// calc_monomer_dip_moments (force_calc.f90:205-264) for every bead: RRt
+// [P][Natoms][3] in, dip_momIt [P][N][3] out, through the context's
+// coordinate staging.
+int pimd_monomer_dipoles(pimd_ctx *x, const double *RRt, double *dip) {
+  if (!x || !RRt || !dip) return fail(PIMD_EINVAL, "null argument");
+  int rc = check_finite(RRt, (size_t)x->c.P * x->natoms * 3, "RRt");
+  if (rc != PIMD_OK) return rc;
+  HIPCHK(hipSetDevice(x->p.device));
+  const size_t P = x->c.P, N = x->c.N;
+  // two intermediates of the force call, rewritten by every force call:
+  // kE [P][4N][3] takes the coordinates, kphi [P][4N] the dipoles
+  HIPCHK(hipMemcpyAsync(x->d.kE, RRt, sizeof(double) * P * 3 * N * 3, hipMemcpyHostToDevice,
+                        x->stream));
+  launch_monomer_dipoles(x->c, x->d.pes, x->d.kE, x->d.kphi, x->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dip, x->d.kphi, sizeof(double) * P * N * 3, hipMemcpyDeviceToHost,
                         x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return PIMD_OK;
@@ -2237,3 +2261,8 @@ int pimd_sync(pimd_ctx *x) {
 }
 
 }  // extern "C"
+
+int pimd::xcd_numbering() {
+  static const int v = env_int("PIMD_XCD_MAP", 1, 0, 1);
+  return v;
+}

@@ -106,6 +106,8 @@ struct DevBufs {
   int *nbrn = nullptr;                          // [P][N] near partners (listed first)
   int *cell_start = nullptr, *cell_mol = nullptr;  // [P][ncells + 1], [P][N] cell list
   double2 *tab = nullptr;                       // [P][T][4N]
+  double2 *tabd = nullptr;                      // [P][T][nd N] the dipole sites' rows
+  float2 *tabdf = nullptr;                      // the same in FP32 (mixed form)
   double *Sq = nullptr, *Sm = nullptr;          // [P][nk][2]
   double *Sv = nullptr;                         // [P][nk][6]
   double *virk = nullptr;                       // [P][nkb][9]
@@ -183,6 +185,7 @@ struct Rep {
 
 struct BlockMap {
   int n;
+  int xcd;  // 1: XCD-contiguous block numbering (xcd_block)
   int off[MAXREP + 1];
   int nx[MAXREP], ny[MAXREP];
   int rep[MAXREP];            // entry of the Rep table
@@ -193,8 +196,18 @@ struct Blk {
   int nx, ny;                 // its own grid's x and y extents
 };
 
+// The hardware hands workgroup w of a launch to XCD w % 8; the logical
+// block index is renumbered so that consecutive logical blocks (the blocks of
+// one bead, which stage the same bead records, tables and k data) run on one
+// XCD and share its L2: XCD x takes the x-th contiguous eighth of the grid.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block(int w, int n) {
+  const int q = n / NXCD, r = n % NXCD, x = w % NXCD;
+  return x * q + min(x, r) + w / NXCD;
+}
+
 __device__ __forceinline__ Blk block_of(const BlockMap &m) {
-  const int g = blockIdx.x;
+  const int g = m.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   int lo = 0, hi = m.n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -225,6 +238,9 @@ struct Plan {
 // Launches one grid per variant (and per MAXREP replicas of it), replicas in
 // table order.  plan(const SysConst &, const MDConst &) -> Plan;
 // launch(int variant, dim3 grid, const BlockMap &, size_t lds).
+// Whether launches number their blocks XCD-contiguously (PIMD_XCD_MAP, default 1).
+int xcd_numbering();
+
 template <class PlanFn, class LaunchFn>
 void launch_over(const Group &g, PlanFn plan, LaunchFn launch) {
   const int n = g.n();
@@ -238,6 +254,7 @@ void launch_over(const Group &g, PlanFn plan, LaunchFn launch) {
       continue;
     }
     BlockMap m{};
+    m.xcd = xcd_numbering();
     size_t lds = 0;
     long total = 0;
     for (int j = i; j < n && m.n < MAXREP; ++j) {
@@ -269,7 +286,8 @@ void launch_monomer(const Group &g, hipStream_t s);
 void launch_neighbors(const Group &g, hipStream_t s);
 void launch_pair_qq(const Group &g, hipStream_t s);
 void launch_tables(const Group &g, hipStream_t s);
-void launch_sfac(const Group &g, int mode, hipStream_t s);
+// refine: SF_D in FP64 for the mixed replicas' refinement step
+void launch_sfac(const Group &g, int mode, hipStream_t s, bool refine = false);
 int sfac_chunks_used(const SysConst &c);
 void launch_converge(const Group &g, int iter, hipStream_t s);
 // iter: the SOR iteration of FD_D, -1 for the FP64 refinement (maxiter + 1);
@@ -294,6 +312,8 @@ void launch_spc_finalize(const Group &g, hipStream_t s);
 // gradient (with record, also the energies and virials of the step record),
 // the centroid's gradient to every bead, and the step record
 void launch_pes_fast(const Group &g, bool record, hipStream_t s);
+void launch_monomer_dipoles(const SysConst &c, const double *pes, const double *RR, double *out,
+                            hipStream_t s);
 void launch_broadcast_beads(const Group &g, hipStream_t s);
 void launch_contract_record(const Group &g, hipStream_t s);
 

@@ -205,6 +205,57 @@ __global__ __launch_bounds__(64 * MONO_W) void k_monomer(const Rep *__restrict__
 // Contracted MD (MD.f90:276-449).
 // ---------------------------------------------------------------------------
 
+// Monomer dipole moment of every molecule of every bead from its dipole-
+// moment-surface charges (force_calc.f90:205-264): the H charges with the
+// M-site share folded in, times the H positions relative to the M site
+// (TTM3-F, q-SPC/Fw) or to the O (TTM2.1-F, SPC/F), minimum image.
+// One thread per (bead, molecule); RR [P][3N][3], out [P][N][3].
+__global__ __launch_bounds__(64) void k_monomer_dipoles(SysConst c, const double *__restrict__ pes,
+                                                        const double *__restrict__ RR,
+                                                        double *__restrict__ out) {
+  const int g = blockIdx.x * 64 + threadIdx.x;
+  if (g >= c.P * c.N) return;
+  const double *rr = RR + (size_t)g * 9;
+  double r[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 3; ++k) r[a][k] = rr[3 * a + k];
+  double e1, dr1[3][3], q3[3], gq[3][3][3];
+  ps_monomer<true>(c, r, e1, dr1, q3, gq, pes);
+  const double gM = c.gammaM, tq = 0.5 * gM / (1.0 - gM);
+  const double qh1 = q3[1] * CHARGECON + tq * (q3[1] * CHARGECON + q3[2] * CHARGECON);
+  const double qh2 = q3[2] * CHARGECON + tq * (q3[1] * CHARGECON + q3[2] * CHARGECON);
+  double o1[3], o2[3];
+  for (int k = 0; k < 3; ++k) {
+    o1[k] = r[1][k] - r[0][k];
+    o2[k] = r[2][k] - r[0][k];
+  }
+  minimg(c, o1[0], o1[1], o1[2]);
+  minimg(c, o2[0], o2[1], o2[2]);
+  const bool msite = c.model == PIMD_TTM3F || c.model == PIMD_QSPCFW;
+  double h1[3], h2[3];
+  for (int k = 0; k < 3; ++k) {
+    h1[k] = o1[k];
+    h2[k] = o2[k];
+  }
+  if (msite) {
+    for (int k = 0; k < 3; ++k) {
+      const double m = 0.5 * gM * (o1[k] + o2[k]) + r[0][k];
+      h1[k] = r[1][k] - m;
+      h2[k] = r[2][k] - m;
+    }
+    minimg(c, h1[0], h1[1], h1[2]);
+    minimg(c, h2[0], h2[1], h2[2]);
+  }
+  for (int k = 0; k < 3; ++k) out[(size_t)g * 3 + k] = qh1 * h1[k] + qh2 * h2[k];
+}
+
+void launch_monomer_dipoles(const SysConst &c, const double *pes, const double *RR, double *out,
+                            hipStream_t s) {
+  const int n = c.P * c.N;
+  hipLaunchKernelGGL(k_monomer_dipoles, dim3((n + 63) / 64), dim3(64), 0, s, c, pes, RR, out);
+}
+
+
 // Fast (intramolecular) forces: the PES gradient of every monomer of every
 // bead into dRR (MD.f90:339-353).  With record, also per molecule the energy,
 // the bead virial and the centroid virial (MD.f90:355-372) into mol[P][N][3].
@@ -674,11 +725,25 @@ __global__ __launch_bounds__(256) void k_tables(const Rep *__restrict__ reps, Bl
   const int hm[3] = {c.hmax, c.kmax, c.lmax};
   int row = 0;
   double2 *tb = tab + (size_t)b * c.T * S + s;
+  // the dipole sites' rows again, contiguous in dipole-site order (mol * nd +
+  // j), for the passes over dipole sites only; FP32 too in the mixed form
+  const int type = s & 3, MD = c.nd * c.N;
+  const bool dsite = type >= c.tdip && type < c.tdip + c.nd;
+  const size_t dm = (size_t)b * c.T * MD + (size_t)(s >> 2) * c.nd + (type - c.tdip);
+  double2 *td = Rp.b.tabd + dm;
+  float2 *tf = c.mixed ? Rp.b.tabdf + dm : nullptr;
+  auto put = [&](int r, double2 v) {
+    tb[(size_t)r * S] = v;
+    if (dsite) {
+      td[(size_t)r * MD] = v;
+      if (tf) tf[(size_t)r * MD] = make_float2((float)v.x, (float)v.y);
+    }
+  };
   for (int ax = 0; ax < 3; ++ax) {
     const double arg = TWOPI * c.boxi[ax] * xyz[ax];
     const double c1 = cos(arg), s1 = sin(arg);
     double cc = 1.0, ss = 0.0;
-    tb[(size_t)row++ * S] = make_double2(1.0, 0.0);
+    put(row++, make_double2(1.0, 0.0));
     for (int h = 1; h <= hm[ax]; ++h) {
       double nc, ns;
       if (h == 1) {
@@ -690,7 +755,7 @@ __global__ __launch_bounds__(256) void k_tables(const Rep *__restrict__ reps, Bl
       }
       cc = nc;
       ss = ns;
-      tb[(size_t)row++ * S] = make_double2(cc, ss);
+      put(row++, make_double2(cc, ss));
     }
   }
 }
@@ -716,12 +781,55 @@ __device__ __forceinline__ int set_site(const SysConst &c, int set, int m) {
 
 // Phase e^{i k.r} of one site from its table rows, in the reference's
 // product order (ewald.f90:30-39).
-__device__ __forceinline__ void phase_from(double2 x, double2 y, double2 z, double &ck,
-                                           double &sk) {
-  const double cxy = x.x * y.x - x.y * y.y;
-  const double sxy = x.y * y.x + x.x * y.y;
+template <class V, class R>
+__device__ __forceinline__ void phase_from(V x, V y, V z, R &ck, R &sk) {
+  const R cxy = x.x * y.x - x.y * y.y;
+  const R sxy = x.y * y.x + x.x * y.y;
   ck = cxy * z.x - sxy * z.y;
   sk = sxy * z.x + cxy * z.y;
+}
+
+// Two- and four-component vectors of an arithmetic type.
+template <class R> struct Vec;
+template <> struct Vec<double> {
+  using v2 = double2;
+  using v4 = double4;
+  static __device__ __forceinline__ double2 two(double a, double b) { return make_double2(a, b); }
+  static __device__ __forceinline__ double4 four(double a, double b, double c, double d) {
+    return make_double4(a, b, c, d);
+  }
+};
+template <> struct Vec<float> {
+  using v2 = float2;
+  using v4 = float4;
+  static __device__ __forceinline__ float2 two(double a, double b) {
+    return make_float2((float)a, (float)b);
+  }
+  static __device__ __forceinline__ float4 four(double a, double b, double c, double d) {
+    return make_float4((float)a, (float)b, (float)c, (float)d);
+  }
+};
+
+// Phase-table rows of the m-th member of a site set: the dipole sites from
+// their contiguous table (FP32 in the mixed form's SOR iterations), the
+// other sets from the all-site table.
+template <class R, int SET>
+__device__ __forceinline__ const typename Vec<R>::v2 *set_rows(const Rep &Rp, int b, int &stride) {
+  const SysConst &c = Rp.c;
+  if constexpr (SET == SET_DIP) {
+    stride = c.nd * c.N;
+    const size_t off = (size_t)b * c.T * stride;
+    if constexpr (sizeof(R) == 4) return Rp.b.tabdf + off;
+    else return Rp.b.tabd + off;
+  } else {
+    static_assert(sizeof(R) == 8, "the all-site table is FP64");
+    stride = 4 * c.N;
+    return Rp.b.tab + (size_t)b * c.T * stride;
+  }
+}
+template <int SET>
+__device__ __forceinline__ int row_index(const SysConst &c, int m) {
+  return SET == SET_DIP ? m : set_site(c, SET, m);
 }
 
 // Site chunk of the column structure-factor kernel and its padded LDS row
@@ -757,20 +865,21 @@ __host__ __device__ inline int sfac_sites(const SysConst &c, int mode) {
 // SPLIT = 2: the block holds two lane sets for the same segments, each over
 // half of the chunk's sites; the second hands its sums to the first through
 // LDS, which adds them and writes (twice the waves for the same partials).
-template <int MODE, int KZ, int SPLIT = 1>
+template <int MODE, int KZ, int SPLIT = 1, class R = double>
 __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, BlockMap bm) {
+  using R2 = typename Vec<R>::v2;
   constexpr int SET = MODE == SF_Q ? SET_CHG : SET_DIP;
   constexpr int NW = MODE == SF_Q ? 1 : 3;
   constexpr int NV = MODE == SF_V ? 6 : 2;  // values per k
   // sites per unrolled step: the grid, not the registers, limits the waves
   // per SIMD, so more independent phase chains per wave hide FP64 latency
   constexpr int SITE_UNROLL = MODE == SF_V ? 2 : 4;
-  extern __shared__ double2 lds_sf[];  // [T][scp] tables, then [sc][NW] weights
+  extern __shared__ double2 lds_raw[];
+  R2 *lds_sf = reinterpret_cast<R2 *>(lds_raw);  // [T][scp] tables, then [sc][NW] weights
   const Blk bk = block_of(bm);
   const Rep &Rp = reps[bk.r];
   const SysConst &c = Rp.c;
   const Site *__restrict__ site = Rp.b.site;
-  const double2 *__restrict__ tab = Rp.b.tab;
   const double *__restrict__ dip = Rp.b.dip;
   const int4 *__restrict__ cols = Rp.b.segs;
   const int *__restrict__ colk = Rp.b.segk;
@@ -778,16 +887,17 @@ __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, 
   double *__restrict__ part = Rp.b.part;
   int nch, sc, scp;
   sfac_chunking(c, MODE, sfac_sites(c, MODE), nch, sc, scp);
-  double *lw = reinterpret_cast<double *>(lds_sf + (size_t)c.T * scp);
+  R *lw = reinterpret_cast<R *>(lds_sf + (size_t)c.T * scp);
   const int b = bk.z;
   if (MODE == SF_D && !active[b]) return;
   const int S = 4 * c.N, M = set_size(c, SET);
   const int chunk = bk.y;
   const int m0 = chunk * sc, nt = min(sc, M - m0);
-  const double2 *tb = tab + (size_t)b * c.T * S;
+  int RS;
+  const R2 *__restrict__ tb = set_rows<R, SET>(Rp, b, RS);
   for (int e = threadIdx.x; e < c.T * sc; e += blockDim.x) {
     const int row = e / sc, l = e % sc;
-    if (l < nt) lds_sf[row * scp + l] = tb[(size_t)row * S + set_site(c, SET, m0 + l)];
+    if (l < nt) lds_sf[row * scp + l] = tb[(size_t)row * RS + row_index<SET>(c, m0 + l)];
   }
   for (int l = threadIdx.x; l < nt; l += blockDim.x) {
     const int sidx = set_site(c, SET, m0 + l);
@@ -795,9 +905,9 @@ __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, 
       lw[l] = site[(size_t)b * S + sidx].q;
     } else {
       const double *mu = dip + ((size_t)b * c.N * c.nd + m0 + l) * 3;
-      lw[l * 3 + 0] = mu[0];
-      lw[l * 3 + 1] = mu[1];
-      lw[l * 3 + 2] = mu[2];
+      lw[l * 3 + 0] = (R)mu[0];
+      lw[l * 3 + 1] = (R)mu[1];
+      lw[l * 3 + 2] = (R)mu[2];
     }
   }
   __syncthreads();
@@ -811,26 +921,30 @@ __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, 
   const int k0 = colk[live ? col : 0];
   const int rowx = cl.x, rowy = c.hmax + 1 + abs(cl.y), rowz = c.hmax + c.kmax + 2 + abs(cl.z);
   const int rowz1 = c.hmax + c.kmax + 3;
-  const double sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
-  const double kvx = TWOPI * c.boxi[0] * (double)cl.x, kvy = TWOPI * c.boxi[1] * (double)cl.y;
+  const R sgy = cl.y < 0 ? -1.0 : 1.0, sgz = cl.z < 0 ? -1.0 : 1.0;
+  const R kvx = TWOPI * c.boxi[0] * (double)cl.x, kvy = TWOPI * c.boxi[1] * (double)cl.y;
   const double tpz = TWOPI * c.boxi[2];
+  // kz of each k of the segment
+  R kzs[KZ];
+#pragma unroll
+  for (int i = 0; i < KZ; ++i) kzs[i] = (R)(tpz * (double)(cl.z + i));
   // SF_Q, SF_D: (re, im) per k; SF_V: (sum mu cos, sum mu sin) per component
-  double acc[NV][KZ];
+  R acc[NV][KZ];
 #pragma unroll
   for (int v = 0; v < NV; ++v)
 #pragma unroll
     for (int i = 0; i < KZ; ++i) acc[v][i] = 0.0;
   #pragma unroll SITE_UNROLL
   for (int e = live ? e0 : e1; e < e1; ++e) {
-    const double2 x = lds_sf[rowx * scp + e];
-    double2 y = lds_sf[rowy * scp + e];
-    double2 z = lds_sf[rowz * scp + e];
-    const double2 z1 = c.lmax >= 1 ? lds_sf[rowz1 * scp + e] : make_double2(1.0, 0.0);
+    const R2 x = lds_sf[rowx * scp + e];
+    R2 y = lds_sf[rowy * scp + e];
+    R2 z = lds_sf[rowz * scp + e];
+    const R2 z1 = c.lmax >= 1 ? lds_sf[rowz1 * scp + e] : Vec<R>::two(1.0, 0.0);
     y.y *= sgy;
     z.y *= sgz;
-    double ck, sk;
+    R ck, sk;
     phase_from(x, y, z, ck, sk);
-    double w = 0.0, wxy = 0.0, mz = 0.0;
+    R w = 0.0, wxy = 0.0, mz = 0.0;
     if (MODE == SF_Q) {
       w = lw[e];
     } else if (MODE == SF_D) {
@@ -841,7 +955,7 @@ __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, 
     for (int i = 0; i < KZ; ++i) {
       if (i < n) {
         if (i > 0) {
-          const double nc = ck * z1.x - sk * z1.y;
+          const R nc = ck * z1.x - sk * z1.y;
           sk = sk * z1.x + ck * z1.y;
           ck = nc;
         }
@@ -849,7 +963,7 @@ __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, 
           acc[0][i] += w * ck;
           acc[1][i] += w * sk;
         } else if (MODE == SF_D) {
-          const double dk = wxy + (tpz * (double)(cl.z + i)) * mz;
+          const R dk = wxy + kzs[i] * mz;
           acc[0][i] -= dk * sk;
           acc[1][i] += dk * ck;
         } else {
@@ -864,26 +978,27 @@ __global__ __launch_bounds__(512) void k_sfac_col(const Rep *__restrict__ reps, 
   }
   if (SPLIT == 2) {
     // after the weights, 16-byte aligned: [NV][KZ][nl] sums of the second set
-    double *red = lw + ((sc * NW + 1) & ~1);
+    constexpr int AL = 16 / sizeof(R);
+    R *red = lw + ((sc * NW + AL - 1) & ~(AL - 1));
     if (half_id == 1 && live)
 #pragma unroll
       for (int v = 0; v < NV; ++v)
 #pragma unroll
         for (int i = 0; i < KZ; ++i) red[(v * KZ + i) * nl + threadIdx.x % nl] = acc[v][i];
     __syncthreads();
-    if (half_id == 1) return;
+    if (half_id == 0)
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
+      for (int v = 0; v < NV; ++v)
 #pragma unroll
-      for (int i = 0; i < KZ; ++i) acc[v][i] += red[(v * KZ + i) * nl + threadIdx.x];
+        for (int i = 0; i < KZ; ++i) acc[v][i] += red[(v * KZ + i) * nl + threadIdx.x];
   }
-  if (!live) return;
+  if (!live || half_id != 0) return;
   double *o = part + (((size_t)b * nch + chunk) * c.nk + k0) * NV;
 #pragma unroll
   for (int i = 0; i < KZ; ++i)
     if (i < n)
 #pragma unroll
-      for (int v = 0; v < NV; ++v) o[NV * i + v] = acc[v][i];
+      for (int v = 0; v < NV; ++v) o[NV * i + v] = (double)acc[v][i];
 }
 
 // Structure factors with one k-vector per lane.  A block holds 256 k of one
@@ -906,7 +1021,6 @@ __global__ __launch_bounds__(256) void k_sfac_k(const Rep *__restrict__ reps, Bl
   const Rep &Rp = reps[bk.r];
   const SysConst &c = Rp.c;
   const Site *__restrict__ site = Rp.b.site;
-  const double2 *__restrict__ tab = Rp.b.tab;
   const double *__restrict__ dip = Rp.b.dip;
   const double4 *__restrict__ kv = Rp.b.kv;
   const int4 *__restrict__ khkl = Rp.b.khkl;
@@ -920,10 +1034,11 @@ __global__ __launch_bounds__(256) void k_sfac_k(const Rep *__restrict__ reps, Bl
   const int S = 4 * c.N, M = set_size(c, SET);
   const int chunk = bk.y;
   const int m0 = chunk * sc, nt = min(sc, M - m0);
-  const double2 *tb = tab + (size_t)b * c.T * S;
+  int RS;
+  const double2 *__restrict__ tb = set_rows<double, SET>(Rp, b, RS);
   for (int e = threadIdx.x; e < c.T * sc; e += blockDim.x) {
     const int row = e / sc, l = e % sc;
-    if (l < nt) lds_sf[row * scp + l] = tb[(size_t)row * S + set_site(c, SET, m0 + l)];
+    if (l < nt) lds_sf[row * scp + l] = tb[(size_t)row * RS + row_index<SET>(c, m0 + l)];
   }
   for (int l = threadIdx.x; l < nt; l += blockDim.x) {
     if (MODE == SF_Q) {
@@ -998,8 +1113,9 @@ __global__ __launch_bounds__(256) void k_sfac_reduce(const Rep *__restrict__ rep
   out[(size_t)b * c.nk * nv + e] = v;
 }
 
-static size_t sfac_lds(const SysConst &c, int mode, int sc, int scp) {
-  return sizeof(double2) * c.T * scp + sizeof(double) * sc * (mode == SF_Q ? 1 : 3);
+static size_t sfac_lds(const SysConst &c, int mode, int sc, int scp, bool f32 = false) {
+  const size_t r = f32 ? sizeof(float) : sizeof(double);
+  return 2 * r * c.T * scp + r * sc * (mode == SF_Q ? 1 : 3);
 }
 
 // The most site chunks any structure factor of a force call uses (the
@@ -1036,21 +1152,23 @@ int sfac_chunks_used(const SysConst &c) {
 constexpr int FD_SB = 64;  // sites per block, one per lane
 // iter: the SOR iteration (FD_D), or -1 for the mixed form's FP64
 // refinement, the step after the replica's last (maxiter + 1).
-template <int MODE, int MODEL, int CG, bool KL>
+template <int MODE, int MODEL, int CG, bool KL, class R = double>
 __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps, BlockMap bm,
                                                    int iter) {
+  using R2 = typename Vec<R>::v2;
+  using R4 = typename Vec<R>::v4;
   constexpr int SET = MODE == FD_D    ? SET_DIP
                       : MODE == FD_QD ? (MODEL == 2 ? SET_ALL_T : SET_CHG_T)
                                       : (MODEL == 2 ? SET_ALL : SET_CHG);
   // [T][64] phase tables of the block's sites (later reused for the
   // reduction), then with KL the bead's k data: [nk] (w Re S, w Im S, kz, w)
   // with w = 4 pi A_k / V, and for FD_QD [nk] (w Re(Sq+Sm), w Im(Sq+Sm))
-  extern __shared__ double2 lds_tab[];
+  extern __shared__ double2 lds_raw[];
+  R2 *lds_tab = reinterpret_cast<R2 *>(lds_raw);
   const Blk bk = block_of(bm);
   const Rep &Rp = reps[bk.r];
   const SysConst &c = Rp.c;
   const Site *__restrict__ site = Rp.b.site;
-  const double2 *__restrict__ tab = Rp.b.tab;
   const double4 *__restrict__ kv = Rp.b.kv;
   const int4 *__restrict__ cols = Rp.b.cols;
   const int *__restrict__ colk = Rp.b.colk;
@@ -1072,19 +1190,20 @@ __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps,
   const int M = set_size(c, SET);
   const int S = 4 * c.N;
   const int mbase = bk.x * FD_SB;
-  const double2 *tb = tab + (size_t)b * c.T * S;
+  int RS;
+  const R2 *__restrict__ tb = set_rows<R, SET>(Rp, b, RS);
   for (int e = threadIdx.x; e < c.T * FD_SB; e += 64 * CG) {
     const int row = e >> 6, l = e & 63;
-    lds_tab[e] = tb[(size_t)row * S + set_site(c, SET, min(mbase + l, M - 1))];
+    lds_tab[e] = tb[(size_t)row * RS + row_index<SET>(c, min(mbase + l, M - 1))];
   }
-  double4 *kd = reinterpret_cast<double4 *>(lds_tab + c.fd_tab_d2);
+  R4 *kd = reinterpret_cast<R4 *>(lds_raw + c.fd_tab_d2);
   double2 *kq = reinterpret_cast<double2 *>(kd + c.nk);
   if (KL) {
     const double *sx = (MODE == FD_Q ? Sq : Sm) + (size_t)b * c.nk * 2;
     const double *sq2 = Sq + (size_t)b * c.nk * 2;
     for (int k = threadIdx.x; k < c.nk; k += 64 * CG) {
       const double4 kk = kv[k];
-      kd[k] = make_double4(kk.w * sx[2 * k], kk.w * sx[2 * k + 1], kk.z, kk.w);
+      kd[k] = Vec<R>::four(kk.w * sx[2 * k], kk.w * sx[2 * k + 1], kk.z, kk.w);
       if (MODE == FD_QD) kq[k] = make_double2(kk.w * (sq2[2 * k] + sx[2 * k]),
                                               kk.w * (sq2[2 * k + 1] + sx[2 * k + 1]));
     }
@@ -1102,41 +1221,41 @@ __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps,
     for (int k = 0; k < 3; ++k) mu[k] = dip[dslot * 3 + k];
   const double q = (MODE == FD_QD) ? site[(size_t)b * S + s].q : 0.0;
   // e^{i 2 pi z/L}; with lmax = 0 every column holds one k and no step is taken
-  const double2 zstep =
-      c.lmax >= 1 ? lds_tab[(c.hmax + c.kmax + 3) * FD_SB + lane] : make_double2(1.0, 0.0);
+  const R2 zstep =
+      c.lmax >= 1 ? lds_tab[(c.hmax + c.kmax + 3) * FD_SB + lane] : Vec<R>::two(1.0, 0.0);
   const double *sq = Sq + (size_t)b * c.nk * 2, *sm = Sm + (size_t)b * c.nk * 2;
   double e0 = 0, e1 = 0, e2 = 0, ph = 0;
   for (int col = g; col < c.ncol; col += CG) {
     const int4 cl = cols[col];
     const int k0 = colk[col];
-    double2 x = lds_tab[cl.x * FD_SB + lane];
-    double2 y = lds_tab[(c.hmax + 1 + abs(cl.y)) * FD_SB + lane];
-    double2 z = lds_tab[(c.hmax + c.kmax + 2 + abs(cl.z)) * FD_SB + lane];
+    R2 x = lds_tab[cl.x * FD_SB + lane];
+    R2 y = lds_tab[(c.hmax + 1 + abs(cl.y)) * FD_SB + lane];
+    R2 z = lds_tab[(c.hmax + c.kmax + 2 + abs(cl.z)) * FD_SB + lane];
     if (cl.y < 0) y.y = -y.y;
     if (cl.z < 0) z.y = -z.y;
-    double ck, sk;
+    R ck, sk;
     phase_from(x, y, z, ck, sk);
     const double kx = kv[k0].x, ky = kv[k0].y;
     // column sums of t and kz t (t carries the sign of each mode's field)
-    double at = 0.0, az = 0.0;
+    R at = 0.0, az = 0.0;
     // FD_QD dipole sites: k.mu = (kx mux + ky muy) + kz muz
     const double dxy = (MODE == FD_QD) ? kx * mu[0] + ky * mu[1] : 0.0;
 #pragma unroll 4
     for (int i = 0; i < cl.w; ++i) {
       const int k = k0 + i;
       if (KL) {
-        const double4 q4 = kd[k];  // w Re S, w Im S, kz, w
+        const R4 q4 = kd[k];  // w Re S, w Im S, kz, w
         if (MODE == FD_Q) {
-          const double t = ck * q4.y - sk * q4.x;
+          const R t = ck * q4.y - sk * q4.x;
           at += t;
           az += q4.z * t;
           ph += ck * q4.x + sk * q4.y;
         } else if (MODE == FD_D) {
-          const double t = ck * q4.y - sk * q4.x;
+          const R t = ck * q4.y - sk * q4.x;
           at += t;
           az += q4.z * t;
         } else {
-          double t = 0.0;
+          R t = 0.0;
           if (has_q) {
             t = -(q * (ck * q4.y - sk * q4.x));
             ph += ck * q4.x + sk * q4.y;
@@ -1149,12 +1268,13 @@ __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps,
           at += t;
           az += q4.z * t;
         }
-        const double nc = ck * zstep.x - sk * zstep.y;
+        const R nc = ck * zstep.x - sk * zstep.y;
         sk = sk * zstep.x + ck * zstep.y;
         ck = nc;
         continue;
       }
       const double4 kk = kv[k];  // kx, ky, kz, 4 pi A_k / V
+      const R kw = kk.w, kz = kk.z;
       if (MODE == FD_Q) {
         const double re = sq[2 * k], im = sq[2 * k + 1];
         const double t = kk.w * (ck * im - sk * re);
@@ -1162,10 +1282,10 @@ __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps,
         az += kk.z * t;
         ph += kk.w * (ck * re + sk * im);
       } else if (MODE == FD_D) {
-        const double re = sm[2 * k], im = sm[2 * k + 1];
-        const double t = kk.w * (ck * im - sk * re);
+        const R re = sm[2 * k], im = sm[2 * k + 1];
+        const R t = kw * (ck * im - sk * re);
         at += t;
-        az += kk.z * t;
+        az += kz * t;
       } else {
         const double rqm = sm[2 * k], iqm = sm[2 * k + 1];
         double t = 0.0;
@@ -1183,13 +1303,13 @@ __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps,
         az += kk.z * t;
       }
       // advance the phase to the next kz of the column
-      const double nc = ck * zstep.x - sk * zstep.y;
+      const R nc = ck * zstep.x - sk * zstep.y;
       sk = sk * zstep.x + ck * zstep.y;
       ck = nc;
     }
-    e0 -= kx * at;
-    e1 -= ky * at;
-    e2 -= az;
+    e0 -= kx * (double)at;
+    e1 -= ky * (double)at;
+    e2 -= (double)az;
   }
   // combine the CG column groups of each site in a fixed order
   __syncthreads();
@@ -1233,9 +1353,11 @@ __global__ __launch_bounds__(64 * CG) void k_field(const Rep *__restrict__ reps,
 
 // Shared memory of k_field: the phase tables (reused for the CG-way
 // reduction; fd_tab_d2 double2 slots cover both), then the staged k data.
-static size_t field_lds(const SysConst &c, int mode, bool kl) {
+static size_t field_lds(const SysConst &c, int mode, bool kl, bool f32 = false) {
   return sizeof(double2) * c.fd_tab_d2 +
-         (kl ? c.nk * (sizeof(double4) + (mode == FD_QD ? sizeof(double2) : 0)) : 0);
+         (kl ? c.nk * ((f32 ? sizeof(float4) : sizeof(double4)) +
+                       (mode == FD_QD ? sizeof(double2) : 0))
+             : 0);
 }
 
 
@@ -2173,55 +2295,73 @@ void launch_tables(const Group &g, hipStream_t s) {
 // bits 5.. the lane count of a set in units of 64.
 static const int KZ_BUCKET[6] = {4, 8, 12, 16, 24, 32};
 
-template <int MODE, int KZ>
+template <int MODE, int KZ, class R>
 static void launch_sfac_col_v(bool split, int threads, dim3 grid, const BlockMap &m, size_t lds,
                               const Group &g, hipStream_t s) {
-  if (split) hipLaunchKernelGGL((k_sfac_col<MODE, KZ, 2>), grid, dim3(2 * threads), lds, s, g.d, m);
-  else hipLaunchKernelGGL((k_sfac_col<MODE, KZ>), grid, dim3(threads), lds, s, g.d, m);
+  if (split) hipLaunchKernelGGL((k_sfac_col<MODE, KZ, 2, R>), grid, dim3(2 * threads), lds, s, g.d, m);
+  else hipLaunchKernelGGL((k_sfac_col<MODE, KZ, 1, R>), grid, dim3(threads), lds, s, g.d, m);
 }
 
+template <int MODE, class R>
+static void launch_sfac_kz(int v, dim3 grid, const BlockMap &m, size_t lds, const Group &g,
+                           hipStream_t s) {
+  const bool split = (v & 16) != 0;
+  const int threads = ((v >> 5) & 7) * 64;
+  switch ((v >> 1) & 7) {
+    case 0: launch_sfac_col_v<MODE, 4, R>(split, threads, grid, m, lds, g, s); break;
+    case 1: launch_sfac_col_v<MODE, 8, R>(split, threads, grid, m, lds, g, s); break;
+    case 2: launch_sfac_col_v<MODE, 12, R>(split, threads, grid, m, lds, g, s); break;
+    case 3: launch_sfac_col_v<MODE, 16, R>(split, threads, grid, m, lds, g, s); break;
+    case 4: launch_sfac_col_v<MODE, 24, R>(split, threads, grid, m, lds, g, s); break;
+    default: launch_sfac_col_v<MODE, 32, R>(split, threads, grid, m, lds, g, s); break;
+  }
+}
+
+// refine: the FP64 dipole structure factor of the mixed form's refinement
+// step (mixed replicas only); otherwise the mixed form's SOR iterations take
+// it in FP32 (variant bit 8).
 template <int MODE>
-static void launch_sfac_mode(const Group &g, hipStream_t s) {
+static void launch_sfac_mode(const Group &g, bool refine, hipStream_t s) {
   constexpr int NV = MODE == SF_V ? 6 : 2;
-  launch_over(
-      g,
-      [](const SysConst &c, const MDConst &) {
+  auto plan = [&](const SysConst &c) {
         if (MODE != SF_Q && !is_ttm(c)) return Plan{-1, 0, 0, 0, 0};
+        if (refine && !c.mixed) return Plan{-1, 0, 0, 0, 0};
+        const bool f32 = MODE == SF_D && c.mixed && !refine;
         int nch, sc, scp;
         sfac_chunking(c, MODE, sfac_sites(c, MODE), nch, sc, scp);
-        const size_t lds = sfac_lds(c, MODE, sc, scp);
+        const size_t lds = sfac_lds(c, MODE, sc, scp, f32);
         int kzb = -1;
         for (int i = 0; i < 6; ++i)
           if (kzb < 0 && c.segmax <= KZ_BUCKET[i]) kzb = i;
         const bool col = c.sf_kind == 1 && (MODE != SF_V || c.sfv_col) && lds <= 64 * 1024 && kzb >= 0;
-        if (!col) return Plan{0, idiv(c.nk, 256), nch, c.P, lds};
+        if (!col) return Plan{0, idiv(c.nk, 256), nch, c.P, sfac_lds(c, MODE, sc, scp)};
         const int threads = std::min(256, idiv(c.nseg, 64) * 64);
         // two lane sets per block where the second set's sums fit beside the tables
-        const size_t red = sizeof(double) * ((size_t)NV * KZ_BUCKET[kzb] * threads + 2);
+        const size_t red = (f32 ? sizeof(float) : sizeof(double)) *
+                           ((size_t)NV * KZ_BUCKET[kzb] * threads + 4);
         const bool split = MODE != SF_V && c.sf_split && lds + red <= 96 * 1024;
-        return Plan{1 | (kzb << 1) | (split ? 16 : 0) | ((threads / 64) << 5), idiv(c.nseg, threads),
-                    nch, c.P, split ? lds + red : lds};
-      },
+        return Plan{1 | (kzb << 1) | (split ? 16 : 0) | ((threads / 64) << 5) | (f32 ? 256 : 0),
+                    idiv(c.nseg, threads), nch, c.P, split ? lds + red : lds};
+  };
+  launch_over(
+      g, [&](const SysConst &c, const MDConst &) { return plan(c); },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
         if (!(v & 1)) {
           hipLaunchKernelGGL((k_sfac_k<MODE>), grid, dim3(256), lds, s, g.d, m);
           return;
         }
-        const bool split = (v & 16) != 0;
-        const int threads = (v >> 5) * 64;
-        switch ((v >> 1) & 7) {
-          case 0: launch_sfac_col_v<MODE, 4>(split, threads, grid, m, lds, g, s); break;
-          case 1: launch_sfac_col_v<MODE, 8>(split, threads, grid, m, lds, g, s); break;
-          case 2: launch_sfac_col_v<MODE, 12>(split, threads, grid, m, lds, g, s); break;
-          case 3: launch_sfac_col_v<MODE, 16>(split, threads, grid, m, lds, g, s); break;
-          case 4: launch_sfac_col_v<MODE, 24>(split, threads, grid, m, lds, g, s); break;
-          default: launch_sfac_col_v<MODE, 32>(split, threads, grid, m, lds, g, s); break;
+        if constexpr (MODE == SF_D) {
+          if (v & 256) {
+            launch_sfac_kz<MODE, float>(v, grid, m, lds, g, s);
+            return;
+          }
         }
+        launch_sfac_kz<MODE, double>(v, grid, m, lds, g, s);
       });
   launch_over(
       g,
-      [](const SysConst &c, const MDConst &) {
-        if (MODE != SF_Q && !is_ttm(c)) return Plan{-1, 0, 0, 0, 0};
+      [&](const SysConst &c, const MDConst &) {
+        if (plan(c).variant < 0) return Plan{-1, 0, 0, 0, 0};
         return Plan{0, idiv((long)NV * c.nk, 256), c.P, 1, 0};
       },
       [&](int, dim3 grid, const BlockMap &m, size_t) {
@@ -2229,10 +2369,28 @@ static void launch_sfac_mode(const Group &g, hipStream_t s) {
       });
 }
 
-void launch_sfac(const Group &g, int mode, hipStream_t s) {
-  if (mode == SF_Q) launch_sfac_mode<SF_Q>(g, s);
-  else if (mode == SF_D) launch_sfac_mode<SF_D>(g, s);
-  else launch_sfac_mode<SF_V>(g, s);
+void launch_sfac(const Group &g, int mode, hipStream_t s, bool refine) {
+  if (mode == SF_Q) launch_sfac_mode<SF_Q>(g, false, s);
+  else if (mode == SF_D) launch_sfac_mode<SF_D>(g, refine, s);
+  else launch_sfac_mode<SF_V>(g, false, s);
+}
+
+// The FP32 dipole field of the mixed form's SOR iterations (variant bits as
+// launch_field_mode's).
+static void launch_field_f32(int v, bool kl, dim3 grid, const BlockMap &m, size_t lds, int iter,
+                             const Group &g, hipStream_t s) {
+#define PIMD_FIELD32(ML, CG)                                                                      \
+  if (kl) hipLaunchKernelGGL((k_field<FD_D, ML, CG, true, float>), grid, dim3(64 * CG), lds, s, g.d, m, iter); \
+  else hipLaunchKernelGGL((k_field<FD_D, ML, CG, false, float>), grid, dim3(64 * CG), lds, s, g.d, m, iter)
+  switch (v & 7) {
+    case 0: PIMD_FIELD32(2, 4); break;
+    case 1: PIMD_FIELD32(3, 4); break;
+    case 2: PIMD_FIELD32(2, 8); break;
+    case 3: PIMD_FIELD32(3, 8); break;
+    case 4: PIMD_FIELD32(2, 16); break;
+    default: PIMD_FIELD32(3, 16); break;
+  }
+#undef PIMD_FIELD32
 }
 
 // Reciprocal-space field `mode`.  Variant: bit 0 TTM3-F site sets (the
@@ -2253,12 +2411,20 @@ static void launch_field_mode(const Group &g, int iter, bool sel_mixed, hipStrea
         int cg = c.fd_cg;
         if (cg == 0) cg = c.ncol < 8 ? 4 : c.ncol < 16 ? 8 : 16;
         const int cgi = cg == 4 ? 0 : cg == 8 ? 1 : 2;
-        const bool kl = c.fd_klds && field_lds(c, MO, true) <= 160 * 1024;
-        return Plan{(ML == 3 ? 1 : 0) | (cgi << 1) | (kl ? 8 : 0), idiv(M, FD_SB), c.P, 1,
-                    field_lds(c, MO, kl)};
+        // the mixed form's SOR iterations in FP32 (its refinement, iter < 0, in FP64)
+        const bool f32 = MO == FD_D && c.mixed && iter > 0;
+        const bool kl = c.fd_klds && field_lds(c, MO, true, f32) <= 160 * 1024;
+        return Plan{(ML == 3 ? 1 : 0) | (cgi << 1) | (kl ? 8 : 0) | (f32 ? 16 : 0), idiv(M, FD_SB),
+                    c.P, 1, field_lds(c, MO, kl, f32)};
       },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
         const bool kl = (v & 8) != 0;
+        if constexpr (MO == FD_D) {
+          if (v & 16) {
+            launch_field_f32(v, kl, grid, m, lds, iter, g, s);
+            return;
+          }
+        }
 #define PIMD_FIELD(ML, CG)                                                                     \
   if (kl) hipLaunchKernelGGL((k_field<MO, ML, CG, true>), grid, dim3(64 * CG), lds, s, g.d, m, iter); \
   else hipLaunchKernelGGL((k_field<MO, ML, CG, false>), grid, dim3(64 * CG), lds, s, g.d, m, iter)

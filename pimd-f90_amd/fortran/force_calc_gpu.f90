@@ -70,17 +70,26 @@ contains
     Uind = e5(4)
   end subroutine potential
 
-  ! SIESTA helpers of the reference module (force_calc.f90:176-264) are not on
-  ! the TTM path.
+  ! The SIESTA monomer helper of the reference module (force_calc.f90:176-198)
+  ! is not on the TTM path.
   subroutine siesta_monomer(r1, dr1, e1)
     double precision, dimension(3, 3) :: r1, dr1
     double precision :: e1
     error stop 'force_calc (GPU): SIESTA monomer calculations are not supported'
   end subroutine siesta_monomer
 
+  ! force_calc.f90:205-264, every bead in one launch (contracted_MD and the
+  ! SIESTA branch call it after each step, MD.f90:229,407).
   subroutine calc_monomer_dip_moments(dip_momIt, RRt)
-    double precision, dimension(:, :, :) :: dip_momIt, RRt
-    error stop 'force_calc (GPU): SIESTA dipole moments are not supported'
+    use main_stuff, only: pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
+                          polar_sor, polar_eps, guess_initdip, massO, massH, CONTRACTION
+    double precision, dimension(:, :, :), intent(out) :: dip_momIt
+    double precision, dimension(:, :, :), intent(in) :: RRt
+    if (.not. gpu_ready() .or. (gpu_contraction() .neqv. CONTRACTION)) then
+      call gpu_init_pot(pot_model, Natoms, Nbeads, box, Rc, rc1, eps_ewald, polar_maxiter, &
+                        polar_sor, polar_eps, guess_initdip, massO, massH, contraction=CONTRACTION)
+    endif
+    call gpu_monomer_dipoles(RRt, dip_momIt)
   end subroutine calc_monomer_dip_moments
 
 end module force_calc

@@ -18,7 +18,7 @@ module pimd_gpu
   use pimd_ttm_c
   implicit none
   private
-  public :: gpu_init_pot, gpu_full_bead_forces, gpu_potential, gpu_energy_terms, &
+  public :: gpu_init_pot, gpu_full_bead_forces, gpu_potential, gpu_energy_terms, gpu_monomer_dipoles, &
             gpu_init_normal_modes, gpu_evolve_rings, gpu_init_langevin_nm, gpu_langevin_nm, &
             gpu_finalize, gpu_ready, gpu_nm_data, gpu_contraction
 
@@ -119,6 +119,14 @@ contains
     double precision, intent(out) :: terms(5, g_nbeads)
     call check(pimd_energy_terms(ctx_all, terms), 'pimd_energy_terms')
   end subroutine gpu_energy_terms
+
+  ! Monomer dipole moments of every bead (force_calc.f90:205-264).
+  subroutine gpu_monomer_dipoles(RRt, dip_momIt)
+    double precision, intent(in) :: RRt(3, g_natoms, g_nbeads)
+    double precision, intent(out) :: dip_momIt(3, g_nwaters, g_nbeads)
+    if (.not. c_associated(ctx_all)) error stop 'pimd_gpu: gpu_init_pot has not been called'
+    call check(pimd_monomer_dipoles(ctx_all, RRt, dip_momIt), 'pimd_monomer_dipoles')
+  end subroutine gpu_monomer_dipoles
 
   ! One bead with the argument list of pot_ttm (pot_ttm.f90:3-15).
   ! terms (optional): Umon, Uvdw, Uelec, Uind, Uvdw_lrc of this call.

@@ -46,7 +46,7 @@ module pimd_ttm_c
   end type pimd_md_params
 
   public :: pimd_ctx_create, pimd_ctx_destroy, pimd_last_error, pimd_ctx_info, &
-            pimd_forces_all_beads, pimd_energy_terms, pimd_nm_init, pimd_nm_get, &
+            pimd_forces_all_beads, pimd_energy_terms, pimd_monomer_dipoles, pimd_nm_init, pimd_nm_get, &
             pimd_nm_evolve, pimd_langevin_init, pimd_langevin, pimd_nose_hoover, &
             pimd_md_init, pimd_state_set, pimd_state_get, pimd_md_run, pimd_sync, &
             pimd_md_observables, pimd_error_message
@@ -84,6 +84,12 @@ module pimd_ttm_c
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: terms(*)
+    end function
+    integer(c_int) function pimd_monomer_dipoles(ctx, RRt, dip) bind(C, name="pimd_monomer_dipoles")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: RRt(*)
+      real(c_double), intent(out) :: dip(*)
     end function
     integer(c_int) function pimd_nm_init(ctx, omegan, delta, setnm) bind(C, name="pimd_nm_init")
       import :: c_int, c_ptr, c_double

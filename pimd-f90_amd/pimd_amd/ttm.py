@@ -37,7 +37,7 @@ EXPORTS = (
     "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
     "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy", "pimd_ttm_params_get",
     "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
-    "pimd_forces_all_beads_noalloc",
+    "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
         "pimd_forces_all_beads": [vp, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp, dp, ip],
         "pimd_forces_all_beads_dev": [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp],
         "pimd_energy_terms": [vp, dp],
+        "pimd_monomer_dipoles": [vp, dp, dp],
         "pimd_nm_init": [vp, C.c_double, C.c_double, C.c_double],
         "pimd_nm_get": [vp, dp, dp, dp, dp, dp, dp, dp],
         "pimd_nm_evolve": [vp, dp, dp, dp],
@@ -242,6 +243,15 @@ class TTMForceField:
         t = np.zeros((self.params.nbeads, 5))
         _check(load_library().pimd_energy_terms(self._h, _ptr(t)))
         return t
+
+    def monomer_dipoles(self, RRt) -> np.ndarray:
+        """[P][N][3] monomer dipole moments of every bead from the dipole-moment-
+        surface charges (calc_monomer_dip_moments, force_calc.f90:205-264)."""
+        p = self.params
+        R = _arr(RRt, (p.nbeads, 3 * p.nwaters, 3))
+        out = np.zeros((p.nbeads, p.nwaters, 3))
+        _check(load_library().pimd_monomer_dipoles(self._h, _ptr(R), _ptr(out)))
+        return out
 
     def pes_params(self) -> np.ndarray:
         """The monomer PES parameters in use (combined c5z, deohA, alphaoh, roh, phh1A, phh2)."""

@@ -73,3 +73,45 @@ def test_dropin_program_matches_reference_run(exe, ptype, tmp_path):
     d = float(np.max(np.abs(mine - theirs)))
     print(exe, ptype, "final image: largest difference", d, flush=True)
     assert d <= 1e-6 * (1 + 1e-6)      # one unit of f12.6
+
+
+@pytest.mark.parametrize("model", [2, 3, 4])
+def test_monomer_dipoles_vs_oracle_charges(model):
+    """pimd_monomer_dipoles (force_calc.f90:205-264) on every bead against the
+    oracle's dipole-moment-surface charges (pinned by the reference's monomer
+    fixtures) put through the formula of force_calc.f90:221-255: the M-site
+    share folded into the H charges, H positions relative to the M site for
+    TTM3-F and q-SPC/Fw (gamma_M 0.46 and 0), to the O for TTM2.1-F."""
+    import oracle
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import CHARGECON, make_system
+    from pimd_amd.ttm import TTMForceField
+    N, P = 16, 3
+    RRt, _, RRc, L = make_system(N, P, seed=31)
+    p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L)).resolved()
+    got = TTMForceField(p).monomer_dipoles(RRt)
+    gM = {2: 0.426706882, 3: 0.46, 4: 0.0}[model]
+    dms = {2: (0.0, 0.0, 0.0), 3: (0.5, 0.9578, 0.012), 4: (0.0, 0.0, 0.0)}[model]
+    box = np.array([L, L, L])
+    want = np.zeros((P, N, 3))
+    tq = 0.5 * gM / (1.0 - gM)
+    for b in range(P):
+        for w in range(N):
+            r = RRt[b, 3 * w:3 * w + 3]
+            q = oracle.monomer(r, box, dms)[2] * CHARGECON
+            qh1, qh2 = q[1] + tq * (q[1] + q[2]), q[2] + tq * (q[1] + q[2])
+            o1 = r[1] - r[0]
+            o1 -= box * np.rint(o1 / box)
+            o2 = r[2] - r[0]
+            o2 -= box * np.rint(o2 / box)
+            h1, h2 = o1, o2
+            if model in (3, 4):
+                m = 0.5 * gM * (o1 + o2) + r[0]
+                h1 = r[1] - m
+                h1 -= box * np.rint(h1 / box)
+                h2 = r[2] - m
+                h2 -= box * np.rint(h2 / box)
+            want[b, w] = qh1 * h1 + qh2 * h2
+    err = float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    print("model", model, "monomer dipoles rel err", err, flush=True)
+    assert err < 1e-12

@@ -112,7 +112,7 @@ def test_trajectory_vs_reference(name):
     print(name, "observables rel err", np.max(oerr), flush=True)
     assert max(oerr[:5]) < 1e-10 and max(oerr[5:]) < 1e-8
     if "RRc_end" in g:
-        assert max_abs(st["RRc"], g["RRc_end"]) < 1e-12
+        assert rel_err(st["RRc"], g["RRc_end"]) < 1e-12
 
 
 @pytest.mark.parametrize("name", golden_names("contracted_"))
