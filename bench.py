@@ -155,7 +155,9 @@ def cpu_baseline(p, RRt, RRc, budget_s=15.0):
 
 
 PEAK_FP32_TFLOPS = 157.3
-FP32_KERNELS = ("pair_qq", "pair_dd", "pair_final")  # FP32 arithmetic in the mixed form
+# FP32 arithmetic in the mixed form: the real-space pair passes and the SOR
+# iterations' reciprocal dipole pass
+FP32_KERNELS = ("pair_qq", "pair_dd", "pair_final", "sfac_d", "field_d_sor")
 
 
 def run_form(p, RRt, PPt, RRc, args):

@@ -1567,7 +1567,13 @@ struct pimd_batch {
   hipStream_t stream = nullptr;
   Rep *table = nullptr;  // device replica records of every group, group by group
   int table_cap = 0;
-  Group vv;
+  // plain PIMD members, one group per force class (model and arithmetic)
+  std::vector<Group> vv;
+  // one stream per group beyond the first, so that the groups' kernel chains
+  // run side by side (parallel branches of the step's graph)
+  std::vector<hipStream_t> lanes;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
   struct Con {
     int intra;
     Group outer, cen;
@@ -1606,10 +1612,50 @@ static int batch_tables(pimd_batch *b) {
     const int flags[5] = {h.m.bead_tt, h.m.centroid, h.m.global_nh, h.m.cmd, x->md.intra_timesteps};
     add_sig(flags, sizeof flags);
   };
-  b->vv = Group{};
+  b->vv.clear();
   b->con.clear();
-  for (pimd_ctx *x : b->ctx)
-    if (x->md.intra_timesteps == 0) push(b->vv, x);
+  std::vector<std::pair<int, int>> keys;
+  for (pimd_ctx *x : b->ctx) {
+    if (x->md.intra_timesteps != 0) continue;
+    const std::pair<int, int> k{x->c.model, x->c.mixed};
+    if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
+  }
+  // members per lane: a force class; while there are fewer lanes than
+  // PIMD_BATCH_LANES (default 4, the hardware queues of a process), the
+  // costliest class is dealt into two (members sorted by molecule-beads,
+  // dealt in turn).  64 uneven replicas, FP64: 106.3 ms per step on two
+  // lanes, 102.5 on four.
+  std::vector<std::vector<pimd_ctx *>> lanes;
+  for (const auto &k : keys) {
+    lanes.emplace_back();
+    for (pimd_ctx *x : b->ctx)
+      if (x->md.intra_timesteps == 0 && x->c.model == k.first && x->c.mixed == k.second)
+        lanes.back().push_back(x);
+  }
+  const int want = env_int("PIMD_BATCH_LANES", 4, 0, 8);
+  auto cost = [](const std::vector<pimd_ctx *> &v) {
+    long w = 0;
+    for (const pimd_ctx *x : v) w += (long)x->c.P * x->c.N;
+    return w;
+  };
+  while ((int)lanes.size() < want) {
+    size_t big = 0;
+    for (size_t i = 1; i < lanes.size(); ++i)
+      if (cost(lanes[i]) > cost(lanes[big])) big = i;
+    if (lanes[big].size() < 2) break;
+    std::vector<pimd_ctx *> v = lanes[big];
+    std::stable_sort(v.begin(), v.end(), [](const pimd_ctx *a, const pimd_ctx *c2) {
+      return (long)a->c.P * a->c.N > (long)c2->c.P * c2->c.N;
+    });
+    std::vector<pimd_ctx *> h0, h1;
+    for (size_t i = 0; i < v.size(); ++i) ((i % 2 == 0) ? h0 : h1).push_back(v[i]);
+    lanes[big] = h0;
+    lanes.push_back(h1);
+  }
+  for (const auto &l : lanes) {
+    b->vv.emplace_back();
+    for (pimd_ctx *x : l) push(b->vv.back(), x);
+  }
   std::vector<int> intras;
   for (pimd_ctx *x : b->ctx)
     if (x->md.intra_timesteps > 0 &&
@@ -1635,8 +1681,10 @@ static int batch_tables(pimd_batch *b) {
     b->graph_ok = false;
   }
   int off = 0;
-  b->vv.d = b->table + off;
-  off += b->vv.n();
+  for (Group &g : b->vv) {
+    g.d = b->table + off;
+    off += g.n();
+  }
   for (auto &cn : b->con) {
     cn.outer.d = b->table + off;
     off += cn.outer.n();
@@ -1645,6 +1693,20 @@ static int batch_tables(pimd_batch *b) {
   }
   HIPCHK(hipMemcpyAsync(b->table, host.data(), sizeof(Rep) * n, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  // streams and events of the side lanes (default priority)
+  const size_t nl = b->vv.size() + b->con.size();
+  if (!b->fork && nl > 1 && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    return fail(PIMD_EDEVICE, "hipEventCreate failed");
+  while (nl > 1 && b->lanes.size() < nl - 1) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+      return fail(PIMD_EDEVICE, "hipStreamCreate failed");
+    b->lanes.push_back(st);
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+      return fail(PIMD_EDEVICE, "hipEventCreate failed");
+    b->joins.push_back(ev);
+  }
   if (sig != b->sig) {
     b->sig.swap(sig);
     b->graph_ok = false;
@@ -1652,13 +1714,30 @@ static int batch_tables(pimd_batch *b) {
   return PIMD_OK;
 }
 
+// One MD step of every member: each group (a force class of plain PIMD
+// members, or the contracted members of one inner-step count) on a lane of
+// its own, the first on the batch's stream, the others forked from it and
+// joined back.
 static int batch_step(pimd_batch *b) {
   auto no_noise = [](int) { return false; };
+  const size_t nl = b->vv.size() + b->con.size();
+  if (nl > 1) {
+    HIPCHK(hipEventRecord(b->fork, b->stream));
+    for (size_t i = 0; i + 1 < nl; ++i) HIPCHK(hipStreamWaitEvent(b->lanes[i], b->fork, 0));
+  }
   int rc = PIMD_OK;
-  if (b->vv.n() && (rc = step_vv(b->vv, b->stream, nullptr, no_noise)) != PIMD_OK) return rc;
+  size_t lane = 0;
+  auto stream_of = [&](size_t i) { return i == 0 ? b->stream : b->lanes[i - 1]; };
+  for (Group &g : b->vv)
+    if ((rc = step_vv(g, stream_of(lane++), nullptr, no_noise)) != PIMD_OK) return rc;
   for (auto &cn : b->con)
-    if ((rc = step_contracted(cn.outer, cn.cen, cn.intra, b->stream, nullptr, no_noise)) != PIMD_OK)
+    if ((rc = step_contracted(cn.outer, cn.cen, cn.intra, stream_of(lane++), nullptr, no_noise)) !=
+        PIMD_OK)
       return rc;
+  for (size_t i = 0; i + 1 < nl; ++i) {
+    HIPCHK(hipEventRecord(b->joins[i], b->lanes[i]));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->joins[i], 0));
+  }
   return PIMD_OK;
 }
 
@@ -1766,7 +1845,8 @@ extern "C" int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out)
     delete b;
     return fail(PIMD_EDEVICE, "hipStreamCreate failed");
   }
-  int rc = unify_shapes(b);
+  // PIMD_BATCH_UNIFY=0 keeps every member's own launch shapes
+  int rc = env_int("PIMD_BATCH_UNIFY", 1, 0, 1) ? unify_shapes(b) : PIMD_OK;
   if (rc != PIMD_OK) {
     (void)pimd_batch_destroy(b);
     return rc;
@@ -1778,6 +1858,10 @@ extern "C" int pimd_batch_create(pimd_ctx *const *ctxs, int n, pimd_batch **out)
 extern "C" int pimd_batch_destroy(pimd_batch *b) {
   if (!b) return PIMD_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (hipStream_t st : b->lanes) (void)hipStreamSynchronize(st);
+  for (hipStream_t st : b->lanes) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : b->joins) (void)hipEventDestroy(ev);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   if (b->table) (void)hipFree(b->table);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1930,7 +2014,7 @@ extern "C" int pimd_batch_md_run(pimd_batch *b, int t0, int nsteps, const uint64
   if ((rc = batch_tables(b)) != PIMD_OK) return rc;
   // the clocks, and the kinetic energies of the loaded states (needed by the
   // first thermostat call)
-  for (Group *g : {&b->vv}) {
+  for (Group *g = b->vv.data(); g != b->vv.data() + b->vv.size(); ++g) {
     if (!g->n()) continue;
     launch_set_clock(*g, t0, 0, b->stream);
     launch_kinetic(*g, b->stream);

@@ -196,25 +196,30 @@ struct Blk {
   int nx, ny;                 // its own grid's x and y extents
 };
 
-// The hardware hands workgroup w of a launch to XCD w % 8; the logical
-// block index is renumbered so that consecutive logical blocks (the blocks of
-// one bead, which stage the same bead records, tables and k data) run on one
-// XCD and share its L2: XCD x takes the x-th contiguous eighth of the grid.
+// The hardware hands workgroup w of a launch to XCD w % 8.  Within each
+// replica's range of the grid the block index is renumbered so that
+// consecutive blocks of the replica (the blocks of one bead, which stage the
+// same bead records, tables and k data) run on one XCD and share its L2: of
+// the replica's n blocks, those on the x-th XCD class take the x-th
+// contiguous eighth.  A replica's blocks still spread over all eight XCDs,
+// so replicas of different cost do not pile onto one of them.
 constexpr int NXCD = 8;
-__device__ __forceinline__ int xcd_block(int w, int n) {
-  const int q = n / NXCD, r = n % NXCD, x = w % NXCD;
-  return x * q + min(x, r) + w / NXCD;
+__device__ __forceinline__ int xcd_block(int p, int n) {
+  const int q = n / NXCD, r = n % NXCD, x = p % NXCD;
+  return x * q + min(x, r) + p / NXCD;
 }
 
 __device__ __forceinline__ Blk block_of(const BlockMap &m) {
-  const int g = m.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int g = blockIdx.x;
   int lo = 0, hi = m.n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (m.off[mid] <= g) lo = mid;
     else hi = mid - 1;
   }
-  const int l = g - m.off[lo], nx = m.nx[lo], ny = m.ny[lo];
+  const int n = m.off[lo + 1] - m.off[lo];
+  const int l = m.xcd ? xcd_block(g - m.off[lo], n) : g - m.off[lo];
+  const int nx = m.nx[lo], ny = m.ny[lo];
   return Blk{m.rep[lo], l % nx, (l / nx) % ny, l / (nx * ny), nx, ny};
 }
 

@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "ttm_device.h"
+#include <cstdlib>
 
 namespace pimd {
 
@@ -1674,7 +1675,19 @@ static size_t dd_lds(const SysConst &c) { return sizeof(double) * DD_REC * c.N *
 
 
 
-bool pair_dd_tests(const SysConst &c) { return c.g_dd > 0 && dd_lds(c) <= 64 * 1024; }
+// The LDS-staged dipole pass holds a whole bead's dipole sites: up to
+// PIMD_DD_LDS_KB (default 64) KiB of LDS per workgroup, the gather kernel
+// beyond.
+static size_t dd_lds_max() {
+  static const size_t v = [] {
+    const char *e = std::getenv("PIMD_DD_LDS_KB");
+    long kb = e ? std::strtol(e, nullptr, 10) : 64;
+    if (kb < 16 || kb > 150) kb = 64;
+    return (size_t)kb * 1024;
+  }();
+  return v;
+}
+bool pair_dd_tests(const SysConst &c) { return c.g_dd > 0 && dd_lds(c) <= dd_lds_max(); }
 
 
 
