@@ -549,6 +549,9 @@ def run_fit(args):
                      ("pes_adjoint", lambda grp: grp.ff.pes_param_grad(
                          grp.RRt, np.ones(grp.RRt.shape[0]), np.ones_like(grp.RRt))),
                      ("ttm_tangent", lambda grp: grp.ff.ttm_param_grad(
+                         grp.RRt, np.ones(grp.RRt.shape[0]), np.ones_like(grp.RRt))),
+                     # not part of the step: the 84 DMS coefficients' tangents
+                     ("dms_tangent_84", lambda grp: grp.ff.dms_coef_grad(
                          grp.RRt, np.ones(grp.RRt.shape[0]), np.ones_like(grp.RRt)))):
         t1 = time.perf_counter()
         for grp in groups:

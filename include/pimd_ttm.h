@@ -282,6 +282,18 @@ int pimd_ttm_params_set(pimd_ctx *ctx, const double *theta);
 int pimd_ttm_param_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
                         double *grad, double *E, double *dEdtheta);
 
+/* The 84 coefficients coefD of the dipole moment surface that charges the
+ * sites (nasa_mod.f90 values; dms_nasa, nasa.f90:185-242).  No counterpart
+ * in the reference, where they are constants.  A context starts with the
+ * reference's values; later force calls use the ones set. */
+#define PIMD_DMS_NCOEF 84
+int pimd_dms_coefs_get(pimd_ctx *ctx, double *coef);
+int pimd_dms_coefs_set(pimd_ctx *ctx, const double *coef);
+/* pimd_ttm_param_grad with respect to the 84 DMS coefficients: grad[84],
+ * and optionally E[P] and dEdc[P][84]. */
+int pimd_dms_coef_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
+                       double *grad, double *E, double *dEdc);
+
 /* ---- instrumentation --------------------------------------------------- */
 
 /* Enables per-kernel HIP-event timing on the context stream. */

@@ -577,7 +577,7 @@ void layout_system(pimd_ctx *x, Arena &a, bool full) {
     a.take(&d.cell_start, P * ((size_t)c.ncell[0] * c.ncell[1] * c.ncell[2] + 1));
     a.take(&d.cell_mol, P * N);
   }
-  a.take(&d.pes, PES_NPARAM);
+  a.take(&d.pes, PES_NALL);
   a.take(&d.clk_own, 2);
   a.take(&d.flags, 1);
   a.take(&x->rep_dev, 1);
@@ -606,7 +606,7 @@ int upload_system(pimd_ctx *x, const HostTables &t, hipStream_t s) {
   const SysConst &c = x->c;
   DevBufs &d = x->d;
   const size_t nk = c.nk;
-  double theta[PES_NPARAM];
+  double theta[PES_NALL];
   pes_default_params(theta);
   HIPCHK(hipMemcpyAsync(d.pes, theta, sizeof theta, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.kv, t.kv.data(), sizeof(double4) * nk, hipMemcpyHostToDevice, s));
@@ -2104,8 +2104,11 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
   return PIMD_OK;
 }
 
-extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double *ew,
-                                   const double *gw, double *grad, double *E, double *dEdtheta) {
+// Tangent sweeps along ndir directions (the TTM parameters, or with dms_coef
+// the DMS coefficients): the loss gradient, and optionally each
+// configuration's energy and its derivatives.
+static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
+                        int dms_coef, int ndir, double *grad, double *E, double *dEdtheta) {
   if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
   if (x->p.contraction)
     return fail(PIMD_EINVAL, "parameter derivatives need a context with contraction = 0");
@@ -2115,8 +2118,8 @@ extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double 
   const SysConst &c = x->c;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
-  const size_t P = (size_t)c.P, n = P * x->natoms * 3, nout = P * TTM_NPARAM * 4;
-  const size_t ws = tangent_ws_duals(c) * P * TTM_NPARAM * 2, cfg = tangent_cfg_doubles(c) * P;
+  const size_t P = (size_t)c.P, n = P * x->natoms * 3, nout = P * ndir * 4;
+  const size_t ws = tangent_ws_duals(c) * P * ndir * 2, cfg = tangent_cfg_doubles(c) * P;
   const size_t bytes = sizeof(double) * (ws + cfg + 2 * n + P + nout);
   if (bytes > ((size_t)32 << 30))
     return fail(PIMD_EINVAL, "parameter derivatives of %zu configurations need %.1f GB of workspace; "
@@ -2141,6 +2144,8 @@ extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double 
   L.kv = d.kv;
   L.khkl = d.khkl;
   L.dG = nullptr;
+  L.dms_coef = dms_coef;
+  L.ndir = ndir;
   std::copy(x->ttm, x->ttm + TTM_NPARAM, L.theta);
   const double Rc = x->p.rc, pre = TWOPI * (double)(c.N * c.N) / c.vol;
   L.lrc_coef[0] = pre / (9.0 * std::pow(Rc, 9));
@@ -2149,21 +2154,53 @@ extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double 
   HIPCHK(hipMemcpyAsync(dRR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dgw, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dew, ew, sizeof(double) * P, hipMemcpyHostToDevice, s));
-  x->timed("ttm_tangent", [&] { launch_ttm_tangent(c, L, s); });
+  x->timed(dms_coef ? "dms_tangent" : "ttm_tangent", [&] { launch_ttm_tangent(c, L, s); });
   HIPCHK(hipGetLastError());
   std::vector<double> out(nout);
   HIPCHK(hipMemcpyAsync(out.data(), L.out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (int j = 0; j < TTM_NPARAM; ++j) {
+  for (int j = 0; j < ndir; ++j) {
     double g = 0.0;
-    for (size_t b = 0; b < P; ++b) g += out[(b * TTM_NPARAM + j) * 4 + 2];
+    for (size_t b = 0; b < P; ++b) g += out[(b * ndir + j) * 4 + 2];
     grad[j] = g;
   }
   for (size_t b = 0; b < P; ++b) {
-    if (E) E[b] = out[(b * TTM_NPARAM) * 4];
+    if (E) E[b] = out[(b * ndir) * 4];
     if (dEdtheta)
-      for (int j = 0; j < TTM_NPARAM; ++j) dEdtheta[b * TTM_NPARAM + j] = out[(b * TTM_NPARAM + j) * 4 + 1];
+      for (int j = 0; j < ndir; ++j) dEdtheta[b * ndir + j] = out[(b * ndir + j) * 4 + 1];
   }
+  return PIMD_OK;
+}
+
+extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double *ew,
+                                   const double *gw, double *grad, double *E, double *dEdtheta) {
+  return tangent_grad(x, RRt, ew, gw, 0, TTM_NPARAM, grad, E, dEdtheta);
+}
+
+extern "C" int pimd_dms_coef_grad(pimd_ctx *x, const double *RRt, const double *ew,
+                                  const double *gw, double *grad, double *E, double *dEdc) {
+  return tangent_grad(x, RRt, ew, gw, 1, DMS_NCOEF, grad, E, dEdc);
+}
+
+extern "C" int pimd_dms_coefs_get(pimd_ctx *x, double *coef) {
+  if (!x || !coef) return fail(PIMD_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(coef, x->d.pes + PES_DMSC, sizeof(double) * DMS_NCOEF, hipMemcpyDeviceToHost));
+  return PIMD_OK;
+}
+
+extern "C" int pimd_dms_coefs_set(pimd_ctx *x, const double *coef) {
+  if (!x || !coef) return fail(PIMD_EINVAL, "null argument");
+  for (int j = 0; j < DMS_NCOEF; ++j)
+    if (!std::isfinite(coef[j])) return fail(PIMD_EINVAL, "DMS coefficient %d is not finite", j);
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(x->d.pes + PES_DMSC, coef, sizeof(double) * DMS_NCOEF, hipMemcpyHostToDevice));
+  // the centroid context of contracted MD charges its sites with them too
+  if (x->cen)
+    HIPCHK(hipMemcpy(x->cen->d.pes + PES_DMSC, coef, sizeof(double) * DMS_NCOEF,
+                     hipMemcpyHostToDevice));
   return PIMD_OK;
 }
 

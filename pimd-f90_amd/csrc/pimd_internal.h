@@ -32,6 +32,11 @@ constexpr double MASSCONI = 1.0 / 418.4;       // consts.f90:25
 constexpr int PES_NC = 245;
 constexpr int PES_DEOHA = 245, PES_ALPHAOH = 246, PES_ROH = 247, PES_PHH1A = 248, PES_PHH2 = 249;
 constexpr int PES_NPARAM = 250;
+// The 84 coefficients of the dipole moment surface (coefD, nasa_mod.f90)
+// follow the PES parameters in a context's parameter array.
+constexpr int DMS_NCOEF = 84;
+constexpr int PES_DMSC = PES_NPARAM;
+constexpr int PES_NALL = PES_NPARAM + DMS_NCOEF;
 
 // One site of a molecule record: position and charge (charge in
 // CHARGECON-scaled units, as the reference's charge array).
@@ -132,7 +137,7 @@ struct DevBufs {
   double *uk = nullptr;                         // [1] kinetic energy
   double *ukpart = nullptr;                     // [256] kinetic partial sums
   double *radius = nullptr;                     // [P][N] site radius about O
-  double *pes = nullptr;                        // [PES_NPARAM] PES parameters
+  double *pes = nullptr;                        // [PES_NALL] PES parameters, then coefD
   double *adj = nullptr;                        // adjoint inputs and partial sums
   double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
   int *clk = nullptr;                           // device step clock {t, record slot}
@@ -322,6 +327,7 @@ void launch_monomer_dipoles(const SysConst &c, const double *pes, const double *
 void launch_broadcast_beads(const Group &g, hipStream_t s);
 void launch_contract_record(const Group &g, hipStream_t s);
 
+// theta[PES_NALL]: the reference's PES parameters and DMS coefficients
 void pes_default_params(double *theta);
 void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
                         const double *gw, double *partial, int nblocks, double *grad,
@@ -347,6 +353,8 @@ struct TanLaunch {
   double *out, *dG;
   double theta[TTM_NPARAM];
   double lrc_coef[3];
+  int dms_coef;  // 0: the TTM parameters (ndir = TTM_NPARAM), 1: the DMS coefficients
+  int ndir;
 };
 size_t tangent_cfg_doubles(const SysConst &c);
 size_t tangent_ws_duals(const SysConst &c);

@@ -20,13 +20,12 @@ __device__ __forceinline__ void flag_nonfinite(int *flags, const double (&r)[3][
 }
 
 
-// Exponent tables of the PES and DMS bases and the DMS coefficients, built
-// at compile time into the constant memory of each unit that uses them (the
-// PES coefficients are per context).
+// Exponent tables of the PES and DMS bases, built at compile time into the
+// constant memory of each unit that uses them (the PES and DMS coefficients
+// are per context, in its parameter array).
 struct PsTables {
   int idx[PS_NTERM][3];
   int idxd[PS_NDMS][3];
-  double coefd[PS_NDMS];
 };
 constexpr PsTables make_ps_tables() {
   PsTables t{};
@@ -39,7 +38,6 @@ constexpr PsTables make_ps_tables() {
     t.idxd[j][0] = PS_IDXD1[j];
     t.idxd[j][1] = PS_IDXD2[j];
     t.idxd[j][2] = PS_IDXD3[j];
-    t.coefd[j] = PS_COEFD[j];
   }
   return t;
 }
@@ -424,7 +422,7 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   for (int j = 1; j < PS_NDMS; ++j) {
     if (L > 1 && (j - 1) % L != part) continue;
     int I = c_ps.idxd[j][0], J = c_ps.idxd[j][1], K = c_ps.idxd[j][2];
-    double cj = c_ps.coefd[j];
+    double cj = pes[PES_DMSC + j];
     P1 = P1 + cj * f1[I] * f2[J] * f3[K];
     P2 = P2 + cj * f1[J] * f2[I] * f3[K];
     p1r1 = p1r1 + cj * ((double)(I - 1) * f1[I - 1] * f2[J] * f3[K]);
@@ -464,8 +462,8 @@ __device__ inline void ps_monomer(const SysConst &c, const double r[3][3], doubl
   p2r1 = p2r1 * efd + P2 * defdr1 + dpc0dr1;
   p2r2 = p2r2 * efd + P2 * defdr2 + dpc0dr2;
   p2c = p2c * efd + dpc0dc;
-  P1 = c_ps.coefd[0] + P1 * efd + PC0 * B2A;
-  P2 = c_ps.coefd[0] + P2 * efd + PC0 * B2A;
+  P1 = pes[PES_DMSC] + P1 * efd + PC0 * B2A;
+  P2 = pes[PES_DMSC] + P2 * efd + PC0 * B2A;
   q3[0] = -(P1 + P2);
   q3[1] = P1;
   q3[2] = P2;

@@ -23,7 +23,8 @@
 namespace pimd {
 
 // c5z = f5z*c5zA + fbasis*cbasis + fcore*ccore + frest*crest (nasa.f90:45).
-// The reference's PES parameters in the order of the pes arrays.
+// The reference's PES parameters in the order of the pes arrays, then its
+// DMS coefficients coefD (nasa_mod.f90).
 void pes_default_params(double *theta) {
   for (int j = 0; j < PS_NTERM; ++j)
     theta[j] = PS_F5Z * PS_C5ZA[j] + PS_FBASIS * PS_CBASIS[j] + PS_FCORE * PS_CCORE[j] +
@@ -33,6 +34,7 @@ void pes_default_params(double *theta) {
   theta[PES_ROH] = PS_ROH;
   theta[PES_PHH1A] = PS_PHH1A;
   theta[PES_PHH2] = PS_PHH2;
+  for (int j = 0; j < DMS_NCOEF; ++j) theta[PES_DMSC + j] = PS_COEFD[j];
 }
 
 

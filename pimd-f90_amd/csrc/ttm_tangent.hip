@@ -110,6 +110,7 @@ struct TanArgs {
   double *dG;            // NULL or [P][dirs][Natoms][3] dG/dtheta
   double theta[TTM_NPARAM];
   double lrc_coef[3];    // dUvdw_lrc / d(vdwA, vdwB, vdwC)
+  int dms_coef;          // 1: the directions are the DMS coefficients coefD
 };
 
 // The derivative of the DMS charges and their gradients with respect to
@@ -153,6 +154,81 @@ __device__ inline void dms_tangent(const SysConst &c, const double r[3][3], int 
   const double f1q1r13 = (p1r1 - (p1c * costh / d1)) / d1, f1q1r23 = p1c / (d1 * d2);
   const double f2q1r23 = -(p1c * costh / d2) / d2, f2q1r13 = p1c / (d2 * d1);
   const double f1q2r13 = -(p2c * costh / d1) / d1, f1q2r23 = p2c / (d1 * d2);
+  const double f2q2r23 = (p2r2 - (p2c * costh / d2)) / d2, f2q2r13 = p2c / (d2 * d1);
+  for (int k = 0; k < 3; ++k) {
+    dgq[0][0][k] = f1q1r13 * R1[k] + f1q1r23 * R2[k];
+    dgq[1][0][k] = f2q1r13 * R1[k] + f2q1r23 * R2[k];
+    dgq[2][0][k] = -(dgq[0][0][k] + dgq[1][0][k]);
+    dgq[0][1][k] = f1q2r13 * R1[k] + f1q2r23 * R2[k];
+    dgq[1][1][k] = f2q2r13 * R1[k] + f2q2r23 * R2[k];
+    dgq[2][1][k] = -(dgq[0][1][k] + dgq[1][1][k]);
+    dgq[0][2][k] = -(dgq[0][0][k] + dgq[0][1][k]);
+    dgq[1][2][k] = -(dgq[1][0][k] + dgq[1][1][k]);
+    dgq[2][2][k] = -(dgq[2][0][k] + dgq[2][1][k]);
+  }
+}
+
+// The derivative of the DMS charges and their gradients with respect to the
+// DMS coefficient coefD(j+1) (nasa.f90:185-242): the charges are linear in
+// the coefficients, so this is the surface of ps_monomer with that one
+// coefficient set to 1 and every other term (the others, the constant part
+// PC0 and the dms_param corrections) left out.  Layout as ps_monomer's.
+__device__ inline void dms_coef_tangent(const SysConst &c, const double r[3][3], int j,
+                                        double dq3[3], double dgq[3][3][3]) {
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 3; ++q)
+      for (int k = 0; k < 3; ++k) dgq[a][q][k] = 0.0;
+  if (j == 0) {  // the constant term of both P1 and P2
+    dq3[0] = -2.0;
+    dq3[1] = 1.0;
+    dq3[2] = 1.0;
+    return;
+  }
+  double R1[3], R2[3];
+  for (int k = 0; k < 3; ++k) {
+    R1[k] = r[1][k] - r[0][k];
+    R2[k] = r[2][k] - r[0][k];
+  }
+  minimg(c, R1[0], R1[1], R1[2]);
+  minimg(c, R2[0], R2[1], R2[2]);
+  const double d1 = sqrt(R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2]);
+  const double d2 = sqrt(R2[0] * R2[0] + R2[1] * R2[1] + R2[2] * R2[2]);
+  const double costh = (R1[0] * R2[0] + R1[1] * R2[1] + R1[2] * R2[2]) / (d1 * d2);
+  const double costhe = -.24780227221366464506;
+  const double x[3] = {(d1 - PS_REOH) / PS_REOH, (d2 - PS_REOH) / PS_REOH, costh - costhe};
+  const int I = c_ps.idxd[j][0], J = c_ps.idxd[j][1], K = c_ps.idxd[j][2];
+  // x^(n-1), with x^(-1) := 0
+  auto pw = [](double v, int n) {
+    if (n <= 0) return 0.0;
+    double y = 1.0;
+    for (int i = 1; i < n; ++i) y *= v;
+    return y;
+  };
+  const double B2A = 0.529177249, bfac = 1.0 / (double)0.529177249f;
+  const double efd = exp(-DMS_B1D * ((d1 - PS_REOH) * (d1 - PS_REOH) + (d2 - PS_REOH) * (d2 - PS_REOH)));
+  const double defdr1 = -2.0 * DMS_B1D * (d1 - PS_REOH) * efd * B2A;
+  const double defdr2 = -2.0 * DMS_B1D * (d2 - PS_REOH) * efd * B2A;
+  const double t1 = pw(x[0], I) * pw(x[1], J) * pw(x[2], K);
+  const double t2 = pw(x[0], J) * pw(x[1], I) * pw(x[2], K);
+  double p1r1 = (double)(I - 1) * pw(x[0], I - 1) * pw(x[1], J) * pw(x[2], K) / PS_REOH * B2A;
+  double p1r2 = (double)(J - 1) * pw(x[0], I) * pw(x[1], J - 1) * pw(x[2], K) / PS_REOH * B2A;
+  double p1c = (double)(K - 1) * pw(x[0], I) * pw(x[1], J) * pw(x[2], K - 1);
+  double p2r1 = (double)(J - 1) * pw(x[0], J - 1) * pw(x[1], I) * pw(x[2], K) / PS_REOH * B2A;
+  double p2r2 = (double)(I - 1) * pw(x[0], J) * pw(x[1], I - 1) * pw(x[2], K) / PS_REOH * B2A;
+  double p2c = (double)(K - 1) * pw(x[0], J) * pw(x[1], I) * pw(x[2], K - 1);
+  p1r1 = (p1r1 * efd + t1 * defdr1) * bfac;
+  p1r2 = (p1r2 * efd + t1 * defdr2) * bfac;
+  p1c = p1c * efd;
+  p2r1 = (p2r1 * efd + t2 * defdr1) * bfac;
+  p2r2 = (p2r2 * efd + t2 * defdr2) * bfac;
+  p2c = p2c * efd;
+  const double P1 = t1 * efd, P2 = t2 * efd;
+  dq3[0] = -(P1 + P2);
+  dq3[1] = P1;
+  dq3[2] = P2;
+  const double f1q1r13 = (p1r1 - (p1c * costh / d1)) / d1, f1q1r23 = p1c / (d1 * d2);
+  const double f2q1r23 = (p1r2 - (p1c * costh / d2)) / d2, f2q1r13 = p1c / (d2 * d1);
+  const double f1q2r13 = (p2r1 - (p2c * costh / d1)) / d1, f1q2r23 = p2c / (d1 * d2);
   const double f2q2r23 = (p2r2 - (p2c * costh / d2)) / d2, f2q2r13 = p2c / (d2 * d1);
   for (int k = 0; k < 3; ++k) {
     dgq[0][0][k] = f1q1r13 * R1[k] + f1q1r23 * R2[k];
@@ -238,7 +314,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
   __shared__ TanPar P;
   __shared__ double red[TAN_BLOCK];
   if (tid == 0) {
-    auto par = [&](int i) { return Dual(a.theta[i], i == dir ? 1.0 : 0.0); };
+    auto par = [&](int i) { return Dual(a.theta[i], (!a.dms_coef && i == dir) ? 1.0 : 0.0); };
     for (int i = 0; i < 5; ++i) P.vdw[i] = par(i);
     P.polar[0] = par(5);
     P.polar[1] = par(6);
@@ -256,7 +332,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         P.raCC[i][j] = P.apCC[i][j].v > 0.0 ? dpow(P.apCC[i][j], 1.0 / 3.0) : Dual(0.0);
         P.apDD[i][j] = aDD * pf[i] * pf[j];
       }
-    P.lrc = Dual(c.Uvdw_lrc, dir < 3 ? a.lrc_coef[dir] : 0.0);
+    P.lrc = Dual(c.Uvdw_lrc, (!a.dms_coef && dir < 3) ? a.lrc_coef[dir] : 0.0);
   }
   __syncthreads();
   const double gM = c.gammaM, tmpq = 0.5 * gM / (1.0 - gM);
@@ -270,7 +346,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       for (int k = 0; k < 3; ++k) r[at][k] = rr[3 * at + k];
     ps_monomer<true>(c, r, e1, dr1, q3, gq, a.pes);
     umon += e1;
-    if (dir >= TP_DMS1) dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
+    if (a.dms_coef) dms_coef_tangent(c, r, dir, dq3, dgq);
+    else if (dir >= TP_DMS1) dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
     Dual qd[3];
     for (int i = 0; i < 3; ++i) qd[i] = Dual(q3[i] * CHARGECON, dq3[i] * CHARGECON);
     q[w] = Dual(0.0);
@@ -684,7 +761,8 @@ void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s) {
   a.dG = L.dG;
   for (int i = 0; i < TTM_NPARAM; ++i) a.theta[i] = L.theta[i];
   for (int i = 0; i < 3; ++i) a.lrc_coef[i] = L.lrc_coef[i];
-  hipLaunchKernelGGL(k_ttm_tangent, dim3(c.P, TTM_NPARAM), dim3(TAN_BLOCK), 0, s, c, a);
+  a.dms_coef = L.dms_coef;
+  hipLaunchKernelGGL(k_ttm_tangent, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
 }
 
 }  // namespace pimd

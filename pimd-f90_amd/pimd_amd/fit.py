@@ -10,29 +10,36 @@ configuration and target gradients per atom.  The loss is
 over theta: the Partridge-Schwenke parameters (combined c5z, deohA, alphaoh,
 roh, phh1A, phh2; PES_NPARAM values), optionally followed by the TTM
 parameters (vdW, polarisabilities, Thole factors, DMS corrections;
-TTM_NPARAM values, see TTM_PARAM_NAMES).  Forward values come from the GPU
-force path; the gradient from the hand-written derivative kernels
-(pimd_pes_param_grad for the PES, pimd_ttm_param_grad for the TTM
-parameters).  PESLoss is a torch.autograd.Function over theta, so any torch
+TTM_NPARAM values, see TTM_PARAM_NAMES) and then optionally by the 84
+coefficients of the dipole moment surface that charges the sites (DMS_NCOEF).
+Forward values come from the GPU force path; the gradient from the
+hand-written derivative kernels (pimd_pes_param_grad for the PES,
+pimd_ttm_param_grad for the TTM parameters, pimd_dms_coef_grad for the DMS
+coefficients).  PESLoss is a torch.autograd.Function over theta, so any torch
 optimiser can drive a fit.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .ttm import PES_NPARAM, TTM_NPARAM, TTMForceField
+from .ttm import DMS_NCOEF, PES_NPARAM, TTM_NPARAM, TTMForceField
 
 NPARAM_ALL = PES_NPARAM + TTM_NPARAM
+NPARAM_DMS = NPARAM_ALL + DMS_NCOEF
 
 
 def split_theta(theta):
-    """(PES part, TTM part or None) of a parameter vector of either length."""
+    """(PES part, TTM part or None, DMS coefficients or None) of a parameter
+    vector of any of the three lengths."""
     th = np.asarray(theta, dtype=np.float64)
     if th.size == PES_NPARAM:
-        return th, None
+        return th, None, None
     if th.size == NPARAM_ALL:
-        return th[:PES_NPARAM], th[PES_NPARAM:]
-    raise ValueError(f"theta must hold {PES_NPARAM} or {NPARAM_ALL} values, got {th.size}")
+        return th[:PES_NPARAM], th[PES_NPARAM:], None
+    if th.size == NPARAM_DMS:
+        return th[:PES_NPARAM], th[PES_NPARAM:NPARAM_ALL], th[NPARAM_ALL:]
+    raise ValueError(f"theta must hold {PES_NPARAM}, {NPARAM_ALL} or {NPARAM_DMS} values, "
+                     f"got {th.size}")
 
 
 class FitGroup:
@@ -46,10 +53,12 @@ class FitGroup:
         self.G_target = np.ascontiguousarray(G_target, dtype=np.float64)
 
     def evaluate(self, theta, grad: bool):
-        pes, ttm = split_theta(theta)
+        pes, ttm, dms = split_theta(theta)
         self.ff.set_pes_params(pes)
         if ttm is not None:
             self.ff.set_ttm_params(ttm)
+        if dms is not None:
+            self.ff.set_dms_coefs(dms)
         r = self.ff.full_bead_forces(self.RRt, self.RRc)
         dE = r["Upott"] - self.E_target
         dG = r["dRRt"] - self.G_target
@@ -59,6 +68,8 @@ class FitGroup:
             g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)
             if ttm is not None:
                 g = np.concatenate([g, self.ff.ttm_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0]])
+            if dms is not None:
+                g = np.concatenate([g, self.ff.dms_coef_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0]])
         return loss, g
 
 
@@ -77,8 +88,8 @@ def _autograd_function():
     import torch
 
     class PESLoss(torch.autograd.Function):
-        """loss = PESLoss.apply(theta, groups): theta a float64 tensor of PES_NPARAM or
-        PES_NPARAM + TTM_NPARAM values."""
+        """loss = PESLoss.apply(theta, groups): theta a float64 tensor of PES_NPARAM,
+        PES_NPARAM + TTM_NPARAM or PES_NPARAM + TTM_NPARAM + DMS_NCOEF values."""
 
         @staticmethod
         def forward(ctx, theta, groups):

@@ -37,7 +37,8 @@ EXPORTS = (
     "pimd_batch_shape", "pimd_batch_md_run", "pimd_pes_params_get", "pimd_pes_params_set",
     "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy", "pimd_ttm_params_get",
     "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
-    "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles",
+    "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles", "pimd_dms_coefs_get",
+    "pimd_dms_coefs_set", "pimd_dms_coef_grad",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -46,6 +47,7 @@ TTM_PARAM_NAMES = ("vdwA", "vdwB", "vdwC", "vdwD", "vdwE", "polarO", "polarH", "
                    "polfacO", "polfacH", "polfacM", "aDD", "aCCaCD", "dms_param1", "dms_param2",
                    "dms_param3")
 TTM_NPARAM = len(TTM_PARAM_NAMES)
+DMS_NCOEF = 84     # coefD of the dipole moment surface (nasa_mod.f90)
 # per-step observables of write_out (pimd_md_observables)
 OBS_NAMES = ("sys_temp", "virial_energy", "virial_press", "simple_energy", "simple_press",
              "dip_x", "dip_y", "dip_z", "dip_mag_sum")
@@ -118,6 +120,9 @@ def load_library(path: str = LIB_PATH):
         "pimd_ttm_params_get": [vp, dp],
         "pimd_ttm_params_set": [vp, dp],
         "pimd_ttm_param_grad": [vp, dp, dp, dp, dp, dp, dp],
+        "pimd_dms_coefs_get": [vp, dp],
+        "pimd_dms_coefs_set": [vp, dp],
+        "pimd_dms_coef_grad": [vp, dp, dp, dp, dp, dp, dp],
         "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
@@ -294,6 +299,27 @@ class TTMForceField:
         grad, E, dE = np.zeros(TTM_NPARAM), np.zeros(P), np.zeros((P, TTM_NPARAM))
         _check(load_library().pimd_ttm_param_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(grad),
                                                   _ptr(E), _ptr(dE)))
+        return grad, E, dE
+
+    def dms_coefs(self) -> np.ndarray:
+        """The 84 dipole-moment-surface coefficients coefD in use."""
+        c = np.zeros(DMS_NCOEF)
+        _check(load_library().pimd_dms_coefs_get(self._h, _ptr(c)))
+        return c
+
+    def set_dms_coefs(self, coef):
+        c = _arr(coef, (DMS_NCOEF,))
+        _check(load_library().pimd_dms_coefs_set(self._h, _ptr(c)))
+        if self._single is not None:
+            self._single.set_dms_coefs(c)
+
+    def dms_coef_grad(self, RRt, ew, gw):
+        """Tangents of the DMS coefficients: (grad, E, dE/dcoef), as ttm_param_grad."""
+        P, na = self.params.nbeads, self.params.natoms
+        r, e, g = _arr(RRt, (P, na, 3)), _arr(ew, (P,)), _arr(gw, (P, na, 3))
+        grad, E, dE = np.zeros(DMS_NCOEF), np.zeros(P), np.zeros((P, DMS_NCOEF))
+        _check(load_library().pimd_dms_coef_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(grad),
+                                                 _ptr(E), _ptr(dE)))
         return grad, E, dE
 
     def potential(self, RR, RRc, t: int = 1) -> dict:

@@ -250,3 +250,85 @@ def test_fit_step_reduces_loss(fitset):
     loss_and_grad(groups, full, grad=False)
     print("loss", L0, "->", L1, flush=True)
     assert L1 < L0
+
+
+def _dms_h(c, j, rel):
+    return rel * max(abs(c[j]), 1e-3)
+
+
+def _dms_points(ff, grp, c0, j, h):
+    """Energies and gradients at c0 + m h e_j, m = 2, 1, -1, -2."""
+    out = []
+    for m in (2, 1, -1, -2):
+        c = c0.copy()
+        c[j] += m * h
+        ff.set_dms_coefs(c)
+        r = ff.full_bead_forces(grp.RRt, grp.RRc)
+        out.append((r["Upott"], r["dRRt"]))
+    ff.set_dms_coefs(c0)
+    return out
+
+
+def test_dms_coef_energy_derivatives_vs_fd(fitset):
+    """dE_b/dcoefD_j (pimd_dms_coef_grad) of every configuration against the
+    five-point difference (h = 1e-3 relative; the charges are linear in the
+    coefficients and the energy smooth in the charges, so the truncation
+    error, O(h^4), is far below the bar) of the forward energies, for all 84
+    coefficients of the dipole moment surface.  The bar is normwise over the
+    84 derivatives: the largest error below 1e-7 of the largest derivative
+    (some derivatives are 1e-5 of the largest, and their differences sit at
+    the noise of the forward energies, ~1e-9 kcal/mol).  The same tangent
+    sweep's value parts reproduce the forward energies.  (A central
+    difference at h = 1e-6 relative drowns in that noise for coefficients of
+    order 1e-3.)"""
+    model, groups, theta0, pes0, L0, g = fitset
+    for grp in groups:
+        P, na = grp.RRt.shape[:2]
+        grp.ff.set_ttm_params(theta0)
+        c0 = grp.ff.dms_coefs()
+        r0 = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        _, E, dE = grp.ff.dms_coef_grad(grp.RRt, np.zeros(P), np.zeros((P, na, 3)))
+        assert np.max(np.abs(E - r0["Upott"]) / np.abs(r0["Upott"])) < 1e-10
+        fd = np.zeros_like(dE)
+        for j in range(c0.size):
+            h = _dms_h(c0, j, 1e-3)
+            E2, E1, Em1, Em2 = [e for e, _ in _dms_points(grp.ff, grp, c0, j, h)]
+            fd[:, j] = (8 * (E1 - Em1) - (E2 - Em2)) / (12 * h)
+        err, scale = np.max(np.abs(fd - dE)), np.max(np.abs(fd))
+        big = np.abs(fd) > 1e-3 * scale
+        rel_big = np.max(np.abs(fd - dE)[big] / np.abs(fd)[big])
+        print(f"model {model} N {na // 3}: 84 DMS coefficients, max |dE/dc| {scale:.3e}, "
+              f"largest error {err:.2e} ({err / scale:.1e} of it); componentwise "
+              f"{rel_big:.1e} where |dE/dc| > 1e-3 of the largest", flush=True)
+        assert err < FD5_REL * scale
+        again = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        assert np.array_equal(again["dRRt"], r0["dRRt"])
+
+
+def test_dms_coef_loss_gradient_vs_fd(fitset):
+    """The loss gradient with respect to the DMS coefficients against the
+    five-point difference (h = 1e-3 relative) of the loss over both groups
+    (energies and gradients), each difference L(c + m h) - L(c - m h) summed
+    as (a+ - a-)(a+ + a- - 2 target); normwise over the 84 components, 1e-6
+    of the largest."""
+    model, groups, theta0, pes0, L0, g = fitset
+    for grp in groups:
+        grp.ff.set_ttm_params(theta0)
+    c0 = groups[0].ff.dms_coefs()
+    gc = np.zeros(c0.size)
+    for grp in groups:
+        gc += grp.ff.dms_coef_grad(grp.RRt, *_weights(grp))[0]
+    fds = np.zeros(c0.size)
+    for j in range(c0.size):
+        h = _dms_h(c0, j, 1e-3)
+        D = {1: 0.0, 2: 0.0}
+        for grp in groups:
+            (E2, G2), (E1, G1), (Em1, Gm1), (Em2, Gm2) = _dms_points(grp.ff, grp, c0, j, h)
+            for m, (Ep, Gp, Em, Gm) in ((1, (E1, G1, Em1, Gm1)), (2, (E2, G2, Em2, Gm2))):
+                wE, wG = Ep + Em - 2 * grp.E_target, Gp + Gm - 2 * grp.G_target
+                D[m] += np.sum((Ep - Em) * wE) + np.sum((Gp - Gm) * wG)
+        fds[j] = (8 * D[1] - D[2]) / (12 * h)
+    err, scale = np.max(np.abs(fds - gc)), np.max(np.abs(fds))
+    print(f"model {model}: loss gradient over 84 DMS coefficients, max {scale:.3e}, "
+          f"largest error {err:.2e} ({err / scale:.1e} of it)", flush=True)
+    assert err < FD_REL * scale
