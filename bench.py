@@ -155,6 +155,7 @@ def cpu_baseline(p, RRt, RRc, budget_s=15.0):
 
 
 PEAK_FP32_TFLOPS = 157.3
+SOR_KERNELS = ("pair_dd", "sfac_d", "field_d_sor")  # launched once per SOR iteration
 # FP32 arithmetic in the mixed form: the real-space pair passes and the SOR
 # iterations' reciprocal dipole pass
 FP32_KERNELS = ("pair_qq", "pair_dd", "pair_final", "sfac_d", "field_d_sor")
@@ -196,10 +197,16 @@ def run_form(p, RRt, PPt, RRc, args):
     nnb = (4.0 / 3.0) * np.pi * p.rc ** 3 / (p.box[0] * p.box[1] * p.box[2]) * p.nwaters
     c = {"N": p.nwaters, "P": p.nbeads, "nk": info["nkvec"], "nnb": min(nnb, p.nwaters - 1),
          "nd": {2: 3, 3: 1}.get(p.pot_model, 0), "model": p.pot_model}
+    # beads a SOR launch works on, on average (converged beads leave the
+    # remaining iterations' launches early)
+    bead_iters = kernels.pop("sor_bead_iterations", (0, 0.0))[0]
+    n_sor = kernels.get("field_d_sor", (0, 0.0))[0]
+    active = bead_iters / n_sor if n_sor else float(p.nbeads)
+    out["sor_active_beads_per_launch"] = round(active, 2)
     tot = {k: v[1] for k, v in kernels.items()}
     dom = max(tot, key=tot.get)
     cnt, tms = kernels[dom]
-    fl = flop_model(dom, c)
+    fl = flop_model(dom, {**c, "P": active} if dom in SOR_KERNELS else c)
     if fl is not None and cnt > 0:
         fp32 = p.precision == 1 and dom in FP32_KERNELS
         peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
@@ -425,7 +432,7 @@ def replica_params(sp, precision=0):
     """Model parameters of one replica of replica_specs()."""
     from pimd_amd.params import TTMParams
     from pimd_amd.systems import make_system
-    L = make_system(sp["nwaters"], 1, seed=sp["seed"])[3]
+    L = make_system(sp["nwaters"], 1, seed=sp["seed"], spaced=True)[3]
     return TTMParams(pot_model=sp["model"], nwaters=sp["nwaters"], nbeads=sp["nbeads"],
                      box=(L, L, L), rc=-1.0, eps_ewald=1e-8, mass_o=15.995, mass_h=1.008,
                      guess_initdip=True, precision=precision)
@@ -436,7 +443,8 @@ def make_replica(sp, precision=0):
     gradient: (integrator, force field, RRt, PPt, dRRt)."""
     from pimd_amd.systems import make_system
     from pimd_amd.ttm import PIMDIntegrator, TTMForceField
-    RRt, PPt, RRc, _ = make_system(sp["nwaters"], sp["nbeads"], temp=sp["temp"], seed=sp["seed"])
+    RRt, PPt, RRc, _ = make_system(sp["nwaters"], sp["nbeads"], temp=sp["temp"], seed=sp["seed"],
+                                   spaced=True)
     ff = TTMForceField(replica_params(sp, precision))
     integ = PIMDIntegrator(ff, delt_fs=0.25, temp=sp["temp"], bead_thermostat=1,
                            centroid_thermostat=True, tau_centroid=0.1)

@@ -131,6 +131,10 @@ struct pimd_ctx {
     }
     pending.clear();
   }
+  // a counter among the timed entries (no time): k more of n
+  void count(const char *n, long k) {
+    if (prof) launches[kid(n)] += k;
+  }
   template <class F>
   void timed(const char *n, F &&f) {
     if (!prof) {
@@ -515,7 +519,7 @@ struct Arena {
   template <class T>
   void take(T **p, size_t n) {
     off = (off + 255) & ~(size_t)255;
-    if (base) *p = reinterpret_cast<T *>(base + off);
+    if (base) *p = (T *)(base + off);  // a C cast: T may carry the kernels' address space
     off += std::max<size_t>(n, 1) * sizeof(T);
   }
 };
@@ -703,6 +707,8 @@ int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
     // kernel can hold it; the last iteration's test stays a launch
     const bool fold = all_fold && !poll;
     for (int it = 1; it <= maxiter; ++it) {
+      // with timing on: the beads this iteration works on, for per-launch work counts
+      if (poll && prof) px->count("sor_bead_iterations", it == 1 ? px->c.P : px->h_nactive[it - 1]);
       timed(px, "pair_dd", [&] { launch_pair_dd(g, false, s, fold && it > 1 ? it - 1 : 0); });
       timed(px, "sfac_d", [&] { launch_sfac(g, SF_D, s); });
       timed(px, "field_d_sor", [&] { launch_field(g, FD_D, it, s); });

@@ -92,71 +92,82 @@ struct SysConst {
 enum Tally { T_UMON = 0, T_UVDW, T_UELEC, T_UIND, T_VIRC, T_VIRT /* 9 slots, [col][row] */,
              T_COUNT = T_VIRT + 9 };
 
+// Buffer addresses as the kernels see them: global-memory pointers, so that
+// their loads are global ones (scalar ones where the address is the same on
+// every lane) rather than flat.  The host sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using dptr = __attribute__((address_space(1))) T *;
+#else
+template <class T>
+using dptr = T *;
+#endif
+
 // Device buffers of one context.
 struct DevBufs {
-  double *RR = nullptr, *RRc = nullptr;         // staging of host inputs
-  Site *site = nullptr;                         // [P][N][4]
-  double *Rcs = nullptr;                        // [N][4][3] centroid sites
-  double *grdq = nullptr;                       // [P][N][27]
-  double *phi = nullptr;                        // [P][N][4]
-  double *Efq = nullptr;                        // [P][N][4][3]
-  double *dR = nullptr;                         // [P][N][4][3]
-  double *dip = nullptr;                        // [P][N][nd][3]
-  double *dip_prev = nullptr;                   // [P][N][nd][3] input of the last SOR step
-  double *Efd = nullptr;                        // [P][N][nd][3]
-  double *diff2 = nullptr;                      // [P][N*nd]
-  double *tally = nullptr;                      // [P][N][T_COUNT]
-  double *hist = nullptr;                       // [P][4][N*nd][3]
-  int *nbr = nullptr, *nbrc = nullptr;          // [P][N][cap], [P][N]
-  int *nbrn = nullptr;                          // [P][N] near partners (listed first)
-  int *cell_start = nullptr, *cell_mol = nullptr;  // [P][ncells + 1], [P][N] cell list
-  double2 *tab = nullptr;                       // [P][T][4N]
-  double2 *tabd = nullptr;                      // [P][T][nd N] the dipole sites' rows
-  float2 *tabdf = nullptr;                      // the same in FP32 (mixed form)
-  double *Sq = nullptr, *Sm = nullptr;          // [P][nk][2]
-  double *Sv = nullptr;                         // [P][nk][6]
-  double *virk = nullptr;                       // [P][nkb][9]
-  double *part = nullptr;                       // [P][sf_chunks_max][nk][2]
-  int *active = nullptr, *iters = nullptr;      // [P]
-  int *nactive = nullptr;                       // [1]
-  double4 *kv = nullptr;                        // [nk] kx, ky, kz, Ak
-  int4 *khkl = nullptr;                         // [nk] h, k, l, 0
-  int4 *cols = nullptr;                         // [ncol] h, k, l_first, count
-  int *colk = nullptr;                          // [ncol] first k index
-  int4 *segs = nullptr;                         // [nseg] h, k, l_first, count
-  int *segk = nullptr;                          // [nseg] first k index
+  dptr<double> RR = nullptr, RRc = nullptr;  // staging of host inputs
+  dptr<Site> site = nullptr;  // [P][N][4]
+  dptr<double> Rcs = nullptr;  // [N][4][3] centroid sites
+  dptr<double> grdq = nullptr;  // [P][N][27]
+  dptr<double> phi = nullptr;  // [P][N][4]
+  dptr<double> Efq = nullptr;  // [P][N][4][3]
+  dptr<double> dR = nullptr;  // [P][N][4][3]
+  dptr<double> dip = nullptr;  // [P][N][nd][3]
+  dptr<double> dip_prev = nullptr;  // [P][N][nd][3] input of the last SOR step
+  dptr<double> Efd = nullptr;  // [P][N][nd][3]
+  dptr<double> diff2 = nullptr;  // [P][N*nd]
+  dptr<double> tally = nullptr;  // [P][N][T_COUNT]
+  dptr<double> hist = nullptr;  // [P][4][N*nd][3]
+  dptr<int> nbr = nullptr, nbrc = nullptr;  // [P][N][cap], [P][N]
+  dptr<int> nbrn = nullptr;  // [P][N] near partners (listed first)
+  dptr<int> cell_start = nullptr, cell_mol = nullptr;  // [P][ncells + 1], [P][N] cell list
+  dptr<double2> tab = nullptr;  // [P][T][4N]
+  dptr<double2> tabd = nullptr;  // [P][T][nd N] the dipole sites' rows
+  dptr<float2> tabdf = nullptr;  // the same in FP32 (mixed form)
+  dptr<double> Sq = nullptr, Sm = nullptr;  // [P][nk][2]
+  dptr<double> Sv = nullptr;  // [P][nk][6]
+  dptr<double> virk = nullptr;  // [P][nkb][9]
+  dptr<double> part = nullptr;  // [P][sf_chunks_max][nk][2]
+  dptr<int> active = nullptr, iters = nullptr;  // [P]
+  dptr<int> nactive = nullptr;  // [1]
+  dptr<double4> kv = nullptr;  // [nk] kx, ky, kz, Ak
+  dptr<int4> khkl = nullptr;  // [nk] h, k, l, 0
+  dptr<int4> cols = nullptr;  // [ncol] h, k, l_first, count
+  dptr<int> colk = nullptr;  // [ncol] first k index
+  dptr<int4> segs = nullptr;  // [nseg] h, k, l_first, count
+  dptr<int> segk = nullptr;  // [nseg] first k index
   // outputs of the last force call
-  double *Upot = nullptr, *virt = nullptr, *virialc = nullptr, *terms = nullptr;  // [P],[P][9],[P],[P][5]
-  double *dRR = nullptr, *dipI = nullptr, *dipE = nullptr, *chg = nullptr;
-  int *iters_out = nullptr;
+  dptr<double> Upot = nullptr, virt = nullptr, virialc = nullptr, terms = nullptr;  // [P],[P][9],[P],[P][5]
+  dptr<double> dRR = nullptr, dipI = nullptr, dipE = nullptr, chg = nullptr;
+  dptr<int> iters_out = nullptr;
   // ring polymer
-  double *C = nullptr, *A = nullptr;            // [P][P], [4][P] (A11,A12,A21,A22)
-  double *msf = nullptr, *c1 = nullptr, *c2 = nullptr;
-  double *masses = nullptr;                     // [Natoms]
-  double *PP = nullptr, *noise = nullptr;       // [P][Natoms][3], [2][Natoms][P][3]
-  double *uk = nullptr;                         // [1] kinetic energy
-  double *ukpart = nullptr;                     // [256] kinetic partial sums
-  double *radius = nullptr;                     // [P][N] site radius about O
-  double *pes = nullptr;                        // [PES_NALL] PES parameters, then coefD
-  double *adj = nullptr;                        // adjoint inputs and partial sums
-  double *dRRfast = nullptr, *mol = nullptr;    // contracted MD: [P][Natoms][3], [P][N][3]
-  int *clk = nullptr;                           // device step clock {t, record slot}
-  double *obs_part = nullptr;                   // [64][5] partial sums of the observables
-  int *flags = nullptr;                         // [1] bit 0: non-finite coordinates seen
-  double *vxi_beads = nullptr;                  // [Natoms][P][3][2] bead Nose-Hoover chains
-  double *nh_vxi = nullptr, *nh_s = nullptr;    // global chain velocities [2], its scale [1]
+  dptr<double> C = nullptr, A = nullptr;  // [P][P], [4][P] (A11,A12,A21,A22)
+  dptr<double> msf = nullptr, c1 = nullptr, c2 = nullptr;
+  dptr<double> masses = nullptr;  // [Natoms]
+  dptr<double> PP = nullptr, noise = nullptr;  // [P][Natoms][3], [2][Natoms][P][3]
+  dptr<double> uk = nullptr;  // [1] kinetic energy
+  dptr<double> ukpart = nullptr;  // [256] kinetic partial sums
+  dptr<double> radius = nullptr;  // [P][N] site radius about O
+  dptr<double> pes = nullptr;  // [PES_NALL] PES parameters, then coefD
+  dptr<double> adj = nullptr;  // adjoint inputs and partial sums
+  dptr<double> dRRfast = nullptr, mol = nullptr;  // contracted MD: [P][Natoms][3], [P][N][3]
+  dptr<int> clk = nullptr;  // device step clock {t, record slot}
+  dptr<double> obs_part = nullptr;  // [64][5] partial sums of the observables
+  dptr<int> flags = nullptr;  // [1] bit 0: non-finite coordinates seen
+  dptr<double> vxi_beads = nullptr;  // [Natoms][P][3][2] bead Nose-Hoover chains
+  dptr<double> nh_vxi = nullptr, nh_s = nullptr;  // global chain velocities [2], its scale [1]
   // contracted MD: the outputs of the centroid's force call (one bead)
-  const double *cen_Upot = nullptr, *cen_virt = nullptr, *cen_virialc = nullptr;
-  const double *cen_dRR = nullptr;
-  double *nh_tau = nullptr;                     // [P] their time constants per mode
-  int *clk_own = nullptr;                       // this context's clock (clk may be its owner's)
-  double *tan = nullptr;                        // TTM parameter tangents: inputs, tables, workspace
+  dptr<const double> cen_Upot = nullptr, cen_virt = nullptr, cen_virialc = nullptr;
+  dptr<const double> cen_dRR = nullptr;
+  dptr<double> nh_tau = nullptr;  // [P] their time constants per mode
+  dptr<int> clk_own = nullptr;  // this context's clock (clk may be its owner's)
+  dptr<double> tan = nullptr;  // TTM parameter tangents: inputs, tables, workspace
   size_t tan_bytes = 0;
-  double *red = nullptr;                        // scratch for reductions
+  dptr<double> red = nullptr;  // scratch for reductions
   // reciprocal-space field, potential (FD_Q) and gradient, potential (FD_QD)
   // of the last field pass, [P][N][4][3] and [P][N][4]; their consumers add
   // them to Efq / dR and phi, so the real-space pair pass may run beside
-  double *kE = nullptr, *kphi = nullptr;
+  dptr<double> kE = nullptr, kphi = nullptr;
 };
 
 // ---------------------------------------------------------------------------

@@ -163,10 +163,56 @@ def ring_polymer(centroids: np.ndarray, nbeads: int, temp: float = 300.0, seed: 
     return np.ascontiguousarray(RRt), np.ascontiguousarray(PPt)
 
 
+def spaced_water_box(nwaters: int, box: float | None = None, min_contact: float = 1.9,
+                     tries: int = 200, seed: int = 0) -> tuple[np.ndarray, float]:
+    """Centroid coordinates (Natoms, 3) and box edge with no short contacts:
+    the oxygens on nwaters randomly chosen sites of the smallest n^3 grid that
+    holds them (spacing L/n, at least 2.4 A at liquid density for the sizes
+    used here), each monomer rotated by the first of up to `tries`
+    Haar-uniform rotations whose atoms keep min_contact (A) from every atom of
+    the molecules already placed (minimum image), else the best one tried.
+    A lattice with a short axis (water_box with 4 x 4 x 8 at 128 waters puts
+    oxygens 1.96 A apart) throws TTM2.1-F, whose H sites are polarisable, into
+    the polarisation catastrophe; these boxes do not."""
+    rng = np.random.default_rng(seed)
+    if box is None:
+        box = box_edge_for_density(nwaters)
+    n = 1
+    while n ** 3 < nwaters:
+        n += 1
+    sites = rng.choice(n ** 3, size=nwaters, replace=False)
+    mono = monomer_geometry()
+    placed = np.empty((0, 3))
+    out = np.empty((nwaters, 3, 3))
+    for iw, sidx in enumerate(sites):
+        ix, iy, iz = sidx // (n * n), (sidx // n) % n, sidx % n
+        centre = (np.array([ix, iy, iz]) + 0.5) * box / n - box / 2.0
+        best, best_d = None, -1.0
+        for _ in range(tries):
+            cand = centre + mono @ _random_rotation(rng).T
+            if placed.shape[0] == 0:
+                best = cand
+                break
+            d = cand[:, None, :] - placed[None, :, :]
+            d -= box * np.rint(d / box)
+            dmin = float(np.sqrt(np.min(np.sum(d * d, axis=-1))))
+            if dmin > best_d:
+                best, best_d = cand, dmin
+            if dmin >= min_contact:
+                break
+        out[iw] = best
+        placed = np.vstack([placed, best])
+    return out.reshape(3 * nwaters, 3), float(box)
+
+
 def make_system(nwaters: int, nbeads: int, temp: float = 300.0, seed: int = 0, box=None,
-                lattice=None):
-    """Centroid box + ring polymer: returns (RRt, PPt, RRc, box_edge)."""
-    cen, L = water_box(nwaters, box=box, lattice=lattice, seed=seed)
+                lattice=None, spaced: bool = False):
+    """Centroid box + ring polymer: returns (RRt, PPt, RRc, box_edge).  spaced:
+    the centroids from spaced_water_box instead of water_box."""
+    if spaced:
+        cen, L = spaced_water_box(nwaters, box=box, seed=seed)
+    else:
+        cen, L = water_box(nwaters, box=box, lattice=lattice, seed=seed)
     RRt, PPt = ring_polymer(cen, nbeads, temp=temp, seed=seed + 7919)
     RRc = RRt.mean(axis=0)
     return RRt, PPt, RRc, L
