@@ -659,6 +659,7 @@ def main():
         "cpu_baseline_threads": cpu_par,
         "force_only": r["force_only"],
         "kernels_ms_per_step": _kern(r, args.steps),
+        "sor_active_beads_per_launch": r.get("sor_active_beads_per_launch"),
         "sor_iterations_per_step": (round(r["kernels"]["field_d_sor"][0] / max(args.steps, 1), 2)
                                     if "field_d_sor" in r["kernels"] else None),
     }

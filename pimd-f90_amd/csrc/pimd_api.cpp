@@ -698,7 +698,8 @@ int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
   });
   // the reciprocal-space charge pass (it writes kE / kphi, added by their consumers)
   timed(px, "ewald_tables", [&] { launch_tables(g, s); });
-  timed(px, "sfac_q", [&] { launch_sfac(g, SF_Q, s); });
+  timed(px, "sfac_q", [&] { launch_sfac(g, SF_Q, s, false, 1); });
+  timed(px, "sfac_q_reduce", [&] { launch_sfac(g, SF_Q, s, false, 2); });
   timed(px, "field_q", [&] { launch_field(g, FD_Q, 0, s); });
   if (any_ttm) {
     timed(px, "post_qq", [&] { launch_post_qq(g, s); });
@@ -710,7 +711,8 @@ int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
       // with timing on: the beads this iteration works on, for per-launch work counts
       if (poll && prof) px->count("sor_bead_iterations", it == 1 ? px->c.P : px->h_nactive[it - 1]);
       timed(px, "pair_dd", [&] { launch_pair_dd(g, false, s, fold && it > 1 ? it - 1 : 0); });
-      timed(px, "sfac_d", [&] { launch_sfac(g, SF_D, s); });
+      timed(px, "sfac_d", [&] { launch_sfac(g, SF_D, s, false, 1); });
+      timed(px, "sfac_d_reduce", [&] { launch_sfac(g, SF_D, s, false, 2); });
       timed(px, "field_d_sor", [&] { launch_field(g, FD_D, it, s); });
       if (!fold || it == maxiter) timed(px, "converge", [&] { launch_converge(g, it, s); });
       // No host round trip: converged beads are switched off on the device
@@ -736,7 +738,8 @@ int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
       });
     }
     timed(px, "pair_final", [&] { launch_pair_final(g, s); });
-    timed(px, "sfac_v", [&] { launch_sfac(g, SF_V, s); });
+    timed(px, "sfac_v", [&] { launch_sfac(g, SF_V, s, false, 1); });
+    timed(px, "sfac_v_reduce", [&] { launch_sfac(g, SF_V, s, false, 2); });
     timed(px, "field_qd", [&] { launch_field(g, FD_QD, 0, s); });
   }
   timed(px, "virial_k", [&] { launch_virk(g, s); });

@@ -307,8 +307,9 @@ void launch_monomer(const Group &g, hipStream_t s);
 void launch_neighbors(const Group &g, hipStream_t s);
 void launch_pair_qq(const Group &g, hipStream_t s);
 void launch_tables(const Group &g, hipStream_t s);
-// refine: SF_D in FP64 for the mixed replicas' refinement step
-void launch_sfac(const Group &g, int mode, hipStream_t s, bool refine = false);
+// refine: SF_D in FP64 for the mixed replicas' refinement step.  part: 1 the
+// per-chunk partial sums, 2 their reduction into the structure factor, 3 both.
+void launch_sfac(const Group &g, int mode, hipStream_t s, bool refine = false, int part = 3);
 int sfac_chunks_used(const SysConst &c);
 void launch_converge(const Group &g, int iter, hipStream_t s);
 // iter: the SOR iteration of FD_D, -1 for the FP64 refinement (maxiter + 1);

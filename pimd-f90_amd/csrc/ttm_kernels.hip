@@ -2335,8 +2335,9 @@ static void launch_sfac_kz(int v, dim3 grid, const BlockMap &m, size_t lds, cons
 // refine: the FP64 dipole structure factor of the mixed form's refinement
 // step (mixed replicas only); otherwise the mixed form's SOR iterations take
 // it in FP32 (variant bit 8).
+// part: 1 the chunk partial sums, 2 their fixed-order reduction, 3 both.
 template <int MODE>
-static void launch_sfac_mode(const Group &g, bool refine, hipStream_t s) {
+static void launch_sfac_mode(const Group &g, bool refine, int part, hipStream_t s) {
   constexpr int NV = MODE == SF_V ? 6 : 2;
   auto plan = [&](const SysConst &c) {
         if (MODE != SF_Q && !is_ttm(c)) return Plan{-1, 0, 0, 0, 0};
@@ -2358,7 +2359,7 @@ static void launch_sfac_mode(const Group &g, bool refine, hipStream_t s) {
         return Plan{1 | (kzb << 1) | (split ? 16 : 0) | ((threads / 64) << 5) | (f32 ? 256 : 0),
                     idiv(c.nseg, threads), nch, c.P, split ? lds + red : lds};
   };
-  launch_over(
+  if (part & 1) launch_over(
       g, [&](const SysConst &c, const MDConst &) { return plan(c); },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
         if (!(v & 1)) {
@@ -2373,7 +2374,7 @@ static void launch_sfac_mode(const Group &g, bool refine, hipStream_t s) {
         }
         launch_sfac_kz<MODE, double>(v, grid, m, lds, g, s);
       });
-  launch_over(
+  if (part & 2) launch_over(
       g,
       [&](const SysConst &c, const MDConst &) {
         if (plan(c).variant < 0) return Plan{-1, 0, 0, 0, 0};
@@ -2384,10 +2385,10 @@ static void launch_sfac_mode(const Group &g, bool refine, hipStream_t s) {
       });
 }
 
-void launch_sfac(const Group &g, int mode, hipStream_t s, bool refine) {
-  if (mode == SF_Q) launch_sfac_mode<SF_Q>(g, false, s);
-  else if (mode == SF_D) launch_sfac_mode<SF_D>(g, refine, s);
-  else launch_sfac_mode<SF_V>(g, false, s);
+void launch_sfac(const Group &g, int mode, hipStream_t s, bool refine, int part) {
+  if (mode == SF_Q) launch_sfac_mode<SF_Q>(g, false, part, s);
+  else if (mode == SF_D) launch_sfac_mode<SF_D>(g, refine, part, s);
+  else launch_sfac_mode<SF_V>(g, false, part, s);
 }
 
 // The FP32 dipole field of the mixed form's SOR iterations (variant bits as
