@@ -162,9 +162,11 @@ void invalidate_graphs(pimd_ctx *x) {
 template <class T>
 int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
   if (e != hipSuccess) return fail(PIMD_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T),
                                    hipGetErrorString(e));
+  *p = (T *)q;  // a C cast: T may carry the kernels' address space
   return PIMD_OK;
 }
 
@@ -445,7 +447,10 @@ void plan_system(pimd_ctx *x, HostTables &t) {
   segs.clear();
   segk.clear();
   {
-    const int seg = env_int("PIMD_SFAC_SEG", 8, 1, 32);
+    // 4 kz per segment: twice the lanes of 8 for the same sites (512 H2O x 32
+    // beads: 1.273 -> 1.251 ms per mixed step, 1.592 -> 1.586 FP64; the
+    // 32-water, 32-bead Rc 4.9 A system gains more)
+    const int seg = env_int("PIMD_SFAC_SEG", 4, 1, 32);
     for (size_t i = 0; i < cols.size(); ++i)
       for (int o = 0; o < cols[i].w; o += seg) {
         segs.push_back(make_int4(cols[i].x, cols[i].y, cols[i].z + o, std::min(seg, cols[i].w - o)));
@@ -1414,7 +1419,10 @@ static int step_contracted(const Group &g, const Group &cen, int intra, hipStrea
   timed(px, "broadcast", [&] { launch_broadcast_beads(g, s); });
   timed(px, "kinetic", [&] { launch_kinetic(g, s); });
   launch_contract_record(g, s);
-  timed(px, "observables", [&] { launch_observables(g, false, false, s); });  // no bead dipoles
+  // the bead dipoles of contracted_MD: monomer dipoles of every bead
+  // (calc_monomer_dip_moments after the centroid call, MD.f90:407)
+  timed(px, "monomer_dipoles", [&] { launch_monomer_dipoles(g, s); });
+  timed(px, "observables", [&] { launch_observables(g, true, false, s); });
   bead_thermostat(g, f, s, px, 1, KICK_DRR, load_noise);  // with the outer half kick
   timed(px, "kinetic", [&] { launch_kinetic(g, s); });
   if (f.global_nh) timed(px, "nose_hoover", [&] { launch_global_nh(g, s); });
@@ -2137,8 +2145,10 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
     if (d.tan) (void)hipFree(d.tan);
     d.tan = nullptr;
     d.tan_bytes = 0;
-    int rc = dalloc((char **)&d.tan, bytes);
+    char *t = nullptr;
+    int rc = dalloc(&t, bytes);
     if (rc != PIMD_OK) return rc;
+    d.tan = (decltype(d.tan))t;
     d.tan_bytes = bytes;
   }
   TanLaunch L;

@@ -336,6 +336,8 @@ void launch_spc_finalize(const Group &g, hipStream_t s);
 void launch_pes_fast(const Group &g, bool record, hipStream_t s);
 void launch_monomer_dipoles(const SysConst &c, const double *pes, const double *RR, double *out,
                             hipStream_t s);
+// every replica's monomer dipoles from its RR into its dipI
+void launch_monomer_dipoles(const Group &g, hipStream_t s);
 void launch_broadcast_beads(const Group &g, hipStream_t s);
 void launch_contract_record(const Group &g, hipStream_t s);
 

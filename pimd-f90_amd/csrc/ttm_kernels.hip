@@ -213,12 +213,8 @@ __global__ __launch_bounds__(64 * MONO_W) void k_monomer(const Rep *__restrict__
 // M-site share folded in, times the H positions relative to the M site
 // (TTM3-F, q-SPC/Fw) or to the O (TTM2.1-F, SPC/F), minimum image.
 // One thread per (bead, molecule); RR [P][3N][3], out [P][N][3].
-__global__ __launch_bounds__(64) void k_monomer_dipoles(SysConst c, const double *__restrict__ pes,
-                                                        const double *__restrict__ RR,
-                                                        double *__restrict__ out) {
-  const int g = blockIdx.x * 64 + threadIdx.x;
-  if (g >= c.P * c.N) return;
-  const double *rr = RR + (size_t)g * 9;
+__device__ inline void monomer_dipole(const SysConst &c, const double *__restrict__ pes,
+                                      const double *__restrict__ rr, double *__restrict__ out) {
   double r[3][3];
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 3; ++k) r[a][k] = rr[3 * a + k];
@@ -249,7 +245,39 @@ __global__ __launch_bounds__(64) void k_monomer_dipoles(SysConst c, const double
     minimg(c, h1[0], h1[1], h1[2]);
     minimg(c, h2[0], h2[1], h2[2]);
   }
-  for (int k = 0; k < 3; ++k) out[(size_t)g * 3 + k] = qh1 * h1[k] + qh2 * h2[k];
+  for (int k = 0; k < 3; ++k) out[k] = qh1 * h1[k] + qh2 * h2[k];
+}
+
+__global__ __launch_bounds__(64) void k_monomer_dipoles(SysConst c, const double *__restrict__ pes,
+                                                        const double *__restrict__ RR,
+                                                        double *__restrict__ out) {
+  const int g = blockIdx.x * 64 + threadIdx.x;
+  if (g >= c.P * c.N) return;
+  monomer_dipole(c, pes, RR + (size_t)g * 9, out + (size_t)g * 3);
+}
+
+// The same for every replica of a group, from its coordinates RR into its
+// dipI (contracted MD's bead dipoles, which its force call does not make).
+__global__ __launch_bounds__(64) void k_monomer_dipoles_rep(const Rep *__restrict__ reps,
+                                                            BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const int g = bk.x * 64 + threadIdx.x;
+  if (g >= c.P * c.N) return;
+  monomer_dipole(c, Rp.b.pes, Rp.b.RR + (size_t)g * 9, Rp.b.dipI + (size_t)g * 3);
+}
+
+void launch_monomer_dipoles(const Group &g, hipStream_t s) {
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        const bool ttm = c.model == PIMD_TTM21F || c.model == PIMD_TTM3F;
+        return Plan{ttm ? 0 : -1, (int)(((long)c.P * c.N + 63) / 64), 1, 1, 0};
+      },
+      [&](int, dim3 grid, const BlockMap &m, size_t) {
+        hipLaunchKernelGGL(k_monomer_dipoles_rep, grid, dim3(64), 0, s, g.d, m);
+      });
 }
 
 void launch_monomer_dipoles(const SysConst &c, const double *pes, const double *RR, double *out,
@@ -2188,7 +2216,8 @@ void launch_monomer(const Group &g, hipStream_t s) {
   launch_over(
       g,
       [](const SysConst &c, const MDConst &) {
-        return Plan{is_ttm(c) ? 0 : -1, idiv((long)c.P * c.N, 64), 1, 1, 0};
+        const bool ttm = c.model == PIMD_TTM21F || c.model == PIMD_TTM3F;
+        return Plan{ttm ? 0 : -1, (int)(((long)c.P * c.N + 63) / 64), 1, 1, 0};
       },
       [&](int, dim3 grid, const BlockMap &m, size_t) {
         hipLaunchKernelGGL(k_monomer, grid, dim3(64 * MONO_W), 0, s, g.d, m);
@@ -2466,7 +2495,8 @@ void launch_post_qq(const Group &g, hipStream_t s) {
   launch_over(
       g,
       [](const SysConst &c, const MDConst &) {
-        return Plan{is_ttm(c) ? 0 : -1, idiv((long)c.P * c.N, 64), 1, 1, 0};
+        const bool ttm = c.model == PIMD_TTM21F || c.model == PIMD_TTM3F;
+        return Plan{ttm ? 0 : -1, (int)(((long)c.P * c.N + 63) / 64), 1, 1, 0};
       },
       [&](int, dim3 grid, const BlockMap &m, size_t) {
         hipLaunchKernelGGL(k_post_qq, grid, dim3(64), 0, s, g.d, m);

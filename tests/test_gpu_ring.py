@@ -9,7 +9,7 @@ import pytest
 
 from conftest import golden_names, load_golden, params_from
 from helpers import max_abs, rel_err
-from pimd_amd.systems import HBAR, KB_AMUA2PS2PERK, atom_masses
+from pimd_amd.systems import CHARGECON, DEBYE, HBAR, KB_AMUA2PS2PERK, atom_masses
 
 pytestmark = pytest.mark.gpu
 
@@ -140,6 +140,15 @@ def test_contracted_vs_reference(name):
     assert errs["Upot"] < 1e-10 and errs["uk"] < 1e-10
     assert errs["virial"] < 1e-9 and errs["virialc"] < 1e-9
     assert errs["R"] < 1e-12 and errs["P"] < 1e-10 and errs["dR"] < 1e-8 and errs["Rc"] < 1e-12
+    # the step's dipole observables: the monomer dipoles of every bead
+    # (calc_monomer_dip_moments after the centroid call, MD.f90:407) averaged
+    # over beads per molecule, in Debye, at the last step's coordinates
+    obs = integ.observables(nsteps)
+    mono = ff.monomer_dipoles(st["RRt"])
+    v = mono.mean(axis=0) * (DEBYE / CHARGECON)
+    want = np.concatenate([v.sum(axis=0), [np.sum(np.linalg.norm(v, axis=1))]])
+    assert rel_err(obs[-1, 5:], want) < 1e-12
+    assert np.all(np.isfinite(obs))
     # a second run continues from the state, the fast gradient included
     rec2 = integ.run(1, t0=nsteps + 1)
     assert np.all(np.isfinite(rec2["Upot"]))
