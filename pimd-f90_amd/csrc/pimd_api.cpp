@@ -486,11 +486,22 @@ void plan_system(pimd_ctx *x, HostTables &t) {
     return g;
   };
   const int dflt_qq = pow2_clamp(131072 / std::max(mb, 1L), 8, 32);
-  // the FP32 dipole pass runs lighter lanes: twice the lanes per molecule
-  // (512 H2O x 32 beads: 32 instead of 16, 1.685 -> 1.657 ms per force call);
-  // TTM2.1-F's three dipole sites at most 32 (128 x 32: 1.50 -> 1.37 ms FP64)
-  const int dflt_dd = pow2_clamp((c.mixed ? 524288 : 262144) / std::max(mb, 1L), 16,
-                                 c.model == 2 ? 32 : 64);
+  // The dipole pairs' screening factors stay fixed through the SOR solve:
+  // kept from the first iteration up to PIMD_DD_CACHE_MB (default 4096) per
+  // context, recomputed every iteration beyond (512 H2O x 32 beads, dipole
+  // pass per step: FP64 0.356 -> 0.251 ms, mixed 0.238 -> 0.209 ms; TTM2.1-F
+  // 128 x 32: 0.438 -> 0.282 ms, 0.316 -> 0.264 ms)
+  const double cache_mb = (double)dd_cache_elems(c) * (c.mixed ? 8 : 16) / (1 << 20);
+  c.dd_cache = (c.model == PIMD_TTM21F || c.model == PIMD_TTM3F) &&
+               env_int("PIMD_DD_CACHE", 1, 0, 1) &&
+               cache_mb <= env_int("PIMD_DD_CACHE_MB", 4096, 0, 1 << 20);
+  // lanes per molecule of the dipole pass: 16 from 16384 molecule-beads on;
+  // an FP32 pass that computes its factors runs lighter lanes, twice as many
+  // (512 H2O x 32 beads: 1.685 -> 1.657 ms per force call); TTM2.1-F's three
+  // dipole sites at most 32 (128 x 32: 1.50 -> 1.37 ms FP64), 16 with the
+  // factors kept (0.312 -> 0.282 ms per step)
+  const int dflt_dd = pow2_clamp((c.mixed && !c.dd_cache ? 524288 : 262144) / std::max(mb, 1L), 16,
+                                 c.model == 2 ? (c.dd_cache ? 16 : 32) : 64);
   // structure-factor site chunks: about 8 per bead from 4096 molecule-beads
   // on (with two lane sets per block; 512 H2O x 32 beads: 1.66 -> 1.61 ms per
   // mixed force call, TTM2.1-F 128 x 32: 1.75 -> 1.66 ms FP64), the staged
@@ -557,6 +568,11 @@ void layout_system(pimd_ctx *x, Arena &a, bool full) {
   a.take(&d.tab, P * (size_t)c.T * S);
   a.take(&d.tabd, P * (size_t)c.T * N * nd);
   if (c.mixed) a.take(&d.tabdf, P * (size_t)c.T * N * nd);
+  if (c.dd_cache) {
+    const size_t n = dd_cache_elems(c);
+    a.take(&d.ddts, c.mixed ? (n + 1) / 2 : n);
+    d.ddtsf = (decltype(d.ddtsf))d.ddts;
+  }
   a.take(&d.Sq, P * nk * 2);
   a.take(&d.Sm, P * nk * 2);
   a.take(&d.Sv, P * nk * 6);
@@ -715,7 +731,7 @@ int run_forces(const Group &g, hipStream_t s, pimd_ctx *px) {
     for (int it = 1; it <= maxiter; ++it) {
       // with timing on: the beads this iteration works on, for per-launch work counts
       if (poll && prof) px->count("sor_bead_iterations", it == 1 ? px->c.P : px->h_nactive[it - 1]);
-      timed(px, "pair_dd", [&] { launch_pair_dd(g, false, s, fold && it > 1 ? it - 1 : 0); });
+      timed(px, "pair_dd", [&] { launch_pair_dd(g, false, s, fold && it > 1 ? it - 1 : 0, false, it); });
       timed(px, "sfac_d", [&] { launch_sfac(g, SF_D, s, false, 1); });
       timed(px, "sfac_d_reduce", [&] { launch_sfac(g, SF_D, s, false, 2); });
       timed(px, "field_d_sor", [&] { launch_field(g, FD_D, it, s); });
@@ -2281,9 +2297,12 @@ extern "C" int pimd_forces_all_beads_noalloc(const pimd_params *params, const do
   if ((rc = setup_system(&x)) != PIMD_OK) return rc;
   HostTables tabs;
   plan_system(&x, tabs);
-  // one stream and no host polling: nothing here creates streams or pinned memory
+  // one stream and no host polling: nothing here creates streams or pinned
+  // memory; the dipole pass recomputes its pair screening factors every SOR
+  // iteration (the same values as the cache would hold)
   x.c.poll = 0;
   x.c.overlap = 0;
+  x.c.dd_cache = 0;
   // partial sums for the chunks the structure factors use (the same
   // chunking as a context's, so the results are the same bits)
   x.c.sf_chunks_max = sfac_chunks_used(x.c);

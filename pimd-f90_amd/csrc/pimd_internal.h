@@ -71,6 +71,7 @@ struct SysConst {
   int g_nolds;             // lanes per molecule of the pair kernels that do not stage in LDS
   int poll;                // 1: read the active-bead count back after every SOR iteration
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
+  int dd_cache;            // 1: the SOR dipole pass keeps each pair's screening factors
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
@@ -124,6 +125,11 @@ struct DevBufs {
   dptr<double2> tab = nullptr;  // [P][T][4N]
   dptr<double2> tabd = nullptr;  // [P][T][nd N] the dipole sites' rows
   dptr<float2> tabdf = nullptr;  // the same in FP32 (mixed form)
+  // screening factors (ts1, ts2) of every dipole-site pair of the neighbour
+  // lists, [P][N][cap][nd*nd], written by the first SOR iteration and read by
+  // the others: FP64 (ddts) or, in the mixed form, FP32 (ddtsf, same memory)
+  dptr<double2> ddts = nullptr;
+  dptr<float2> ddtsf = nullptr;
   dptr<double> Sq = nullptr, Sm = nullptr;  // [P][nk][2]
   dptr<double> Sv = nullptr;  // [P][nk][6]
   dptr<double> virk = nullptr;  // [P][nkb][9]
@@ -318,10 +324,16 @@ void launch_field(const Group &g, int mode, int iter, hipStream_t s, bool sel_mi
 void launch_post_qq(const Group &g, hipStream_t s);
 // test_iter > 0: the kernel first runs the convergence test of SOR iteration
 // test_iter (in place of launch_converge); only where pair_dd_tests(c) holds
-// for every replica.
+// for every replica.  sor_iter: the SOR iteration (1 writes the pair
+// screening factors of replicas with dd_cache, later ones read them), 0 for
+// the FP64 refinement, which computes its own.
 void launch_pair_dd(const Group &g, bool fp64, hipStream_t s, int test_iter = 0,
-                    bool sel_mixed = false);
+                    bool sel_mixed = false, int sor_iter = 0);
 bool pair_dd_tests(const SysConst &c);
+// (ts1, ts2) pairs of the SOR dipole pass's screening-factor cache
+inline size_t dd_cache_elems(const SysConst &c) {
+  return (size_t)c.P * c.N * (size_t)c.nbr_cap * c.nd * c.nd;
+}
 void launch_refine_prep(const Group &g, hipStream_t s);
 void launch_pair_final(const Group &g, hipStream_t s);
 void launch_virk(const Group &g, hipStream_t s);

@@ -98,22 +98,33 @@ __device__ __forceinline__ void walk_list(const int *__restrict__ list, int firs
   }
 }
 
-// walk_list with D indices in flight: for visits too short to cover one
-// global load, the next D indices are requested ahead.  Same visiting order.
-template <int G, int D, class F>
-__device__ __forceinline__ void walk_list_deep(const int *__restrict__ list, int first, int end,
-                                               F &&f) {
-  int q[D];
+// walk_list with D entries in flight, for visits too short to cover one
+// global load: load(at) fetches entry `at` of the list (the partner index and
+// what the visit reads with it) D entries ahead; f(entry, at) visits them in
+// walk_list's order.
+template <int G, int D, class L, class F>
+__device__ __forceinline__ void walk_deep(int first, int end, L &&load, F &&f) {
+  using E = decltype(load(0));
+  E q[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) q[d] = first + d * G < end ? list[first + d * G] : 0;
+  for (int d = 0; d < D; ++d)
+    if (first + d * G < end) q[d] = load(first + d * G);
   for (int n = first; n < end; n += D * G) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int j = q[d], at = n + d * G, ahead = at + D * G;
-      q[d] = ahead < end ? list[ahead] : 0;
-      if (at < end) f(j);
+      const E e = q[d];
+      const int at = n + d * G, ahead = at + D * G;
+      if (ahead < end) q[d] = load(ahead);
+      if (at < end) f(e, at);
     }
   }
+}
+
+// Makes v an opaque register value, so that the arithmetic that follows is
+// compiled the same whether v was just computed or loaded from memory.
+template <typename R>
+__device__ __forceinline__ void opaque(R &v) {
+  asm("" : "+v"(v));
 }
 
 __device__ __forceinline__ void vadd9(double *t, const double *v) {

@@ -1597,11 +1597,32 @@ __device__ bool sor_test(const SysConst &c, const double *__restrict__ diff2, in
   return !(conv || last);
 }
 
-template <int MODEL, int G, typename R>
+// (ts1, ts2) of one dipole-site pair in the SOR arithmetic type, and the
+// context's cache of them
+template <typename R>
+struct TsPair;
+template <>
+struct TsPair<double> {
+  using T = double2;
+  static __device__ dptr<double2> cache(const DevBufs &b) { return b.ddts; }
+};
+template <>
+struct TsPair<float> {
+  using T = float2;
+  static __device__ dptr<float2> cache(const DevBufs &b) { return b.ddtsf; }
+};
+
+// CACHED: the pair screening factors come from the cache the first SOR
+// iteration wrote (store != 0 there); otherwise they are computed.  Either
+// way the field is the same, bit for bit.
+template <int MODEL, int G, typename R, bool CACHED>
 __global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(const Rep *__restrict__ reps,
-                                                                   BlockMap bm, int test_iter) {
+                                                                   BlockMap bm, int test_iter,
+                                                                   int store) {
   constexpr int ND = MODEL == 2 ? 3 : 1;
+  constexpr int NP = ND * ND;
   constexpr int T0 = MODEL == 2 ? 0 : 3;
+  using R2 = typename TsPair<R>::T;
   extern __shared__ double lds_dd[];  // [N][ND][DD_REC]: x, y, z, mu_x, mu_y, mu_z, pad
   __shared__ double red[256];
   __shared__ int act;
@@ -1648,34 +1669,75 @@ __global__ __launch_bounds__(dd_block<MODEL>()) void k_pair_dd_lds(const Rep *__
   for (int u = 0; u < ND; ++u) ef[u][0] = ef[u][1] = ef[u][2] = 0.0;
   const int nf = nbrc[grp], nn = nbrn[grp];
   const int *list = nbr + (size_t)grp * c.nbr_cap;
-  auto visit = [&](const int j, auto thole_tag) {
-    constexpr bool THOLE = decltype(thole_tag)::value;
-    const bool lo = a < j;
-#pragma unroll
-    for (int ua = 0; ua < ND; ++ua) {
-#pragma unroll
-      for (int ub = 0; ub < ND; ++ub) {
-        const int sa = T0 + ua, sb = T0 + ub;
-        const double *B = lds_dd + ((size_t)j * ND + ub) * DD_REC;
-        double v[3] = {A[ua][0] - B[0], A[ua][1] - B[1], A[ua][2] - B[2]};
-        minimg(c, v[0], v[1], v[2]);
-        const double d2 = norm2_exact(v[0], v[1], v[2]);
-        R ts1, ts2;
-        smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa]), ts1, ts2);
-        const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
-        const R mr[3] = {(R)B[3], (R)B[4], (R)B[5]};
-        R o[3];
-        dd3_apply(vr, ts1, ts2, mr, o);
-        for (int k = 0; k < 3; ++k) ef[ua][k] += (double)o[k];
-      }
-    }
+  // this molecule's row of the cache: [cap][NP] (ts1, ts2)
+  R2 *tsrow = (R2 *)TsPair<R>::cache(Rp.b) + (size_t)grp * c.nbr_cap * NP;
+  // one dipole-site pair: the staged partner site B, the minimum-image
+  // vector v to it and the field of its dipole
+  auto apply = [&](const double *B, const double v[3], int ua, R ts1, R ts2) {
+    const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
+    const R mr[3] = {(R)B[3], (R)B[4], (R)B[5]};
+    R o[3];
+    dd3_apply(vr, ts1, ts2, mr, o);
+    for (int k = 0; k < 3; ++k) ef[ua][k] += (double)o[k];
   };
   // a TTM3-F visit is one short dipole-tensor product: keep DD_AHEAD
   // neighbour indices in flight (TTM2.1-F's nine products cover one load)
   constexpr int AHEAD = MODEL == 2 ? 1 : DD_AHEAD;
-  walk_list_deep<G, AHEAD>(list, lane, nn, [&](int j) { visit(j, std::true_type{}); });
-  walk_list_deep<G, AHEAD>(list, c.nbr_cap - nf + lane, c.nbr_cap,
-                           [&](int j) { visit(j, std::false_type{}); });
+  if constexpr (CACHED) {
+    struct E {
+      int j;
+      R2 t[NP];
+    };
+    auto load = [&](int at) {
+      E e;
+      e.j = list[at];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) e.t[p] = tsrow[(size_t)at * NP + p];
+      return e;
+    };
+    auto visit = [&](const E &e, int) {
+#pragma unroll
+      for (int ua = 0; ua < ND; ++ua)
+#pragma unroll
+        for (int ub = 0; ub < ND; ++ub) {
+          const double *B = lds_dd + ((size_t)e.j * ND + ub) * DD_REC;
+          double v[3] = {A[ua][0] - B[0], A[ua][1] - B[1], A[ua][2] - B[2]};
+          minimg(c, v[0], v[1], v[2]);
+          R2 t = e.t[ua * ND + ub];
+          opaque(t.x);
+          opaque(t.y);
+          apply(B, v, ua, t.x, t.y);
+        }
+    };
+    walk_deep<G, AHEAD>(lane, nn, load, visit);
+    walk_deep<G, AHEAD>(c.nbr_cap - nf + lane, c.nbr_cap, load, visit);
+  } else {
+    auto visit = [&](const int j, const int at, auto thole_tag) {
+      constexpr bool THOLE = decltype(thole_tag)::value;
+      const bool lo = a < j;
+#pragma unroll
+      for (int ua = 0; ua < ND; ++ua) {
+#pragma unroll
+        for (int ub = 0; ub < ND; ++ub) {
+          const int sa = T0 + ua, sb = T0 + ub;
+          const double *B = lds_dd + ((size_t)j * ND + ub) * DD_REC;
+          double v[3] = {A[ua][0] - B[0], A[ua][1] - B[1], A[ua][2] - B[2]};
+          minimg(c, v[0], v[1], v[2]);
+          const double d2 = norm2_exact(v[0], v[1], v[2]);
+          R ts1, ts2;
+          smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolDD[sa][sb] : c.apolDD[sb][sa]), ts1, ts2);
+          opaque(ts1);
+          opaque(ts2);
+          if (store) tsrow[(size_t)at * NP + ua * ND + ub] = R2{ts1, ts2};
+          apply(B, v, ua, ts1, ts2);
+        }
+      }
+    };
+    auto idx = [&](int at) { return list[at]; };
+    walk_deep<G, AHEAD>(lane, nn, idx, [&](int j, int at) { visit(j, at, std::true_type{}); });
+    walk_deep<G, AHEAD>(c.nbr_cap - nf + lane, c.nbr_cap, idx,
+                        [&](int j, int at) { visit(j, at, std::false_type{}); });
+  }
   for (int u = 0; u < ND; ++u)
     for (int k = 0; k < 3; ++k) ef[u][k] = group_sum<G>(ef[u][k]);
   if (lane == 0) {
@@ -2504,18 +2566,26 @@ void launch_post_qq(const Group &g, hipStream_t s) {
 }
 
 // Variant of the dipole pass: bit 0 TTM3-F, bit 1 FP32, bit 2 LDS-staged
-// (with the folded convergence test), bits 3.. the lanes-per-molecule index.
-template <int ML, typename R>
-static void launch_dd_v(bool lds_kernel, int gi, dim3 grid, const BlockMap &m, size_t lds,
-                        int test_iter, const Group &g, hipStream_t s) {
+// (with the folded convergence test), bits 3-5 the lanes-per-molecule index,
+// bits 6-7 the screening-factor cache (0 none, 1 write, 2 read).
+template <int ML, typename R, bool CACHED>
+static void launch_dd_lds(int gi, dim3 grid, const BlockMap &m, size_t lds, int test_iter,
+                          int store, const Group &g, hipStream_t s) {
   constexpr int B = dd_block<ML>();
+  switch (gi) {
+    case 1: hipLaunchKernelGGL((k_pair_dd_lds<ML, 8, R, CACHED>), grid, dim3(B), lds, s, g.d, m, test_iter, store); break;
+    case 3: hipLaunchKernelGGL((k_pair_dd_lds<ML, 32, R, CACHED>), grid, dim3(B), lds, s, g.d, m, test_iter, store); break;
+    case 4: hipLaunchKernelGGL((k_pair_dd_lds<ML, 64, R, CACHED>), grid, dim3(B), lds, s, g.d, m, test_iter, store); break;
+    default: hipLaunchKernelGGL((k_pair_dd_lds<ML, 16, R, CACHED>), grid, dim3(B), lds, s, g.d, m, test_iter, store); break;
+  }
+}
+
+template <int ML, typename R>
+static void launch_dd_v(bool lds_kernel, int gi, int cache, dim3 grid, const BlockMap &m,
+                        size_t lds, int test_iter, const Group &g, hipStream_t s) {
   if (lds_kernel) {
-    switch (gi) {
-      case 1: hipLaunchKernelGGL((k_pair_dd_lds<ML, 8, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
-      case 3: hipLaunchKernelGGL((k_pair_dd_lds<ML, 32, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
-      case 4: hipLaunchKernelGGL((k_pair_dd_lds<ML, 64, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
-      default: hipLaunchKernelGGL((k_pair_dd_lds<ML, 16, R>), grid, dim3(B), lds, s, g.d, m, test_iter); break;
-    }
+    if (cache == 2) launch_dd_lds<ML, R, true>(gi, grid, m, lds, test_iter, 0, g, s);
+    else launch_dd_lds<ML, R, false>(gi, grid, m, lds, test_iter, cache == 1, g, s);
     return;
   }
   switch (gi) {
@@ -2527,7 +2597,8 @@ static void launch_dd_v(bool lds_kernel, int gi, dim3 grid, const BlockMap &m, s
 
 // fp64 forces the FP64 form (the refinement pass of the mixed solve);
 // sel_mixed: only the mixed-form replicas.
-void launch_pair_dd(const Group &g, bool fp64, hipStream_t s, int test_iter, bool sel_mixed) {
+void launch_pair_dd(const Group &g, bool fp64, hipStream_t s, int test_iter, bool sel_mixed,
+                    int sor_iter) {
   launch_over(
       g,
       [&](const SysConst &c, const MDConst &) {
@@ -2536,19 +2607,24 @@ void launch_pair_dd(const Group &g, bool fp64, hipStream_t s, int test_iter, boo
         if (pair_dd_tests(c)) {
           const int G = c.g_dd == 64 ? 64 : c.g_dd == 32 ? 32 : c.g_dd == 8 ? 8 : 16;
           const int B = c.model == 2 ? dd_block<2>() : dd_block<3>();
-          return Plan{v0 | 4 | (gidx(G) << 3), idiv(c.N, B / G), c.P, 1, dd_lds(c)};
+          // the cache holds the factors of the SOR iterations' arithmetic
+          // type; the FP64 refinement of the mixed form computes its own
+          const int cache = !c.dd_cache || sor_iter <= 0 || (c.mixed && fp64) ? 0
+                            : sor_iter == 1                                 ? 1
+                                                                            : 2;
+          return Plan{v0 | 4 | (gidx(G) << 3) | (cache << 6), idiv(c.N, B / G), c.P, 1, dd_lds(c)};
         }
         const int G = c.g_nolds == 64 ? 64 : c.g_nolds == 32 ? 32 : 16;
         return Plan{v0 | (gidx(G) << 3), idiv((long)c.P * c.N * G, 256), 1, 1, 0};
       },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
         const bool lk = (v & 4) != 0;
-        const int gi = v >> 3;
+        const int gi = (v >> 3) & 7, cache = v >> 6;
         switch (v & 3) {
-          case 0: launch_dd_v<2, double>(lk, gi, grid, m, lds, test_iter, g, s); break;
-          case 1: launch_dd_v<3, double>(lk, gi, grid, m, lds, test_iter, g, s); break;
-          case 2: launch_dd_v<2, float>(lk, gi, grid, m, lds, test_iter, g, s); break;
-          default: launch_dd_v<3, float>(lk, gi, grid, m, lds, test_iter, g, s); break;
+          case 0: launch_dd_v<2, double>(lk, gi, cache, grid, m, lds, test_iter, g, s); break;
+          case 1: launch_dd_v<3, double>(lk, gi, cache, grid, m, lds, test_iter, g, s); break;
+          case 2: launch_dd_v<2, float>(lk, gi, cache, grid, m, lds, test_iter, g, s); break;
+          default: launch_dd_v<3, float>(lk, gi, cache, grid, m, lds, test_iter, g, s); break;
         }
       });
 }
