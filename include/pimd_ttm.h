@@ -294,6 +294,19 @@ int pimd_dms_coefs_set(pimd_ctx *ctx, const double *coef);
 int pimd_dms_coef_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
                        double *grad, double *E, double *dEdc);
 
+/* The M-site fraction gammaM (pot_mod.f90:12,24): the M site sits at
+ * r_O + gammaM (r_OH1 + r_OH2) / 2 and carries q_O / (1 - gammaM), the H
+ * sites gain gammaM / (2 (1 - gammaM)) (q_H1 + q_H2) (pot_ttm.f90:74-79,
+ * 134-137).  No counterpart in the reference, where it is a constant.  A
+ * context starts with its model's value; later force calls use the one set,
+ * which must lie in [0, 1). */
+int pimd_gamma_m_get(pimd_ctx *ctx, double *gamma_m);
+int pimd_gamma_m_set(pimd_ctx *ctx, double gamma_m);
+/* pimd_ttm_param_grad with respect to gammaM, the M sites moving with it:
+ * grad[1], and optionally E[P] and dEdg[P]. */
+int pimd_gamma_m_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
+                      double *grad, double *E, double *dEdg);
+
 /* ---- instrumentation --------------------------------------------------- */
 
 /* Enables per-kernel HIP-event timing on the context stream. */

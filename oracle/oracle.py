@@ -37,6 +37,9 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_get_ttm_params.argtypes = [C.c_void_p, dp]
         L.orc_set_ttm_params.argtypes = [C.c_void_p, dp]
+        L.orc_get_gamma_m.restype = C.c_double
+        L.orc_get_gamma_m.argtypes = [C.c_void_p]
+        L.orc_set_gamma_m.argtypes = [C.c_void_p, C.c_double]
         L.orc_nkvec.restype = C.c_int
         L.orc_nkvec.argtypes = [C.c_void_p]
         L.orc_consts.argtypes = [C.c_void_p, dp]
@@ -88,6 +91,12 @@ class System:
 
     def set_ttm_params(self, theta):
         lib().orc_set_ttm_params(self.h, np.ascontiguousarray(theta, dtype=np.float64))
+
+    def gamma_m(self):
+        return lib().orc_get_gamma_m(self.h)
+
+    def set_gamma_m(self, g):
+        lib().orc_set_gamma_m(self.h, float(g))
 
     def consts(self):
         o = np.zeros(11)

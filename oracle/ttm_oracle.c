@@ -465,6 +465,11 @@ void orc_set_ttm_params(orc_sys *s, const double *th) {
   derive_params(s);
 }
 
+/* The M-site fraction gammaM (pot_mod.f90:12,24), for finite-difference
+ * checks of its derivative. */
+double orc_get_gamma_m(const orc_sys *s) { return s->gammaM; }
+void orc_set_gamma_m(orc_sys *s, double g) { s->gammaM = g; }
+
 orc_sys *orc_create(const orc_params *p) {
   orc_sys *s = calloc(1, sizeof *s);
   s->p = *p;

@@ -24,6 +24,8 @@ orc_sys *orc_create(const orc_params *p);
 void orc_destroy(orc_sys *s);
 void orc_get_ttm_params(const orc_sys *s, double *theta16);
 void orc_set_ttm_params(orc_sys *s, const double *theta16);
+double orc_get_gamma_m(const orc_sys *s);
+void orc_set_gamma_m(orc_sys *s, double gamma_m);
 int orc_nkvec(const orc_sys *s);
 /* aewald, Uvdw_lrc, shiftA, shiftB, const_ts1..3, polfac[4] (10 doubles) */
 void orc_consts(const orc_sys *s, double *out);

@@ -2137,11 +2137,11 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
   return PIMD_OK;
 }
 
-// Tangent sweeps along ndir directions (the TTM parameters, or with dms_coef
-// the DMS coefficients): the loss gradient, and optionally each
+// Tangent sweeps along ndir directions (mode 0 the TTM parameters, 1 the DMS
+// coefficients, 2 gammaM): the loss gradient, and optionally each
 // configuration's energy and its derivatives.
 static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
-                        int dms_coef, int ndir, double *grad, double *E, double *dEdtheta) {
+                        int mode, int ndir, double *grad, double *E, double *dEdtheta) {
   if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
   if (x->p.contraction)
     return fail(PIMD_EINVAL, "parameter derivatives need a context with contraction = 0");
@@ -2179,7 +2179,8 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
   L.kv = d.kv;
   L.khkl = d.khkl;
   L.dG = nullptr;
-  L.dms_coef = dms_coef;
+  L.dms_coef = mode == 1;
+  L.gamma_m = mode == 2;
   L.ndir = ndir;
   std::copy(x->ttm, x->ttm + TTM_NPARAM, L.theta);
   const double Rc = x->p.rc, pre = TWOPI * (double)(c.N * c.N) / c.vol;
@@ -2189,7 +2190,8 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
   HIPCHK(hipMemcpyAsync(dRR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dgw, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dew, ew, sizeof(double) * P, hipMemcpyHostToDevice, s));
-  x->timed(dms_coef ? "dms_tangent" : "ttm_tangent", [&] { launch_ttm_tangent(c, L, s); });
+  static const char *const names[3] = {"ttm_tangent", "dms_tangent", "gamma_m_tangent"};
+  x->timed(names[mode], [&] { launch_ttm_tangent(c, L, s); });
   HIPCHK(hipGetLastError());
   std::vector<double> out(nout);
   HIPCHK(hipMemcpyAsync(out.data(), L.out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
@@ -2215,6 +2217,35 @@ extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double 
 extern "C" int pimd_dms_coef_grad(pimd_ctx *x, const double *RRt, const double *ew,
                                   const double *gw, double *grad, double *E, double *dEdc) {
   return tangent_grad(x, RRt, ew, gw, 1, DMS_NCOEF, grad, E, dEdc);
+}
+
+extern "C" int pimd_gamma_m_grad(pimd_ctx *x, const double *RRt, const double *ew,
+                                 const double *gw, double *grad, double *E, double *dEdg) {
+  return tangent_grad(x, RRt, ew, gw, 2, 1, grad, E, dEdg);
+}
+
+extern "C" int pimd_gamma_m_get(pimd_ctx *x, double *gamma_m) {
+  if (!x || !gamma_m) return fail(PIMD_EINVAL, "null argument");
+  *gamma_m = x->c.gammaM;
+  return PIMD_OK;
+}
+
+extern "C" int pimd_gamma_m_set(pimd_ctx *x, double gamma_m) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  if (x->p.pot_model >= PIMD_QSPCFW)
+    return fail(PIMD_EINVAL, "gammaM is a parameter of the TTM models (pot_model 2, 3)");
+  if (!(gamma_m >= 0.0 && gamma_m < 1.0))
+    return fail(PIMD_EINVAL, "gammaM must lie in [0, 1) (the M charge is q_O / (1 - gammaM))");
+  HIPCHK(hipSetDevice(x->p.device));
+  HIPCHK(hipStreamSynchronize(x->stream));  // launches already queued keep the old value
+  for (pimd_ctx *y : {x, x->cen}) {
+    if (!y) continue;
+    y->c.gammaM = gamma_m;
+    const int rc = sync_rep(y);
+    if (rc != PIMD_OK) return rc;
+  }
+  invalidate_graphs(x);  // captured launches hold the model constants
+  return PIMD_OK;
 }
 
 extern "C" int pimd_dms_coefs_get(pimd_ctx *x, double *coef) {

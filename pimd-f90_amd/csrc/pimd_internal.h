@@ -380,6 +380,7 @@ struct TanLaunch {
   double theta[TTM_NPARAM];
   double lrc_coef[3];
   int dms_coef;  // 0: the TTM parameters (ndir = TTM_NPARAM), 1: the DMS coefficients
+  int gamma_m;   // 1: the one direction is the M-site fraction gammaM (moves the M sites)
   int ndir;
 };
 size_t tangent_cfg_doubles(const SysConst &c);

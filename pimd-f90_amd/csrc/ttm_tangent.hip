@@ -15,13 +15,18 @@
 // polarM, polfacO, polfacH, polfacM, aDD, aCCaCD, dms_param1..3
 // (potential.f90:84-113, 152-160).  Geometry (sites, k-space phases) does not
 // depend on them and stays plain FP64; charges, dipoles, fields, potentials
-// and the screened interaction tensors carry the derivative.
+// and the screened interaction tensors carry the derivative.  The M-site
+// fraction gammaM (pot_mod.f90:12,24) also moves the M sites (pot_ttm.f90:
+// 74-79): its direction runs an instance whose site positions and phases are
+// dual numbers too (GD = true), with dM/dgammaM = (r_OH1 + r_OH2) / 2.
 //
 // Work layout: grid (configuration, direction), one 256-lane workgroup each;
 // lanes stride over sites, k-vectors or molecules per phase, each lane owns
 // what it writes (every site-pair term is evaluated from both ends), so no
 // atomics.  Per configuration a preparatory kernel writes the sites and the
 // per-axis phase tables once for all directions.
+#include <type_traits>
+
 #include "dual.h"
 #include "ttm_device.h"
 
@@ -41,24 +46,50 @@ constexpr int TAN_BLOCK = 256;
 
 // Per-configuration FP64 data: sites [4N][3] (O, H1, H2, M blocks, as
 // pot_ttm.f90:56-80), then the phase tables exp(2 pi i m x / L) per axis,
-// [m][site] as (cos, sin): m = 0..hmax, -kmax..kmax, -lmax..lmax.
+// [m][site] as (cos, sin): m = 0..hmax, -kmax..kmax, -lmax..lmax; then the
+// M sites' derivatives with respect to gammaM, [N][3].
+static size_t tangent_tab_doubles(const SysConst &c) {
+  const size_t ns = 4 * (size_t)c.N;
+  return 2 * ns * (size_t)((c.hmax + 1) + (2 * c.kmax + 1) + (2 * c.lmax + 1));
+}
 size_t tangent_cfg_doubles(const SysConst &c) {
   const size_t ns = 4 * (size_t)c.N;
-  return ns * 3 + 2 * ns * (size_t)((c.hmax + 1) + (2 * c.kmax + 1) + (2 * c.lmax + 1));
+  return ns * 3 + tangent_tab_doubles(c) + 3 * (size_t)c.N;
 }
 
 // Per (configuration, direction) dual workspace: charge, phi [4N], Efq, dR
 // [4N][3], dip, olddip, Efd [nd N][3], grdq [N][27], S(k) of the charges and
-// of the dipoles [nk][2].
+// of the dipoles [nk][2], the monomer energy [1].
 size_t tangent_ws_duals(const SysConst &c) {
   const size_t ns = 4 * (size_t)c.N, nds = (size_t)c.nd * c.N;
-  return 8 * ns + 9 * nds + 27 * (size_t)c.N + 4 * (size_t)c.nk;
+  return 8 * ns + 9 * nds + 27 * (size_t)c.N + 4 * (size_t)c.nk + 1;
 }
 
+// The workspace arrays of one (configuration, direction).
+struct TanWs {
+  Dual *q, *phi, *Efq, *dR, *dip, *old, *Efd, *grdq, *Sq, *Sm, *umon;
+  __device__ TanWs(Dual *base, const SysConst &c) {
+    const int ns = 4 * c.N, nds = c.nd * c.N;
+    q = base;
+    phi = q + ns;
+    Efq = phi + ns;
+    dR = Efq + 3 * ns;
+    dip = dR + 3 * ns;
+    old = dip + 3 * nds;
+    Efd = old + 3 * nds;
+    grdq = Efd + 3 * nds;
+    Sq = grdq + 27 * c.N;
+    Sm = Sq + 2 * c.nk;
+    umon = Sm + 2 * c.nk;
+  }
+};
+
 __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_prep(SysConst c, const double *__restrict__ RR,
-                                                           double *__restrict__ cfg, size_t stride) {
+                                                           double *__restrict__ cfg, size_t stride,
+                                                           size_t dm_off) {
   const int b = blockIdx.x, N = c.N, ns = 4 * N;
   double *Rs = cfg + (size_t)b * stride;
+  double *dM = Rs + dm_off;
   const double *rr = RR + (size_t)b * 9 * N;
   const double gM = c.gammaM;
   for (int w = threadIdx.x; w < N; w += TAN_BLOCK) {
@@ -72,7 +103,10 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_prep(SysConst c, const do
     }
     minimg(c, roh1[0], roh1[1], roh1[2]);
     minimg(c, roh2[0], roh2[1], roh2[2]);
-    for (int k = 0; k < 3; ++k) Rs[(3 * N + w) * 3 + k] = 0.5 * gM * (roh1[k] + roh2[k]) + rr[9 * w + k];
+    for (int k = 0; k < 3; ++k) {
+      Rs[(3 * N + w) * 3 + k] = 0.5 * gM * (roh1[k] + roh2[k]) + rr[9 * w + k];
+      dM[3 * w + k] = 0.5 * (roh1[k] + roh2[k]);
+    }
   }
   __syncthreads();
   double2 *tab = (double2 *)(Rs + (size_t)ns * 3);
@@ -111,6 +145,8 @@ struct TanArgs {
   double theta[TTM_NPARAM];
   double lrc_coef[3];    // dUvdw_lrc / d(vdwA, vdwB, vdwC)
   int dms_coef;          // 1: the directions are the DMS coefficients coefD
+  int gamma_m;           // 1: the one direction is gammaM
+  size_t dm_off;         // offset of dM/dgammaM [N][3] in a configuration's data
 };
 
 // The derivative of the DMS charges and their gradients with respect to
@@ -247,7 +283,8 @@ __device__ inline void dms_coef_tangent(const SysConst &c, const double r[3][3],
 // ewald.f90:403-437 and ewald.f90:20-33).
 struct Phases {
   const double2 *tx, *ty, *tz;
-  int ns, kmax, lmax;
+  const double *dM;  // M sites' dr/dgammaM [N][3]
+  int ns, kmax, lmax, N;
   __device__ __forceinline__ void at(int s, int4 h, double &ck, double &sk) const {
     const double2 a = tx[(size_t)h.x * ns + s], b = ty[(size_t)(h.y + kmax) * ns + s],
                   z = tz[(size_t)(h.z + lmax) * ns + s];
@@ -255,7 +292,52 @@ struct Phases {
     ck = cxy * z.x - sxy * z.y;
     sk = sxy * z.x + cxy * z.y;
   }
+  // with moving M sites: d exp(i k.r)/dgammaM = i (k . dr/dgammaM) exp(i k.r)
+  __device__ __forceinline__ void at(int s, int4 h, const double4 &kv, Dual &ck, Dual &sk) const {
+    double c, sn;
+    at(s, h, c, sn);
+    double kd = 0.0;
+    if (s >= 3 * N) {
+      const double *d = dM + 3 * (s - 3 * N);
+      kd = kv.x * d[0] + kv.y * d[1] + kv.z * d[2];
+    }
+    ck = Dual(c, -kd * sn);
+    sk = Dual(sn, kd * c);
+  }
+  __device__ __forceinline__ void at(int s, int4 h, const double4 &, double &ck, double &sk) const {
+    at(s, h, ck, sk);
+  }
 };
+
+// Geometry of the GD = true instance: minimum image on the value part (the
+// box shift is constant along gammaM).
+__device__ __forceinline__ void minimg(const SysConst &c, Dual &x, Dual &y, Dual &z) {
+  minimg(c, x.v, y.v, z.v);
+}
+__device__ __forceinline__ Dual as_dual(Dual x) { return x; }
+__device__ __forceinline__ Dual as_dual(double x) { return Dual(x); }
+
+// self1 / self2 (smear.f90:110-156) of a dual distance, the same steps as
+// those of the plain ones.
+__device__ __forceinline__ void self1(const SysConst &c, Dual drsq, Dual &ts0, Dual &ts1) {
+  const Dual dd = dsqrt(drsq), drsqi = Dual(1.0) / drsq, dri = dsqrt(drsqi);
+  const Dual expon = dexp(-(c.a2 * drsq));
+  const Dual kr0 = erfc_as<Dual>(c.aewald * dd, expon) * dri;
+  const Dual kr1 = (kr0 + c.cts1 * expon) * drsqi;
+  const Dual ch0 = -dri, ch1 = ch0 * drsqi;
+  ts0 = ch0 + kr0;
+  ts1 = kr1 + ch1;
+}
+__device__ __forceinline__ void self2(const SysConst &c, Dual drsq, Dual &ts1, Dual &ts2) {
+  const Dual dd = dsqrt(drsq), drsqi = Dual(1.0) / drsq, dri = dsqrt(drsqi);
+  const Dual expon = dexp(-(c.a2 * drsq));
+  const Dual kr0 = erfc_as<Dual>(c.aewald * dd, expon) * dri;
+  const Dual kr1 = (kr0 + c.cts1 * expon) * drsqi;
+  const Dual kr2 = (kr1 + c.cts2 * expon) * drsqi;
+  const Dual ch0 = -dri, ch1 = ch0 * drsqi, ch2 = ch1 * drsqi;
+  ts1 = kr1 + ch1;
+  ts2 = kr2 + ch2;
+}
 
 __device__ __forceinline__ double block_sum(double v, double *red) {
   red[threadIdx.x] = v;
@@ -288,56 +370,30 @@ __device__ __forceinline__ Dual group_sum(Dual x, int G) {
 }
 
 // (3 t2 r r^T - t1) v, get_dd3 applied (pot_ttm.f90:633-647).
-__device__ __forceinline__ void dd3d(const double r[3], Dual t1, Dual t2, const Dual v[3], Dual out[3]) {
+template <class G>
+__device__ __forceinline__ void dd3d(const G r[3], Dual t1, Dual t2, const Dual v[3], Dual out[3]) {
   const Dual t = 3.0 * t2;
   const Dual rv = v[0] * r[0] + v[1] * r[1] + v[2] * r[2];
   for (int k = 0; k < 3; ++k) out[k] = t * r[k] * rv - t1 * v[k];
 }
 
-__global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a) {
-  const int b = blockIdx.x, dir = blockIdx.y, ndir = gridDim.y, tid = threadIdx.x;
-  const int N = c.N, ns = 4 * N, nd = c.nd, tdip = c.tdip, nds = nd * N, nk = c.nk;
-  const int tend = tdip + nd;  // dipole site types [tdip, tend): M (TTM3-F) or O, H1, H2 (TTM2.1-F)
-  const double *Rs = a.cfg + (size_t)b * a.cfg_stride;
-  Phases ph;
-  ph.ns = ns;
-  ph.kmax = c.kmax;
-  ph.lmax = c.lmax;
-  ph.tx = (const double2 *)(Rs + (size_t)ns * 3);
-  ph.ty = ph.tx + (size_t)(c.hmax + 1) * ns;
-  ph.tz = ph.ty + (size_t)(2 * c.kmax + 1) * ns;
-  Dual *q = a.ws + ((size_t)b * ndir + dir) * a.ws_stride;
-  Dual *phi = q + ns, *Efq = phi + ns, *dR = Efq + 3 * ns, *dip = dR + 3 * ns;
-  Dual *old = dip + 3 * nds, *Efd = old + 3 * nds, *grdq = Efd + 3 * nds, *Sq = grdq + 27 * N;
-  Dual *Sm = Sq + 2 * nk;
-
-  __shared__ TanPar P;
+// The monomer phase of the tangent evaluation (pot_ttm.f90:84-151): DMS
+// charges and their coordinate gradients with their derivatives along the
+// direction, the PES gradient and the monomer energy of the configuration,
+// into the (configuration, direction) workspace.  A kernel of its own: the
+// PES and DMS polynomials hold more registers than any later phase.
+template <bool GD>
+__global__ __launch_bounds__(TAN_BLOCK) void k_tangent_monomer(SysConst c, TanArgs a) {
+  using GT = std::conditional_t<GD, Dual, double>;
+  const int b = blockIdx.x, dir = blockIdx.y, ndir = gridDim.y, tid = threadIdx.x, N = c.N;
+  const TanWs W(a.ws + ((size_t)b * ndir + dir) * a.ws_stride, c);
+  Dual *q = W.q, *phi = W.phi, *Efq = W.Efq, *dR = W.dR, *grdq = W.grdq;
   __shared__ double red[TAN_BLOCK];
-  if (tid == 0) {
-    auto par = [&](int i) { return Dual(a.theta[i], (!a.dms_coef && i == dir) ? 1.0 : 0.0); };
-    for (int i = 0; i < 5; ++i) P.vdw[i] = par(i);
-    P.polar[0] = par(5);
-    P.polar[1] = par(6);
-    P.polar[2] = par(6);
-    P.polar[3] = par(7);
-    // polfac = (polfac^(1/6))^-3 (potential.f90:157-160)
-    const Dual f6[4] = {dpow(par(8), 1.0 / 6.0), dpow(par(9), 1.0 / 6.0), dpow(par(9), 1.0 / 6.0),
-                        dpow(par(10), 1.0 / 6.0)};
-    Dual pf[4];
-    for (int i = 0; i < 4; ++i) pf[i] = Dual(1.0) / (f6[i] * f6[i] * f6[i]);
-    const Dual aDD = par(11), aCC = par(12);
-    for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 4; ++j) {
-        P.apCC[i][j] = aCC * pf[i] * pf[j];
-        P.raCC[i][j] = P.apCC[i][j].v > 0.0 ? dpow(P.apCC[i][j], 1.0 / 3.0) : Dual(0.0);
-        P.apDD[i][j] = aDD * pf[i] * pf[j];
-      }
-    P.lrc = Dual(c.Uvdw_lrc, (!a.dms_coef && dir < 3) ? a.lrc_coef[dir] : 0.0);
-  }
-  __syncthreads();
-  const double gM = c.gammaM, tmpq = 0.5 * gM / (1.0 - gM);
+  // gammaM and the charge placement factors (pot_ttm.f90:134-137)
+  GT gM = c.gammaM;
+  if constexpr (GD) gM.d = 1.0;
+  const GT tmpq = 0.5 * gM / (1.0 - gM), qMf = GT(1.0) / (1.0 - gM);
 
-  // ---- monomers: DMS charges, charge gradients, PES gradient (pot_ttm.f90:84-151)
   double umon = 0.0;
   for (int w = tid; w < N; w += TAN_BLOCK) {
     double r[3][3], e1, dr1[3][3], q3[3], gq[3][3][3], dq3[3] = {0, 0, 0}, dgq[3][3][3] = {};
@@ -347,13 +403,13 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
     ps_monomer<true>(c, r, e1, dr1, q3, gq, a.pes);
     umon += e1;
     if (a.dms_coef) dms_coef_tangent(c, r, dir, dq3, dgq);
-    else if (dir >= TP_DMS1) dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
+    else if (!a.gamma_m && dir >= TP_DMS1) dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
     Dual qd[3];
     for (int i = 0; i < 3; ++i) qd[i] = Dual(q3[i] * CHARGECON, dq3[i] * CHARGECON);
     q[w] = Dual(0.0);
     q[N + w] = qd[1] + tmpq * (qd[1] + qd[2]);
     q[2 * N + w] = qd[2] + tmpq * (qd[1] + qd[2]);
-    q[3 * N + w] = qd[0] / (1.0 - gM);
+    q[3 * N + w] = qd[0] * qMf;
     Dual *g = grdq + 27 * w;
     for (int d = 0; d < 3; ++d)
       for (int k = 0; k < 3; ++k) {
@@ -375,13 +431,77 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       for (int k = 0; k < 3; ++k) Efq[(t * N + w) * 3 + k] = Dual(0.0);
     }
   }
+  const double u = block_sum(umon, red);
+  if (tid == 0) *W.umon = Dual(u);
+}
+
+// GD: the gammaM instance, with dual site positions (GT = Dual) for the M
+// sites and dual phases; otherwise the geometry is plain FP64 (GT = double).
+template <bool GD>
+__global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a) {
+  using GT = std::conditional_t<GD, Dual, double>;
+  const int b = blockIdx.x, dir = blockIdx.y, ndir = gridDim.y, tid = threadIdx.x;
+  const int N = c.N, ns = 4 * N, nd = c.nd, tdip = c.tdip, nds = nd * N, nk = c.nk;
+  const int tend = tdip + nd;  // dipole site types [tdip, tend): M (TTM3-F) or O, H1, H2 (TTM2.1-F)
+  const double *Rs = a.cfg + (size_t)b * a.cfg_stride;
+  Phases ph;
+  ph.ns = ns;
+  ph.kmax = c.kmax;
+  ph.lmax = c.lmax;
+  ph.N = N;
+  ph.dM = Rs + a.dm_off;
+  ph.tx = (const double2 *)(Rs + (size_t)ns * 3);
+  ph.ty = ph.tx + (size_t)(c.hmax + 1) * ns;
+  ph.tz = ph.ty + (size_t)(2 * c.kmax + 1) * ns;
+  // site s's position, with its derivative along gammaM in the GD instance
+  auto site = [&](int s, GT r[3]) {
+    for (int k = 0; k < 3; ++k) r[k] = Rs[3 * s + k];
+    if constexpr (GD)
+      if (s >= 3 * N)
+        for (int k = 0; k < 3; ++k) r[k].d = ph.dM[3 * (s - 3 * N) + k];
+  };
+  const TanWs W(a.ws + ((size_t)b * ndir + dir) * a.ws_stride, c);
+  Dual *q = W.q, *phi = W.phi, *Efq = W.Efq, *dR = W.dR, *dip = W.dip, *old = W.old,
+       *Efd = W.Efd, *grdq = W.grdq, *Sq = W.Sq, *Sm = W.Sm;
+
+  __shared__ TanPar P;
+  __shared__ double red[TAN_BLOCK];
+  if (tid == 0) {
+    auto par = [&](int i) {
+      return Dual(a.theta[i], (!a.dms_coef && !a.gamma_m && i == dir) ? 1.0 : 0.0);
+    };
+    for (int i = 0; i < 5; ++i) P.vdw[i] = par(i);
+    P.polar[0] = par(5);
+    P.polar[1] = par(6);
+    P.polar[2] = par(6);
+    P.polar[3] = par(7);
+    // polfac = (polfac^(1/6))^-3 (potential.f90:157-160)
+    const Dual f6[4] = {dpow(par(8), 1.0 / 6.0), dpow(par(9), 1.0 / 6.0), dpow(par(9), 1.0 / 6.0),
+                        dpow(par(10), 1.0 / 6.0)};
+    Dual pf[4];
+    for (int i = 0; i < 4; ++i) pf[i] = Dual(1.0) / (f6[i] * f6[i] * f6[i]);
+    const Dual aDD = par(11), aCC = par(12);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        P.apCC[i][j] = aCC * pf[i] * pf[j];
+        P.raCC[i][j] = P.apCC[i][j].v > 0.0 ? dpow(P.apCC[i][j], 1.0 / 3.0) : Dual(0.0);
+        P.apDD[i][j] = aDD * pf[i] * pf[j];
+      }
+    P.lrc = Dual(c.Uvdw_lrc, (!a.dms_coef && !a.gamma_m && dir < 3) ? a.lrc_coef[dir] : 0.0);
+  }
   __syncthreads();
+  // gammaM, which redistributes the M-site gradient (pot_ttm.f90:597-602)
+  GT gM = c.gammaM;
+  if constexpr (GD) gM.d = 1.0;
+  // charges, their gradients, the monomer gradients and energy: k_tangent_monomer
 
   // ---- real space: vdW and charge-charge terms (pot_ttm.f90:159-228), per site
   Dual uvdw = 0.0;
   for (int s = tid; s < ns; s += TAN_BLOCK) {
     const int iw = s % N, ti = s / N;
-    const double *Ri = Rs + 3 * s, *ROi = Rs + 3 * iw;
+    const double *ROi = Rs + 3 * iw;
+    GT Ri[3];
+    site(s, Ri);
     Dual pv = 0.0, ef[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
     for (int jw = 0; jw < N; ++jw) {
       if (jw == iw) continue;
@@ -400,15 +520,17 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         for (int k = 0; k < 3; ++k) gv[k] += fsc * Rij[k];
         if (iw < jw) uvdw += uij;
       }
-      double Ra[3];
+      GT Ra[3];
       for (int k = 0; k < 3; ++k) Ra[k] = Ri[k] + (Rij[k] - ROi[k] + ROj[k]);
       for (int tj = ti == 0 ? 1 : 0; tj < 4; ++tj) {
         const int sj = tj * N + jw;
-        double r[3] = {Ra[0] - Rs[3 * sj], Ra[1] - Rs[3 * sj + 1], Ra[2] - Rs[3 * sj + 2]};
+        GT Rj[3];
+        site(sj, Rj);
+        GT r[3] = {Ra[0] - Rj[0], Ra[1] - Rj[1], Ra[2] - Rj[2]};
         minimg(c, r[0], r[1], r[2]);
-        const double d2s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const GT d2s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         Dual ts0, ts1;
-        smear01<true, Dual>(c, Dual(d2s), P.apCC[ti][tj], P.raCC[ti][tj], ts0, ts1);
+        smear01<true, Dual>(c, as_dual(d2s), P.apCC[ti][tj], P.raCC[ti][tj], ts0, ts1);
         const Dual qj = q[sj];
         pv += ts0 * qj;
         const Dual f = ts1 * qj;
@@ -429,10 +551,12 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
     for (int ti = 0; ti < 3; ++ti)
       for (int tj = ti + 1; tj < 4; ++tj) {
         const int si = ti * N + w, sj = tj * N + w;
-        double r[3] = {Rs[3 * si] - Rs[3 * sj], Rs[3 * si + 1] - Rs[3 * sj + 1],
-                       Rs[3 * si + 2] - Rs[3 * sj + 2]};
+        GT Ri[3], Rj[3];
+        site(si, Ri);
+        site(sj, Rj);
+        GT r[3] = {Ri[0] - Rj[0], Ri[1] - Rj[1], Ri[2] - Rj[2]};
         minimg(c, r[0], r[1], r[2]);
-        double ts0, ts1;
+        GT ts0, ts1;
         self1(c, r[0] * r[0] + r[1] * r[1] + r[2] * r[2], ts0, ts1);
         phi[si] += ts0 * q[sj];
         phi[sj] += ts0 * q[si];
@@ -447,10 +571,11 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
   // ---- reciprocal space, charges (ewald_std_qq, ewald.f90:1-52)
   for (int k = tid; k < nk; k += TAN_BLOCK) {
     const int4 h = a.khkl[k];
+    const double4 kv = a.kv[k];
     Dual rQ = 0.0, iQ = 0.0;
     for (int s = N; s < ns; ++s) {
-      double ck, sk;
-      ph.at(s, h, ck, sk);
+      GT ck, sk;
+      ph.at(s, h, kv, ck, sk);
       rQ += q[s] * ck;
       iQ += q[s] * sk;
     }
@@ -465,8 +590,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       const int4 h = a.khkl[k];
       const double4 kv = a.kv[k];
       const double p4vAk = kv.w;
-      double ck, sk;
-      ph.at(s, h, ck, sk);
+      GT ck, sk;
+      ph.at(s, h, kv, ck, sk);
       const Dual rQ = Sq[2 * k], iQ = Sq[2 * k + 1];
       const Dual t = p4vAk * (ck * iQ - sk * rQ);
       ef[0] -= t * kv.x;
@@ -504,7 +629,9 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
   for (int it = 1; it <= c.maxiter; ++it) {
     for (int i = tid; i < nds; i += TAN_BLOCK) {
       const int ti = tdip + i / N, iw = i % N, s = ti * N + iw;
-      const double *Ri = Rs + 3 * s, *ROi = Rs + 3 * iw;
+      const double *ROi = Rs + 3 * iw;
+      GT Ri[3];
+      site(s, Ri);
       Dual e[3] = {0.0, 0.0, 0.0};
       for (int jw = 0; jw < N; ++jw) {
         if (jw == iw) continue;
@@ -516,11 +643,12 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         minimg(c, R0[0], R0[1], R0[2]);
         for (int tj = tdip; tj < tend; ++tj) {
           const int sj = tj * N + jw, j = (tj - tdip) * N + jw;
-          double r[3];
-          for (int k = 0; k < 3; ++k) r[k] = Ri[k] + R0[k] - Rs[3 * sj + k];
+          GT Rj[3], r[3];
+          site(sj, Rj);
+          for (int k = 0; k < 3; ++k) r[k] = Ri[k] + R0[k] - Rj[k];
           minimg(c, r[0], r[1], r[2]);
           Dual t1, t2, o[3];
-          smear2<true, Dual>(c, Dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
+          smear2<true, Dual>(c, as_dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
           dd3d(r, t1, t2, dip + 3 * j, o);
           for (int k = 0; k < 3; ++k) e[k] += o[k];
         }
@@ -529,10 +657,12 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       for (int tj = tdip; tj < tend; ++tj) {
         if (tj == ti) continue;
         const int sj = tj * N + iw, j = (tj - tdip) * N + iw;
-        double r[3] = {Ri[0] - Rs[3 * sj], Ri[1] - Rs[3 * sj + 1], Ri[2] - Rs[3 * sj + 2]};
+        GT Rj[3];
+        site(sj, Rj);
+        GT r[3] = {Ri[0] - Rj[0], Ri[1] - Rj[1], Ri[2] - Rj[2]};
         minimg(c, r[0], r[1], r[2]);
         Dual t1, t2, o[3];
-        smear2<true, Dual>(c, Dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
+        smear2<true, Dual>(c, as_dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
         dd3d(r, t1, t2, dip + 3 * j, o);
         for (int k = 0; k < 3; ++k) e[k] += o[k];
       }
@@ -545,8 +675,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       Dual rQ = 0.0, iQ = 0.0;
       for (int i = 0; i < nds; ++i) {
         const int s = (tdip + i / N) * N + i % N;
-        double ck, sk;
-        ph.at(s, h, ck, sk);
+        GT ck, sk;
+        ph.at(s, h, kv, ck, sk);
         const Dual dK = dip[3 * i] * kv.x + dip[3 * i + 1] * kv.y + dip[3 * i + 2] * kv.z;
         rQ -= dK * sk;
         iQ += dK * ck;
@@ -562,8 +692,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       for (int k = tid % Gd; k < nk; k += Gd) {
         const int4 h = a.khkl[k];
         const double4 kv = a.kv[k];
-        double ck, sk;
-        ph.at(s, h, ck, sk);
+        GT ck, sk;
+        ph.at(s, h, kv, ck, sk);
         const Dual t = kv.w * (ck * Sm[2 * k + 1] - sk * Sm[2 * k]);
         e[0] -= kv.x * t;
         e[1] -= kv.y * t;
@@ -599,7 +729,9 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
   for (int s = tid; s < ns; s += TAN_BLOCK) {
     const int iw = s % N, ti = s / N;
     const bool adip = ti >= tdip && ti < tend;
-    const double *Ri = Rs + 3 * s, *ROi = Rs + 3 * iw;
+    const double *ROi = Rs + 3 * iw;
+    GT Ri[3];
+    site(s, Ri);
     const Dual qi = q[s];
     Dual di[3] = {0.0, 0.0, 0.0};
     if (adip)
@@ -620,10 +752,11 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         if (same && tj == ti) continue;
         const int sj = tj * N + jw;
         const bool bdip = tj >= tdip && tj < tend;
-        double r[3];
-        for (int k = 0; k < 3; ++k) r[k] = same ? Ri[k] - Rs[3 * sj + k] : Ri[k] + R0[k] - Rs[3 * sj + k];
+        GT Rj[3], r[3];
+        site(sj, Rj);
+        for (int k = 0; k < 3; ++k) r[k] = same ? Ri[k] - Rj[k] : Ri[k] + R0[k] - Rj[k];
         minimg(c, r[0], r[1], r[2]);
-        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const GT d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         const Dual qj = q[sj];
         Dual dj[3] = {0.0, 0.0, 0.0};
         if (bdip)
@@ -631,7 +764,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         if (adip && bdip) {
           // dipole-dipole: d_i . T3(r) . d_j with T3 the third-rank tensor of PBCsmear3
           Dual t1, t2, t3;
-          smear3<true, Dual>(c, Dual(d2), P.apDD[ti][tj], t1, t2, t3);
+          smear3<true, Dual>(c, as_dual(d2), P.apDD[ti][tj], t1, t2, t3);
           const Dual dir_ = di[0] * r[0] + di[1] * r[1] + di[2] * r[2];
           const Dual djr = dj[0] * r[0] + dj[1] * r[1] + dj[2] * r[2];
           const Dual dd = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
@@ -643,14 +776,14 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         Dual ts1 = 0.0, ts2 = 0.0;
         bool qd = true;
         if (same) {
-          double s1, s2;
+          GT s1, s2;
           self2(c, d2, s1, s2);
           ts1 = s1;
           ts2 = s2;
         } else {
           const double qsq = qdqd[0].v * qdqd[0].v + qdqd[1].v * qdqd[1].v + qdqd[2].v * qdqd[2].v;
           qd = qsq > 1e-9;
-          if (qd) smear2<true, Dual>(c, Dual(d2), P.apCC[ti][tj], ts1, ts2);
+          if (qd) smear2<true, Dual>(c, as_dual(d2), P.apCC[ti][tj], ts1, ts2);
         }
         if (qd) {
           Dual o[3];
@@ -679,8 +812,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
       const int4 h = a.khkl[k];
       const double4 kv = a.kv[k];
       const double p4vAk = kv.w;
-      double ck, sk;
-      ph.at(s, h, ck, sk);
+      GT ck, sk;
+      ph.at(s, h, kv, ck, sk);
       const Dual rQm = Sm[2 * k], iQm = Sm[2 * k + 1];
       if (schg) {
         pv += p4vAk * (ck * rQm + sk * iQm);
@@ -729,7 +862,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         if (a.dG) a.dG[((size_t)b * ndir + dir) * 9 * N + 9 * w + 3 * at + k] = G[at][k].d;
       }
   }
-  const double Ev = block_sum(umon, red) + block_sum(uvdw.v, red) + 0.5 * block_sum(uel.v, red) -
+  const double Ev = W.umon->v + block_sum(uvdw.v, red) + 0.5 * block_sum(uel.v, red) -
                     0.5 * block_sum(uind.v, red) + P.lrc.v;
   const double Ed = block_sum(uvdw.d, red) + 0.5 * block_sum(uel.d, red) - 0.5 * block_sum(uind.d, red) +
                     P.lrc.d;
@@ -744,8 +877,9 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
 }
 
 void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s) {
+  const size_t dm_off = 3 * 4 * (size_t)c.N + tangent_tab_doubles(c);
   hipLaunchKernelGGL(k_tangent_prep, dim3(c.P), dim3(TAN_BLOCK), 0, s, c, L.RR, L.cfg,
-                     tangent_cfg_doubles(c));
+                     tangent_cfg_doubles(c), dm_off);
   TanArgs a;
   a.RR = L.RR;
   a.pes = L.pes;
@@ -762,7 +896,15 @@ void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s) {
   for (int i = 0; i < TTM_NPARAM; ++i) a.theta[i] = L.theta[i];
   for (int i = 0; i < 3; ++i) a.lrc_coef[i] = L.lrc_coef[i];
   a.dms_coef = L.dms_coef;
-  hipLaunchKernelGGL(k_ttm_tangent, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
+  a.gamma_m = L.gamma_m;
+  a.dm_off = dm_off;
+  if (L.gamma_m) {
+    hipLaunchKernelGGL(k_tangent_monomer<true>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
+    hipLaunchKernelGGL(k_ttm_tangent<true>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
+  } else {
+    hipLaunchKernelGGL(k_tangent_monomer<false>, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
+    hipLaunchKernelGGL(k_ttm_tangent<false>, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
+  }
 }
 
 }  // namespace pimd

@@ -10,12 +10,12 @@ configuration and target gradients per atom.  The loss is
 over theta: the Partridge-Schwenke parameters (combined c5z, deohA, alphaoh,
 roh, phh1A, phh2; PES_NPARAM values), optionally followed by the TTM
 parameters (vdW, polarisabilities, Thole factors, DMS corrections;
-TTM_NPARAM values, see TTM_PARAM_NAMES) and then optionally by the 84
-coefficients of the dipole moment surface that charges the sites (DMS_NCOEF).
-Forward values come from the GPU force path; the gradient from the
-hand-written derivative kernels (pimd_pes_param_grad for the PES,
-pimd_ttm_param_grad for the TTM parameters, pimd_dms_coef_grad for the DMS
-coefficients).  PESLoss is a torch.autograd.Function over theta, so any torch
+TTM_NPARAM values, see TTM_PARAM_NAMES), then optionally by the 84
+coefficients of the dipole moment surface that charges the sites (DMS_NCOEF)
+and then optionally by the M-site fraction gammaM.  Forward values come from
+the GPU force path; the gradient from the hand-written derivative kernels
+(pimd_pes_param_grad for the PES, pimd_ttm_param_grad for the TTM parameters,
+pimd_dms_coef_grad for the DMS coefficients, pimd_gamma_m_grad for gammaM).  PESLoss is a torch.autograd.Function over theta, so any torch
 optimiser can drive a fit.
 """
 from __future__ import annotations
@@ -26,20 +26,22 @@ from .ttm import DMS_NCOEF, PES_NPARAM, TTM_NPARAM, TTMForceField
 
 NPARAM_ALL = PES_NPARAM + TTM_NPARAM
 NPARAM_DMS = NPARAM_ALL + DMS_NCOEF
+NPARAM_GM = NPARAM_DMS + 1
 
 
 def split_theta(theta):
-    """(PES part, TTM part or None, DMS coefficients or None) of a parameter
-    vector of any of the three lengths."""
+    """(PES part, TTM part, DMS coefficients, gammaM) of a parameter vector
+    of any of the four lengths; the parts it does not hold are None."""
     th = np.asarray(theta, dtype=np.float64)
     if th.size == PES_NPARAM:
-        return th, None, None
+        return th, None, None, None
     if th.size == NPARAM_ALL:
-        return th[:PES_NPARAM], th[PES_NPARAM:], None
-    if th.size == NPARAM_DMS:
-        return th[:PES_NPARAM], th[PES_NPARAM:NPARAM_ALL], th[NPARAM_ALL:]
-    raise ValueError(f"theta must hold {PES_NPARAM}, {NPARAM_ALL} or {NPARAM_DMS} values, "
-                     f"got {th.size}")
+        return th[:PES_NPARAM], th[PES_NPARAM:], None, None
+    if th.size in (NPARAM_DMS, NPARAM_GM):
+        gm = float(th[NPARAM_DMS]) if th.size == NPARAM_GM else None
+        return th[:PES_NPARAM], th[PES_NPARAM:NPARAM_ALL], th[NPARAM_ALL:NPARAM_DMS], gm
+    raise ValueError(f"theta must hold {PES_NPARAM}, {NPARAM_ALL}, {NPARAM_DMS} or {NPARAM_GM} "
+                     f"values, got {th.size}")
 
 
 class FitGroup:
@@ -53,12 +55,14 @@ class FitGroup:
         self.G_target = np.ascontiguousarray(G_target, dtype=np.float64)
 
     def evaluate(self, theta, grad: bool):
-        pes, ttm, dms = split_theta(theta)
+        pes, ttm, dms, gm = split_theta(theta)
         self.ff.set_pes_params(pes)
         if ttm is not None:
             self.ff.set_ttm_params(ttm)
         if dms is not None:
             self.ff.set_dms_coefs(dms)
+        if gm is not None:
+            self.ff.set_gamma_m(gm)
         r = self.ff.full_bead_forces(self.RRt, self.RRc)
         dE = r["Upott"] - self.E_target
         dG = r["dRRt"] - self.G_target
@@ -70,6 +74,8 @@ class FitGroup:
                 g = np.concatenate([g, self.ff.ttm_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0]])
             if dms is not None:
                 g = np.concatenate([g, self.ff.dms_coef_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0]])
+            if gm is not None:
+                g = np.append(g, self.ff.gamma_m_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0])
         return loss, g
 
 
@@ -89,7 +95,8 @@ def _autograd_function():
 
     class PESLoss(torch.autograd.Function):
         """loss = PESLoss.apply(theta, groups): theta a float64 tensor of PES_NPARAM,
-        PES_NPARAM + TTM_NPARAM or PES_NPARAM + TTM_NPARAM + DMS_NCOEF values."""
+        PES_NPARAM + TTM_NPARAM, PES_NPARAM + TTM_NPARAM + DMS_NCOEF values, or
+        those and gammaM."""
 
         @staticmethod
         def forward(ctx, theta, groups):

@@ -38,7 +38,8 @@ EXPORTS = (
     "pimd_pes_param_grad", "pimd_state_forces", "pimd_profile_copy", "pimd_ttm_params_get",
     "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
     "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles", "pimd_dms_coefs_get",
-    "pimd_dms_coefs_set", "pimd_dms_coef_grad",
+    "pimd_dms_coefs_set", "pimd_dms_coef_grad", "pimd_gamma_m_get", "pimd_gamma_m_set",
+    "pimd_gamma_m_grad",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -123,6 +124,9 @@ def load_library(path: str = LIB_PATH):
         "pimd_dms_coefs_get": [vp, dp],
         "pimd_dms_coefs_set": [vp, dp],
         "pimd_dms_coef_grad": [vp, dp, dp, dp, dp, dp, dp],
+        "pimd_gamma_m_get": [vp, dp],
+        "pimd_gamma_m_set": [vp, C.c_double],
+        "pimd_gamma_m_grad": [vp, dp, dp, dp, dp, dp, dp],
         "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
@@ -322,6 +326,27 @@ class TTMForceField:
                                                  _ptr(E), _ptr(dE)))
         return grad, E, dE
 
+    def gamma_m(self) -> float:
+        """The M-site fraction gammaM in use."""
+        g = np.zeros(1)
+        _check(load_library().pimd_gamma_m_get(self._h, _ptr(g)))
+        return float(g[0])
+
+    def set_gamma_m(self, g: float):
+        _check(load_library().pimd_gamma_m_set(self._h, float(g)))
+        if self._single is not None:
+            self._single.set_gamma_m(g)
+
+    def gamma_m_grad(self, RRt, ew, gw):
+        """The tangent along gammaM (the M sites moving with it): (grad, E,
+        dE/dgammaM), as ttm_param_grad."""
+        P, na = self.params.nbeads, self.params.natoms
+        r, e, g = _arr(RRt, (P, na, 3)), _arr(ew, (P,)), _arr(gw, (P, na, 3))
+        grad, E, dE = np.zeros(1), np.zeros(P), np.zeros(P)
+        _check(load_library().pimd_gamma_m_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(grad),
+                                                _ptr(E), _ptr(dE)))
+        return float(grad[0]), E, dE
+
     def potential(self, RR, RRc, t: int = 1) -> dict:
         """One bead (force_calc.f90:112-170): the reference's per-rank call."""
         if self._single is None:
@@ -329,6 +354,9 @@ class TTMForceField:
             self._single = TTMForceField(q)
             self._single.set_pes_params(self.pes_params())
             self._single.set_ttm_params(self.ttm_params())
+            if self.params.pot_model in (2, 3):
+                self._single.set_dms_coefs(self.dms_coefs())
+                self._single.set_gamma_m(self.gamma_m())
         r = self._single.full_bead_forces(np.asarray(RR)[None], RRc, t)
         return {k: (v[0] if isinstance(v, np.ndarray) and v.ndim > 0 else v) for k, v in r.items()}
 

@@ -332,3 +332,116 @@ def test_dms_coef_loss_gradient_vs_fd(fitset):
     print(f"model {model}: loss gradient over 84 DMS coefficients, max {scale:.3e}, "
           f"largest error {err:.2e} ({err / scale:.1e} of it)", flush=True)
     assert err < FD_REL * scale
+
+
+def _gm_forward(groups, gm):
+    out = []
+    for grp in groups:
+        grp.ff.set_gamma_m(gm)
+        r = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        out.append((r["Upott"], r["dRRt"]))
+    return out
+
+
+def test_gamma_m_derivatives_vs_fd(fitset):
+    """dE_b/dgammaM of every configuration and the loss gradient along gammaM
+    (pimd_gamma_m_grad: the M sites move with it, pot_ttm.f90:74-79, and the
+    charges are placed with it, pot_ttm.f90:134-137) against central
+    differences (h = 1e-6 relative) of the GPU forward model at 1e-6 relative
+    plus the rounding bound.  No five-point check: the M sites' distances
+    enter the argument of the Thole charge term's Gamma(2/3, x), which jumps
+    where gammq2_3 changes its number of terms.  The value parts reproduce
+    the forward energies."""
+    model, groups, theta0, pes0, L0, g = fitset
+    gm0 = groups[0].ff.gamma_m()
+    assert gm0 == (0.46 if model == 3 else 0.426706882)
+    h = 1e-6 * gm0
+    gsum = 0.0
+    for grp in groups:
+        grp.ff.set_ttm_params(theta0)
+        P, na = grp.RRt.shape[:2]
+        r0 = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        _, E, dE = grp.ff.gamma_m_grad(grp.RRt, np.zeros(P), np.zeros((P, na, 3)))
+        assert np.max(np.abs(E - r0["Upott"]) / np.abs(r0["Upott"])) < 1e-10
+        (Ep, _), = _gm_forward([grp], gm0 + h)
+        (Em, _), = _gm_forward([grp], gm0 - h)
+        grp.ff.set_gamma_m(gm0)
+        fd = (Ep - Em) / (2 * h)
+        scale, err = np.max(np.abs(fd)), np.max(np.abs(fd - dE))
+        noise = 4 * EPS * np.max(np.abs(Ep)) / (2 * h)
+        print(f"model {model} N {na // 3}: |dE/dgammaM| {scale:.3e} err {err:.2e} "
+              f"(rounding bound {noise:.1e})", flush=True)
+        assert err < FD_REL * scale + noise
+        gsum += grp.ff.gamma_m_grad(grp.RRt, *_weights(grp))[0]
+    fd, noise = _central(_gm_forward(groups, gm0 + h), _gm_forward(groups, gm0 - h), groups, 2 * h)
+    _gm_forward(groups, gm0)
+    err = abs(fd - gsum)
+    print(f"model {model} loss gradient along gammaM: tangent {gsum: .10e} fd {fd: .10e} "
+          f"rel {err / abs(gsum):.2e} (rounding bound {noise:.1e})", flush=True)
+    assert err < FD_REL * abs(gsum) + noise
+
+
+def test_gamma_m_vs_oracle(fitset):
+    """On the cluster group: the loss gradient along gammaM against central
+    differences of the oracle with its gammaM overridden, and the forward
+    model at a changed gammaM against the oracle at the same value (1e-10
+    relative energies, 1e-8 gradient)."""
+    import oracle
+    from helpers import E_REL, F_ABS, max_abs, rel_err
+    model, groups, theta0, pes0, L0, g = fitset
+    grp = groups[0]
+    grp.ff.set_ttm_params(theta0)
+    sys_ = oracle.System(grp.ff.params)
+    gm0 = grp.ff.gamma_m()
+    assert sys_.gamma_m() == gm0
+    gc = grp.ff.gamma_m_grad(grp.RRt, *_weights(grp))[0]
+
+    def fwd(gm):
+        sys_.set_gamma_m(gm)
+        o = sys_.forces(grp.RRt, grp.RRc, t=1)
+        return [(o["En"], o["dRR"])]
+
+    h = 1e-6 * gm0
+    fd, noise = _central(fwd(gm0 + h), fwd(gm0 - h), [grp], 2 * h)
+    err = abs(fd - gc)
+    print(f"oracle model {model} gammaM: tangent {gc: .10e} fd {fd: .10e} "
+          f"(rounding bound {noise:.1e})", flush=True)
+    assert err < FD_REL * abs(gc) + noise
+    gm1 = 0.9 * gm0
+    sys_.set_gamma_m(gm1)
+    grp.ff.set_gamma_m(gm1)
+    try:
+        r = grp.ff.full_bead_forces(grp.RRt, grp.RRc)
+        o = sys_.forces(grp.RRt, grp.RRc, t=1)
+    finally:
+        grp.ff.set_gamma_m(gm0)
+    e, f = rel_err(r["Upott"], o["En"]), max_abs(r["dRRt"], o["dRR"])
+    print(f"forward at gammaM {gm1}: energy rel {e:.2e} gradient {f:.2e}", flush=True)
+    assert e < E_REL and f < F_ABS
+
+
+def test_gamma_m_set_refuses_out_of_range(fitset):
+    from pimd_amd.ttm import PimdError
+    model, groups, theta0, pes0, L0, g = fitset
+    ff = groups[0].ff
+    gm0 = ff.gamma_m()
+    for bad in (1.0, -0.1, float("nan")):
+        with pytest.raises(PimdError):
+            ff.set_gamma_m(bad)
+    assert ff.gamma_m() == gm0
+
+
+def test_fit_vector_with_dms_and_gamma_m(fitset):
+    """The fitting driver over all 351 parameters (250 PES, 16 TTM, 84 DMS,
+    gammaM): the same loss at the same parameters, and its TTM and gammaM
+    components equal to the separate tangent sweeps'."""
+    from pimd_amd.fit import NPARAM_GM, loss_and_grad
+    model, groups, theta0, pes0, L0, g = fitset
+    c0, gm0 = groups[0].ff.dms_coefs(), groups[0].ff.gamma_m()
+    full = np.concatenate([pes0, theta0, c0, [gm0]])
+    assert full.size == NPARAM_GM
+    L, gall = loss_and_grad(groups, full)
+    assert abs(L - L0) <= 1e-12 * L0
+    assert np.allclose(gall[pes0.size:pes0.size + theta0.size], g, rtol=1e-12, atol=0)
+    ggm = sum(grp.ff.gamma_m_grad(grp.RRt, *_weights(grp))[0] for grp in groups)
+    assert abs(gall[-1] - ggm) <= 1e-12 * abs(ggm)
