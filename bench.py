@@ -551,6 +551,11 @@ def run_fit(args):
     for _ in range(args.steps):
         L0, g = loss_and_grad(groups, theta0)
     s = (time.perf_counter() - t0) / max(args.steps, 1)
+    # the same step with the TTM part from its 16 tangent sweeps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, gt = loss_and_grad(groups, theta0, method="tangent")
+    s_tan = (time.perf_counter() - t0) / max(args.steps, 1)
     # the parts of one step
     parts = {}
     for name, fn in (("forward", lambda grp: grp.ff.full_bead_forces(grp.RRt, grp.RRc)),
@@ -560,6 +565,10 @@ def run_fit(args):
                          grp.RRt, np.ones(grp.RRt.shape[0]), np.ones_like(grp.RRt))),
                      # not part of the step: the 84 DMS coefficients' tangents
                      ("dms_tangent_84", lambda grp: grp.ff.dms_coef_grad(
+                         grp.RRt, np.ones(grp.RRt.shape[0]), np.ones_like(grp.RRt))),
+                     # one adjoint sweep over the 16 TTM parameters, 84 DMS
+                     # coefficients and gammaM
+                     ("ttm_adjoint_101", lambda grp: grp.ff.ttm_adjoint(
                          grp.RRt, np.ones(grp.RRt.shape[0]), np.ones_like(grp.RRt)))):
         t1 = time.perf_counter()
         for grp in groups:
@@ -592,6 +601,9 @@ def run_fit(args):
         "config": {"workload": FIT256, "clusters": ncl, "boxes": nbx, "parameters": int(theta0.size),
                    "pes_parameters": 250, "ttm_parameters": 16, "eps_ewald": 1e-8,
                    "polar_eps": 1e-12},
+        "gradient": "adjoint (pimd_ttm_adjoint) for the TTM part, PES adjoint",
+        "tangent_step": {"ms_per_step": round(1000 * s_tan, 3),
+                         "grad_rel_diff": float(np.max(np.abs(gt - g)) / np.max(np.abs(gt)))},
         "loss": L0, "grad_norm": float(np.linalg.norm(g)),
         "parts_ms": parts,
         "cpu_baseline": cpu,

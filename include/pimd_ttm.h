@@ -307,6 +307,24 @@ int pimd_gamma_m_set(pimd_ctx *ctx, double gamma_m);
 int pimd_gamma_m_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
                       double *grad, double *E, double *dEdg);
 
+/* The adjoint of the same loss with respect to the first nparam of the
+ * PIMD_TTM_ADJ_NPARAM parameters: the 16 TTM parameters, the 84 DMS
+ * coefficients, then gammaM (the orders of pimd_ttm_params_get and
+ * pimd_dms_coefs_get); nparam is 16, 100 or 101.  One sweep per
+ * configuration for all of the first 100: the induction energy is stationary
+ * in the converged dipoles, so dE/dtheta is the explicit derivative at fixed
+ * dipoles and the SOR solve needs no separate adjoint solve; the gradient
+ * term gw . d(dE/dr)/dtheta is that derivative's rate of change along the
+ * atoms' direction gw, carried through the force path as a dual number with
+ * the screening functions' derivatives taken from the chain the reference's
+ * gradient uses.  gammaM (nparam 101) takes one tangent sweep more.  Exact up
+ * to the SOR convergence (the tangents above differentiate the iterations as
+ * executed): use polar_eps of 1e-10 or below for fitting.
+ * grad[nparam]; E[P] and dEdtheta[P][nparam] may be NULL. */
+#define PIMD_TTM_ADJ_NPARAM (PIMD_TTM_NPARAM + PIMD_DMS_NCOEF + 1)
+int pimd_ttm_adjoint(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
+                     int nparam, double *grad, double *E, double *dEdtheta);
+
 /* ---- instrumentation --------------------------------------------------- */
 
 /* Enables per-kernel HIP-event timing on the context stream. */

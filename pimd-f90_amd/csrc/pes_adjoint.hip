@@ -12,6 +12,8 @@
 // exactly with dual numbers (value, derivative along w) through the closed
 // forms of dEmon/dtheta.
 #include "dual.h"
+#include <mutex>
+
 #include "ttm_device.h"
 
 namespace pimd {
@@ -22,16 +24,20 @@ constexpr int ADJ_BLOCK = 64;
 // Exponent table of the PES basis (nasa_mod.f90 idx), this unit's own copy.
 __constant__ int c_idx_adj[PS_NTERM][3];
 
-static void set_adjoint_tables() {
-  static bool done = false;
-  if (done) return;
-  int idx[PS_NTERM][3];
-  for (int j = 0; j < PS_NTERM; ++j) {
-    idx[j][0] = PS_IDX1[j];
-    idx[j][1] = PS_IDX2[j];
-    idx[j][2] = PS_IDX3[j];
-  }
-  done = hipMemcpyToSymbol(HIP_SYMBOL(c_idx_adj), idx, sizeof idx) == hipSuccess;
+// Once per process; false if the copy failed.
+static bool set_adjoint_tables() {
+  static std::once_flag once;
+  static bool ok = false;
+  std::call_once(once, [] {
+    int idx[PS_NTERM][3];
+    for (int j = 0; j < PS_NTERM; ++j) {
+      idx[j][0] = PS_IDX1[j];
+      idx[j][1] = PS_IDX2[j];
+      idx[j][2] = PS_IDX3[j];
+    }
+    ok = hipMemcpyToSymbol(HIP_SYMBOL(c_idx_adj), idx, sizeof idx) == hipSuccess;
+  });
+  return ok;
 }
 
 // One thread per (bead, molecule); each block sums its molecules' terms for
@@ -128,13 +134,14 @@ __global__ __launch_bounds__(256) void k_pes_adjoint_sum(const double *__restric
 
 int pes_adjoint_blocks(const SysConst &c) { return (c.P * c.N + ADJ_BLOCK - 1) / ADJ_BLOCK; }
 
-void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
+bool launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
                         const double *gw, double *partial, int nblocks, double *grad,
                         hipStream_t s) {
-  set_adjoint_tables();
+  if (!set_adjoint_tables()) return false;
   hipLaunchKernelGGL(k_pes_adjoint, dim3(nblocks), dim3(ADJ_BLOCK), 0, s, c, RR, ew, gw, b.pes,
                      partial);
   hipLaunchKernelGGL(k_pes_adjoint_sum, dim3(1), dim3(256), 0, s, partial, nblocks, grad);
+  return true;
 }
 
 }  // namespace pimd

@@ -159,6 +159,7 @@ void invalidate_graphs(pimd_ctx *x) {
   x->fgraph_ok = false;
 }
 
+
 template <class T>
 int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
@@ -2103,7 +2104,9 @@ extern "C" int pimd_pes_param_grad(pimd_ctx *x, const double *RRt, const double 
   HIPCHK(hipMemcpyAsync(dR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dG, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dE, ew, sizeof(double) * c.P, hipMemcpyHostToDevice, s));
-  x->timed("pes_adjoint", [&] { launch_pes_adjoint(c, d, dR, dE, dG, part, nb, dgrad, s); });
+  bool ok = true;
+  x->timed("pes_adjoint", [&] { ok = launch_pes_adjoint(c, d, dR, dE, dG, part, nb, dgrad, s); });
+  if (!ok) return fail(PIMD_EDEVICE, "copying the PES exponent table to the device failed");
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(grad, dgrad, sizeof(double) * PES_NPARAM, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -2138,8 +2141,9 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
 }
 
 // Tangent sweeps along ndir directions (mode 0 the TTM parameters, 1 the DMS
-// coefficients, 2 gammaM): the loss gradient, and optionally each
-// configuration's energy and its derivatives.
+// coefficients, 2 gammaM), or with mode 3 the adjoint of all TTM_ADJ_NPARAM
+// parameters in one sweep (ndir = TTM_ADJ_NPARAM outputs): the loss
+// gradient, and optionally each configuration's energy and its derivatives.
 static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
                         int mode, int ndir, double *grad, double *E, double *dEdtheta) {
   if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
@@ -2151,9 +2155,12 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
   const SysConst &c = x->c;
   DevBufs &d = x->d;
   hipStream_t s = x->stream;
-  const size_t P = (size_t)c.P, n = P * x->natoms * 3, nout = P * ndir * 4;
-  const size_t ws = tangent_ws_duals(c) * P * ndir * 2, cfg = tangent_cfg_doubles(c) * P;
-  const size_t bytes = sizeof(double) * (ws + cfg + 2 * n + P + nout);
+  const bool adj = mode == 3;
+  const int nsweep = adj ? 1 : ndir;  // workgroups per configuration
+  const size_t P = (size_t)c.P, n = P * x->natoms * 3, nout = P * nsweep * 4;
+  const size_t nadj = adj ? P * TTM_ADJ_NPARAM * 2 : 0;
+  const size_t ws = tangent_ws_duals(c) * P * nsweep * 2, cfg = tangent_cfg_doubles(c) * P;
+  const size_t bytes = sizeof(double) * (ws + cfg + 2 * n + P + nout + nadj);
   if (bytes > ((size_t)32 << 30))
     return fail(PIMD_EINVAL, "parameter derivatives of %zu configurations need %.1f GB of workspace; "
                 "split them into smaller contexts", P, bytes / 1e9);
@@ -2181,7 +2188,9 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
   L.dG = nullptr;
   L.dms_coef = mode == 1;
   L.gamma_m = mode == 2;
-  L.ndir = ndir;
+  L.adjoint = adj;
+  L.adj = L.out + nout;
+  L.ndir = nsweep;
   std::copy(x->ttm, x->ttm + TTM_NPARAM, L.theta);
   const double Rc = x->p.rc, pre = TWOPI * (double)(c.N * c.N) / c.vol;
   L.lrc_coef[0] = pre / (9.0 * std::pow(Rc, 9));
@@ -2190,12 +2199,29 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
   HIPCHK(hipMemcpyAsync(dRR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dgw, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dew, ew, sizeof(double) * P, hipMemcpyHostToDevice, s));
-  static const char *const names[3] = {"ttm_tangent", "dms_tangent", "gamma_m_tangent"};
+  static const char *const names[4] = {"ttm_tangent", "dms_tangent", "gamma_m_tangent",
+                                       "ttm_adjoint"};
   x->timed(names[mode], [&] { launch_ttm_tangent(c, L, s); });
   HIPCHK(hipGetLastError());
-  std::vector<double> out(nout);
-  HIPCHK(hipMemcpyAsync(out.data(), L.out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  std::vector<double> out(nout + nadj);
+  HIPCHK(hipMemcpyAsync(out.data(), L.out, sizeof(double) * (nout + nadj), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (adj) {
+    // grad_j = sum_b ew_b dE_b/dtheta_j + gw . d(dE_b/dr)/dtheta_j
+    const double *A = out.data() + nout;
+    for (int j = 0; j < ndir; ++j) {
+      double g = 0.0;
+      for (size_t b = 0; b < P; ++b)
+        g += ew[b] * A[(b * ndir + j) * 2] + A[(b * ndir + j) * 2 + 1];
+      grad[j] = g;
+    }
+    for (size_t b = 0; b < P; ++b) {
+      if (E) E[b] = out[b * 4];
+      if (dEdtheta)
+        for (int j = 0; j < ndir; ++j) dEdtheta[b * ndir + j] = A[(b * ndir + j) * 2];
+    }
+    return PIMD_OK;
+  }
   for (int j = 0; j < ndir; ++j) {
     double g = 0.0;
     for (size_t b = 0; b < P; ++b) g += out[(b * ndir + j) * 4 + 2];
@@ -2222,6 +2248,35 @@ extern "C" int pimd_dms_coef_grad(pimd_ctx *x, const double *RRt, const double *
 extern "C" int pimd_gamma_m_grad(pimd_ctx *x, const double *RRt, const double *ew,
                                  const double *gw, double *grad, double *E, double *dEdg) {
   return tangent_grad(x, RRt, ew, gw, 2, 1, grad, E, dEdg);
+}
+
+extern "C" int pimd_ttm_adjoint(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
+                                int nparam, double *grad, double *E, double *dEdtheta) {
+  if (nparam != TTM_NPARAM && nparam != TTM_NPARAM + DMS_NCOEF && nparam != TTM_ADJ_NPARAM)
+    return fail(PIMD_EINVAL, "nparam must be %d, %d or %d", TTM_NPARAM, TTM_NPARAM + DMS_NCOEF,
+                TTM_ADJ_NPARAM);
+  if (!x || !grad) return fail(PIMD_EINVAL, "null argument");
+  const size_t P = (size_t)x->c.P;
+  std::vector<double> g(TTM_ADJ_NPARAM), dE(dEdtheta ? P * TTM_ADJ_NPARAM : 0);
+  int rc = tangent_grad(x, RRt, ew, gw, 3, TTM_ADJ_NPARAM, g.data(), E,
+                        dEdtheta ? dE.data() : nullptr);
+  if (rc != PIMD_OK) return rc;
+  std::copy(g.begin(), g.begin() + nparam, grad);
+  if (dEdtheta)
+    for (size_t b = 0; b < P; ++b)
+      std::copy(dE.begin() + b * TTM_ADJ_NPARAM, dE.begin() + b * TTM_ADJ_NPARAM + nparam,
+                dEdtheta + b * nparam);
+  if (nparam < TTM_ADJ_NPARAM) return PIMD_OK;
+  // gammaM moves the M sites, so its energy derivative is a position
+  // derivative of the energy as computed, which the reference's gradient
+  // matches only to ~1e-5: one tangent sweep along it instead
+  std::vector<double> dEg(P);
+  double gg = 0.0;
+  if ((rc = tangent_grad(x, RRt, ew, gw, 2, 1, &gg, nullptr, dEg.data())) != PIMD_OK) return rc;
+  grad[TTM_ADJ_NPARAM - 1] = gg;
+  if (dEdtheta)
+    for (size_t b = 0; b < P; ++b) dEdtheta[b * TTM_ADJ_NPARAM + TTM_ADJ_NPARAM - 1] = dEg[b];
+  return PIMD_OK;
 }
 
 extern "C" int pimd_gamma_m_get(pimd_ctx *x, double *gamma_m) {

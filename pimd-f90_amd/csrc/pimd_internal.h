@@ -355,7 +355,7 @@ void launch_contract_record(const Group &g, hipStream_t s);
 
 // theta[PES_NALL]: the reference's PES parameters and DMS coefficients
 void pes_default_params(double *theta);
-void launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
+bool launch_pes_adjoint(const SysConst &c, DevBufs &b, const double *RR, const double *ew,
                         const double *gw, double *partial, int nblocks, double *grad,
                         hipStream_t s);
 int pes_adjoint_blocks(const SysConst &c);
@@ -365,6 +365,8 @@ enum FieldMode { FD_Q = 0, FD_D = 1, FD_QD = 2 };
 
 // TTM model parameters for fitting, in the order of pimd_ttm_params_get.
 constexpr int TTM_NPARAM = 16;
+// parameters of the adjoint: the TTM parameters, the DMS coefficients, gammaM
+constexpr int TTM_ADJ_NPARAM = TTM_NPARAM + DMS_NCOEF + 1;
 enum TtmParam { TP_VDWA = 0, TP_VDWB, TP_VDWC, TP_VDWD, TP_VDWE, TP_POLARO, TP_POLARH, TP_POLARM,
                 TP_POLFACO, TP_POLFACH, TP_POLFACM, TP_ADD, TP_ACC, TP_DMS1, TP_DMS2, TP_DMS3 };
 
@@ -381,6 +383,8 @@ struct TanLaunch {
   double lrc_coef[3];
   int dms_coef;  // 0: the TTM parameters (ndir = TTM_NPARAM), 1: the DMS coefficients
   int gamma_m;   // 1: the one direction is the M-site fraction gammaM (moves the M sites)
+  int adjoint;   // 1: the adjoint of all TTM_ADJ_NPARAM parameters (one sweep along gw)
+  double *adj;   // adjoint output [P][TTM_ADJ_NPARAM][2]
   int ndir;
 };
 size_t tangent_cfg_doubles(const SysConst &c);

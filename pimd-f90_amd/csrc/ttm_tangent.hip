@@ -42,19 +42,58 @@ struct RM<Dual> {
   static __device__ __forceinline__ Dual abs_(Dual x) { return dabs(x); }
 };
 
+// Dual numbers over duals: v is a value with its derivative along the
+// adjoint's position direction, a the derivative of v with respect to one
+// Thole factor, itself with its position derivative.  The adjoint runs the
+// smearing functions on them for d ts / d a along the direction.
+struct HD {
+  Dual v, a;
+  HD() = default;
+  __device__ constexpr HD(double x) : v(x), a(0.0) {}
+  __device__ constexpr HD(Dual x, Dual y) : v(x), a(y) {}
+};
+__device__ __forceinline__ HD operator+(HD x, HD y) { return HD(x.v + y.v, x.a + y.a); }
+__device__ __forceinline__ HD operator-(HD x, HD y) { return HD(x.v - y.v, x.a - y.a); }
+__device__ __forceinline__ HD operator-(HD x) { return HD(-x.v, -x.a); }
+__device__ __forceinline__ HD operator*(HD x, HD y) { return HD(x.v * y.v, x.a * y.v + x.v * y.a); }
+__device__ __forceinline__ HD operator/(HD x, HD y) {
+  const Dual q = x.v / y.v;
+  return HD(q, (x.a - q * y.a) / y.v);
+}
+__device__ __forceinline__ bool operator<(HD x, HD y) { return x.v.v < y.v.v; }
+__device__ __forceinline__ bool operator<=(HD x, HD y) { return x.v.v <= y.v.v; }
+template <>
+struct RM<HD> {
+  static __device__ __forceinline__ HD sqrt_(HD x) {
+    const Dual r = dsqrt(x.v);
+    return HD(r, 0.5 * x.a / r);
+  }
+  static __device__ __forceinline__ HD rcp(HD x) { return HD(1.0) / x; }
+  static __device__ __forceinline__ HD div(HD x, HD y) { return x / y; }
+  static __device__ __forceinline__ HD exp_(HD x) {
+    const Dual e = dexp(x.v);
+    return HD(e, e * x.a);
+  }
+  static __device__ __forceinline__ HD log_(HD x) { return HD(dlog(x.v), x.a / x.v); }
+  static __device__ __forceinline__ HD abs_(HD x) { return x.v.v < 0.0 ? -x : x; }
+};
+
 constexpr int TAN_BLOCK = 256;
+enum TanMode { TAN_PARAMS = 0, TAN_GAMMA_M = 1, TAN_ADJOINT = 2 };
 
 // Per-configuration FP64 data: sites [4N][3] (O, H1, H2, M blocks, as
 // pot_ttm.f90:56-80), then the phase tables exp(2 pi i m x / L) per axis,
 // [m][site] as (cos, sin): m = 0..hmax, -kmax..kmax, -lmax..lmax; then the
-// M sites' derivatives with respect to gammaM, [N][3].
+// sites' derivatives along the geometric direction of the dual-geometry
+// instances, [4N][3]: dr/dgammaM (nonzero for the M sites only) or the
+// position direction of the adjoint's force-loss term.
 static size_t tangent_tab_doubles(const SysConst &c) {
   const size_t ns = 4 * (size_t)c.N;
   return 2 * ns * (size_t)((c.hmax + 1) + (2 * c.kmax + 1) + (2 * c.lmax + 1));
 }
 size_t tangent_cfg_doubles(const SysConst &c) {
   const size_t ns = 4 * (size_t)c.N;
-  return ns * 3 + tangent_tab_doubles(c) + 3 * (size_t)c.N;
+  return ns * 3 + tangent_tab_doubles(c) + 3 * ns;
 }
 
 // Per (configuration, direction) dual workspace: charge, phi [4N], Efq, dR
@@ -84,13 +123,16 @@ struct TanWs {
   }
 };
 
+// mode 1: dS = dr/dgammaM; mode 2: dS = the atoms' direction gw [P][3N][3]
+// (the M site moving with its atoms, pot_ttm.f90:74-79).
 __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_prep(SysConst c, const double *__restrict__ RR,
                                                            double *__restrict__ cfg, size_t stride,
-                                                           size_t dm_off) {
+                                                           size_t ds_off, int mode,
+                                                           const double *__restrict__ gw) {
   const int b = blockIdx.x, N = c.N, ns = 4 * N;
   double *Rs = cfg + (size_t)b * stride;
-  double *dM = Rs + dm_off;
-  const double *rr = RR + (size_t)b * 9 * N;
+  double *dS = Rs + ds_off;
+  const double *rr = RR + (size_t)b * 9 * N, *g = gw + (size_t)b * 9 * N;
   const double gM = c.gammaM;
   for (int w = threadIdx.x; w < N; w += TAN_BLOCK) {
     double roh1[3], roh2[3];
@@ -105,7 +147,14 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_prep(SysConst c, const do
     minimg(c, roh2[0], roh2[1], roh2[2]);
     for (int k = 0; k < 3; ++k) {
       Rs[(3 * N + w) * 3 + k] = 0.5 * gM * (roh1[k] + roh2[k]) + rr[9 * w + k];
-      dM[3 * w + k] = 0.5 * (roh1[k] + roh2[k]);
+      if (mode == 1) {
+        for (int t = 0; t < 3; ++t) dS[(t * N + w) * 3 + k] = 0.0;
+        dS[(3 * N + w) * 3 + k] = 0.5 * (roh1[k] + roh2[k]);
+      } else if (mode == 2) {
+        for (int t = 0; t < 3; ++t) dS[(t * N + w) * 3 + k] = g[9 * w + 3 * t + k];
+        dS[(3 * N + w) * 3 + k] =
+            (1.0 - gM) * g[9 * w + k] + 0.5 * gM * (g[9 * w + 3 + k] + g[9 * w + 6 + k]);
+      }
     }
   }
   __syncthreads();
@@ -127,6 +176,7 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_prep(SysConst c, const do
 // Model parameters as duals along one direction.
 struct TanPar {
   Dual vdw[5], polar[4], apCC[4][4], raCC[4][4], apDD[4][4], lrc;
+  double pf[4], dpf[4];  // polfac^(-1/2) per site type and its derivative by polfac
 };
 
 struct TanArgs {
@@ -146,7 +196,8 @@ struct TanArgs {
   double lrc_coef[3];    // dUvdw_lrc / d(vdwA, vdwB, vdwC)
   int dms_coef;          // 1: the directions are the DMS coefficients coefD
   int gamma_m;           // 1: the one direction is gammaM
-  size_t dm_off;         // offset of dM/dgammaM [N][3] in a configuration's data
+  size_t ds_off;         // offset of the sites' directions dS [4N][3] in a configuration's data
+  double *adj;           // adjoint: [P][TTM_ADJ_NPARAM] (dE/dtheta, gw . d(dE/dr)/dtheta)
 };
 
 // The derivative of the DMS charges and their gradients with respect to
@@ -283,7 +334,7 @@ __device__ inline void dms_coef_tangent(const SysConst &c, const double r[3][3],
 // ewald.f90:403-437 and ewald.f90:20-33).
 struct Phases {
   const double2 *tx, *ty, *tz;
-  const double *dM;  // M sites' dr/dgammaM [N][3]
+  const double *dS;  // the sites' directions [4N][3]
   int ns, kmax, lmax, N;
   __device__ __forceinline__ void at(int s, int4 h, double &ck, double &sk) const {
     const double2 a = tx[(size_t)h.x * ns + s], b = ty[(size_t)(h.y + kmax) * ns + s],
@@ -292,15 +343,12 @@ struct Phases {
     ck = cxy * z.x - sxy * z.y;
     sk = sxy * z.x + cxy * z.y;
   }
-  // with moving M sites: d exp(i k.r)/dgammaM = i (k . dr/dgammaM) exp(i k.r)
+  // with moving sites: d exp(i k.r) = i (k . dr) exp(i k.r)
   __device__ __forceinline__ void at(int s, int4 h, const double4 &kv, Dual &ck, Dual &sk) const {
     double c, sn;
     at(s, h, c, sn);
-    double kd = 0.0;
-    if (s >= 3 * N) {
-      const double *d = dM + 3 * (s - 3 * N);
-      kd = kv.x * d[0] + kv.y * d[1] + kv.z * d[2];
-    }
+    const double *d = dS + 3 * s;
+    const double kd = kv.x * d[0] + kv.y * d[1] + kv.z * d[2];
     ck = Dual(c, -kd * sn);
     sk = Dual(sn, kd * c);
   }
@@ -382,16 +430,16 @@ __device__ __forceinline__ void dd3d(const G r[3], Dual t1, Dual t2, const Dual 
 // direction, the PES gradient and the monomer energy of the configuration,
 // into the (configuration, direction) workspace.  A kernel of its own: the
 // PES and DMS polynomials hold more registers than any later phase.
-template <bool GD>
+template <int MODE>
 __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_monomer(SysConst c, TanArgs a) {
-  using GT = std::conditional_t<GD, Dual, double>;
+  using GT = std::conditional_t<MODE != TAN_PARAMS, Dual, double>;
   const int b = blockIdx.x, dir = blockIdx.y, ndir = gridDim.y, tid = threadIdx.x, N = c.N;
   const TanWs W(a.ws + ((size_t)b * ndir + dir) * a.ws_stride, c);
   Dual *q = W.q, *phi = W.phi, *Efq = W.Efq, *dR = W.dR, *grdq = W.grdq;
   __shared__ double red[TAN_BLOCK];
   // gammaM and the charge placement factors (pot_ttm.f90:134-137)
   GT gM = c.gammaM;
-  if constexpr (GD) gM.d = 1.0;
+  if constexpr (MODE == TAN_GAMMA_M) gM.d = 1.0;
   const GT tmpq = 0.5 * gM / (1.0 - gM), qMf = GT(1.0) / (1.0 - gM);
 
   double umon = 0.0;
@@ -402,8 +450,23 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_monomer(SysConst c, TanAr
       for (int k = 0; k < 3; ++k) r[at][k] = rr[3 * at + k];
     ps_monomer<true>(c, r, e1, dr1, q3, gq, a.pes);
     umon += e1;
-    if (a.dms_coef) dms_coef_tangent(c, r, dir, dq3, dgq);
-    else if (!a.gamma_m && dir >= TP_DMS1) dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
+    if constexpr (MODE == TAN_ADJOINT) {
+      // the charges along the atoms' direction gw: dq = sum_atoms gq . gw, with
+      // gq[disp][charge] in the order (H1, H2, O) and q3 in (O, H1, H2)
+      const double *g = a.gw + ((size_t)b * 3 * N + 3 * w) * 3;
+      const int atom[3] = {1, 2, 0};
+      for (int i = 0; i < 3; ++i) {
+        const int ch = i == 0 ? 2 : i - 1;
+        double v = 0.0;
+        for (int d = 0; d < 3; ++d)
+          for (int k = 0; k < 3; ++k) v += gq[d][ch][k] * g[3 * atom[d] + k];
+        dq3[i] = v;
+      }
+    } else if (MODE == TAN_PARAMS && a.dms_coef) {
+      dms_coef_tangent(c, r, dir, dq3, dgq);
+    } else if (MODE == TAN_PARAMS && dir >= TP_DMS1) {
+      dms_tangent(c, r, dir - TP_DMS1, dq3, dgq);
+    }
     Dual qd[3];
     for (int i = 0; i < 3; ++i) qd[i] = Dual(q3[i] * CHARGECON, dq3[i] * CHARGECON);
     q[w] = Dual(0.0);
@@ -435,10 +498,109 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_tangent_monomer(SysConst c, TanAr
   if (tid == 0) *W.umon = Dual(u);
 }
 
-// GD: the gammaM instance, with dual site positions (GT = Dual) for the M
-// sites and dual phases; otherwise the geometry is plain FP64 (GT = double).
-template <bool GD>
+// The adjoint's position derivatives of the screening functions ts_n(r^2)
+// follow the chain the gradient of pot_ttm uses, ts_(n+1) = -1/((2n+1) r)
+// d ts_n/dr (the erfc-screened and Thole tensors of smear.f90), rather than
+// differentiating the A&S erfc polynomial and the truncated Gamma(2/3) series
+// themselves: the reference's gradient departs from the exact derivative of
+// its energy by ~1e-5 relative there, and the adjoint differentiates the
+// gradient as computed.  d ts_n = -(2n+1)/2 ts_(n+1) d(r^2).
+__device__ __forceinline__ void chain_d(Dual &ts, int n, double ts_next, double dd2) {
+  ts.d = -0.5 * (2 * n + 1) * ts_next * dd2;
+}
+
+// Sum over the workgroup of a dual.
+__device__ __forceinline__ Dual block_sum(Dual v, double *red) {
+  const double x = block_sum(v.v, red);
+  return Dual(x, block_sum(v.d, red));
+}
+
+// The adjoint's outputs for configuration b: per parameter j of the 16 TTM
+// parameters and the 84 DMS coefficients, adj[b][j] = (dE/dtheta_j,
+// gw . d(dE/dr)/dtheta_j).  Beside the Thole and vdW partials gathered by the
+// pair loops: the polarisabilities (-1/2 |mu|^2 / alpha^2 per dipole site,
+// the alpha^-1 term of the stationary induction energy) and the charge
+// parameters (sum of phi dq/dtheta over the charged sites, phi the potential
+// of charges and dipoles, which is dE/dq).  Each is a dual along the atoms'
+// direction gw, the second part the derivative of the first along it.
+// (gammaM, the last of TTM_ADJ_NPARAM, moves the M sites: its slot is filled
+// by the host from a tangent sweep.)
+__device__ void adjoint_out(const SysConst &c, const TanArgs &a, const TanPar &P, const TanWs &W,
+                            int b, const double vE[5], const double vF[5], Dual accCC, Dual accDD,
+                            const Dual accPF[3], double *red) {
+  const int N = c.N, tid = threadIdx.x, nds = c.nd * N, tdip = c.tdip;
+  double *out = a.adj + (size_t)b * TTM_ADJ_NPARAM * 2;
+  const Dual *phi = W.phi, *q = W.q, *dip = W.dip, *dR = W.dR;
+  const double gM = c.gammaM, tq = 0.5 * gM / (1.0 - gM);
+  // polarisabilities by site type (O, H, M)
+  Dual pol[3] = {0.0, 0.0, 0.0};
+  for (int i = tid; i < nds; i += TAN_BLOCK) {
+    const int t = tdip + i / N, pt = t == 0 ? 0 : t == 3 ? 2 : 1;
+    const double al = P.polar[t].v;
+    const Dual *mu = dip + 3 * i;
+    const Dual m2 = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+    if (al > 0.0) pol[pt] -= 0.5 * m2 / (al * al);
+  }
+  const Dual sCC = block_sum(accCC, red), sDD = block_sum(accDD, red);
+  Dual sPF[3], sPol[3];
+  for (int i = 0; i < 3; ++i) {
+    sPF[i] = block_sum(accPF[i], red);
+    sPol[i] = block_sum(pol[i], red);
+  }
+  double sE[5], sF[5];
+  for (int i = 0; i < 5; ++i) {
+    sE[i] = block_sum(vE[i], red);
+    sF[i] = block_sum(vF[i], red);
+  }
+  if (tid == 0) {
+    auto put = [&](int j, Dual v) {
+      out[2 * j] = v.v;
+      out[2 * j + 1] = v.d;
+    };
+    for (int i = 0; i < 5; ++i) put(TP_VDWA + i, Dual(sE[i] + (i < 3 ? a.lrc_coef[i] : 0.0), sF[i]));
+    for (int i = 0; i < 3; ++i) put(TP_POLARO + i, sPol[i]);
+    for (int i = 0; i < 3; ++i) put(TP_POLFACO + i, sPF[i]);
+    put(TP_ADD, sDD);
+    put(TP_ACC, sCC);
+    put(TTM_ADJ_NPARAM - 1, Dual(0.0));
+  }
+  // the charge parameters: dms_param1..3, then the 84 coefficients coefD;
+  // one parameter per lane over the molecules
+  for (int j = tid; j < 3 + DMS_NCOEF; j += TAN_BLOCK) {
+    Dual acc = 0.0;
+    for (int w = 0; w < N; ++w) {
+      const double *rr = a.RR + ((size_t)b * 3 * N + 3 * w) * 3;
+      const double *g = a.gw + ((size_t)b * 3 * N + 3 * w) * 3;
+      double r[3][3], dq3[3], dgq[3][3][3];
+      for (int at = 0; at < 3; ++at)
+        for (int k = 0; k < 3; ++k) r[at][k] = rr[3 * at + k];
+      if (j < 3) dms_tangent(c, r, j, dq3, dgq);
+      else dms_coef_tangent(c, r, j - 3, dq3, dgq);
+      // dq3 in (O, H1, H2), dgq[disp][charge] in (H1, H2, O); along gw
+      const int atom[3] = {1, 2, 0};
+      Dual dqd[3];
+      for (int i = 0; i < 3; ++i) {
+        const int ch = i == 0 ? 2 : i - 1;
+        double v = 0.0;
+        for (int d = 0; d < 3; ++d)
+          for (int k = 0; k < 3; ++k) v += dgq[d][ch][k] * g[3 * atom[d] + k];
+        dqd[i] = Dual(dq3[i] * CHARGECON, v * CHARGECON);
+      }
+      const Dual sH = dqd[1] + dqd[2];
+      acc += phi[N + w] * (dqd[1] + tq * sH) + phi[2 * N + w] * (dqd[2] + tq * sH) +
+             phi[3 * N + w] * dqd[0] * (1.0 / (1.0 - gM));
+    }
+    out[2 * (TP_DMS1 + j)] = acc.v;
+    out[2 * (TP_DMS1 + j) + 1] = acc.d;
+  }
+}
+
+// MODE TAN_PARAMS: one parameter direction per workgroup, plain FP64
+// geometry (GT = double); TAN_GAMMA_M and TAN_ADJOINT: dual site positions
+// and phases (GT = Dual) along dr/dgammaM or along the atoms' direction gw.
+template <int MODE>
 __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a) {
+  constexpr bool GD = MODE != TAN_PARAMS;
   using GT = std::conditional_t<GD, Dual, double>;
   const int b = blockIdx.x, dir = blockIdx.y, ndir = gridDim.y, tid = threadIdx.x;
   const int N = c.N, ns = 4 * N, nd = c.nd, tdip = c.tdip, nds = nd * N, nk = c.nk;
@@ -449,16 +611,16 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
   ph.kmax = c.kmax;
   ph.lmax = c.lmax;
   ph.N = N;
-  ph.dM = Rs + a.dm_off;
+  ph.dS = Rs + a.ds_off;
   ph.tx = (const double2 *)(Rs + (size_t)ns * 3);
   ph.ty = ph.tx + (size_t)(c.hmax + 1) * ns;
   ph.tz = ph.ty + (size_t)(2 * c.kmax + 1) * ns;
-  // site s's position, with its derivative along gammaM in the GD instance
+  // site s's position, with its derivative along the geometric direction in
+  // the dual-geometry instances
   auto site = [&](int s, GT r[3]) {
     for (int k = 0; k < 3; ++k) r[k] = Rs[3 * s + k];
     if constexpr (GD)
-      if (s >= 3 * N)
-        for (int k = 0; k < 3; ++k) r[k].d = ph.dM[3 * (s - 3 * N) + k];
+      for (int k = 0; k < 3; ++k) r[k].d = ph.dS[3 * s + k];
   };
   const TanWs W(a.ws + ((size_t)b * ndir + dir) * a.ws_stride, c);
   Dual *q = W.q, *phi = W.phi, *Efq = W.Efq, *dR = W.dR, *dip = W.dip, *old = W.old,
@@ -467,9 +629,8 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
   __shared__ TanPar P;
   __shared__ double red[TAN_BLOCK];
   if (tid == 0) {
-    auto par = [&](int i) {
-      return Dual(a.theta[i], (!a.dms_coef && !a.gamma_m && i == dir) ? 1.0 : 0.0);
-    };
+    const bool seed = MODE == TAN_PARAMS && !a.dms_coef;
+    auto par = [&](int i) { return Dual(a.theta[i], (seed && i == dir) ? 1.0 : 0.0); };
     for (int i = 0; i < 5; ++i) P.vdw[i] = par(i);
     P.polar[0] = par(5);
     P.polar[1] = par(6);
@@ -480,6 +641,11 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
                         dpow(par(10), 1.0 / 6.0)};
     Dual pf[4];
     for (int i = 0; i < 4; ++i) pf[i] = Dual(1.0) / (f6[i] * f6[i] * f6[i]);
+    for (int i = 0; i < 4; ++i) {
+      const double pfi = a.theta[TP_POLFACO + (i == 0 ? 0 : i == 3 ? 2 : 1)];
+      P.pf[i] = pf[i].v;
+      P.dpf[i] = -0.5 * pf[i].v / pfi;  // d polfac^(-1/2) / d polfac
+    }
     const Dual aDD = par(11), aCC = par(12);
     for (int i = 0; i < 4; ++i)
       for (int j = 0; j < 4; ++j) {
@@ -487,16 +653,19 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         P.raCC[i][j] = P.apCC[i][j].v > 0.0 ? dpow(P.apCC[i][j], 1.0 / 3.0) : Dual(0.0);
         P.apDD[i][j] = aDD * pf[i] * pf[j];
       }
-    P.lrc = Dual(c.Uvdw_lrc, (!a.dms_coef && !a.gamma_m && dir < 3) ? a.lrc_coef[dir] : 0.0);
+    P.lrc = Dual(c.Uvdw_lrc, (seed && dir < 3) ? a.lrc_coef[dir] : 0.0);
   }
   __syncthreads();
   // gammaM, which redistributes the M-site gradient (pot_ttm.f90:597-602)
   GT gM = c.gammaM;
-  if constexpr (GD) gM.d = 1.0;
+  if constexpr (MODE == TAN_GAMMA_M) gM.d = 1.0;
   // charges, their gradients, the monomer gradients and energy: k_tangent_monomer
 
   // ---- real space: vdW and charge-charge terms (pot_ttm.f90:159-228), per site
   Dual uvdw = 0.0;
+  // adjoint: d Uvdw / d(vdwA..E) and gw . d G / d(vdwA..E) of the gradient as
+  // computed (its vdW force, pot_ttm.f90:176, has no vdwA, vdwB terms)
+  double vE[5] = {0, 0, 0, 0, 0}, vF[5] = {0, 0, 0, 0, 0};
   for (int s = tid; s < ns; s += TAN_BLOCK) {
     const int iw = s % N, ti = s / N;
     const double *ROi = Rs + 3 * iw;
@@ -519,6 +688,21 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
                            Dual(c.shiftA * (x * x) + c.shiftB * (x * x * x)));
         for (int k = 0; k < 3; ++k) gv[k] += fsc * Rij[k];
         if (iw < jw) uvdw += uij;
+        if constexpr (MODE == TAN_ADJOINT) {
+          const double Dv = P.vdw[3].v, Ev = P.vdw[4].v, e = exp(-(Ev * dRij));
+          const double *gO = a.gw + ((size_t)b * 3 * N + 3 * iw) * 3;
+          const double gr = gO[0] * Rij[0] + gO[1] * Rij[1] + gO[2] * Rij[2];
+          if (iw < jw) {
+            vE[0] += R6i * R6i;
+            vE[1] += R6i * R4i;
+            vE[2] += R6i;
+            vE[3] += e;
+            vE[4] -= Dv * dRij * e;
+          }
+          vF[2] -= 6.0 * R8i * gr;
+          vF[3] -= e * Ev / dRij * gr;
+          vF[4] -= Dv * e * (1.0 - Ev * dRij) / dRij * gr;
+        }
       }
       GT Ra[3];
       for (int k = 0; k < 3; ++k) Ra[k] = Ri[k] + (Rij[k] - ROi[k] + ROj[k]);
@@ -531,6 +715,12 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         const GT d2s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         Dual ts0, ts1;
         smear01<true, Dual>(c, as_dual(d2s), P.apCC[ti][tj], P.raCC[ti][tj], ts0, ts1);
+        if constexpr (MODE == TAN_ADJOINT) {
+          double u1, u2;
+          smear2<true, double>(c, d2s.v, P.apCC[ti][tj].v, u1, u2);
+          chain_d(ts0, 0, ts1.v, d2s.d);
+          chain_d(ts1, 1, u2, d2s.d);
+        }
         const Dual qj = q[sj];
         pv += ts0 * qj;
         const Dual f = ts1 * qj;
@@ -556,8 +746,15 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         site(sj, Rj);
         GT r[3] = {Ri[0] - Rj[0], Ri[1] - Rj[1], Ri[2] - Rj[2]};
         minimg(c, r[0], r[1], r[2]);
+        const GT d2x = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         GT ts0, ts1;
-        self1(c, r[0] * r[0] + r[1] * r[1] + r[2] * r[2], ts0, ts1);
+        self1(c, d2x, ts0, ts1);
+        if constexpr (MODE == TAN_ADJOINT) {
+          double u1, u2;
+          self2(c, d2x.v, u1, u2);
+          chain_d(ts0, 0, ts1.v, d2x.d);
+          chain_d(ts1, 1, u2, d2x.d);
+        }
         phi[si] += ts0 * q[sj];
         phi[sj] += ts0 * q[si];
         for (int k = 0; k < 3; ++k) {
@@ -648,7 +845,14 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
           for (int k = 0; k < 3; ++k) r[k] = Ri[k] + R0[k] - Rj[k];
           minimg(c, r[0], r[1], r[2]);
           Dual t1, t2, o[3];
-          smear2<true, Dual>(c, as_dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
+          const GT r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+          smear2<true, Dual>(c, as_dual(r2), P.apDD[ti][tj], t1, t2);
+          if constexpr (MODE == TAN_ADJOINT) {
+            double u1, u2, u3;
+            smear3<true, double>(c, r2.v, P.apDD[ti][tj].v, u1, u2, u3);
+            chain_d(t1, 1, t2.v, r2.d);
+            chain_d(t2, 2, u3, r2.d);
+          }
           dd3d(r, t1, t2, dip + 3 * j, o);
           for (int k = 0; k < 3; ++k) e[k] += o[k];
         }
@@ -662,7 +866,14 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
         GT r[3] = {Ri[0] - Rj[0], Ri[1] - Rj[1], Ri[2] - Rj[2]};
         minimg(c, r[0], r[1], r[2]);
         Dual t1, t2, o[3];
-        smear2<true, Dual>(c, as_dual(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), P.apDD[ti][tj], t1, t2);
+        const GT r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        smear2<true, Dual>(c, as_dual(r2), P.apDD[ti][tj], t1, t2);
+        if constexpr (MODE == TAN_ADJOINT) {
+          double u1, u2, u3;
+          smear3<true, double>(c, r2.v, P.apDD[ti][tj].v, u1, u2, u3);
+          chain_d(t1, 1, t2.v, r2.d);
+          chain_d(t2, 2, u3, r2.d);
+        }
         dd3d(r, t1, t2, dip + 3 * j, o);
         for (int k = 0; k < 3; ++k) e[k] += o[k];
       }
@@ -722,6 +933,18 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
 
   // ---- Uind (pot_ttm.f90:390) and the final real-space forces (pot_ttm.f90:399-542)
   Dual uind = 0.0;
+  // adjoint: d U / d aCCaCD, aDD, polfacO/H/M at fixed dipoles, through the
+  // Thole factors a = a0 polfac_i^(-1/2) polfac_j^(-1/2) of each site pair
+  Dual accCC = 0.0, accDD = 0.0, accPF[3] = {0.0, 0.0, 0.0};
+  auto ptype = [](int t) { return t == 0 ? 0 : t == 3 ? 2 : 1; };
+  auto thole_acc = [&](Dual w, bool ddt, int ti, int tj) {
+    const double pi = P.pf[ti], pj = P.pf[tj];
+    if (ddt) accDD += w * (pi * pj);
+    else accCC += w * (pi * pj);
+    const Dual wa = w * a.theta[ddt ? TP_ADD : TP_ACC];
+    accPF[ptype(ti)] += wa * (pj * P.dpf[ti]);
+    accPF[ptype(tj)] += wa * (pi * P.dpf[tj]);
+  };
   for (int i = tid; i < nds; i += TAN_BLOCK) {
     const int s = (tdip + i / N) * N + i % N;
     for (int k = 0; k < 3; ++k) uind += dip[3 * i + k] * Efq[3 * s + k];
@@ -785,12 +1008,52 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
           qd = qsq > 1e-9;
           if (qd) smear2<true, Dual>(c, as_dual(d2), P.apCC[ti][tj], ts1, ts2);
         }
+        // the potential of the dipoles (phi, dE/dq) along the direction; the
+        // gradient terms (g) only feed dE/dr, which the adjoint does not use
+        if constexpr (MODE == TAN_ADJOINT) chain_d(ts1, 1, ts2.v, as_dual(d2).d);
         if (qd) {
           Dual o[3];
           dd3d(r, ts1, ts2, qdqd, o);
           for (int k = 0; k < 3; ++k) g[k] += o[k];
         }
         pv += ts1 * (dj[0] * r[0] + dj[1] * r[1] + dj[2] * r[2]);
+        if constexpr (MODE == TAN_ADJOINT) {
+          // The energy is stationary in the converged dipoles, so its
+          // derivative is the explicit one at fixed dipoles: per ordered site
+          // pair 1/2 qi qj dts0/da (Uelec), -qj (di . r) dts1/da (-mu . Efq)
+          // and -1/2 di . dT/da . dj (-1/2 mu T mu); only the Thole terms
+          // (r < 5 A, smear.f90:23,57) depend on a.
+          if (d2.v < 25.0) {
+            const bool cc = !same && qi.v != 0.0 && qj.v != 0.0, cd = !same && adip && qj.v != 0.0;
+            const double av = P.apCC[ti][tj].v, rav = P.raCC[ti][tj].v;
+            const Dual d2d = as_dual(d2);
+            if ((cc || cd) && av > 0.0) {
+              HD t0h, t1h, u1h, u2h, u3h;
+              smear01<true, HD>(c, HD(d2d, Dual(0.0)), HD(Dual(av), Dual(1.0)),
+                                HD(Dual(rav), Dual(rav / (3.0 * av))), t0h, t1h);
+              smear3<true, HD>(c, HD(d2d, Dual(0.0)), HD(Dual(av), Dual(1.0)), u1h, u2h, u3h);
+              Dual da0 = t0h.a, da1 = u1h.a;
+              chain_d(da0, 0, u1h.a.v, d2d.d);
+              chain_d(da1, 1, u2h.a.v, d2d.d);
+              Dual w = 0.0;
+              if (cc) w += 0.5 * qi * qj * da0;
+              if (cd) w -= qj * (di[0] * r[0] + di[1] * r[1] + di[2] * r[2]) * da1;
+              thole_acc(w, false, ti, tj);
+            }
+            if (adip && bdip) {
+              HD u1h, u2h, u3h;
+              smear3<true, HD>(c, HD(d2d, Dual(0.0)), HD(Dual(P.apDD[ti][tj].v), Dual(1.0)), u1h,
+                               u2h, u3h);
+              Dual da1 = u1h.a, da2 = u2h.a;
+              chain_d(da1, 1, u2h.a.v, d2d.d);
+              chain_d(da2, 2, u3h.a.v, d2d.d);
+              const Dual dir_ = di[0] * r[0] + di[1] * r[1] + di[2] * r[2];
+              const Dual djr = dj[0] * r[0] + dj[1] * r[1] + dj[2] * r[2];
+              const Dual dd = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+              thole_acc(-0.5 * (3.0 * da2 * dir_ * djr - da1 * dd), true, ti, tj);
+            }
+          }
+        }
       }
     }
     for (int k = 0; k < 3; ++k) dR[3 * s + k] += g[k];
@@ -874,12 +1137,14 @@ __global__ __launch_bounds__(TAN_BLOCK) void k_ttm_tangent(SysConst c, TanArgs a
     o[2] = a.ew[b] * Ed + lg;
     o[3] = (double)iters;
   }
+  if constexpr (MODE == TAN_ADJOINT) adjoint_out(c, a, P, W, b, vE, vF, accCC, accDD, accPF, red);
 }
 
 void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s) {
-  const size_t dm_off = 3 * 4 * (size_t)c.N + tangent_tab_doubles(c);
+  const size_t ds_off = 3 * 4 * (size_t)c.N + tangent_tab_doubles(c);
+  const int mode = L.adjoint ? 2 : L.gamma_m ? 1 : 0;
   hipLaunchKernelGGL(k_tangent_prep, dim3(c.P), dim3(TAN_BLOCK), 0, s, c, L.RR, L.cfg,
-                     tangent_cfg_doubles(c), dm_off);
+                     tangent_cfg_doubles(c), ds_off, mode, L.gw);
   TanArgs a;
   a.RR = L.RR;
   a.pes = L.pes;
@@ -897,13 +1162,17 @@ void launch_ttm_tangent(const SysConst &c, const TanLaunch &L, hipStream_t s) {
   for (int i = 0; i < 3; ++i) a.lrc_coef[i] = L.lrc_coef[i];
   a.dms_coef = L.dms_coef;
   a.gamma_m = L.gamma_m;
-  a.dm_off = dm_off;
-  if (L.gamma_m) {
-    hipLaunchKernelGGL(k_tangent_monomer<true>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
-    hipLaunchKernelGGL(k_ttm_tangent<true>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
+  a.ds_off = ds_off;
+  a.adj = L.adj;
+  if (mode == TAN_ADJOINT) {
+    hipLaunchKernelGGL(k_tangent_monomer<TAN_ADJOINT>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
+    hipLaunchKernelGGL(k_ttm_tangent<TAN_ADJOINT>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
+  } else if (mode == TAN_GAMMA_M) {
+    hipLaunchKernelGGL(k_tangent_monomer<TAN_GAMMA_M>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
+    hipLaunchKernelGGL(k_ttm_tangent<TAN_GAMMA_M>, dim3(c.P, 1), dim3(TAN_BLOCK), 0, s, c, a);
   } else {
-    hipLaunchKernelGGL(k_tangent_monomer<false>, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
-    hipLaunchKernelGGL(k_ttm_tangent<false>, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
+    hipLaunchKernelGGL(k_tangent_monomer<TAN_PARAMS>, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
+    hipLaunchKernelGGL(k_ttm_tangent<TAN_PARAMS>, dim3(c.P, L.ndir), dim3(TAN_BLOCK), 0, s, c, a);
   }
 }
 

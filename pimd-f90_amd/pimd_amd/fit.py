@@ -54,7 +54,12 @@ class FitGroup:
         self.E_target = np.asarray(E_target, dtype=np.float64)
         self.G_target = np.ascontiguousarray(G_target, dtype=np.float64)
 
-    def evaluate(self, theta, grad: bool):
+    def evaluate(self, theta, grad: bool, method: str = "adjoint"):
+        """Loss of this group and, with grad, its gradient.  method "adjoint":
+        the TTM, DMS and gammaM parts from one adjoint sweep
+        (pimd_ttm_adjoint, exact up to the SOR convergence); "tangent": from
+        their tangent sweeps (one per parameter, exact for the SOR iterations
+        as executed)."""
         pes, ttm, dms, gm = split_theta(theta)
         self.ff.set_pes_params(pes)
         if ttm is not None:
@@ -70,6 +75,12 @@ class FitGroup:
         g = None
         if grad:
             g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)
+            if method == "adjoint" and ttm is not None:
+                n = TTM_NPARAM + (DMS_NCOEF if dms is not None else 0) + (gm is not None)
+                ga = self.ff.ttm_adjoint(self.RRt, 2.0 * dE, 2.0 * dG, n)[0]
+                return loss, np.concatenate([g, ga])
+            if method not in ("tangent", "adjoint"):
+                raise ValueError(f"method must be 'tangent' or 'adjoint', got {method!r}")
             if ttm is not None:
                 g = np.concatenate([g, self.ff.ttm_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)[0]])
             if dms is not None:
@@ -79,11 +90,12 @@ class FitGroup:
         return loss, g
 
 
-def loss_and_grad(groups, theta, grad: bool = True):
-    """Loss over every group and, with grad, its gradient dL/dtheta."""
+def loss_and_grad(groups, theta, grad: bool = True, method: str = "adjoint"):
+    """Loss over every group and, with grad, its gradient dL/dtheta (method:
+    see FitGroup.evaluate)."""
     total, g = 0.0, np.zeros(np.asarray(theta).size) if grad else None
     for grp in groups:
-        l, gg = grp.evaluate(theta, grad)
+        l, gg = grp.evaluate(theta, grad, method)
         total += l
         if grad:
             g += gg
@@ -99,20 +111,21 @@ def _autograd_function():
         those and gammaM."""
 
         @staticmethod
-        def forward(ctx, theta, groups):
+        def forward(ctx, theta, groups, method="adjoint"):
             th = theta.detach().cpu().numpy().astype(np.float64)
-            loss, g = loss_and_grad(groups, th, grad=True)
+            loss, g = loss_and_grad(groups, th, grad=True, method=method)
             ctx.save_for_backward(torch.from_numpy(g).to(theta.device))
             return theta.new_tensor(loss)
 
         @staticmethod
         def backward(ctx, gout):
             (g,) = ctx.saved_tensors
-            return gout * g, None
+            return gout * g, None, None
 
     return PESLoss
 
 
-def pes_loss(theta, groups):
-    """Differentiable loss of a torch tensor theta (see PESLoss)."""
-    return _autograd_function().apply(theta, groups)
+def pes_loss(theta, groups, method: str = "adjoint"):
+    """Differentiable loss of a torch tensor theta (see PESLoss; method: see
+    FitGroup.evaluate)."""
+    return _autograd_function().apply(theta, groups, method)

@@ -39,7 +39,7 @@ EXPORTS = (
     "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
     "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles", "pimd_dms_coefs_get",
     "pimd_dms_coefs_set", "pimd_dms_coef_grad", "pimd_gamma_m_get", "pimd_gamma_m_set",
-    "pimd_gamma_m_grad",
+    "pimd_gamma_m_grad", "pimd_ttm_adjoint",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -49,6 +49,8 @@ TTM_PARAM_NAMES = ("vdwA", "vdwB", "vdwC", "vdwD", "vdwE", "polarO", "polarH", "
                    "dms_param3")
 TTM_NPARAM = len(TTM_PARAM_NAMES)
 DMS_NCOEF = 84     # coefD of the dipole moment surface (nasa_mod.f90)
+# the adjoint's parameters: TTM parameters, DMS coefficients, gammaM
+ADJ_NPARAM = TTM_NPARAM + DMS_NCOEF + 1
 # per-step observables of write_out (pimd_md_observables)
 OBS_NAMES = ("sys_temp", "virial_energy", "virial_press", "simple_energy", "simple_press",
              "dip_x", "dip_y", "dip_z", "dip_mag_sum")
@@ -127,6 +129,7 @@ def load_library(path: str = LIB_PATH):
         "pimd_gamma_m_get": [vp, dp],
         "pimd_gamma_m_set": [vp, C.c_double],
         "pimd_gamma_m_grad": [vp, dp, dp, dp, dp, dp, dp],
+        "pimd_ttm_adjoint": [vp, dp, dp, dp, C.c_int, dp, dp, dp],
         "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
@@ -346,6 +349,18 @@ class TTMForceField:
         _check(load_library().pimd_gamma_m_grad(self._h, _ptr(r), _ptr(e), _ptr(g), _ptr(grad),
                                                 _ptr(E), _ptr(dE)))
         return float(grad[0]), E, dE
+
+    def ttm_adjoint(self, RRt, ew, gw, nparam: int = ADJ_NPARAM):
+        """The adjoint sweep over the first nparam (16, 100 or 101) of the TTM
+        parameters, the DMS coefficients and gammaM (pimd_ttm_adjoint):
+        (grad, E, dE/dtheta), the parameters in the order TTM_PARAM_NAMES,
+        coefD, gammaM."""
+        P, na = self.params.nbeads, self.params.natoms
+        r, e, g = _arr(RRt, (P, na, 3)), _arr(ew, (P,)), _arr(gw, (P, na, 3))
+        grad, E, dE = np.zeros(nparam), np.zeros(P), np.zeros((P, nparam))
+        _check(load_library().pimd_ttm_adjoint(self._h, _ptr(r), _ptr(e), _ptr(g), int(nparam),
+                                               _ptr(grad), _ptr(E), _ptr(dE)))
+        return grad, E, dE
 
     def potential(self, RR, RRc, t: int = 1) -> dict:
         """One bead (force_calc.f90:112-170): the reference's per-rank call."""

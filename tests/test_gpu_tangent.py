@@ -103,7 +103,7 @@ def fitset(request):
     from pimd_amd.fit import loss_and_grad
     groups, theta0, pes0 = _groups(request.param)
     full = np.concatenate([pes0, theta0])
-    L0, g = loss_and_grad(groups, full)
+    L0, g = loss_and_grad(groups, full, method="tangent")
     return request.param, groups, theta0, pes0, L0, g[pes0.size:]
 
 
@@ -237,7 +237,7 @@ def test_fit_step_reduces_loss(fitset):
     model, groups, theta0, pes0, L0, g = fitset
     full = np.concatenate([pes0, theta0])
     th = torch.tensor(full, dtype=torch.float64, requires_grad=True)
-    loss = pes_loss(th, groups)
+    loss = pes_loss(th, groups, method="tangent")
     loss.backward()
     gall = th.grad.numpy()
     assert np.allclose(gall[pes0.size:], g, rtol=1e-12, atol=0)
@@ -440,7 +440,7 @@ def test_fit_vector_with_dms_and_gamma_m(fitset):
     c0, gm0 = groups[0].ff.dms_coefs(), groups[0].ff.gamma_m()
     full = np.concatenate([pes0, theta0, c0, [gm0]])
     assert full.size == NPARAM_GM
-    L, gall = loss_and_grad(groups, full)
+    L, gall = loss_and_grad(groups, full, method="tangent")
     assert abs(L - L0) <= 1e-12 * L0
     assert np.allclose(gall[pes0.size:pes0.size + theta0.size], g, rtol=1e-12, atol=0)
     ggm = sum(grp.ff.gamma_m_grad(grp.RRt, *_weights(grp))[0] for grp in groups)
