@@ -51,6 +51,19 @@ WORKLOADS = {
 MODEL_NAMES = {2: "TTM2.1-F", 3: "TTM3-F", 4: "q-SPC/Fw", 5: "SPC/F"}
 
 
+def byte_model(kernel, c, fp32):
+    """Algorithmic HBM bytes per launch of the kernels whose bound is memory:
+    the SOR dipole pass with its pair screening factors kept from the first
+    iteration (PIMD_DD_CACHE, on by default) reads, per neighbour-list entry
+    of every live bead, the partner index (4 bytes) and the (ts1, ts2) of
+    each dipole-site pair (2 x 4 or 2 x 8 bytes); per dipole site it stages
+    the position and dipole (48 bytes) and writes the field (24 bytes)."""
+    if kernel != "pair_dd" or os.environ.get("PIMD_DD_CACHE", "1") == "0":
+        return None
+    N, P, nnb, nd = c["N"], c["P"], c["nnb"], c["nd"]
+    return P * N * nnb * (4 + nd * nd * 2 * (4 if fp32 else 8)) + P * N * nd * (48 + 24)
+
+
 def flop_model(kernel, c):
     """Algorithmic FP64/FP32 operations per launch: what the algorithm needs,
     not what the kernel issues.  A fused multiply-add counts two, a
@@ -225,6 +238,16 @@ def run_form(p, RRt, PPt, RRc, args):
                                            if issued else None),
                            "traffic_source": "profiles/hbm_traffic_h2o512_p32_ttm3f.md",
                            "issued_source": "profiles/valu_counters_h2o512_p32_ttm3f.md"}
+        nb = byte_model(dom, {**c, "P": active} if dom in SOR_KERNELS else c, fp32)
+        if nb is not None:
+            # a memory-bound kernel: its roofline is HBM bandwidth, the flop
+            # figures above stay as a second view
+            gbs = nb / avg_s / 1e9
+            r = out["roofline"]
+            r["valu"] = {k: r.pop(k) for k in ("bound", "achieved", "peak", "unit", "frac")}
+            out["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                               "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                               "traffic": traffic, "bytes_per_launch": int(nb), **r}
     ff.close()
     return out
 
