@@ -132,6 +132,23 @@ def build_system(workload, seed=2024):
     return p, RRt, PPt, RRc
 
 
+def host_cpu():
+    """The host CPU's model name and the logical CPUs this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count()
+    return {"model": model, "logical_cpus": ncpu}
+
+
 def cpu_baseline(p, RRt, RRc, budget_s=15.0):
     """The reference Fortran (oracle/_ref) on a bounded sample of beads of the
     same system, one core; falls back to the C restatement."""
@@ -161,7 +178,7 @@ def cpu_baseline(p, RRt, RRc, budget_s=15.0):
         oracle.System(q).forces(RRt[:nb], RRc)
     dt = time.perf_counter() - t0
     return {"value": p.nwaters * nb / dt, "unit": "molecule-bead force evals/s", "cores": 1,
-            "kind": kind,
+            "kind": kind, "host": host_cpu(),
             "sample": f"{'pot_spc' if p.pot_model >= 4 else 'pot_ttm'} on {nb} of the {p.nbeads} beads of the same system "
                       f"({'reference Fortran, amdflang -O2' if kind == 'reference' else 'C restatement, gcc -O2'}), "
                       f"serial, {dt:.1f} s"}
@@ -345,7 +362,7 @@ def cpu_parallel_baseline(p, RRt, RRc, threads=16, budget_s=10.0):
         list(ex.map(work, range(threads)))
     dt = time.perf_counter() - t0
     return {"value": p.nwaters * nb / dt, "unit": "molecule-bead force evals/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"C restatement (gcc -O2) on {nb} of the {p.nbeads} beads, one bead per "
                       f"thread at a time, {threads} threads, {dt:.1f} s"}
 
