@@ -525,6 +525,10 @@ void plan_system(pimd_ctx *x, HostTables &t) {
   c.g_fin = env_int("PIMD_G_FIN", dflt_qq, 0, 32);
   c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
   c.overlap = 0;
+  // the charge pair pass evaluates each molecule pair once (not with the
+  // cell-list neighbour search, whose lists are not in partner order)
+  c.qq_half = (c.model == PIMD_TTM21F || c.model == PIMD_TTM3F) && !c.cells &&
+              env_int("PIMD_QQ_HALF", 1, 0, 1);
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
 }
 
@@ -566,6 +570,8 @@ void layout_system(pimd_ctx *x, Arena &a, bool full) {
   a.take(&d.nbr, P * N * (size_t)c.nbr_cap);
   a.take(&d.nbrc, P * N);
   a.take(&d.nbrn, P * N);
+  a.take(&d.nbrh, P * N * 4);
+  if (c.qq_half) a.take(&d.qqslab, qq_slab_elems(c));
   a.take(&d.tab, P * (size_t)c.T * S);
   a.take(&d.tabd, P * (size_t)c.T * N * nd);
   if (c.mixed) a.take(&d.tabdf, P * (size_t)c.T * N * nd);

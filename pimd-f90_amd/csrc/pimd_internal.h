@@ -72,6 +72,7 @@ struct SysConst {
   int poll;                // 1: read the active-bead count back after every SOR iteration
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   int dd_cache;            // 1: the SOR dipole pass keeps each pair's screening factors
+  int qq_half;             // 1: the charge pair pass evaluates each molecule pair once
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
@@ -121,6 +122,13 @@ struct DevBufs {
   dptr<double> hist = nullptr;  // [P][4][N*nd][3]
   dptr<int> nbr = nullptr, nbrc = nullptr;  // [P][N][cap], [P][N]
   dptr<int> nbrn = nullptr;  // [P][N] near partners (listed first)
+  // [P][N][4]: per part of the list (near, far) the entries with a partner
+  // below the molecule and those up to the end of its half window (the
+  // partners whose pair it owns, see pair_half_window)
+  dptr<int> nbrh = nullptr;
+  // the charge pair pass's partner-side sums, one slab per block of a bead:
+  // [P][blocks][rows][QQ_NV][2] two-word fixed point (2^-32, 2^-64)
+  dptr<long long> qqslab = nullptr;
   dptr<int> cell_start = nullptr, cell_mol = nullptr;  // [P][ncells + 1], [P][N] cell list
   dptr<double2> tab = nullptr;  // [P][T][4N]
   dptr<double2> tabd = nullptr;  // [P][T][nd N] the dipole sites' rows
@@ -333,6 +341,32 @@ bool pair_dd_tests(const SysConst &c);
 // (ts1, ts2) pairs of the SOR dipole pass's screening-factor cache
 inline size_t dd_cache_elems(const SysConst &c) {
   return (size_t)c.P * c.N * (size_t)c.nbr_cap * c.nd * c.nd;
+}
+// The charge pair pass that evaluates each molecule pair once: blocks of
+// QQ_HALF_BLOCK lanes, at most 16 lanes per molecule, and per molecule QQ_NV
+// partner-side values (potentials of the three charged sites, fields of up to
+// four dipole-carrying sites, the oxygen's vdW force)
+constexpr int QQ_HALF_BLOCK = 512;
+constexpr int QQ_NV = 18;
+__host__ __device__ inline int qq_half_lanes(const SysConst &c) { return c.g_qq <= 4 ? 4 : c.g_qq <= 8 ? 8 : 16; }
+// Molecules one block touches with G lanes per molecule: its owners
+// [i0, i0 + QQ_HALF_BLOCK / G) and their partners, which lie at most N/2
+// after their owner; rows are counted from i0 modulo N.
+__host__ __device__ inline int qq_half_rows(int N, int G) {
+  const int w = QQ_HALF_BLOCK / G + N / 2;
+  return w < N ? w : N;
+}
+__host__ __device__ inline int qq_half_blocks(int N, int G) {
+  return (N + QQ_HALF_BLOCK / G - 1) / (QQ_HALF_BLOCK / G);
+}
+// [P][blocks][rows][QQ_NV][2]: the largest over the lane counts
+inline size_t qq_slab_elems(const SysConst &c) {
+  size_t m = 0;
+  for (int G = 4; G <= 16; G *= 2) {
+    const size_t e = (size_t)qq_half_blocks(c.N, G) * qq_half_rows(c.N, G);
+    m = e > m ? e : m;
+  }
+  return (size_t)c.P * m * QQ_NV * 2;
 }
 void launch_refine_prep(const Group &g, hipStream_t s);
 void launch_pair_final(const Group &g, hipStream_t s);

@@ -98,6 +98,31 @@ __device__ __forceinline__ void walk_list(const int *__restrict__ list, int firs
   }
 }
 
+// The molecule pairs of a bead split evenly between their two molecules:
+// molecule i owns the pair (i, j) when (j - i) mod N lies in [1, (N-1)/2], or
+// equals N/2 (N even) and i < j.  Returns the last j of i's window unwrapped,
+// i + E(i), so that i owns the partners j in (i, i + E(i)] (mod N).
+__host__ __device__ __forceinline__ int pair_half_window(int i, int N) {
+  return i + (N - 1) / 2 + ((N % 2 == 0 && i < N / 2) ? 1 : 0);
+}
+
+// walk_list over two ranges of the list, [s0, e0) then [s1, e1), as one
+// index space: entry n of it (n = first, first + G, ...) is list[s0 + n]
+// below e0 - s0 and list[s1 + n - (e0 - s0)] beyond.
+template <int G, class F>
+__device__ __forceinline__ void walk_ranges(const int *__restrict__ list, int first, int s0, int e0,
+                                            int s1, int e1, F &&f) {
+  const int n0 = e0 - s0, end = n0 + (e1 - s1);
+  auto at = [&](int n) { return list[n < n0 ? s0 + n : s1 + (n - n0)]; };
+  if (first >= end) return;
+  int j = at(first);
+  for (int n = first; n < end; n += G) {
+    const int jn = n + G < end ? at(n + G) : 0;
+    f(j);
+    j = jn;
+  }
+}
+
 // walk_list with D entries in flight, for visits too short to cover one
 // global load: load(at) fetches entry `at` of the list (the partner index and
 // what the visit reads with it) D entries ahead; f(entry, at) visits them in

@@ -410,9 +410,12 @@ __global__ __launch_bounds__(256) void k_neighbors(const Rep *__restrict__ reps,
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
+  int *nbrh = Rp.b.nbrh;
   for (int ii = wv; ii < NB_I; ii += blockDim.x >> 6) {
     const int i = bk.x * NB_I + ii;
     if (i >= N) break;
+    // the last partner of i's half window, wrapped (pair_half_window)
+    const int uw = pair_half_window(i, N) % N;
     double xi, yi, zi, radi;
     if (LDS) {
       xi = lds_o[4 * i]; yi = lds_o[4 * i + 1]; zi = lds_o[4 * i + 2]; radi = lds_o[4 * i + 3];
@@ -422,7 +425,7 @@ __global__ __launch_bounds__(256) void k_neighbors(const Rep *__restrict__ reps,
     }
     const size_t row = (size_t)b * N + i;
     int *out = nbr + row * c.nbr_cap;
-    int nnear = 0, nfar = 0;
+    int nnear = 0, nfar = 0, nlo[4] = {0, 0, 0, 0};
     for (int j0 = 0; j0 < N; j0 += 64) {
       const int j = j0 + lane;
       bool in = false, near = false;
@@ -446,10 +449,19 @@ __global__ __launch_bounds__(256) void k_neighbors(const Rep *__restrict__ reps,
       if (in && !near) out[c.nbr_cap - 1 - (nfar + __popcll(mf & below))] = j;
       nnear += __popcll(mn);
       nfar += __popcll(mf);
+      // entries below i and up to the window's end: the lists are in
+      // ascending partner order
+      const unsigned long long li = __ballot(j < i), lw = __ballot(j <= uw);
+      nlo[0] += __popcll(mn & li);
+      nlo[1] += __popcll(mn & lw);
+      nlo[2] += __popcll(mf & li);
+      nlo[3] += __popcll(mf & lw);
     }
     if (lane == 0) {
       nbrn[row] = nnear;
       nbrc[row] = nfar;
+      if (nbrh)
+        for (int k = 0; k < 4; ++k) nbrh[row * 4 + k] = nlo[k];
     }
   }
 }
@@ -726,6 +738,242 @@ __global__ __launch_bounds__(256) void k_pair_qq(const Rep *__restrict__ reps, B
     tl[T_VIRC] += 0.5 * vc;
     vadd9(tl + T_VIRT, vt);
   }
+}
+
+// k_pair_qq with each molecule pair evaluated once, by the molecule that owns
+// it (pair_half_window; lists in ascending partner order, nbrh giving the
+// owned ranges).  The owner's side is summed as in k_pair_qq; the partner's
+// potentials, fields and vdW force go into a per-block LDS table of
+// two-word fixed-point sums (integer addition, so the result does not depend
+// on the order the lanes add in), written out as the block's slab and added
+// to the partner by k_pair_qq_partner in block order.  The block stages only
+// the molecules it touches: its owners and the N/2 after them.  Site pairs: every charged
+// site of one molecule with every charge-or-dipole site of the other, each
+// feeding both sides (TTM2.1-F adds the pairs with the partner's oxygen,
+// which k_pair_qq takes from the partner's side).  Energies and virials per
+// pair are those k_pair_qq halves from its two sides, taken once.
+// A value v goes in as h = rint(v 2^32) (range +-2^31) and the remainder
+// v - h 2^-32, exact in FP64, as rint(. 2^64): each addition is exact to
+// 2^-65, finer than FP64 rounding of the sums, so the result matches an FP64
+// sum and stays smooth in the model parameters.  Integer addition wraps
+// modulo 2^64 and is associative: only totals must fit.
+constexpr double QQ_HI = 4294967296.0;            // 2^32
+constexpr double QQ_LO = 18446744073709551616.0;  // 2^64
+// partner sites per step of the site loop: unrolled in the FP32 form, one at
+// a time in the FP64 form (as k_pair_qq)
+template <typename R>
+struct QqUnroll {
+  static constexpr int v = 1;
+};
+__device__ __forceinline__ void qq_fix_add(unsigned long long *acc, double v) {
+  const double h = rint(fmin(fmax(v * QQ_HI, -9.2e18), 9.2e18));
+  const double rem = fmin(fmax(v - h * (1.0 / QQ_HI), -1.0 / QQ_HI), 1.0 / QQ_HI);
+  atomicAdd(acc, (unsigned long long)(long long)h);
+  atomicAdd(acc + 1, (unsigned long long)__double2ll_rn(rem * QQ_LO));
+}
+
+template <int MODEL, int G, typename R>
+__global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__restrict__ reps,
+                                                                BlockMap bm) {
+  constexpr int SA0 = MODEL == 2 ? 0 : 1;  // first dipole-carrying (field-target) site type
+  constexpr int OWN = QQ_HALF_BLOCK / G;  // owners per block
+  extern __shared__ Site lds_site[];  // [W][4] sites, then [W][QQ_NV][2] partner sums
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ Rcs = Rp.b.Rcs;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc,
+            *__restrict__ nbrn = Rp.b.nbrn, *__restrict__ nbrh = Rp.b.nbrh;
+  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR, *tally = Rp.b.tally;
+  const int N = c.N, b = bk.y, i0 = bk.x * OWN, W = qq_half_rows(N, G);
+  unsigned long long *jacc = (unsigned long long *)(lds_site + 4 * W);
+  for (int e = threadIdx.x; e < 4 * W; e += blockDim.x) {
+    int m = i0 + e / 4;
+    m -= m >= N ? N : 0;
+    lds_site[e] = site[((size_t)b * N + m) * 4 + e % 4];
+  }
+  for (int e = threadIdx.x; e < W * QQ_NV * 2; e += blockDim.x) jacc[e] = 0ull;
+  __syncthreads();
+  const int a0 = i0 + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  const bool valid = a0 < N;
+  const int a = valid ? a0 : i0;
+  const long grp = (long)b * N + a;
+  // LDS row of a molecule: owners at a - i0, partners up to N/2 beyond
+  auto row = [&](int j) { return j - i0 + (j < i0 ? N : 0); };
+  Site A[4];
+  for (int t = 0; t < 4; ++t) A[t] = lds_site[(a - i0) * 4 + t];
+  const double RcO[3] = {Rcs[a * 12 + 0], Rcs[a * 12 + 1], Rcs[a * 12 + 2]};
+  double ph[4] = {0, 0, 0, 0}, ef[4][3] = {{0}}, dro[3] = {0, 0, 0};
+  double uvdw = 0.0, vc = 0.0, vt[9] = {0};
+  const int *list = nbr + (size_t)grp * c.nbr_cap;
+  // partner-side sums of molecule j: ph[1..3], then ef of sites SA0..3, then dro
+  auto jslot = [&](int rj, int k) { return jacc + ((size_t)rj * QQ_NV + k) * 2; };
+  auto visit = [&](const int j, auto thole_tag) {
+    constexpr bool THOLE = decltype(thole_tag)::value;
+    const bool lo = a < j;
+    const int rj = row(j);
+    const Site B0 = lds_site[rj * 4];
+    double r[3] = {A[0].x - B0.x, A[0].y - B0.y, A[0].z - B0.z};
+    minimg(c, r[0], r[1], r[2]);
+    const double r2 = norm2_exact(r[0], r[1], r[2]);
+    const R r2r = (R)r2;
+    const R dRij = RM<R>::sqrt_(r2r);
+    const R R2i = RM<R>::rcp(r2r), R4i = R2i * R2i, R6i = R2i * R2i * R2i, R8i = R6i * R2i;
+    const R ex = (R)c.vdwD * RM<R>::exp_(-(R)c.vdwE * dRij);
+    const R uij = R6i * (((R)c.vdwA * R6i + (R)c.vdwB * R4i) + (R)c.vdwC) + ex;
+    const R xs = dRij - (R)c.rc1;
+    const R fsc = -(R8i * (R)6.0 * (R)c.vdwC + RM<R>::div(ex * (R)c.vdwE, dRij) +
+                    (R)c.shiftA * (xs * xs) + (R)c.shiftB * (xs * xs * xs));
+    const double d[3] = {(double)(fsc * (R)r[0]), (double)(fsc * (R)r[1]),
+                         (double)(fsc * (R)r[2])};
+    for (int k = 0; k < 3; ++k) {
+      dro[k] += d[k];
+      qq_fix_add(jslot(rj, 3 + 3 * (4 - SA0) + k), -d[k]);
+    }
+    uvdw += (double)uij;
+    {
+      double cx = RcO[0] - Rcs[j * 12 + 0], cy = RcO[1] - Rcs[j * 12 + 1],
+             cz = RcO[2] - Rcs[j * 12 + 2];
+      minimg(c, cx, cy, cz);
+      vc += cx * d[0] + cy * d[1] + cz * d[2];
+    }
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row) vt[col * 3 + row] += d[col] * r[row];
+    const double s0[3] = {r[0] - A[0].x + B0.x, r[1] - A[0].y + B0.y, r[2] - A[0].z + B0.z};
+#pragma unroll QqUnroll<R>::v
+    for (int sb = SA0; sb < 4; ++sb) {
+      const Site Bs = lds_site[rj * 4 + sb];
+      const R qb = (R)Bs.q;
+      const double *rcb = Rcs + (j * 4 + sb) * 3;
+      double jp = 0.0, je[3] = {0.0, 0.0, 0.0};  // the partner's site sb
+#pragma unroll
+      for (int sa = SA0; sa < 4; ++sa) {
+        if (sa == 0 && sb == 0) continue;  // the oxygens carry no charge
+        double v[3] = {A[sa].x - Bs.x, A[sa].y - Bs.y, A[sa].z - Bs.z};
+        minimg(c, v[0], v[1], v[2]);
+        const double d2 = norm2_exact(v[0], v[1], v[2]);
+        const R vr[3] = {(R)v[0], (R)v[1], (R)v[2]};
+        R ts0, ts1;
+        smear01<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]),
+                       (R)(lo ? c.raCC[sa][sb] : c.raCC[sb][sa]), ts0, ts1);
+        const R qa = (R)A[sa].q;
+        // this molecule's site sa from the partner's charge, and the
+        // partner's site sb from this one's (the field along -v)
+        if (sa > 0) ph[sa] += (double)(ts0 * qb);
+        jp += (double)(ts0 * qa);
+        for (int k = 0; k < 3; ++k) {
+          ef[sa][k] += (double)(ts1 * qb * vr[k]);
+          je[k] -= (double)(ts1 * qa * vr[k]);
+        }
+        if (sa > 0 && sb > 0) {
+          double dd[3];
+          for (int k = 0; k < 3; ++k) dd[k] = (double)(-(ts1 * qa * qb * vr[k]));
+          // the mean of the two sides' lever arms (s0 - Rc(j, sb) from this
+          // side, -s0 - Rc(a, sa) with the force reversed from the partner's)
+          const double *rca = Rcs + (a * 4 + sa) * 3;
+          double w[3];
+          for (int k = 0; k < 3; ++k) w[k] = s0[k] + 0.5 * (rca[k] - rcb[k]);
+          vc += w[0] * dd[0] + w[1] * dd[1] + w[2] * dd[2];
+          for (int col = 0; col < 3; ++col)
+            for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dd[col] * v[row];
+        }
+      }
+      if (sb > 0) qq_fix_add(jslot(rj, sb - 1), jp);
+      for (int k = 0; k < 3; ++k) qq_fix_add(jslot(rj, 3 + 3 * (sb - SA0) + k), je[k]);
+    }
+  };
+  if (valid) {
+    // entries below a and up to the end of its window: near, then far
+    const int4 h = make_int4(nbrh[grp * 4], nbrh[grp * 4 + 1], nbrh[grp * 4 + 2], nbrh[grp * 4 + 3]);
+    const int nn = nbrn[grp], nf = nbrc[grp], cap = c.nbr_cap;
+    const bool wrap = pair_half_window(a, N) >= N;
+    // owned near entries: [h.x, h.y), or [h.x, nn) and [0, h.y) when the
+    // window wraps; far entries are stored from the back (the k-th in
+    // ascending partner order at cap - 1 - k): [cap - h.w, cap - h.z), or
+    // [cap - nf, cap - h.z) and [cap - h.w, cap)
+    walk_ranges<G>(list, lane, h.x, wrap ? nn : h.y, 0, wrap ? h.y : 0,
+                   [&](int j) { visit(j, std::true_type{}); });
+    walk_ranges<G>(list, lane, wrap ? cap - nf : cap - h.w, cap - h.z, cap - h.w, wrap ? cap : cap - h.w,
+                   [&](int j) { visit(j, std::false_type{}); });
+  }
+  for (int t = 1; t < 4; ++t) ph[t] = group_sum<G>(ph[t]);
+  for (int t = SA0; t < 4; ++t)
+    for (int k = 0; k < 3; ++k) ef[t][k] = group_sum<G>(ef[t][k]);
+  for (int k = 0; k < 3; ++k) dro[k] = group_sum<G>(dro[k]);
+  uvdw = group_sum<G>(uvdw);
+  vc = group_sum<G>(vc);
+  for (int i = 0; i < 9; ++i) vt[i] = group_sum<G>(vt[i]);
+  if (valid && lane == 0) {
+    const size_t base = ((size_t)b * N + a) * 4;
+    for (int t = 1; t < 4; ++t) phi[base + t] += ph[t];
+    for (int t = SA0; t < 4; ++t)
+      for (int k = 0; k < 3; ++k) Efq[(base + t) * 3 + k] += ef[t][k];
+    for (int k = 0; k < 3; ++k) dR[base * 3 + k] += dro[k];
+    // half of Rc(a, site) . (the site's pair forces) from this side's field;
+    // the partner side's half follows in k_pair_qq_partner
+    for (int t = 1; t < 4; ++t) {
+      const double *rc = Rcs + (a * 4 + t) * 3;
+      vc -= 0.5 * A[t].q * (rc[0] * ef[t][0] + rc[1] * ef[t][1] + rc[2] * ef[t][2]);
+    }
+    double *tl = tally + ((size_t)b * N + a) * T_COUNT;
+    tl[T_UVDW] += uvdw;
+    tl[T_VIRC] += vc;
+    vadd9(tl + T_VIRT, vt);
+  }
+  __syncthreads();
+  long long *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV * 2;
+  for (int e = threadIdx.x; e < W * QQ_NV * 2; e += blockDim.x) slab[e] = (long long)jacc[e];
+}
+
+// The partner-side sums of k_pair_qq_half: per molecule, the rows of the
+// blocks whose window holds it added in block order (exact integers), then onto the potentials, fields
+// and oxygen gradient, with the other half of Rc . F of the centroid virial.
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_pair_qq_partner(const Rep *__restrict__ reps, BlockMap bm) {
+  constexpr int SA0 = MODEL == 2 ? 0 : 1;
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const int N = c.N;
+  const long gid = (long)bk.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)c.P * N) return;
+  const int b = (int)(gid / N), a = (int)(gid % N);
+  const int G = qq_half_lanes(c), own = QQ_HALF_BLOCK / G;
+  const int nblk = qq_half_blocks(N, G), W = qq_half_rows(N, G);
+  const long long *slab = Rp.b.qqslab + (size_t)b * nblk * W * QQ_NV * 2;
+  // modulo 2^64: intermediate wraps are harmless
+  unsigned long long sh[QQ_NV] = {0}, sl[QQ_NV] = {0};
+  for (int q = 0; q < nblk; ++q) {
+    int r = a - q * own;
+    r += r < 0 ? N : 0;
+    if (r >= W) continue;
+    const long long *e = slab + ((size_t)q * W + r) * QQ_NV * 2;
+    for (int k = 0; k < QQ_NV; ++k) {
+      sh[k] += (unsigned long long)e[2 * k];
+      sl[k] += (unsigned long long)e[2 * k + 1];
+    }
+  }
+  double v[QQ_NV];
+  for (int k = 0; k < QQ_NV; ++k)
+    v[k] = (double)(long long)sh[k] * (1.0 / QQ_HI) + (double)(long long)sl[k] * (1.0 / QQ_LO);
+  const size_t base = ((size_t)b * N + a) * 4;
+  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR;
+  for (int t = 1; t < 4; ++t) phi[base + t] += v[t - 1];
+  double vc = 0.0;
+  const Site *st = Rp.b.site + base;
+  const double *Rcs = Rp.b.Rcs;
+  for (int t = SA0; t < 4; ++t) {
+    const double *e = v + 3 + 3 * (t - SA0);
+    for (int k = 0; k < 3; ++k) Efq[(base + t) * 3 + k] += e[k];
+    if (t > 0) {
+      const double *rc = Rcs + (a * 4 + t) * 3;
+      vc -= 0.5 * st[t].q * (rc[0] * e[0] + rc[1] * e[1] + rc[2] * e[2]);
+    }
+  }
+  for (int k = 0; k < 3; ++k) dR[base * 3 + k] += v[3 + 3 * (4 - SA0) + k];
+  Rp.b.tally[((size_t)b * N + a) * T_COUNT + T_VIRC] += vc;
 }
 
 // Bytes of a bead's molecule records staged by the LDS pair kernels.
@@ -2370,10 +2618,48 @@ static void launch_qq_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const 
   }
 }
 
+// LDS of k_pair_qq_half: the bead's sites and the partner-side sums
+static size_t qq_half_lds(const SysConst &c) {
+  const size_t W = qq_half_rows(c.N, qq_half_lanes(c));
+  return W * (sizeof(Site) * 4 + sizeof(long long) * QQ_NV * 2);
+}
+static bool qq_half_on(const SysConst &c) {
+  return c.qq_half && is_ttm(c) && !c.cells && c.g_qq > 0 && qq_half_lds(c) <= PAIR_LDS_MAX;
+}
+
+template <int ML, typename R>
+static void launch_qq_half_v(int G, dim3 grid, const BlockMap &m, size_t lds, const Group &g,
+                             hipStream_t s) {
+  const dim3 B(QQ_HALF_BLOCK);
+  if (G == 4) hipLaunchKernelGGL((k_pair_qq_half<ML, 4, R>), grid, B, lds, s, g.d, m);
+  else if (G == 8) hipLaunchKernelGGL((k_pair_qq_half<ML, 8, R>), grid, B, lds, s, g.d, m);
+  else hipLaunchKernelGGL((k_pair_qq_half<ML, 16, R>), grid, B, lds, s, g.d, m);
+}
+
 void launch_pair_qq(const Group &g, hipStream_t s) {
+  // variant bit 6: the pass that evaluates each pair once (bits 3-5 its lanes)
   launch_over(
-      g, [](const SysConst &c, const MDConst &) { return pair_plan(c, c.g_qq, bead_sites_lds(c)); },
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (qq_half_on(c)) {
+          const int G = qq_half_lanes(c);
+          const int v0 = (c.model == 3 ? 1 : 0) | (c.mixed ? 2 : 0);
+          const int gi = G == 4 ? 1 : G == 8 ? 2 : 3;
+          return Plan{v0 | 64 | (gi << 3), idiv(c.N, QQ_HALF_BLOCK / G), c.P, 1, qq_half_lds(c)};
+        }
+        return pair_plan(c, c.g_qq, bead_sites_lds(c));
+      },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        if (v & 64) {
+          const int gi = (v >> 3) & 7, G = gi == 1 ? 4 : gi == 2 ? 8 : 16;
+          switch (v & 3) {
+            case 0: launch_qq_half_v<2, double>(G, grid, m, lds, g, s); break;
+            case 1: launch_qq_half_v<3, double>(G, grid, m, lds, g, s); break;
+            case 2: launch_qq_half_v<2, float>(G, grid, m, lds, g, s); break;
+            default: launch_qq_half_v<3, float>(G, grid, m, lds, g, s); break;
+          }
+          return;
+        }
         const int gi = v >> 3;
         switch (v & 7) {
           case 0: launch_qq_v<2, double, false>(gi, grid, m, lds, g, s); break;
@@ -2385,6 +2671,16 @@ void launch_pair_qq(const Group &g, hipStream_t s) {
           case 6: launch_qq_v<2, float, true>(gi, grid, m, lds, g, s); break;
           default: launch_qq_v<3, float, true>(gi, grid, m, lds, g, s); break;
         }
+      });
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (!qq_half_on(c)) return Plan{-1, 0, 0, 0, 0};
+        return Plan{c.model == 3 ? 1 : 0, idiv((long)c.P * c.N, 256), 1, 1, 0};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t) {
+        if (v == 1) hipLaunchKernelGGL(k_pair_qq_partner<3>, grid, dim3(256), 0, s, g.d, m);
+        else hipLaunchKernelGGL(k_pair_qq_partner<2>, grid, dim3(256), 0, s, g.d, m);
       });
 }
 
