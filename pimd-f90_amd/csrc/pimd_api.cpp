@@ -526,9 +526,11 @@ void plan_system(pimd_ctx *x, HostTables &t) {
   c.poll = env_int("PIMD_SOR_POLL", 0, 0, 1);
   c.overlap = 0;
   // the charge pair pass evaluates each molecule pair once (not with the
-  // cell-list neighbour search, whose lists are not in partner order)
-  c.qq_half = (c.model == PIMD_TTM21F || c.model == PIMD_TTM3F) && !c.cells &&
-              env_int("PIMD_QQ_HALF", 1, 0, 1);
+  // cell-list neighbour search, whose lists are not in partner order):
+  // 0 never, 1 where its blocks fill the device, 2 always
+  c.qq_half = ((c.model == PIMD_TTM21F || c.model == PIMD_TTM3F) && !c.cells)
+                  ? env_int("PIMD_QQ_HALF", 1, 0, 2)
+                  : 0;
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
 }
 

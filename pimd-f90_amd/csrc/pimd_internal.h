@@ -72,7 +72,7 @@ struct SysConst {
   int poll;                // 1: read the active-bead count back after every SOR iteration
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   int dd_cache;            // 1: the SOR dipole pass keeps each pair's screening factors
-  int qq_half;             // 1: the charge pair pass evaluates each molecule pair once
+  int qq_half;             // once-per-pair charge pass: 0 off, 1 where it fills the device, 2 always
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
@@ -127,8 +127,8 @@ struct DevBufs {
   // partners whose pair it owns, see pair_half_window)
   dptr<int> nbrh = nullptr;
   // the charge pair pass's partner-side sums, one slab per block of a bead:
-  // [P][blocks][rows][QQ_NV][2] two-word fixed point (2^-32, 2^-64)
-  dptr<long long> qqslab = nullptr;
+  // [P][blocks][rows][QQ_NV]: a block's partner sums, rounded once to FP64
+  dptr<double> qqslab = nullptr;
   dptr<int> cell_start = nullptr, cell_mol = nullptr;  // [P][ncells + 1], [P][N] cell list
   dptr<double2> tab = nullptr;  // [P][T][4N]
   dptr<double2> tabd = nullptr;  // [P][T][nd N] the dipole sites' rows
@@ -359,14 +359,14 @@ __host__ __device__ inline int qq_half_rows(int N, int G) {
 __host__ __device__ inline int qq_half_blocks(int N, int G) {
   return (N + QQ_HALF_BLOCK / G - 1) / (QQ_HALF_BLOCK / G);
 }
-// [P][blocks][rows][QQ_NV][2]: the largest over the lane counts
+// [P][blocks][rows][QQ_NV]: the largest over the lane counts
 inline size_t qq_slab_elems(const SysConst &c) {
   size_t m = 0;
   for (int G = 4; G <= 16; G *= 2) {
     const size_t e = (size_t)qq_half_blocks(c.N, G) * qq_half_rows(c.N, G);
     m = e > m ? e : m;
   }
-  return (size_t)c.P * m * QQ_NV * 2;
+  return (size_t)c.P * m * QQ_NV;
 }
 void launch_refine_prep(const Group &g, hipStream_t s);
 void launch_pair_final(const Group &g, hipStream_t s);

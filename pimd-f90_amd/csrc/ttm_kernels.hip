@@ -923,57 +923,53 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
     vadd9(tl + T_VIRT, vt);
   }
   __syncthreads();
-  long long *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV * 2;
-  for (int e = threadIdx.x; e < W * QQ_NV * 2; e += blockDim.x) slab[e] = (long long)jacc[e];
+  // the block's exact row sums, rounded once
+  double *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV;
+  for (int e = threadIdx.x; e < W * QQ_NV; e += blockDim.x)
+    slab[e] = (double)(long long)jacc[2 * e] * (1.0 / QQ_HI) +
+              (double)(long long)jacc[2 * e + 1] * (1.0 / QQ_LO);
 }
 
-// The partner-side sums of k_pair_qq_half: per molecule, the rows of the
-// blocks whose window holds it added in block order (exact integers), then onto the potentials, fields
-// and oxygen gradient, with the other half of Rc . F of the centroid virial.
+// The partner-side sums of k_pair_qq_half: 32 lanes per molecule, lane k
+// adding value k over the blocks whose window holds the molecule, in block
+// order, onto its potential, field or oxygen gradient; the lanes then sum the
+// other half of Rc . F of the centroid virial.
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_pair_qq_partner(const Rep *__restrict__ reps, BlockMap bm) {
   constexpr int SA0 = MODEL == 2 ? 0 : 1;
+  constexpr int NVAL = 3 + 3 * (4 - SA0) + 3;  // values in use
   const Blk bk = block_of(bm);
   const Rep &Rp = reps[bk.r];
   const SysConst &c = Rp.c;
   const int N = c.N;
-  const long gid = (long)bk.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)c.P * N) return;
-  const int b = (int)(gid / N), a = (int)(gid % N);
+  const long m = ((long)bk.x * blockDim.x + threadIdx.x) / 32;
+  const int k = threadIdx.x % 32;
+  const bool valid = m < (long)c.P * N;  // alike over the 32 lanes of a molecule
+  const int b = valid ? (int)(m / N) : 0, a = valid ? (int)(m % N) : 0;
   const int G = qq_half_lanes(c), own = QQ_HALF_BLOCK / G;
   const int nblk = qq_half_blocks(N, G), W = qq_half_rows(N, G);
-  const long long *slab = Rp.b.qqslab + (size_t)b * nblk * W * QQ_NV * 2;
-  // modulo 2^64: intermediate wraps are harmless
-  unsigned long long sh[QQ_NV] = {0}, sl[QQ_NV] = {0};
-  for (int q = 0; q < nblk; ++q) {
-    int r = a - q * own;
-    r += r < 0 ? N : 0;
-    if (r >= W) continue;
-    const long long *e = slab + ((size_t)q * W + r) * QQ_NV * 2;
-    for (int k = 0; k < QQ_NV; ++k) {
-      sh[k] += (unsigned long long)e[2 * k];
-      sl[k] += (unsigned long long)e[2 * k + 1];
-    }
-  }
-  double v[QQ_NV];
-  for (int k = 0; k < QQ_NV; ++k)
-    v[k] = (double)(long long)sh[k] * (1.0 / QQ_HI) + (double)(long long)sl[k] * (1.0 / QQ_LO);
-  const size_t base = ((size_t)b * N + a) * 4;
-  double *phi = Rp.b.phi, *Efq = Rp.b.Efq, *dR = Rp.b.dR;
-  for (int t = 1; t < 4; ++t) phi[base + t] += v[t - 1];
   double vc = 0.0;
-  const Site *st = Rp.b.site + base;
-  const double *Rcs = Rp.b.Rcs;
-  for (int t = SA0; t < 4; ++t) {
-    const double *e = v + 3 + 3 * (t - SA0);
-    for (int k = 0; k < 3; ++k) Efq[(base + t) * 3 + k] += e[k];
-    if (t > 0) {
-      const double *rc = Rcs + (a * 4 + t) * 3;
-      vc -= 0.5 * st[t].q * (rc[0] * e[0] + rc[1] * e[1] + rc[2] * e[2]);
+  if (valid && k < NVAL) {
+    const double *slab = Rp.b.qqslab + (size_t)b * nblk * W * QQ_NV + k;
+    double v = 0.0;
+    for (int q = 0; q < nblk; ++q) {
+      int r = a - q * own;
+      r += r < 0 ? N : 0;
+      if (r < W) v += slab[((size_t)q * W + r) * QQ_NV];
+    }
+    const size_t base = ((size_t)b * N + a) * 4;
+    if (k < 3) {
+      Rp.b.phi[base + 1 + k] += v;
+    } else if (k < 3 + 3 * (4 - SA0)) {
+      const int t = SA0 + (k - 3) / 3, x = (k - 3) % 3;
+      Rp.b.Efq[(base + t) * 3 + x] += v;
+      if (t > 0) vc = -0.5 * Rp.b.site[base + t].q * Rp.b.Rcs[(a * 4 + t) * 3 + x] * v;
+    } else {
+      Rp.b.dR[base * 3 + (k - 3 - 3 * (4 - SA0))] += v;
     }
   }
-  for (int k = 0; k < 3; ++k) dR[base * 3 + k] += v[3 + 3 * (4 - SA0) + k];
-  Rp.b.tally[((size_t)b * N + a) * T_COUNT + T_VIRC] += vc;
+  vc = group_sum<32>(vc);
+  if (valid && k == 0) Rp.b.tally[((size_t)b * N + a) * T_COUNT + T_VIRC] += vc;
 }
 
 // Bytes of a bead's molecule records staged by the LDS pair kernels.
@@ -2623,8 +2619,15 @@ static size_t qq_half_lds(const SysConst &c) {
   const size_t W = qq_half_rows(c.N, qq_half_lanes(c));
   return W * (sizeof(Site) * 4 + sizeof(long long) * QQ_NV * 2);
 }
+// The once-per-pair pass (qq_half 1) where its blocks fill at least half the
+// device: one bead of 512 H2O, the centroid call of contracted MD, gets 8-16
+// blocks and runs faster two-sided (1.12 against 1.04 ms per contracted
+// step).  qq_half 2 takes it wherever it applies.
+constexpr int QQ_HALF_MIN_BLOCKS = 128;
 static bool qq_half_on(const SysConst &c) {
-  return c.qq_half && is_ttm(c) && !c.cells && c.g_qq > 0 && qq_half_lds(c) <= PAIR_LDS_MAX;
+  return c.qq_half && is_ttm(c) && !c.cells && c.g_qq > 0 && qq_half_lds(c) <= PAIR_LDS_MAX &&
+         (c.qq_half == 2 ||
+          (long)c.P * qq_half_blocks(c.N, qq_half_lanes(c)) >= QQ_HALF_MIN_BLOCKS);
 }
 
 template <int ML, typename R>
@@ -2676,7 +2679,7 @@ void launch_pair_qq(const Group &g, hipStream_t s) {
       g,
       [](const SysConst &c, const MDConst &) {
         if (!qq_half_on(c)) return Plan{-1, 0, 0, 0, 0};
-        return Plan{c.model == 3 ? 1 : 0, idiv((long)c.P * c.N, 256), 1, 1, 0};
+        return Plan{c.model == 3 ? 1 : 0, idiv((long)c.P * c.N * 32, 256), 1, 1, 0};
       },
       [&](int v, dim3 grid, const BlockMap &m, size_t) {
         if (v == 1) hipLaunchKernelGGL(k_pair_qq_partner<3>, grid, dim3(256), 0, s, g.d, m);

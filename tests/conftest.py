@@ -11,6 +11,11 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# The library takes the once-per-pair charge pass only where its blocks fill
+# the device; the tests take it wherever it applies (test_gpu_forces also
+# runs the two-sided pass, PIMD_QQ_HALF=0, against the oracle).
+os.environ.setdefault("PIMD_QQ_HALF", "2")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (gfx950)")

@@ -91,6 +91,29 @@ def test_forces_vs_oracle_medium(model, N, P, rc, eps, monkeypatch):
     check_against(r, _oracle_forces(p, RRt, RRc, beads), p, f"ttm{model} N{N} P{P}")
 
 
+@pytest.mark.parametrize("model,N,P", [(2, 128, 4), (3, 128, 3)])
+def test_two_sided_charge_pass_vs_oracle(model, N, P, monkeypatch):
+    """The charge pair pass that takes every pair from both sides
+    (PIMD_QQ_HALF=0), against the oracle, and against the once-per-pair pass
+    (PIMD_QQ_HALF=2) on the same input: the two differ by rounding only."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    RRt, _, RRc, L = make_system(N, P, seed=300 + N + model)
+    p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
+                  eps_ewald=1e-6).resolved()
+    print("case", model, N, P, flush=True)
+    out = {}
+    for mode in ("0", "2"):
+        monkeypatch.setenv("PIMD_QQ_HALF", mode)
+        ff = _ff(p)
+        out[mode] = ff.full_bead_forces(RRt, RRc)
+        ff.close()
+    check_against(out["0"], _oracle_forces(p, RRt, RRc, list(range(P))), p, f"ttm{model} two-sided")
+    assert rel_err(out["0"]["Upott"], out["2"]["Upott"]) < 1e-12
+    assert max_abs(out["0"]["dRRt"], out["2"]["dRRt"]) < 1e-9
+    assert rel_err(out["0"]["virialct"], out["2"]["virialct"]) < 1e-11
+
+
 @pytest.fixture(scope="module")
 def big_system():
     """512 H2O, P = 32, TTM3-F, box 24.86 A, Rc = L/2 (the benchmark system)."""
