@@ -531,6 +531,7 @@ void plan_system(pimd_ctx *x, HostTables &t) {
   c.qq_half = ((c.model == PIMD_TTM21F || c.model == PIMD_TTM3F) && !c.cells)
                   ? env_int("PIMD_QQ_HALF", 1, 0, 2)
                   : 0;
+  c.fin_half = env_int("PIMD_FIN_HALF", 1, 0, 1);
   if (c.fd_cg != 0 && c.fd_cg != 4 && c.fd_cg != 8 && c.fd_cg != 16) c.fd_cg = 0;
 }
 

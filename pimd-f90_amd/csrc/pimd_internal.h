@@ -73,6 +73,7 @@ struct SysConst {
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   int dd_cache;            // 1: the SOR dipole pass keeps each pair's screening factors
   int qq_half;             // once-per-pair charge pass: 0 off, 1 where it fills the device, 2 always
+  int fin_half;            // the same for the final dipole pass, FP64 form (with qq_half): 0 off, 1 on
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;
@@ -348,7 +349,9 @@ inline size_t dd_cache_elems(const SysConst &c) {
 // four dipole-carrying sites, the oxygen's vdW force)
 constexpr int QQ_HALF_BLOCK = 512;
 constexpr int QQ_NV = 18;
-__host__ __device__ inline int qq_half_lanes(const SysConst &c) { return c.g_qq <= 4 ? 4 : c.g_qq <= 8 ? 8 : 16; }
+// lanes per molecule of the once-per-pair passes for a lane setting g
+__host__ __device__ inline int half_lanes(int g) { return g <= 4 ? 4 : g <= 8 ? 8 : 16; }
+__host__ __device__ inline int qq_half_lanes(const SysConst &c) { return half_lanes(c.g_qq); }
 // Molecules one block touches with G lanes per molecule: its owners
 // [i0, i0 + QQ_HALF_BLOCK / G) and their partners, which lie at most N/2
 // after their owner; rows are counted from i0 modulo N.

@@ -946,7 +946,7 @@ __global__ __launch_bounds__(256) void k_pair_qq_partner(const Rep *__restrict__
   const int k = threadIdx.x % 32;
   const bool valid = m < (long)c.P * N;  // alike over the 32 lanes of a molecule
   const int b = valid ? (int)(m / N) : 0, a = valid ? (int)(m % N) : 0;
-  const int G = qq_half_lanes(c), own = QQ_HALF_BLOCK / G;
+  const int G = half_lanes(c.g_qq), own = QQ_HALF_BLOCK / G;
   const int nblk = qq_half_blocks(N, G), W = qq_half_rows(N, G);
   double vc = 0.0;
   if (valid && k < NVAL) {
@@ -2248,6 +2248,209 @@ __global__ __launch_bounds__(256) void k_pair_final(const Rep *__restrict__ reps
 
 
 
+// k_pair_final with each molecule pair evaluated once, by the molecule that
+// owns it (as k_pair_qq_half): the partner's site potentials (its dipoles'
+// field terms with r reversed) and forces (the pair force reversed) go into
+// the block's two-word fixed-point LDS rows, added to the partner by
+// k_pair_final_partner.  The virial terms are even under the exchange of the
+// two molecules (site-centroid difference and force both reverse), so the
+// owner adds them whichever molecule is the lower one.  FP64 form only: in
+// FP32 the two-sided pass is faster (512 H2O x 32, TTM3-F: 0.104 against
+// 0.121 ms; the partner sums' LDS atomics outweigh the halved pair work).
+constexpr int FIN_NV = 16;  // partner values: potentials of sites 0-3, then their forces
+template <int MODEL, int G, typename R>
+__global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__restrict__ reps,
+                                                                   BlockMap bm) {
+  constexpr int ND = MODEL == 2 ? 3 : 1;
+  constexpr int T0 = MODEL == 2 ? 0 : 3;
+  constexpr int SB0 = MODEL == 3 ? 1 : 0;
+  constexpr int OWN = QQ_HALF_BLOCK / G;
+  // [W][4] sites, [W][ND][3] dipoles, [W][FIN_NV][2] partner sums
+  extern __shared__ Site lds_site[];
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const Site *__restrict__ site = Rp.b.site;
+  const double *__restrict__ Rcs = Rp.b.Rcs, *__restrict__ dip = Rp.b.dip;
+  const int *__restrict__ nbr = Rp.b.nbr, *__restrict__ nbrc = Rp.b.nbrc,
+            *__restrict__ nbrn = Rp.b.nbrn, *__restrict__ nbrh = Rp.b.nbrh;
+  double *phi = Rp.b.phi, *dR = Rp.b.dR, *tally = Rp.b.tally;
+  const int N = c.N, b = bk.y, i0 = bk.x * OWN, W = qq_half_rows(N, G);
+  double *lds_dip = reinterpret_cast<double *>(lds_site + 4 * W);
+  unsigned long long *jacc = reinterpret_cast<unsigned long long *>(lds_dip + (size_t)W * ND * 3);
+  for (int e = threadIdx.x; e < 4 * W; e += blockDim.x) {
+    int m = i0 + e / 4;
+    m -= m >= N ? N : 0;
+    lds_site[e] = site[((size_t)b * N + m) * 4 + e % 4];
+  }
+  for (int e = threadIdx.x; e < W * ND * 3; e += blockDim.x) {
+    int m = i0 + e / (ND * 3);
+    m -= m >= N ? N : 0;
+    lds_dip[e] = dip[((size_t)b * N + m) * ND * 3 + e % (ND * 3)];
+  }
+  for (int e = threadIdx.x; e < W * FIN_NV * 2; e += blockDim.x) jacc[e] = 0ull;
+  __syncthreads();
+  const int a0 = i0 + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  const bool valid = a0 < N;
+  const int a = valid ? a0 : i0;
+  const long grp = (long)b * N + a;
+  auto row = [&](int j) { return j - i0 + (j < i0 ? N : 0); };
+  Site A[4];
+  double mua[4][3];
+  for (int t = 0; t < 4; ++t) {
+    A[t] = lds_site[(a - i0) * 4 + t];
+    for (int k = 0; k < 3; ++k)
+      mua[t][k] = is_dip<MODEL>(t) ? lds_dip[((a - i0) * ND + (t - T0)) * 3 + k] : 0.0;
+  }
+  double ph[4] = {0, 0, 0, 0}, fr[4][3] = {{0}}, vc = 0.0, vt[9] = {0};
+  const int *list = nbr + (size_t)grp * c.nbr_cap;
+  auto jslot = [&](int rj, int k) { return jacc + ((size_t)rj * FIN_NV + k) * 2; };
+  auto visit = [&](const int j, auto thole_tag) {
+    constexpr bool THOLE = decltype(thole_tag)::value;
+    const bool lo = a < j;
+    const int rj = row(j);
+#pragma unroll 1
+    for (int sb = SB0; sb < 4; ++sb) {
+      const bool db = is_dip<MODEL>(sb);
+      const Site Bs = lds_site[rj * 4 + sb];
+      double mubs[3];
+      for (int k = 0; k < 3; ++k)
+        mubs[k] = db ? lds_dip[(rj * ND + max(sb - T0, 0)) * 3 + k] : 0.0;
+      double jp = 0.0, jf[3] = {0.0, 0.0, 0.0};  // the partner's site sb
+#pragma unroll
+      for (int sa = 0; sa < 4; ++sa) {
+        const bool da = is_dip<MODEL>(sa);
+        if (!da && !db) continue;
+        if (MODEL == 3 && sa == 0) continue;
+        double rd[3] = {A[sa].x - Bs.x, A[sa].y - Bs.y, A[sa].z - Bs.z};
+        minimg(c, rd[0], rd[1], rd[2]);
+        const double d2 = norm2_exact(rd[0], rd[1], rd[2]);
+        const R r[3] = {(R)rd[0], (R)rd[1], (R)rd[2]};
+        const R qa = (R)A[sa].q, qb = (R)Bs.q;
+        R di[3], dj[3];
+        for (int k = 0; k < 3; ++k) {
+          di[k] = (R)mua[sa][k];
+          dj[k] = (R)mubs[k];
+        }
+        R qd[3];
+        for (int k = 0; k < 3; ++k) qd[k] = qb * di[k] - qa * dj[k];
+        const R qsq = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
+        R dijr[3] = {0, 0, 0}, ts1 = 0, ts2 = 0;
+        bool have_ts1 = false;
+        if (da && db) {
+          R t1d, t2d, t3d;
+          smear3<THOLE>(c, (R)d2, (R)(lo ? c.apol3DD[sa][sb] : c.apol3DD[sb][sa]), t1d, t2d, t3d);
+          const R rdi = r[0] * di[0] + r[1] * di[1] + r[2] * di[2];
+          const R rdj = r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2];
+          const R didj = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+          for (int k = 0; k < 3; ++k)
+            dijr[k] = (R)15.0 * t3d * rdi * rdj * r[k] -
+                      (R)3.0 * t2d * (rdi * dj[k] + rdj * di[k] + didj * r[k]);
+          if (qsq > (R)1e-9) {
+            if (c.same_a) {
+              ts1 = t1d;
+              ts2 = t2d;
+            } else {
+              smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]), ts1, ts2);
+            }
+            have_ts1 = true;
+            R o[3];
+            dd3_apply(r, ts1, ts2, qd, o);
+            for (int k = 0; k < 3; ++k) dijr[k] += o[k];
+          }
+        } else if (qsq > (R)1e-9) {
+          smear2<THOLE>(c, (R)d2, (R)(lo ? c.apolCC[sa][sb] : c.apolCC[sb][sa]), ts1, ts2);
+          have_ts1 = true;
+          R o[3];
+          dd3_apply(r, ts1, ts2, qd, o);
+          for (int k = 0; k < 3; ++k) dijr[k] += o[k];
+        }
+        // this site's potential from the partner's dipole, and the partner
+        // site's from this one's (r reversed)
+        if (have_ts1 && db) ph[sa] += (double)(ts1 * (r[0] * dj[0] + r[1] * dj[1] + r[2] * dj[2]));
+        if (have_ts1 && da) jp -= (double)(ts1 * (r[0] * di[0] + r[1] * di[1] + r[2] * di[2]));
+        const double dij[3] = {(double)dijr[0], (double)dijr[1], (double)dijr[2]};
+        for (int k = 0; k < 3; ++k) {
+          fr[sa][k] += dij[k];
+          jf[k] -= dij[k];
+        }
+        // Rij0 of the dipole loop is wrapped to ~0 (pot_ttm.f90:406-407,468)
+        double rcv[3];
+        for (int k = 0; k < 3; ++k) rcv[k] = Rcs[(a * 4 + sa) * 3 + k] - Rcs[(j * 4 + sb) * 3 + k];
+        vc += rcv[0] * dij[0] + rcv[1] * dij[1] + rcv[2] * dij[2];
+        for (int col = 0; col < 3; ++col)
+          for (int rw = 0; rw < 3; ++rw) vt[col * 3 + rw] += dij[rw] * rd[col];
+      }
+      qq_fix_add(jslot(rj, sb), jp);
+      for (int k = 0; k < 3; ++k) qq_fix_add(jslot(rj, 4 + 3 * sb + k), jf[k]);
+    }
+  };
+  if (valid) {
+    const int4 h = make_int4(nbrh[grp * 4], nbrh[grp * 4 + 1], nbrh[grp * 4 + 2], nbrh[grp * 4 + 3]);
+    const int nn = nbrn[grp], nf = nbrc[grp], cap = c.nbr_cap;
+    const bool wrap = pair_half_window(a, N) >= N;
+    // the owned entries, as in k_pair_qq_half
+    walk_ranges<G>(list, lane, h.x, wrap ? nn : h.y, 0, wrap ? h.y : 0,
+                   [&](int j) { visit(j, std::true_type{}); });
+    walk_ranges<G>(list, lane, wrap ? cap - nf : cap - h.w, cap - h.z, cap - h.w, wrap ? cap : cap - h.w,
+                   [&](int j) { visit(j, std::false_type{}); });
+  }
+  for (int t = 0; t < 4; ++t) {
+    ph[t] = group_sum<G>(ph[t]);
+    for (int k = 0; k < 3; ++k) fr[t][k] = group_sum<G>(fr[t][k]);
+  }
+  vc = group_sum<G>(vc);
+  for (int i = 0; i < 9; ++i) vt[i] = group_sum<G>(vt[i]);
+  if (valid && lane == 0) {
+    const size_t base = ((size_t)b * N + a) * 4;
+    for (int t = 0; t < 4; ++t) {
+      phi[base + t] += ph[t];
+      for (int k = 0; k < 3; ++k) dR[(base + t) * 3 + k] += fr[t][k];
+    }
+    double *tl = tally + ((size_t)b * N + a) * T_COUNT;
+    tl[T_VIRC] += vc;
+    vadd9(tl + T_VIRT, vt);
+  }
+  __syncthreads();
+  // the block's exact row sums, rounded once (rows of QQ_NV, as the charge pass)
+  double *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV;
+  for (int e = threadIdx.x; e < W * FIN_NV; e += blockDim.x) {
+    const int rw = e / FIN_NV, k = e % FIN_NV;
+    slab[(size_t)rw * QQ_NV + k] = (double)(long long)jacc[2 * e] * (1.0 / QQ_HI) +
+                                   (double)(long long)jacc[2 * e + 1] * (1.0 / QQ_LO);
+  }
+}
+
+// The partner-side sums of k_pair_final_half (as k_pair_qq_partner): lane k
+// of a molecule's 32 adds value k over the blocks, in block order.
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_pair_final_partner(const Rep *__restrict__ reps,
+                                                            BlockMap bm) {
+  const Blk bk = block_of(bm);
+  const Rep &Rp = reps[bk.r];
+  const SysConst &c = Rp.c;
+  const int N = c.N;
+  const long m = ((long)bk.x * blockDim.x + threadIdx.x) / 32;
+  const int k = threadIdx.x % 32;
+  // TTM3-F: no values for the oxygen (slots 0 and 4-6)
+  const bool used = k < FIN_NV && !(MODEL == 3 && (k == 0 || (k >= 4 && k < 7)));
+  if (m >= (long)c.P * N || !used) return;
+  const int b = (int)(m / N), a = (int)(m % N);
+  const int G = half_lanes(c.g_fin), own = QQ_HALF_BLOCK / G;
+  const int nblk = qq_half_blocks(N, G), W = qq_half_rows(N, G);
+  const double *slab = Rp.b.qqslab + (size_t)b * nblk * W * QQ_NV + k;
+  double v = 0.0;
+  for (int q = 0; q < nblk; ++q) {
+    int r = a - q * own;
+    r += r < 0 ? N : 0;
+    if (r < W) v += slab[((size_t)q * W + r) * QQ_NV];
+  }
+  const size_t base = ((size_t)b * N + a) * 4;
+  if (k < 4) Rp.b.phi[base + k] += v;
+  else Rp.b.dR[(base + (k - 4) / 3) * 3 + (k - 4) % 3] += v;
+}
+
 // Reciprocal-space virial (ewald.f90:194-224), partial sums per block of k:
 // virk[b][blk][0..8] = sum Ureck (delta - coeff2 k k), [9..17] = dipole part.
 __global__ __launch_bounds__(256) void k_virk(const Rep *__restrict__ reps, BlockMap bm) {
@@ -2959,13 +3162,47 @@ static void launch_fin_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const
   }
 }
 
+// LDS of k_pair_final_half: sites, dipoles and partner sums of its rows
+static size_t fin_half_lds(const SysConst &c) {
+  const size_t W = qq_half_rows(c.N, half_lanes(c.g_fin));
+  return W * (sizeof(Site) * 4 + sizeof(double) * c.nd * 3 + sizeof(long long) * FIN_NV * 2);
+}
+static bool fin_half_on(const SysConst &c) {
+  return c.qq_half && c.fin_half && !c.mixed && is_ttm(c) && !c.cells && c.g_fin > 0 &&
+         fin_half_lds(c) <= PAIR_LDS_MAX &&
+         (c.qq_half == 2 ||
+          (long)c.P * qq_half_blocks(c.N, half_lanes(c.g_fin)) >= QQ_HALF_MIN_BLOCKS);
+}
+
+template <int ML>
+static void launch_fin_half_v(int G, dim3 grid, const BlockMap &m, size_t lds, const Group &g,
+                              hipStream_t s) {
+  const dim3 B(QQ_HALF_BLOCK);
+  if (G == 4) hipLaunchKernelGGL((k_pair_final_half<ML, 4, double>), grid, B, lds, s, g.d, m);
+  else if (G == 8) hipLaunchKernelGGL((k_pair_final_half<ML, 8, double>), grid, B, lds, s, g.d, m);
+  else hipLaunchKernelGGL((k_pair_final_half<ML, 16, double>), grid, B, lds, s, g.d, m);
+}
+
 void launch_pair_final(const Group &g, hipStream_t s) {
+  // variant bit 6: the pass that evaluates each pair once (bits 3-5 its lanes)
   launch_over(
       g,
       [](const SysConst &c, const MDConst &) {
+        if (fin_half_on(c)) {
+          const int G = half_lanes(c.g_fin);
+          const int gi = G == 4 ? 1 : G == 8 ? 2 : 3;
+          return Plan{(c.model == 3 ? 1 : 0) | 64 | (gi << 3), idiv(c.N, QQ_HALF_BLOCK / G), c.P, 1,
+                      fin_half_lds(c)};
+        }
         return pair_plan(c, c.g_fin, bead_sites_lds(c) + sizeof(double) * c.N * c.nd * 3);
       },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
+        if (v & 64) {
+          const int gi = (v >> 3) & 7, G = gi == 1 ? 4 : gi == 2 ? 8 : 16;
+          if (v & 1) launch_fin_half_v<3>(G, grid, m, lds, g, s);
+          else launch_fin_half_v<2>(G, grid, m, lds, g, s);
+          return;
+        }
         const int gi = v >> 3;
         switch (v & 7) {
           case 0: launch_fin_v<2, double, false>(gi, grid, m, lds, g, s); break;
@@ -2977,6 +3214,16 @@ void launch_pair_final(const Group &g, hipStream_t s) {
           case 6: launch_fin_v<2, float, true>(gi, grid, m, lds, g, s); break;
           default: launch_fin_v<3, float, true>(gi, grid, m, lds, g, s); break;
         }
+      });
+  launch_over(
+      g,
+      [](const SysConst &c, const MDConst &) {
+        if (!fin_half_on(c)) return Plan{-1, 0, 0, 0, 0};
+        return Plan{c.model == 3 ? 1 : 0, idiv((long)c.P * c.N * 32, 256), 1, 1, 0};
+      },
+      [&](int v, dim3 grid, const BlockMap &m, size_t) {
+        if (v == 1) hipLaunchKernelGGL(k_pair_final_partner<3>, grid, dim3(256), 0, s, g.d, m);
+        else hipLaunchKernelGGL(k_pair_final_partner<2>, grid, dim3(256), 0, s, g.d, m);
       });
 }
 
