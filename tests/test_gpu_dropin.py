@@ -11,7 +11,9 @@ committed bead image for 10 NVE steps, as full PIMD (PIMD_VV) and as
 contracted PIMD (contracted_MD, 3 inner steps, whose centroid call needs the
 CONTRACTION branch of pot_ttm.f90:92-132).  Its Temp/Press lines and final
 bead image must equal those the unmodified reference program printed
-(tests/golden/dropin_ttm3f_n32_p4.npz) to the last printed digit.
+(tests/golden/dropin_ttm3f_n32_p4.npz) to the last printed digit.  The
+same input and image also go through the Python host side (runinput into
+the device integrator), checked against the same outputs.
 """
 import os
 import shutil
@@ -72,6 +74,48 @@ def test_dropin_program_matches_reference_run(exe, ptype, tmp_path):
     assert mine.shape == theirs.shape
     d = float(np.max(np.abs(mine - theirs)))
     print(exe, ptype, "final image: largest difference", d, flush=True)
+    assert d <= 1e-6 * (1 + 1e-6)      # one unit of f12.6
+
+
+@pytest.mark.parametrize("ptype", ["full", "contracted"])
+def test_python_run_from_reference_input(ptype, tmp_path):
+    """The same committed input and bead image through the Python host side:
+    runinput.read_input, read_image (the .img restart path,
+    InputOutput.f90:1030-1074) and run_setup into TTMForceField and
+    PIMDIntegrator, 10 NVE steps on the device.  The temperatures and the
+    final bead image written by runinput.write_image must equal the
+    unmodified reference program's to the last printed digit."""
+    from pimd_amd import runinput
+    from pimd_amd.ttm import PIMDIntegrator, TTMForceField
+    g = load_golden("dropin_ttm3f_n32_p4")
+    inp = runinput.read_input(os.path.join(INPUTS, f"dropin_{ptype}.inp"), warn=lambda m: None)
+    R, Pm, Rc, box, t_init, _ = runinput.read_image(os.path.join(INPUTS, "dropin_n32_p4.img"),
+                                                   inp["massO"], inp["massH"], inp["Nbeads"])
+    p, md = runinput.run_setup(inp, box, R.shape[1])
+    n = inp["eq_timesteps"] + inp["run_timesteps"]
+    print("case python", ptype, "steps", n, flush=True)
+    ff = TTMForceField(p)
+    integ = PIMDIntegrator(ff, **md)
+    integ.set_state(R, Pm, None)
+    integ.run(n, t0=1)
+    obs = integ.observables(n)
+    st = integ.get_state()
+    ff.close()
+    temp = np.round(obs[:, 0], 2)
+    ref = g[f"tp_{ptype}"]
+    dt = float(np.max(np.abs(temp - ref[:, 1])))
+    print("python", ptype, "temperature: largest difference", dt, flush=True)
+    assert dt <= TP_ULP[1] * (1 + 1e-6)
+    # the final checkpoint is written after the step loop (PIMD.f90:74), whose
+    # index then stands one past its bound (InputOutput.f90:870-872)
+    out = tmp_path / "final.img"
+    runinput.write_image(str(out), st["RRt"], st["PPt"], box, n + 1, inp["delt"] * 1e-3,
+                         p.mass_o, p.mass_h, init_time=t_init)
+    mine = image_numbers(out.read_text())
+    theirs = image_numbers(str(g[f"final_{ptype}"]))
+    assert mine.shape == theirs.shape
+    d = float(np.max(np.abs(mine - theirs)))
+    print("python", ptype, "final image: largest difference", d, flush=True)
     assert d <= 1e-6 * (1 + 1e-6)      # one unit of f12.6
 
 
