@@ -571,10 +571,14 @@ def run_fit(args):
         bx.append(r)
     bx = np.array(bx)
     common = dict(pot_model=3, eps_ewald=1e-8, polar_eps=1e-12, polar_maxiter=100)
-    # contexts hold up to 64 configurations (beads) each
+    # contexts hold up to 64 configurations (beads) each.  The box groups'
+    # adjoint sweeps (16 ms each) dominate; their contexts are made first so
+    # that their streams land on different hardware queues (the runtime
+    # spreads streams over its 4 queues in creation order; the last two
+    # made share one) and their sweeps overlap in loss_and_grad
     pcl = TTMParams(nwaters=5, nbeads=64, box=(40.0, 40.0, 40.0), **common)
     pbx = TTMParams(nwaters=32, nbeads=64, box=(L, L, L), rc=4.9, **common)
-    sets = [(pcl, cl[:64]), (pcl, cl[64:]), (pbx, bx[:64]), (pbx, bx[64:])]
+    sets = [(pbx, bx[:64]), (pbx, bx[64:]), (pcl, cl[:64]), (pcl, cl[64:])]
     ffs = [TTMForceField(q) for q, _ in sets]
     theta0 = np.concatenate([ffs[0].pes_params(), ffs[0].ttm_params()])
     rng = np.random.default_rng(6000)

@@ -324,6 +324,16 @@ int pimd_gamma_m_grad(pimd_ctx *ctx, const double *RRt, const double *ew, const 
 #define PIMD_TTM_ADJ_NPARAM (PIMD_TTM_NPARAM + PIMD_DMS_NCOEF + 1)
 int pimd_ttm_adjoint(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
                      int nparam, double *grad, double *E, double *dEdtheta);
+/* pimd_ttm_adjoint in two halves, for nparam 16 or 100: _start copies the
+ * inputs (the arrays are free once it returns) and queues the sweep on the
+ * context's stream; _finish waits for it and returns what pimd_ttm_adjoint
+ * returns, bit for bit.  Sweeps started on several contexts run side by side
+ * (a fit's configuration groups).  No counterpart in the reference, which has
+ * no parameter derivatives.  PIMD_EINVAL: _start while a sweep of the context
+ * is unfinished, _finish with none started. */
+int pimd_ttm_adjoint_start(pimd_ctx *ctx, const double *RRt, const double *ew, const double *gw,
+                           int nparam);
+int pimd_ttm_adjoint_finish(pimd_ctx *ctx, double *grad, double *E, double *dEdtheta);
 
 /* ---- instrumentation --------------------------------------------------- */
 

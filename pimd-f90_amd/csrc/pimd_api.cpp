@@ -89,6 +89,18 @@ struct pimd_ctx {
   // pinned staging of the host-array force call: coordinates in, all outputs
   // out (made on the first call)
   double *h_stage = nullptr;
+  // a parameter-derivative sweep started and not yet finished: its pinned
+  // inputs and outputs, and what finishing it needs
+  double *h_tan = nullptr;
+  size_t h_tan_bytes = 0;
+  int tan_nparam = 0;  // parameters pimd_ttm_adjoint_finish returns
+  struct TanPending {
+    bool on = false;
+    int mode = 0, ndir = 0;
+    size_t nout = 0, nadj = 0;
+    std::vector<double> ew;
+    double *out = nullptr;  // in h_tan
+  } tanp;
   // profiling
   bool prof = false;
   std::vector<std::string> names;
@@ -186,6 +198,9 @@ void free_all(pimd_ctx *x) {
   d = DevBufs{};
   if (x->h_nactive) (void)hipHostFree(x->h_nactive);
   if (x->h_stage) (void)hipHostFree(x->h_stage);
+  if (x->h_tan) (void)hipHostFree(x->h_tan);
+  x->h_tan = nullptr;
+  x->h_tan_bytes = 0;
   for (auto &q : x->pending) {
     (void)hipEventDestroy(q.a);
     (void)hipEventDestroy(q.b);
@@ -2153,9 +2168,14 @@ extern "C" int pimd_ttm_params_set(pimd_ctx *x, const double *theta) {
 // coefficients, 2 gammaM), or with mode 3 the adjoint of all TTM_ADJ_NPARAM
 // parameters in one sweep (ndir = TTM_ADJ_NPARAM outputs): the loss
 // gradient, and optionally each configuration's energy and its derivatives.
-static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
-                        int mode, int ndir, double *grad, double *E, double *dEdtheta) {
-  if (!x || !RRt || !ew || !gw || !grad) return fail(PIMD_EINVAL, "null argument");
+// A parameter-derivative sweep in two halves: tangent_start queues the
+// inputs (through pinned host memory), the sweep and the copy of its outputs
+// on the context's stream and returns; tangent_finish waits for them and
+// forms the gradient.  Sweeps of different contexts run side by side.
+static int tangent_start(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
+                         int mode, int ndir) {
+  if (!x || !RRt || !ew || !gw) return fail(PIMD_EINVAL, "null argument");
+  if (x->tanp.on) return fail(PIMD_EINVAL, "a parameter-derivative sweep is still to be finished");
   if (x->p.contraction)
     return fail(PIMD_EINVAL, "parameter derivatives need a context with contraction = 0");
   if (x->p.pot_model >= PIMD_QSPCFW)
@@ -2205,19 +2225,53 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
   L.lrc_coef[0] = pre / (9.0 * std::pow(Rc, 9));
   L.lrc_coef[1] = pre / (7.0 * std::pow(Rc, 7));
   L.lrc_coef[2] = pre / (3.0 * (Rc * Rc * Rc));
-  HIPCHK(hipMemcpyAsync(dRR, RRt, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dgw, gw, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dew, ew, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  // pinned: the inputs (the caller's arrays are free once this returns), then
+  // the outputs
+  const size_t hbytes = sizeof(double) * (2 * n + P + nout + nadj);
+  if (hbytes > x->h_tan_bytes) {
+    if (x->h_tan) (void)hipHostFree(x->h_tan);
+    x->h_tan = nullptr;
+    x->h_tan_bytes = 0;
+    HIPCHK(hipHostMalloc((void **)&x->h_tan, hbytes, hipHostMallocDefault));
+    x->h_tan_bytes = hbytes;
+  }
+  double *hin = x->h_tan;
+  std::copy(RRt, RRt + n, hin);
+  std::copy(gw, gw + n, hin + n);
+  std::copy(ew, ew + P, hin + 2 * n);
+  HIPCHK(hipMemcpyAsync(dRR, hin, sizeof(double) * (2 * n + P), hipMemcpyHostToDevice, s));
   static const char *const names[4] = {"ttm_tangent", "dms_tangent", "gamma_m_tangent",
                                        "ttm_adjoint"};
   x->timed(names[mode], [&] { launch_ttm_tangent(c, L, s); });
   HIPCHK(hipGetLastError());
-  std::vector<double> out(nout + nadj);
-  HIPCHK(hipMemcpyAsync(out.data(), L.out, sizeof(double) * (nout + nadj), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  double *hout = hin + 2 * n + P;
+  HIPCHK(hipMemcpyAsync(hout, L.out, sizeof(double) * (nout + nadj), hipMemcpyDeviceToHost, s));
+  auto &tp = x->tanp;
+  tp.on = true;
+  tp.mode = mode;
+  tp.ndir = ndir;
+  tp.nout = nout;
+  tp.nadj = nadj;
+  tp.ew.assign(ew, ew + P);
+  tp.out = hout;
+  return PIMD_OK;
+}
+
+static int tangent_finish(pimd_ctx *x, double *grad, double *E, double *dEdtheta) {
+  if (!x || !grad) return fail(PIMD_EINVAL, "null argument");
+  auto &tp = x->tanp;
+  if (!tp.on) return fail(PIMD_EINVAL, "no parameter-derivative sweep was started");
+  HIPCHK(hipSetDevice(x->p.device));
+  tp.on = false;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  const size_t P = (size_t)x->c.P, nout = tp.nout;
+  const int ndir = tp.ndir;
+  const bool adj = tp.mode == 3;
+  const double *ew = tp.ew.data();
+  const double *out = tp.out;
   if (adj) {
     // grad_j = sum_b ew_b dE_b/dtheta_j + gw . d(dE_b/dr)/dtheta_j
-    const double *A = out.data() + nout;
+    const double *A = out + nout;
     for (int j = 0; j < ndir; ++j) {
       double g = 0.0;
       for (size_t b = 0; b < P; ++b)
@@ -2242,6 +2296,13 @@ static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const 
       for (int j = 0; j < ndir; ++j) dEdtheta[b * ndir + j] = out[(b * ndir + j) * 4 + 1];
   }
   return PIMD_OK;
+}
+
+static int tangent_grad(pimd_ctx *x, const double *RRt, const double *ew, const double *gw,
+                        int mode, int ndir, double *grad, double *E, double *dEdtheta) {
+  if (!grad) return fail(PIMD_EINVAL, "null argument");
+  const int rc = tangent_start(x, RRt, ew, gw, mode, ndir);
+  return rc != PIMD_OK ? rc : tangent_finish(x, grad, E, dEdtheta);
 }
 
 extern "C" int pimd_ttm_param_grad(pimd_ctx *x, const double *RRt, const double *ew,
@@ -2285,6 +2346,34 @@ extern "C" int pimd_ttm_adjoint(pimd_ctx *x, const double *RRt, const double *ew
   grad[TTM_ADJ_NPARAM - 1] = gg;
   if (dEdtheta)
     for (size_t b = 0; b < P; ++b) dEdtheta[b * TTM_ADJ_NPARAM + TTM_ADJ_NPARAM - 1] = dEg[b];
+  return PIMD_OK;
+}
+
+extern "C" int pimd_ttm_adjoint_start(pimd_ctx *x, const double *RRt, const double *ew,
+                                      const double *gw, int nparam) {
+  if (nparam != TTM_NPARAM && nparam != TTM_NPARAM + DMS_NCOEF)
+    return fail(PIMD_EINVAL, "nparam must be %d or %d (gammaM: pimd_ttm_adjoint)", TTM_NPARAM,
+                TTM_NPARAM + DMS_NCOEF);
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  const int rc = tangent_start(x, RRt, ew, gw, 3, TTM_ADJ_NPARAM);
+  if (rc == PIMD_OK) x->tan_nparam = nparam;
+  return rc;
+}
+
+extern "C" int pimd_ttm_adjoint_finish(pimd_ctx *x, double *grad, double *E, double *dEdtheta) {
+  if (!x || !grad) return fail(PIMD_EINVAL, "null argument");
+  if (!x->tanp.on || x->tanp.mode != 3)
+    return fail(PIMD_EINVAL, "no adjoint sweep was started (pimd_ttm_adjoint_start)");
+  const size_t P = (size_t)x->c.P;
+  const int nparam = x->tan_nparam;
+  std::vector<double> g(TTM_ADJ_NPARAM), dE(dEdtheta ? P * TTM_ADJ_NPARAM : 0);
+  const int rc = tangent_finish(x, g.data(), E, dEdtheta ? dE.data() : nullptr);
+  if (rc != PIMD_OK) return rc;
+  std::copy(g.begin(), g.begin() + nparam, grad);
+  if (dEdtheta)
+    for (size_t b = 0; b < P; ++b)
+      std::copy(dE.begin() + b * TTM_ADJ_NPARAM, dE.begin() + b * TTM_ADJ_NPARAM + nparam,
+                dEdtheta + b * nparam);
   return PIMD_OK;
 }
 

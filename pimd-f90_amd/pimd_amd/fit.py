@@ -60,7 +60,30 @@ class FitGroup:
         (pimd_ttm_adjoint, exact up to the SOR convergence); "tangent": from
         their tangent sweeps (one per parameter, exact for the SOR iterations
         as executed)."""
+        return self.evaluate_finish(self.evaluate_start(theta, grad, method))
+
+    def evaluate_start(self, theta, grad: bool, method: str = "adjoint"):
+        """evaluate up to the adjoint sweep, which it leaves running on the
+        force field's stream (without gammaM); evaluate_finish completes it.
+        Starting every group before finishing any runs their sweeps side by
+        side."""
         pes, ttm, dms, gm = split_theta(theta)
+        if grad and method == "adjoint" and ttm is not None and gm is None:
+            loss, g, dE, dG = self._forward(pes, ttm, dms, gm, grad)
+            n = TTM_NPARAM + (DMS_NCOEF if dms is not None else 0)
+            self.ff.ttm_adjoint_start(self.RRt, 2.0 * dE, 2.0 * dG, n)
+            return ("pending", loss, g)
+        return ("done",) + self._evaluate(theta, grad, method)
+
+    def evaluate_finish(self, started):
+        if started[0] == "done":
+            return started[1], started[2]
+        _, loss, g = started
+        return loss, np.concatenate([g, self.ff.ttm_adjoint_finish()[0]])
+
+    def _forward(self, pes, ttm, dms, gm, grad):
+        """Sets the parameters, runs the forward pass: (loss, PES gradient or
+        None, dE, dG)."""
         self.ff.set_pes_params(pes)
         if ttm is not None:
             self.ff.set_ttm_params(ttm)
@@ -72,9 +95,13 @@ class FitGroup:
         dE = r["Upott"] - self.E_target
         dG = r["dRRt"] - self.G_target
         loss = float(np.sum(dE * dE) + np.sum(dG * dG))
-        g = None
+        g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG) if grad else None
+        return loss, g, dE, dG
+
+    def _evaluate(self, theta, grad: bool, method: str):
+        pes, ttm, dms, gm = split_theta(theta)
+        loss, g, dE, dG = self._forward(pes, ttm, dms, gm, grad)
         if grad:
-            g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG)
             if method == "adjoint" and ttm is not None:
                 n = TTM_NPARAM + (DMS_NCOEF if dms is not None else 0) + (gm is not None)
                 ga = self.ff.ttm_adjoint(self.RRt, 2.0 * dE, 2.0 * dG, n)[0]
@@ -92,10 +119,13 @@ class FitGroup:
 
 def loss_and_grad(groups, theta, grad: bool = True, method: str = "adjoint"):
     """Loss over every group and, with grad, its gradient dL/dtheta (method:
-    see FitGroup.evaluate)."""
+    see FitGroup.evaluate).  Every group's adjoint sweep is started before
+    any is finished, so the sweeps of the groups (and the forward passes of
+    the later groups) run side by side."""
     total, g = 0.0, np.zeros(np.asarray(theta).size) if grad else None
-    for grp in groups:
-        l, gg = grp.evaluate(theta, grad, method)
+    started = [grp.evaluate_start(theta, grad, method) for grp in groups]
+    for grp, st in zip(groups, started):
+        l, gg = grp.evaluate_finish(st)
         total += l
         if grad:
             g += gg

@@ -39,7 +39,7 @@ EXPORTS = (
     "pimd_ttm_params_set", "pimd_ttm_param_grad", "pimd_md_observables",
     "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles", "pimd_dms_coefs_get",
     "pimd_dms_coefs_set", "pimd_dms_coef_grad", "pimd_gamma_m_get", "pimd_gamma_m_set",
-    "pimd_gamma_m_grad", "pimd_ttm_adjoint",
+    "pimd_gamma_m_grad", "pimd_ttm_adjoint", "pimd_ttm_adjoint_start", "pimd_ttm_adjoint_finish",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -130,6 +130,8 @@ def load_library(path: str = LIB_PATH):
         "pimd_gamma_m_set": [vp, C.c_double],
         "pimd_gamma_m_grad": [vp, dp, dp, dp, dp, dp, dp],
         "pimd_ttm_adjoint": [vp, dp, dp, dp, C.c_int, dp, dp, dp],
+        "pimd_ttm_adjoint_start": [vp, dp, dp, dp, C.c_int],
+        "pimd_ttm_adjoint_finish": [vp, dp, dp, dp],
         "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
         "pimd_profile_copy": [vp, C.c_size_t, dp],
@@ -360,6 +362,23 @@ class TTMForceField:
         grad, E, dE = np.zeros(nparam), np.zeros(P), np.zeros((P, nparam))
         _check(load_library().pimd_ttm_adjoint(self._h, _ptr(r), _ptr(e), _ptr(g), int(nparam),
                                                _ptr(grad), _ptr(E), _ptr(dE)))
+        return grad, E, dE
+
+    def ttm_adjoint_start(self, RRt, ew, gw, nparam: int = TTM_NPARAM):
+        """ttm_adjoint in two halves (nparam 16 or 100): this queues the sweep
+        on the context's stream and returns; ttm_adjoint_finish waits for it
+        and returns (grad, E, dE/dtheta).  Sweeps of several force fields run
+        side by side."""
+        P, na = self.params.nbeads, self.params.natoms
+        r, e, g = _arr(RRt, (P, na, 3)), _arr(ew, (P,)), _arr(gw, (P, na, 3))
+        _check(load_library().pimd_ttm_adjoint_start(self._h, _ptr(r), _ptr(e), _ptr(g),
+                                                     int(nparam)))
+        self._adj_nparam = int(nparam)
+
+    def ttm_adjoint_finish(self):
+        P, n = self.params.nbeads, getattr(self, "_adj_nparam", TTM_NPARAM)
+        grad, E, dE = np.zeros(n), np.zeros(P), np.zeros((P, n))
+        _check(load_library().pimd_ttm_adjoint_finish(self._h, _ptr(grad), _ptr(E), _ptr(dE)))
         return grad, E, dE
 
     def potential(self, RR, RRc, t: int = 1) -> dict:

@@ -115,3 +115,46 @@ def test_adjoint_loss_gradient_vs_fd(adjset):
             continue
         assert err < FD_REL * abs(g[j]) + noise, j
 
+
+
+def test_adjoint_in_two_halves(adjset):
+    """pimd_ttm_adjoint_start / _finish on both groups' contexts, both
+    started before either is finished, give pimd_ttm_adjoint's results bit
+    for bit (16 and 100 parameters); the fit driver's overlapped sweeps equal
+    its one-group-at-a-time sweeps; a second start before the finish, a
+    blocking sweep in between, and a finish with nothing started are
+    refused."""
+    from pimd_amd.fit import loss_and_grad
+    from pimd_amd.ttm import PimdError
+    model, groups, theta0 = adjset
+    ws, ref = [], []
+    for grp in groups:
+        grp.ff.set_ttm_params(theta0)
+        ew, gw = _weights(grp)
+        ws.append((ew, gw))
+        ref.append([grp.ff.ttm_adjoint(grp.RRt, ew, gw, n) for n in (16, 100)])
+    for k, n in enumerate((16, 100)):
+        for grp, (ew, gw) in zip(groups, ws):
+            grp.ff.ttm_adjoint_start(grp.RRt, ew, gw, n)
+        for i, grp in enumerate(groups):
+            got = grp.ff.ttm_adjoint_finish()
+            for a, b in zip(got, ref[i][k]):
+                assert np.array_equal(a, b), (n, i)
+    ff = groups[0].ff
+    with pytest.raises(PimdError):
+        ff.ttm_adjoint_finish()
+    ff.ttm_adjoint_start(groups[0].RRt, *ws[0], 16)
+    with pytest.raises(PimdError):
+        ff.ttm_adjoint_start(groups[0].RRt, *ws[0], 16)
+    with pytest.raises(PimdError):
+        ff.ttm_adjoint(groups[0].RRt, *ws[0], 16)
+    assert np.array_equal(ff.ttm_adjoint_finish()[0], ref[0][0][0])
+    for full in (np.concatenate([ff.pes_params(), theta0]),
+                 np.concatenate([ff.pes_params(), theta0, ff.dms_coefs()])):
+        L1, g1 = loss_and_grad(groups, full)
+        L0, g0 = 0.0, 0.0
+        for grp in groups:
+            l, g = grp._evaluate(full, True, "adjoint")
+            L0 += l
+            g0 = g0 + g
+        assert L1 == L0 and np.array_equal(g1, g0)
