@@ -501,7 +501,11 @@ void plan_system(pimd_ctx *x, HostTables &t) {
     while (g < hi && g < v) g <<= 1;
     return g;
   };
-  const int dflt_qq = pow2_clamp(131072 / std::max(mb, 1L), 8, 32);
+  // One bead (the centroid call of contracted MD): 0, the charge and final
+  // pair passes from global memory with lanes by neighbour count (64 at
+  // 512 H2O, Rc = L/2), twice the workgroups of the LDS form's 32 lanes
+  // (contracted step 1.05 -> 1.00 ms mixed, 1.12 -> 1.04 ms FP64).
+  const int dflt_qq = c.P == 1 ? 0 : pow2_clamp(131072 / std::max(mb, 1L), 8, 32);
   // The dipole pairs' screening factors stay fixed through the SOR solve:
   // kept from the first iteration up to PIMD_DD_CACHE_MB (default 4096) per
   // context, recomputed every iteration beyond (512 H2O x 32 beads, dipole
