@@ -752,24 +752,44 @@ __global__ __launch_bounds__(256) void k_pair_qq(const Rep *__restrict__ reps, B
 // feeding both sides (TTM2.1-F adds the pairs with the partner's oxygen,
 // which k_pair_qq takes from the partner's side).  Energies and virials per
 // pair are those k_pair_qq halves from its two sides, taken once.
-// A value v goes in as h = rint(v 2^32) (range +-2^31) and the remainder
-// v - h 2^-32, exact in FP64, as rint(. 2^64): each addition is exact to
-// 2^-65, finer than FP64 rounding of the sums, so the result matches an FP64
-// sum and stays smooth in the model parameters.  Integer addition wraps
-// modulo 2^64 and is associative: only totals must fit.
+// FP64 form: a value v goes in as h = rint(v 2^32) (range +-2^31) and the
+// remainder v - h 2^-32, exact in FP64, as rint(. 2^64): each addition is
+// exact to 2^-65, finer than FP64 rounding of the sums, so the result
+// matches an FP64 sum and stays smooth in the model parameters.  FP32 form:
+// one word, rint(v 2^40) (range +-2^23): its pair terms already round at
+// 1e-7 relative, far above that quantum, and one LDS atomic per value in
+// place of two.  Integer addition wraps modulo 2^64 and is associative: only
+// totals must fit.
 constexpr double QQ_HI = 4294967296.0;            // 2^32
 constexpr double QQ_LO = 18446744073709551616.0;  // 2^64
-// partner sites per step of the site loop: unrolled in the FP32 form, one at
-// a time in the FP64 form (as k_pair_qq)
+constexpr double QQ_ONE = 1099511627776.0;        // 2^40
+template <typename R>
+struct QqWords {
+  static constexpr int n = sizeof(R) == 8 ? 2 : 1;
+};
+// partner sites per step of the site loop: one at a time (unrolled, the
+// per-site partner sums spill)
 template <typename R>
 struct QqUnroll {
   static constexpr int v = 1;
 };
+template <int NW>
 __device__ __forceinline__ void qq_fix_add(unsigned long long *acc, double v) {
-  const double h = rint(fmin(fmax(v * QQ_HI, -9.2e18), 9.2e18));
-  const double rem = fmin(fmax(v - h * (1.0 / QQ_HI), -1.0 / QQ_HI), 1.0 / QQ_HI);
-  atomicAdd(acc, (unsigned long long)(long long)h);
-  atomicAdd(acc + 1, (unsigned long long)__double2ll_rn(rem * QQ_LO));
+  if constexpr (NW == 2) {
+    const double h = rint(fmin(fmax(v * QQ_HI, -9.2e18), 9.2e18));
+    const double rem = fmin(fmax(v - h * (1.0 / QQ_HI), -1.0 / QQ_HI), 1.0 / QQ_HI);
+    atomicAdd(acc, (unsigned long long)(long long)h);
+    atomicAdd(acc + 1, (unsigned long long)__double2ll_rn(rem * QQ_LO));
+  } else {
+    atomicAdd(acc, (unsigned long long)__double2ll_rn(fmin(fmax(v * QQ_ONE, -9.2e18), 9.2e18)));
+  }
+}
+template <int NW>
+__device__ __forceinline__ double qq_fix_value(const unsigned long long *acc) {
+  if constexpr (NW == 2)
+    return (double)(long long)acc[0] * (1.0 / QQ_HI) + (double)(long long)acc[1] * (1.0 / QQ_LO);
+  else
+    return (double)(long long)acc[0] * (1.0 / QQ_ONE);
 }
 
 template <int MODEL, int G, typename R>
@@ -777,7 +797,8 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
                                                                 BlockMap bm) {
   constexpr int SA0 = MODEL == 2 ? 0 : 1;  // first dipole-carrying (field-target) site type
   constexpr int OWN = QQ_HALF_BLOCK / G;  // owners per block
-  extern __shared__ Site lds_site[];  // [W][4] sites, then [W][QQ_NV][2] partner sums
+  constexpr int NW = QqWords<R>::n;       // words per partner value
+  extern __shared__ Site lds_site[];  // [W][4] sites, then [W][QQ_NV][NW] partner sums
   const Blk bk = block_of(bm);
   const Rep &Rp = reps[bk.r];
   const SysConst &c = Rp.c;
@@ -793,7 +814,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
     m -= m >= N ? N : 0;
     lds_site[e] = site[((size_t)b * N + m) * 4 + e % 4];
   }
-  for (int e = threadIdx.x; e < W * QQ_NV * 2; e += blockDim.x) jacc[e] = 0ull;
+  for (int e = threadIdx.x; e < W * QQ_NV * NW; e += blockDim.x) jacc[e] = 0ull;
   __syncthreads();
   const int a0 = i0 + threadIdx.x / G;
   const int lane = threadIdx.x % G;
@@ -809,7 +830,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
   double uvdw = 0.0, vc = 0.0, vt[9] = {0};
   const int *list = nbr + (size_t)grp * c.nbr_cap;
   // partner-side sums of molecule j: ph[1..3], then ef of sites SA0..3, then dro
-  auto jslot = [&](int rj, int k) { return jacc + ((size_t)rj * QQ_NV + k) * 2; };
+  auto jslot = [&](int rj, int k) { return jacc + ((size_t)rj * QQ_NV + k) * NW; };
   auto visit = [&](const int j, auto thole_tag) {
     constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
@@ -830,7 +851,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
                          (double)(fsc * (R)r[2])};
     for (int k = 0; k < 3; ++k) {
       dro[k] += d[k];
-      qq_fix_add(jslot(rj, 3 + 3 * (4 - SA0) + k), -d[k]);
+      qq_fix_add<NW>(jslot(rj, 3 + 3 * (4 - SA0) + k), -d[k]);
     }
     uvdw += (double)uij;
     {
@@ -880,8 +901,8 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
             for (int row = 0; row < 3; ++row) vt[col * 3 + row] += dd[col] * v[row];
         }
       }
-      if (sb > 0) qq_fix_add(jslot(rj, sb - 1), jp);
-      for (int k = 0; k < 3; ++k) qq_fix_add(jslot(rj, 3 + 3 * (sb - SA0) + k), je[k]);
+      if (sb > 0) qq_fix_add<NW>(jslot(rj, sb - 1), jp);
+      for (int k = 0; k < 3; ++k) qq_fix_add<NW>(jslot(rj, 3 + 3 * (sb - SA0) + k), je[k]);
     }
   };
   if (valid) {
@@ -926,8 +947,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_qq_half(const Rep *__res
   // the block's exact row sums, rounded once
   double *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV;
   for (int e = threadIdx.x; e < W * QQ_NV; e += blockDim.x)
-    slab[e] = (double)(long long)jacc[2 * e] * (1.0 / QQ_HI) +
-              (double)(long long)jacc[2 * e + 1] * (1.0 / QQ_LO);
+    slab[e] = qq_fix_value<NW>(jacc + (size_t)e * NW);
 }
 
 // The partner-side sums of k_pair_qq_half: 32 lanes per molecule, lane k
@@ -2382,8 +2402,8 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
         for (int col = 0; col < 3; ++col)
           for (int rw = 0; rw < 3; ++rw) vt[col * 3 + rw] += dij[rw] * rd[col];
       }
-      qq_fix_add(jslot(rj, sb), jp);
-      for (int k = 0; k < 3; ++k) qq_fix_add(jslot(rj, 4 + 3 * sb + k), jf[k]);
+      qq_fix_add<2>(jslot(rj, sb), jp);
+      for (int k = 0; k < 3; ++k) qq_fix_add<2>(jslot(rj, 4 + 3 * sb + k), jf[k]);
     }
   };
   if (valid) {
@@ -2417,8 +2437,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
   double *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV;
   for (int e = threadIdx.x; e < W * FIN_NV; e += blockDim.x) {
     const int rw = e / FIN_NV, k = e % FIN_NV;
-    slab[(size_t)rw * QQ_NV + k] = (double)(long long)jacc[2 * e] * (1.0 / QQ_HI) +
-                                   (double)(long long)jacc[2 * e + 1] * (1.0 / QQ_LO);
+    slab[(size_t)rw * QQ_NV + k] = qq_fix_value<2>(jacc + 2 * (size_t)e);
   }
 }
 
@@ -2820,7 +2839,7 @@ static void launch_qq_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const 
 // LDS of k_pair_qq_half: the bead's sites and the partner-side sums
 static size_t qq_half_lds(const SysConst &c) {
   const size_t W = qq_half_rows(c.N, qq_half_lanes(c));
-  return W * (sizeof(Site) * 4 + sizeof(long long) * QQ_NV * 2);
+  return W * (sizeof(Site) * 4 + sizeof(long long) * QQ_NV * (c.mixed ? 1 : 2));
 }
 // The once-per-pair pass (qq_half 1) where its blocks fill at least half the
 // device: one bead of 512 H2O, the centroid call of contracted MD, gets 8-16
