@@ -99,6 +99,18 @@ int pimd_ctx_info(const pimd_ctx *ctx, double *out);
 int pimd_forces_all_beads(pimd_ctx *ctx, const double *RRt, const double *RRc, int t,
                           double *Upott, double *dRRt, double *virt, double *virialct,
                           double *dip_momIt, double *dip_momEt, double *chg, int *iters);
+/* pimd_forces_all_beads in two halves: _start checks and copies the inputs
+ * (the arrays are free once it returns) and queues the call and the copy of
+ * every output on the context's stream; _finish waits and fills the outputs
+ * given (any may be NULL), the same values as pimd_forces_all_beads.  Calls
+ * started on several contexts run side by side.  No counterpart in the
+ * reference (its ranks each call pot_ttm and wait).  PIMD_EINVAL: _start
+ * while a started call of the context is unfinished, _finish with none
+ * started. */
+int pimd_forces_all_beads_start(pimd_ctx *ctx, const double *RRt, const double *RRc, int t);
+int pimd_forces_all_beads_finish(pimd_ctx *ctx, double *Upott, double *dRRt, double *virt,
+                                 double *virialct, double *dip_momIt, double *dip_momEt,
+                                 double *chg, int *iters);
 
 /* Same computation on device-resident arrays (same shapes), enqueued on the
  * context's stream.  Outputs may be NULL.  Does not synchronise: non-finite

@@ -89,6 +89,7 @@ struct pimd_ctx {
   // pinned staging of the host-array force call: coordinates in, all outputs
   // out (made on the first call)
   double *h_stage = nullptr;
+  bool fab_pending = false;  // a host-array force call started, not finished
   // a parameter-derivative sweep started and not yet finished: its pinned
   // inputs and outputs, and what finishing it needs
   double *h_tan = nullptr;
@@ -1004,10 +1005,35 @@ int pimd_forces_all_beads_dev(pimd_ctx *x, const double *RR, const double *RRc, 
   return PIMD_OK;
 }
 
-int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int t,
-                          double *Upott, double *dRRt, double *virt, double *virialct,
-                          double *dipI, double *dipE, double *chg, int *iters) {
+// The host-array force call in two halves.  fab_start stages the
+// coordinates in pinned memory, queues the call and the copies of the outputs
+// (those set in want, bit i for output i of fab_outs) into the staging block
+// and returns; fab_finish waits, copies the wanted outputs to the caller and
+// reports non-finite input.
+struct FabOut {
+  void *dst;
+  const void *src;
+  size_t off, bytes;
+};
+static void fab_outs(pimd_ctx *x, FabOut *o, void *const dst[8]) {
+  const size_t P = x->c.P, na = x->natoms, N = x->c.N, nR = P * na * 3, nc = na * 3;
+  const size_t o_U = nR + nc, o_dR = o_U + P, o_vt = o_dR + nR, o_vc = o_vt + P * 9,
+               o_dI = o_vc + P, o_dE = o_dI + P * N * 3, o_ch = o_dE + P * N * 3, o_it = o_ch + P * na;
+  DevBufs &d = x->d;
+  const FabOut all[8] = {{dst[0], d.Upot, o_U, sizeof(double) * P},
+                         {dst[1], d.dRR, o_dR, sizeof(double) * nR},
+                         {dst[2], d.virt, o_vt, sizeof(double) * P * 9},
+                         {dst[3], d.virialc, o_vc, sizeof(double) * P},
+                         {dst[4], d.dipI, o_dI, sizeof(double) * P * N * 3},
+                         {dst[5], d.dipE, o_dE, sizeof(double) * P * N * 3},
+                         {dst[6], d.chg, o_ch, sizeof(double) * P * na},
+                         {dst[7], d.iters_out, o_it, sizeof(int) * P}};
+  for (int i = 0; i < 8; ++i) o[i] = all[i];
+}
+
+static int fab_start(pimd_ctx *x, const double *RRt, const double *RRc, int t, unsigned want) {
   if (!x || !RRt || !RRc) return fail(PIMD_EINVAL, "null argument");
+  if (x->fab_pending) return fail(PIMD_EINVAL, "a started force call is still to be finished");
   HIPCHK(hipSetDevice(x->p.device));
   const size_t P = x->c.P, na = x->natoms, N = x->c.N;
   int rc0;
@@ -1019,9 +1045,7 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   // the caller's arrays go through a pinned staging block: DMA copies at
   // full link speed instead of the runtime's pageable path
   const size_t nR = P * na * 3, nc = na * 3;
-  const size_t o_U = nR + nc, o_dR = o_U + P, o_vt = o_dR + nR, o_vc = o_vt + P * 9,
-               o_dI = o_vc + P, o_dE = o_dI + P * N * 3, o_ch = o_dE + P * N * 3,
-               o_it = o_ch + P * na, n_stage = o_it + P;
+  const size_t n_stage = 2 * nR + nc + P * 11 + 2 * P * N * 3 + P * na + P;
   if (!x->h_stage)
     HIPCHK(hipHostMalloc((void **)&x->h_stage, sizeof(double) * n_stage, hipHostMallocDefault));
   double *h = x->h_stage;
@@ -1032,24 +1056,48 @@ int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int
   launch_set_clock(x->self, t, 0, s);
   int rc = force_call(x);
   if (rc != PIMD_OK) return rc;
-  struct Out {
-    void *dst;
-    const void *src;
-    size_t off, bytes;
-  } outs[] = {{Upott, d.Upot, o_U, sizeof(double) * P},
-              {dRRt, d.dRR, o_dR, sizeof(double) * nR},
-              {virt, d.virt, o_vt, sizeof(double) * P * 9},
-              {virialct, d.virialc, o_vc, sizeof(double) * P},
-              {dipI, d.dipI, o_dI, sizeof(double) * P * N * 3},
-              {dipE, d.dipE, o_dE, sizeof(double) * P * N * 3},
-              {chg, d.chg, o_ch, sizeof(double) * P * na},
-              {iters, d.iters_out, o_it, sizeof(int) * P}};
-  for (const Out &o : outs)
-    if (o.dst) HIPCHK(hipMemcpyAsync(h + o.off, o.src, o.bytes, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  for (const Out &o : outs)
-    if (o.dst) std::memcpy(o.dst, h + o.off, o.bytes);
+  FabOut o[8];
+  void *const none[8] = {};
+  fab_outs(x, o, none);
+  for (int i = 0; i < 8; ++i)
+    if (want >> i & 1) HIPCHK(hipMemcpyAsync(h + o[i].off, o[i].src, o[i].bytes, hipMemcpyDeviceToHost, s));
+  x->fab_pending = true;
+  return PIMD_OK;
+}
+
+static int fab_finish(pimd_ctx *x, void *const dst[8]) {
+  if (!x) return fail(PIMD_EINVAL, "null argument");
+  if (!x->fab_pending) return fail(PIMD_EINVAL, "no force call was started");
+  HIPCHK(hipSetDevice(x->p.device));
+  x->fab_pending = false;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  FabOut o[8];
+  fab_outs(x, o, dst);
+  for (int i = 0; i < 8; ++i)
+    if (o[i].dst) std::memcpy(o[i].dst, x->h_stage + o[i].off, o[i].bytes);
   return take_flags(x);
+}
+
+int pimd_forces_all_beads(pimd_ctx *x, const double *RRt, const double *RRc, int t,
+                          double *Upott, double *dRRt, double *virt, double *virialct,
+                          double *dipI, double *dipE, double *chg, int *iters) {
+  void *const dst[8] = {Upott, dRRt, virt, virialct, dipI, dipE, chg, iters};
+  unsigned want = 0;
+  for (int i = 0; i < 8; ++i)
+    if (dst[i]) want |= 1u << i;
+  const int rc = fab_start(x, RRt, RRc, t, want);
+  return rc != PIMD_OK ? rc : fab_finish(x, dst);
+}
+
+int pimd_forces_all_beads_start(pimd_ctx *x, const double *RRt, const double *RRc, int t) {
+  return fab_start(x, RRt, RRc, t, 0xffu);
+}
+
+int pimd_forces_all_beads_finish(pimd_ctx *x, double *Upott, double *dRRt, double *virt,
+                                 double *virialct, double *dipI, double *dipE, double *chg,
+                                 int *iters) {
+  void *const dst[8] = {Upott, dRRt, virt, virialct, dipI, dipE, chg, iters};
+  return fab_finish(x, dst);
 }
 
 int pimd_energy_terms(pimd_ctx *x, double *terms) {

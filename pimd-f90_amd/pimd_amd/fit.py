@@ -60,30 +60,9 @@ class FitGroup:
         (pimd_ttm_adjoint, exact up to the SOR convergence); "tangent": from
         their tangent sweeps (one per parameter, exact for the SOR iterations
         as executed)."""
-        return self.evaluate_finish(self.evaluate_start(theta, grad, method))
+        return self._evaluate(theta, grad, method)
 
-    def evaluate_start(self, theta, grad: bool, method: str = "adjoint"):
-        """evaluate up to the adjoint sweep, which it leaves running on the
-        force field's stream (without gammaM); evaluate_finish completes it.
-        Starting every group before finishing any runs their sweeps side by
-        side."""
-        pes, ttm, dms, gm = split_theta(theta)
-        if grad and method == "adjoint" and ttm is not None and gm is None:
-            loss, g, dE, dG = self._forward(pes, ttm, dms, gm, grad)
-            n = TTM_NPARAM + (DMS_NCOEF if dms is not None else 0)
-            self.ff.ttm_adjoint_start(self.RRt, 2.0 * dE, 2.0 * dG, n)
-            return ("pending", loss, g)
-        return ("done",) + self._evaluate(theta, grad, method)
-
-    def evaluate_finish(self, started):
-        if started[0] == "done":
-            return started[1], started[2]
-        _, loss, g = started
-        return loss, np.concatenate([g, self.ff.ttm_adjoint_finish()[0]])
-
-    def _forward(self, pes, ttm, dms, gm, grad):
-        """Sets the parameters, runs the forward pass: (loss, PES gradient or
-        None, dE, dG)."""
+    def _set_params(self, pes, ttm, dms, gm):
         self.ff.set_pes_params(pes)
         if ttm is not None:
             self.ff.set_ttm_params(ttm)
@@ -91,12 +70,20 @@ class FitGroup:
             self.ff.set_dms_coefs(dms)
         if gm is not None:
             self.ff.set_gamma_m(gm)
-        r = self.ff.full_bead_forces(self.RRt, self.RRc)
+
+    def _residuals(self, r, grad):
+        """(loss, PES gradient or None, dE, dG) of a forward result."""
         dE = r["Upott"] - self.E_target
         dG = r["dRRt"] - self.G_target
         loss = float(np.sum(dE * dE) + np.sum(dG * dG))
         g = self.ff.pes_param_grad(self.RRt, 2.0 * dE, 2.0 * dG) if grad else None
         return loss, g, dE, dG
+
+    def _forward(self, pes, ttm, dms, gm, grad):
+        """Sets the parameters, runs the forward pass: (loss, PES gradient or
+        None, dE, dG)."""
+        self._set_params(pes, ttm, dms, gm)
+        return self._residuals(self.ff.full_bead_forces(self.RRt, self.RRc), grad)
 
     def _evaluate(self, theta, grad: bool, method: str):
         pes, ttm, dms, gm = split_theta(theta)
@@ -119,16 +106,31 @@ class FitGroup:
 
 def loss_and_grad(groups, theta, grad: bool = True, method: str = "adjoint"):
     """Loss over every group and, with grad, its gradient dL/dtheta (method:
-    see FitGroup.evaluate).  Every group's adjoint sweep is started before
-    any is finished, so the sweeps of the groups (and the forward passes of
-    the later groups) run side by side."""
+    see FitGroup.evaluate).  With the adjoint (and no gammaM) every group's
+    forward pass is started before any is finished, and every group's
+    adjoint sweep before any is finished, so the groups run side by side on
+    their force fields' streams."""
     total, g = 0.0, np.zeros(np.asarray(theta).size) if grad else None
-    started = [grp.evaluate_start(theta, grad, method) for grp in groups]
-    for grp, st in zip(groups, started):
-        l, gg = grp.evaluate_finish(st)
+    pes, ttm, dms, gm = split_theta(theta)
+    if not (grad and method == "adjoint" and ttm is not None and gm is None):
+        for grp in groups:
+            l, gg = grp.evaluate(theta, grad, method)
+            total += l
+            if grad:
+                g += gg
+        return total, g
+    n = TTM_NPARAM + (DMS_NCOEF if dms is not None else 0)
+    for grp in groups:
+        grp._set_params(pes, ttm, dms, gm)
+        grp.ff.full_bead_forces_start(grp.RRt, grp.RRc)
+    parts = []
+    for grp in groups:
+        l, gp, dE, dG = grp._residuals(grp.ff.full_bead_forces_finish(), True)
+        grp.ff.ttm_adjoint_start(grp.RRt, 2.0 * dE, 2.0 * dG, n)
+        parts.append((l, gp))
+    for grp, (l, gp) in zip(groups, parts):
         total += l
-        if grad:
-            g += gg
+        g += np.concatenate([gp, grp.ff.ttm_adjoint_finish()[0]])
     return total, g
 
 

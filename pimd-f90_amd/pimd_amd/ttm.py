@@ -40,6 +40,7 @@ EXPORTS = (
     "pimd_forces_all_beads_noalloc", "pimd_monomer_dipoles", "pimd_dms_coefs_get",
     "pimd_dms_coefs_set", "pimd_dms_coef_grad", "pimd_gamma_m_get", "pimd_gamma_m_set",
     "pimd_gamma_m_grad", "pimd_ttm_adjoint", "pimd_ttm_adjoint_start", "pimd_ttm_adjoint_finish",
+    "pimd_forces_all_beads_start", "pimd_forces_all_beads_finish",
 )
 
 PES_NPARAM = 250   # combined c5z (245), deohA, alphaoh, roh, phh1A, phh2
@@ -131,6 +132,8 @@ def load_library(path: str = LIB_PATH):
         "pimd_gamma_m_grad": [vp, dp, dp, dp, dp, dp, dp],
         "pimd_ttm_adjoint": [vp, dp, dp, dp, C.c_int, dp, dp, dp],
         "pimd_ttm_adjoint_start": [vp, dp, dp, dp, C.c_int],
+        "pimd_forces_all_beads_start": [vp, dp, dp, C.c_int],
+        "pimd_forces_all_beads_finish": [vp, dp, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int)],
         "pimd_ttm_adjoint_finish": [vp, dp, dp, dp],
         "pimd_md_observables": [vp, C.c_int, dp],
         "pimd_state_forces": [vp, C.c_int],
@@ -230,18 +233,42 @@ class TTMForceField:
         here for convenience.  iters < 0 marks beads that reached polar_maxiter.
         """
         p = self.params
-        P, na, N = p.nbeads, p.natoms, p.nwaters
+        P, na = p.nbeads, p.natoms
         RRt = _arr(RRt, (P, na, 3))
         RRc = _arr(RRc, (na, 3))
-        out = {"Upott": np.zeros(P), "dRRt": np.zeros((P, na, 3)), "virt_cm": np.zeros((P, 3, 3)),
-               "virialct": np.zeros(P), "dip_momIt": np.zeros((P, N, 3)),
-               "dip_momEt": np.zeros((P, N, 3)), "chg": np.zeros((P, na)),
-               "iters": np.zeros(P, dtype=np.int32)}
-        _check(load_library().pimd_forces_all_beads(
-            self._h, _ptr(RRt), _ptr(RRc), int(t), _ptr(out["Upott"]), _ptr(out["dRRt"]),
-            _ptr(out["virt_cm"]), _ptr(out["virialct"]), _ptr(out["dip_momIt"]),
-            _ptr(out["dip_momEt"]), _ptr(out["chg"]),
-            out["iters"].ctypes.data_as(C.POINTER(C.c_int))))
+        out = self._force_outputs()
+        _check(load_library().pimd_forces_all_beads(self._h, _ptr(RRt), _ptr(RRc), int(t),
+                                                    *self._force_ptrs(out)))
+        return self._force_done(out)
+
+    def full_bead_forces_start(self, RRt, RRc, t: int = 1):
+        """full_bead_forces in two halves: this queues the call on the
+        context's stream and returns; full_bead_forces_finish waits and
+        returns its dict.  Calls of several force fields run side by side."""
+        P, na = self.params.nbeads, self.params.natoms
+        _check(load_library().pimd_forces_all_beads_start(
+            self._h, _ptr(_arr(RRt, (P, na, 3))), _ptr(_arr(RRc, (na, 3))), int(t)))
+
+    def full_bead_forces_finish(self) -> dict:
+        out = self._force_outputs()
+        _check(load_library().pimd_forces_all_beads_finish(self._h, *self._force_ptrs(out)))
+        return self._force_done(out)
+
+    def _force_outputs(self) -> dict:
+        P, na, N = self.params.nbeads, self.params.natoms, self.params.nwaters
+        return {"Upott": np.zeros(P), "dRRt": np.zeros((P, na, 3)), "virt_cm": np.zeros((P, 3, 3)),
+                "virialct": np.zeros(P), "dip_momIt": np.zeros((P, N, 3)),
+                "dip_momEt": np.zeros((P, N, 3)), "chg": np.zeros((P, na)),
+                "iters": np.zeros(P, dtype=np.int32)}
+
+    @staticmethod
+    def _force_ptrs(out: dict):
+        return (_ptr(out["Upott"]), _ptr(out["dRRt"]), _ptr(out["virt_cm"]), _ptr(out["virialct"]),
+                _ptr(out["dip_momIt"]), _ptr(out["dip_momEt"]), _ptr(out["chg"]),
+                out["iters"].ctypes.data_as(C.POINTER(C.c_int)))
+
+    @staticmethod
+    def _force_done(out: dict) -> dict:
         out["virt"] = np.transpose(out["virt_cm"], (0, 2, 1)).copy()
         out["virialt"] = np.trace(out["virt"], axis1=1, axis2=2)
         return out

@@ -227,3 +227,38 @@ def test_cell_list_matches_full_search(model, N, P, rc, monkeypatch):
     assert rel_err(rc_["Upott"], rf["Upott"]) < 1e-12
     assert max_abs(rc_["dRRt"], rf["dRRt"]) < 1e-10
     assert np.array_equal(rc_["iters"], rf["iters"])
+
+
+def test_force_call_in_two_halves():
+    """pimd_forces_all_beads_start / _finish on two contexts, both started
+    before either is finished, give pimd_forces_all_beads's outputs bit for
+    bit; a second start before the finish and a finish with none started are
+    refused."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    from pimd_amd.ttm import PimdError
+    cases = []
+    for model, N, P in ((3, 64, 4), (2, 32, 2)):
+        RRt, _, RRc, L = make_system(N, P, seed=400 + N)
+        p = TTMParams(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0,
+                      eps_ewald=1e-6).resolved()
+        ff = _ff(p)
+        cases.append((ff, RRt, RRc, ff.full_bead_forces(RRt, RRc, t=3)))
+    print("case two halves", flush=True)
+    for ff, RRt, RRc, _ in cases:
+        ff.full_bead_forces_start(RRt, RRc, t=3)
+    for ff, RRt, RRc, ref in cases:
+        got = ff.full_bead_forces_finish()
+        for k in ref:
+            assert np.array_equal(got[k], ref[k]), k
+    ff, RRt, RRc, ref = cases[0]
+    with pytest.raises(PimdError):
+        ff.full_bead_forces_finish()
+    ff.full_bead_forces_start(RRt, RRc, t=3)
+    with pytest.raises(PimdError):
+        ff.full_bead_forces_start(RRt, RRc, t=3)
+    with pytest.raises(PimdError):
+        ff.full_bead_forces(RRt, RRc, t=3)
+    assert np.array_equal(ff.full_bead_forces_finish()["dRRt"], ref["dRRt"])
+    for c in cases:
+        c[0].close()
