@@ -2274,9 +2274,10 @@ __global__ __launch_bounds__(256) void k_pair_final(const Rep *__restrict__ reps
 // the block's two-word fixed-point LDS rows, added to the partner by
 // k_pair_final_partner.  The virial terms are even under the exchange of the
 // two molecules (site-centroid difference and force both reverse), so the
-// owner adds them whichever molecule is the lower one.  FP64 form only: in
-// FP32 the two-sided pass is faster (512 H2O x 32, TTM3-F: 0.104 against
-// 0.121 ms; the partner sums' LDS atomics outweigh the halved pair work).
+// owner adds them whichever molecule is the lower one.  The FP64 forms and
+// TTM3-F's FP32 form (one-word partner sums: 512 H2O x 32, 0.104 -> 0.092 ms
+// against the two-sided pass); TTM2.1-F's FP32 form spills and stays
+// two-sided.
 constexpr int FIN_NV = 16;  // partner values: potentials of sites 0-3, then their forces
 template <int MODEL, int G, typename R>
 __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__restrict__ reps,
@@ -2285,7 +2286,8 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
   constexpr int T0 = MODEL == 2 ? 0 : 3;
   constexpr int SB0 = MODEL == 3 ? 1 : 0;
   constexpr int OWN = QQ_HALF_BLOCK / G;
-  // [W][4] sites, [W][ND][3] dipoles, [W][FIN_NV][2] partner sums
+  constexpr int NW = QqWords<R>::n;  // words per partner value
+  // [W][4] sites, [W][ND][3] dipoles, [W][FIN_NV][NW] partner sums
   extern __shared__ Site lds_site[];
   const Blk bk = block_of(bm);
   const Rep &Rp = reps[bk.r];
@@ -2308,7 +2310,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
     m -= m >= N ? N : 0;
     lds_dip[e] = dip[((size_t)b * N + m) * ND * 3 + e % (ND * 3)];
   }
-  for (int e = threadIdx.x; e < W * FIN_NV * 2; e += blockDim.x) jacc[e] = 0ull;
+  for (int e = threadIdx.x; e < W * FIN_NV * NW; e += blockDim.x) jacc[e] = 0ull;
   __syncthreads();
   const int a0 = i0 + threadIdx.x / G;
   const int lane = threadIdx.x % G;
@@ -2325,7 +2327,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
   }
   double ph[4] = {0, 0, 0, 0}, fr[4][3] = {{0}}, vc = 0.0, vt[9] = {0};
   const int *list = nbr + (size_t)grp * c.nbr_cap;
-  auto jslot = [&](int rj, int k) { return jacc + ((size_t)rj * FIN_NV + k) * 2; };
+  auto jslot = [&](int rj, int k) { return jacc + ((size_t)rj * FIN_NV + k) * NW; };
   auto visit = [&](const int j, auto thole_tag) {
     constexpr bool THOLE = decltype(thole_tag)::value;
     const bool lo = a < j;
@@ -2402,8 +2404,8 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
         for (int col = 0; col < 3; ++col)
           for (int rw = 0; rw < 3; ++rw) vt[col * 3 + rw] += dij[rw] * rd[col];
       }
-      qq_fix_add<2>(jslot(rj, sb), jp);
-      for (int k = 0; k < 3; ++k) qq_fix_add<2>(jslot(rj, 4 + 3 * sb + k), jf[k]);
+      qq_fix_add<NW>(jslot(rj, sb), jp);
+      for (int k = 0; k < 3; ++k) qq_fix_add<NW>(jslot(rj, 4 + 3 * sb + k), jf[k]);
     }
   };
   if (valid) {
@@ -2437,7 +2439,7 @@ __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__
   double *slab = Rp.b.qqslab + ((size_t)b * bk.nx + bk.x) * (size_t)W * QQ_NV;
   for (int e = threadIdx.x; e < W * FIN_NV; e += blockDim.x) {
     const int rw = e / FIN_NV, k = e % FIN_NV;
-    slab[(size_t)rw * QQ_NV + k] = qq_fix_value<2>(jacc + 2 * (size_t)e);
+    slab[(size_t)rw * QQ_NV + k] = qq_fix_value<NW>(jacc + NW * (size_t)e);
   }
 }
 
@@ -3184,22 +3186,24 @@ static void launch_fin_v(int gi, dim3 grid, const BlockMap &m, size_t lds, const
 // LDS of k_pair_final_half: sites, dipoles and partner sums of its rows
 static size_t fin_half_lds(const SysConst &c) {
   const size_t W = qq_half_rows(c.N, half_lanes(c.g_fin));
-  return W * (sizeof(Site) * 4 + sizeof(double) * c.nd * 3 + sizeof(long long) * FIN_NV * 2);
+  return W * (sizeof(Site) * 4 + sizeof(double) * c.nd * 3 +
+              sizeof(long long) * FIN_NV * (c.mixed ? 1 : 2));
 }
 static bool fin_half_on(const SysConst &c) {
-  return c.qq_half && c.fin_half && !c.mixed && is_ttm(c) && !c.cells && c.g_fin > 0 &&
+  return c.qq_half && c.fin_half && (!c.mixed || c.model == 3) && is_ttm(c) && !c.cells &&
+         c.g_fin > 0 &&
          fin_half_lds(c) <= PAIR_LDS_MAX &&
          (c.qq_half == 2 ||
           (long)c.P * qq_half_blocks(c.N, half_lanes(c.g_fin)) >= QQ_HALF_MIN_BLOCKS);
 }
 
-template <int ML>
+template <int ML, typename R>
 static void launch_fin_half_v(int G, dim3 grid, const BlockMap &m, size_t lds, const Group &g,
                               hipStream_t s) {
   const dim3 B(QQ_HALF_BLOCK);
-  if (G == 4) hipLaunchKernelGGL((k_pair_final_half<ML, 4, double>), grid, B, lds, s, g.d, m);
-  else if (G == 8) hipLaunchKernelGGL((k_pair_final_half<ML, 8, double>), grid, B, lds, s, g.d, m);
-  else hipLaunchKernelGGL((k_pair_final_half<ML, 16, double>), grid, B, lds, s, g.d, m);
+  if (G == 4) hipLaunchKernelGGL((k_pair_final_half<ML, 4, R>), grid, B, lds, s, g.d, m);
+  else if (G == 8) hipLaunchKernelGGL((k_pair_final_half<ML, 8, R>), grid, B, lds, s, g.d, m);
+  else hipLaunchKernelGGL((k_pair_final_half<ML, 16, R>), grid, B, lds, s, g.d, m);
 }
 
 void launch_pair_final(const Group &g, hipStream_t s) {
@@ -3210,16 +3214,17 @@ void launch_pair_final(const Group &g, hipStream_t s) {
         if (fin_half_on(c)) {
           const int G = half_lanes(c.g_fin);
           const int gi = G == 4 ? 1 : G == 8 ? 2 : 3;
-          return Plan{(c.model == 3 ? 1 : 0) | 64 | (gi << 3), idiv(c.N, QQ_HALF_BLOCK / G), c.P, 1,
-                      fin_half_lds(c)};
+          return Plan{(c.model == 3 ? 1 : 0) | (c.mixed ? 2 : 0) | 64 | (gi << 3),
+                      idiv(c.N, QQ_HALF_BLOCK / G), c.P, 1, fin_half_lds(c)};
         }
         return pair_plan(c, c.g_fin, bead_sites_lds(c) + sizeof(double) * c.N * c.nd * 3);
       },
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
         if (v & 64) {
           const int gi = (v >> 3) & 7, G = gi == 1 ? 4 : gi == 2 ? 8 : 16;
-          if (v & 1) launch_fin_half_v<3>(G, grid, m, lds, g, s);
-          else launch_fin_half_v<2>(G, grid, m, lds, g, s);
+          if (v & 2) launch_fin_half_v<3, float>(G, grid, m, lds, g, s);
+          else if (v & 1) launch_fin_half_v<3, double>(G, grid, m, lds, g, s);
+          else launch_fin_half_v<2, double>(G, grid, m, lds, g, s);
           return;
         }
         const int gi = v >> 3;
