@@ -73,7 +73,7 @@ struct SysConst {
   int overlap;             // 1: real-space dipole field beside the structure factor (2 streams)
   int dd_cache;            // 1: the SOR dipole pass keeps each pair's screening factors
   int qq_half;             // once-per-pair charge pass: 0 off, 1 where it fills the device, 2 always
-  int fin_half;            // the same for the final dipole pass (with qq_half; not TTM2.1-F FP32): 0 off, 1 on
+  int fin_half;            // the same for the final dipole pass (with qq_half): 0 off, 1 on
   double box[3], boxi[3], vol;
   double rc2, rc1;
   double vdwA, vdwB, vdwC, vdwD, vdwE, shiftA, shiftB;

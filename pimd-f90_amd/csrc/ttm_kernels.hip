@@ -2274,10 +2274,10 @@ __global__ __launch_bounds__(256) void k_pair_final(const Rep *__restrict__ reps
 // the block's two-word fixed-point LDS rows, added to the partner by
 // k_pair_final_partner.  The virial terms are even under the exchange of the
 // two molecules (site-centroid difference and force both reverse), so the
-// owner adds them whichever molecule is the lower one.  The FP64 forms and
-// TTM3-F's FP32 form (one-word partner sums: 512 H2O x 32, 0.104 -> 0.092 ms
-// against the two-sided pass); TTM2.1-F's FP32 form spills and stays
-// two-sided.
+// owner adds them whichever molecule is the lower one.  Against the
+// two-sided pass, with one-word partner sums in FP32: 512 H2O x 32 TTM3-F
+// 0.104 -> 0.092 ms, 128 x 32 TTM2.1-F 0.060 -> 0.049 ms (two words: slower
+// than two-sided).
 constexpr int FIN_NV = 16;  // partner values: potentials of sites 0-3, then their forces
 template <int MODEL, int G, typename R>
 __global__ __launch_bounds__(QQ_HALF_BLOCK) void k_pair_final_half(const Rep *__restrict__ reps,
@@ -3190,7 +3190,7 @@ static size_t fin_half_lds(const SysConst &c) {
               sizeof(long long) * FIN_NV * (c.mixed ? 1 : 2));
 }
 static bool fin_half_on(const SysConst &c) {
-  return c.qq_half && c.fin_half && (!c.mixed || c.model == 3) && is_ttm(c) && !c.cells &&
+  return c.qq_half && c.fin_half && is_ttm(c) && !c.cells &&
          c.g_fin > 0 &&
          fin_half_lds(c) <= PAIR_LDS_MAX &&
          (c.qq_half == 2 ||
@@ -3222,9 +3222,12 @@ void launch_pair_final(const Group &g, hipStream_t s) {
       [&](int v, dim3 grid, const BlockMap &m, size_t lds) {
         if (v & 64) {
           const int gi = (v >> 3) & 7, G = gi == 1 ? 4 : gi == 2 ? 8 : 16;
-          if (v & 2) launch_fin_half_v<3, float>(G, grid, m, lds, g, s);
-          else if (v & 1) launch_fin_half_v<3, double>(G, grid, m, lds, g, s);
-          else launch_fin_half_v<2, double>(G, grid, m, lds, g, s);
+          switch (v & 3) {
+            case 0: launch_fin_half_v<2, double>(G, grid, m, lds, g, s); break;
+            case 1: launch_fin_half_v<3, double>(G, grid, m, lds, g, s); break;
+            case 2: launch_fin_half_v<2, float>(G, grid, m, lds, g, s); break;
+            default: launch_fin_half_v<3, float>(G, grid, m, lds, g, s); break;
+          }
           return;
         }
         const int gi = v >> 3;
