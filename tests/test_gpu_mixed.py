@@ -102,3 +102,21 @@ def test_mixed_vs_fp64_other_systems(model, N, P, rc, eps):
     r64 = _ff(TTMParams(**kw).resolved()).full_bead_forces(RRt, RRc)
     rmx = _ff(TTMParams(**kw, precision=1).resolved()).full_bead_forces(RRt, RRc)
     _compare(rmx, _as_ref(r64), f"mixed vs fp64 ttm{model} N{N} P{P}")
+
+
+@pytest.mark.parametrize("model,N,P,half", [(2, 128, 4, "0"), (3, 128, 3, "0"),
+                                            (2, 128, 1, "1"), (3, 256, 1, "1")])
+def test_mixed_two_sided_pair_passes(model, N, P, half, monkeypatch):
+    """The FP32 charge and final pair passes that take every pair from both
+    sides: the LDS form (PIMD_QQ_HALF=0) and, with one bead, the
+    global-memory form that one-bead contexts use; mixed against FP64 at the
+    same bar."""
+    from pimd_amd.params import TTMParams
+    from pimd_amd.systems import make_system
+    monkeypatch.setenv("PIMD_QQ_HALF", half)
+    RRt, _, RRc, L = make_system(N, P, seed=700 + N + model)
+    kw = dict(pot_model=model, nwaters=N, nbeads=P, box=(L, L, L), rc=-1.0, eps_ewald=1e-6)
+    print("case two-sided", model, N, P, half, flush=True)
+    r64 = _ff(TTMParams(**kw).resolved()).full_bead_forces(RRt, RRc)
+    rmx = _ff(TTMParams(**kw, precision=1).resolved()).full_bead_forces(RRt, RRc)
+    _compare(rmx, _as_ref(r64), f"two-sided mixed vs fp64 ttm{model} N{N} P{P}")
